@@ -1,0 +1,518 @@
+// C ABI of libbbwadg_hip.so: context, uploads, single launches and the LSRK45 step loop.
+// The step loop restates Wave_RK / RK_step_WADG_subelem (cpp/src/WaveOKL3d.cpp:2014-2109,
+// elas_cpp/src/WaveOKL3d.cpp:1605-1693) on one HIP stream, optionally replayed from a
+// captured HIP graph.
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <vector>
+
+#include "bbw_internal.h"
+
+using namespace bbw;
+
+namespace {
+
+int fail(bbw_ctx *c, int code, const std::string &msg) {
+  if (c) c->err = msg;
+  return code;
+}
+
+int hipfail(bbw_ctx *c, hipError_t e, const char *what) {
+  return fail(c, BBW_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+#define HIPCHK(c, call)                          \
+  do {                                           \
+    hipError_t e_ = (call);                      \
+    if (e_ != hipSuccess) return hipfail(c, e_, #call); \
+  } while (0)
+
+template <typename T>
+int dalloc(bbw_ctx *c, T **p, size_t n) {
+  if (*p) return BBW_OK;
+  if (n == 0) n = 1;
+  hipError_t e = hipMalloc((void **)p, n * sizeof(T));
+  if (e != hipSuccess) {
+    *p = nullptr;
+    return fail(c, BBW_ERR_NOMEM, std::string("hipMalloc failed: ") + hipGetErrorString(e));
+  }
+  return BBW_OK;
+}
+
+template <typename T>
+int upload(bbw_ctx *c, T **dst, const T *src, size_t n) {
+  int rc = dalloc(c, dst, n);
+  if (rc) return rc;
+  HIPCHK(c, hipMemcpyAsync(*dst, src, n * sizeof(T), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return BBW_OK;
+}
+
+template <typename T>
+void dfree(T *&p) {
+  if (p) hipFree((void *)p);
+  p = nullptr;
+}
+
+size_t nstate(const bbw_ctx *c) { return (size_t)c->K * c->Nf * c->Np; }
+
+double ricker(double t, double f0) {
+  const double a = M_PI * f0 * (t - 1.0 / f0);
+  return (1.0 - 2.0 * a * a) * std::exp(-a * a);
+}
+
+int check_ready(bbw_ctx *c) {
+  if (!c->have_ops) return fail(c, BBW_ERR_STATE, "operators not uploaded");
+  if (!c->have_mesh) return fail(c, BBW_ERR_STATE, "mesh not uploaded");
+  if (!c->have_media) return fail(c, BBW_ERR_STATE, "media not uploaded");
+  return BBW_OK;
+}
+
+void reset_graph(bbw_ctx *c) {
+  if (c->graph.exec) hipGraphExecDestroy(c->graph.exec);
+  c->graph = Graph();
+}
+
+// One LSRK stage; ftd (device) or ft (host value) is the source time function value.
+int do_stage(bbw_ctx *c, int i, double dt, const float *ftd, float ft, int mode, int projU) {
+  const float fa = (float)c->rk4a[i], fb = (float)c->rk4b[i], fdt = (float)dt;
+  hipError_t e;
+  if (c->Nf == 4) {
+    if (mode == BBW_MODE_FUSED) {
+      e = launch_fused(c, c->Q[c->cur], c->Q[1 - c->cur], fa, fb, fdt, ftd, ft, projU);
+      if (e != hipSuccess) return hipfail(c, e, "fused stage launch");
+      c->cur = 1 - c->cur;
+      return BBW_OK;
+    }
+    if ((e = launch_volume(c)) != hipSuccess) return hipfail(c, e, "volume launch");
+    if ((e = launch_surface(c)) != hipSuccess) return hipfail(c, e, "surface launch");
+    if ((e = launch_update_wadg(c, fa, fb, fdt, ftd, ft)) != hipSuccess)
+      return hipfail(c, e, "update launch");
+    return BBW_OK;
+  }
+  if ((e = launch_volume_elas(c)) != hipSuccess) return hipfail(c, e, "elastic volume launch");
+  if ((e = launch_surface_elas(c)) != hipSuccess) return hipfail(c, e, "elastic surface launch");
+  if ((e = launch_update_elas(c, fa, fb, fdt, ftd, ft)) != hipSuccess)
+    return hipfail(c, e, "elastic update launch");
+  return BBW_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int bbw_create(const bbw_config *cfg, bbw_ctx **out) {
+  if (!cfg || !out) return BBW_ERR_ARG;
+  *out = nullptr;
+  if (!order_supported(cfg->N)) return BBW_ERR_UNSUPPORTED;
+  if (cfg->Nfields != 4 && cfg->Nfields != 9) return BBW_ERR_UNSUPPORTED;
+  if (cfg->K <= 0 || cfg->Nq != nq_of(cfg->N)) return BBW_ERR_ARG;
+  if ((double)cfg->K * cfg->Nfields * np_of(cfg->N) >= 2147483647.0) return BBW_ERR_ARG;
+  bbw_ctx *c = new bbw_ctx();
+  c->N = cfg->N;
+  c->Np = np_of(cfg->N);
+  c->Nfp = nfp_of(cfg->N);
+  c->Nq = cfg->Nq;
+  c->Nf = cfg->Nfields;
+  c->K = cfg->K;
+  c->dev = cfg->device;
+  c->KblkV = cfg->KblkV;
+  c->KblkS = cfg->KblkS;
+  c->KblkU = cfg->KblkU;
+  c->KblkF = cfg->KblkF;
+  if (hipSetDevice(c->dev) != hipSuccess || hipStreamCreateWithFlags(&c->own_stream, hipStreamNonBlocking) != hipSuccess) {
+    delete c;
+    return BBW_ERR_HIP;
+  }
+  c->stream = c->own_stream;
+  const size_t n = nstate(c);
+  int rc = dalloc(c, &c->Q[0], n);
+  if (!rc) rc = dalloc(c, &c->Q[1], n);
+  if (!rc) rc = dalloc(c, &c->res, n);
+  if (!rc) rc = dalloc(c, &c->rhs, n);
+  if (rc) {
+    bbw_destroy(c);
+    return rc;
+  }
+  for (float *p : {c->Q[0], c->Q[1], c->res, c->rhs})
+    if (hipMemsetAsync(p, 0, n * sizeof(float), c->stream) != hipSuccess) {
+      bbw_destroy(c);
+      return BBW_ERR_HIP;
+    }
+  if (hipStreamSynchronize(c->stream) != hipSuccess) {
+    bbw_destroy(c);
+    return BBW_ERR_HIP;
+  }
+  *out = c;
+  return BBW_OK;
+}
+
+void bbw_destroy(bbw_ctx *c) {
+  if (!c) return;
+  hipSetDevice(c->dev);
+  if (c->stream) hipStreamSynchronize(c->stream);
+  reset_graph(c);
+  dfree(c->Dids); dfree(c->Dvals); dfree(c->Fmask); dfree(c->EEL_ids); dfree(c->L0_ids);
+  dfree(c->EEL_vals); dfree(c->L0_vals); dfree(c->Vq); dfree(c->Pq); dfree(c->Vq64);
+  dfree(c->wq64); dfree(c->vgeo); dfree(c->fgeo); dfree(c->vmapP); dfree(c->J);
+  dfree(c->c2q); dfree(c->Jq); dfree(c->rhoinv); dfree(c->lambda); dfree(c->mu);
+  dfree(c->Q[0]); dfree(c->Q[1]); dfree(c->res); dfree(c->rhs); dfree(c->src_elems);
+  dfree(c->src_coef); dfree(c->rec_elems); dfree(c->rec_basis); dfree(c->rec_trace);
+  dfree(c->partial); dfree(c->ftime_win);
+  if (c->own_stream) hipStreamDestroy(c->own_stream);
+  delete c;
+}
+
+const char *bbw_last_error(const bbw_ctx *c) { return c ? c->err.c_str() : "null context"; }
+
+int bbw_set_stream(bbw_ctx *c, void *s) {
+  if (!c) return BBW_ERR_ARG;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  c->stream = s ? (hipStream_t)s : c->own_stream;
+  reset_graph(c);
+  return BBW_OK;
+}
+
+int bbw_sync(bbw_ctx *c) {
+  if (!c) return BBW_ERR_ARG;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return BBW_OK;
+}
+
+int bbw_upload_operators(bbw_ctx *c, const bbw_ops *o) {
+  if (!c || !o) return BBW_ERR_ARG;
+  if (!o->Fmask || !o->D_ids || !o->Dvals || !o->EEL_ids || !o->EEL_vals || !o->L0_ids ||
+      !o->L0_vals || !o->Vq || !o->Pq || !o->wq || !o->rk4a || !o->rk4b || !o->rk4c)
+    return fail(c, BBW_ERR_ARG, "null operator array");
+  const int Np = c->Np, Nfp = c->Nfp, Nq = c->Nq;
+  if (o->EEL_nnz != Nfp + 3 || o->L0_nnz != (Nfp < 7 ? Nfp : 7))
+    return fail(c, BBW_ERR_ARG, "EEL_nnz must be Nfp+3 and L0_nnz min(Nfp,7)");
+  for (int i = 0; i < 4 * Nfp; ++i)
+    if (o->Fmask[i] < 0 || o->Fmask[i] >= Np) return fail(c, BBW_ERR_ARG, "Fmask out of range");
+  for (int i = 0; i < 16 * Np; ++i)
+    if (o->D_ids[i] < 0 || o->D_ids[i] >= Np) return fail(c, BBW_ERR_ARG, "D_ids out of range");
+  for (int i = 0; i < o->EEL_nnz * Np; ++i)
+    if (o->EEL_ids[i] < 0 || o->EEL_ids[i] >= 4 * Nfp) return fail(c, BBW_ERR_ARG, "EEL_ids out of range");
+  for (int i = 0; i < o->L0_nnz * Nfp; ++i)
+    if (o->L0_ids[i] < 0 || o->L0_ids[i] >= Nfp) return fail(c, BBW_ERR_ARG, "L0_ids out of range");
+  int rc;
+  if ((rc = upload(c, &c->Fmask, (const int *)o->Fmask, (size_t)4 * Nfp))) return rc;
+  if ((rc = upload(c, &c->Dids, (const int4 *)o->D_ids, (size_t)4 * Np))) return rc;
+  if ((rc = upload(c, &c->Dvals, (const float4 *)o->Dvals, (size_t)Np))) return rc;
+  if ((rc = upload(c, &c->EEL_ids, (const int *)o->EEL_ids, (size_t)o->EEL_nnz * Np))) return rc;
+  if ((rc = upload(c, &c->EEL_vals, o->EEL_vals, (size_t)o->EEL_nnz * Np))) return rc;
+  if ((rc = upload(c, &c->L0_ids, (const int *)o->L0_ids, (size_t)o->L0_nnz * Nfp))) return rc;
+  if ((rc = upload(c, &c->L0_vals, o->L0_vals, (size_t)o->L0_nnz * Nfp))) return rc;
+  if ((rc = upload(c, &c->Vq, o->Vq, (size_t)Nq * Np))) return rc;
+  if ((rc = upload(c, &c->Pq, o->Pq, (size_t)Nq * Np))) return rc;
+  std::vector<double> v64((size_t)Nq * Np);
+  for (size_t i = 0; i < v64.size(); ++i) v64[i] = (double)o->Vq[i];
+  if ((rc = upload(c, &c->Vq64, v64.data(), v64.size()))) return rc;
+  if ((rc = upload(c, &c->wq64, o->wq, (size_t)Nq))) return rc;
+  std::memcpy(c->rk4a, o->rk4a, 5 * sizeof(double));
+  std::memcpy(c->rk4b, o->rk4b, 5 * sizeof(double));
+  std::memcpy(c->rk4c, o->rk4c, 6 * sizeof(double));
+  c->have_ops = true;
+  reset_graph(c);
+  return BBW_OK;
+}
+
+int bbw_upload_mesh(bbw_ctx *c, const float *vgeo, const float *fgeo, const int32_t *vmapP,
+                    const double *J) {
+  if (!c || !vgeo || !fgeo || !vmapP || !J) return BBW_ERR_ARG;
+  const size_t K = c->K, NfpNf = 4 * (size_t)c->Nfp;
+  const long long lim = (long long)K * c->Nf * c->Np;
+  for (size_t k = 0; k < K; ++k)
+    for (size_t i = 0; i < NfpNf; ++i) {
+      const long long v = vmapP[k * NfpNf + i];
+      // every exterior trace index must address field 0 of a node inside Q
+      if (v < 0 || v >= lim || (v % ((long long)c->Nf * c->Np)) >= c->Np)
+        return fail(c, BBW_ERR_ARG, "vmapP entry out of range");
+    }
+  int rc;
+  if ((rc = upload(c, &c->vgeo, vgeo, K * 9))) return rc;
+  if ((rc = upload(c, &c->fgeo, fgeo, K * 16))) return rc;
+  if ((rc = upload(c, &c->vmapP, (const int *)vmapP, K * NfpNf))) return rc;
+  if ((rc = upload(c, &c->J, J, K))) return rc;
+  c->have_mesh = true;
+  reset_graph(c);
+  return BBW_OK;
+}
+
+int bbw_upload_media(bbw_ctx *c, const float *c2q, const float *Jq) {
+  if (!c || !c2q) return BBW_ERR_ARG;
+  if (c->Nf != 4) return fail(c, BBW_ERR_ARG, "acoustic media on an elastic context");
+  const size_t n = (size_t)c->K * c->Nq;
+  int rc;
+  if ((rc = upload(c, &c->c2q, c2q, n))) return rc;
+  if (Jq) {
+    if ((rc = upload(c, &c->Jq, Jq, n))) return rc;
+  } else {
+    dfree(c->Jq);
+  }
+  c->have_media = true;
+  reset_graph(c);
+  return BBW_OK;
+}
+
+int bbw_upload_elastic_media(bbw_ctx *c, const float *rhoinvq, const float *lambdaq,
+                             const float *muq, float tau_v, float tau_s) {
+  if (!c || !rhoinvq || !lambdaq || !muq) return BBW_ERR_ARG;
+  if (c->Nf != 9) return fail(c, BBW_ERR_ARG, "elastic media on an acoustic context");
+  const size_t n = (size_t)c->K * c->Nq;
+  int rc;
+  if ((rc = upload(c, &c->rhoinv, rhoinvq, n))) return rc;
+  if ((rc = upload(c, &c->lambda, lambdaq, n))) return rc;
+  if ((rc = upload(c, &c->mu, muq, n))) return rc;
+  c->tau_v = tau_v;
+  c->tau_s = tau_s;
+  c->have_media = true;
+  reset_graph(c);
+  return BBW_OK;
+}
+
+int bbw_set_source(bbw_ctx *c, int nsrc, const int32_t *elems, const float *coef,
+                   const float *weights, double f0, double amp) {
+  if (!c || nsrc < 0) return BBW_ERR_ARG;
+  dfree(c->src_elems);
+  dfree(c->src_coef);
+  c->nsrc = 0;
+  c->src_first = 0;
+  c->src_last = -1;
+  reset_graph(c);
+  if (nsrc == 0) return BBW_OK;
+  if (!elems || !coef || !weights || !(f0 > 0)) return fail(c, BBW_ERR_ARG, "bad source");
+  for (int i = 0; i < nsrc; ++i) {
+    if (elems[i] < 0 || elems[i] >= c->K) return fail(c, BBW_ERR_ARG, "source element out of range");
+    if (i && elems[i] <= elems[i - 1]) return fail(c, BBW_ERR_ARG, "source elements must ascend");
+  }
+  int rc;
+  if ((rc = upload(c, &c->src_elems, (const int *)elems, (size_t)nsrc))) return rc;
+  if ((rc = upload(c, &c->src_coef, coef, (size_t)nsrc * c->Np))) return rc;
+  c->nsrc = nsrc;
+  c->src_first = elems[0];
+  c->src_last = elems[nsrc - 1];
+  for (int f = 0; f < 9; ++f) c->src_w[f] = f < c->Nf ? (float)(weights[f] * amp) : 0.f;
+  c->src_f0 = f0;
+  c->src_amp = amp;
+  return BBW_OK;
+}
+
+int bbw_set_receivers(bbw_ctx *c, int nrec, const int32_t *elems, const float *basis,
+                      int max_steps) {
+  if (!c || nrec < 0 || max_steps < 0) return BBW_ERR_ARG;
+  dfree(c->rec_elems);
+  dfree(c->rec_basis);
+  dfree(c->rec_trace);
+  c->nrec = 0;
+  c->rec_steps = 0;
+  c->rec_max_steps = 0;
+  reset_graph(c);
+  if (nrec == 0) return BBW_OK;
+  if (!elems || !basis || max_steps == 0) return fail(c, BBW_ERR_ARG, "bad receivers");
+  for (int i = 0; i < nrec; ++i)
+    if (elems[i] < 0 || elems[i] >= c->K) return fail(c, BBW_ERR_ARG, "receiver element out of range");
+  int rc;
+  if ((rc = upload(c, &c->rec_elems, (const int *)elems, (size_t)nrec))) return rc;
+  if ((rc = upload(c, &c->rec_basis, basis, (size_t)nrec * c->Np))) return rc;
+  if ((rc = dalloc(c, &c->rec_trace, (size_t)nrec * max_steps))) return rc;
+  HIPCHK(c, hipMemsetAsync(c->rec_trace, 0, sizeof(float) * (size_t)nrec * max_steps, c->stream));
+  c->nrec = nrec;
+  c->rec_max_steps = max_steps;
+  return BBW_OK;
+}
+
+int bbw_get_traces(bbw_ctx *c, float *out, int nsteps) {
+  if (!c || !out || nsteps < 0 || nsteps > c->rec_max_steps) return BBW_ERR_ARG;
+  if (c->nrec == 0 || nsteps == 0) return BBW_OK;
+  HIPCHK(c, hipMemcpyAsync(out, c->rec_trace, sizeof(float) * (size_t)c->nrec * nsteps,
+                           hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return BBW_OK;
+}
+
+static int copy_state(bbw_ctx *c, float *dev, const float *hin, float *hout) {
+  if (!c || !dev || (!hin && !hout)) return BBW_ERR_ARG;
+  const size_t bytes = nstate(c) * sizeof(float);
+  if (hin) HIPCHK(c, hipMemcpyAsync(dev, hin, bytes, hipMemcpyHostToDevice, c->stream));
+  else HIPCHK(c, hipMemcpyAsync(hout, dev, bytes, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return BBW_OK;
+}
+
+int bbw_set_state(bbw_ctx *c, const float *Q) { return c ? copy_state(c, c->Q[c->cur], Q, nullptr) : BBW_ERR_ARG; }
+int bbw_get_state(bbw_ctx *c, float *Q) { return c ? copy_state(c, c->Q[c->cur], nullptr, Q) : BBW_ERR_ARG; }
+int bbw_set_res(bbw_ctx *c, const float *r) { return c ? copy_state(c, c->res, r, nullptr) : BBW_ERR_ARG; }
+int bbw_get_res(bbw_ctx *c, float *r) { return c ? copy_state(c, c->res, nullptr, r) : BBW_ERR_ARG; }
+int bbw_set_rhs(bbw_ctx *c, const float *r) { return c ? copy_state(c, c->rhs, r, nullptr) : BBW_ERR_ARG; }
+int bbw_get_rhs(bbw_ctx *c, float *r) { return c ? copy_state(c, c->rhs, nullptr, r) : BBW_ERR_ARG; }
+
+int bbw_volume(bbw_ctx *c) {
+  if (!c) return BBW_ERR_ARG;
+  if (!c->have_ops || !c->have_mesh) return fail(c, BBW_ERR_STATE, "operators/mesh not uploaded");
+  hipError_t e = c->Nf == 4 ? launch_volume(c) : launch_volume_elas(c);
+  return e == hipSuccess ? BBW_OK : hipfail(c, e, "volume launch");
+}
+
+int bbw_surface(bbw_ctx *c) {
+  if (!c) return BBW_ERR_ARG;
+  if (!c->have_ops || !c->have_mesh) return fail(c, BBW_ERR_STATE, "operators/mesh not uploaded");
+  hipError_t e = c->Nf == 4 ? launch_surface(c) : launch_surface_elas(c);
+  return e == hipSuccess ? BBW_OK : hipfail(c, e, "surface launch");
+}
+
+int bbw_update_wadg(bbw_ctx *c, float fa, float fb, float fdt, float ftime) {
+  if (!c) return BBW_ERR_ARG;
+  int rc = check_ready(c);
+  if (rc) return rc;
+  hipError_t e = c->Nf == 4 ? launch_update_wadg(c, fa, fb, fdt, nullptr, ftime)
+                            : launch_update_elas(c, fa, fb, fdt, nullptr, ftime);
+  return e == hipSuccess ? BBW_OK : hipfail(c, e, "update launch");
+}
+
+int bbw_update(bbw_ctx *c, float fa, float fb, float fdt) {
+  if (!c) return BBW_ERR_ARG;
+  hipError_t e = launch_update(c, fa, fb, fdt);
+  return e == hipSuccess ? BBW_OK : hipfail(c, e, "rk update launch");
+}
+
+int bbw_stage_fused(bbw_ctx *c, float fa, float fb, float fdt, float ftime, int projU) {
+  if (!c) return BBW_ERR_ARG;
+  int rc = check_ready(c);
+  if (rc) return rc;
+  if (c->Nf != 4) return fail(c, BBW_ERR_UNSUPPORTED, "fused stage is acoustic only");
+  hipError_t e = launch_fused(c, c->Q[c->cur], c->Q[1 - c->cur], fa, fb, fdt, nullptr, ftime, projU);
+  if (e != hipSuccess) return hipfail(c, e, "fused stage launch");
+  c->cur = 1 - c->cur;
+  return BBW_OK;
+}
+
+int bbw_step(bbw_ctx *c, int nsteps, double dt, double t0, int mode, int flags) {
+  if (!c || nsteps < 0 || !(dt > 0) || (mode != BBW_MODE_PLAIN && mode != BBW_MODE_FUSED))
+    return BBW_ERR_ARG;
+  int rc = check_ready(c);
+  if (rc) return rc;
+  if (mode == BBW_MODE_FUSED && c->Nf != 4)
+    return fail(c, BBW_ERR_UNSUPPORTED, "fused stage is acoustic only");
+  if (c->nrec && c->rec_steps + nsteps > c->rec_max_steps)
+    return fail(c, BBW_ERR_ARG, "receiver trace buffer too short");
+  const int projU = flags & 1;
+  const bool use_graph = (flags & 2) != 0;
+  const int G = 10;  // steps per captured graph (even: the fused ping-pong parity is restored)
+
+  // stage time-function values for all steps (host FP64 -> float)
+  const bool has_src = c->nsrc > 0;
+  std::vector<float> ft;
+  if (has_src) {
+    ft.resize((size_t)nsteps * 5);
+    for (int s = 0; s < nsteps; ++s)
+      for (int i = 0; i < 5; ++i)
+        ft[(size_t)s * 5 + i] = (float)ricker(t0 + s * dt + c->rk4c[i] * dt, c->src_f0);
+  }
+
+  int s = 0;
+  if (use_graph && nsteps >= G) {
+    // window buffers (graph nodes read their stage values and write receiver rows there)
+    if (c->ftime_win_len < G * 5) {
+      dfree(c->ftime_win);
+      if ((rc = dalloc(c, &c->ftime_win, (size_t)G * 5 + (size_t)G * (c->nrec > 0 ? c->nrec : 1)))) return rc;
+      c->ftime_win_len = G * 5;
+      reset_graph(c);
+    }
+    float *rec_win = c->ftime_win + G * 5;
+    const int parity = mode == BBW_MODE_FUSED ? c->cur : 0;
+    Graph &g = c->graph;
+    if (!g.exec || g.steps != G || g.mode != mode || g.flags != flags || g.parity != parity || g.dt != dt) {
+      reset_graph(c);
+      HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+      const int cur0 = c->cur;
+      int crc = BBW_OK;
+      for (int j = 0; j < G && !crc; ++j) {
+        for (int i = 0; i < 5 && !crc; ++i)
+          crc = do_stage(c, i, dt, has_src ? c->ftime_win + j * 5 + i : nullptr, 0.f, mode, projU);
+        if (!crc && c->nrec) {
+          hipError_t e = launch_record(c, c->Q[c->cur], rec_win + (size_t)j * c->nrec);
+          if (e != hipSuccess) crc = hipfail(c, e, "record launch");
+        }
+      }
+      hipGraph_t graph = nullptr;
+      hipError_t e = hipStreamEndCapture(c->stream, &graph);
+      c->cur = cur0;
+      if (crc) {
+        if (graph) hipGraphDestroy(graph);
+        return crc;
+      }
+      if (e != hipSuccess) return hipfail(c, e, "hipStreamEndCapture");
+      e = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
+      hipGraphDestroy(graph);
+      if (e != hipSuccess) return hipfail(c, e, "hipGraphInstantiate");
+      g.steps = G; g.mode = mode; g.flags = flags; g.parity = parity; g.dt = dt;
+    }
+    float *ft_dev = nullptr;
+    if (has_src) {
+      if ((rc = dalloc(c, &ft_dev, ft.size()))) return rc;
+      HIPCHK(c, hipMemcpyAsync(ft_dev, ft.data(), ft.size() * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    }
+    for (; s + G <= nsteps; s += G) {
+      if (has_src)
+        HIPCHK(c, hipMemcpyAsync(c->ftime_win, ft_dev + (size_t)s * 5, sizeof(float) * G * 5,
+                                 hipMemcpyDeviceToDevice, c->stream));
+      HIPCHK(c, hipGraphLaunch(g.exec, c->stream));
+      if (c->nrec) {
+        HIPCHK(c, hipMemcpyAsync(c->rec_trace + (size_t)c->rec_steps * c->nrec, rec_win,
+                                 sizeof(float) * G * c->nrec, hipMemcpyDeviceToDevice, c->stream));
+        c->rec_steps += G;
+      }
+    }
+    if (ft_dev) {
+      HIPCHK(c, hipStreamSynchronize(c->stream));
+      hipFree(ft_dev);
+    }
+  }
+  for (; s < nsteps; ++s) {
+    for (int i = 0; i < 5; ++i)
+      if ((rc = do_stage(c, i, dt, nullptr, has_src ? ft[(size_t)s * 5 + i] : 0.f, mode, projU))) return rc;
+    if (c->nrec) {
+      hipError_t e = launch_record(c, c->Q[c->cur], c->rec_trace + (size_t)c->rec_steps * c->nrec);
+      if (e != hipSuccess) return hipfail(c, e, "record launch");
+      c->rec_steps += 1;
+    }
+  }
+  return BBW_OK;
+}
+
+int bbw_energy(bbw_ctx *c, double *E) {
+  if (!c || !E) return BBW_ERR_ARG;
+  int rc = check_ready(c);
+  if (rc) return rc;
+  if (c->Nf != 4) return fail(c, BBW_ERR_UNSUPPORTED, "energy is acoustic only");
+  int nb = 0;
+  launch_energy(c, nullptr, &nb);
+  if (c->npartial < nb + 1) {
+    dfree(c->partial);
+    if ((rc = dalloc(c, &c->partial, (size_t)nb + 1))) return rc;
+    c->npartial = nb + 1;
+  }
+  hipError_t e = launch_energy(c, c->partial, &nb);
+  if (e != hipSuccess) return hipfail(c, e, "energy launch");
+  e = launch_reduce(c, c->partial, nb, c->partial + nb);
+  if (e != hipSuccess) return hipfail(c, e, "reduce launch");
+  HIPCHK(c, hipMemcpyAsync(E, c->partial + nb, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return BBW_OK;
+}
+
+int bbw_device_ptr(bbw_ctx *c, int which, void **ptr) {
+  if (!c || !ptr) return BBW_ERR_ARG;
+  switch (which) {
+    case 0: *ptr = c->Q[c->cur]; break;
+    case 1: *ptr = c->res; break;
+    case 2: *ptr = c->rhs; break;
+    case 3: *ptr = c->c2q; break;
+    default: return fail(c, BBW_ERR_ARG, "unknown buffer");
+  }
+  return BBW_OK;
+}
+
+}  // extern "C"

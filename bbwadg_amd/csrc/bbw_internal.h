@@ -1,0 +1,124 @@
+// Internal definitions shared by the HIP translation units of libbbwadg_hip.so.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/bbwadg.h"
+
+namespace bbw {
+
+// Compile-time sizes for polynomial order N.
+template <int N>
+struct Dims {
+  static constexpr int Np = (N + 1) * (N + 2) * (N + 3) / 6;
+  static constexpr int Nfp = (N + 1) * (N + 2) / 2;
+  static constexpr int NfpNf = 4 * Nfp;
+  // points of the degree 2N+1 tetrahedral rule used by the WADG update
+  static constexpr int Nq = N == 1 ? 6 : N == 2 ? 14 : N == 3 ? 31 : N == 4 ? 57 : N == 5 ? 95 : 146;
+  static constexpr int EEL_nnz = Nfp + 3;
+  static constexpr int L0_nnz = Nfp < 7 ? Nfp : 7;
+};
+
+inline int np_of(int N) { return (N + 1) * (N + 2) * (N + 3) / 6; }
+inline int nfp_of(int N) { return (N + 1) * (N + 2) / 2; }
+inline int nq_of(int N) {
+  static const int t[7] = {0, 6, 14, 31, 57, 95, 146};
+  return (N >= 1 && N <= 6) ? t[N] : -1;
+}
+
+// Source description passed to the update kernels.
+struct SrcArgs {
+  int nsrc;
+  int first, last;        // smallest / largest source element id
+  const int *elems;       // ascending element ids
+  const float *coef;      // [nsrc][Np]
+  float w[9];             // per-field weights x amplitude x time function (set per stage)
+};
+
+struct Graph {
+  hipGraphExec_t exec = nullptr;
+  int steps = 0, mode = -1, flags = -1, parity = -1;
+  double dt = 0;
+};
+
+}  // namespace bbw
+
+struct bbw_ctx {
+  int N = 0, Np = 0, Nfp = 0, Nq = 0, Nf = 0, K = 0, dev = 0;
+  int KblkV = 0, KblkS = 0, KblkU = 0, KblkF = 0;
+  hipStream_t own_stream = nullptr, stream = nullptr;
+  std::string err;
+
+  // operators
+  int4 *Dids = nullptr;
+  float4 *Dvals = nullptr;
+  int *Fmask = nullptr, *EEL_ids = nullptr, *L0_ids = nullptr;
+  float *EEL_vals = nullptr, *L0_vals = nullptr, *Vq = nullptr, *Pq = nullptr;
+  double *Vq64 = nullptr, *wq64 = nullptr;
+  double rk4a[5] = {0}, rk4b[5] = {0}, rk4c[6] = {0};
+  bool have_ops = false;
+
+  // mesh
+  float *vgeo = nullptr, *fgeo = nullptr;
+  int *vmapP = nullptr;
+  double *J = nullptr;
+  bool have_mesh = false;
+
+  // media
+  float *c2q = nullptr, *Jq = nullptr;
+  float *rhoinv = nullptr, *lambda = nullptr, *mu = nullptr;
+  float tau_v = 1.f, tau_s = 1.f;
+  bool have_media = false;
+
+  // state
+  float *Q[2] = {nullptr, nullptr};
+  int cur = 0;
+  float *res = nullptr, *rhs = nullptr;
+
+  // source
+  int nsrc = 0, src_first = 0, src_last = -1;
+  int *src_elems = nullptr;
+  float *src_coef = nullptr;
+  float src_w[9] = {0};
+  double src_f0 = 0, src_amp = 0;
+
+  // receivers
+  int nrec = 0, rec_max_steps = 0, rec_steps = 0;
+  int *rec_elems = nullptr;
+  float *rec_basis = nullptr, *rec_trace = nullptr;
+
+  // energy scratch
+  double *partial = nullptr;
+  int npartial = 0;
+
+  // graph cache and per-window stage data
+  bbw::Graph graph;
+  float *ftime_win = nullptr;  // [steps_in_window * 5]
+  int ftime_win_len = 0;
+};
+
+namespace bbw {
+
+// Launchers implemented in bbw_acoustic.hip / bbw_elastic.hip; return hipError_t.
+hipError_t launch_volume(bbw_ctx *c);
+hipError_t launch_surface(bbw_ctx *c);
+hipError_t launch_update_wadg(bbw_ctx *c, float fa, float fb, float fdt, const float *ftime_dev,
+                              float ftime);
+hipError_t launch_update(bbw_ctx *c, float fa, float fb, float fdt);
+hipError_t launch_fused(bbw_ctx *c, const float *Qin, float *Qout, float fa, float fb, float fdt,
+                        const float *ftime_dev, float ftime, int project_velocity);
+hipError_t launch_energy(bbw_ctx *c, double *E_dev_partial, int *nblocks);
+hipError_t launch_reduce(bbw_ctx *c, const double *partial, int n, double *out);
+hipError_t launch_record(bbw_ctx *c, const float *Q, float *row);
+
+hipError_t launch_volume_elas(bbw_ctx *c);
+hipError_t launch_surface_elas(bbw_ctx *c);
+hipError_t launch_update_elas(bbw_ctx *c, float fa, float fb, float fdt, const float *ftime_dev,
+                              float ftime);
+
+bool order_supported(int N);
+
+}  // namespace bbw

@@ -1,0 +1,252 @@
+"""Reference-element operators for the BB-WADG solver (FP64 host setup).
+
+Restates the Bernstein block of ``StartUp3d`` (``cpp/src/StartUp3d.cpp:22-57``,
+``:228-407``, ``:517-537``) and the WADG projection of ``InitWADG_subelem``
+(``cpp/src/WaveOKL3d.cpp:419-483``):
+
+* ``Fmask``: face-node lists (faces t=-1, s=-1, r+s+t=-1, r=-1);
+* ``D_ids``/``D_vals``: the four barycentric derivative operators, 4 non-zeros
+  per row, as fixed-width sparse rows (``get_sparse_ids``, relative tolerance
+  1e-5, columns ascending; ``cpp/src/Basis.cpp:980-1007``).  The reference keeps
+  one value array shared by all four operators (the values of the last one); the
+  per-row value pattern (l, k, j, i) is the same for all four, so this is exact;
+* the BB lift ``LIFTB = MB^-1 EmatB`` factored as ``EEL . blkdiag(L0, L0, L0, L0)``
+  with ``EEL`` (Np x 4Nfp, Nfp+3 non-zeros per row) built from triangle degree
+  elevation matrices scaled by ``(-1)^i C(N,i)/(1+i)`` and a row permutation per
+  face, and ``L0 = LIFTB[:Nfp, :Nfp]`` (at most 7 non-zeros per row);
+* ``Vq``/``Pq``: BB interpolation to the degree 2N+1 cubature and the L2
+  projection back, ``Pq = (Vq^T W Vq)^-1 Vq^T W``;
+* the Carpenter--Kennedy LSRK45 coefficients.
+"""
+from dataclasses import dataclass, field
+from math import comb
+
+import numpy as np
+
+from . import basis
+
+NODETOL = 1e-6  # cpp/include/Mesh.h:13
+NFACES = 4
+
+
+def get_sparse_ids(A, tol=1e-5):
+    """Fixed-width sparse rows of A: (cols, vals), ascending columns, zero padded.
+
+    Entries with |A_ij| > tol * max|A| are kept (``cpp/src/Basis.cpp:980-1007``).
+    """
+    A = np.asarray(A, dtype=np.float64)
+    thresh = tol * np.abs(A).max()
+    mask = np.abs(A) > thresh
+    width = int(mask.sum(axis=1).max())
+    cols = np.zeros((A.shape[0], width), dtype=np.int32)
+    vals = np.zeros((A.shape[0], width))
+    for i in range(A.shape[0]):
+        nz = np.nonzero(mask[i])[0]
+        cols[i, : len(nz)] = nz
+        vals[i, : len(nz)] = A[i, nz]
+    return cols, vals
+
+
+def _fixed_width(cols, vals, width):
+    """Pad or truncate sparse rows to ``width`` (the reference's fixed nnz)."""
+    n = cols.shape[0]
+    c = np.zeros((n, width), dtype=np.int32)
+    v = np.zeros((n, width))
+    w = min(width, cols.shape[1])
+    c[:, :w] = cols[:, :w]
+    v[:, :w] = vals[:, :w]
+    return c, v, cols.shape[1]
+
+
+def face_masks(r, s, t, N):
+    """Fmask[f] = node ids on face f, in node order (``StartUp3d.cpp:41-57``)."""
+    conds = [np.abs(1 + t), np.abs(1 + s), np.abs(1 + r + s + t), np.abs(1 + r)]
+    Fm = [np.nonzero(c < NODETOL)[0] for c in conds]
+    nfp = basis.np_tri(N)
+    for f in Fm:
+        if len(f) != nfp:
+            raise RuntimeError("face node count mismatch")
+    return np.stack(Fm).astype(np.int32)  # (4, Nfp)
+
+
+def lsrk45_coefficients():
+    """Carpenter--Kennedy (1994) 5-stage 4th-order 2N-storage RK (``StartUp3d.cpp:517-537``)."""
+    a = np.array([0.0,
+                  -567301805773.0 / 1357537059087.0,
+                  -2404267990393.0 / 2016746695238.0,
+                  -3550918686646.0 / 2091501179385.0,
+                  -1275806237668.0 / 842570457699.0])
+    b = np.array([1432997174477.0 / 9575080441755.0,
+                  5161836677717.0 / 13612068292357.0,
+                  1720146321549.0 / 2090206949498.0,
+                  3134564353537.0 / 4481467310338.0,
+                  2277821191437.0 / 14882151754819.0])
+    c = np.array([0.0,
+                  1432997174477.0 / 9575080441755.0,
+                  2526269341429.0 / 6820363962896.0,
+                  2006345519317.0 / 3224310063776.0,
+                  2802321613138.0 / 2924317926251.0,
+                  1.0])
+    return a, b, c
+
+
+@dataclass
+class RefOps:
+    """Reference-element operators for one polynomial order N (all FP64)."""
+    N: int
+    Np: int
+    Nfp: int
+    r: np.ndarray
+    s: np.ndarray
+    t: np.ndarray
+    Fmask: np.ndarray          # (4, Nfp) int32
+    VB: np.ndarray             # (Np, Np) BB -> nodal values
+    invVB: np.ndarray
+    DB: list                   # 4 dense (Np, Np) barycentric derivative matrices
+    D_ids: np.ndarray          # (4 ops, Np, 4) int32: D_ids[m, i, j] = j-th column of row i of D_{m+1}
+    D_vals: np.ndarray         # (Np, 4) values shared by the four operators
+    LIFTB: np.ndarray          # (Np, 4Nfp)
+    EEL: np.ndarray            # (Np, 4Nfp) dense
+    EEL_ids: np.ndarray        # (Np, EEL_nnz) int32
+    EEL_vals: np.ndarray       # (Np, EEL_nnz)
+    EEL_nnz: int
+    L0: np.ndarray             # (Nfp, Nfp) dense
+    L0_ids: np.ndarray         # (Nfp, L0_nnz) int32
+    L0_vals: np.ndarray
+    L0_nnz: int
+    cEL: np.ndarray            # (N+1,)
+    rowperm: np.ndarray        # (Np, 3)
+    MB: np.ndarray             # BB mass matrix on the reference element
+    rq: np.ndarray
+    sq: np.ndarray
+    tq: np.ndarray
+    wq: np.ndarray
+    Vq: np.ndarray             # (Nq, Np) BB interpolation to cubature (WADG rule)
+    Pq: np.ndarray             # (Np, Nq) L2 projection
+    rk4a: np.ndarray = field(default_factory=lambda: lsrk45_coefficients()[0])
+    rk4b: np.ndarray = field(default_factory=lambda: lsrk45_coefficients()[1])
+    rk4c: np.ndarray = field(default_factory=lambda: lsrk45_coefficients()[2])
+    sparse_widths: dict = field(default_factory=dict)
+
+    @property
+    def Nq(self):
+        return len(self.wq)
+
+    @property
+    def NfpNfaces(self):
+        return NFACES * self.Nfp
+
+
+def build_ref_ops(N):
+    """Build every reference-element operator for order N (``StartUp3d`` + WADG)."""
+    if N < 1:
+        raise ValueError("N must be >= 1")
+    Np, Nfp = basis.np_tet(N), basis.np_tri(N)
+    r, s, t = basis.nodes3d(N)
+    Fmask = face_masks(r, s, t, N)
+
+    VB = basis.bern_tet(N, r, s, t)
+    invVB = np.linalg.inv(VB)
+    G = basis.grad_bern_tet(N, r, s, t)
+    DB = [np.linalg.solve(VB, Gm) for Gm in G]
+
+    widths = {}
+    D_ids = np.zeros((4, Np, 4), dtype=np.int32)
+    D_vals = None
+    for m in range(4):
+        cols, vals = get_sparse_ids(DB[m])
+        c, v, w = _fixed_width(cols, vals, 4)
+        widths[f"D{m + 1}"] = w
+        D_ids[m] = c
+        D_vals = v  # overwritten by each operator, as StartUp3d.cpp:255-259
+
+    # --- BB lift: LIFTB = MB \ EmatB (StartUp3d.cpp:286-307)
+    qdeg = min(21, 2 * N + 1)
+    rqt, sqt, wqt = basis.tri_cubature(qdeg)
+    Vface = basis.bern_tri(N, rqt, sqt)
+    massFace = Vface.T @ (wqt[:, None] * Vface)
+    EmatB = np.zeros((Np, NFACES * Nfp))
+    for f in range(NFACES):
+        for i in range(Nfp):
+            EmatB[Fmask[f, i], f * Nfp:(f + 1) * Nfp] += massFace[i]
+    rq, sq, tq, wq = basis.tet_cubature(qdeg)
+    VBq = basis.bern_tet(N, rq, sq, tq)
+    MB = VBq.T @ (wq[:, None] * VBq)
+    LIFTB = np.linalg.solve(MB, EmatB)
+
+    # --- EEL1: stacked cEL_i * E_{N-i}^{N T} (StartUp3d.cpp:312-327)
+    cEL = np.array([(-1) ** i * comb(N, i) / (1 + i) for i in range(N + 1)])
+    VBTri = basis.bern_tri(N, rqt, sqt)
+    blocks = []
+    for i in range(N + 1):
+        VBi = basis.bern_tri(N - i, rqt, sqt)
+        Ei = np.linalg.lstsq(VBTri, VBi, rcond=None)[0]  # (Nfp, NpElev): elevation N-i -> N
+        blocks.append(cEL[i] * Ei.T)
+    EEL1 = np.vstack(blocks)
+    assert EEL1.shape == (Np, Nfp)
+
+    # --- face permutations by matching LIFTB rows (StartUp3d.cpp:330-356)
+    LIFTBf1 = LIFTB[:, :Nfp]
+    maxVal = np.abs(LIFTB).max()
+    rowperm = np.zeros((Np, NFACES - 1), dtype=np.int32)
+    for f in range(1, NFACES):
+        LIFTBf = LIFTB[:, f * Nfp:(f + 1) * Nfp]
+        for i in range(Np):
+            diff = np.abs(LIFTBf1 - LIFTBf[i][None, :]).max(axis=1)
+            hits = np.nonzero(diff < NODETOL * maxVal)[0]
+            if len(hits) == 0:
+                raise RuntimeError(f"no lift row match for face {f}, row {i}")
+            rowperm[i, f - 1] = hits[0]
+    EEL = np.zeros((Np, NFACES * Nfp))
+    EEL[:, :Nfp] = EEL1
+    for f in range(1, NFACES):
+        EEL[:, f * Nfp:(f + 1) * Nfp] = EEL1[rowperm[:, f - 1]]
+    cols, vals = get_sparse_ids(EEL)
+    EEL_nnz = Nfp + 3  # StartUp3d.cpp:375
+    EEL_ids, EEL_vals, widths["EEL"] = _fixed_width(cols, vals, EEL_nnz)
+
+    L0 = LIFTB[:Nfp, :Nfp].copy()
+    cols, vals = get_sparse_ids(L0)
+    L0_nnz = min(Nfp, 7)  # StartUp3d.cpp:395, WaveOKL3d.cpp:1492
+    L0_ids, L0_vals, widths["L0"] = _fixed_width(cols, vals, L0_nnz)
+
+    # --- WADG projection on the degree 2N+1 rule (WaveOKL3d.cpp:425-483)
+    if 2 * N + 1 > 15:
+        raise NotImplementedError("orders with 2N+1 > 15 use a tensor-product rule (not provided)")
+    rqw, sqw, tqw, wqw = basis.tet_cubature(min(21, 2 * N + 1))
+    Vq = basis.bern_tet(N, rqw, sqw, tqw)
+    Mq = Vq.T @ (wqw[:, None] * Vq)
+    Pq = np.linalg.solve(Mq, Vq.T * wqw[None, :])
+
+    return RefOps(N=N, Np=Np, Nfp=Nfp, r=r, s=s, t=t, Fmask=Fmask, VB=VB, invVB=invVB,
+                  DB=DB, D_ids=D_ids, D_vals=D_vals, LIFTB=LIFTB, EEL=EEL,
+                  EEL_ids=EEL_ids, EEL_vals=EEL_vals, EEL_nnz=EEL_nnz, L0=L0,
+                  L0_ids=L0_ids, L0_vals=L0_vals, L0_nnz=L0_nnz, cEL=cEL, rowperm=rowperm,
+                  MB=MB, rq=rqw, sq=sqw, tq=tqw, wq=wqw, Vq=Vq, Pq=Pq, sparse_widths=widths)
+
+
+def packed_device_ops(ops):
+    """Pack operators in the device layouts of ``WaveInitOCCA3d`` (FP32/int32).
+
+    * ``D_ids`` as 4 int4 arrays: ``D_idsj[i] = (D1[i][j], D2[i][j], D3[i][j], D4[i][j])``
+      (``cpp/src/WaveOKL3d.cpp:1452-1480``); here shape (4 j, Np, 4 m);
+    * ``Dvals4[i] = D_vals[i][0:4]`` (row-major float4);
+    * ``EEL``, ``L0`` ids/values column-major: entry (i, j) at ``i + j*rows``
+      (``cpp/src/WaveOKL3d.cpp:1485-1501``);
+    * ``Vq`` column-major Nq x Np, ``Pq`` column-major Np x Nq
+      (``cpp/okl/WaveKernelsWADG.okl:1812,1849``);
+    * ``Fmask`` row-major (4, Nfp).
+    """
+    f32 = np.float32
+    D_ids = np.ascontiguousarray(np.transpose(ops.D_ids, (2, 1, 0)), dtype=np.int32)  # [j][i][m]
+    return {
+        "Fmask": np.ascontiguousarray(ops.Fmask, dtype=np.int32),
+        "D_ids": D_ids,
+        "Dvals": np.ascontiguousarray(ops.D_vals, dtype=f32),
+        "EEL_ids": np.ascontiguousarray(ops.EEL_ids.T, dtype=np.int32),
+        "EEL_vals": np.ascontiguousarray(ops.EEL_vals.T, dtype=f32),
+        "L0_ids": np.ascontiguousarray(ops.L0_ids.T, dtype=np.int32),
+        "L0_vals": np.ascontiguousarray(ops.L0_vals.T, dtype=f32),
+        "Vq": np.ascontiguousarray(ops.Vq.T, dtype=f32),   # column-major Nq x Np
+        "Pq": np.ascontiguousarray(ops.Pq.T, dtype=f32),   # column-major Np x Nq
+    }

@@ -1,0 +1,226 @@
+"""Host-side driver mirroring the reference's solver interface, over the HIP C ABI.
+
+Reference call sequence (``cpp/main_subelem.cpp:19-104``)::
+
+    mesh = ReadGmsh3d(file); FacePair3d(mesh); StartUp3d(mesh)
+    dt = WaveInitOCCA3d(mesh, KblkV, KblkS, KblkU, ...)
+    InitWADG_subelem(mesh, c2); WaveSetU0(mesh, Q, 0, uex); WaveSetData3d(Q)
+    Wave_RK(mesh, FinalTime, dt, 1); WaveGetData3d(mesh, Q); compute_error(...)
+
+Here::
+
+    s = WaveSolver(mesh, N)                 # FacePair3d + StartUp3d + WaveInitOCCA3d
+    s.InitWADG_subelem(c2)                  # c^2 at the WADG cubature -> device
+    Q = s.WaveSetU0(uex); s.WaveSetData3d(Q)
+    s.Wave_RK(FinalTime, dt)                # or s.step(nsteps, dt, mode=...)
+    Q = s.WaveGetData3d()                   # BB coefficients, [K][Nf][Np]
+    L2err, relL2err = s.compute_error(FinalTime, uex)
+
+Every numerical call goes to ``libbbwadg_hip.so``; there is no CPU path.
+``WaveGetData3d`` returns BB coefficients (the reference converts to nodal
+values with VB on read-back; ``to_nodal`` does that).
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib, media, mesh as meshmod, operators
+from ._lib import MODE_FUSED, MODE_PLAIN, check, f32, f64, i32, ptr
+
+
+class WaveSolver:
+    def __init__(self, mesh, N, nfields=4, KblkV=0, KblkS=0, KblkU=0, KblkF=0, device=0,
+                 ops=None, dmesh=None):
+        if nfields not in (4, 9):
+            raise ValueError("nfields must be 4 (acoustic) or 9 (elastic)")
+        self.lib = _lib.load()
+        self.mesh = mesh
+        if mesh.EToE is None:
+            meshmod.face_pair(mesh)
+        self.ops = ops if ops is not None else operators.build_ref_ops(N)
+        self.N, self.nfields = N, nfields
+        self.dmesh = dmesh if dmesh is not None else meshmod.device_arrays(mesh, self.ops, nfields)
+        self.K = mesh.K
+        cfg = _lib.Config(N=N, Nfields=nfields, K=self.K, Nq=self.ops.Nq, KblkV=KblkV,
+                          KblkS=KblkS, KblkU=KblkU, KblkF=KblkF, device=device)
+        h = ctypes.c_void_p()
+        rc = self.lib.bbw_create(ctypes.byref(cfg), ctypes.byref(h))
+        if rc != 0:
+            raise _lib.BBWError(rc, "bbw_create failed")
+        self.ctx = h
+        self._upload_ops()
+        dm = self.dmesh
+        self._keep = [f32(dm.vgeo), f32(dm.fgeo), i32(dm.vmapP), f64(dm.J)]
+        check(self.ctx, self.lib.bbw_upload_mesh(self.ctx, *[ptr(a) for a in self._keep]))
+        # reference time step estimate (cpp/src/WaveOKL3d.cpp:1677; elastic :1296-1297)
+        if nfields == 4:
+            self.dt_ref = 0.25 / ((N + 1) ** 2 * dm.FscaleMax)
+        else:
+            self.dt_ref = 0.25 / ((N + 1) * (N + 3) / 3.0 * dm.FscaleMax)
+        self.time = 0.0
+
+    # -- setup ---------------------------------------------------------------------
+    def _upload_ops(self):
+        o = operators.packed_device_ops(self.ops)
+        self._ops_keep = dict(o, wq=f64(self.ops.wq), rk4a=f64(self.ops.rk4a),
+                              rk4b=f64(self.ops.rk4b), rk4c=f64(self.ops.rk4c))
+        k = self._ops_keep
+        s = _lib.Ops(Fmask=ptr(k["Fmask"]).value, D_ids=ptr(k["D_ids"]).value,
+                     Dvals=ptr(k["Dvals"]).value, EEL_ids=ptr(k["EEL_ids"]).value,
+                     EEL_vals=ptr(k["EEL_vals"]).value, EEL_nnz=self.ops.EEL_nnz,
+                     L0_ids=ptr(k["L0_ids"]).value, L0_vals=ptr(k["L0_vals"]).value,
+                     L0_nnz=self.ops.L0_nnz, Vq=ptr(k["Vq"]).value, Pq=ptr(k["Pq"]).value,
+                     wq=ptr(k["wq"]).value, rk4a=ptr(k["rk4a"]).value, rk4b=ptr(k["rk4b"]).value,
+                     rk4c=ptr(k["rk4c"]).value)
+        check(self.ctx, self.lib.bbw_upload_operators(self.ctx, ctypes.byref(s)))
+
+    def cubature_points(self):
+        return media.cubature_points(self.mesh, self.ops)
+
+    def InitWADG_subelem(self, c2, Jq=None):
+        """c2: callable c2(x, y, z) or array (K, Nq) at the WADG cubature."""
+        if callable(c2):
+            xq, yq, zq = self.cubature_points()
+            c2q = c2(xq, yq, zq)
+        else:
+            c2q = np.asarray(c2)
+        self.c2q = f32(np.broadcast_to(c2q, (self.K, self.ops.Nq)))
+        Jq = None if Jq is None else f32(Jq)
+        check(self.ctx, self.lib.bbw_upload_media(self.ctx, ptr(self.c2q), ptr(Jq)))
+
+    def InitElastic(self, rhoinvq, lamq, muq, tau_v=1.0, tau_s=1.0):
+        self.rhoinvq, self.lamq, self.muq = f32(rhoinvq), f32(lamq), f32(muq)
+        check(self.ctx, self.lib.bbw_upload_elastic_media(self.ctx, ptr(self.rhoinvq), ptr(self.lamq),
+                                                          ptr(self.muq), tau_v, tau_s))
+
+    def set_source(self, elems, coef, f0, amp=1.0, weights=None):
+        w = np.zeros(9)
+        if weights is None:
+            w[0] = 1.0
+        else:
+            w[:len(weights)] = weights
+        elems, coef = i32(elems), f32(coef)
+        check(self.ctx, self.lib.bbw_set_source(self.ctx, len(elems), ptr(elems), ptr(coef),
+                                                ptr(f32(w)), float(f0), float(amp)))
+
+    def set_receivers(self, elems, basis_vals, max_steps):
+        elems, basis_vals = i32(elems), f32(basis_vals)
+        self.nrec = len(elems)
+        check(self.ctx, self.lib.bbw_set_receivers(self.ctx, len(elems), ptr(elems), ptr(basis_vals),
+                                                   int(max_steps)))
+
+    def get_traces(self, nsteps):
+        out = np.zeros((nsteps, self.nrec), dtype=np.float32)
+        check(self.ctx, self.lib.bbw_get_traces(self.ctx, ptr(out), nsteps))
+        return out
+
+    def WaveSetU0(self, uex, field=0, time=0.0):
+        """Interpolate uex at the nodes and convert to BB coefficients (``WaveSetU0``)."""
+        x, y, z = meshmod.affine_points(self.mesh, self.ops.r, self.ops.s, self.ops.t)
+        Q = np.zeros((self.K, self.nfields, self.ops.Np))
+        Q[:, field] = (self.ops.invVB @ uex(x, y, z, time)).T
+        return f32(Q)
+
+    # -- state ---------------------------------------------------------------------
+    def _state(self, fn, arr=None):
+        if arr is None:
+            out = np.empty((self.K, self.nfields, self.ops.Np), dtype=np.float32)
+            check(self.ctx, fn(self.ctx, ptr(out)))
+            return out
+        a = f32(arr).reshape(self.K, self.nfields, self.ops.Np)
+        check(self.ctx, fn(self.ctx, ptr(a)))
+
+    def WaveSetData3d(self, Q):
+        self._state(self.lib.bbw_set_state, Q)
+
+    def WaveGetData3d(self):
+        return self._state(self.lib.bbw_get_state)
+
+    def set_res(self, r):
+        self._state(self.lib.bbw_set_res, r)
+
+    def get_res(self):
+        return self._state(self.lib.bbw_get_res)
+
+    def set_rhs(self, r):
+        self._state(self.lib.bbw_set_rhs, r)
+
+    def get_rhs(self):
+        return self._state(self.lib.bbw_get_rhs)
+
+    def to_nodal(self, Q):
+        return np.einsum("ij,kfj->kfi", self.ops.VB, Q)
+
+    # -- kernels -------------------------------------------------------------------
+    def volume(self):
+        check(self.ctx, self.lib.bbw_volume(self.ctx))
+
+    def surface(self):
+        check(self.ctx, self.lib.bbw_surface(self.ctx))
+
+    def update_wadg(self, fa, fb, fdt, ftime=0.0):
+        check(self.ctx, self.lib.bbw_update_wadg(self.ctx, fa, fb, fdt, ftime))
+
+    def update(self, fa, fb, fdt):
+        check(self.ctx, self.lib.bbw_update(self.ctx, fa, fb, fdt))
+
+    def stage_fused(self, fa, fb, fdt, ftime=0.0, project_velocity=False):
+        check(self.ctx, self.lib.bbw_stage_fused(self.ctx, fa, fb, fdt, ftime, int(project_velocity)))
+
+    def RK_step_WADG_subelem(self, fa, fb, fdt, ftime=0.0):
+        """One stage of the plain path: volume, surface, WADG update (three launches)."""
+        self.volume()
+        self.surface()
+        self.update_wadg(fa, fb, fdt, ftime)
+
+    def step(self, nsteps, dt, mode=MODE_FUSED, project_velocity=False, graph=True, t0=None):
+        t0 = self.time if t0 is None else t0
+        flags = (1 if project_velocity else 0) | (2 if graph else 0)
+        check(self.ctx, self.lib.bbw_step(self.ctx, int(nsteps), float(dt), float(t0), int(mode), flags))
+        self.time = t0 + nsteps * dt
+
+    def Wave_RK(self, FinalTime, dt, mode=MODE_PLAIN, **kw):
+        """Steps of size dt to FinalTime, the last one shortened (``Wave_RK``)."""
+        nfull = int(np.floor(FinalTime / dt + 1e-12))
+        self.step(nfull, dt, mode=mode, **kw)
+        rem = FinalTime - nfull * dt
+        if rem > 1e-14 * max(1.0, FinalTime):
+            self.step(1, rem, mode=mode, graph=False)
+
+    def sync(self):
+        check(self.ctx, self.lib.bbw_sync(self.ctx))
+
+    def energy(self):
+        E = ctypes.c_double()
+        check(self.ctx, self.lib.bbw_energy(self.ctx, ctypes.byref(E)))
+        return E.value
+
+    def compute_error(self, time, uex, Q=None, field=0):
+        """(L2err, relL2err) of one field at the WADG cubature (``compute_error``)."""
+        Q = self.WaveGetData3d() if Q is None else Q
+        xq, yq, zq = self.cubature_points()
+        uq = np.asarray(Q, dtype=np.float64)[:, field] @ self.ops.Vq.T
+        ue = uex(xq, yq, zq, time)
+        w = self.ops.wq[None, :] * self.dmesh.J[:, None]
+        e2 = float(np.sum((uq - ue) ** 2 * w))
+        n2 = float(np.sum(ue ** 2 * w))
+        return np.sqrt(e2), np.sqrt(e2) / np.sqrt(n2)
+
+    def device_ptr(self, which):
+        p = ctypes.c_void_p()
+        check(self.ctx, self.lib.bbw_device_ptr(self.ctx, which, ctypes.byref(p)))
+        return p.value
+
+    def set_stream(self, stream_handle):
+        check(self.ctx, self.lib.bbw_set_stream(self.ctx, ctypes.c_void_p(stream_handle)))
+
+    def close(self):
+        if getattr(self, "ctx", None):
+            self.lib.bbw_destroy(self.ctx)
+            self.ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

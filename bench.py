@@ -1,0 +1,182 @@
+"""Benchmark: DOF-stage updates/s of the BB-WADG acoustic solver on one MI355X.
+
+Workload (default): heterogeneous acoustic single shot, cube [-0.5,0.5]^3 split into
+48^3 x 6 = 663,552 tets, N = 5 (Np = 56, Nq = 95), seeded Fourier c^2 in [0.6, 1.4] at the
+cubature points, Ricker point-like source f0 = 10 at a seeded location, dt = 0.5 h / N^2,
+FP32.  A step is one LSRK45 step (5 stages) over the whole mesh, inputs resident in HBM.
+
+Reported: the fused path (one kernel per stage, HIP-graph replay) as ``value``; the plain
+three-launch path of the reference (volume, surface, WADG update) beside it; the
+roofline of the fused stage kernel; the CPU oracle timed on a bounded sample.
+
+    python bench.py [--steps K] [--warmup W] [--gpus 1] [--mesh 48] [--N 5]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def stage_bytes(N, Nf=4, Nm=1):
+    """Compulsory HBM bytes per element-stage of the fused stage: Q read + write, res read +
+    write (4 Nf Np), neighbour traces (Nf 4 Nfp), geometry (9 + 16), connectivity (4 Nfp),
+    media at cubature (Nm Nq); 4 bytes per word."""
+    Np = (N + 1) * (N + 2) * (N + 3) // 6
+    Nfp = (N + 1) * (N + 2) // 2
+    Nq = {1: 6, 2: 14, 3: 31, 4: 57, 5: 95, 6: 146}[N]
+    return 4 * (4 * Nf * Np + Nf * 4 * Nfp + 25 + 4 * Nfp + Nm * Nq)
+
+
+def build_problem(n, N, seed=1):
+    from bbwadg_amd import media, mesh as M, operators as O
+    ops = O.build_ref_ops(N)
+    m = M.cube_mesh(n)
+    M.face_pair(m)
+    dm = M.device_arrays(m, ops, 4)
+    xq, yq, zq = media.cubature_points(m, ops)
+    c2q = media.c2_fourier(seed)(xq, yq, zq).astype(np.float32)
+    rng = np.random.default_rng(seed + 1)
+    x0 = rng.uniform(-0.3, 0.3, 3)
+    src = media.gaussian_source(m, ops, x0, a=100.0)
+    return ops, m, dm, c2q, src, x0
+
+
+def cpu_baseline(ops, dm, c2q, K, sample_elems, threads):
+    """Oracle FP32 (OpenMP) over the first ``sample_elems`` elements for one LSRK step."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    os.environ["OMP_NUM_THREADS"] = str(threads)
+    import oracle as orc
+    oo = orc.OracleOps(ops, np.float32)
+    Ks = min(sample_elems, K)
+    rng = np.random.default_rng(0)
+    Q = rng.uniform(-1, 1, (K, 4, ops.Np)).astype(np.float32)
+    res = np.zeros_like(Q)
+    rhs = np.zeros_like(Q)
+    vgeo = np.ascontiguousarray(dm.vgeo)
+    fgeo = np.ascontiguousarray(dm.fgeo)
+    t0 = time.perf_counter()
+    for i in range(5):
+        orc.volume(oo, Ks, vgeo, Q, rhs)
+        orc.surface(oo, Ks, fgeo, dm.vmapP, Q, rhs)
+        orc.update_wadg(oo, Ks, c2q, np.float32(ops.rk4a[i]), np.float32(ops.rk4b[i]),
+                        np.float32(1e-4), rhs, res, Q)
+    el = time.perf_counter() - t0
+    return Ks * ops.Np * 4 * 5 / el, el, Ks
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--mesh", type=int, default=48)
+    ap.add_argument("--N", type=int, default=5)
+    ap.add_argument("--plain-steps", type=int, default=200)
+    ap.add_argument("--cpu-sample", type=int, default=32768)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported "
+              "(shots and element partitions shard across GPUs by design, not measured here)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from bbwadg_amd.solver import WaveSolver, MODE_FUSED, MODE_PLAIN
+
+    N, n = args.N, args.mesh
+    t_setup = time.perf_counter()
+    ops, m, dm, c2q, (selems, scoef), x0 = build_problem(n, N)
+    K = m.K
+    dt = 0.5 * (1.0 / n) / N ** 2
+    s = WaveSolver(m, N, ops=ops, dmesh=dm)
+    s.InitWADG_subelem(c2q)
+    s.set_source(selems, scoef, f0=10.0, amp=1.0)
+    stream = torch.cuda.current_stream()
+    s.set_stream(stream.cuda_stream)
+    Q0 = np.zeros((K, 4, ops.Np), dtype=np.float32)
+    s.WaveSetData3d(Q0)
+    setup_s = time.perf_counter() - t_setup
+    print(f"setup {setup_s:.1f} s, K={K}, N={N}, dt={dt:.3e}", file=sys.stderr, flush=True)
+
+    def timed(nsteps, mode):
+        torch.cuda.synchronize()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        s.step(nsteps, dt, mode=mode, graph=True, t0=0.0)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1)
+
+    s.step(args.warmup, dt, mode=MODE_FUSED, graph=True, t0=0.0)
+    ms = timed(args.steps, MODE_FUSED)
+    dof_stage = K * ops.Np * 4 * 5
+    value = dof_stage * args.steps / (ms / 1e3)
+    stage_ms = ms / (5 * args.steps)
+    E = s.energy()
+    if not np.isfinite(E):
+        raise RuntimeError("non-finite energy after the timed run")
+
+    # plain path (the reference's three launches per stage)
+    s.WaveSetData3d(Q0)
+    s.set_res(np.zeros_like(Q0))
+    s.step(2, dt, mode=MODE_PLAIN, graph=True, t0=0.0)
+    ms_plain = timed(args.plain_steps, MODE_PLAIN)
+    plain_value = dof_stage * args.plain_steps / (ms_plain / 1e3)
+
+    Bst = stage_bytes(N)
+    achieved = Bst * K / (stage_ms / 1e3) / 1e9
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", f"traffic_fused_N{N}_n{n}.json")
+    if os.path.exists(tfile):
+        with open(tfile) as f:
+            traffic = json.load(f).get("bytes_per_launch")
+
+    cpu = None
+    if not args.no_cpu:
+        threads = max(1, min(os.cpu_count() or 1, 16))
+        rate, el, Ks = cpu_baseline(ops, dm, c2q, K, args.cpu_sample, threads)
+        cpu = {"value": rate, "unit": "DOF-stage/s", "cores": threads, "kind": "port",
+               "sample": f"oracle FP32 C restatement (OpenMP), 1 LSRK45 step (5 stages) over the "
+                         f"first {Ks} of {K} elements of the same mesh/order, {el:.1f} s"}
+
+    out = {
+        "metric": "DOF-stage updates/s N=5 FP32, batched vs plain; adjoint-grad/forward time ratio",
+        "value": value,
+        "unit": "DOF-stage/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded Fourier c^2, seeded Ricker source position)",
+        "config": {"workload": f"acoustic heterogeneous single shot, {n}^3x6={K} tets, N={N}, "
+                               f"{args.steps} steps, fused stage kernel + HIP graph",
+                   "K": K, "N": N, "Np": ops.Np, "Nfields": 4, "dt": dt},
+        "plain": {"value": plain_value, "steps": args.plain_steps,
+                  "ms_per_step": ms_plain / args.plain_steps},
+        "fused_over_plain": value / plain_value,
+        "energy_after": E,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "kernel": "k_fused<5,false>", "bytes_per_element_stage": Bst,
+                     "avg_launch_ms": stage_ms},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

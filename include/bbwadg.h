@@ -1,0 +1,148 @@
+/*
+ * bbwadg.h -- C ABI of libbbwadg_hip.so, the MI355X (gfx950) BB-WADG wave solver.
+ *
+ * The library replaces the device half of the reference's solver: the OCCA
+ * device/memory/kernel globals and their launches in cpp/src/WaveOKL3d.cpp
+ * (and elas_cpp/src/WaveOKL3d.cpp).  The reference's host setup (mesh,
+ * operators, geometric factors, maps: Mesh3d.cpp, StartUp3d.cpp,
+ * InitWADG_subelem) keeps producing the arrays; these entry points take them
+ * as plain pointers in the reference's layouts:
+ *
+ *   Q / rhsQ / resQ   float [K][Nfields][Np]
+ *   vgeo              float [K][9]     (rx ry rz sx sy sz tx ty tz)
+ *   fgeo              float [K][4][4]  (Fscale nx ny nz per face)
+ *   vmapP             int32 [K][4*Nfp] flat index into Q of the exterior trace
+ *   Fmask             int32 [4][Nfp]
+ *   D_ids             int32 [4][Np][4]: D_ids[j][i] = (D1[i][j],D2[i][j],D3[i][j],D4[i][j])
+ *   Dvals             float [Np][4]
+ *   EEL_ids/vals      [EEL_nnz][Np] (column-major Np x EEL_nnz)
+ *   L0_ids/vals       [L0_nnz][Nfp] (column-major Nfp x L0_nnz)
+ *   Vq                float, column-major Nq x Np   (entry (q,j) at q + j*Nq)
+ *   Pq                float, column-major Np x Nq   (entry (n,q) at n + q*Np)
+ *   c2q, Jq           float [K][Nq]
+ *
+ * Conventions: every function returns 0 (BBW_OK) or a negative error class;
+ * bbw_last_error() describes the last failure on a context.  Host pointers are
+ * borrowed for the duration of the call.  All device work is enqueued on the
+ * context's stream (bbw_set_stream may substitute the caller's); copies back to
+ * the host synchronize that stream.  One context per host thread.
+ */
+#ifndef BBWADG_H
+#define BBWADG_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define BBW_OK 0
+#define BBW_ERR_ARG (-1)          /* invalid argument or size */
+#define BBW_ERR_HIP (-2)          /* HIP runtime error */
+#define BBW_ERR_STATE (-3)        /* required data not uploaded yet */
+#define BBW_ERR_UNSUPPORTED (-4)  /* order / field count not compiled */
+#define BBW_ERR_NOMEM (-5)        /* device allocation failed */
+
+#define BBW_MODE_PLAIN 0  /* volume, surface, update launches per stage (RK_step_WADG_subelem) */
+#define BBW_MODE_FUSED 1  /* one fused stage kernel per stage, ping-pong state */
+
+typedef struct bbw_ctx bbw_ctx;
+
+typedef struct {
+  int N;        /* polynomial order (1..6 compiled) */
+  int Nfields;  /* 4 acoustic (p,u,v,w), 9 elastic (v1,v2,v3,sxx,syy,szz,syz,sxz,sxy) */
+  int K;        /* number of elements */
+  int Nq;       /* WADG cubature points per element */
+  int KblkV, KblkS, KblkU, KblkF; /* elements per block: volume, surface, update, fused (0 = default) */
+  int device;   /* HIP device ordinal */
+} bbw_config;
+
+typedef struct {
+  const int32_t *Fmask;
+  const int32_t *D_ids;
+  const float *Dvals;
+  const int32_t *EEL_ids;
+  const float *EEL_vals;
+  int EEL_nnz;
+  const int32_t *L0_ids;
+  const float *L0_vals;
+  int L0_nnz;
+  const float *Vq;
+  const float *Pq;
+  const double *wq;   /* cubature weights (Nq), FP64: energy reduction */
+  const double *rk4a; /* LSRK45 coefficients (5, 5, 6 values), FP64 */
+  const double *rk4b;
+  const double *rk4c;
+} bbw_ops;
+
+/* ≈ WaveInitOCCA3d, device part (cpp/src/WaveOKL3d.cpp:1400-1680): device,
+ * stream and state buffers Q/resQ/rhsQ (zeroed). */
+int bbw_create(const bbw_config *cfg, bbw_ctx **out);
+void bbw_destroy(bbw_ctx *ctx);
+const char *bbw_last_error(const bbw_ctx *ctx);
+int bbw_set_stream(bbw_ctx *ctx, void *hip_stream); /* NULL restores the context's own stream */
+int bbw_sync(bbw_ctx *ctx);
+
+/* ≈ the operator uploads of WaveInitOCCA3d (:1426-1526) and InitWADG_subelem (:906-915). */
+int bbw_upload_operators(bbw_ctx *ctx, const bbw_ops *ops);
+/* ≈ the vgeo/fgeo/vmapP uploads of WaveInitOCCA3d (:1528-1616).  J: FP64 [K] (energy). */
+int bbw_upload_mesh(bbw_ctx *ctx, const float *vgeo, const float *fgeo, const int32_t *vmapP,
+                    const double *J);
+/* ≈ c_c2q / c_Jq_reduced uploads of InitWADG_subelem (:906-909).  Jq may be NULL (1/J = 1). */
+int bbw_upload_media(bbw_ctx *ctx, const float *c2q, const float *Jq);
+/* Elastic media at cubature (elas_cpp/src/WaveOKL3d.cpp:445-511): rhoinv (the kernel's rhoq
+ * multiplies), lambda, mu, each [K][Nq]; penalties tau_v, tau_s (:516-517). */
+int bbw_upload_elastic_media(bbw_ctx *ctx, const float *rhoinvq, const float *lambdaq,
+                             const float *muq, float tau_v, float tau_s);
+/* Point-like source: nsrc elements (ascending ids), BB coefficients coef[nsrc][Np] added to
+ * the right-hand side of each field f as amp * w[f] * ricker(t, f0) * coef (elas_cpp fsrc/ftime,
+ * :487-538, :1677-1680).  weights: Nfields floats.  nsrc = 0 removes the source. */
+int bbw_set_source(bbw_ctx *ctx, int nsrc, const int32_t *elems, const float *coef,
+                   const float *weights, double f0, double amp);
+/* Receivers: element id and BB basis values at the receiver point ([nrec][Np]); field 0 is
+ * recorded after every step of bbw_step into a trace [nsteps][nrec]. */
+int bbw_set_receivers(bbw_ctx *ctx, int nrec, const int32_t *elems, const float *basis,
+                      int max_steps);
+int bbw_get_traces(bbw_ctx *ctx, float *out, int nsteps);
+
+/* ≈ WaveSetData3d / WaveGetData3d device copies (:2527-2537).  Q in BB coefficients. */
+int bbw_set_state(bbw_ctx *ctx, const float *Q);
+int bbw_get_state(bbw_ctx *ctx, float *Q);
+int bbw_set_res(bbw_ctx *ctx, const float *res);
+int bbw_get_res(bbw_ctx *ctx, float *res);
+int bbw_set_rhs(bbw_ctx *ctx, const float *rhs);
+int bbw_get_rhs(bbw_ctx *ctx, float *rhs);
+
+/* Single launches on the context's current state, positional equivalents of:
+ *   bbw_volume       rk_volume_bern  (cpp/okl/WaveKernels.okl:344)      rhsQ  = vol(Q)
+ *   bbw_surface      rk_surface_bern (cpp/okl/WaveKernels.okl:479)      rhsQ += lift(flux(Q))
+ *   bbw_update_wadg  rk_update_WADG  (cpp/okl/WaveKernelsWADG.okl:1760) / rk_update_elas
+ *                    (elas_cpp/okl/ElasKernelsWADG.okl:286): projection + LSRK stage
+ *   bbw_update       rk_update       (cpp/okl/WaveKernels.okl:262)
+ * ftime is the source time-function value for this stage (0 without a source). */
+int bbw_volume(bbw_ctx *ctx);
+int bbw_surface(bbw_ctx *ctx);
+int bbw_update_wadg(bbw_ctx *ctx, float fa, float fb, float fdt, float ftime);
+int bbw_update(bbw_ctx *ctx, float fa, float fb, float fdt);
+/* One fused stage (volume + surface + WADG + LSRK) reading the current state and writing
+ * the other state buffer, which becomes current.  project_velocity: 1 projects the
+ * velocity fields through Pq*Vq as rk_update_WADG does, 0 uses Pq*Vq = I. */
+int bbw_stage_fused(bbw_ctx *ctx, float fa, float fb, float fdt, float ftime, int project_velocity);
+
+/* ≈ Wave_RK inner loop (cpp/src/WaveOKL3d.cpp:2029-2080): nsteps LSRK45 steps of size dt
+ * from time t0; stage coefficients cast to float as the reference does.  mode: BBW_MODE_*.
+ * flags bit 0: project velocity in the fused path; bit 1: capture steps in a HIP graph. */
+int bbw_step(bbw_ctx *ctx, int nsteps, double dt, double t0, int mode, int flags);
+
+/* FP64 energy 1/2 sum_K J sum_q w_q (p_q^2 / c2_q + |u_q|^2) (acoustic), reduced on device. */
+int bbw_energy(bbw_ctx *ctx, double *E);
+
+/* Device addresses of the context's buffers (for copies into caller-owned device memory):
+ * which = 0 current Q, 1 resQ, 2 rhsQ, 3 c2q. */
+int bbw_device_ptr(bbw_ctx *ctx, int which, void **ptr);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* BBWADG_H */

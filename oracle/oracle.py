@@ -1,0 +1,205 @@
+"""Oracle for the BB-WADG hot path -- TEST INFRASTRUCTURE ONLY.
+
+Only ``tests/``, ``__graft_entry__.smoke()`` and the CPU-baseline leg of
+``bench.py`` use this module, and only as the checker (or as the timed CPU
+baseline); the product never imports it.
+
+It wraps ``liborc.so`` (``bbw_oracle.c``: a loop-for-loop C restatement of the
+reference's OKL kernels in FP64 and FP32) and adds, in numpy FP64:
+
+* ``rk_run``     -- the LSRK45 driver of ``Wave_RK``/``RK_step_WADG_subelem``
+  (``cpp/src/WaveOKL3d.cpp:2014-2109``): per stage volume -> surface ->
+  WADG update, stage coefficients cast to the working precision as the
+  reference does (``:2052-2053``);
+* ``energy_quadrature`` -- 1/2 sum_K J sum_q w_q (p_q^2 / c^2_q + |u_q|^2);
+* ``energy_wadg``       -- the exact WADG energy 1/2 sum_K [p^T M (M_{c^2})^-1 M p + u^T M u]
+  (weighted mass of ``matlab/WADG/check_conservation_loss.m:51-54``);
+* ``l2_error``          -- ``compute_error`` (``cpp/src/WaveOKL3d.cpp:1683-1726``) with the
+  relative error computed as sqrt(err2)/sqrt(norm2) (the reference takes an
+  extra square root, ``:1722``).
+
+Parity pins for each function are in ``tests/test_oracle.py`` and
+``tests/test_operators.py``; see the module docstrings there.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "liborc.so")
+        if not os.path.exists(path):
+            raise RuntimeError("oracle/liborc.so is not built (run make -C oracle)")
+        _LIB = ctypes.CDLL(path)
+    return _LIB
+
+
+def _p(a):
+    return a.ctypes.data_as(ctypes.c_void_p) if a is not None else None
+
+
+def _sfx(dtype):
+    return "f64" if np.dtype(dtype) == np.float64 else "f32"
+
+
+class OracleOps:
+    """Operator arrays cast to one working precision, in the reference's layouts."""
+
+    def __init__(self, ops, dtype):
+        dt = np.dtype(dtype)
+        self.dtype = dt
+        self.N, self.Np, self.Nfp, self.Nq = ops.N, ops.Np, ops.Nfp, ops.Nq
+        self.Dids = np.ascontiguousarray(np.transpose(ops.D_ids, (2, 1, 0)), dtype=np.int32)
+        self.Dvals = np.ascontiguousarray(ops.D_vals, dtype=dt)
+        self.Fmask = np.ascontiguousarray(ops.Fmask.reshape(-1), dtype=np.int32)
+        self.EEL_ids = np.ascontiguousarray(ops.EEL_ids.T, dtype=np.int32)
+        self.EEL_vals = np.ascontiguousarray(ops.EEL_vals.T, dtype=dt)
+        self.EEL_nnz = ops.EEL_nnz
+        self.L0_ids = np.ascontiguousarray(ops.L0_ids.T, dtype=np.int32)
+        self.L0_vals = np.ascontiguousarray(ops.L0_vals.T, dtype=dt)
+        self.L0_nnz = ops.L0_nnz
+        self.Vq = np.ascontiguousarray(ops.Vq.T, dtype=dt)
+        self.Pq = np.ascontiguousarray(ops.Pq.T, dtype=dt)
+        self.rk4a, self.rk4b, self.rk4c = ops.rk4a, ops.rk4b, ops.rk4c
+
+
+def volume(oo, K, vgeo, Q, rhs, nfields=4):
+    dt = oo.dtype
+    vgeo = np.ascontiguousarray(vgeo, dtype=dt)
+    name = "orc_volume_" if nfields == 4 else "orc_volume_elas_"
+    getattr(lib(), name + _sfx(dt))(ctypes.c_int(K), ctypes.c_int(oo.Np), _p(vgeo), _p(oo.Dids),
+                                    _p(oo.Dvals), _p(Q), _p(rhs))
+
+
+def surface(oo, K, fgeo, vmapP, Q, rhs, nfields=4, tau_v=1.0, tau_s=1.0):
+    dt = oo.dtype
+    fgeo = np.ascontiguousarray(fgeo, dtype=dt)
+    vmapP = np.ascontiguousarray(vmapP, dtype=np.int32)
+    R = ctypes.c_double if dt == np.float64 else ctypes.c_float
+    if nfields == 4:
+        getattr(lib(), "orc_surface_" + _sfx(dt))(
+            ctypes.c_int(K), ctypes.c_int(oo.Np), ctypes.c_int(oo.Nfp), _p(fgeo), _p(oo.Fmask),
+            _p(vmapP), _p(oo.EEL_ids), _p(oo.EEL_vals), ctypes.c_int(oo.EEL_nnz), _p(oo.L0_ids),
+            _p(oo.L0_vals), ctypes.c_int(oo.L0_nnz), _p(Q), _p(rhs))
+    else:
+        getattr(lib(), "orc_surface_elas_" + _sfx(dt))(
+            ctypes.c_int(K), ctypes.c_int(oo.Np), ctypes.c_int(oo.Nfp), _p(fgeo), _p(oo.Fmask),
+            _p(vmapP), _p(oo.EEL_ids), _p(oo.EEL_vals), ctypes.c_int(oo.EEL_nnz), _p(oo.L0_ids),
+            _p(oo.L0_vals), ctypes.c_int(oo.L0_nnz), R(tau_v), R(tau_s), _p(Q), _p(rhs))
+
+
+def update_wadg(oo, K, c2q, fa, fb, fdt, rhs, res, Q, Jq=None, nproj=4, ftime=0.0, src=None):
+    dt = oo.dtype
+    R = ctypes.c_double if dt == np.float64 else ctypes.c_float
+    c2q = np.ascontiguousarray(c2q, dtype=dt)
+    Jq = None if Jq is None else np.ascontiguousarray(Jq, dtype=dt)
+    src = None if src is None else np.ascontiguousarray(src, dtype=dt)
+    getattr(lib(), "orc_update_wadg_" + _sfx(dt))(
+        ctypes.c_int(K), ctypes.c_int(oo.Np), ctypes.c_int(oo.Nq), ctypes.c_int(nproj), _p(oo.Vq),
+        _p(oo.Pq), _p(Jq), _p(c2q), R(fa), R(fb), R(fdt), R(ftime), _p(src), _p(rhs), _p(res), _p(Q))
+
+
+def update_elas(oo, K, rhoinvq, lamq, muq, fa, fb, fdt, rhs, res, Q, ftime=0.0, src=None, srcw=None):
+    dt = oo.dtype
+    R = ctypes.c_double if dt == np.float64 else ctypes.c_float
+    rhoinvq, lamq, muq = (np.ascontiguousarray(a, dtype=dt) for a in (rhoinvq, lamq, muq))
+    src = None if src is None else np.ascontiguousarray(src, dtype=dt)
+    srcw = np.ascontiguousarray(np.zeros(9) if srcw is None else srcw, dtype=dt)
+    getattr(lib(), "orc_update_elas_" + _sfx(dt))(
+        ctypes.c_int(K), ctypes.c_int(oo.Np), ctypes.c_int(oo.Nq), _p(oo.Vq), _p(oo.Pq), _p(rhoinvq),
+        _p(lamq), _p(muq), R(ftime), _p(src), _p(srcw), R(fa), R(fb), R(fdt), _p(rhs), _p(res), _p(Q))
+
+
+def update(oo, fa, fb, fdt, rhs, res, Q):
+    dt = oo.dtype
+    R = ctypes.c_double if dt == np.float64 else ctypes.c_float
+    getattr(lib(), "orc_update_" + _sfx(dt))(ctypes.c_long(Q.size), R(fa), R(fb), R(fdt), _p(rhs),
+                                             _p(res), _p(Q))
+
+
+def ricker(t, f0):
+    """Ricker wavelet (1 - 2a^2) exp(-a^2), a = pi f0 (t - 1/f0) (``elas_cpp/src/WaveOKL3d.cpp:1677-1680``)."""
+    a = np.pi * f0 * (t - 1.0 / f0)
+    return (1.0 - 2.0 * a * a) * np.exp(-a * a)
+
+
+def rk_run(oo, K, vgeo, fgeo, vmapP, c2q, Q, dt, nsteps, t0=0.0, nproj=4, Jq=None,
+           src=None, f0=None, amp=1.0, res=None, record=None):
+    """Advance Q (modified in place) by ``nsteps`` LSRK45 steps of the acoustic WADG scheme.
+
+    Each stage: rhs = volume(Q); rhs += surface(Q); WADG update (``RK_step_WADG_subelem``).
+    ``src`` (K, Np) BB coefficients are added to the p right-hand side with
+    amplitude amp*ricker(t + c_i dt, f0).  ``record(step, Q)`` is called after each step.
+    """
+    wd = oo.dtype
+    rhs = np.zeros_like(Q)
+    if res is None:
+        res = np.zeros_like(Q)
+    for step in range(nsteps):
+        t = t0 + step * dt
+        for i in range(5):
+            fa, fb, fdt = wd.type(oo.rk4a[i]), wd.type(oo.rk4b[i]), wd.type(dt)
+            volume(oo, K, vgeo, Q, rhs)
+            surface(oo, K, fgeo, vmapP, Q, rhs)
+            ft = 0.0 if src is None else amp * ricker(t + oo.rk4c[i] * dt, f0)
+            update_wadg(oo, K, c2q, fa, fb, fdt, rhs, res, Q, Jq=Jq, nproj=nproj, ftime=ft, src=src)
+        if record is not None:
+            record(step, Q)
+    return Q, res
+
+
+def rk_run_elas(oo, K, vgeo, fgeo, vmapP, rhoinvq, lamq, muq, Q, dt, nsteps, t0=0.0,
+                src=None, srcw=None, f0=None, amp=1.0, tau_v=1.0, tau_s=1.0, res=None):
+    wd = oo.dtype
+    rhs = np.zeros_like(Q)
+    if res is None:
+        res = np.zeros_like(Q)
+    for step in range(nsteps):
+        t = t0 + step * dt
+        for i in range(5):
+            fa, fb, fdt = wd.type(oo.rk4a[i]), wd.type(oo.rk4b[i]), wd.type(dt)
+            volume(oo, K, vgeo, Q, rhs, nfields=9)
+            surface(oo, K, fgeo, vmapP, Q, rhs, nfields=9, tau_v=tau_v, tau_s=tau_s)
+            ft = 0.0 if src is None else amp * ricker(t + oo.rk4c[i] * dt, f0)
+            update_elas(oo, K, rhoinvq, lamq, muq, fa, fb, fdt, rhs, res, Q, ftime=ft, src=src, srcw=srcw)
+    return Q, res
+
+
+def energy_quadrature(ops, J, c2q, Q):
+    """1/2 sum_K J_K sum_q w_q (p_q^2/c^2_q + u_q^2 + v_q^2 + w_q^2), FP64."""
+    K, Np = len(J), ops.Np
+    Qr = np.asarray(Q, dtype=np.float64).reshape(K, -1, Np)
+    Uq = np.einsum("qj,kfj->kfq", ops.Vq, Qr)
+    dens = Uq[:, 0] ** 2 / np.asarray(c2q, dtype=np.float64) + (Uq[:, 1:] ** 2).sum(axis=1)
+    return 0.5 * float(np.sum(J[:, None] * ops.wq[None, :] * dens))
+
+
+def energy_wadg(ops, J, c2q, Q):
+    """Exact WADG energy 1/2 sum_K [p^T M (M_{c2})^-1 M p + sum_i u_i^T M u_i], M = J * MB."""
+    K, Np = len(J), ops.Np
+    Qr = np.asarray(Q, dtype=np.float64).reshape(K, -1, Np)
+    MB = ops.Vq.T @ (ops.wq[:, None] * ops.Vq)
+    Mc = np.einsum("qi,kq,qj->kij", ops.Vq, ops.wq[None, :] * np.asarray(c2q, dtype=np.float64), ops.Vq)
+    Mp = Qr[:, 0] @ MB  # (K, Np) = (MB p) per element (MB symmetric)
+    y = np.linalg.solve(Mc, Mp[:, :, None])[:, :, 0]
+    Ep = np.sum(Mp * y, axis=1) * J
+    Eu = np.einsum("kfi,ij,kfj->k", Qr[:, 1:], MB, Qr[:, 1:]) * J
+    return 0.5 * float(np.sum(Ep + Eu))
+
+
+def l2_error(ops, xq, yq, zq, J, Q, exact, time, field=0, nfields=4):
+    """(L2err, relL2err) of one field against exact(x, y, z, t) at the WADG cubature."""
+    K, Np = len(J), ops.Np
+    Qr = np.asarray(Q, dtype=np.float64).reshape(K, nfields, Np)
+    uq = Qr[:, field] @ ops.Vq.T  # (K, Nq)
+    ue = exact(xq.T, yq.T, zq.T, time)
+    w = ops.wq[None, :] * J[:, None]
+    err2 = float(np.sum((uq - ue) ** 2 * w))
+    nrm2 = float(np.sum(ue ** 2 * w))
+    return np.sqrt(err2), np.sqrt(err2) / np.sqrt(nrm2)

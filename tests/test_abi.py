@@ -1,0 +1,39 @@
+"""The C-ABI library loads and exports every symbol include/bbwadg.h declares (no GPU calls)."""
+import ctypes
+import os
+import re
+
+from bbwadg_amd import _lib
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    names = set()
+    for fn in os.listdir(os.path.join(ROOT, "include")):
+        if fn.endswith(".h"):
+            text = open(os.path.join(ROOT, "include", fn)).read()
+            names |= set(re.findall(r"^\s*(?:int|void|const char \*)\s*\*?(bbw_\w+)\s*\(", text, re.M))
+    return names
+
+
+def test_header_declares_exports():
+    assert declared_symbols() == set(_lib.EXPORTS)
+
+
+def test_library_exports_all_symbols():
+    lib = ctypes.CDLL(_lib.LIB_PATH)
+    for name in declared_symbols():
+        assert hasattr(lib, name), name
+
+
+def test_bad_config_rejected_without_gpu_work():
+    lib = _lib.load()
+    h = ctypes.c_void_p()
+    cfg = _lib.Config(N=9, Nfields=4, K=10, Nq=95)
+    assert lib.bbw_create(ctypes.byref(cfg), ctypes.byref(h)) == _lib.BBW_ERR_UNSUPPORTED
+    cfg = _lib.Config(N=5, Nfields=5, K=10, Nq=95)
+    assert lib.bbw_create(ctypes.byref(cfg), ctypes.byref(h)) == _lib.BBW_ERR_UNSUPPORTED
+    cfg = _lib.Config(N=5, Nfields=4, K=10, Nq=94)
+    assert lib.bbw_create(ctypes.byref(cfg), ctypes.byref(h)) == _lib.BBW_ERR_ARG
+    assert lib.bbw_create(None, ctypes.byref(h)) == _lib.BBW_ERR_ARG

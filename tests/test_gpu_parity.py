@@ -1,0 +1,194 @@
+"""GPU parity: each HIP kernel and the step loop against the oracle (oracle/bbw_oracle.c)
+on the same seeded inputs.  Tolerances (FP32 arithmetic, summation order differs from the
+oracle's): single kernels 2e-5 relative to the output's max-norm against the FP32 oracle
+and 1e-5 against the FP64 oracle evaluated on the same FP32 inputs; whole runs 1e-5
+relative L2 against the FP64 oracle."""
+import numpy as np
+import pytest
+
+from bbwadg_amd.solver import WaveSolver, MODE_FUSED, MODE_PLAIN
+import oracle as orc
+from cases import Problem, rel, rel_l2
+
+pytestmark = pytest.mark.gpu
+
+
+def make(N, n, seed=0, c2="fourier", **kw):
+    P = Problem(N, n, seed=seed, c2=c2)
+    s = WaveSolver(P.mesh, N, ops=P.ops, dmesh=P.dm, **kw)
+    s.InitWADG_subelem(P.c2q)
+    return P, s
+
+
+def oracle_ops(P, dtype):
+    return orc.OracleOps(P.ops, dtype)
+
+
+@pytest.mark.parametrize("N", [3, 4, 5])
+def test_volume(N):
+    print(f"case volume N={N}", flush=True)
+    P, s = make(N, 3)
+    Q = P.random_state()
+    s.WaveSetData3d(Q)
+    s.volume()
+    got = s.get_rhs()
+    for dt, tol in ((np.float32, 2e-5), (np.float64, 1e-5)):
+        ref = np.zeros(Q.shape, dtype=dt)
+        orc.volume(oracle_ops(P, dt), P.K, P.dm.vgeo.astype(dt), Q.astype(dt), ref)
+        assert rel(got, ref) < tol
+
+
+@pytest.mark.parametrize("N", [3, 4, 5])
+def test_surface(N):
+    print(f"case surface N={N}", flush=True)
+    P, s = make(N, 3)
+    Q = P.random_state()
+    R0 = P.random_state()
+    s.WaveSetData3d(Q)
+    s.set_rhs(R0)
+    s.surface()
+    got = s.get_rhs()
+    for dt, tol in ((np.float32, 2e-5), (np.float64, 1e-5)):
+        ref = R0.astype(dt).copy()
+        orc.surface(oracle_ops(P, dt), P.K, P.dm.fgeo.astype(dt), P.dm.vmapP, Q.astype(dt), ref)
+        assert rel(got, ref) < tol
+
+
+@pytest.mark.parametrize("N", [3, 4, 5])
+def test_update_wadg(N):
+    print(f"case update_wadg N={N}", flush=True)
+    P, s = make(N, 3)
+    Q, R, res = P.random_state(), P.random_state(), P.random_state()
+    s.WaveSetData3d(Q)
+    s.set_rhs(R)
+    s.set_res(res)
+    fa, fb, fdt = np.float32(P.ops.rk4a[2]), np.float32(P.ops.rk4b[2]), np.float32(1e-2)
+    s.update_wadg(fa, fb, fdt)
+    gQ, gres = s.WaveGetData3d(), s.get_res()
+    for dt, tol in ((np.float32, 2e-5), (np.float64, 2e-5)):
+        rQ, rres = Q.astype(dt).copy(), res.astype(dt).copy()
+        orc.update_wadg(oracle_ops(P, dt), P.K, P.c2q, fa, fb, fdt, R.astype(dt), rres, rQ)
+        assert rel(gres, rres) < tol
+        assert rel(gQ, rQ) < tol
+
+
+def test_update_plain():
+    P, s = make(3, 2)
+    Q, R, res = P.random_state(), P.random_state(), P.random_state()
+    s.WaveSetData3d(Q); s.set_rhs(R); s.set_res(res)
+    s.update(np.float32(-0.5), np.float32(0.25), np.float32(0.1))
+    rres = np.float32(-0.5) * res + np.float32(0.1) * R
+    rQ = Q + np.float32(0.25) * rres
+    assert rel(s.get_res(), rres) < 1e-6 and rel(s.WaveGetData3d(), rQ) < 1e-6
+
+
+@pytest.mark.parametrize("N,proj", [(3, False), (4, True), (5, False), (5, True)])
+def test_fused_stage(N, proj):
+    print(f"case fused N={N} proj={proj}", flush=True)
+    P, s = make(N, 3)
+    Q, res = P.random_state(), P.random_state()
+    s.WaveSetData3d(Q); s.set_res(res)
+    fa, fb, fdt = np.float32(P.ops.rk4a[1]), np.float32(P.ops.rk4b[1]), np.float32(1e-2)
+    s.stage_fused(fa, fb, fdt, project_velocity=proj)
+    gQ, gres = s.WaveGetData3d(), s.get_res()
+    dt = np.float64
+    oo = oracle_ops(P, dt)
+    rhs = np.zeros(Q.shape)
+    rQ, rres = Q.astype(dt).copy(), res.astype(dt).copy()
+    orc.volume(oo, P.K, P.dm.vgeo, Q.astype(dt), rhs)
+    orc.surface(oo, P.K, P.dm.fgeo, P.dm.vmapP, Q.astype(dt), rhs)
+    orc.update_wadg(oo, P.K, P.c2q, fa, fb, fdt, rhs, rres, rQ, nproj=4 if proj else 1)
+    assert rel(gres, rres) < 2e-5
+    assert rel(gQ, rQ) < 2e-5
+
+
+@pytest.mark.parametrize("mode", [MODE_PLAIN, MODE_FUSED])
+def test_cavity_run_matches_fp64_oracle(mode):
+    """CPU-runnable configuration shape (N=3, c2=1, cavity mode, dt = 0.5h/N^2), 40 steps."""
+    print(f"case cavity mode={mode}", flush=True)
+    P, s = make(3, 4, c2="one")
+    Q0 = P.cavity_state()
+    s.WaveSetData3d(Q0)
+    dt, nsteps = P.dt_cfl(), 40
+    s.step(nsteps, dt, mode=mode, graph=True)
+    got = s.WaveGetData3d()
+    oo = oracle_ops(P, np.float64)
+    ref = Q0.astype(np.float64).copy()
+    orc.rk_run(oo, P.K, P.dm.vgeo, P.dm.fgeo, P.dm.vmapP, P.c2q, ref, dt, nsteps)
+    assert rel_l2(got, ref) < 1e-5
+    # and the plain path against the FP32 oracle, stage by stage the same arithmetic order class
+    if mode == MODE_PLAIN:
+        oo32 = oracle_ops(P, np.float32)
+        r32 = Q0.copy()
+        orc.rk_run(oo32, P.K, P.dm.vgeo, P.dm.fgeo, P.dm.vmapP, P.c2q, r32, dt, nsteps)
+        assert rel_l2(got, r32) < 1e-5
+
+
+def test_heterogeneous_run_fused_vs_plain_vs_oracle():
+    print("case heterogeneous N=5", flush=True)
+    P, s = make(5, 2)
+    Q0 = P.random_state()
+    dt, nsteps = P.dt_cfl() * 0.5, 20
+    s.WaveSetData3d(Q0)
+    s.step(nsteps, dt, mode=MODE_PLAIN, graph=False)
+    plain = s.WaveGetData3d()
+    s.WaveSetData3d(Q0); s.set_res(np.zeros_like(Q0))
+    s.step(nsteps, dt, mode=MODE_FUSED, project_velocity=True, graph=True)
+    fused_p = s.WaveGetData3d()
+    s.WaveSetData3d(Q0); s.set_res(np.zeros_like(Q0))
+    s.step(nsteps, dt, mode=MODE_FUSED, graph=False)
+    fused = s.WaveGetData3d()
+    oo = oracle_ops(P, np.float64)
+    ref = Q0.astype(np.float64).copy()
+    orc.rk_run(oo, P.K, P.dm.vgeo, P.dm.fgeo, P.dm.vmapP, P.c2q, ref, dt, nsteps)
+    for got in (plain, fused_p, fused):
+        assert rel_l2(got, ref) < 1e-5
+
+
+def test_graph_replay_is_bitwise_eager():
+    P, s = make(4, 2)
+    Q0 = P.random_state()
+    dt = P.dt_cfl()
+    outs = []
+    for graph in (False, True, True):
+        s.WaveSetData3d(Q0); s.set_res(np.zeros_like(Q0))
+        s.step(30, dt, mode=MODE_FUSED, graph=graph)
+        outs.append(s.WaveGetData3d())
+    assert np.array_equal(outs[0], outs[1]) and np.array_equal(outs[1], outs[2])
+
+
+def test_energy_matches_oracle():
+    P, s = make(4, 3)
+    Q = P.random_state()
+    s.WaveSetData3d(Q)
+    E = s.energy()
+    Eref = orc.energy_quadrature(P.ops, P.dm.J, P.c2q.astype(np.float32), Q)
+    assert abs(E - Eref) < 1e-10 * abs(Eref)
+
+
+def test_source_and_receivers_match_oracle():
+    from bbwadg_amd import media
+    print("case source+receivers", flush=True)
+    P, s = make(3, 3)
+    elems, coef = media.gaussian_source(P.mesh, P.ops, (0.05, -0.1, 0.02), a=20.0)
+    f0, amp = 10.0, 3.0
+    s.set_source(elems, coef, f0=f0, amp=amp)
+    pts = media.surface_receivers(1, 7)
+    relems, B = media.receivers(P.mesh, P.ops, pts)
+    nsteps = 30
+    s.set_receivers(relems, B, nsteps)
+    Q0 = np.zeros((P.K, 4, P.ops.Np), dtype=np.float32)
+    s.WaveSetData3d(Q0)
+    dt = P.dt_cfl()
+    s.step(nsteps, dt, mode=MODE_FUSED, project_velocity=True, graph=True)
+    got = s.WaveGetData3d()
+    tr = s.get_traces(nsteps)
+    src = np.zeros((P.K, P.ops.Np))
+    src[elems] = coef
+    ref = Q0.astype(np.float64).copy()
+    rec = []
+    orc.rk_run(orc.OracleOps(P.ops, np.float64), P.K, P.dm.vgeo, P.dm.fgeo, P.dm.vmapP, P.c2q,
+               ref, dt, nsteps, src=src, f0=f0, amp=amp,
+               record=lambda st, Qs: rec.append(np.einsum("rj,rj->r", B, Qs[relems, 0])))
+    assert rel_l2(got, ref) < 1e-5
+    assert rel_l2(tr, np.array(rec)) < 1e-5
