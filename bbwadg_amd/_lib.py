@@ -52,6 +52,7 @@ class Ops(ctypes.Structure):
                 ("EEL_ids", ctypes.c_void_p), ("EEL_vals", ctypes.c_void_p), ("EEL_nnz", ctypes.c_int),
                 ("L0_ids", ctypes.c_void_p), ("L0_vals", ctypes.c_void_p), ("L0_nnz", ctypes.c_int),
                 ("Vq", ctypes.c_void_p), ("Pq", ctypes.c_void_p), ("wq", ctypes.c_void_p),
+                ("Vq64", ctypes.c_void_p),
                 ("rk4a", ctypes.c_void_p), ("rk4b", ctypes.c_void_p), ("rk4c", ctypes.c_void_p)]
 
 

@@ -207,7 +207,7 @@ int bbw_upload_operators(bbw_ctx *c, const bbw_ops *o) {
   if ((rc = upload(c, &c->Vq, o->Vq, (size_t)Nq * Np))) return rc;
   if ((rc = upload(c, &c->Pq, o->Pq, (size_t)Nq * Np))) return rc;
   std::vector<double> v64((size_t)Nq * Np);
-  for (size_t i = 0; i < v64.size(); ++i) v64[i] = (double)o->Vq[i];
+  for (size_t i = 0; i < v64.size(); ++i) v64[i] = o->Vq64 ? o->Vq64[i] : (double)o->Vq[i];
   if ((rc = upload(c, &c->Vq64, v64.data(), v64.size()))) return rc;
   if ((rc = upload(c, &c->wq64, o->wq, (size_t)Nq))) return rc;
   std::memcpy(c->rk4a, o->rk4a, 5 * sizeof(double));

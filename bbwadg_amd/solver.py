@@ -62,7 +62,7 @@ class WaveSolver:
     # -- setup ---------------------------------------------------------------------
     def _upload_ops(self):
         o = operators.packed_device_ops(self.ops)
-        self._ops_keep = dict(o, wq=f64(self.ops.wq), rk4a=f64(self.ops.rk4a),
+        self._ops_keep = dict(o, wq=f64(self.ops.wq), Vq64=f64(self.ops.Vq.T), rk4a=f64(self.ops.rk4a),
                               rk4b=f64(self.ops.rk4b), rk4c=f64(self.ops.rk4c))
         k = self._ops_keep
         s = _lib.Ops(Fmask=ptr(k["Fmask"]).value, D_ids=ptr(k["D_ids"]).value,
@@ -70,7 +70,7 @@ class WaveSolver:
                      EEL_vals=ptr(k["EEL_vals"]).value, EEL_nnz=self.ops.EEL_nnz,
                      L0_ids=ptr(k["L0_ids"]).value, L0_vals=ptr(k["L0_vals"]).value,
                      L0_nnz=self.ops.L0_nnz, Vq=ptr(k["Vq"]).value, Pq=ptr(k["Pq"]).value,
-                     wq=ptr(k["wq"]).value, rk4a=ptr(k["rk4a"]).value, rk4b=ptr(k["rk4b"]).value,
+                     wq=ptr(k["wq"]).value, Vq64=ptr(k["Vq64"]).value, rk4a=ptr(k["rk4a"]).value, rk4b=ptr(k["rk4b"]).value,
                      rk4c=ptr(k["rk4c"]).value)
         check(self.ctx, self.lib.bbw_upload_operators(self.ctx, ctypes.byref(s)))
 

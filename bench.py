@@ -80,7 +80,7 @@ def main():
     ap.add_argument("--mesh", type=int, default=48)
     ap.add_argument("--N", type=int, default=5)
     ap.add_argument("--plain-steps", type=int, default=200)
-    ap.add_argument("--cpu-sample", type=int, default=32768)
+    ap.add_argument("--cpu-sample", type=int, default=1 << 30)
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:

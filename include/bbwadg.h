@@ -70,6 +70,7 @@ typedef struct {
   const float *Vq;
   const float *Pq;
   const double *wq;   /* cubature weights (Nq), FP64: energy reduction */
+  const double *Vq64; /* Vq in FP64 (same layout as Vq) for the energy reduction; NULL: Vq */
   const double *rk4a; /* LSRK45 coefficients (5, 5, 6 values), FP64 */
   const double *rk4b;
   const double *rk4c;
