@@ -27,8 +27,10 @@ EXPORTS = [
     "bbw_set_source", "bbw_set_receivers", "bbw_get_traces", "bbw_set_state", "bbw_get_state",
     "bbw_set_res", "bbw_get_res", "bbw_set_rhs", "bbw_get_rhs", "bbw_volume", "bbw_surface",
     "bbw_update_wadg", "bbw_update", "bbw_stage_fused", "bbw_step", "bbw_energy",
-    "bbw_device_ptr",
+    "bbw_device_ptr", "bbw_set_option",
 ]
+OPT_FUSED_IMPL, OPT_KBLK_V, OPT_KBLK_S, OPT_KBLK_U, OPT_KBLK_F = 0, 1, 2, 3, 4
+FUSED_VECTOR, FUSED_MFMA = 0, 1
 
 
 class BBWLibraryError(RuntimeError):
@@ -93,6 +95,7 @@ def load():
         "bbw_step": ([P, I, D, D, I, I], I),
         "bbw_energy": ([P, ctypes.POINTER(D)], I),
         "bbw_device_ptr": ([P, I, ctypes.POINTER(P)], I),
+        "bbw_set_option": ([P, I, I], I),
     }
     for name, (args, res) in sig.items():
         fn = getattr(lib, name)

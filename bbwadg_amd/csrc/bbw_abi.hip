@@ -80,7 +80,9 @@ int do_stage(bbw_ctx *c, int i, double dt, const float *ftd, float ft, int mode,
   hipError_t e;
   if (c->Nf == 4) {
     if (mode == BBW_MODE_FUSED) {
-      e = launch_fused(c, c->Q[c->cur], c->Q[1 - c->cur], fa, fb, fdt, ftd, ft, projU);
+      e = c->fused_impl == BBW_FUSED_MFMA
+              ? launch_fused_mfma(c, c->Q[c->cur], c->Q[1 - c->cur], fa, fb, fdt, ftd, ft, projU)
+              : launch_fused(c, c->Q[c->cur], c->Q[1 - c->cur], fa, fb, fdt, ftd, ft, projU);
       if (e != hipSuccess) return hipfail(c, e, "fused stage launch");
       c->cur = 1 - c->cur;
       return BBW_OK;
@@ -382,7 +384,9 @@ int bbw_stage_fused(bbw_ctx *c, float fa, float fb, float fdt, float ftime, int 
   int rc = check_ready(c);
   if (rc) return rc;
   if (c->Nf != 4) return fail(c, BBW_ERR_UNSUPPORTED, "fused stage is acoustic only");
-  hipError_t e = launch_fused(c, c->Q[c->cur], c->Q[1 - c->cur], fa, fb, fdt, nullptr, ftime, projU);
+  hipError_t e = c->fused_impl == BBW_FUSED_MFMA
+                     ? launch_fused_mfma(c, c->Q[c->cur], c->Q[1 - c->cur], fa, fb, fdt, nullptr, ftime, projU)
+                     : launch_fused(c, c->Q[c->cur], c->Q[1 - c->cur], fa, fb, fdt, nullptr, ftime, projU);
   if (e != hipSuccess) return hipfail(c, e, "fused stage launch");
   c->cur = 1 - c->cur;
   return BBW_OK;
@@ -500,6 +504,23 @@ int bbw_energy(bbw_ctx *c, double *E) {
   if (e != hipSuccess) return hipfail(c, e, "reduce launch");
   HIPCHK(c, hipMemcpyAsync(E, c->partial + nb, sizeof(double), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
+  return BBW_OK;
+}
+
+int bbw_set_option(bbw_ctx *c, int option, int value) {
+  if (!c) return BBW_ERR_ARG;
+  switch (option) {
+    case BBW_OPT_FUSED_IMPL:
+      if (value != BBW_FUSED_VECTOR && value != BBW_FUSED_MFMA) return fail(c, BBW_ERR_ARG, "bad fused impl");
+      c->fused_impl = value;
+      break;
+    case BBW_OPT_KBLK_V: c->KblkV = value > 0 ? value : 0; break;
+    case BBW_OPT_KBLK_S: c->KblkS = value > 0 ? value : 0; break;
+    case BBW_OPT_KBLK_U: c->KblkU = value > 0 ? value : 0; break;
+    case BBW_OPT_KBLK_F: c->KblkF = value > 0 ? value : 0; break;
+    default: return fail(c, BBW_ERR_ARG, "unknown option");
+  }
+  reset_graph(c);
   return BBW_OK;
 }
 

@@ -49,6 +49,7 @@ struct Graph {
 struct bbw_ctx {
   int N = 0, Np = 0, Nfp = 0, Nq = 0, Nf = 0, K = 0, dev = 0;
   int KblkV = 0, KblkS = 0, KblkU = 0, KblkF = 0;
+  int fused_impl = 1;  // BBW_FUSED_VECTOR or BBW_FUSED_MFMA
   hipStream_t own_stream = nullptr, stream = nullptr;
   std::string err;
 
@@ -110,6 +111,8 @@ hipError_t launch_update_wadg(bbw_ctx *c, float fa, float fb, float fdt, const f
 hipError_t launch_update(bbw_ctx *c, float fa, float fb, float fdt);
 hipError_t launch_fused(bbw_ctx *c, const float *Qin, float *Qout, float fa, float fb, float fdt,
                         const float *ftime_dev, float ftime, int project_velocity);
+hipError_t launch_fused_mfma(bbw_ctx *c, const float *Qin, float *Qout, float fa, float fb,
+                             float fdt, const float *ftime_dev, float ftime, int project_velocity);
 hipError_t launch_energy(bbw_ctx *c, double *E_dev_partial, int *nblocks);
 hipError_t launch_reduce(bbw_ctx *c, const double *partial, int n, double *out);
 hipError_t launch_record(bbw_ctx *c, const float *Q, float *row);

@@ -206,6 +206,9 @@ class WaveSolver:
         n2 = float(np.sum(ue ** 2 * w))
         return np.sqrt(e2), np.sqrt(e2) / np.sqrt(n2)
 
+    def set_option(self, option, value):
+        check(self.ctx, self.lib.bbw_set_option(self.ctx, int(option), int(value)))
+
     def device_ptr(self, which):
         p = ctypes.c_void_p()
         check(self.ctx, self.lib.bbw_device_ptr(self.ctx, which, ctypes.byref(p)))

@@ -138,6 +138,18 @@ int bbw_step(bbw_ctx *ctx, int nsteps, double dt, double t0, int mode, int flags
 /* FP64 energy 1/2 sum_K J sum_q w_q (p_q^2 / c2_q + |u_q|^2) (acoustic), reduced on device. */
 int bbw_energy(bbw_ctx *ctx, double *E);
 
+/* Tuning options (no reference counterpart beyond the Kblk* compile-time defines of
+ * WaveInitOCCA3d, cpp/src/WaveOKL3d.cpp:1631-1635): fused kernel implementation and
+ * elements per block; used by the autotuner. */
+#define BBW_OPT_FUSED_IMPL 0 /* BBW_FUSED_VECTOR or BBW_FUSED_MFMA (default) */
+#define BBW_OPT_KBLK_V 1
+#define BBW_OPT_KBLK_S 2
+#define BBW_OPT_KBLK_U 3
+#define BBW_OPT_KBLK_F 4 /* elements per block of the vector fused kernel */
+#define BBW_FUSED_VECTOR 0
+#define BBW_FUSED_MFMA 1
+int bbw_set_option(bbw_ctx *ctx, int option, int value);
+
 /* Device addresses of the context's buffers (for copies into caller-owned device memory):
  * which = 0 current Q, 1 resQ, 2 rhsQ, 3 c2q. */
 int bbw_device_ptr(bbw_ctx *ctx, int which, void **ptr);

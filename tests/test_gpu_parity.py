@@ -6,6 +6,7 @@ relative L2 against the FP64 oracle."""
 import numpy as np
 import pytest
 
+from bbwadg_amd import _lib
 from bbwadg_amd.solver import WaveSolver, MODE_FUSED, MODE_PLAIN
 import oracle as orc
 from cases import Problem, rel, rel_l2
@@ -82,10 +83,12 @@ def test_update_plain():
     assert rel(s.get_res(), rres) < 1e-6 and rel(s.WaveGetData3d(), rQ) < 1e-6
 
 
-@pytest.mark.parametrize("N,proj", [(3, False), (4, True), (5, False), (5, True)])
-def test_fused_stage(N, proj):
-    print(f"case fused N={N} proj={proj}", flush=True)
+@pytest.mark.parametrize("impl", [_lib.FUSED_VECTOR, _lib.FUSED_MFMA])
+@pytest.mark.parametrize("N,proj", [(3, False), (4, True), (5, False), (5, True), (6, False), (2, True)])
+def test_fused_stage(N, proj, impl):
+    print(f"case fused N={N} proj={proj} impl={impl}", flush=True)
     P, s = make(N, 3)
+    s.set_option(_lib.OPT_FUSED_IMPL, impl)
     Q, res = P.random_state(), P.random_state()
     s.WaveSetData3d(Q); s.set_res(res)
     fa, fb, fdt = np.float32(P.ops.rk4a[1]), np.float32(P.ops.rk4b[1]), np.float32(1e-2)
@@ -138,10 +141,14 @@ def test_heterogeneous_run_fused_vs_plain_vs_oracle():
     s.WaveSetData3d(Q0); s.set_res(np.zeros_like(Q0))
     s.step(nsteps, dt, mode=MODE_FUSED, graph=False)
     fused = s.WaveGetData3d()
+    s.set_option(_lib.OPT_FUSED_IMPL, _lib.FUSED_VECTOR)
+    s.WaveSetData3d(Q0); s.set_res(np.zeros_like(Q0))
+    s.step(nsteps, dt, mode=MODE_FUSED, graph=True)
+    fused_v = s.WaveGetData3d()
     oo = oracle_ops(P, np.float64)
     ref = Q0.astype(np.float64).copy()
     orc.rk_run(oo, P.K, P.dm.vgeo, P.dm.fgeo, P.dm.vmapP, P.c2q, ref, dt, nsteps)
-    for got in (plain, fused_p, fused):
+    for got in (plain, fused_p, fused, fused_v):
         assert rel_l2(got, ref) < 1e-5
 
 
