@@ -166,3 +166,48 @@ def monomial_integral_tri(a, b):
             coef = comb(a, i) * comb(b, j) * 2.0 ** (i + j) * (-1.0) ** (a - i + b - j)
             total += coef * factorial(i) * factorial(j) / factorial(i + j + 2)
     return 4.0 * total
+
+
+def nodal_rhs_elastic(Dr, Ds, Dt, LIFT, vgeo, fgeo, vmapM, vmapP, q, tau_v=1.0, tau_s=1.0):
+    """Elastic velocity-stress nodal DG RHS, FP64: the reference's nodal kernels
+    rk_volume_elas / rk_surface_elas (``elas_cpp/okl/ElasKernelsWADG.okl:25-284``) on nodal
+    values q (K, 9, Np): fields v1 v2 v3 sxx syy szz syz sxz sxy; boundary Q+ = 0."""
+    K, _, Np = q.shape
+    g = np.asarray(vgeo, dtype=np.float64)
+    rx, ry, rz, sx, sy, sz, tx, ty, tz = (g[:, i:i + 1] for i in range(9))
+    Qx, Qy, Qz = [], [], []
+    for f in range(9):
+        fr, fs, ft = q[:, f] @ Dr.T, q[:, f] @ Ds.T, q[:, f] @ Dt.T
+        Qx.append(rx * fr + sx * fs + tx * ft)
+        Qy.append(ry * fr + sy * fs + ty * ft)
+        Qz.append(rz * fr + sz * fs + tz * ft)
+    out = np.empty_like(q)
+    out[:, 0] = Qx[3] + Qy[8] + Qz[7]
+    out[:, 1] = Qx[8] + Qy[4] + Qz[6]
+    out[:, 2] = Qx[7] + Qy[6] + Qz[5]
+    out[:, 3] = Qx[0]
+    out[:, 4] = Qy[1]
+    out[:, 5] = Qz[2]
+    out[:, 6] = Qy[2] + Qz[1]
+    out[:, 7] = Qx[2] + Qz[0]
+    out[:, 8] = Qx[1] + Qy[0]
+    flat = q.transpose(1, 0, 2).reshape(9, -1)
+    Nfp = vmapM.shape[1] // 4
+    F = np.repeat(np.asarray(fgeo, dtype=np.float64), Nfp, axis=1)
+    Fscale, nx, ny, nz = F[..., 0], F[..., 1], F[..., 2], F[..., 3]
+    bnd = vmapM == vmapP
+    dQ = np.where(bnd[None], 0.0, flat[:, vmapP]) - flat[:, vmapM]
+    fc = [dQ[3] * nx + dQ[8] * ny + dQ[7] * nz,
+          dQ[8] * nx + dQ[4] * ny + dQ[6] * nz,
+          dQ[7] * nx + dQ[6] * ny + dQ[5] * nz,
+          dQ[0] * nx, dQ[1] * ny, dQ[2] * nz,
+          dQ[2] * ny + dQ[1] * nz, dQ[2] * nx + dQ[0] * nz, dQ[1] * nx + dQ[0] * ny]
+    fl = [fc[0] + tau_v * (fc[3] * nx + fc[8] * ny + fc[7] * nz),
+          fc[1] + tau_v * (fc[8] * nx + fc[4] * ny + fc[6] * nz),
+          fc[2] + tau_v * (fc[7] * nx + fc[6] * ny + fc[5] * nz),
+          fc[3] + tau_s * fc[0] * nx, fc[4] + tau_s * fc[1] * ny, fc[5] + tau_s * fc[2] * nz,
+          fc[6] + tau_s * (fc[2] * ny + fc[1] * nz), fc[7] + tau_s * (fc[2] * nx + fc[0] * nz),
+          fc[8] + tau_s * (fc[1] * nx + fc[0] * ny)]
+    for f in range(9):
+        out[:, f] += (0.5 * Fscale * fl[f]) @ LIFT.T
+    return out
