@@ -156,7 +156,7 @@ void bbw_destroy(bbw_ctx *c) {
   if (c->stream) hipStreamSynchronize(c->stream);
   reset_graph(c);
   dfree(c->Dids); dfree(c->Dvals); dfree(c->Fmask); dfree(c->EEL_ids); dfree(c->L0_ids);
-  dfree(c->EEL_vals); dfree(c->L0_vals); dfree(c->Vq); dfree(c->Pq); dfree(c->Vq64);
+  dfree(c->EEL_vals); dfree(c->L0_vals); dfree(c->Vq); dfree(c->Pq); dfree(c->Vq64); dfree(c->VqPad); dfree(c->PqPad); dfree(c->EEL_ids4);
   dfree(c->wq64); dfree(c->vgeo); dfree(c->fgeo); dfree(c->vmapP); dfree(c->J);
   dfree(c->c2q); dfree(c->Jq); dfree(c->rhoinv); dfree(c->lambda); dfree(c->mu);
   dfree(c->Q[0]); dfree(c->Q[1]); dfree(c->res); dfree(c->rhs); dfree(c->src_elems);
@@ -208,6 +208,22 @@ int bbw_upload_operators(bbw_ctx *c, const bbw_ops *o) {
   if ((rc = upload(c, &c->L0_vals, o->L0_vals, (size_t)o->L0_nnz * Nfp))) return rc;
   if ((rc = upload(c, &c->Vq, o->Vq, (size_t)Nq * Np))) return rc;
   if ((rc = upload(c, &c->Pq, o->Pq, (size_t)Nq * Np))) return rc;
+  {
+    const int KJ = 4 * ((Np + 3) / 4), NQP = 16 * ((Nq + 15) / 16), NPP = 16 * ((Np + 15) / 16);
+    std::vector<float> vp((size_t)KJ * NQP, 0.f), pp((size_t)NQP * NPP, 0.f);
+    for (int j = 0; j < Np; ++j)
+      for (int q = 0; q < Nq; ++q) vp[(size_t)j * NQP + q] = o->Vq[q + (size_t)j * Nq];
+    for (int q = 0; q < Nq; ++q)
+      for (int n = 0; n < Np; ++n) pp[(size_t)q * NPP + n] = o->Pq[n + (size_t)q * Np];
+    dfree(c->VqPad);
+    dfree(c->PqPad);
+    if ((rc = upload(c, &c->VqPad, vp.data(), vp.size()))) return rc;
+    if ((rc = upload(c, &c->PqPad, pp.data(), pp.size()))) return rc;
+    std::vector<int> e4((size_t)o->EEL_nnz * Np);
+    for (size_t i = 0; i < e4.size(); ++i) e4[i] = o->EEL_ids[i] * 4;  // float offset of a float4 slot
+    dfree(c->EEL_ids4);
+    if ((rc = upload(c, &c->EEL_ids4, e4.data(), e4.size()))) return rc;
+  }
   std::vector<double> v64((size_t)Nq * Np);
   for (size_t i = 0; i < v64.size(); ++i) v64[i] = o->Vq64 ? o->Vq64[i] : (double)o->Vq[i];
   if ((rc = upload(c, &c->Vq64, v64.data(), v64.size()))) return rc;

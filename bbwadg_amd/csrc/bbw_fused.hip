@@ -1,17 +1,20 @@
 // Fused acoustic stage for gfx950 with the WADG projection on FP32 MFMA.
 //
 // One workgroup (4 waves) owns a tile of TB = 16 consecutive elements:
-//   1  Q_in tile, geometry -> LDS (flat coalesced); c2 at cubature prefetched into registers
-//      in the MFMA output layout
-//   2  contravariant velocity (Ur, Us, Ut)
-//   3  volume terms (4-nnz barycentric operators) -> rhs      [operator row in registers,
-//                                                               reused for the group's elements]
-//   4  upwind flux at face nodes, neighbour traces from Q_in
-//   5  L0 per face                                             [row in registers]
-//   6  EEL lift accumulated into rhs                           [row in registers]
+//   1  Q_in tile -> LDS (float4, coalesced), geometry; prefetched into registers: the thread's
+//      res and Q_in row of every tile element, c2 in the MFMA output layout, vmapP entries
+//   2  (p, Ur, Us, Ut) as one float4 per node; neighbour traces gathered into registers
+//   3  volume terms (4-nnz barycentric operators, 16 float4 LDS reads per node) -> rhs
+//   4  upwind flux at face nodes -> float2 per face node
+//   5  L0 per face, stored as (p, U nx, U ny, U nz) per face node       [row in registers]
+//   6  EEL lift: one float4 LDS read per non-zero                        [row in registers]
 //   7  S = (Vq . rhs_p) * c2      MFMA 16x16x4, M = cubature points, N = 16 elements, K = Np
 //   8  rhs_p = Pq . S             MFMA 16x16x4, M = nodes, N = 16 elements, K = Nq
-//   9  LSRK: res = fa res + fdt rhs (+ source); Q_out = Q_in + fb res (flat coalesced)
+//   9  LSRK: res = fa res + fdt rhs (+ source); Q_out = Q_in + fb res (coalesced rows)
+// Thread mappings are fixed per thread (node, face node or state row) and loop over the
+// tile's elements, so operator rows are loaded once per tile and no index is divided.
+// LDS regions are reused across phases (Q_in tile -> lifted face data -> S; (p, U) -> flux)
+// to stay at 52 KB per workgroup (N = 5), three workgroups per CU.
 // With PROJ_U the velocity fields go through Pq . Vq as well (N = 64 columns).
 //
 // v_mfma_f32_16x16x4f32 lane layout (checked on gfx950): A[i][k] from lane i + 16k,
@@ -38,206 +41,255 @@ template <int N, bool PROJ_U>
 struct FusedLayout {
   static constexpr int Np = Dims<N>::Np, Nfp = Dims<N>::Nfp, NfpNf = Dims<N>::NfpNf;
   static constexpr int Nq = Dims<N>::Nq;
-  static constexpr int TB = 16;
-  static constexpr int RS = 4 * Np + 1;          // padded element stride of the rhs tile
+  static constexpr int TB = 16;                  // elements per tile = MFMA N
+  static constexpr int RS = 4 * Np + 3;          // odd rhs row stride; covers the K padding
   static constexpr int NQT = (Nq + 15) / 16, NQP = NQT * 16;
-  static constexpr int NPT = (Np + 15) / 16;
+  static constexpr int NPT = (Np + 15) / 16, NPP = NPT * 16;
   static constexpr int KTV = (Np + 3) / 4, KTP = NQP / 4;
   static constexpr int NP = PROJ_U ? 4 : 1;
   static constexpr int SX = NP * TB;             // S row length
-  static constexpr int X0 = 4 * NfpNf > 3 * Np ? 4 * NfpNf : 3 * Np;
-  static constexpr int XS = X0 > (NQP * NP) ? X0 : NQP * NP;   // per element work floats
+  static constexpr int cmax(int a, int b) { return a > b ? a : b; }
+  // W1: Q_in tile [TB][4Np] | lifted face data float4 [TB][NfpNf] | S [NQP][SX]
+  static constexpr int W1 = cmax(cmax(4 * Np, 4 * NfpNf), NQP * NP);
+  // W2: (p, Ur, Us, Ut) float4 [TB][Np] | flux float2 [TB][NfpNf]
+  static constexpr int W2 = cmax(4 * Np, 2 * NfpNf);
   static constexpr int QTW = (NQT + 3) / 4;     // q tiles per wave
-  static constexpr size_t floats = (size_t)TB * (4 * Np + RS + 9 + 16 + XS) + TB;  // + slots
+  static constexpr int GS = ((Np + 63) / 64) * 64, NG = 256 / GS;        // node groups
+  static constexpr int NG3 = 256 / NfpNf, EPT3 = (TB + NG3 - 1) / NG3;   // face-node groups
+  static constexpr int EPTV = (TB + NG - 1) / NG;
+  static constexpr bool PF = 4 * Np <= 256;    // one state row per thread: prefetch res, Q_in
+  static constexpr size_t floats = (size_t)TB * (W1 + W2 + RS + 9 + 16) + TB;
 };
 
 template <int N, bool PROJ_U>
 __global__ __launch_bounds__(256) void k_fused_mfma(
     int K, const float *__restrict__ vgeo, const float *__restrict__ fgeo,
     const int *__restrict__ vmapP, const int *__restrict__ Fmask, const int4 *__restrict__ Dids,
-    const float4 *__restrict__ Dvals, const int *__restrict__ EEL_ids,
+    const float4 *__restrict__ Dvals, const int *__restrict__ EEL_ids4,
     const float *__restrict__ EEL_vals, const int *__restrict__ L0_ids,
-    const float *__restrict__ L0_vals, const float *__restrict__ Vq, const float *__restrict__ Pq,
-    const float *__restrict__ c2q, float fa, float fb, float fdt, SrcArgs src,
-    const float *ftime_dev, float ftime, const float *__restrict__ Qin, float *__restrict__ res,
-    float *__restrict__ Qout) {
+    const float *__restrict__ L0_vals, const float *__restrict__ VqPad,
+    const float *__restrict__ PqPad, const float *__restrict__ c2q, float fa, float fb, float fdt,
+    SrcArgs src, const float *ftime_dev, float ftime, const float *__restrict__ Qin,
+    float *__restrict__ res, float *__restrict__ Qout) {
   using L = FusedLayout<N, PROJ_U>;
   constexpr int Np = L::Np, Nfp = L::Nfp, NfpNf = L::NfpNf, Nq = L::Nq, TB = L::TB, RS = L::RS;
-  constexpr int Nf = 4, EEL_nnz = Dims<N>::EEL_nnz, L0_nnz = Dims<N>::L0_nnz;
-  constexpr int NP = L::NP, SX = L::SX, XS = L::XS;
+  constexpr int Nf = 4, NpNf = 4 * Np, EEL_nnz = Dims<N>::EEL_nnz, L0_nnz = Dims<N>::L0_nnz;
+  constexpr int NP = L::NP, SX = L::SX, NQP = L::NQP, NPP = L::NPP;
   extern __shared__ float lds[];
-  float *sQ = lds;                  // [TB][4][Np]
-  float *sR = sQ + TB * Nf * Np;    // [TB][RS]
-  float *sG = sR + TB * RS;         // [TB][9]
-  float *sF = sG + TB * 9;          // [TB][16]
-  float *sX = sF + TB * 16;         // [TB][XS] work area
-  int *sSlot = (int *)(sX + TB * XS);
+  float *W1 = lds;                       // [TB * W1]
+  float *W2 = W1 + TB * L::W1;           // [TB * W2]
+  float *sR = W2 + TB * L::W2;           // [TB][RS]
+  float *sG = sR + TB * RS;              // [TB][9]
+  float *sF = sG + TB * 9;               // [TB][16]
+  int *sSlot = (int *)(sF + TB * 16);
+  float *sQ = W1;                        // phases 1-4
+  float4 *sL = (float4 *)W1;             // phases 5-6
+  float *S = W1;                         // phases 7-8
+  float4 *sV = (float4 *)W2;             // phases 2-3
+  float2 *sFl = (float2 *)W2;            // phases 4-5
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int k0 = blockIdx.x * TB;
   const int nk = min(TB, K - k0);
+  const size_t base = (size_t)k0 * NpNf;
+  // face-node mapping (phases 2, 4, 5)
+  const int g3 = tid / NfpNf, m3 = tid - g3 * NfpNf;
+  const bool on3 = g3 < L::NG3;
+  const int f3 = m3 / Nfp;
+  // node mapping (phases 2, 3, 6)
+  const int gv = tid / L::GS, nv = tid - gv * L::GS;
+  const bool onv = gv < L::NG && nv < Np;
 
-  // ---- 1: tile loads; c2 prefetch in the MFMA output layout
+  // ---- 1: tile loads and register prefetch
   {
-    const float *Qt = Qin + (size_t)k0 * Nf * Np;
-    for (int i = tid; i < nk * Nf * Np; i += 256) sQ[i] = Qt[i];
+    const float4 *Qt4 = (const float4 *)(Qin + base);
+    float4 *sQ4 = (float4 *)sQ;
+    for (int i = tid; i < nk * Np; i += 256) sQ4[i] = Qt4[i];
     for (int i = tid; i < nk * 9; i += 256) sG[i] = vgeo[(size_t)k0 * 9 + i];
     for (int i = tid; i < nk * 16; i += 256) sF[i] = fgeo[(size_t)k0 * 16 + i];
-    for (int i = nk * RS + tid; i < TB * RS; i += 256) sR[i] = 0.f;
+    for (int i = tid; i < TB * RS; i += 256) sR[i] = 0.f;
     if (tid < TB) sSlot[tid] = tid < nk ? src_slot2(src, k0 + tid) : -1;
   }
+  float qreg[L::PF ? TB : 1], rreg[L::PF ? TB : 1];
+  if constexpr (L::PF) {
+#pragma unroll
+    for (int e = 0; e < TB; ++e) {
+      const bool ok = tid < NpNf && e < nk;
+      qreg[e] = ok ? Qin[base + e * NpNf + tid] : 0.f;
+      rreg[e] = ok ? res[base + e * NpNf + tid] : 0.f;
+    }
+  }
   float c2r[L::QTW][4];
-  {
-    const int e = lane & 15;
 #pragma unroll
-    for (int t = 0; t < L::QTW; ++t) {
-      const int qt = wave + 4 * t;
+  for (int t = 0; t < L::QTW; ++t) {
+    const int qt = wave + 4 * t;
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int q = qt * 16 + 4 * (lane >> 4) + r;
-        c2r[t][r] = (qt < L::NQT && q < Nq && e < nk) ? c2q[(size_t)(k0 + e) * Nq + q] : 0.f;
-      }
+    for (int r = 0; r < 4; ++r) {
+      const int q = qt * 16 + 4 * (lane >> 4) + r, e = lane & 15;
+      c2r[t][r] = (qt < L::NQT && q < Nq && e < nk) ? c2q[(size_t)(k0 + e) * Nq + q] : 0.f;
     }
   }
-  __syncthreads();
-
-  // ---- 2: contravariant velocity into the work area
-  for (int w = tid; w < nk * Np; w += 256) {
-    const int e = w / Np, n = w - e * Np;
-    const float *s = sQ + e * Nf * Np;
-    const float *G = sG + 9 * e;
-    float *u = sX + e * XS;
-    const float uu = s[Np + n], vv = s[2 * Np + n], ww = s[3 * Np + n];
-    u[n] = uu * G[0] + vv * G[1] + ww * G[2];
-    u[Np + n] = uu * G[3] + vv * G[4] + ww * G[5];
-    u[2 * Np + n] = uu * G[6] + vv * G[7] + ww * G[8];
+  int vm[L::EPT3];
+  const int fid3 = on3 ? Fmask[m3] : 0;
+#pragma unroll
+  for (int i = 0; i < L::EPT3; ++i) {
+    const int e = g3 + L::NG3 * i;
+    vm[i] = (on3 && e < nk) ? vmapP[(size_t)(k0 + e) * NfpNf + m3] : -1;
   }
   __syncthreads();
 
-  // ---- 3: volume terms (thread <-> node, loop over the group's elements)
-  {
-    constexpr int GS = ((Np + 63) / 64) * 64, NG = 256 / GS;
-    const int g = tid / GS, n = tid % GS;
-    if (g < NG && n < Np) {
-      const float4 dv = Dvals[n];
-      const int4 D1 = Dids[n], D2 = Dids[Np + n], D3 = Dids[2 * Np + n], D4 = Dids[3 * Np + n];
-      for (int e = g; e < nk; e += NG) {
-        const float *sp = sQ + e * Nf * Np;
-        const float *sUr = sX + e * XS, *sUs = sUr + Np, *sUt = sUr + 2 * Np;
+  // ---- 2: (p, Ur, Us, Ut) per node; neighbour traces into registers
+  if (onv) {
+#pragma unroll
+    for (int i = 0; i < L::EPTV; ++i) {
+      const int e = gv + L::NG * i;
+      if (e < nk) {
+        const float *s = sQ + e * NpNf;
         const float *G = sG + 9 * e;
-        const float p1 = dv.x * sp[D1.x] + dv.y * sp[D2.x] + dv.z * sp[D3.x] + dv.w * sp[D4.x];
-        const float p2 = dv.x * sp[D1.y] + dv.y * sp[D2.y] + dv.z * sp[D3.y] + dv.w * sp[D4.y];
-        const float p3 = dv.x * sp[D1.z] + dv.y * sp[D2.z] + dv.z * sp[D3.z] + dv.w * sp[D4.z];
-        const float p4 = dv.x * sp[D1.w] + dv.y * sp[D2.w] + dv.z * sp[D3.w] + dv.w * sp[D4.w];
-        float dU1 = dv.x * (sUr[D1.y] + sUs[D1.z] + sUt[D1.w]);
-        dU1 += dv.y * (sUr[D2.y] + sUs[D2.z] + sUt[D2.w]);
-        dU1 += dv.z * (sUr[D3.y] + sUs[D3.z] + sUt[D3.w]);
-        dU1 += dv.w * (sUr[D4.y] + sUs[D4.z] + sUt[D4.w]);
-        float dU2 = dv.x * (sUr[D1.x] + sUs[D1.x] + sUt[D1.x]);
-        dU2 += dv.y * (sUr[D2.x] + sUs[D2.x] + sUt[D2.x]);
-        dU2 += dv.z * (sUr[D3.x] + sUs[D3.x] + sUt[D3.x]);
-        dU2 += dv.w * (sUr[D4.x] + sUs[D4.x] + sUt[D4.x]);
-        const float dpdr = .5f * (p2 - p1), dpds = .5f * (p3 - p1), dpdt = .5f * (p4 - p1);
-        float *r = sR + e * RS;
-        r[n] = -.5f * (dU1 - dU2);
-        r[Np + n] = -(G[0] * dpdr + G[3] * dpds + G[6] * dpdt);
-        r[2 * Np + n] = -(G[1] * dpdr + G[4] * dpds + G[7] * dpdt);
-        r[3 * Np + n] = -(G[2] * dpdr + G[5] * dpds + G[8] * dpdt);
+        const float uu = s[Np + nv], vv = s[2 * Np + nv], ww = s[3 * Np + nv];
+        sV[e * Np + nv] = make_float4(s[nv], uu * G[0] + vv * G[1] + ww * G[2],
+                                      uu * G[3] + vv * G[4] + ww * G[5],
+                                      uu * G[6] + vv * G[7] + ww * G[8]);
       }
     }
   }
-  __syncthreads();
-
-  // ---- 4: upwind flux at face nodes (own trace from LDS, neighbour trace from Q_in)
-  for (int w = tid; w < nk * NfpNf; w += 256) {
-    const int e = w / NfpNf, n = w - e * NfpNf, f = n / Nfp, k = k0 + e;
-    const float *F = sF + 16 * e + 4 * f;
-    const float *s = sQ + e * Nf * Np;
-    const int fid = Fmask[n];
-    const int idM = fid + k * Np * Nf;
-    const int idP = vmapP[(size_t)k * NfpNf + n];
-    const float pM = s[fid], uM = s[Np + fid], vM = s[2 * Np + fid], wM = s[3 * Np + fid];
-    float pjump, Unjump;
-    if (idM == idP) {
-      pjump = -2.f * pM;
-      Unjump = 0.f;
-    } else {
-      const float pP = Qin[idP], uP = Qin[idP + Np], vP = Qin[idP + 2 * Np], wP = Qin[idP + 3 * Np];
-      pjump = pP - pM;
-      Unjump = (uP - uM) * F[1] + (vP - vM) * F[2] + (wP - wM) * F[3];
-    }
-    float *X = sX + e * XS;
-    X[n] = .5f * (pjump - Unjump) * F[0];
-    X[NfpNf + n] = .5f * (Unjump - pjump) * F[0];
-  }
-  __syncthreads();
-
-  // ---- 5: L0 per face (thread <-> face node)
-  {
-    constexpr int NG = 256 / NfpNf;
-    const int g = tid / NfpNf, m = tid % NfpNf;
-    if (g < NG) {
-      const int f = m / Nfp, nt = m - f * Nfp;
-      float Lv[L0_nnz];
-      int Li[L0_nnz];
+  float tr[L::EPT3][4];
 #pragma unroll
-      for (int j = 0; j < L0_nnz; ++j) {
-        Lv[j] = L0_vals[nt + j * Nfp];
-        Li[j] = L0_ids[nt + j * Nfp] + f * Nfp;
+  for (int i = 0; i < L::EPT3; ++i) {
+    const int e = g3 + L::NG3 * i;
+    const int idM = fid3 + (k0 + e) * NpNf;
+    const bool inter = vm[i] >= 0 && vm[i] != idM;
+#pragma unroll
+    for (int f = 0; f < 4; ++f) tr[i][f] = inter ? Qin[vm[i] + f * Np] : 0.f;
+  }
+  __syncthreads();
+
+  // ---- 3: volume terms
+  if (onv) {
+    const float4 dv = Dvals[nv];
+    const int4 D1 = Dids[nv], D2 = Dids[Np + nv], D3 = Dids[2 * Np + nv], D4 = Dids[3 * Np + nv];
+#pragma unroll
+    for (int i = 0; i < L::EPTV; ++i) {
+      const int e = gv + L::NG * i;
+      if (e < nk) {
+        const float4 *V = sV + e * Np;
+        const float4 a1 = V[D1.x], a2 = V[D2.x], a3 = V[D3.x], a4 = V[D4.x];
+        const float4 b1 = V[D1.y], b2 = V[D2.y], b3 = V[D3.y], b4 = V[D4.y];
+        const float4 c1 = V[D1.z], c2 = V[D2.z], c3 = V[D3.z], c4 = V[D4.z];
+        const float4 d1 = V[D1.w], d2 = V[D2.w], d3 = V[D3.w], d4 = V[D4.w];
+        const float p1 = dv.x * a1.x + dv.y * a2.x + dv.z * a3.x + dv.w * a4.x;
+        const float p2 = dv.x * b1.x + dv.y * b2.x + dv.z * b3.x + dv.w * b4.x;
+        const float p3 = dv.x * c1.x + dv.y * c2.x + dv.z * c3.x + dv.w * c4.x;
+        const float p4 = dv.x * d1.x + dv.y * d2.x + dv.z * d3.x + dv.w * d4.x;
+        float dU1 = dv.x * (b1.y + c1.z + d1.w);
+        dU1 += dv.y * (b2.y + c2.z + d2.w);
+        dU1 += dv.z * (b3.y + c3.z + d3.w);
+        dU1 += dv.w * (b4.y + c4.z + d4.w);
+        float dU2 = dv.x * (a1.y + a1.z + a1.w);
+        dU2 += dv.y * (a2.y + a2.z + a2.w);
+        dU2 += dv.z * (a3.y + a3.z + a3.w);
+        dU2 += dv.w * (a4.y + a4.z + a4.w);
+        const float dpdr = .5f * (p2 - p1), dpds = .5f * (p3 - p1), dpdt = .5f * (p4 - p1);
+        const float *G = sG + 9 * e;
+        float *r = sR + e * RS;
+        r[nv] = -.5f * (dU1 - dU2);
+        r[Np + nv] = -(G[0] * dpdr + G[3] * dpds + G[6] * dpdt);
+        r[2 * Np + nv] = -(G[1] * dpdr + G[4] * dpds + G[7] * dpdt);
+        r[3 * Np + nv] = -(G[2] * dpdr + G[5] * dpds + G[8] * dpdt);
       }
-      for (int e = g; e < nk; e += NG) {
-        float *X = sX + e * XS;
+    }
+  }
+  __syncthreads();
+
+  // ---- 4: upwind flux (own trace from the Q_in tile, neighbour trace from registers)
+  if (on3) {
+#pragma unroll
+    for (int i = 0; i < L::EPT3; ++i) {
+      const int e = g3 + L::NG3 * i;
+      if (e < nk) {
+        const float *s = sQ + e * NpNf;
+        const float *F = sF + 16 * e + 4 * f3;
+        const float pM = s[fid3], uM = s[Np + fid3], vM = s[2 * Np + fid3], wM = s[3 * Np + fid3];
+        const int idM = fid3 + (k0 + e) * NpNf;
+        float pjump, Unjump;
+        if (vm[i] == idM) {
+          pjump = -2.f * pM;
+          Unjump = 0.f;
+        } else {
+          pjump = tr[i][0] - pM;
+          Unjump = (tr[i][1] - uM) * F[1] + (tr[i][2] - vM) * F[2] + (tr[i][3] - wM) * F[3];
+        }
+        sFl[e * NfpNf + m3] = make_float2(.5f * (pjump - Unjump) * F[0], .5f * (Unjump - pjump) * F[0]);
+      }
+    }
+  }
+  __syncthreads();
+
+  // ---- 5: L0 per face; store (p, U nx, U ny, U nz) so the lift needs no face lookup
+  if (on3) {
+    const int nt = m3 - f3 * Nfp;
+    float Lv[L0_nnz];
+    int Li[L0_nnz];
+#pragma unroll
+    for (int j = 0; j < L0_nnz; ++j) {
+      Lv[j] = L0_vals[nt + j * Nfp];
+      Li[j] = L0_ids[nt + j * Nfp] + f3 * Nfp;
+    }
+#pragma unroll
+    for (int i = 0; i < L::EPT3; ++i) {
+      const int e = g3 + L::NG3 * i;
+      if (e < nk) {
+        const float2 *X = sFl + e * NfpNf;
         float v1 = 0.f, v2 = 0.f;
 #pragma unroll
         for (int j = 0; j < L0_nnz; ++j) {
-          v1 += Lv[j] * X[Li[j]];
-          v2 += Lv[j] * X[NfpNf + Li[j]];
+          const float2 x = X[Li[j]];
+          v1 += Lv[j] * x.x;
+          v2 += Lv[j] * x.y;
         }
-        X[2 * NfpNf + m] = v1;
-        X[3 * NfpNf + m] = v2;
+        const float *F = sF + 16 * e + 4 * f3;
+        sL[e * NfpNf + m3] = make_float4(v1, v2 * F[1], v2 * F[2], v2 * F[3]);
       }
     }
   }
   __syncthreads();
 
-  // ---- 6: EEL lift into rhs (thread <-> node)
-  {
-    constexpr int GS = ((Np + 63) / 64) * 64, NG = 256 / GS;
-    const int g = tid / GS, n = tid % GS;
-    if (g < NG && n < Np) {
-      float Ev[EEL_nnz];
-      int Ei[EEL_nnz];
+  // ---- 6: EEL lift into rhs
+  if (onv) {
+    float Ev[EEL_nnz];
+    int Ei[EEL_nnz];
 #pragma unroll
-      for (int j = 0; j < EEL_nnz; ++j) {
-        Ev[j] = EEL_vals[n + j * Np];
-        Ei[j] = EEL_ids[n + j * Np];
-      }
-      for (int e = g; e < nk; e += NG) {
-        const float *pt = sX + e * XS + 2 * NfpNf, *ut = pt + NfpNf, *F = sF + 16 * e;
-        float *r = sR + e * RS;
-        float v1 = r[n], v2 = r[Np + n], v3 = r[2 * Np + n], v4 = r[3 * Np + n];
+    for (int j = 0; j < EEL_nnz; ++j) {
+      Ev[j] = EEL_vals[nv + j * Np];
+      Ei[j] = EEL_ids4[nv + j * Np];
+    }
+#pragma unroll
+    for (int i = 0; i < L::EPTV; ++i) {
+      const int e = gv + L::NG * i;
+      if (e < nk) {
+        const float *Lt = (const float *)(sL + e * NfpNf);
+        float v1 = 0.f, v2 = 0.f, v3 = 0.f, v4 = 0.f;
 #pragma unroll
         for (int j = 0; j < EEL_nnz; ++j) {
-          const int col = Ei[j];
-          const int fc = col / Nfp;
-          const float Uf = Ev[j] * ut[col];
-          v1 += Ev[j] * pt[col];
-          v2 += Uf * F[4 * fc + 1];
-          v3 += Uf * F[4 * fc + 2];
-          v4 += Uf * F[4 * fc + 3];
+          const float4 x = *(const float4 *)(Lt + Ei[j]);
+          v1 += Ev[j] * x.x;
+          v2 += Ev[j] * x.y;
+          v3 += Ev[j] * x.z;
+          v4 += Ev[j] * x.w;
         }
-        r[n] = v1;
-        r[Np + n] = v2;
-        r[2 * Np + n] = v3;
-        r[3 * Np + n] = v4;
+        float *r = sR + e * RS;
+        r[nv] += v1;
+        r[Np + nv] += v2;
+        r[2 * Np + nv] += v3;
+        r[3 * Np + nv] += v4;
       }
     }
   }
   __syncthreads();
 
-  // ---- 7: S = (Vq . rhs) x c2 on MFMA; S is [NQP][SX] in the work area
-  float *S = sX;
+  // ---- 7: S = (Vq . rhs) x c2 on MFMA (A from the zero-padded Vq, fixed offsets)
   {
     const int col = lane & 15, kr = lane >> 4;
+    const float *Bp = sR + col * RS + kr;
 #pragma unroll
     for (int t = 0; t < L::QTW; ++t) {
       const int qt = wave + 4 * t;
@@ -245,17 +297,13 @@ __global__ __launch_bounds__(256) void k_fused_mfma(
       f4 acc[NP];
 #pragma unroll
       for (int b = 0; b < NP; ++b) acc[b] = f4{0.f, 0.f, 0.f, 0.f};
-      const int qa = qt * 16 + col;
-#pragma unroll 7
-      for (int kt = 0; kt < L::KTV; ++kt) {
-        const int j = kt * 4 + kr;
-        const bool ok = j < Np;
-        const float a = (ok && qa < Nq) ? Vq[qa + j * Nq] : 0.f;
+      const float *Ap = VqPad + kr * NQP + qt * 16 + col;
 #pragma unroll
-        for (int b = 0; b < NP; ++b) {
-          const float bv = ok ? sR[col * RS + b * Np + j] : 0.f;
-          acc[b] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, bv, acc[b], 0, 0, 0);
-        }
+      for (int kt = 0; kt < L::KTV; ++kt) {
+        const float a = Ap[kt * 4 * NQP];
+#pragma unroll
+        for (int b = 0; b < NP; ++b)
+          acc[b] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, Bp[b * Np + kt * 4], acc[b], 0, 0, 0);
       }
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
@@ -268,21 +316,21 @@ __global__ __launch_bounds__(256) void k_fused_mfma(
   }
   __syncthreads();
 
-  // ---- 8: rhs = Pq . S on MFMA, written back into the rhs tile
+  // ---- 8: rhs = Pq . S on MFMA
   {
     const int col = lane & 15, kr = lane >> 4;
     for (int mt = wave; mt < L::NPT; mt += 4) {
       f4 acc[NP];
 #pragma unroll
       for (int b = 0; b < NP; ++b) acc[b] = f4{0.f, 0.f, 0.f, 0.f};
-      const int na = mt * 16 + col;
+      const float *Ap = PqPad + kr * NPP + mt * 16 + col;
+      const float *Bp = S + kr * SX + col;
 #pragma unroll 8
       for (int kt = 0; kt < L::KTP; ++kt) {
-        const int q = kt * 4 + kr;
-        const float a = (na < Np && q < Nq) ? Pq[na + q * Np] : 0.f;
+        const float a = Ap[kt * 4 * NPP];
 #pragma unroll
         for (int b = 0; b < NP; ++b)
-          acc[b] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, S[q * SX + b * 16 + col], acc[b], 0, 0, 0);
+          acc[b] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, Bp[kt * 4 * SX + b * 16], acc[b], 0, 0, 0);
       }
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
@@ -296,20 +344,36 @@ __global__ __launch_bounds__(256) void k_fused_mfma(
   }
   __syncthreads();
 
-  // ---- 9: LSRK stage, flat and coalesced
+  // ---- 9: LSRK stage: res = fa res + fdt rhs (+ source); Q_out = Q_in + fb res
   const float ft = ftime_dev ? *ftime_dev : ftime;
-  const size_t base = (size_t)k0 * Nf * Np;
-  for (int i = tid; i < nk * Nf * Np; i += 256) {
-    const int e = i / (Nf * Np), rem = i - e * Nf * Np;
-    float v = sR[e * RS + rem];
-    const int slot = sSlot[e];
-    if (slot >= 0) {
-      const int f = rem / Np, n = rem - f * Np;
-      v += src.w[f] * ft * src.coef[(size_t)slot * Np + n];
+  if constexpr (L::PF) {
+    if (tid < NpNf) {
+      const int fr = tid / Np, nr = tid - fr * Np;
+#pragma unroll
+      for (int e = 0; e < TB; ++e) {
+        if (e < nk) {
+          float v = sR[e * RS + tid];
+          const int slot = sSlot[e];
+          if (slot >= 0) v += src.w[fr] * ft * src.coef[(size_t)slot * Np + nr];
+          const float x = fa * rreg[e] + fdt * v;
+          res[base + e * NpNf + tid] = x;
+          Qout[base + e * NpNf + tid] = qreg[e] + fb * x;
+        }
+      }
     }
-    const float x = fa * res[base + i] + fdt * v;
-    res[base + i] = x;
-    Qout[base + i] = sQ[i] + fb * x;
+  } else {
+    for (int rr = tid; rr < NpNf; rr += 256) {
+      const int fr = rr / Np, nr = rr - fr * Np;
+      for (int e = 0; e < nk; ++e) {
+        float v = sR[e * RS + rr];
+        const int slot = sSlot[e];
+        if (slot >= 0) v += src.w[fr] * ft * src.coef[(size_t)slot * Np + nr];
+        const size_t g = base + e * NpNf + rr;
+        const float x = fa * res[g] + fdt * v;
+        res[g] = x;
+        Qout[g] = Qin[g] + fb * x;
+      }
+    }
   }
 }
 
@@ -325,9 +389,9 @@ static hipError_t fused_mfma_n(bbw_ctx *c, const float *Qin, float *Qout, float 
     if (e != hipSuccess) return e;
   }
   hipLaunchKernelGGL((k_fused_mfma<N, P>), dim3(grid), dim3(256), sh, c->stream, c->K, c->vgeo,
-                     c->fgeo, c->vmapP, c->Fmask, c->Dids, c->Dvals, c->EEL_ids, c->EEL_vals,
-                     c->L0_ids, c->L0_vals, c->Vq, c->Pq, c->c2q, fa, fb, fdt, s, ftd, ft, Qin,
-                     c->res, Qout);
+                     c->fgeo, c->vmapP, c->Fmask, c->Dids, c->Dvals, c->EEL_ids4, c->EEL_vals,
+                     c->L0_ids, c->L0_vals, c->VqPad, c->PqPad, c->c2q, fa, fb, fdt, s, ftd, ft,
+                     Qin, c->res, Qout);
   return hipGetLastError();
 }
 

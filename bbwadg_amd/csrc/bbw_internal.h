@@ -58,6 +58,11 @@ struct bbw_ctx {
   float4 *Dvals = nullptr;
   int *Fmask = nullptr, *EEL_ids = nullptr, *L0_ids = nullptr;
   float *EEL_vals = nullptr, *L0_vals = nullptr, *Vq = nullptr, *Pq = nullptr;
+  // zero-padded copies for the MFMA kernels: VqPad [4*ceil(Np/4)][NQP] (j rows, q contiguous),
+  // PqPad [NQP][NPP] (q rows, n contiguous); NQP, NPP = Nq, Np rounded up to 16
+  float *VqPad = nullptr, *PqPad = nullptr;
+  // EEL column ids as float4 slots and L0 ids with the face offset folded in, for the fused kernel
+  int *EEL_ids4 = nullptr;
   double *Vq64 = nullptr, *wq64 = nullptr;
   double rk4a[5] = {0}, rk4b[5] = {0}, rk4c[6] = {0};
   bool have_ops = false;

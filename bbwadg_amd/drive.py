@@ -1,0 +1,57 @@
+"""Run the acoustic hot path on one GPU for profiling or timing (the analogue of the
+reference's ``runBlkTimings`` / ``profileKernels`` runs, ``cpp/runBlkTimings:4-24``).
+
+    python -m bbwadg_amd.drive --mesh 32 --N 5 --steps 10 --mode fused --impl mfma
+
+Builds a seeded cube mesh with Fourier c^2, runs ``--steps`` LSRK45 steps in the chosen
+mode and prints one line with the time per stage.
+"""
+import argparse
+import time
+
+import numpy as np
+
+from . import _lib, media, mesh as M, operators as O
+from .solver import WaveSolver, MODE_FUSED, MODE_PLAIN
+
+
+def setup(n, N, seed=1, **kw):
+    ops = O.build_ref_ops(N)
+    m = M.cube_mesh(n)
+    M.face_pair(m)
+    dm = M.device_arrays(m, ops, 4)
+    xq, yq, zq = media.cubature_points(m, ops)
+    c2q = media.c2_fourier(seed)(xq, yq, zq).astype(np.float32)
+    s = WaveSolver(m, N, ops=ops, dmesh=dm, **kw)
+    s.InitWADG_subelem(c2q)
+    rng = np.random.default_rng(seed)
+    s.WaveSetData3d(rng.uniform(-1, 1, (m.K, 4, ops.Np)).astype(np.float32))
+    return s
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--mesh", type=int, default=32)
+    ap.add_argument("--N", type=int, default=5)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--mode", choices=["fused", "plain"], default="fused")
+    ap.add_argument("--impl", choices=["mfma", "vector"], default="mfma")
+    ap.add_argument("--project-velocity", action="store_true")
+    args = ap.parse_args()
+    s = setup(args.mesh, args.N)
+    s.set_option(_lib.OPT_FUSED_IMPL, _lib.FUSED_MFMA if args.impl == "mfma" else _lib.FUSED_VECTOR)
+    mode = MODE_FUSED if args.mode == "fused" else MODE_PLAIN
+    dt = 0.5 / args.mesh / args.N ** 2
+    s.step(2, dt, mode=mode, project_velocity=args.project_velocity)
+    s.sync()
+    t0 = time.perf_counter()
+    s.step(args.steps, dt, mode=mode, project_velocity=args.project_velocity)
+    s.sync()
+    el = time.perf_counter() - t0
+    dofs = s.K * s.ops.Np * 4 * 5 * args.steps
+    print(f"K={s.K} N={args.N} mode={args.mode} impl={args.impl} steps={args.steps} "
+          f"ms/stage={el / (5 * args.steps) * 1e3:.4f} DOF-stage/s={dofs / el:.4e}", flush=True)
+
+
+if __name__ == "__main__":
+    main()
