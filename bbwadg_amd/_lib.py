@@ -27,7 +27,8 @@ EXPORTS = [
     "bbw_set_source", "bbw_set_receivers", "bbw_get_traces", "bbw_set_state", "bbw_get_state",
     "bbw_set_res", "bbw_get_res", "bbw_set_rhs", "bbw_get_rhs", "bbw_volume", "bbw_surface",
     "bbw_update_wadg", "bbw_update", "bbw_stage_fused", "bbw_step", "bbw_energy",
-    "bbw_device_ptr", "bbw_set_option",
+    "bbw_device_ptr", "bbw_set_option", "bbw_upload_adjoint_maps", "bbw_misfit",
+    "bbw_adjoint_gradient",
 ]
 OPT_FUSED_IMPL, OPT_KBLK_V, OPT_KBLK_S, OPT_KBLK_U, OPT_KBLK_F = 0, 1, 2, 3, 4
 FUSED_VECTOR, FUSED_MFMA = 0, 1
@@ -96,6 +97,9 @@ def load():
         "bbw_energy": ([P, ctypes.POINTER(D)], I),
         "bbw_device_ptr": ([P, I, ctypes.POINTER(P)], I),
         "bbw_set_option": ([P, I, I], I),
+        "bbw_upload_adjoint_maps": ([P, P], I),
+        "bbw_misfit": ([P, I, D, D, P, ctypes.POINTER(D)], I),
+        "bbw_adjoint_gradient": ([P, I, D, D, P, I, P, ctypes.POINTER(D)], I),
     }
     for name, (args, res) in sig.items():
         fn = getattr(lib, name)

@@ -1,0 +1,119 @@
+"""Receiver-misfit gradients w.r.t. c^2 through the hand-written discrete adjoint.
+
+No counterpart in the reference (its solver has no adjoint); the forward scheme is the
+reference's WADG LSRK45 path (``cpp/src/WaveOKL3d.cpp:2014-2109``).
+
+    inv = AcousticInversion(solver, shots, rec_points, nsteps, dt, ckpt_every=50)
+    J, g = inv.misfit_and_gradient(c2q, d_obs)          # numpy, summed over shots
+    J = C2Misfit.apply(c2q_tensor, inv, d_obs)          # torch.autograd.Function
+    J.backward(); c2q_tensor.grad                       # dJ/dc2q at the cubature points
+
+Each shot is a source (element ids, BB coefficients, Ricker f0, amplitude).  All shots
+share the receivers.  Every call goes to libbbwadg_hip.so (bbw_misfit,
+bbw_adjoint_gradient); the gradient is that of the FP32 discrete scheme.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib, media, mesh as meshmod
+from ._lib import check, f32, i32, ptr
+
+
+class Shot:
+    def __init__(self, elems, coef, f0, amp=1.0, weights=None):
+        self.elems, self.coef, self.f0, self.amp, self.weights = elems, coef, f0, amp, weights
+
+
+class AcousticInversion:
+    def __init__(self, solver, shots, rec_points, nsteps, dt, ckpt_every=50, t0=0.0):
+        if solver.nfields != 4:
+            raise ValueError("the adjoint is acoustic only")
+        self.s = solver
+        self.shots = list(shots)
+        self.nsteps, self.dt, self.ckpt_every, self.t0 = int(nsteps), float(dt), int(ckpt_every), t0
+        relems, B = media.receivers(solver.mesh, solver.ops, rec_points)
+        self.rec_elems, self.rec_B = relems, B
+        solver.set_receivers(relems, B, self.nsteps)
+        mapP = i32(meshmod.build_mapP(solver.mesh, solver.ops))
+        check(solver.ctx, solver.lib.bbw_upload_adjoint_maps(solver.ctx, ptr(mapP)))
+        self.Q0 = np.zeros((solver.K, 4, solver.ops.Np), dtype=np.float32)
+
+    def _set_shot(self, shot):
+        self.s.set_source(shot.elems, shot.coef, shot.f0, shot.amp, shot.weights)
+        self.s.WaveSetData3d(self.Q0)
+
+    def set_c2(self, c2q):
+        self.s.InitWADG_subelem(np.asarray(c2q, dtype=np.float32))
+
+    def observe(self, c2q):
+        """Synthetic observations: receiver traces (nshots, nsteps, nrec) for model c2q."""
+        self.set_c2(c2q)
+        out = []
+        for sh in self.shots:
+            self._set_shot(sh)
+            self.s.rec_reset()
+            self.s.step(self.nsteps, self.dt, t0=self.t0)
+            out.append(self.s.get_traces(self.nsteps))
+        return np.stack(out)
+
+    def misfit(self, c2q, d_obs):
+        self.set_c2(c2q)
+        J = 0.0
+        for sh, d in zip(self.shots, d_obs):
+            self._set_shot(sh)
+            v = ctypes.c_double()
+            d = f32(d)
+            check(self.s.ctx, self.s.lib.bbw_misfit(self.s.ctx, self.nsteps, self.dt, self.t0, ptr(d),
+                                                    ctypes.byref(v)))
+            J += v.value
+        return J
+
+    def misfit_and_gradient(self, c2q, d_obs):
+        self.set_c2(c2q)
+        J = 0.0
+        g = np.zeros((self.s.K, self.s.ops.Nq), dtype=np.float64)
+        gs = np.zeros((self.s.K, self.s.ops.Nq), dtype=np.float32)
+        for sh, d in zip(self.shots, d_obs):
+            self._set_shot(sh)
+            v = ctypes.c_double()
+            d = f32(d)
+            check(self.s.ctx, self.s.lib.bbw_adjoint_gradient(self.s.ctx, self.nsteps, self.dt, self.t0,
+                                                              ptr(d), self.ckpt_every, ptr(gs),
+                                                              ctypes.byref(v)))
+            J += v.value
+            g += gs
+        return J, g
+
+
+def _torch_function():
+    import torch
+
+    class C2Misfit(torch.autograd.Function):
+        """J(c2q) = sum over shots of 1/2 ||receiver traces - d_obs||^2; backward returns the
+        adjoint gradient (computed with the forward, checkpoint-and-recompute inside)."""
+
+        @staticmethod
+        def forward(ctx, c2q, inv, d_obs):
+            c2 = c2q.detach().to("cpu", torch.float32).numpy()
+            J, g = inv.misfit_and_gradient(c2, d_obs)
+            ctx.save_for_backward(torch.as_tensor(g, dtype=c2q.dtype, device=c2q.device))
+            return torch.tensor(J, dtype=torch.float64, device=c2q.device)
+
+        @staticmethod
+        def backward(ctx, gJ):
+            (g,) = ctx.saved_tensors
+            return (gJ.to(g.dtype) * g, None, None)
+
+    return C2Misfit
+
+
+_C2MISFIT = None
+
+
+def c2_misfit(c2q, inv, d_obs):
+    """torch entry point: ``c2_misfit(c2q_tensor, inv, d_obs)`` -> scalar tensor with grad."""
+    global _C2MISFIT
+    if _C2MISFIT is None:
+        _C2MISFIT = _torch_function()
+    return _C2MISFIT.apply(c2q, inv, d_obs)

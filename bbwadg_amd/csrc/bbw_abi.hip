@@ -157,6 +157,10 @@ void bbw_destroy(bbw_ctx *c) {
   reset_graph(c);
   dfree(c->Dids); dfree(c->Dvals); dfree(c->Fmask); dfree(c->EEL_ids); dfree(c->L0_ids);
   dfree(c->EEL_vals); dfree(c->L0_vals); dfree(c->Vq); dfree(c->Pq); dfree(c->Vq64); dfree(c->VqPad); dfree(c->PqPad); dfree(c->EEL_ids4);
+  dfree(c->DTids); dfree(c->DTvals); dfree(c->EELT_ids); dfree(c->EELT_vals); dfree(c->L0T_ids);
+  dfree(c->L0T_vals); dfree(c->FmaskInv); dfree(c->mapP); dfree(c->lQ); dfree(c->lres);
+  dfree(c->Zadj); dfree(c->Hadj); dfree(c->grad); dfree(c->Ybuf); dfree(c->ckpt);
+  dfree(c->resid); dfree(c->Q0save);
   dfree(c->wq64); dfree(c->vgeo); dfree(c->fgeo); dfree(c->vmapP); dfree(c->J);
   dfree(c->c2q); dfree(c->Jq); dfree(c->rhoinv); dfree(c->lambda); dfree(c->mu);
   dfree(c->Q[0]); dfree(c->Q[1]); dfree(c->res); dfree(c->rhs); dfree(c->src_elems);
@@ -219,6 +223,72 @@ int bbw_upload_operators(bbw_ctx *c, const bbw_ops *o) {
     dfree(c->PqPad);
     if ((rc = upload(c, &c->VqPad, vp.data(), vp.size()))) return rc;
     if ((rc = upload(c, &c->PqPad, pp.data(), pp.size()))) return rc;
+    // transposed operators for the adjoint
+    std::vector<double> D(4 * (size_t)Np * Np, 0.0);
+    for (int m = 0; m < 4; ++m)
+      for (int i = 0; i < Np; ++i)
+        for (int j = 0; j < 4; ++j)
+          D[((size_t)m * Np + i) * Np + o->D_ids[(j * Np + i) * 4 + m]] += o->Dvals[i * 4 + j];
+    std::vector<int> dti(16 * (size_t)Np, 0);
+    std::vector<float> dtv(16 * (size_t)Np, 0.f);
+    for (int m = 0; m < 4; ++m)
+      for (int cc = 0; cc < Np; ++cc) {
+        int w = 0;
+        for (int i = 0; i < Np; ++i) {
+          const double v = D[((size_t)m * Np + i) * Np + cc];
+          if (v != 0.0) {
+            if (w == 4) return fail(c, BBW_ERR_ARG, "transposed derivative row has more than 4 entries");
+            dti[((size_t)m * Np + cc) * 4 + w] = i;
+            dtv[((size_t)m * Np + cc) * 4 + w] = (float)v;
+            ++w;
+          }
+        }
+      }
+    dfree(c->DTids);
+    dfree(c->DTvals);
+    if ((rc = upload(c, &c->DTids, (const int4 *)dti.data(), (size_t)4 * Np))) return rc;
+    if ((rc = upload(c, &c->DTvals, (const float4 *)dtv.data(), (size_t)4 * Np))) return rc;
+    const int NfpNf = 4 * Nfp;
+    auto transpose_sparse = [&](const int32_t *ids, const float *vals, int rows, int cols, int nnz,
+                                std::vector<int> &tid, std::vector<float> &tval, int &width) {
+      std::vector<std::vector<std::pair<int, float>>> colv(cols);
+      for (int i = 0; i < rows; ++i)
+        for (int j = 0; j < nnz; ++j) {
+          const float v = vals[i + (size_t)j * rows];
+          if (v != 0.f) colv[ids[i + (size_t)j * rows]].push_back({i, v});
+        }
+      width = 1;
+      for (auto &cv : colv) width = (int)cv.size() > width ? (int)cv.size() : width;
+      tid.assign((size_t)width * cols, 0);
+      tval.assign((size_t)width * cols, 0.f);
+      for (int r = 0; r < cols; ++r)
+        for (size_t j = 0; j < colv[r].size(); ++j) {
+          tid[r + j * cols] = colv[r][j].first;
+          tval[r + j * cols] = colv[r][j].second;
+        }
+    };
+    std::vector<int> ti;
+    std::vector<float> tv;
+    transpose_sparse(o->EEL_ids, o->EEL_vals, Np, NfpNf, o->EEL_nnz, ti, tv, c->WE);
+    dfree(c->EELT_ids);
+    dfree(c->EELT_vals);
+    if ((rc = upload(c, &c->EELT_ids, ti.data(), ti.size()))) return rc;
+    if ((rc = upload(c, &c->EELT_vals, tv.data(), tv.size()))) return rc;
+    transpose_sparse(o->L0_ids, o->L0_vals, Nfp, Nfp, o->L0_nnz, ti, tv, c->WL);
+    dfree(c->L0T_ids);
+    dfree(c->L0T_vals);
+    if ((rc = upload(c, &c->L0T_ids, ti.data(), ti.size()))) return rc;
+    if ((rc = upload(c, &c->L0T_vals, tv.data(), tv.size()))) return rc;
+    std::vector<int> finv(3 * (size_t)Np, -1);
+    for (int r = 0; r < NfpNf; ++r) {
+      const int n = o->Fmask[r];
+      int a = 0;
+      while (a < 3 && finv[3 * n + a] >= 0) ++a;
+      if (a == 3) return fail(c, BBW_ERR_ARG, "a node lies on more than three faces");
+      finv[3 * n + a] = r;
+    }
+    dfree(c->FmaskInv);
+    if ((rc = upload(c, &c->FmaskInv, finv.data(), finv.size()))) return rc;
     std::vector<int> e4((size_t)o->EEL_nnz * Np);
     for (size_t i = 0; i < e4.size(); ++i) e4[i] = o->EEL_ids[i] * 4;  // float offset of a float4 slot
     dfree(c->EEL_ids4);
@@ -521,6 +591,153 @@ int bbw_energy(bbw_ctx *c, double *E) {
   HIPCHK(c, hipMemcpyAsync(E, c->partial + nb, sizeof(double), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   return BBW_OK;
+}
+
+int bbw_upload_adjoint_maps(bbw_ctx *c, const int32_t *mapP) {
+  if (!c || !mapP) return BBW_ERR_ARG;
+  const size_t n = (size_t)c->K * 4 * c->Nfp;
+  for (size_t i = 0; i < n; ++i)
+    if (mapP[i] < 0 || (size_t)mapP[i] >= n || (size_t)mapP[mapP[i]] != i)
+      return fail(c, BBW_ERR_ARG, "mapP must be an involution on the face nodes");
+  return upload(c, &c->mapP, (const int *)mapP, n);
+}
+
+namespace {
+
+// Forward nsteps from the current state, recording receivers, keeping a copy of the state at
+// the start of every checkpoint segment (ckpt may be null).  Returns traces on the host.
+int forward_record(bbw_ctx *c, int nsteps, double dt, double t0, int C, std::vector<float> &tr) {
+  int rc;
+  c->rec_steps = 0;
+  const size_t n = nstate(c);
+  for (int s0 = 0; s0 < nsteps; s0 += C) {
+    const int ns = nsteps - s0 < C ? nsteps - s0 : C;
+    if (c->ckpt)
+      HIPCHK(c, hipMemcpyAsync(c->ckpt + (size_t)(s0 / C) * n, c->Q[c->cur], n * sizeof(float),
+                               hipMemcpyDeviceToDevice, c->stream));
+    if ((rc = bbw_step(c, ns, dt, t0 + s0 * dt, BBW_MODE_FUSED, 2))) return rc;
+  }
+  tr.resize((size_t)nsteps * c->nrec);
+  return bbw_get_traces(c, tr.data(), nsteps);
+}
+
+int check_inverse(bbw_ctx *c, int nsteps, const float *d_obs) {
+  int rc = check_ready(c);
+  if (rc) return rc;
+  if (c->Nf != 4) return fail(c, BBW_ERR_UNSUPPORTED, "the adjoint is acoustic only");
+  if (!d_obs || nsteps <= 0) return fail(c, BBW_ERR_ARG, "bad observations");
+  if (c->nrec == 0 || c->rec_max_steps < nsteps) return fail(c, BBW_ERR_STATE, "receivers not set for nsteps");
+  return BBW_OK;
+}
+
+int save_state(bbw_ctx *c) {
+  const size_t n = nstate(c);
+  int rc = dalloc(c, &c->Q0save, n);
+  if (rc) return rc;
+  HIPCHK(c, hipMemcpyAsync(c->Q0save, c->Q[c->cur], n * sizeof(float), hipMemcpyDeviceToDevice, c->stream));
+  HIPCHK(c, hipMemsetAsync(c->res, 0, n * sizeof(float), c->stream));
+  return BBW_OK;
+}
+
+int restore_state(bbw_ctx *c) {
+  const size_t n = nstate(c);
+  HIPCHK(c, hipMemcpyAsync(c->Q[c->cur], c->Q0save, n * sizeof(float), hipMemcpyDeviceToDevice, c->stream));
+  HIPCHK(c, hipMemsetAsync(c->res, 0, n * sizeof(float), c->stream));
+  c->rec_steps = 0;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return BBW_OK;
+}
+
+double misfit_host(const std::vector<float> &tr, const float *d, size_t n, std::vector<float> *resid) {
+  double J = 0.0;
+  if (resid) resid->resize(n);
+  for (size_t i = 0; i < n; ++i) {
+    const double r = (double)tr[i] - (double)d[i];
+    J += 0.5 * r * r;
+    if (resid) (*resid)[i] = (float)r;
+  }
+  return J;
+}
+
+}  // namespace
+
+int bbw_misfit(bbw_ctx *c, int nsteps, double dt, double t0, const float *d_obs, double *J) {
+  if (!c || !J) return BBW_ERR_ARG;
+  int rc = check_inverse(c, nsteps, d_obs);
+  if (rc) return rc;
+  if ((rc = save_state(c))) return rc;
+  float *ck = c->ckpt;
+  c->ckpt = nullptr;
+  std::vector<float> tr;
+  rc = forward_record(c, nsteps, dt, t0, nsteps, tr);
+  c->ckpt = ck;
+  if (rc) return rc;
+  *J = misfit_host(tr, d_obs, tr.size(), nullptr);
+  return restore_state(c);
+}
+
+int bbw_adjoint_gradient(bbw_ctx *c, int nsteps, double dt, double t0, const float *d_obs,
+                         int ckpt_every, float *grad_out, double *J) {
+  if (!c || !grad_out || !J || ckpt_every <= 0) return BBW_ERR_ARG;
+  int rc = check_inverse(c, nsteps, d_obs);
+  if (rc) return rc;
+  if (!c->mapP) return fail(c, BBW_ERR_STATE, "adjoint maps not uploaded");
+  const int C = ckpt_every < nsteps ? ckpt_every : nsteps;
+  const int nseg = (nsteps + C - 1) / C;
+  const size_t n = nstate(c), nq = (size_t)c->K * c->Nq, nh = (size_t)c->K * 4 * c->Nfp;
+  // buffers (kept for later calls of the same or smaller size)
+  if (c->ckpt_len < (size_t)nseg * n) { dfree(c->ckpt); c->ckpt_len = 0; }
+  if ((rc = dalloc(c, &c->ckpt, (size_t)nseg * n))) return rc;
+  c->ckpt_len = (size_t)nseg * n > c->ckpt_len ? (size_t)nseg * n : c->ckpt_len;
+  if (c->Ybuf_len < (size_t)C * 5 * nq) { dfree(c->Ybuf); c->Ybuf_len = 0; }
+  if ((rc = dalloc(c, &c->Ybuf, (size_t)C * 5 * nq))) return rc;
+  c->Ybuf_len = (size_t)C * 5 * nq > c->Ybuf_len ? (size_t)C * 5 * nq : c->Ybuf_len;
+  if (c->resid_len < (size_t)nsteps * c->nrec) { dfree(c->resid); c->resid_len = 0; }
+  if ((rc = dalloc(c, &c->resid, (size_t)nsteps * c->nrec))) return rc;
+  c->resid_len = (size_t)nsteps * c->nrec > c->resid_len ? (size_t)nsteps * c->nrec : c->resid_len;
+  if ((rc = dalloc(c, &c->lQ, n)) || (rc = dalloc(c, &c->lres, n)) || (rc = dalloc(c, &c->Zadj, n)) ||
+      (rc = dalloc(c, &c->Hadj, nh)) || (rc = dalloc(c, &c->grad, nq)))
+    return rc;
+  if ((rc = save_state(c))) return rc;
+
+  // 1. forward with checkpoints and receiver traces
+  std::vector<float> tr, resid;
+  if ((rc = forward_record(c, nsteps, dt, t0, C, tr))) return rc;
+  *J = misfit_host(tr, d_obs, tr.size(), &resid);
+  HIPCHK(c, hipMemcpyAsync(c->resid, resid.data(), resid.size() * sizeof(float), hipMemcpyHostToDevice, c->stream));
+
+  // 2. backward over segments: recompute the segment keeping Vq (A Q)_p of every stage
+  HIPCHK(c, hipMemsetAsync(c->lQ, 0, n * sizeof(float), c->stream));
+  HIPCHK(c, hipMemsetAsync(c->lres, 0, n * sizeof(float), c->stream));
+  HIPCHK(c, hipMemsetAsync(c->grad, 0, nq * sizeof(float), c->stream));
+  std::vector<float> ft(5);
+  for (int seg = nseg - 1; seg >= 0; --seg) {
+    const int s0 = seg * C, ns = nsteps - s0 < C ? nsteps - s0 : C;
+    HIPCHK(c, hipMemcpyAsync(c->Q[c->cur], c->ckpt + (size_t)seg * n, n * sizeof(float),
+                             hipMemcpyDeviceToDevice, c->stream));
+    HIPCHK(c, hipMemsetAsync(c->res, 0, n * sizeof(float), c->stream));
+    for (int j = 0; j < ns; ++j) {
+      for (int i = 0; i < 5; ++i) {
+        const float f = c->nsrc ? (float)ricker(t0 + (s0 + j) * dt + c->rk4c[i] * dt, c->src_f0) : 0.f;
+        hipError_t e = launch_fused_mfma(c, c->Q[c->cur], c->Q[1 - c->cur], (float)c->rk4a[i],
+                                         (float)c->rk4b[i], (float)dt, nullptr, f, 0,
+                                         c->Ybuf + ((size_t)j * 5 + i) * nq);
+        if (e != hipSuccess) return hipfail(c, e, "forward recompute");
+        c->cur = 1 - c->cur;
+      }
+    }
+    for (int j = ns - 1; j >= 0; --j) {
+      hipError_t e = launch_adj_inject(c, c->resid + (size_t)(s0 + j) * c->nrec);
+      if (e != hipSuccess) return hipfail(c, e, "receiver injection");
+      for (int i = 4; i >= 0; --i) {
+        e = launch_adj_stage(c, c->Ybuf + ((size_t)j * 5 + i) * nq, (float)c->rk4a[i],
+                             (float)c->rk4b[i], (float)dt);
+        if (e != hipSuccess) return hipfail(c, e, "adjoint stage");
+      }
+    }
+  }
+  HIPCHK(c, hipMemcpyAsync(grad_out, c->grad, nq * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+  return restore_state(c);
 }
 
 int bbw_set_option(bbw_ctx *c, int option, int value) {

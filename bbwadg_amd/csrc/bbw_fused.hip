@@ -70,7 +70,7 @@ __global__ __launch_bounds__(256) void k_fused_mfma(
     const float *__restrict__ L0_vals, const float *__restrict__ VqPad,
     const float *__restrict__ PqPad, const float *__restrict__ c2q, float fa, float fb, float fdt,
     SrcArgs src, const float *ftime_dev, float ftime, const float *__restrict__ Qin,
-    float *__restrict__ res, float *__restrict__ Qout) {
+    float *__restrict__ res, float *__restrict__ Qout, float *__restrict__ yout) {
   using L = FusedLayout<N, PROJ_U>;
   constexpr int Np = L::Np, Nfp = L::Nfp, NfpNf = L::NfpNf, Nq = L::Nq, TB = L::TB, RS = L::RS;
   constexpr int Nf = 4, NpNf = 4 * Np, EEL_nnz = Dims<N>::EEL_nnz, L0_nnz = Dims<N>::L0_nnz;
@@ -305,6 +305,14 @@ __global__ __launch_bounds__(256) void k_fused_mfma(
         for (int b = 0; b < NP; ++b)
           acc[b] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, Bp[b * Np + kt * 4], acc[b], 0, 0, 0);
       }
+      if (yout && col < nk) {
+        // Vq . rhs_p at the cubature points, kept for the adjoint's c2 gradient
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int q = qt * 16 + 4 * kr + r;
+          if (q < Nq) yout[(size_t)(k0 + col) * Nq + q] = acc[0][r];
+        }
+      }
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int q = qt * 16 + 4 * kr + r;
@@ -379,7 +387,8 @@ __global__ __launch_bounds__(256) void k_fused_mfma(
 
 template <int N, bool P>
 static hipError_t fused_mfma_n(bbw_ctx *c, const float *Qin, float *Qout, float fa, float fb,
-                               float fdt, const float *ftd, float ft, const SrcArgs &s) {
+                               float fdt, const float *ftd, float ft, const SrcArgs &s,
+                               float *yout) {
   using L = FusedLayout<N, P>;
   const size_t sh = sizeof(float) * L::floats;
   const int grid = (c->K + L::TB - 1) / L::TB;
@@ -391,20 +400,20 @@ static hipError_t fused_mfma_n(bbw_ctx *c, const float *Qin, float *Qout, float 
   hipLaunchKernelGGL((k_fused_mfma<N, P>), dim3(grid), dim3(256), sh, c->stream, c->K, c->vgeo,
                      c->fgeo, c->vmapP, c->Fmask, c->Dids, c->Dvals, c->EEL_ids4, c->EEL_vals,
                      c->L0_ids, c->L0_vals, c->VqPad, c->PqPad, c->c2q, fa, fb, fdt, s, ftd, ft,
-                     Qin, c->res, Qout);
+                     Qin, c->res, Qout, yout);
   return hipGetLastError();
 }
 
 template <int N>
 static hipError_t fused_mfma_sel(bbw_ctx *c, const float *Qin, float *Qout, float fa, float fb,
                                  float fdt, const float *ftd, float ft, int projU,
-                                 const SrcArgs &s) {
-  return projU ? fused_mfma_n<N, true>(c, Qin, Qout, fa, fb, fdt, ftd, ft, s)
-               : fused_mfma_n<N, false>(c, Qin, Qout, fa, fb, fdt, ftd, ft, s);
+                                 const SrcArgs &s, float *yout) {
+  return projU ? fused_mfma_n<N, true>(c, Qin, Qout, fa, fb, fdt, ftd, ft, s, yout)
+               : fused_mfma_n<N, false>(c, Qin, Qout, fa, fb, fdt, ftd, ft, s, yout);
 }
 
 hipError_t launch_fused_mfma(bbw_ctx *c, const float *Qin, float *Qout, float fa, float fb,
-                             float fdt, const float *ftd, float ft, int projU) {
+                             float fdt, const float *ftd, float ft, int projU, float *yout) {
   SrcArgs s;
   s.nsrc = c->nsrc;
   s.first = c->src_first;
@@ -413,12 +422,12 @@ hipError_t launch_fused_mfma(bbw_ctx *c, const float *Qin, float *Qout, float fa
   s.coef = c->src_coef;
   for (int f = 0; f < 9; ++f) s.w[f] = c->src_w[f];
   switch (c->N) {
-    case 1: return fused_mfma_sel<1>(c, Qin, Qout, fa, fb, fdt, ftd, ft, projU, s);
-    case 2: return fused_mfma_sel<2>(c, Qin, Qout, fa, fb, fdt, ftd, ft, projU, s);
-    case 3: return fused_mfma_sel<3>(c, Qin, Qout, fa, fb, fdt, ftd, ft, projU, s);
-    case 4: return fused_mfma_sel<4>(c, Qin, Qout, fa, fb, fdt, ftd, ft, projU, s);
-    case 5: return fused_mfma_sel<5>(c, Qin, Qout, fa, fb, fdt, ftd, ft, projU, s);
-    case 6: return fused_mfma_sel<6>(c, Qin, Qout, fa, fb, fdt, ftd, ft, projU, s);
+    case 1: return fused_mfma_sel<1>(c, Qin, Qout, fa, fb, fdt, ftd, ft, projU, s, yout);
+    case 2: return fused_mfma_sel<2>(c, Qin, Qout, fa, fb, fdt, ftd, ft, projU, s, yout);
+    case 3: return fused_mfma_sel<3>(c, Qin, Qout, fa, fb, fdt, ftd, ft, projU, s, yout);
+    case 4: return fused_mfma_sel<4>(c, Qin, Qout, fa, fb, fdt, ftd, ft, projU, s, yout);
+    case 5: return fused_mfma_sel<5>(c, Qin, Qout, fa, fb, fdt, ftd, ft, projU, s, yout);
+    case 6: return fused_mfma_sel<6>(c, Qin, Qout, fa, fb, fdt, ftd, ft, projU, s, yout);
   }
   return hipErrorInvalidValue;
 }

@@ -100,6 +100,17 @@ struct bbw_ctx {
   double *partial = nullptr;
   int npartial = 0;
 
+  // adjoint: transposed operators (D_m^T rows [4][Np] int4/float4, EEL^T [WE][4Nfp], L0^T
+  // [WL][Nfp]), node -> face-node slots [Np][3], face-node neighbour map [K][4Nfp], buffers
+  int4 *DTids = nullptr;
+  float4 *DTvals = nullptr;
+  int *EELT_ids = nullptr, *L0T_ids = nullptr, *FmaskInv = nullptr, *mapP = nullptr;
+  float *EELT_vals = nullptr, *L0T_vals = nullptr;
+  int WE = 0, WL = 0;
+  float *lQ = nullptr, *lres = nullptr, *Zadj = nullptr, *Hadj = nullptr, *grad = nullptr;
+  float *Ybuf = nullptr, *ckpt = nullptr, *resid = nullptr, *Q0save = nullptr;
+  size_t Ybuf_len = 0, ckpt_len = 0, resid_len = 0;
+
   // graph cache and per-window stage data
   bbw::Graph graph;
   float *ftime_win = nullptr;  // [steps_in_window * 5]
@@ -117,7 +128,10 @@ hipError_t launch_update(bbw_ctx *c, float fa, float fb, float fdt);
 hipError_t launch_fused(bbw_ctx *c, const float *Qin, float *Qout, float fa, float fb, float fdt,
                         const float *ftime_dev, float ftime, int project_velocity);
 hipError_t launch_fused_mfma(bbw_ctx *c, const float *Qin, float *Qout, float fa, float fb,
-                             float fdt, const float *ftime_dev, float ftime, int project_velocity);
+                             float fdt, const float *ftime_dev, float ftime, int project_velocity,
+                             float *yout = nullptr);
+hipError_t launch_adj_stage(bbw_ctx *c, const float *Y, float fa, float fb, float fdt);
+hipError_t launch_adj_inject(bbw_ctx *c, const float *resid_row);
 hipError_t launch_energy(bbw_ctx *c, double *E_dev_partial, int *nblocks);
 hipError_t launch_reduce(bbw_ctx *c, const double *partial, int n, double *out);
 hipError_t launch_record(bbw_ctx *c, const float *Q, float *row);

@@ -234,6 +234,28 @@ def face_node_match_table(ops):
 PERMS = [(0, 1, 2), (0, 2, 1), (1, 0, 2), (1, 2, 0), (2, 0, 1), (2, 1, 0)]
 
 
+def build_mapP(mesh, ops):
+    """mapP: (K, 4*Nfp) global face-node index ``k2*4Nfp + f2*Nfp + n2`` of the exterior
+    trace of each face node (itself on boundary faces); the face-node counterpart of vmapP
+    (the reference keeps it as ``mapP``, ``cpp/src/StartUp3d.cpp:659-712``)."""
+    if mesh.EToE is None:
+        face_pair(mesh)
+    K, Nfp = mesh.K, ops.Nfp
+    NfpNf = NFACES * Nfp
+    mapP = np.arange(K * NfpNf, dtype=np.int64).reshape(K, NfpNf)
+    tab = face_node_match_table(ops)
+    k1, f1 = np.nonzero(mesh.EToE != np.arange(K)[:, None])
+    k2, f2 = mesh.EToE[k1, f1], mesh.EToF[k1, f1]
+    gv1 = mesh.EToV[k1[:, None], FACE_VERTS[f1]]
+    gv2 = mesh.EToV[k2[:, None], FACE_VERTS[f2]]
+    pidx = np.full(len(k1), -1, dtype=np.int64)
+    for pi, p in enumerate(PERMS):
+        pidx[np.all(gv2[:, list(p)] == gv1, axis=1)] = pi
+    n2 = tab[f1, f2, pidx]
+    mapP[k1[:, None], f1[:, None] * Nfp + np.arange(Nfp)[None, :]] = (k2 * NfpNf + f2 * Nfp)[:, None] + n2
+    return mapP
+
+
 def build_maps(mesh, ops):
     """vmapM, vmapP: (K, 4*Nfp) node ids ``k*Np + n`` of the interior / exterior trace.
 

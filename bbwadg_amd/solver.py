@@ -106,8 +106,13 @@ class WaveSolver:
     def set_receivers(self, elems, basis_vals, max_steps):
         elems, basis_vals = i32(elems), f32(basis_vals)
         self.nrec = len(elems)
+        self._rec = (elems, basis_vals, int(max_steps))
         check(self.ctx, self.lib.bbw_set_receivers(self.ctx, len(elems), ptr(elems), ptr(basis_vals),
                                                    int(max_steps)))
+
+    def rec_reset(self):
+        """Restart receiver recording at trace row 0."""
+        self.set_receivers(*self._rec)
 
     def get_traces(self, nsteps):
         out = np.zeros((nsteps, self.nrec), dtype=np.float32)

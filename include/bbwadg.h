@@ -138,6 +138,16 @@ int bbw_step(bbw_ctx *ctx, int nsteps, double dt, double t0, int mode, int flags
 /* FP64 energy 1/2 sum_K J sum_q w_q (p_q^2 / c2_q + |u_q|^2) (acoustic), reduced on device. */
 int bbw_energy(bbw_ctx *ctx, double *E);
 
+/* Discrete adjoint (no reference counterpart; acoustic, fused forward with Pq Vq = I on the
+ * velocity).  Misfit J = 1/2 sum_n sum_r (p_r(Q_{n+1}) - d_obs[n][r])^2 over nsteps steps from the
+ * current state, receivers as set by bbw_set_receivers; gradient dJ/dc2q [K][Nq] by the
+ * reverse LSRK45 stages with the state checkpointed every ckpt_every steps and each segment
+ * recomputed.  Both calls leave the context's state as it was. */
+int bbw_upload_adjoint_maps(bbw_ctx *ctx, const int32_t *mapP); /* [K][4Nfp] face-node map */
+int bbw_misfit(bbw_ctx *ctx, int nsteps, double dt, double t0, const float *d_obs, double *J);
+int bbw_adjoint_gradient(bbw_ctx *ctx, int nsteps, double dt, double t0, const float *d_obs,
+                         int ckpt_every, float *grad, double *J);
+
 /* Tuning options (no reference counterpart beyond the Kblk* compile-time defines of
  * WaveInitOCCA3d, cpp/src/WaveOKL3d.cpp:1631-1635): fused kernel implementation and
  * elements per block; used by the autotuner. */

@@ -1,0 +1,81 @@
+"""GPU discrete adjoint against the FP64 host adjoint (oracle/adjoint.py, itself pinned by the
+dot-product identity and central differences) on the same seeded problem; checkpoint
+placement does not change the result; GPU central differences agree along random
+directions; the torch.autograd.Function returns the same gradient."""
+import numpy as np
+import pytest
+
+from bbwadg_amd import media
+from bbwadg_amd.adjoint import AcousticInversion, Shot, c2_misfit
+from bbwadg_amd.solver import WaveSolver
+import oracle as orc
+import adjoint as adjm
+from cases import Problem, rel_l2
+
+pytestmark = pytest.mark.gpu
+
+
+def setup(N=3, n=3, nsteps=24, nshots=2):
+    P = Problem(N, n, seed=5)
+    s = WaveSolver(P.mesh, N, ops=P.ops, dmesh=P.dm)
+    shots = []
+    for i in range(nshots):
+        x0 = (0.15 * (i - 0.5), -0.1, 0.05 * i)
+        el, co = media.gaussian_source(P.mesh, P.ops, x0, a=8.0)
+        shots.append(Shot(el, co, f0=8.0 + 2 * i, amp=30.0))
+    rec = media.surface_receivers(2, 9)
+    dt = P.dt_cfl()
+    inv = AcousticInversion(s, shots, rec, nsteps, dt, ckpt_every=7)
+    c2_true = P.c2q
+    d = inv.observe(c2_true)
+    c2 = np.ones_like(c2_true)
+    return P, s, inv, d, c2, shots
+
+
+def test_gradient_matches_fp64_host_adjoint():
+    print("case adjoint vs FP64 host adjoint", flush=True)
+    P, s, inv, d, c2, shots = setup()
+    J, g = inv.misfit_and_gradient(c2, d)
+    oo = orc.OracleOps(P.ops, np.float64)
+    adj = adjm.AdjointOps(P.ops, P.dm)
+    Jr, gr = 0.0, np.zeros_like(g)
+    for sh, dd in zip(shots, d):
+        src = np.zeros((P.K, P.ops.Np)); src[sh.elems] = sh.coef
+        gi, Ji = adjm.gradient(oo, P.ops, P.dm, adj, c2.astype(np.float32).astype(np.float64),
+                               np.zeros((P.K, 4, P.ops.Np)), inv.dt, inv.nsteps, inv.rec_elems,
+                               inv.rec_B, dd.astype(np.float64), src, sh.f0, sh.amp)
+        Jr += Ji
+        gr += gi
+    assert J > 0
+    assert abs(J - Jr) < 1e-4 * Jr
+    assert rel_l2(g, gr) < 1e-3
+
+
+def test_checkpoint_interval_is_exact_and_fd_agrees():
+    print("case adjoint checkpoints + FD", flush=True)
+    P, s, inv, d, c2, shots = setup(nshots=1)
+    J1, g1 = inv.misfit_and_gradient(c2, d)
+    inv.ckpt_every = 24
+    J2, g2 = inv.misfit_and_gradient(c2, d)
+    assert J1 == J2 and np.array_equal(g1, g2)
+    assert abs(inv.misfit(c2, d) - J1) < 1e-6 * J1
+    rng = np.random.default_rng(4)
+    eps = 1e-3  # FP32 forward: larger step than the FP64 check, same central formula
+    errs = []
+    for _ in range(4):
+        v = rng.standard_normal(c2.shape)
+        fd = (inv.misfit(c2 + eps * v, d) - inv.misfit(c2 - eps * v, d)) / (2 * eps)
+        ad = float(np.sum(g1 * v))
+        errs.append(abs(fd - ad) / abs(ad))
+    assert max(errs) < 2e-2, errs
+
+
+def test_autograd_function():
+    import torch
+    P, s, inv, d, c2, shots = setup(nshots=1)
+    J, g = inv.misfit_and_gradient(c2, d)
+    t = torch.tensor(c2, dtype=torch.float32, device="cuda", requires_grad=True)
+    Jt = c2_misfit(t, inv, d)
+    Jt.backward()
+    assert abs(float(Jt) - J) < 1e-12 * J
+    assert np.allclose(t.grad.cpu().numpy(), g.astype(np.float32))
