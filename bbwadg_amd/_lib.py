@@ -68,6 +68,12 @@ def load():
         return _lib
     if not os.path.exists(LIB_PATH):
         raise BBWLibraryError(f"{LIB_PATH} is not built; run __graft_entry__.build()")
+    # torch ships its own libamdhip64.so.7; the library must bind to the same HIP runtime as
+    # torch (one runtime per process), so torch is loaded first when it is installed
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     try:
         lib = ctypes.CDLL(LIB_PATH)
     except OSError as e:

@@ -72,6 +72,41 @@ def cpu_baseline(ops, dm, c2q, K, sample_elems, threads):
     return Ks * ops.Np * 4 * 5 / el, el, Ks
 
 
+def adjoint_ratio(nsteps, mesh_n=32, N=5, ckpt=50):
+    """Time of one adjoint gradient (forward with checkpoints + segment recompute + reverse
+    stages) over the time of one forward misfit evaluation, on the adjoint-gradient
+    configuration's shape (32^3 x 6 tets, N = 5, 256 surface receivers, c2 = 1 start model,
+    seeded Fourier true model, one Ricker shot)."""
+    from bbwadg_amd import media, mesh as M, operators as O
+    from bbwadg_amd.adjoint import AcousticInversion, Shot
+    from bbwadg_amd.solver import WaveSolver
+    ops = O.build_ref_ops(N)
+    m = M.cube_mesh(mesh_n)
+    M.face_pair(m)
+    dm = M.device_arrays(m, ops, 4)
+    s = WaveSolver(m, N, ops=ops, dmesh=dm)
+    xq, yq, zq = media.cubature_points(m, ops)
+    c2_true = media.c2_fourier(3)(xq, yq, zq)
+    el, co = media.gaussian_source(m, ops, (0.05, -0.08, 0.1), a=100.0)
+    dt = 0.5 / mesh_n / N ** 2
+    inv = AcousticInversion(s, [Shot(el, co, f0=10.0, amp=1.0)], media.surface_receivers(4, 256),
+                            nsteps, dt, ckpt_every=ckpt)
+    d = inv.observe(c2_true)
+    c2 = np.ones_like(c2_true)
+    inv.misfit(c2, d)  # warm-up (graph capture, allocations)
+    inv.misfit_and_gradient(c2, d)
+    t0 = time.perf_counter()
+    Jf = inv.misfit(c2, d)
+    tf = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    Jg, g = inv.misfit_and_gradient(c2, d)
+    tg = time.perf_counter() - t0
+    s.close()
+    return {"ratio": tg / tf, "forward_s": tf, "gradient_s": tg, "steps": nsteps, "K": m.K,
+            "ckpt_every": ckpt, "misfit": Jf, "grad_norm": float(np.linalg.norm(g)),
+            "misfit_agree": abs(Jf - Jg) / max(Jf, 1e-300)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -82,6 +117,7 @@ def main():
     ap.add_argument("--plain-steps", type=int, default=200)
     ap.add_argument("--cpu-sample", type=int, default=1 << 30)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--adjoint-steps", type=int, default=200)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported "
@@ -141,6 +177,9 @@ def main():
         with open(tfile) as f:
             traffic = json.load(f).get("bytes_per_launch")
 
+    s.close()
+    adj = adjoint_ratio(args.adjoint_steps) if args.adjoint_steps > 0 else None
+
     cpu = None
     if not args.no_cpu:
         threads = max(1, min(os.cpu_count() or 1, 16))
@@ -168,6 +207,8 @@ def main():
         "plain": {"value": plain_value, "steps": args.plain_steps,
                   "ms_per_step": ms_plain / args.plain_steps},
         "fused_over_plain": value / plain_value,
+        "adjoint_over_forward": adj["ratio"] if adj else None,
+        "adjoint": adj,
         "energy_after": E,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
