@@ -30,7 +30,7 @@ EXPORTS = [
     "bbw_device_ptr", "bbw_set_option", "bbw_upload_adjoint_maps", "bbw_misfit",
     "bbw_adjoint_gradient",
 ]
-OPT_FUSED_IMPL, OPT_KBLK_V, OPT_KBLK_S, OPT_KBLK_U, OPT_KBLK_F = 0, 1, 2, 3, 4
+OPT_FUSED_IMPL, OPT_KBLK_V, OPT_KBLK_S, OPT_KBLK_U, OPT_KBLK_F, OPT_ADJ_CACHE_MB, OPT_ADJ_IMPL = 0, 1, 2, 3, 4, 5, 6
 FUSED_VECTOR, FUSED_MFMA = 0, 1
 
 

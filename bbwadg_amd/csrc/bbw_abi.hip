@@ -156,11 +156,11 @@ void bbw_destroy(bbw_ctx *c) {
   if (c->stream) hipStreamSynchronize(c->stream);
   reset_graph(c);
   dfree(c->Dids); dfree(c->Dvals); dfree(c->Fmask); dfree(c->EEL_ids); dfree(c->L0_ids);
-  dfree(c->EEL_vals); dfree(c->L0_vals); dfree(c->Vq); dfree(c->Pq); dfree(c->Vq64); dfree(c->VqPad); dfree(c->PqPad); dfree(c->EEL_ids4);
+  dfree(c->EEL_vals); dfree(c->L0_vals); dfree(c->Vq); dfree(c->Pq); dfree(c->Vq64); dfree(c->VqPad); dfree(c->PqPad); dfree(c->EEL_ids4); dfree(c->PqTPad); dfree(c->VqTPad);
   dfree(c->DTids); dfree(c->DTvals); dfree(c->EELT_ids); dfree(c->EELT_vals); dfree(c->L0T_ids);
   dfree(c->L0T_vals); dfree(c->FmaskInv); dfree(c->mapP); dfree(c->lQ); dfree(c->lres);
   dfree(c->Zadj); dfree(c->Hadj); dfree(c->grad); dfree(c->Ybuf); dfree(c->ckpt);
-  dfree(c->resid); dfree(c->Q0save);
+  dfree(c->resid); dfree(c->Q0save); dfree(c->Ycache);
   dfree(c->wq64); dfree(c->vgeo); dfree(c->fgeo); dfree(c->vmapP); dfree(c->J);
   dfree(c->c2q); dfree(c->Jq); dfree(c->rhoinv); dfree(c->lambda); dfree(c->mu);
   dfree(c->Q[0]); dfree(c->Q[1]); dfree(c->res); dfree(c->rhs); dfree(c->src_elems);
@@ -223,6 +223,15 @@ int bbw_upload_operators(bbw_ctx *c, const bbw_ops *o) {
     dfree(c->PqPad);
     if ((rc = upload(c, &c->VqPad, vp.data(), vp.size()))) return rc;
     if ((rc = upload(c, &c->PqPad, pp.data(), pp.size()))) return rc;
+    std::vector<float> ptp((size_t)KJ * NQP, 0.f), vtp((size_t)NQP * NPP, 0.f);
+    for (int nn = 0; nn < Np; ++nn)
+      for (int q = 0; q < Nq; ++q) ptp[(size_t)nn * NQP + q] = o->Pq[nn + (size_t)q * Np];
+    for (int q = 0; q < Nq; ++q)
+      for (int nn = 0; nn < Np; ++nn) vtp[(size_t)q * NPP + nn] = o->Vq[q + (size_t)nn * Nq];
+    dfree(c->PqTPad);
+    dfree(c->VqTPad);
+    if ((rc = upload(c, &c->PqTPad, ptp.data(), ptp.size()))) return rc;
+    if ((rc = upload(c, &c->VqTPad, vtp.data(), vtp.size()))) return rc;
     // transposed operators for the adjoint
     std::vector<double> D(4 * (size_t)Np * Np, 0.0);
     for (int m = 0; m < 4; ++m)
@@ -682,59 +691,105 @@ int bbw_adjoint_gradient(bbw_ctx *c, int nsteps, double dt, double t0, const flo
   int rc = check_inverse(c, nsteps, d_obs);
   if (rc) return rc;
   if (!c->mapP) return fail(c, BBW_ERR_STATE, "adjoint maps not uploaded");
-  const int C = ckpt_every < nsteps ? ckpt_every : nsteps;
-  const int nseg = (nsteps + C - 1) / C;
   const size_t n = nstate(c), nq = (size_t)c->K * c->Nq, nh = (size_t)c->K * 4 * c->Nfp;
-  // buffers (kept for later calls of the same or smaller size)
-  if (c->ckpt_len < (size_t)nseg * n) { dfree(c->ckpt); c->ckpt_len = 0; }
-  if ((rc = dalloc(c, &c->ckpt, (size_t)nseg * n))) return rc;
-  c->ckpt_len = (size_t)nseg * n > c->ckpt_len ? (size_t)nseg * n : c->ckpt_len;
-  if (c->Ybuf_len < (size_t)C * 5 * nq) { dfree(c->Ybuf); c->Ybuf_len = 0; }
-  if ((rc = dalloc(c, &c->Ybuf, (size_t)C * 5 * nq))) return rc;
-  c->Ybuf_len = (size_t)C * 5 * nq > c->Ybuf_len ? (size_t)C * 5 * nq : c->Ybuf_len;
-  if (c->resid_len < (size_t)nsteps * c->nrec) { dfree(c->resid); c->resid_len = 0; }
-  if ((rc = dalloc(c, &c->resid, (size_t)nsteps * c->nrec))) return rc;
-  c->resid_len = (size_t)nsteps * c->nrec > c->resid_len ? (size_t)nsteps * c->nrec : c->resid_len;
+  const size_t ystep = 5 * nq;  // floats of Vq (A Q)_p per step
   if ((rc = dalloc(c, &c->lQ, n)) || (rc = dalloc(c, &c->lres, n)) || (rc = dalloc(c, &c->Zadj, n)) ||
       (rc = dalloc(c, &c->Hadj, nh)) || (rc = dalloc(c, &c->grad, nq)))
     return rc;
+  if (c->resid_len < (size_t)nsteps * c->nrec) { dfree(c->resid); c->resid_len = 0; }
+  if ((rc = dalloc(c, &c->resid, (size_t)nsteps * c->nrec))) return rc;
+  c->resid_len = (size_t)nsteps * c->nrec > c->resid_len ? (size_t)nsteps * c->nrec : c->resid_len;
+
+  // how many final steps keep their Vq (A Q)_p from the first forward pass
+  size_t ncache = 0;
+  if (c->ycache_mb >= 0) {
+    size_t budget;
+    if (c->ycache_mb > 0) {
+      budget = (size_t)c->ycache_mb << 20;
+    } else {
+      size_t fr = 0, tot = 0;
+      HIPCHK(c, hipMemGetInfo(&fr, &tot));
+      budget = (size_t)(0.6 * (double)(fr + c->Ycache_len * sizeof(float)));
+    }
+    ncache = budget / (ystep * sizeof(float));
+    if (ncache > (size_t)nsteps) ncache = nsteps;
+  }
+  const int s_cache0 = nsteps - (int)ncache;
+  if (ncache) {
+    if (c->Ycache_len < ncache * ystep) { dfree(c->Ycache); c->Ycache_len = 0; }
+    if ((rc = dalloc(c, &c->Ycache, ncache * ystep))) return rc;
+    c->Ycache_len = ncache * ystep > c->Ycache_len ? ncache * ystep : c->Ycache_len;
+  }
+  const int C = ckpt_every < s_cache0 ? ckpt_every : (s_cache0 > 0 ? s_cache0 : 1);
+  const int nseg = s_cache0 > 0 ? (s_cache0 + C - 1) / C : 0;
+  if (nseg) {
+    if (c->ckpt_len < (size_t)nseg * n) { dfree(c->ckpt); c->ckpt_len = 0; }
+    if ((rc = dalloc(c, &c->ckpt, (size_t)nseg * n))) return rc;
+    c->ckpt_len = (size_t)nseg * n > c->ckpt_len ? (size_t)nseg * n : c->ckpt_len;
+    if (c->Ybuf_len < (size_t)C * ystep) { dfree(c->Ybuf); c->Ybuf_len = 0; }
+    if ((rc = dalloc(c, &c->Ybuf, (size_t)C * ystep))) return rc;
+    c->Ybuf_len = (size_t)C * ystep > c->Ybuf_len ? (size_t)C * ystep : c->Ybuf_len;
+  }
   if ((rc = save_state(c))) return rc;
 
-  // 1. forward with checkpoints and receiver traces
-  std::vector<float> tr, resid;
-  if ((rc = forward_record(c, nsteps, dt, t0, C, tr))) return rc;
+  auto stage_time = [&](int step, int i) -> float {
+    return c->nsrc ? (float)ricker(t0 + step * dt + c->rk4c[i] * dt, c->src_f0) : 0.f;
+  };
+  auto fwd_step_y = [&](int step, float *Y) -> int {
+    for (int i = 0; i < 5; ++i) {
+      hipError_t e = launch_fused_mfma(c, c->Q[c->cur], c->Q[1 - c->cur], (float)c->rk4a[i],
+                                       (float)c->rk4b[i], (float)dt, nullptr, stage_time(step, i),
+                                       0, Y + (size_t)i * nq);
+      if (e != hipSuccess) return hipfail(c, e, "forward stage with Vq (A Q)_p capture");
+      c->cur = 1 - c->cur;
+    }
+    return BBW_OK;
+  };
+  auto adj_step = [&](int step, const float *Y) -> int {
+    hipError_t e = launch_adj_inject(c, c->resid + (size_t)step * c->nrec);
+    if (e != hipSuccess) return hipfail(c, e, "receiver injection");
+    for (int i = 4; i >= 0; --i) {
+      e = c->adj_impl ? launch_adj_stage_tiled(c, Y + (size_t)i * nq, (float)c->rk4a[i], (float)c->rk4b[i], (float)dt)
+                      : launch_adj_stage(c, Y + (size_t)i * nq, (float)c->rk4a[i], (float)c->rk4b[i], (float)dt);
+      if (e != hipSuccess) return hipfail(c, e, "adjoint stage");
+    }
+    return BBW_OK;
+  };
+
+  // 1. forward: checkpointed segments, then the cached final steps; receiver traces
+  c->rec_steps = 0;
+  for (int s0 = 0; s0 < s_cache0; s0 += C) {
+    const int ns = s_cache0 - s0 < C ? s_cache0 - s0 : C;
+    HIPCHK(c, hipMemcpyAsync(c->ckpt + (size_t)(s0 / C) * n, c->Q[c->cur], n * sizeof(float),
+                             hipMemcpyDeviceToDevice, c->stream));
+    if ((rc = bbw_step(c, ns, dt, t0 + s0 * dt, BBW_MODE_FUSED, 2))) return rc;
+  }
+  for (int s = s_cache0; s < nsteps; ++s) {
+    if ((rc = fwd_step_y(s, c->Ycache + (size_t)(s - s_cache0) * ystep))) return rc;
+    hipError_t e = launch_record(c, c->Q[c->cur], c->rec_trace + (size_t)c->rec_steps * c->nrec);
+    if (e != hipSuccess) return hipfail(c, e, "record launch");
+    c->rec_steps += 1;
+  }
+  std::vector<float> tr((size_t)nsteps * c->nrec), resid;
+  if ((rc = bbw_get_traces(c, tr.data(), nsteps))) return rc;
   *J = misfit_host(tr, d_obs, tr.size(), &resid);
   HIPCHK(c, hipMemcpyAsync(c->resid, resid.data(), resid.size() * sizeof(float), hipMemcpyHostToDevice, c->stream));
 
-  // 2. backward over segments: recompute the segment keeping Vq (A Q)_p of every stage
+  // 2. reverse: cached steps first, then checkpointed segments (recomputed with capture)
   HIPCHK(c, hipMemsetAsync(c->lQ, 0, n * sizeof(float), c->stream));
   HIPCHK(c, hipMemsetAsync(c->lres, 0, n * sizeof(float), c->stream));
   HIPCHK(c, hipMemsetAsync(c->grad, 0, nq * sizeof(float), c->stream));
-  std::vector<float> ft(5);
+  for (int s = nsteps - 1; s >= s_cache0; --s)
+    if ((rc = adj_step(s, c->Ycache + (size_t)(s - s_cache0) * ystep))) return rc;
   for (int seg = nseg - 1; seg >= 0; --seg) {
-    const int s0 = seg * C, ns = nsteps - s0 < C ? nsteps - s0 : C;
+    const int s0 = seg * C, ns = s_cache0 - s0 < C ? s_cache0 - s0 : C;
     HIPCHK(c, hipMemcpyAsync(c->Q[c->cur], c->ckpt + (size_t)seg * n, n * sizeof(float),
                              hipMemcpyDeviceToDevice, c->stream));
     HIPCHK(c, hipMemsetAsync(c->res, 0, n * sizeof(float), c->stream));
-    for (int j = 0; j < ns; ++j) {
-      for (int i = 0; i < 5; ++i) {
-        const float f = c->nsrc ? (float)ricker(t0 + (s0 + j) * dt + c->rk4c[i] * dt, c->src_f0) : 0.f;
-        hipError_t e = launch_fused_mfma(c, c->Q[c->cur], c->Q[1 - c->cur], (float)c->rk4a[i],
-                                         (float)c->rk4b[i], (float)dt, nullptr, f, 0,
-                                         c->Ybuf + ((size_t)j * 5 + i) * nq);
-        if (e != hipSuccess) return hipfail(c, e, "forward recompute");
-        c->cur = 1 - c->cur;
-      }
-    }
-    for (int j = ns - 1; j >= 0; --j) {
-      hipError_t e = launch_adj_inject(c, c->resid + (size_t)(s0 + j) * c->nrec);
-      if (e != hipSuccess) return hipfail(c, e, "receiver injection");
-      for (int i = 4; i >= 0; --i) {
-        e = launch_adj_stage(c, c->Ybuf + ((size_t)j * 5 + i) * nq, (float)c->rk4a[i],
-                             (float)c->rk4b[i], (float)dt);
-        if (e != hipSuccess) return hipfail(c, e, "adjoint stage");
-      }
-    }
+    for (int j = 0; j < ns; ++j)
+      if ((rc = fwd_step_y(s0 + j, c->Ybuf + (size_t)j * ystep))) return rc;
+    for (int j = ns - 1; j >= 0; --j)
+      if ((rc = adj_step(s0 + j, c->Ybuf + (size_t)j * ystep))) return rc;
   }
   HIPCHK(c, hipMemcpyAsync(grad_out, c->grad, nq * sizeof(float), hipMemcpyDeviceToHost, c->stream));
   return restore_state(c);
@@ -751,6 +806,8 @@ int bbw_set_option(bbw_ctx *c, int option, int value) {
     case BBW_OPT_KBLK_S: c->KblkS = value > 0 ? value : 0; break;
     case BBW_OPT_KBLK_U: c->KblkU = value > 0 ? value : 0; break;
     case BBW_OPT_KBLK_F: c->KblkF = value > 0 ? value : 0; break;
+    case BBW_OPT_ADJ_CACHE_MB: c->ycache_mb = value; break;
+    case BBW_OPT_ADJ_IMPL: c->adj_impl = value ? 1 : 0; break;
     default: return fail(c, BBW_ERR_ARG, "unknown option");
   }
   reset_graph(c);

@@ -248,3 +248,287 @@ hipError_t launch_adj_inject(bbw_ctx *c, const float *resid_row) {
 }
 
 }  // namespace bbw
+
+// ---------------------------------------------------------------------------------------------
+// Tiled adjoint stage: two kernels per reverse stage, 16-element tiles, W^T on FP32 MFMA.
+namespace bbw {
+
+typedef float f4a __attribute__((ext_vector_type(4)));
+
+template <int N>
+struct AdjLayout {
+  static constexpr int Np = Dims<N>::Np, Nfp = Dims<N>::Nfp, NfpNf = Dims<N>::NfpNf;
+  static constexpr int Nq = Dims<N>::Nq, TB = 16;
+  static constexpr int RS = 4 * Np + 3;
+  static constexpr int NQT = (Nq + 15) / 16, NQP = 16 * NQT, NPT = (Np + 15) / 16, NPP = 16 * NPT;
+  static constexpr int KTV = (Np + 3) / 4, KTP = NQP / 4, QTW = (NQT + 3) / 4;
+  static constexpr int NG3 = 256 / NfpNf, EPT3 = (TB + NG3 - 1) / NG3;
+  static constexpr int GS = ((Np + 63) / 64) * 64, NG = 256 / GS, EPTV = (TB + NG - 1) / NG;
+  // sg/sz [TB][RS] | S [NQP][16] | sT float4 [TB][NfpNf] | sF [TB][16]
+  static constexpr size_t floats1 = (size_t)TB * RS + NQP * 16 + (size_t)TB * 4 * NfpNf + TB * 16;
+  // sZ4 float4 [TB][Np] | sH [TB][NfpNf] | sG [TB][9] | sF [TB][16]
+  static constexpr size_t floats2 = (size_t)TB * 4 * Np + (size_t)TB * NfpNf + TB * 9 + TB * 16;
+};
+
+// g = lres + b lQ; lres <- a g; t = Pq^T g_p; grad += dt t y; z_p = Vq^T (c2 t); z_u = g_u -> Z;
+// h = 1/2 Fscale (L0^T EEL^T z_p - n . L0^T EEL^T z_u) -> H
+template <int N>
+__global__ __launch_bounds__(256) void k_adj1(
+    int K, const float *__restrict__ PqTPad, const float *__restrict__ VqTPad,
+    const float *__restrict__ c2q, const float *__restrict__ Y, const float *__restrict__ fgeo,
+    const int *__restrict__ EELT_ids, const float *__restrict__ EELT_vals, int WE,
+    const int *__restrict__ L0T_ids, const float *__restrict__ L0T_vals, int WL, float fa,
+    float fb, float fdt, const float *__restrict__ lQ, float *__restrict__ lres,
+    float *__restrict__ Z, float *__restrict__ H, float *__restrict__ grad) {
+  using L = AdjLayout<N>;
+  constexpr int Np = L::Np, Nfp = L::Nfp, NfpNf = L::NfpNf, Nq = L::Nq, TB = L::TB, RS = L::RS;
+  constexpr int NpNf = 4 * Np, NQP = L::NQP, NPP = L::NPP;
+  extern __shared__ float lds[];
+  float *sg = lds;                        // [TB][RS]
+  float *S = sg + TB * RS;                // [NQP][16]
+  float4 *sT = (float4 *)(S + NQP * 16);  // [TB][NfpNf]
+  float *sF = (float *)(sT + TB * NfpNf); // [TB][16]
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int k0 = blockIdx.x * TB;
+  const int nk = min(TB, K - k0);
+  const size_t base = (size_t)k0 * NpNf;
+  for (int i = tid; i < TB * RS; i += 256) sg[i] = 0.f;
+  for (int i = tid; i < nk * 16; i += 256) sF[i] = fgeo[(size_t)k0 * 16 + i];
+  __syncthreads();
+  for (int rr = tid; rr < NpNf; rr += 256)
+    for (int e = 0; e < nk; ++e) {
+      const size_t gi = base + (size_t)e * NpNf + rr;
+      const float g = lres[gi] + fb * lQ[gi];
+      lres[gi] = fa * g;
+      sg[e * RS + rr] = g;
+    }
+  __syncthreads();
+  // t = Pq^T g_p (M = cubature points, N = elements, K = nodes); gradient; S = c2 t
+  {
+    const int col = lane & 15, kr = lane >> 4;
+    const float *Bp = sg + col * RS + kr;
+#pragma unroll
+    for (int t = 0; t < L::QTW; ++t) {
+      const int qt = wave + 4 * t;
+      if (qt >= L::NQT) break;
+      f4a acc = {0.f, 0.f, 0.f, 0.f};
+      const float *Ap = PqTPad + kr * NQP + qt * 16 + col;
+#pragma unroll
+      for (int kt = 0; kt < L::KTV; ++kt)
+        acc = __builtin_amdgcn_mfma_f32_16x16x4f32(Ap[kt * 4 * NQP], Bp[kt * 4], acc, 0, 0, 0);
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int q = qt * 16 + 4 * kr + r;
+        float sv = 0.f;
+        if (q < Nq && col < nk) {
+          const size_t id = (size_t)(k0 + col) * Nq + q;
+          grad[id] += fdt * acc[r] * Y[id];
+          sv = c2q[id] * acc[r];
+        }
+        S[q * 16 + col] = sv;
+      }
+    }
+  }
+  __syncthreads();
+  // z_p = Vq^T S (M = nodes, K = cubature points), written over g_p
+  {
+    const int col = lane & 15, kr = lane >> 4;
+    for (int mt = wave; mt < L::NPT; mt += 4) {
+      f4a acc = {0.f, 0.f, 0.f, 0.f};
+      const float *Ap = VqTPad + kr * NPP + mt * 16 + col;
+      const float *Bp = S + kr * 16 + col;
+#pragma unroll 8
+      for (int kt = 0; kt < L::KTP; ++kt)
+        acc = __builtin_amdgcn_mfma_f32_16x16x4f32(Ap[kt * 4 * NPP], Bp[kt * 64], acc, 0, 0, 0);
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int n = mt * 16 + 4 * kr + r;
+        if (n < Np) sg[col * RS + n] = acc[r];
+      }
+    }
+  }
+  __syncthreads();
+  // Z out; EEL^T z per face node (4 fields) -> sT
+  for (int rr = tid; rr < NpNf; rr += 256)
+    for (int e = 0; e < nk; ++e) Z[base + (size_t)e * NpNf + rr] = sg[e * RS + rr];
+  {
+    const int g3 = tid / NfpNf, m3 = tid - g3 * NfpNf;
+    if (g3 < L::NG3) {
+      for (int e = g3; e < nk; e += L::NG3) {
+        const float *z = sg + e * RS;
+        float a0 = 0.f, a1 = 0.f, a2 = 0.f, a3 = 0.f;
+        for (int j = 0; j < WE; ++j) {
+          const int cc = EELT_ids[m3 + j * NfpNf];
+          const float v = EELT_vals[m3 + j * NfpNf];
+          a0 += v * z[cc];
+          a1 += v * z[Np + cc];
+          a2 += v * z[2 * Np + cc];
+          a3 += v * z[3 * Np + cc];
+        }
+        sT[e * NfpNf + m3] = make_float4(a0, a1, a2, a3);
+      }
+    }
+  }
+  __syncthreads();
+  {
+    const int g3 = tid / NfpNf, m3 = tid - g3 * NfpNf;
+    if (g3 < L::NG3) {
+      const int f = m3 / Nfp, nn = m3 - f * Nfp;
+      for (int e = g3; e < nk; e += L::NG3) {
+        const float4 *t = sT + e * NfpNf + f * Nfp;
+        float gp = 0.f, gu = 0.f, gv = 0.f, gw = 0.f;
+        for (int j = 0; j < WL; ++j) {
+          const float v = L0T_vals[nn + j * Nfp];
+          const float4 x = t[L0T_ids[nn + j * Nfp]];
+          gp += v * x.x;
+          gu += v * x.y;
+          gv += v * x.z;
+          gw += v * x.w;
+        }
+        const float *F = sF + 16 * e + 4 * f;
+        H[(size_t)(k0 + e) * NfpNf + m3] = .5f * F[0] * (gp - (F[1] * gu + F[2] * gv + F[3] * gw));
+      }
+    }
+  }
+}
+
+// lQ += dt A^T z: volume transpose from (z_p, Zr, Zs, Zt) float4 per node, flux transpose from
+// own H and the neighbour's H through mapP
+template <int N>
+__global__ __launch_bounds__(256) void k_adj2(int K, const float *__restrict__ vgeo,
+                                              const float *__restrict__ fgeo,
+                                              const int4 *__restrict__ DTids,
+                                              const float4 *__restrict__ DTvals,
+                                              const int *__restrict__ FmaskInv,
+                                              const int *__restrict__ mapP,
+                                              const float *__restrict__ Z,
+                                              const float *__restrict__ H, float fdt,
+                                              float *__restrict__ lQ) {
+  using L = AdjLayout<N>;
+  constexpr int Np = L::Np, Nfp = L::Nfp, NfpNf = L::NfpNf, TB = L::TB, NpNf = 4 * Np;
+  extern __shared__ float lds[];
+  float4 *sZ = (float4 *)lds;              // [TB][Np]
+  float *sH = (float *)(sZ + TB * Np);     // [TB][NfpNf]
+  float *sG = sH + TB * NfpNf;             // [TB][9]
+  float *sF = sG + TB * 9;                 // [TB][16]
+  const int tid = threadIdx.x;
+  const int k0 = blockIdx.x * TB;
+  const int nk = min(TB, K - k0);
+  const size_t base = (size_t)k0 * NpNf;
+  for (int i = tid; i < nk * 9; i += 256) sG[i] = vgeo[(size_t)k0 * 9 + i];
+  for (int i = tid; i < nk * 16; i += 256) sF[i] = fgeo[(size_t)k0 * 16 + i];
+  for (int i = tid; i < nk * NfpNf; i += 256) sH[i] = H[(size_t)k0 * NfpNf + i];
+  __syncthreads();
+  const int gv = tid / L::GS, nv = tid - gv * L::GS;
+  const bool on = gv < L::NG && nv < Np;
+  if (on) {
+#pragma unroll
+    for (int i = 0; i < L::EPTV; ++i) {
+      const int e = gv + L::NG * i;
+      if (e < nk) {
+        const float *z = Z + base + (size_t)e * NpNf;
+        const float *G = sG + 9 * e;
+        const float u = z[Np + nv], v = z[2 * Np + nv], w = z[3 * Np + nv];
+        sZ[e * Np + nv] = make_float4(z[nv], G[0] * u + G[1] * v + G[2] * w,
+                                      G[3] * u + G[4] * v + G[5] * w, G[6] * u + G[7] * v + G[8] * w);
+      }
+    }
+  }
+  __syncthreads();
+  if (!on) return;
+  int4 di[4];
+  float4 dv[4];
+#pragma unroll
+  for (int m = 0; m < 4; ++m) {
+    di[m] = DTids[m * Np + nv];
+    dv[m] = DTvals[m * Np + nv];
+  }
+  int slot[3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) slot[a] = FmaskInv[3 * nv + a];
+#pragma unroll
+  for (int i = 0; i < L::EPTV; ++i) {
+    const int e = gv + L::NG * i;
+    if (e >= nk) break;
+    const int k = k0 + e;
+    const float4 *V = sZ + e * Np;
+    float dp[4], dz[4];
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+      const float4 a = V[di[m].x], b = V[di[m].y], c = V[di[m].z], d = V[di[m].w];
+      dp[m] = dv[m].x * a.x + dv[m].y * b.x + dv[m].z * c.x + dv[m].w * d.x;
+      if (m == 0)
+        dz[0] = dv[0].x * (a.y + a.z + a.w) + dv[0].y * (b.y + b.z + b.w) +
+                dv[0].z * (c.y + c.z + c.w) + dv[0].w * (d.y + d.z + d.w);
+      else if (m == 1)
+        dz[1] = dv[1].x * a.y + dv[1].y * b.y + dv[1].z * c.y + dv[1].w * d.y;
+      else if (m == 2)
+        dz[2] = dv[2].x * a.z + dv[2].y * b.z + dv[2].z * c.z + dv[2].w * d.z;
+      else
+        dz[3] = dv[3].x * a.w + dv[3].y * b.w + dv[3].z * c.w + dv[3].w * d.w;
+    }
+    const float *G = sG + 9 * e;
+    float op = -.5f * (dz[1] + dz[2] + dz[3] - dz[0]);
+    const float wr = .5f * (dp[1] - dp[0]), ws = .5f * (dp[2] - dp[0]), wt = .5f * (dp[3] - dp[0]);
+    float ou = -(G[0] * wr + G[3] * ws + G[6] * wt);
+    float ov = -(G[1] * wr + G[4] * ws + G[7] * wt);
+    float ow = -(G[2] * wr + G[5] * ws + G[8] * wt);
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      const int r = slot[a];
+      if (r < 0) continue;
+      const size_t own = (size_t)k * NfpNf + r;
+      const float h = sH[e * NfpNf + r];
+      const int mp = mapP[own];
+      const float *F = sF + 16 * e + 4 * (r / Nfp);
+      if ((size_t)mp == own) {
+        op -= 2.f * h;
+      } else {
+        const float hn = H[mp];
+        op += hn - h;
+        const float hs = h + hn;
+        ou += hs * F[1];
+        ov += hs * F[2];
+        ow += hs * F[3];
+      }
+    }
+    float *o = lQ + base + (size_t)e * NpNf + nv;
+    o[0] += fdt * op;
+    o[Np] += fdt * ou;
+    o[2 * Np] += fdt * ov;
+    o[3 * Np] += fdt * ow;
+  }
+}
+
+template <int N>
+static hipError_t adj_tiled_n(bbw_ctx *c, const float *Y, float fa, float fb, float fdt) {
+  using L = AdjLayout<N>;
+  const int grid = (c->K + L::TB - 1) / L::TB;
+  const size_t s1 = sizeof(float) * L::floats1, s2 = sizeof(float) * L::floats2;
+  if (s1 > 65536)
+    hipFuncSetAttribute((const void *)k_adj1<N>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)s1);
+  if (s2 > 65536)
+    hipFuncSetAttribute((const void *)k_adj2<N>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)s2);
+  hipLaunchKernelGGL(k_adj1<N>, dim3(grid), dim3(256), s1, c->stream, c->K, c->PqTPad, c->VqTPad,
+                     c->c2q, Y, c->fgeo, c->EELT_ids, c->EELT_vals, c->WE, c->L0T_ids, c->L0T_vals,
+                     c->WL, fa, fb, fdt, c->lQ, c->lres, c->Zadj, c->Hadj, c->grad);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_adj2<N>, dim3(grid), dim3(256), s2, c->stream, c->K, c->vgeo, c->fgeo,
+                     c->DTids, c->DTvals, c->FmaskInv, c->mapP, c->Zadj, c->Hadj, fdt, c->lQ);
+  return hipGetLastError();
+}
+
+hipError_t launch_adj_stage_tiled(bbw_ctx *c, const float *Y, float fa, float fb, float fdt) {
+  switch (c->N) {
+    case 1: return adj_tiled_n<1>(c, Y, fa, fb, fdt);
+    case 2: return adj_tiled_n<2>(c, Y, fa, fb, fdt);
+    case 3: return adj_tiled_n<3>(c, Y, fa, fb, fdt);
+    case 4: return adj_tiled_n<4>(c, Y, fa, fb, fdt);
+    case 5: return adj_tiled_n<5>(c, Y, fa, fb, fdt);
+    case 6: return adj_tiled_n<6>(c, Y, fa, fb, fdt);
+  }
+  return hipErrorInvalidValue;
+}
+
+}  // namespace bbw

@@ -50,6 +50,7 @@ struct bbw_ctx {
   int N = 0, Np = 0, Nfp = 0, Nq = 0, Nf = 0, K = 0, dev = 0;
   int KblkV = 0, KblkS = 0, KblkU = 0, KblkF = 0;
   int fused_impl = 1;  // BBW_FUSED_VECTOR or BBW_FUSED_MFMA
+  int adj_impl = 1;    // 0: per-element adjoint kernels, 1: tiled MFMA adjoint kernels
   hipStream_t own_stream = nullptr, stream = nullptr;
   std::string err;
 
@@ -61,6 +62,8 @@ struct bbw_ctx {
   // zero-padded copies for the MFMA kernels: VqPad [4*ceil(Np/4)][NQP] (j rows, q contiguous),
   // PqPad [NQP][NPP] (q rows, n contiguous); NQP, NPP = Nq, Np rounded up to 16
   float *VqPad = nullptr, *PqPad = nullptr;
+  // the same for the adjoint: PqTPad [4*ceil(Np/4)][NQP] = Pq^T, VqTPad [NQP][NPP] = Vq^T
+  float *PqTPad = nullptr, *VqTPad = nullptr;
   // EEL column ids as float4 slots and L0 ids with the face offset folded in, for the fused kernel
   int *EEL_ids4 = nullptr;
   double *Vq64 = nullptr, *wq64 = nullptr;
@@ -110,6 +113,10 @@ struct bbw_ctx {
   float *lQ = nullptr, *lres = nullptr, *Zadj = nullptr, *Hadj = nullptr, *grad = nullptr;
   float *Ybuf = nullptr, *ckpt = nullptr, *resid = nullptr, *Q0save = nullptr;
   size_t Ybuf_len = 0, ckpt_len = 0, resid_len = 0;
+  // Vq (A Q)_p of the final steps kept from the first forward pass (no recompute for them)
+  float *Ycache = nullptr;
+  size_t Ycache_len = 0;
+  int ycache_mb = 0;  // device memory for Ycache in MiB: 0 = automatic, < 0 = none
 
   // graph cache and per-window stage data
   bbw::Graph graph;
@@ -131,6 +138,7 @@ hipError_t launch_fused_mfma(bbw_ctx *c, const float *Qin, float *Qout, float fa
                              float fdt, const float *ftime_dev, float ftime, int project_velocity,
                              float *yout = nullptr);
 hipError_t launch_adj_stage(bbw_ctx *c, const float *Y, float fa, float fb, float fdt);
+hipError_t launch_adj_stage_tiled(bbw_ctx *c, const float *Y, float fa, float fb, float fdt);
 hipError_t launch_adj_inject(bbw_ctx *c, const float *resid_row);
 hipError_t launch_energy(bbw_ctx *c, double *E_dev_partial, int *nblocks);
 hipError_t launch_reduce(bbw_ctx *c, const double *partial, int n, double *out);

@@ -117,7 +117,7 @@ def main():
     ap.add_argument("--plain-steps", type=int, default=200)
     ap.add_argument("--cpu-sample", type=int, default=1 << 30)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--adjoint-steps", type=int, default=200)
+    ap.add_argument("--adjoint-steps", type=int, default=800)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported "

@@ -156,6 +156,11 @@ int bbw_adjoint_gradient(bbw_ctx *ctx, int nsteps, double dt, double t0, const f
 #define BBW_OPT_KBLK_S 2
 #define BBW_OPT_KBLK_U 3
 #define BBW_OPT_KBLK_F 4 /* elements per block of the vector fused kernel */
+/* adjoint: MiB of device memory that keep Vq (A Q)_p of the final steps from the first forward
+ * pass, so those steps are not recomputed (0 = automatic: up to 60% of free memory, < 0 = none;
+ * earlier steps are checkpointed and recomputed) */
+#define BBW_OPT_ADJ_CACHE_MB 5
+#define BBW_OPT_ADJ_IMPL 6 /* adjoint kernels: 0 per-element, 1 tiled MFMA (default) */
 #define BBW_FUSED_VECTOR 0
 #define BBW_FUSED_MFMA 1
 int bbw_set_option(bbw_ctx *ctx, int option, int value);

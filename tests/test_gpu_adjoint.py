@@ -5,7 +5,7 @@ directions; the torch.autograd.Function returns the same gradient."""
 import numpy as np
 import pytest
 
-from bbwadg_amd import media
+from bbwadg_amd import _lib, media
 from bbwadg_amd.adjoint import AcousticInversion, Shot, c2_misfit
 from bbwadg_amd.solver import WaveSolver
 import oracle as orc
@@ -54,12 +54,19 @@ def test_gradient_matches_fp64_host_adjoint():
 def test_checkpoint_interval_is_exact_and_fd_agrees():
     print("case adjoint checkpoints + FD", flush=True)
     P, s, inv, d, c2, shots = setup(nshots=1)
-    J1, g1 = inv.misfit_and_gradient(c2, d)
-    inv.ckpt_every = 24
+    J1, g1 = inv.misfit_and_gradient(c2, d)            # all steps cached (automatic budget)
+    s.set_option(_lib.OPT_ADJ_CACHE_MB, -1)             # no cache: checkpoint every 7, recompute
     J2, g2 = inv.misfit_and_gradient(c2, d)
-    # same scheme; only the atomic order of receiver injections and the rounding of segment
-    # start times differ
-    assert abs(J1 - J2) <= 1e-6 * J1 and rel_l2(g1, g2) < 1e-5
+    inv.ckpt_every = 24
+    J3, g3 = inv.misfit_and_gradient(c2, d)
+    s.set_option(_lib.OPT_ADJ_IMPL, 0)                  # per-element adjoint kernels
+    J4, g4 = inv.misfit_and_gradient(c2, d)
+    s.set_option(_lib.OPT_ADJ_IMPL, 1)
+    s.set_option(_lib.OPT_ADJ_CACHE_MB, 0)
+    # same scheme; only the atomic order of receiver injections, the rounding of segment
+    # start times and the summation order of the two kernel sets differ
+    for J, g in ((J2, g2), (J3, g3), (J4, g4)):
+        assert abs(J1 - J) <= 1e-6 * J1 and rel_l2(g, g1) < 1e-5
     assert abs(inv.misfit(c2, d) - J1) < 1e-6 * J1
     rng = np.random.default_rng(4)
     eps = 1e-3  # FP32 forward: larger step than the FP64 check, same central formula
