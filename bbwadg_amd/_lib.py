@@ -28,7 +28,9 @@ EXPORTS = [
     "bbw_set_res", "bbw_get_res", "bbw_set_rhs", "bbw_get_rhs", "bbw_volume", "bbw_surface",
     "bbw_update_wadg", "bbw_update", "bbw_stage_fused", "bbw_step", "bbw_energy",
     "bbw_device_ptr", "bbw_set_option", "bbw_upload_adjoint_maps", "bbw_misfit",
-    "bbw_adjoint_gradient",
+    "bbw_adjoint_gradient", "bbw_batch_create", "bbw_batch_destroy", "bbw_batch_last_error",
+    "bbw_batch_upload_operators", "bbw_batch_add_mesh", "bbw_batch_add_shot", "bbw_batch_run",
+    "bbw_batch_get_state", "bbw_batch_reset", "bbw_batch_last_ms",
 ]
 OPT_FUSED_IMPL, OPT_KBLK_V, OPT_KBLK_S, OPT_KBLK_U, OPT_KBLK_F, OPT_ADJ_CACHE_MB, OPT_ADJ_IMPL = 0, 1, 2, 3, 4, 5, 6
 FUSED_VECTOR, FUSED_MFMA = 0, 1
@@ -106,6 +108,16 @@ def load():
         "bbw_upload_adjoint_maps": ([P, P], I),
         "bbw_misfit": ([P, I, D, D, P, ctypes.POINTER(D)], I),
         "bbw_adjoint_gradient": ([P, I, D, D, P, I, P, ctypes.POINTER(D)], I),
+        "bbw_batch_create": ([I, I, ctypes.POINTER(P)], I),
+        "bbw_batch_destroy": ([P], None),
+        "bbw_batch_last_error": ([P], ctypes.c_char_p),
+        "bbw_batch_upload_operators": ([P, ctypes.POINTER(Ops)], I),
+        "bbw_batch_add_mesh": ([P, I, P, P, P, ctypes.POINTER(I)], I),
+        "bbw_batch_add_shot": ([P, I, P, P, I, P, P, P, D, D, D, I, ctypes.POINTER(I)], I),
+        "bbw_batch_run": ([P, I, I, I], I),
+        "bbw_batch_get_state": ([P, I, P], I),
+        "bbw_batch_reset": ([P], I),
+        "bbw_batch_last_ms": ([P], D),
     }
     for name, (args, res) in sig.items():
         fn = getattr(lib, name)

@@ -1,0 +1,427 @@
+// Per-tile device functions of the fused acoustic stage (included by bbw_fused.hip and
+// bbw_batch.hip).  See bbw_fused.hip for the phase structure.
+#pragma once
+
+#include "bbw_internal.h"
+
+namespace bbw {
+
+typedef float f4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ int src_slot2(const SrcArgs &s, int k) {
+  if (s.nsrc == 0 || k < s.first || k > s.last) return -1;
+  int lo = 0, hi = s.nsrc - 1;
+  while (lo <= hi) {
+    const int mid = (lo + hi) >> 1;
+    const int v = s.elems[mid];
+    if (v == k) return mid;
+    if (v < k) lo = mid + 1; else hi = mid - 1;
+  }
+  return -1;
+}
+
+template <int N, bool PROJ_U>
+struct FusedLayout {
+  static constexpr int Np = Dims<N>::Np, Nfp = Dims<N>::Nfp, NfpNf = Dims<N>::NfpNf;
+  static constexpr int Nq = Dims<N>::Nq;
+  static constexpr int TB = 16;                  // elements per tile = MFMA N
+  static constexpr int RS = 4 * Np + 3;          // odd rhs row stride; covers the K padding
+  static constexpr int NQT = (Nq + 15) / 16, NQP = NQT * 16;
+  static constexpr int NPT = (Np + 15) / 16, NPP = NPT * 16;
+  static constexpr int KTV = (Np + 3) / 4, KTP = NQP / 4;
+  static constexpr int NP = PROJ_U ? 4 : 1;
+  static constexpr int SX = NP * TB;             // S row length
+  static constexpr int cmax(int a, int b) { return a > b ? a : b; }
+  // W1: Q_in tile [TB][4Np] | lifted face data float4 [TB][NfpNf] | S [NQP][SX]
+  static constexpr int W1 = cmax(cmax(4 * Np, 4 * NfpNf), NQP * NP);
+  // W2: (p, Ur, Us, Ut) float4 [TB][Np] | flux float2 [TB][NfpNf]
+  static constexpr int W2 = cmax(4 * Np, 2 * NfpNf);
+  static constexpr int QTW = (NQT + 3) / 4;     // q tiles per wave
+  static constexpr int GS = ((Np + 63) / 64) * 64, NG = 256 / GS;        // node groups
+  static constexpr int NG3 = 256 / NfpNf, EPT3 = (TB + NG3 - 1) / NG3;   // face-node groups
+  static constexpr int EPTV = (TB + NG - 1) / NG;
+  static constexpr bool PF = 4 * Np <= 256;    // one state row per thread: prefetch res, Q_in
+  static constexpr size_t floats = (size_t)TB * (W1 + W2 + RS + 9 + 16) + TB;
+};
+
+// Operator, mesh and per-shot arguments of one tile.
+struct FusedOps {
+  const int *Fmask;
+  const int4 *Dids;
+  const float4 *Dvals;
+  const int *EEL_ids4;
+  const float *EEL_vals;
+  const int *L0_ids;
+  const float *L0_vals;
+  const float *VqPad;
+  const float *PqPad;
+};
+
+struct FusedMesh {
+  int K;
+  const float *vgeo, *fgeo;
+  const int *vmapP;
+};
+
+struct FusedShot {
+  const float *c2q;
+  const float *Qin;
+  float *Qout, *res;
+  float *yout;   // Vq (A Q)_p for the adjoint, or null
+  float ft;      // source time function at this stage
+  float fdt;     // time step
+};
+
+template <int N, bool PROJ_U>
+using FL = FusedLayout<N, PROJ_U>;
+
+// Geometry of a tile: vgeo/fgeo into LDS, the thread's vmapP entries and face-node id into
+// registers.  Shared by every shot processed on the same tile.
+template <int N, bool PROJ_U>
+__device__ __forceinline__ void tile_geometry(const FusedMesh &m, const FusedOps &o, int k0, int nk,
+                                              float *lds, int (&vm)[FL<N, PROJ_U>::EPT3],
+                                              int &fid3) {
+  using L = FL<N, PROJ_U>;
+  constexpr int NfpNf = L::NfpNf, TB = L::TB;
+  float *sG = lds + TB * (L::W1 + L::W2 + L::RS);
+  float *sF = sG + TB * 9;
+  const int tid = threadIdx.x;
+  for (int i = tid; i < nk * 9; i += 256) sG[i] = m.vgeo[(size_t)k0 * 9 + i];
+  for (int i = tid; i < nk * 16; i += 256) sF[i] = m.fgeo[(size_t)k0 * 16 + i];
+  const int g3 = tid / NfpNf, m3 = tid - g3 * NfpNf;
+  const bool on3 = g3 < L::NG3;
+  fid3 = on3 ? o.Fmask[m3] : 0;
+#pragma unroll
+  for (int i = 0; i < L::EPT3; ++i) {
+    const int e = g3 + L::NG3 * i;
+    vm[i] = (on3 && e < nk) ? m.vmapP[(size_t)(k0 + e) * NfpNf + m3] : -1;
+  }
+}
+
+// One fused LSRK stage of one shot on one tile (phases 1-9 above).  Ends with a barrier so
+// the caller may process another shot on the same tile.
+template <int N, bool PROJ_U>
+__device__ __forceinline__ void tile_shot(const FusedMesh &m, const FusedOps &o, const FusedShot &sh,
+                                          const SrcArgs &src, float fa, float fb, int k0, int nk,
+                                          float *lds, const int (&vm)[FL<N, PROJ_U>::EPT3],
+                                          int fid3) {
+  using L = FusedLayout<N, PROJ_U>;
+  constexpr int Np = L::Np, Nfp = L::Nfp, NfpNf = L::NfpNf, Nq = L::Nq, TB = L::TB, RS = L::RS;
+  constexpr int Nf = 4, NpNf = 4 * Np, EEL_nnz = Dims<N>::EEL_nnz, L0_nnz = Dims<N>::L0_nnz;
+  constexpr int NP = L::NP, SX = L::SX, NQP = L::NQP, NPP = L::NPP;
+  const float *__restrict__ Qin = sh.Qin;
+  float *__restrict__ Qout = sh.Qout;
+  float *__restrict__ res = sh.res;
+  float *__restrict__ yout = sh.yout;
+  const float *__restrict__ c2q = sh.c2q;
+  const float fdt = sh.fdt;
+  const int4 *__restrict__ Dids = o.Dids;
+  const float4 *__restrict__ Dvals = o.Dvals;
+  const int *__restrict__ EEL_ids4 = o.EEL_ids4;
+  const float *__restrict__ EEL_vals = o.EEL_vals;
+  const int *__restrict__ L0_ids = o.L0_ids;
+  const float *__restrict__ L0_vals = o.L0_vals;
+  const float *__restrict__ VqPad = o.VqPad;
+  const float *__restrict__ PqPad = o.PqPad;
+  float *W1 = lds;                       // [TB * W1]
+  float *W2 = W1 + TB * L::W1;           // [TB * W2]
+  float *sR = W2 + TB * L::W2;           // [TB][RS]
+  float *sG = sR + TB * RS;              // [TB][9]
+  float *sF = sG + TB * 9;               // [TB][16]
+  int *sSlot = (int *)(sF + TB * 16);
+  float *sQ = W1;                        // phases 1-4
+  float4 *sL = (float4 *)W1;             // phases 5-6
+  float *S = W1;                         // phases 7-8
+  float4 *sV = (float4 *)W2;             // phases 2-3
+  float2 *sFl = (float2 *)W2;            // phases 4-5
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const size_t base = (size_t)k0 * NpNf;
+  // face-node mapping (phases 2, 4, 5)
+  const int g3 = tid / NfpNf, m3 = tid - g3 * NfpNf;
+  const bool on3 = g3 < L::NG3;
+  const int f3 = m3 / Nfp;
+  // node mapping (phases 2, 3, 6)
+  const int gv = tid / L::GS, nv = tid - gv * L::GS;
+  const bool onv = gv < L::NG && nv < Np;
+
+  // ---- 1: tile loads and register prefetch
+  {
+    const float4 *Qt4 = (const float4 *)(Qin + base);
+    float4 *sQ4 = (float4 *)sQ;
+    for (int i = tid; i < nk * Np; i += 256) sQ4[i] = Qt4[i];
+    for (int i = tid; i < TB * RS; i += 256) sR[i] = 0.f;
+    if (tid < TB) sSlot[tid] = tid < nk ? src_slot2(src, k0 + tid) : -1;
+  }
+  float qreg[L::PF ? TB : 1], rreg[L::PF ? TB : 1];
+  if constexpr (L::PF) {
+#pragma unroll
+    for (int e = 0; e < TB; ++e) {
+      const bool ok = tid < NpNf && e < nk;
+      qreg[e] = ok ? Qin[base + e * NpNf + tid] : 0.f;
+      rreg[e] = ok ? res[base + e * NpNf + tid] : 0.f;
+    }
+  }
+  float c2r[L::QTW][4];
+#pragma unroll
+  for (int t = 0; t < L::QTW; ++t) {
+    const int qt = wave + 4 * t;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int q = qt * 16 + 4 * (lane >> 4) + r, e = lane & 15;
+      c2r[t][r] = (qt < L::NQT && q < Nq && e < nk) ? c2q[(size_t)(k0 + e) * Nq + q] : 0.f;
+    }
+  }
+  __syncthreads();
+
+  // ---- 2: (p, Ur, Us, Ut) per node; neighbour traces into registers
+  if (onv) {
+#pragma unroll
+    for (int i = 0; i < L::EPTV; ++i) {
+      const int e = gv + L::NG * i;
+      if (e < nk) {
+        const float *s = sQ + e * NpNf;
+        const float *G = sG + 9 * e;
+        const float uu = s[Np + nv], vv = s[2 * Np + nv], ww = s[3 * Np + nv];
+        sV[e * Np + nv] = make_float4(s[nv], uu * G[0] + vv * G[1] + ww * G[2],
+                                      uu * G[3] + vv * G[4] + ww * G[5],
+                                      uu * G[6] + vv * G[7] + ww * G[8]);
+      }
+    }
+  }
+  float tr[L::EPT3][4];
+#pragma unroll
+  for (int i = 0; i < L::EPT3; ++i) {
+    const int e = g3 + L::NG3 * i;
+    const int idM = fid3 + (k0 + e) * NpNf;
+    const bool inter = vm[i] >= 0 && vm[i] != idM;
+#pragma unroll
+    for (int f = 0; f < 4; ++f) tr[i][f] = inter ? Qin[vm[i] + f * Np] : 0.f;
+  }
+  __syncthreads();
+
+  // ---- 3: volume terms
+  if (onv) {
+    const float4 dv = Dvals[nv];
+    const int4 D1 = Dids[nv], D2 = Dids[Np + nv], D3 = Dids[2 * Np + nv], D4 = Dids[3 * Np + nv];
+#pragma unroll
+    for (int i = 0; i < L::EPTV; ++i) {
+      const int e = gv + L::NG * i;
+      if (e < nk) {
+        const float4 *V = sV + e * Np;
+        const float4 a1 = V[D1.x], a2 = V[D2.x], a3 = V[D3.x], a4 = V[D4.x];
+        const float4 b1 = V[D1.y], b2 = V[D2.y], b3 = V[D3.y], b4 = V[D4.y];
+        const float4 c1 = V[D1.z], c2 = V[D2.z], c3 = V[D3.z], c4 = V[D4.z];
+        const float4 d1 = V[D1.w], d2 = V[D2.w], d3 = V[D3.w], d4 = V[D4.w];
+        const float p1 = dv.x * a1.x + dv.y * a2.x + dv.z * a3.x + dv.w * a4.x;
+        const float p2 = dv.x * b1.x + dv.y * b2.x + dv.z * b3.x + dv.w * b4.x;
+        const float p3 = dv.x * c1.x + dv.y * c2.x + dv.z * c3.x + dv.w * c4.x;
+        const float p4 = dv.x * d1.x + dv.y * d2.x + dv.z * d3.x + dv.w * d4.x;
+        float dU1 = dv.x * (b1.y + c1.z + d1.w);
+        dU1 += dv.y * (b2.y + c2.z + d2.w);
+        dU1 += dv.z * (b3.y + c3.z + d3.w);
+        dU1 += dv.w * (b4.y + c4.z + d4.w);
+        float dU2 = dv.x * (a1.y + a1.z + a1.w);
+        dU2 += dv.y * (a2.y + a2.z + a2.w);
+        dU2 += dv.z * (a3.y + a3.z + a3.w);
+        dU2 += dv.w * (a4.y + a4.z + a4.w);
+        const float dpdr = .5f * (p2 - p1), dpds = .5f * (p3 - p1), dpdt = .5f * (p4 - p1);
+        const float *G = sG + 9 * e;
+        float *r = sR + e * RS;
+        r[nv] = -.5f * (dU1 - dU2);
+        r[Np + nv] = -(G[0] * dpdr + G[3] * dpds + G[6] * dpdt);
+        r[2 * Np + nv] = -(G[1] * dpdr + G[4] * dpds + G[7] * dpdt);
+        r[3 * Np + nv] = -(G[2] * dpdr + G[5] * dpds + G[8] * dpdt);
+      }
+    }
+  }
+  __syncthreads();
+
+  // ---- 4: upwind flux (own trace from the Q_in tile, neighbour trace from registers)
+  if (on3) {
+#pragma unroll
+    for (int i = 0; i < L::EPT3; ++i) {
+      const int e = g3 + L::NG3 * i;
+      if (e < nk) {
+        const float *s = sQ + e * NpNf;
+        const float *F = sF + 16 * e + 4 * f3;
+        const float pM = s[fid3], uM = s[Np + fid3], vM = s[2 * Np + fid3], wM = s[3 * Np + fid3];
+        const int idM = fid3 + (k0 + e) * NpNf;
+        float pjump, Unjump;
+        if (vm[i] == idM) {
+          pjump = -2.f * pM;
+          Unjump = 0.f;
+        } else {
+          pjump = tr[i][0] - pM;
+          Unjump = (tr[i][1] - uM) * F[1] + (tr[i][2] - vM) * F[2] + (tr[i][3] - wM) * F[3];
+        }
+        sFl[e * NfpNf + m3] = make_float2(.5f * (pjump - Unjump) * F[0], .5f * (Unjump - pjump) * F[0]);
+      }
+    }
+  }
+  __syncthreads();
+
+  // ---- 5: L0 per face; store (p, U nx, U ny, U nz) so the lift needs no face lookup
+  if (on3) {
+    const int nt = m3 - f3 * Nfp;
+    float Lv[L0_nnz];
+    int Li[L0_nnz];
+#pragma unroll
+    for (int j = 0; j < L0_nnz; ++j) {
+      Lv[j] = L0_vals[nt + j * Nfp];
+      Li[j] = L0_ids[nt + j * Nfp] + f3 * Nfp;
+    }
+#pragma unroll
+    for (int i = 0; i < L::EPT3; ++i) {
+      const int e = g3 + L::NG3 * i;
+      if (e < nk) {
+        const float2 *X = sFl + e * NfpNf;
+        float v1 = 0.f, v2 = 0.f;
+#pragma unroll
+        for (int j = 0; j < L0_nnz; ++j) {
+          const float2 x = X[Li[j]];
+          v1 += Lv[j] * x.x;
+          v2 += Lv[j] * x.y;
+        }
+        const float *F = sF + 16 * e + 4 * f3;
+        sL[e * NfpNf + m3] = make_float4(v1, v2 * F[1], v2 * F[2], v2 * F[3]);
+      }
+    }
+  }
+  __syncthreads();
+
+  // ---- 6: EEL lift into rhs
+  if (onv) {
+    float Ev[EEL_nnz];
+    int Ei[EEL_nnz];
+#pragma unroll
+    for (int j = 0; j < EEL_nnz; ++j) {
+      Ev[j] = EEL_vals[nv + j * Np];
+      Ei[j] = EEL_ids4[nv + j * Np];
+    }
+#pragma unroll
+    for (int i = 0; i < L::EPTV; ++i) {
+      const int e = gv + L::NG * i;
+      if (e < nk) {
+        const float *Lt = (const float *)(sL + e * NfpNf);
+        float v1 = 0.f, v2 = 0.f, v3 = 0.f, v4 = 0.f;
+#pragma unroll
+        for (int j = 0; j < EEL_nnz; ++j) {
+          const float4 x = *(const float4 *)(Lt + Ei[j]);
+          v1 += Ev[j] * x.x;
+          v2 += Ev[j] * x.y;
+          v3 += Ev[j] * x.z;
+          v4 += Ev[j] * x.w;
+        }
+        float *r = sR + e * RS;
+        r[nv] += v1;
+        r[Np + nv] += v2;
+        r[2 * Np + nv] += v3;
+        r[3 * Np + nv] += v4;
+      }
+    }
+  }
+  __syncthreads();
+
+  // ---- 7: S = (Vq . rhs) x c2 on MFMA (A from the zero-padded Vq, fixed offsets)
+  {
+    const int col = lane & 15, kr = lane >> 4;
+    const float *Bp = sR + col * RS + kr;
+#pragma unroll
+    for (int t = 0; t < L::QTW; ++t) {
+      const int qt = wave + 4 * t;
+      if (qt >= L::NQT) break;
+      f4 acc[NP];
+#pragma unroll
+      for (int b = 0; b < NP; ++b) acc[b] = f4{0.f, 0.f, 0.f, 0.f};
+      const float *Ap = VqPad + kr * NQP + qt * 16 + col;
+#pragma unroll
+      for (int kt = 0; kt < L::KTV; ++kt) {
+        const float a = Ap[kt * 4 * NQP];
+#pragma unroll
+        for (int b = 0; b < NP; ++b)
+          acc[b] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, Bp[b * Np + kt * 4], acc[b], 0, 0, 0);
+      }
+      if (yout && col < nk) {
+        // Vq . rhs_p at the cubature points, kept for the adjoint's c2 gradient
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int q = qt * 16 + 4 * kr + r;
+          if (q < Nq) yout[(size_t)(k0 + col) * Nq + q] = acc[0][r];
+        }
+      }
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int q = qt * 16 + 4 * kr + r;
+        S[q * SX + col] = acc[0][r] * c2r[t][r];
+#pragma unroll
+        for (int b = 1; b < NP; ++b) S[q * SX + b * 16 + col] = q < Nq ? acc[b][r] : 0.f;
+      }
+    }
+  }
+  __syncthreads();
+
+  // ---- 8: rhs = Pq . S on MFMA
+  {
+    const int col = lane & 15, kr = lane >> 4;
+    for (int mt = wave; mt < L::NPT; mt += 4) {
+      f4 acc[NP];
+#pragma unroll
+      for (int b = 0; b < NP; ++b) acc[b] = f4{0.f, 0.f, 0.f, 0.f};
+      const float *Ap = PqPad + kr * NPP + mt * 16 + col;
+      const float *Bp = S + kr * SX + col;
+#pragma unroll 8
+      for (int kt = 0; kt < L::KTP; ++kt) {
+        const float a = Ap[kt * 4 * NPP];
+#pragma unroll
+        for (int b = 0; b < NP; ++b)
+          acc[b] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, Bp[kt * 4 * SX + b * 16], acc[b], 0, 0, 0);
+      }
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int n = mt * 16 + 4 * kr + r;
+        if (n < Np) {
+#pragma unroll
+          for (int b = 0; b < NP; ++b) sR[col * RS + b * Np + n] = acc[b][r];
+        }
+      }
+    }
+  }
+  __syncthreads();
+
+  // ---- 9: LSRK stage: res = fa res + fdt rhs (+ source); Q_out = Q_in + fb res
+  const float ft = sh.ft;
+  if constexpr (L::PF) {
+    if (tid < NpNf) {
+      const int fr = tid / Np, nr = tid - fr * Np;
+#pragma unroll
+      for (int e = 0; e < TB; ++e) {
+        if (e < nk) {
+          float v = sR[e * RS + tid];
+          const int slot = sSlot[e];
+          if (slot >= 0) v += src.w[fr] * ft * src.coef[(size_t)slot * Np + nr];
+          const float x = fa * rreg[e] + fdt * v;
+          res[base + e * NpNf + tid] = x;
+          Qout[base + e * NpNf + tid] = qreg[e] + fb * x;
+        }
+      }
+    }
+  } else {
+    for (int rr = tid; rr < NpNf; rr += 256) {
+      const int fr = rr / Np, nr = rr - fr * Np;
+      for (int e = 0; e < nk; ++e) {
+        float v = sR[e * RS + rr];
+        const int slot = sSlot[e];
+        if (slot >= 0) v += src.w[fr] * ft * src.coef[(size_t)slot * Np + nr];
+        const size_t g = base + e * NpNf + rr;
+        const float x = fa * res[g] + fdt * v;
+        res[g] = x;
+        Qout[g] = Qin[g] + fb * x;
+      }
+    }
+  }
+  __syncthreads();
+}
+
+FusedOps fused_ops(const bbw_ctx *c);
+
+}  // namespace bbw

@@ -148,6 +148,27 @@ int bbw_misfit(bbw_ctx *ctx, int nsteps, double dt, double t0, const float *d_ob
 int bbw_adjoint_gradient(bbw_ctx *ctx, int nsteps, double dt, double t0, const float *d_obs,
                          int ckpt_every, float *grad, double *J);
 
+/* Batched multi-shot acoustic runs (no reference counterpart): many shots on several meshes,
+ * uneven durations, one persistent launch per LSRK stage for all active shots, HIP-graph
+ * replay.  Each shot: its mesh, c2q [K][Nq], initial state [K][4][Np], a source as in
+ * bbw_set_source, time step dt and step count.  bbw_batch_run advances every shot to its own
+ * step count; shots_per_tile shots of one mesh share a tile's geometry; blocks_per_cu sets the
+ * persistent grid.  bbw_batch_reset restores every shot's initial state. */
+typedef struct bbw_batch bbw_batch;
+int bbw_batch_create(int N, int device, bbw_batch **out);
+void bbw_batch_destroy(bbw_batch *b);
+const char *bbw_batch_last_error(const bbw_batch *b);
+int bbw_batch_upload_operators(bbw_batch *b, const bbw_ops *ops);
+int bbw_batch_add_mesh(bbw_batch *b, int K, const float *vgeo, const float *fgeo,
+                       const int32_t *vmapP, int *mesh_id);
+int bbw_batch_add_shot(bbw_batch *b, int mesh_id, const float *c2q, const float *Q0, int nsrc,
+                       const int32_t *src_elems, const float *src_coef, const float *weights,
+                       double f0, double amp, double dt, int nsteps, int *shot_id);
+int bbw_batch_run(bbw_batch *b, int shots_per_tile, int blocks_per_cu, int use_graph);
+int bbw_batch_get_state(bbw_batch *b, int shot_id, float *Q);
+int bbw_batch_reset(bbw_batch *b);
+double bbw_batch_last_ms(const bbw_batch *b); /* device time of the last bbw_batch_run, ms */
+
 /* Tuning options (no reference counterpart beyond the Kblk* compile-time defines of
  * WaveInitOCCA3d, cpp/src/WaveOKL3d.cpp:1631-1635): fused kernel implementation and
  * elements per block; used by the autotuner. */

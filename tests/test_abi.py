@@ -13,7 +13,7 @@ def declared_symbols():
     for fn in os.listdir(os.path.join(ROOT, "include")):
         if fn.endswith(".h"):
             text = open(os.path.join(ROOT, "include", fn)).read()
-            names |= set(re.findall(r"^\s*(?:int|void|const char \*)\s*\*?(bbw_\w+)\s*\(", text, re.M))
+            names |= set(re.findall(r"^\s*(?:int|void|double|const char \*)\s*\*?(bbw_\w+)\s*\(", text, re.M))
     return names
 
 
