@@ -30,7 +30,7 @@ EXPORTS = [
     "bbw_device_ptr", "bbw_set_option", "bbw_upload_adjoint_maps", "bbw_misfit",
     "bbw_adjoint_gradient", "bbw_batch_create", "bbw_batch_destroy", "bbw_batch_last_error",
     "bbw_batch_upload_operators", "bbw_batch_add_mesh", "bbw_batch_add_shot", "bbw_batch_run",
-    "bbw_batch_get_state", "bbw_batch_reset", "bbw_batch_last_ms",
+    "bbw_batch_get_state", "bbw_batch_reset", "bbw_batch_last_ms", "bbw_batch_set_step_cap",
 ]
 OPT_FUSED_IMPL, OPT_KBLK_V, OPT_KBLK_S, OPT_KBLK_U, OPT_KBLK_F, OPT_ADJ_CACHE_MB, OPT_ADJ_IMPL = 0, 1, 2, 3, 4, 5, 6
 FUSED_VECTOR, FUSED_MFMA = 0, 1
@@ -118,6 +118,7 @@ def load():
         "bbw_batch_get_state": ([P, I, P], I),
         "bbw_batch_reset": ([P], I),
         "bbw_batch_last_ms": ([P], D),
+        "bbw_batch_set_step_cap": ([P, I], I),
     }
     for name, (args, res) in sig.items():
         fn = getattr(lib, name)

@@ -90,6 +90,10 @@ class ShotBatch:
         self._check(self.lib.bbw_batch_run(self.b, int(shots_per_tile), int(blocks_per_cu), int(bool(graph))))
         return self.lib.bbw_batch_last_ms(self.b)
 
+    def set_step_cap(self, cap):
+        """Run every shot at most ``cap`` steps (0: full durations), e.g. for tuning trials."""
+        self._check(self.lib.bbw_batch_set_step_cap(self.b, int(cap)))
+
     def reset(self):
         self._check(self.lib.bbw_batch_reset(self.b))
 
@@ -116,10 +120,24 @@ class ShotBatch:
             pass
 
 
+def _c2_evaluator(m, ops):
+    """Cubature points of a mesh and an evaluator of seeded Fourier c^2 there (FP64; on the
+    GPU through torch when a device is present, which keeps 64-shot setups short)."""
+    cub = media.cubature_points(m, ops)
+    try:
+        import torch
+        if torch.cuda.is_available():
+            tx, ty, tz = (torch.as_tensor(c, dtype=torch.float64, device="cuda") for c in cub)
+            return cub, lambda seed: media.c2_fourier(seed)(tx, ty, tz).to(torch.float32).cpu().numpy()
+    except ImportError:
+        pass
+    return cub, lambda seed: media.c2_fourier(seed)(*cub)
+
+
 def c4_workload(seed=0, nshots=64, mesh_sizes=(16, 24, 32, 40), durations=(300, 1200), N=5,
-                step_scale=1.0, ops=None):
+                step_scale=1.0, ops=None, log=None):
     """The batched configuration; returns (batch, description).  ``step_scale`` < 1 shortens
-    every duration by that factor (the description states it)."""
+    every duration by that factor (the description states it).  ``log(msg)`` reports progress."""
     rng = np.random.default_rng(seed)
     ops = ops if ops is not None else operators.build_ref_ops(N)
     b = ShotBatch(N, ops=ops)
@@ -128,18 +146,22 @@ def c4_workload(seed=0, nshots=64, mesh_sizes=(16, 24, 32, 40), durations=(300, 
         m = meshmod.cube_mesh(n)
         meshmod.face_pair(m)
         mid = b.add_mesh(m)
-        meshes.append((mid, m, n))
+        cub, c2eval = _c2_evaluator(m, ops)
+        meshes.append((mid, m, n, cub, c2eval))
+        if log:
+            log(f"batch mesh {n}^3 x 6 = {m.K} tets ready")
     per_mesh = nshots // len(mesh_sizes)
     total_steps = 0
     for i in range(nshots):
-        mid, m, n = meshes[min(i // per_mesh, len(meshes) - 1)]
-        xq, yq, zq = media.cubature_points(m, ops)
-        c2q = media.c2_fourier(int(rng.integers(1 << 30)))(xq, yq, zq)
+        mid, m, n, cub, c2eval = meshes[min(i // per_mesh, len(meshes) - 1)]
+        c2q = c2eval(int(rng.integers(1 << 30)))
         x0 = rng.uniform(-0.3, 0.3, 3)
         f0 = float(rng.uniform(5.0, 15.0))
         nsteps = max(1, int(round(rng.integers(durations[0], durations[1] + 1) * step_scale)))
         total_steps += nsteps
-        src = media.gaussian_source(m, ops, x0, a=100.0)
+        if log and i % 8 == 7:
+            log(f"batch shot {i + 1}/{nshots}")
+        src = media.gaussian_source(m, ops, x0, a=100.0, cub=cub)
         dt = 0.5 / n / N ** 2
         Q0 = np.zeros((m.K, 4, ops.Np), dtype=np.float32)
         b.add_shot(mid, c2q, Q0, src=src, f0=f0, amp=1.0, dt=dt, nsteps=nsteps)

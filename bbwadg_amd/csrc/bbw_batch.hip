@@ -88,6 +88,8 @@ struct bbw_batch {
   };
   std::vector<Shot> shots;
   double last_ms = 0;
+  int step_cap = 0;  // > 0: every shot runs min(nsteps, step_cap) steps (tuning trials)
+  std::vector<int> last_steps;  // steps each shot ran in the last run
 };
 
 namespace {
@@ -271,13 +273,17 @@ int bbw_batch_run(bbw_batch *b, int shots_per_tile, int blocks_per_cu, int use_g
   if (hipGetDeviceProperties(&prop, c->dev) != hipSuccess) return bfail(b, BBW_ERR_HIP, "device query");
   const int max_grid = prop.multiProcessorCount * blocks_per_cu;
 
+  std::vector<int> nst(ns);
+  for (int i = 0; i < ns; ++i)
+    nst[i] = b->step_cap > 0 ? std::min(b->shots[i].nsteps, b->step_cap) : b->shots[i].nsteps;
+  b->last_steps = nst;
   // shots by duration (longest first): the active set at any step is a prefix
   std::vector<int> order(ns);
   for (int i = 0; i < ns; ++i) order[i] = i;
-  std::stable_sort(order.begin(), order.end(), [&](int a, int d) { return b->shots[a].nsteps > b->shots[d].nsteps; });
+  std::stable_sort(order.begin(), order.end(), [&](int a, int d) { return nst[a] > nst[d]; });
   std::vector<int> bounds;  // distinct durations ascending, with 0
   bounds.push_back(0);
-  for (auto &s : b->shots) bounds.push_back(s.nsteps);
+  for (int v : nst) bounds.push_back(v);
   std::sort(bounds.begin(), bounds.end());
   bounds.erase(std::unique(bounds.begin(), bounds.end()), bounds.end());
 
@@ -313,7 +319,7 @@ int bbw_batch_run(bbw_batch *b, int shots_per_tile, int blocks_per_cu, int use_g
     for (int m = 0; m < (int)b->meshes.size(); ++m) {
       std::vector<int> act;
       for (int i : order)
-        if (b->shots[i].nsteps > s0 && b->shots[i].mesh == m) act.push_back(i);
+        if (nst[i] > s0 && b->shots[i].mesh == m) act.push_back(i);
       for (size_t g = 0; g < act.size(); g += S) {
         const int l0 = (int)list.size();
         const int cnt = (int)std::min<size_t>(S, act.size() - g);
@@ -425,8 +431,9 @@ int bbw_batch_get_state(bbw_batch *b, int shot_id, float *Q) {
   if (!b || !Q || shot_id < 0 || shot_id >= (int)b->shots.size()) return BBW_ERR_ARG;
   auto &s = b->shots[shot_id];
   const size_t n = (size_t)b->meshes[s.mesh].K * 4 * b->ops->Np;
-  // five stage flips per step: after nsteps steps the state is in buffer nsteps % 2
-  if (hipMemcpyAsync(Q, s.Q[s.nsteps & 1], n * sizeof(float), hipMemcpyDeviceToHost, b->ops->stream) != hipSuccess ||
+  // five stage flips per step: after n steps the state is in buffer n % 2
+  const int ran = (size_t)shot_id < b->last_steps.size() ? b->last_steps[shot_id] : 0;
+  if (hipMemcpyAsync(Q, s.Q[ran & 1], n * sizeof(float), hipMemcpyDeviceToHost, b->ops->stream) != hipSuccess ||
       hipStreamSynchronize(b->ops->stream) != hipSuccess)
     return bfail(b, BBW_ERR_HIP, "state copy failed");
   return BBW_OK;
@@ -434,8 +441,15 @@ int bbw_batch_get_state(bbw_batch *b, int shot_id, float *Q) {
 
 double bbw_batch_last_ms(const bbw_batch *b) { return b ? b->last_ms : -1.0; }
 
+int bbw_batch_set_step_cap(bbw_batch *b, int cap) {
+  if (!b || cap < 0) return BBW_ERR_ARG;
+  b->step_cap = cap;
+  return BBW_OK;
+}
+
 int bbw_batch_reset(bbw_batch *b) {
   if (!b) return BBW_ERR_ARG;
+  b->last_steps.assign(b->shots.size(), 0);
   for (auto &s : b->shots) {
     const size_t n = (size_t)b->meshes[s.mesh].K * 4 * b->ops->Np;
     if (hipMemcpyAsync(s.Q[0], s.Q0, n * sizeof(float), hipMemcpyDeviceToDevice, b->ops->stream) != hipSuccess ||

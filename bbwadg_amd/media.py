@@ -25,6 +25,12 @@ def c2_fourier(seed, nterms=8, amp=0.1):
     phi = rng.uniform(0.0, 2.0 * np.pi, nterms)
 
     def c2(x, y, z):
+        if not isinstance(x, np.ndarray) and hasattr(x, "sin"):  # torch tensors (same FP64 formula)
+            acc = 0.0 * x
+            for k in range(nterms):
+                acc = acc + float(a[k]) * (2 * np.pi * (float(w[k, 0]) * x + float(w[k, 1]) * y
+                                                        + float(w[k, 2]) * z) + float(phi[k])).sin()
+            return 1.0 + 0.5 * acc
         x, y, z = (np.asarray(v, dtype=np.float64) for v in (x, y, z))
         acc = np.zeros(np.broadcast(x, y, z).shape)
         for k in range(nterms):
@@ -71,15 +77,33 @@ def cubature_points(m, ops):
     return xq.T.copy(), yq.T.copy(), zq.T.copy()
 
 
-def gaussian_source(m, ops, x0, a=100.0, rel_tol=1e-10):
-    """(elems ascending int32, coef (nsrc, Np)) of the projected smoothed source."""
-    xq, yq, zq = cubature_points(m, ops)
+def gaussian_source(m, ops, x0, a=100.0, rel_tol=1e-10, cub=None):
+    """(elems ascending int32, coef (nsrc, Np)) of the projected smoothed source.
+
+    Only elements within exp(-a^2 r^2) > rel_tol of x0 (plus their size) are evaluated;
+    ``cub`` may pass precomputed cubature points."""
+    GX, GY, GZ = m.GX, m.GY, m.GZ
+    cx, cy, cz = GX.mean(axis=1), GY.mean(axis=1), GZ.mean(axis=1)
+    size = np.sqrt(((GX - cx[:, None]) ** 2 + (GY - cy[:, None]) ** 2 + (GZ - cz[:, None]) ** 2).max(axis=1))
+    reach = np.sqrt(-np.log(rel_tol * 1e-3)) / a
+    near = np.nonzero(np.sqrt((cx - x0[0]) ** 2 + (cy - x0[1]) ** 2 + (cz - x0[2]) ** 2) < reach + size)[0]
+    if cub is None:
+        xq, yq, zq = (v.T for v in meshmod.affine_points(_Sub(m, near), ops.rq, ops.sq, ops.tq))
+    else:
+        xq, yq, zq = (c[near] for c in cub)
     r2 = (xq - x0[0]) ** 2 + (yq - x0[1]) ** 2 + (zq - x0[2]) ** 2
     fq = np.exp(-a * a * r2)
     fq /= np.abs(fq).max()
     keep = np.nonzero(fq.max(axis=1) > rel_tol)[0]
     coef = fq[keep] @ ops.Pq.T  # (nsrc, Np)
-    return keep.astype(np.int32), coef
+    return near[keep].astype(np.int32), coef
+
+
+class _Sub:
+    """Element subset view with the vertex arrays a mesh exposes."""
+
+    def __init__(self, m, idx):
+        self.GX, self.GY, self.GZ = m.GX[idx], m.GY[idx], m.GZ[idx]
 
 
 def locate(m, pts):

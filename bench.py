@@ -107,6 +107,54 @@ def adjoint_ratio(nsteps, mesh_n=32, N=5, ckpt=50):
             "misfit_agree": abs(Jf - Jg) / max(Jf, 1e-300)}
 
 
+def batched_throughput(step_scale=1.0, tune_steps=20):
+    """Batched configuration (64 shots on 16^3..40^3 x 6 meshes, N = 5, durations 300-1200
+    steps): one persistent launch per stage with the autotuned shots-per-tile and
+    blocks-per-CU, against the single-shot plain (three launches per stage) and fused paths
+    run shot after shot, whose times are the per-mesh single-shot rates applied to every
+    shot's DOF-stages."""
+    from bbwadg_amd import autotune, operators as O
+    from bbwadg_amd.batch import c4_workload
+    ops = O.build_ref_ops(5)
+    log = lambda msg: print(msg, file=sys.stderr, flush=True)
+    b, desc = c4_workload(seed=7, step_scale=step_scale, ops=ops, log=log)
+    best, trials = autotune.tune_batch(b, step_cap=tune_steps)
+    log(f"batch autotune: {best} from {trials}")
+    ms = b.run(shots_per_tile=best[0], blocks_per_cu=best[1])
+    log(f"batch run {ms:.1f} ms")
+    total = b.dof_stages()
+    rate = total / (ms / 1e3)
+    # single-shot rates per mesh (plain and fused), one representative shot per mesh
+    from bbwadg_amd.solver import WaveSolver, MODE_FUSED, MODE_PLAIN
+    import torch
+    per_mesh = {}
+    for mid, (m, dm) in enumerate(b.meshes):
+        s = WaveSolver(m, 5, ops=ops, dmesh=dm)
+        s.InitWADG_subelem(1.0)
+        s.WaveSetData3d(np.zeros((m.K, 4, ops.Np), dtype=np.float32))
+        n = round((m.K / 6) ** (1 / 3))
+        dt = 0.5 / n / 25
+        r = {}
+        for name, mode, nst in (("plain", MODE_PLAIN, 20), ("fused", MODE_FUSED, 40)):
+            s.step(2, dt, mode=mode)
+            s.sync()
+            t0 = time.perf_counter()
+            s.step(nst, dt, mode=mode)
+            s.sync()
+            r[name] = m.K * ops.Np * 4 * 5 * nst / (time.perf_counter() - t0)
+        per_mesh[mid] = r
+        s.close()
+    t_plain = sum(b.meshes[mid][0].K * ops.Np * 20 * nst / per_mesh[mid]["plain"] for mid, nst in b.shots)
+    t_fused = sum(b.meshes[mid][0].K * ops.Np * 20 * nst / per_mesh[mid]["fused"] for mid, nst in b.shots)
+    b.close()
+    return {"workload": desc, "value": rate, "ms": ms, "dof_stages": total,
+            "shots_per_tile": best[0], "blocks_per_cu": best[1],
+            "tune_trials": [[list(k), v] for k, v in trials],
+            "plain_sequential_value": total / t_plain, "fused_sequential_value": total / t_fused,
+            "batched_over_plain": rate / (total / t_plain),
+            "per_mesh_single_shot": {str(k): v for k, v in per_mesh.items()}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -118,6 +166,8 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=1 << 30)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--adjoint-steps", type=int, default=800)
+    ap.add_argument("--batched-scale", type=float, default=1.0,
+                    help="duration scale of the batched configuration (0 skips it)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported "
@@ -179,6 +229,7 @@ def main():
 
     s.close()
     adj = adjoint_ratio(args.adjoint_steps) if args.adjoint_steps > 0 else None
+    bat = batched_throughput(args.batched_scale) if args.batched_scale > 0 else None
 
     cpu = None
     if not args.no_cpu:
@@ -209,6 +260,7 @@ def main():
         "fused_over_plain": value / plain_value,
         "adjoint_over_forward": adj["ratio"] if adj else None,
         "adjoint": adj,
+        "batched": bat,
         "energy_after": E,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,

@@ -167,6 +167,7 @@ int bbw_batch_add_shot(bbw_batch *b, int mesh_id, const float *c2q, const float 
 int bbw_batch_run(bbw_batch *b, int shots_per_tile, int blocks_per_cu, int use_graph);
 int bbw_batch_get_state(bbw_batch *b, int shot_id, float *Q);
 int bbw_batch_reset(bbw_batch *b);
+int bbw_batch_set_step_cap(bbw_batch *b, int cap); /* > 0: run at most cap steps per shot (tuning) */
 double bbw_batch_last_ms(const bbw_batch *b); /* device time of the last bbw_batch_run, ms */
 
 /* Tuning options (no reference counterpart beyond the Kblk* compile-time defines of
