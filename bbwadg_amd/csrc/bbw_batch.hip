@@ -35,8 +35,25 @@ struct BatchShotDev {
   int mesh;
 };
 
+// Descriptor values are the same for the whole workgroup: move them to scalar registers.
+__device__ __forceinline__ int su(int v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ __forceinline__ float sf(float v) {
+  return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
+}
+// Pointers read from the descriptor tables: uniform (scalar registers) and global memory, so
+// accesses through them compile to global loads with a scalar base, as for kernel arguments.
+template <typename T>
+__device__ __forceinline__ T *sp(T *p) {
+  const unsigned long long u = (unsigned long long)p;
+  const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)u);
+  const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(u >> 32));
+  typedef __attribute__((address_space(1))) T GT;
+  return (T *)(GT *)(((unsigned long long)hi << 32) | lo);
+}
+
 template <int N>
-__global__ __launch_bounds__(256) void k_fused_batch(int nitems, const BatchItem *__restrict__ items,
+// three waves per SIMD (three workgroups per CU, as LDS allows): caps registers at 168
+__global__ __launch_bounds__(256, 3) void k_fused_batch(int nitems, const BatchItem *__restrict__ items,
                                                      const FusedMesh *__restrict__ meshes,
                                                      const BatchShotDev *__restrict__ shots,
                                                      const int *__restrict__ list, FusedOps o,
@@ -45,23 +62,44 @@ __global__ __launch_bounds__(256) void k_fused_batch(int nitems, const BatchItem
   using L = FusedLayout<N, false>;
   extern __shared__ float lds[];
   const int st = *step;
+#pragma unroll 1
   for (int w = blockIdx.x; w < nitems; w += gridDim.x) {
-    const BatchItem it = items[w];
-    const FusedMesh m = meshes[it.mesh];
-    const int nk = min(L::TB, m.K - it.k0);
+    // thread index laundered per item: address arithmetic derived from it is redone per tile
+    // instead of being hoisted out of the persistent loop and kept live (it would spill)
+    int tid;
+    asm volatile("v_mov_b32 %0, %1" : "=v"(tid) : "v"((int)threadIdx.x));
+    const int mesh = su(items[w].mesh), k0 = su(items[w].k0);
+    const int shot0 = su(items[w].shot0), nshot = su(items[w].nshot);
+    FusedMesh m;
+    m.K = su(meshes[mesh].K);
+    m.vgeo = sp(meshes[mesh].vgeo);
+    m.fgeo = sp(meshes[mesh].fgeo);
+    m.vmapP = sp(meshes[mesh].vmapP);
+    const int nk = min(L::TB, m.K - k0);
     int vm[L::EPT3], fid3;
-    tile_geometry<N, false>(m, o, it.k0, nk, lds, vm, fid3);
-    for (int j = 0; j < it.nshot; ++j) {
-      const BatchShotDev &sd = shots[list[it.shot0 + j]];
+    tile_geometry<N, false>(m, o, k0, nk, lds, vm, fid3, tid);
+#pragma unroll 1
+    for (int j = 0; j < nshot; ++j) {
+      int tj;
+      asm volatile("v_mov_b32 %0, %1" : "=v"(tj) : "v"(tid));
+      const BatchShotDev *sd = shots + su(list[shot0 + j]);
       FusedShot sh;
-      sh.c2q = sd.c2q;
-      sh.Qin = sd.Q[par];
-      sh.Qout = sd.Q[1 - par];
-      sh.res = sd.res;
+      sh.c2q = sp(sd->c2q);
+      sh.Qin = sp(sd->Q[par]);
+      sh.Qout = sp(sd->Q[1 - par]);
+      sh.res = sp(sd->res);
       sh.yout = nullptr;
-      sh.ft = sd.src.nsrc ? sd.ftab[st * 5 + stage] : 0.f;
-      sh.fdt = sd.dt;
-      tile_shot<N, false>(m, o, sh, sd.src, fa, fb, it.k0, nk, lds, vm, fid3);
+      SrcArgs src;
+      src.nsrc = su(sd->src.nsrc);
+      src.first = su(sd->src.first);
+      src.last = su(sd->src.last);
+      src.elems = sp(sd->src.elems);
+      src.coef = sp(sd->src.coef);
+#pragma unroll
+      for (int f = 0; f < 9; ++f) src.w[f] = sf(sd->src.w[f]);
+      sh.ft = src.nsrc ? sf(sp(sd->ftab)[st * 5 + stage]) : 0.f;
+      sh.fdt = sf(sd->dt);
+      tile_shot<N, false>(m, o, sh, src, fa, fb, k0, nk, lds, vm, fid3, tj);
     }
   }
 }

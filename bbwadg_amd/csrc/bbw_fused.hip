@@ -33,8 +33,8 @@ __global__ __launch_bounds__(256) void k_fused_mfma(FusedMesh m, FusedOps o, Fus
   const int nk = min(L::TB, m.K - k0);
   if (ftime_dev) sh.ft = *ftime_dev;
   int vm[L::EPT3], fid3;
-  tile_geometry<N, PROJ_U>(m, o, k0, nk, lds, vm, fid3);
-  tile_shot<N, PROJ_U>(m, o, sh, src, fa, fb, k0, nk, lds, vm, fid3);
+  tile_geometry<N, PROJ_U>(m, o, k0, nk, lds, vm, fid3, threadIdx.x);
+  tile_shot<N, PROJ_U>(m, o, sh, src, fa, fb, k0, nk, lds, vm, fid3, threadIdx.x);
 }
 
 FusedOps fused_ops(const bbw_ctx *c) {

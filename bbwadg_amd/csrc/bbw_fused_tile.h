@@ -20,6 +20,11 @@ __device__ __forceinline__ int src_slot2(const SrcArgs &s, int k) {
   return -1;
 }
 
+// source weight of acoustic field f (0..3) without indexing the array by a per-lane value
+__device__ __forceinline__ float w4(const SrcArgs &s, int f) {
+  return f == 0 ? s.w[0] : f == 1 ? s.w[1] : f == 2 ? s.w[2] : s.w[3];
+}
+
 template <int N, bool PROJ_U>
 struct FusedLayout {
   static constexpr int Np = Dims<N>::Np, Nfp = Dims<N>::Nfp, NfpNf = Dims<N>::NfpNf;
@@ -80,12 +85,11 @@ using FL = FusedLayout<N, PROJ_U>;
 template <int N, bool PROJ_U>
 __device__ __forceinline__ void tile_geometry(const FusedMesh &m, const FusedOps &o, int k0, int nk,
                                               float *lds, int (&vm)[FL<N, PROJ_U>::EPT3],
-                                              int &fid3) {
+                                              int &fid3, const int tid) {
   using L = FL<N, PROJ_U>;
   constexpr int NfpNf = L::NfpNf, TB = L::TB;
   float *sG = lds + TB * (L::W1 + L::W2 + L::RS);
   float *sF = sG + TB * 9;
-  const int tid = threadIdx.x;
   for (int i = tid; i < nk * 9; i += 256) sG[i] = m.vgeo[(size_t)k0 * 9 + i];
   for (int i = tid; i < nk * 16; i += 256) sF[i] = m.fgeo[(size_t)k0 * 16 + i];
   const int g3 = tid / NfpNf, m3 = tid - g3 * NfpNf;
@@ -104,7 +108,7 @@ template <int N, bool PROJ_U>
 __device__ __forceinline__ void tile_shot(const FusedMesh &m, const FusedOps &o, const FusedShot &sh,
                                           const SrcArgs &src, float fa, float fb, int k0, int nk,
                                           float *lds, const int (&vm)[FL<N, PROJ_U>::EPT3],
-                                          int fid3) {
+                                          int fid3, const int tid) {
   using L = FusedLayout<N, PROJ_U>;
   constexpr int Np = L::Np, Nfp = L::Nfp, NfpNf = L::NfpNf, Nq = L::Nq, TB = L::TB, RS = L::RS;
   constexpr int Nf = 4, NpNf = 4 * Np, EEL_nnz = Dims<N>::EEL_nnz, L0_nnz = Dims<N>::L0_nnz;
@@ -134,7 +138,7 @@ __device__ __forceinline__ void tile_shot(const FusedMesh &m, const FusedOps &o,
   float *S = W1;                         // phases 7-8
   float4 *sV = (float4 *)W2;             // phases 2-3
   float2 *sFl = (float2 *)W2;            // phases 4-5
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int lane = tid & 63, wave = tid >> 6;
   const size_t base = (size_t)k0 * NpNf;
   // face-node mapping (phases 2, 4, 5)
   const int g3 = tid / NfpNf, m3 = tid - g3 * NfpNf;
@@ -393,12 +397,13 @@ __device__ __forceinline__ void tile_shot(const FusedMesh &m, const FusedOps &o,
   if constexpr (L::PF) {
     if (tid < NpNf) {
       const int fr = tid / Np, nr = tid - fr * Np;
+      const float wsel = w4(src, fr);
 #pragma unroll
       for (int e = 0; e < TB; ++e) {
         if (e < nk) {
           float v = sR[e * RS + tid];
           const int slot = sSlot[e];
-          if (slot >= 0) v += src.w[fr] * ft * src.coef[(size_t)slot * Np + nr];
+          if (slot >= 0) v += wsel * ft * src.coef[(size_t)slot * Np + nr];
           const float x = fa * rreg[e] + fdt * v;
           res[base + e * NpNf + tid] = x;
           Qout[base + e * NpNf + tid] = qreg[e] + fb * x;
@@ -411,7 +416,7 @@ __device__ __forceinline__ void tile_shot(const FusedMesh &m, const FusedOps &o,
       for (int e = 0; e < nk; ++e) {
         float v = sR[e * RS + rr];
         const int slot = sSlot[e];
-        if (slot >= 0) v += src.w[fr] * ft * src.coef[(size_t)slot * Np + nr];
+        if (slot >= 0) v += w4(src, fr) * ft * src.coef[(size_t)slot * Np + nr];
         const size_t g = base + e * NpNf + rr;
         const float x = fa * res[g] + fdt * v;
         res[g] = x;

@@ -120,6 +120,9 @@ def batched_throughput(step_scale=1.0, tune_steps=20):
     b, desc = c4_workload(seed=7, step_scale=step_scale, ops=ops, log=log)
     best, trials = autotune.tune_batch(b, step_cap=tune_steps)
     log(f"batch autotune: {best} from {trials}")
+    est = min(t for _, t in trials) / 1e3 * sum(b.meshes[m][0].K * n for m, n in b.shots) / \
+        (tune_steps * sum(b.meshes[m][0].K for m, _ in b.shots))
+    log(f"batch run: estimated {est:.0f} s")
     ms = b.run(shots_per_tile=best[0], blocks_per_cu=best[1])
     log(f"batch run {ms:.1f} ms")
     total = b.dof_stages()
