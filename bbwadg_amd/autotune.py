@@ -54,9 +54,10 @@ def tune_single(solver, dt, nsteps=10, kblk_vector=(4, 8, 16), kblk_plain=(1, 2,
             "plain_choice": best_plain[0]}
 
 
-def tune_batch(batch, shots_per_tile=(1, 2, 4), blocks_per_cu=(1, 2, 3, 4), step_cap=0):
+def tune_batch(batch, shots_per_tile=(1, 2, 4), blocks_per_cu=(0, 2, 3, 4), step_cap=0):
     """Time every (shots-per-tile, blocks-per-CU) pair on the batch (each shot capped at
-    ``step_cap`` steps when > 0); returns (best, trials).  The batch is reset afterwards."""
+    ``step_cap`` steps when > 0; blocks-per-CU 0 = one workgroup per work item); returns (best,
+    trials).  The batch is reset afterwards."""
     trials = []
     batch.set_step_cap(step_cap)
     for s in shots_per_tile:

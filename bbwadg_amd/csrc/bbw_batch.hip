@@ -301,7 +301,7 @@ int bbw_batch_add_shot(bbw_batch *b, int mesh_id, const float *c2q, const float 
 }
 
 int bbw_batch_run(bbw_batch *b, int shots_per_tile, int blocks_per_cu, int use_graph) {
-  if (!b || shots_per_tile <= 0 || blocks_per_cu <= 0) return BBW_ERR_ARG;
+  if (!b || shots_per_tile <= 0 || blocks_per_cu < 0) return BBW_ERR_ARG;
   bbw_ctx *c = b->ops;
   if (!c->have_ops) return bfail(b, BBW_ERR_STATE, "operators not uploaded");
   const int S = shots_per_tile, TB = 16;
@@ -309,7 +309,8 @@ int bbw_batch_run(bbw_batch *b, int shots_per_tile, int blocks_per_cu, int use_g
   if (ns == 0) return BBW_OK;
   hipDeviceProp_t prop;
   if (hipGetDeviceProperties(&prop, c->dev) != hipSuccess) return bfail(b, BBW_ERR_HIP, "device query");
-  const int max_grid = prop.multiProcessorCount * blocks_per_cu;
+  // blocks_per_cu = 0: one workgroup per item (the hardware dispatcher balances the tiles)
+  const int max_grid = blocks_per_cu > 0 ? prop.multiProcessorCount * blocks_per_cu : (1 << 30);
 
   std::vector<int> nst(ns);
   for (int i = 0; i < ns; ++i)

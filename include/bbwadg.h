@@ -153,7 +153,7 @@ int bbw_adjoint_gradient(bbw_ctx *ctx, int nsteps, double dt, double t0, const f
  * replay.  Each shot: its mesh, c2q [K][Nq], initial state [K][4][Np], a source as in
  * bbw_set_source, time step dt and step count.  bbw_batch_run advances every shot to its own
  * step count; shots_per_tile shots of one mesh share a tile's geometry; blocks_per_cu sets the
- * persistent grid.  bbw_batch_reset restores every shot's initial state. */
+ * persistent grid (0: one workgroup per work item).  bbw_batch_reset restores every shot's initial state. */
 typedef struct bbw_batch bbw_batch;
 int bbw_batch_create(int N, int device, bbw_batch **out);
 void bbw_batch_destroy(bbw_batch *b);
