@@ -37,7 +37,19 @@ def main():
     ap.add_argument("--mode", choices=["fused", "plain"], default="fused")
     ap.add_argument("--impl", choices=["mfma", "vector"], default="mfma")
     ap.add_argument("--project-velocity", action="store_true")
+    ap.add_argument("--copy-bytes", type=int, default=0,
+                    help="instead: one device-to-device copy of this many bytes (counter calibration)")
     args = ap.parse_args()
+    if args.copy_bytes:
+        import torch
+        n = args.copy_bytes // 4
+        a = torch.ones(n, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        b.copy_(a)
+        torch.cuda.synchronize()
+        print(f"copied {n * 4} bytes", flush=True)
+        return
     s = setup(args.mesh, args.N)
     s.set_option(_lib.OPT_FUSED_IMPL, _lib.FUSED_MFMA if args.impl == "mfma" else _lib.FUSED_VECTOR)
     mode = MODE_FUSED if args.mode == "fused" else MODE_PLAIN

@@ -267,7 +267,7 @@ def main():
         "energy_after": E,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "k_fused<5,false>", "bytes_per_element_stage": Bst,
+                     "kernel": "k_fused_mfma<5,false>", "bytes_per_element_stage": Bst,
                      "avg_launch_ms": stage_ms},
         "cpu_baseline": cpu,
     }
