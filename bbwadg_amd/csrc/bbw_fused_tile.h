@@ -80,30 +80,62 @@ struct FusedShot {
 template <int N, bool PROJ_U>
 using FL = FusedLayout<N, PROJ_U>;
 
+// Buffer resources: a scalar base and byte size per array (or per tile), addressed by 32-bit
+// per-lane offsets with the constant part in the instruction's offset field.  Reads past the
+// size return 0 and writes past it are dropped, which is how partial tiles are handled.
+typedef __amdgpu_buffer_rsrc_t brsrc;
+__device__ __forceinline__ brsrc mkrsrc(const void *p, unsigned bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), (short)0, (int)bytes, 0x00020000);
+}
+__device__ __forceinline__ float bld(brsrc r, unsigned off) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0));
+}
+__device__ __forceinline__ int bldi(brsrc r, unsigned off) {
+  return (int)__builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0);
+}
+// (the 128-bit result is converted as a whole: extracting its elements one by one before the
+// conversion was folded to a single element by this compiler)
+__device__ __forceinline__ float4 bld4(brsrc r, unsigned off) {
+  const f4 v = __builtin_bit_cast(f4, __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0));
+  return make_float4(v[0], v[1], v[2], v[3]);
+}
+// operator tables: every offset is in range by construction, so the uniform part may go to the
+// scalar offset (which the range check does not cover)
+__device__ __forceinline__ float bld_s(brsrc r, unsigned voff, unsigned soff) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
+}
+__device__ __forceinline__ void bst(brsrc r, float v, unsigned off) {
+  __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, off, 0, 0);
+}
+
 // Geometry of a tile: vgeo/fgeo into LDS, the thread's vmapP entries and face-node id into
-// registers.  Shared by every shot processed on the same tile.
+// registers.  Shared by every shot processed on the same tile.  Entries of elements past the
+// tile's end read as 0 (a valid trace index; their results are never stored).
 template <int N, bool PROJ_U>
 __device__ __forceinline__ void tile_geometry(const FusedMesh &m, const FusedOps &o, int k0, int nk,
                                               float *lds, int (&vm)[FL<N, PROJ_U>::EPT3],
                                               int &fid3, const int tid) {
   using L = FL<N, PROJ_U>;
   constexpr int NfpNf = L::NfpNf, TB = L::TB;
+  static_assert(TB * 16 <= 256, "one geometry word per thread");
   float *sG = lds + TB * (L::W1 + L::W2 + L::RS);
   float *sF = sG + TB * 9;
-  for (int i = tid; i < nk * 9; i += 256) sG[i] = m.vgeo[(size_t)k0 * 9 + i];
-  for (int i = tid; i < nk * 16; i += 256) sF[i] = m.fgeo[(size_t)k0 * 16 + i];
+  const brsrc rg = mkrsrc(m.vgeo + (size_t)k0 * 9, (unsigned)nk * 36u);
+  const brsrc rf = mkrsrc(m.fgeo + (size_t)k0 * 16, (unsigned)nk * 64u);
+  const brsrc rv = mkrsrc(m.vmapP + (size_t)k0 * NfpNf, (unsigned)nk * NfpNf * 4u);
+  if (tid < TB * 9) sG[tid] = bld(rg, tid * 4u);
+  if (tid < TB * 16) sF[tid] = bld(rf, tid * 4u);
   const int g3 = tid / NfpNf, m3 = tid - g3 * NfpNf;
   const bool on3 = g3 < L::NG3;
   fid3 = on3 ? o.Fmask[m3] : 0;
+  const unsigned vo = (unsigned)(g3 * NfpNf + m3) * 4u;
 #pragma unroll
-  for (int i = 0; i < L::EPT3; ++i) {
-    const int e = g3 + L::NG3 * i;
-    vm[i] = (on3 && e < nk) ? m.vmapP[(size_t)(k0 + e) * NfpNf + m3] : -1;
-  }
+  for (int i = 0; i < L::EPT3; ++i) vm[i] = bldi(rv, vo + i * L::NG3 * NfpNf * 4u);
 }
 
 // One fused LSRK stage of one shot on one tile (phases 1-9 above).  Ends with a barrier so
-// the caller may process another shot on the same tile.
+// the caller may process another shot on the same tile.  Elements past the tile's end are
+// computed on zeros and never stored.  Requires K * 4 * Np * 4 < 2^32 (32-bit trace offsets).
 template <int N, bool PROJ_U>
 __device__ __forceinline__ void tile_shot(const FusedMesh &m, const FusedOps &o, const FusedShot &sh,
                                           const SrcArgs &src, float fa, float fb, int k0, int nk,
@@ -111,13 +143,9 @@ __device__ __forceinline__ void tile_shot(const FusedMesh &m, const FusedOps &o,
                                           int fid3, const int tid) {
   using L = FusedLayout<N, PROJ_U>;
   constexpr int Np = L::Np, Nfp = L::Nfp, NfpNf = L::NfpNf, Nq = L::Nq, TB = L::TB, RS = L::RS;
-  constexpr int Nf = 4, NpNf = 4 * Np, EEL_nnz = Dims<N>::EEL_nnz, L0_nnz = Dims<N>::L0_nnz;
+  constexpr int NpNf = 4 * Np, EEL_nnz = Dims<N>::EEL_nnz, L0_nnz = Dims<N>::L0_nnz;
   constexpr int NP = L::NP, SX = L::SX, NQP = L::NQP, NPP = L::NPP;
-  const float *__restrict__ Qin = sh.Qin;
-  float *__restrict__ Qout = sh.Qout;
-  float *__restrict__ res = sh.res;
-  float *__restrict__ yout = sh.yout;
-  const float *__restrict__ c2q = sh.c2q;
+  constexpr unsigned ROW = NpNf * 4u;    // bytes of one element's state
   const float fdt = sh.fdt;
   const int4 *__restrict__ Dids = o.Dids;
   const float4 *__restrict__ Dvals = o.Dvals;
@@ -125,8 +153,6 @@ __device__ __forceinline__ void tile_shot(const FusedMesh &m, const FusedOps &o,
   const float *__restrict__ EEL_vals = o.EEL_vals;
   const int *__restrict__ L0_ids = o.L0_ids;
   const float *__restrict__ L0_vals = o.L0_vals;
-  const float *__restrict__ VqPad = o.VqPad;
-  const float *__restrict__ PqPad = o.PqPad;
   float *W1 = lds;                       // [TB * W1]
   float *W2 = W1 + TB * L::W1;           // [TB * W2]
   float *sR = W2 + TB * L::W2;           // [TB][RS]
@@ -138,8 +164,14 @@ __device__ __forceinline__ void tile_shot(const FusedMesh &m, const FusedOps &o,
   float *S = W1;                         // phases 7-8
   float4 *sV = (float4 *)W2;             // phases 2-3
   float2 *sFl = (float2 *)W2;            // phases 4-5
-  const int lane = tid & 63, wave = tid >> 6;
+  const int lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const size_t base = (size_t)k0 * NpNf;
+  const unsigned tbytes = (unsigned)nk * ROW;
+  const brsrc rQ = mkrsrc(sh.Qin + base, tbytes);
+  const brsrc rR = mkrsrc(sh.res + base, tbytes);
+  const brsrc rO = mkrsrc(sh.Qout + base, tbytes);
+  const brsrc rC = mkrsrc(sh.c2q + (size_t)k0 * Nq, (unsigned)nk * Nq * 4u);
+  const brsrc rT = mkrsrc(sh.Qin, (unsigned)m.K * ROW);   // neighbour traces
   // face-node mapping (phases 2, 4, 5)
   const int g3 = tid / NfpNf, m3 = tid - g3 * NfpNf;
   const bool on3 = g3 < L::NG3;
@@ -147,32 +179,39 @@ __device__ __forceinline__ void tile_shot(const FusedMesh &m, const FusedOps &o,
   // node mapping (phases 2, 3, 6)
   const int gv = tid / L::GS, nv = tid - gv * L::GS;
   const bool onv = gv < L::NG && nv < Np;
+  // source: elements are sorted, so a tile outside [first, last] has none (uniform test)
+  const bool has_src = src.nsrc > 0 && k0 + nk - 1 >= src.first && k0 <= src.last;
 
   // ---- 1: tile loads and register prefetch
   {
-    const float4 *Qt4 = (const float4 *)(Qin + base);
+    constexpr unsigned QB = TB * ROW, QIT = (QB + 4095) / 4096;
+    constexpr bool QFIT = QIT * 4096 <= TB * L::W1 * 4u;
     float4 *sQ4 = (float4 *)sQ;
-    for (int i = tid; i < nk * Np; i += 256) sQ4[i] = Qt4[i];
-    for (int i = tid; i < TB * RS; i += 256) sR[i] = 0.f;
-    if (tid < TB) sSlot[tid] = tid < nk ? src_slot2(src, k0 + tid) : -1;
+#pragma unroll
+    for (unsigned it = 0; it < QIT; ++it) {
+      const unsigned off = tid * 16u + it * 4096u;
+      if (QFIT || off < QB) sQ4[off / 16] = bld4(rQ, off);
+    }
+    if (tid < TB * 3) sR[(tid / 3) * RS + NpNf + tid % 3] = 0.f;   // K padding of the MFMA
+    if (has_src && tid < TB) sSlot[tid] = tid < nk ? src_slot2(src, k0 + tid) : -1;
   }
   float qreg[L::PF ? TB : 1], rreg[L::PF ? TB : 1];
   if constexpr (L::PF) {
 #pragma unroll
     for (int e = 0; e < TB; ++e) {
-      const bool ok = tid < NpNf && e < nk;
-      qreg[e] = ok ? Qin[base + e * NpNf + tid] : 0.f;
-      rreg[e] = ok ? res[base + e * NpNf + tid] : 0.f;
+      qreg[e] = bld(rQ, tid * 4u + e * ROW);
+      rreg[e] = bld(rR, tid * 4u + e * ROW);
     }
   }
   float c2r[L::QTW][4];
+  {
+    // rows q >= Nq read a neighbouring value; the padded Vq rows make them 0 in S
+    const unsigned co = (unsigned)((lane & 15) * Nq + 4 * (lane >> 4)) * 4u;
 #pragma unroll
-  for (int t = 0; t < L::QTW; ++t) {
-    const int qt = wave + 4 * t;
+    for (int t = 0; t < L::QTW; ++t) {
+      const int qt = wave + 4 * t;
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      const int q = qt * 16 + 4 * (lane >> 4) + r, e = lane & 15;
-      c2r[t][r] = (qt < L::NQT && q < Nq && e < nk) ? c2q[(size_t)(k0 + e) * Nq + q] : 0.f;
+      for (int r = 0; r < 4; ++r) c2r[t][r] = qt < L::NQT ? bld(rC, co + (qt * 16 + r) * 4u) : 0.f;
     }
   }
   __syncthreads();
@@ -182,7 +221,7 @@ __device__ __forceinline__ void tile_shot(const FusedMesh &m, const FusedOps &o,
 #pragma unroll
     for (int i = 0; i < L::EPTV; ++i) {
       const int e = gv + L::NG * i;
-      if (e < nk) {
+      if (L::NG * L::EPTV == TB || e < TB) {
         const float *s = sQ + e * NpNf;
         const float *G = sG + 9 * e;
         const float uu = s[Np + nv], vv = s[2 * Np + nv], ww = s[3 * Np + nv];
@@ -195,27 +234,32 @@ __device__ __forceinline__ void tile_shot(const FusedMesh &m, const FusedOps &o,
   float tr[L::EPT3][4];
 #pragma unroll
   for (int i = 0; i < L::EPT3; ++i) {
-    const int e = g3 + L::NG3 * i;
-    const int idM = fid3 + (k0 + e) * NpNf;
-    const bool inter = vm[i] >= 0 && vm[i] != idM;
+    const unsigned to = (unsigned)vm[i] * 4u;
 #pragma unroll
-    for (int f = 0; f < 4; ++f) tr[i][f] = inter ? Qin[vm[i] + f * Np] : 0.f;
+    for (int f = 0; f < 4; ++f) tr[i][f] = bld(rT, to + f * Np * 4u);
   }
   __syncthreads();
 
-  // ---- 3: volume terms
+  // ---- 3: volume terms (operand addresses fixed per thread, element offsets immediate)
   if (onv) {
     const float4 dv = Dvals[nv];
     const int4 D1 = Dids[nv], D2 = Dids[Np + nv], D3 = Dids[2 * Np + nv], D4 = Dids[3 * Np + nv];
+    const float4 *Vb = sV + gv * Np;
+    const float4 *pa1 = Vb + D1.x, *pa2 = Vb + D2.x, *pa3 = Vb + D3.x, *pa4 = Vb + D4.x;
+    const float4 *pb1 = Vb + D1.y, *pb2 = Vb + D2.y, *pb3 = Vb + D3.y, *pb4 = Vb + D4.y;
+    const float4 *pc1 = Vb + D1.z, *pc2 = Vb + D2.z, *pc3 = Vb + D3.z, *pc4 = Vb + D4.z;
+    const float4 *pd1 = Vb + D1.w, *pd2 = Vb + D2.w, *pd3 = Vb + D3.w, *pd4 = Vb + D4.w;
+    float *rb = sR + gv * RS + nv;
+    const float *Gb = sG + 9 * gv;
 #pragma unroll
     for (int i = 0; i < L::EPTV; ++i) {
       const int e = gv + L::NG * i;
-      if (e < nk) {
-        const float4 *V = sV + e * Np;
-        const float4 a1 = V[D1.x], a2 = V[D2.x], a3 = V[D3.x], a4 = V[D4.x];
-        const float4 b1 = V[D1.y], b2 = V[D2.y], b3 = V[D3.y], b4 = V[D4.y];
-        const float4 c1 = V[D1.z], c2 = V[D2.z], c3 = V[D3.z], c4 = V[D4.z];
-        const float4 d1 = V[D1.w], d2 = V[D2.w], d3 = V[D3.w], d4 = V[D4.w];
+      if (L::NG * L::EPTV == TB || e < TB) {
+        const int o4 = i * L::NG * Np;
+        const float4 a1 = pa1[o4], a2 = pa2[o4], a3 = pa3[o4], a4 = pa4[o4];
+        const float4 b1 = pb1[o4], b2 = pb2[o4], b3 = pb3[o4], b4 = pb4[o4];
+        const float4 c1 = pc1[o4], c2 = pc2[o4], c3 = pc3[o4], c4 = pc4[o4];
+        const float4 d1 = pd1[o4], d2 = pd2[o4], d3 = pd3[o4], d4 = pd4[o4];
         const float p1 = dv.x * a1.x + dv.y * a2.x + dv.z * a3.x + dv.w * a4.x;
         const float p2 = dv.x * b1.x + dv.y * b2.x + dv.z * b3.x + dv.w * b4.x;
         const float p3 = dv.x * c1.x + dv.y * c2.x + dv.z * c3.x + dv.w * c4.x;
@@ -229,12 +273,12 @@ __device__ __forceinline__ void tile_shot(const FusedMesh &m, const FusedOps &o,
         dU2 += dv.z * (a3.y + a3.z + a3.w);
         dU2 += dv.w * (a4.y + a4.z + a4.w);
         const float dpdr = .5f * (p2 - p1), dpds = .5f * (p3 - p1), dpdt = .5f * (p4 - p1);
-        const float *G = sG + 9 * e;
-        float *r = sR + e * RS;
-        r[nv] = -.5f * (dU1 - dU2);
-        r[Np + nv] = -(G[0] * dpdr + G[3] * dpds + G[6] * dpdt);
-        r[2 * Np + nv] = -(G[1] * dpdr + G[4] * dpds + G[7] * dpdt);
-        r[3 * Np + nv] = -(G[2] * dpdr + G[5] * dpds + G[8] * dpdt);
+        const float *G = Gb + 9 * L::NG * i;
+        float *r = rb + L::NG * i * RS;
+        r[0] = -.5f * (dU1 - dU2);
+        r[Np] = -(G[0] * dpdr + G[3] * dpds + G[6] * dpdt);
+        r[2 * Np] = -(G[1] * dpdr + G[4] * dpds + G[7] * dpdt);
+        r[3 * Np] = -(G[2] * dpdr + G[5] * dpds + G[8] * dpdt);
       }
     }
   }
@@ -245,7 +289,7 @@ __device__ __forceinline__ void tile_shot(const FusedMesh &m, const FusedOps &o,
 #pragma unroll
     for (int i = 0; i < L::EPT3; ++i) {
       const int e = g3 + L::NG3 * i;
-      if (e < nk) {
+      if (L::NG3 * L::EPT3 == TB || e < TB) {
         const float *s = sQ + e * NpNf;
         const float *F = sF + 16 * e + 4 * f3;
         const float pM = s[fid3], uM = s[Np + fid3], vM = s[2 * Np + fid3], wM = s[3 * Np + fid3];
@@ -268,21 +312,22 @@ __device__ __forceinline__ void tile_shot(const FusedMesh &m, const FusedOps &o,
   if (on3) {
     const int nt = m3 - f3 * Nfp;
     float Lv[L0_nnz];
-    int Li[L0_nnz];
+    const float2 *Xp[L0_nnz];
+    const float2 *Xb = sFl + g3 * NfpNf + f3 * Nfp;
 #pragma unroll
     for (int j = 0; j < L0_nnz; ++j) {
       Lv[j] = L0_vals[nt + j * Nfp];
-      Li[j] = L0_ids[nt + j * Nfp] + f3 * Nfp;
+      Xp[j] = Xb + L0_ids[nt + j * Nfp];
     }
 #pragma unroll
     for (int i = 0; i < L::EPT3; ++i) {
       const int e = g3 + L::NG3 * i;
-      if (e < nk) {
-        const float2 *X = sFl + e * NfpNf;
+      if (L::NG3 * L::EPT3 == TB || e < TB) {
+        const int o2 = i * L::NG3 * NfpNf;
         float v1 = 0.f, v2 = 0.f;
 #pragma unroll
         for (int j = 0; j < L0_nnz; ++j) {
-          const float2 x = X[Li[j]];
+          const float2 x = Xp[j][o2];
           v1 += Lv[j] * x.x;
           v2 += Lv[j] * x.y;
         }
@@ -293,34 +338,36 @@ __device__ __forceinline__ void tile_shot(const FusedMesh &m, const FusedOps &o,
   }
   __syncthreads();
 
-  // ---- 6: EEL lift into rhs
+  // ---- 6: EEL lift into rhs (operand addresses fixed per thread)
   if (onv) {
     float Ev[EEL_nnz];
-    int Ei[EEL_nnz];
+    const float4 *Ep[EEL_nnz];
+    const float *Lb = (const float *)(sL + gv * NfpNf);
 #pragma unroll
     for (int j = 0; j < EEL_nnz; ++j) {
       Ev[j] = EEL_vals[nv + j * Np];
-      Ei[j] = EEL_ids4[nv + j * Np];
+      Ep[j] = (const float4 *)(Lb + EEL_ids4[nv + j * Np]);
     }
+    float *rb = sR + gv * RS + nv;
 #pragma unroll
     for (int i = 0; i < L::EPTV; ++i) {
       const int e = gv + L::NG * i;
-      if (e < nk) {
-        const float *Lt = (const float *)(sL + e * NfpNf);
+      if (L::NG * L::EPTV == TB || e < TB) {
+        const int o4 = i * L::NG * NfpNf;
         float v1 = 0.f, v2 = 0.f, v3 = 0.f, v4 = 0.f;
 #pragma unroll
         for (int j = 0; j < EEL_nnz; ++j) {
-          const float4 x = *(const float4 *)(Lt + Ei[j]);
+          const float4 x = Ep[j][o4];
           v1 += Ev[j] * x.x;
           v2 += Ev[j] * x.y;
           v3 += Ev[j] * x.z;
           v4 += Ev[j] * x.w;
         }
-        float *r = sR + e * RS;
-        r[nv] += v1;
-        r[Np + nv] += v2;
-        r[2 * Np + nv] += v3;
-        r[3 * Np + nv] += v4;
+        float *r = rb + L::NG * i * RS;
+        r[0] += v1;
+        r[Np] += v2;
+        r[2 * Np] += v3;
+        r[3 * Np] += v4;
       }
     }
   }
@@ -330,6 +377,8 @@ __device__ __forceinline__ void tile_shot(const FusedMesh &m, const FusedOps &o,
   {
     const int col = lane & 15, kr = lane >> 4;
     const float *Bp = sR + col * RS + kr;
+    const brsrc rA = mkrsrc(o.VqPad, 4u * L::KTV * NQP * 4u);
+    const brsrc rY = mkrsrc(sh.yout ? sh.yout + (size_t)k0 * Nq : nullptr, sh.yout ? (unsigned)nk * Nq * 4u : 0u);
 #pragma unroll
     for (int t = 0; t < L::QTW; ++t) {
       const int qt = wave + 4 * t;
@@ -337,20 +386,20 @@ __device__ __forceinline__ void tile_shot(const FusedMesh &m, const FusedOps &o,
       f4 acc[NP];
 #pragma unroll
       for (int b = 0; b < NP; ++b) acc[b] = f4{0.f, 0.f, 0.f, 0.f};
-      const float *Ap = VqPad + kr * NQP + qt * 16 + col;
+      const unsigned ao = (unsigned)(kr * NQP + col) * 4u;
 #pragma unroll
       for (int kt = 0; kt < L::KTV; ++kt) {
-        const float a = Ap[kt * 4 * NQP];
+        const float a = bld_s(rA, ao, (unsigned)(kt * 4 * NQP + qt * 16) * 4u);
 #pragma unroll
         for (int b = 0; b < NP; ++b)
           acc[b] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, Bp[b * Np + kt * 4], acc[b], 0, 0, 0);
       }
-      if (yout && col < nk) {
+      if (sh.yout) {
         // Vq . rhs_p at the cubature points, kept for the adjoint's c2 gradient
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
           const int q = qt * 16 + 4 * kr + r;
-          if (q < Nq) yout[(size_t)(k0 + col) * Nq + q] = acc[0][r];
+          if (q < Nq) bst(rY, acc[0][r], (unsigned)(col * Nq + q) * 4u);
         }
       }
 #pragma unroll
@@ -367,15 +416,16 @@ __device__ __forceinline__ void tile_shot(const FusedMesh &m, const FusedOps &o,
   // ---- 8: rhs = Pq . S on MFMA
   {
     const int col = lane & 15, kr = lane >> 4;
+    const brsrc rA = mkrsrc(o.PqPad, (unsigned)NQP * NPP * 4u);
     for (int mt = wave; mt < L::NPT; mt += 4) {
       f4 acc[NP];
 #pragma unroll
       for (int b = 0; b < NP; ++b) acc[b] = f4{0.f, 0.f, 0.f, 0.f};
-      const float *Ap = PqPad + kr * NPP + mt * 16 + col;
+      const unsigned ao = (unsigned)(kr * NPP + col) * 4u;
       const float *Bp = S + kr * SX + col;
 #pragma unroll 8
       for (int kt = 0; kt < L::KTP; ++kt) {
-        const float a = Ap[kt * 4 * NPP];
+        const float a = bld_s(rA, ao, (unsigned)(kt * 4 * NPP + mt * 16) * 4u);
 #pragma unroll
         for (int b = 0; b < NP; ++b)
           acc[b] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, Bp[kt * 4 * SX + b * 16], acc[b], 0, 0, 0);
@@ -396,17 +446,24 @@ __device__ __forceinline__ void tile_shot(const FusedMesh &m, const FusedOps &o,
   const float ft = sh.ft;
   if constexpr (L::PF) {
     if (tid < NpNf) {
-      const int fr = tid / Np, nr = tid - fr * Np;
-      const float wsel = w4(src, fr);
+      if (!has_src) {
 #pragma unroll
-      for (int e = 0; e < TB; ++e) {
-        if (e < nk) {
+        for (int e = 0; e < TB; ++e) {
+          const float x = fa * rreg[e] + fdt * sR[e * RS + tid];
+          bst(rR, x, tid * 4u + e * ROW);
+          bst(rO, qreg[e] + fb * x, tid * 4u + e * ROW);
+        }
+      } else {
+        const int fr = tid / Np, nr = tid - fr * Np;
+        const float wsel = w4(src, fr);
+#pragma unroll
+        for (int e = 0; e < TB; ++e) {
           float v = sR[e * RS + tid];
           const int slot = sSlot[e];
           if (slot >= 0) v += wsel * ft * src.coef[(size_t)slot * Np + nr];
           const float x = fa * rreg[e] + fdt * v;
-          res[base + e * NpNf + tid] = x;
-          Qout[base + e * NpNf + tid] = qreg[e] + fb * x;
+          bst(rR, x, tid * 4u + e * ROW);
+          bst(rO, qreg[e] + fb * x, tid * 4u + e * ROW);
         }
       }
     }
@@ -415,12 +472,14 @@ __device__ __forceinline__ void tile_shot(const FusedMesh &m, const FusedOps &o,
       const int fr = rr / Np, nr = rr - fr * Np;
       for (int e = 0; e < nk; ++e) {
         float v = sR[e * RS + rr];
-        const int slot = sSlot[e];
-        if (slot >= 0) v += w4(src, fr) * ft * src.coef[(size_t)slot * Np + nr];
-        const size_t g = base + e * NpNf + rr;
-        const float x = fa * res[g] + fdt * v;
-        res[g] = x;
-        Qout[g] = Qin[g] + fb * x;
+        if (has_src) {
+          const int slot = sSlot[e];
+          if (slot >= 0) v += w4(src, fr) * ft * src.coef[(size_t)slot * Np + nr];
+        }
+        const unsigned g = rr * 4u + e * ROW;
+        const float x = fa * bld(rR, g) + fdt * v;
+        bst(rR, x, g);
+        bst(rO, bld(rQ, g) + fb * x, g);
       }
     }
   }
