@@ -156,7 +156,7 @@ void bbw_destroy(bbw_ctx *c) {
   if (c->stream) hipStreamSynchronize(c->stream);
   reset_graph(c);
   dfree(c->Dids); dfree(c->Dvals); dfree(c->Fmask); dfree(c->EEL_ids); dfree(c->L0_ids);
-  dfree(c->EEL_vals); dfree(c->L0_vals); dfree(c->Vq); dfree(c->Pq); dfree(c->Vq64); dfree(c->VqPad); dfree(c->PqPad); dfree(c->EEL_ids4); dfree(c->PqTPad); dfree(c->VqTPad);
+  dfree(c->EEL_vals); dfree(c->L0_vals); dfree(c->Vq); dfree(c->Pq); dfree(c->Vq64); dfree(c->VqPad); dfree(c->PqPad); dfree(c->EEL_ids4); dfree(c->LiftPad); dfree(c->PqTPad); dfree(c->VqTPad);
   dfree(c->DTids); dfree(c->DTvals); dfree(c->EELT_ids); dfree(c->EELT_vals); dfree(c->L0T_ids);
   dfree(c->L0T_vals); dfree(c->FmaskInv); dfree(c->mapP); dfree(c->lQ); dfree(c->lres);
   dfree(c->Zadj); dfree(c->Hadj); dfree(c->grad); dfree(c->Ybuf); dfree(c->ckpt);
@@ -302,6 +302,30 @@ int bbw_upload_operators(bbw_ctx *c, const bbw_ops *o) {
     for (size_t i = 0; i < e4.size(); ++i) e4[i] = o->EEL_ids[i] * 4;  // float offset of a float4 slot
     dfree(c->EEL_ids4);
     if ((rc = upload(c, &c->EEL_ids4, e4.data(), e4.size()))) return rc;
+    // dense lift per face: LIFT[n][f Nfp + t] = sum over EEL entries (n, f Nfp + s) of
+    // EEL(n, f Nfp + s) L0(s, t), accumulated in double from the FP32 factors
+    {
+      const int KF = 4 * ((Nfp + 3) / 4);
+      std::vector<double> lift((size_t)Np * NfpNf, 0.0);
+      for (int n = 0; n < Np; ++n)
+        for (int j = 0; j < o->EEL_nnz; ++j) {
+          const int col = o->EEL_ids[n + (size_t)j * Np];
+          const double ev = o->EEL_vals[n + (size_t)j * Np];
+          if (ev == 0.0) continue;
+          const int f = col / Nfp, s = col - f * Nfp;
+          for (int jj = 0; jj < o->L0_nnz; ++jj) {
+            const double lv = o->L0_vals[s + (size_t)jj * Nfp];
+            if (lv != 0.0) lift[(size_t)n * NfpNf + f * Nfp + o->L0_ids[s + (size_t)jj * Nfp]] += ev * lv;
+          }
+        }
+      std::vector<float> lp((size_t)4 * KF * NPP, 0.f);
+      for (int f = 0; f < 4; ++f)
+        for (int t = 0; t < Nfp; ++t)
+          for (int n = 0; n < Np; ++n)
+            lp[((size_t)f * KF + t) * NPP + n] = (float)lift[(size_t)n * NfpNf + f * Nfp + t];
+      dfree(c->LiftPad);
+      if ((rc = upload(c, &c->LiftPad, lp.data(), lp.size()))) return rc;
+    }
   }
   std::vector<double> v64((size_t)Nq * Np);
   for (size_t i = 0; i < v64.size(); ++i) v64[i] = o->Vq64 ? o->Vq64[i] : (double)o->Vq[i];

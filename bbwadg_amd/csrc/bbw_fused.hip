@@ -1,20 +1,23 @@
 // Fused acoustic stage for gfx950 with the WADG projection on FP32 MFMA.
 //
 // One workgroup (4 waves) owns a tile of TB = 16 consecutive elements:
-//   1  Q_in tile -> LDS (float4, coalesced), geometry; prefetched into registers: the thread's
-//      res and Q_in row of every tile element, c2 in the MFMA output layout, vmapP entries
-//   2  (p, Ur, Us, Ut) as one float4 per node; neighbour traces gathered into registers
-//   3  volume terms (4-nnz barycentric operators, 16 float4 LDS reads per node) -> rhs
-//   4  upwind flux at face nodes -> float2 per face node
-//   5  L0 per face, stored as (p, U nx, U ny, U nz) per face node       [row in registers]
-//   6  EEL lift: one float4 LDS read per non-zero                        [row in registers]
-//   7  S = (Vq . rhs_p) * c2      MFMA 16x16x4, M = cubature points, N = 16 elements, K = Np
-//   8  rhs_p = Pq . S             MFMA 16x16x4, M = nodes, N = 16 elements, K = Nq
-//   9  LSRK: res = fa res + fdt rhs (+ source); Q_out = Q_in + fb res (coalesced rows)
+//   1  Q_in tile -> LDS (buffer loads, 16 B per lane); prefetched into registers: the thread's
+//      res and Q_in row of every tile element, c2 in the MFMA output layout, neighbour traces
+//   2  (p, Ur, Us, Ut) as one float4 per node; upwind flux phi at face nodes into registers
+//      (the p and U fluxes are phi and -phi)
+//   3  phi -> LDS [face node][element]; volume terms (4-nnz barycentric operators, 16 float4
+//      LDS reads per node) into registers
+//   4  volume terms -> rhs (aliasing the (p, U) tile)
+//   5  lift: P_f = LIFT_f . phi_f per face on MFMA 16x16x4 (M = nodes, N = 16 elements,
+//      K = face nodes); rhs_p += sum_f P_f, rhs_U -= sum_f n_f P_f.  LIFT = EEL . blkdiag(L0)
+//      is applied densely: on the matrix cores the factored sparse form saves nothing
+//   6  S = (Vq . rhs_p) * c2      MFMA 16x16x4, M = cubature points, N = 16 elements, K = Np
+//   7  rhs_p = Pq . S             MFMA 16x16x4, M = nodes, N = 16 elements, K = Nq
+//   8  LSRK: res = fa res + fdt rhs (+ source); Q_out = Q_in + fb res (coalesced rows)
 // Thread mappings are fixed per thread (node, face node or state row) and loop over the
 // tile's elements, so operator rows are loaded once per tile and no index is divided.
-// LDS regions are reused across phases (Q_in tile -> lifted face data -> S; (p, U) -> flux)
-// to stay at 52 KB per workgroup (N = 5), three workgroups per CU.
+// LDS regions are reused across phases (Q_in tile -> face flux -> S; (p, U) -> rhs) to stay
+// at 30 KB per workgroup (N = 5).
 // With PROJ_U the velocity fields go through Pq . Vq as well (N = 64 columns).
 //
 // v_mfma_f32_16x16x4f32 lane layout (checked on gfx950): A[i][k] from lane i + 16k,
@@ -23,8 +26,13 @@
 
 namespace bbw {
 
+// workgroups per CU the register allocation is sized for (LDS allows 5 at N = 5)
+#ifndef BBW_FUSED_WG_PER_CU
+#define BBW_FUSED_WG_PER_CU 3
+#endif
+
 template <int N, bool PROJ_U>
-__global__ __launch_bounds__(256) void k_fused_mfma(FusedMesh m, FusedOps o, FusedShot sh,
+__global__ __launch_bounds__(256, BBW_FUSED_WG_PER_CU) void k_fused_mfma(FusedMesh m, FusedOps o, FusedShot sh,
                                                     SrcArgs src, float fa, float fb,
                                                     const float *ftime_dev) {
   using L = FusedLayout<N, PROJ_U>;
@@ -48,6 +56,7 @@ FusedOps fused_ops(const bbw_ctx *c) {
   o.L0_vals = c->L0_vals;
   o.VqPad = c->VqPad;
   o.PqPad = c->PqPad;
+  o.LiftPad = c->LiftPad;
   return o;
 }
 

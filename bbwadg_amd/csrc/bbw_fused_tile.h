@@ -34,19 +34,21 @@ struct FusedLayout {
   static constexpr int NQT = (Nq + 15) / 16, NQP = NQT * 16;
   static constexpr int NPT = (Np + 15) / 16, NPP = NPT * 16;
   static constexpr int KTV = (Np + 3) / 4, KTP = NQP / 4;
+  static constexpr int KF = 4 * ((Nfp + 3) / 4); // face nodes per face, padded for MFMA K
+  static constexpr int KB = NfpNf + 4;           // rows of the face-flux tile (zero tail)
   static constexpr int NP = PROJ_U ? 4 : 1;
   static constexpr int SX = NP * TB;             // S row length
   static constexpr int cmax(int a, int b) { return a > b ? a : b; }
-  // W1: Q_in tile [TB][4Np] | lifted face data float4 [TB][NfpNf] | S [NQP][SX]
-  static constexpr int W1 = cmax(cmax(4 * Np, 4 * NfpNf), NQP * NP);
-  // W2: (p, Ur, Us, Ut) float4 [TB][Np] | flux float2 [TB][NfpNf]
-  static constexpr int W2 = cmax(4 * Np, 2 * NfpNf);
+  // W1: Q_in tile [TB][4Np] | face flux [KB][TB] | S [NQP][SX]
+  static constexpr int W1 = cmax(cmax(4 * Np, KB), NQP * NP);
+  // W2: (p, Ur, Us, Ut) float4 [TB][Np] | rhs [TB][RS]
+  static constexpr int W2 = cmax(4 * Np, RS);
   static constexpr int QTW = (NQT + 3) / 4;     // q tiles per wave
   static constexpr int GS = ((Np + 63) / 64) * 64, NG = 256 / GS;        // node groups
   static constexpr int NG3 = 256 / NfpNf, EPT3 = (TB + NG3 - 1) / NG3;   // face-node groups
   static constexpr int EPTV = (TB + NG - 1) / NG;
   static constexpr bool PF = 4 * Np <= 256;    // one state row per thread: prefetch res, Q_in
-  static constexpr size_t floats = (size_t)TB * (W1 + W2 + RS + 9 + 16) + TB;
+  static constexpr size_t floats = (size_t)TB * (W1 + W2 + 9 + 16) + TB;
 };
 
 // Operator, mesh and per-shot arguments of one tile.
@@ -60,6 +62,7 @@ struct FusedOps {
   const float *L0_vals;
   const float *VqPad;
   const float *PqPad;
+  const float *LiftPad;
 };
 
 struct FusedMesh {
@@ -118,7 +121,7 @@ __device__ __forceinline__ void tile_geometry(const FusedMesh &m, const FusedOps
   using L = FL<N, PROJ_U>;
   constexpr int NfpNf = L::NfpNf, TB = L::TB;
   static_assert(TB * 16 <= 256, "one geometry word per thread");
-  float *sG = lds + TB * (L::W1 + L::W2 + L::RS);
+  float *sG = lds + TB * (L::W1 + L::W2);
   float *sF = sG + TB * 9;
   const brsrc rg = mkrsrc(m.vgeo + (size_t)k0 * 9, (unsigned)nk * 36u);
   const brsrc rf = mkrsrc(m.fgeo + (size_t)k0 * 16, (unsigned)nk * 64u);
@@ -133,9 +136,10 @@ __device__ __forceinline__ void tile_geometry(const FusedMesh &m, const FusedOps
   for (int i = 0; i < L::EPT3; ++i) vm[i] = bldi(rv, vo + i * L::NG3 * NfpNf * 4u);
 }
 
-// One fused LSRK stage of one shot on one tile (phases 1-9 above).  Ends with a barrier so
-// the caller may process another shot on the same tile.  Elements past the tile's end are
-// computed on zeros and never stored.  Requires K * 4 * Np * 4 < 2^32 (32-bit trace offsets).
+// One fused LSRK stage of one shot on one tile (phases 1-8 of bbw_fused.hip).  Ends with a
+// barrier so the caller may process another shot on the same tile.  Elements past the tile's
+// end are computed on zeros and never stored.  Requires K * 4 * Np * 4 < 2^32 (32-bit trace
+// offsets).
 template <int N, bool PROJ_U>
 __device__ __forceinline__ void tile_shot(const FusedMesh &m, const FusedOps &o, const FusedShot &sh,
                                           const SrcArgs &src, float fa, float fb, int k0, int nk,
@@ -143,27 +147,22 @@ __device__ __forceinline__ void tile_shot(const FusedMesh &m, const FusedOps &o,
                                           int fid3, const int tid) {
   using L = FusedLayout<N, PROJ_U>;
   constexpr int Np = L::Np, Nfp = L::Nfp, NfpNf = L::NfpNf, Nq = L::Nq, TB = L::TB, RS = L::RS;
-  constexpr int NpNf = 4 * Np, EEL_nnz = Dims<N>::EEL_nnz, L0_nnz = Dims<N>::L0_nnz;
+  constexpr int NpNf = 4 * Np, KF = L::KF, KB = L::KB;
   constexpr int NP = L::NP, SX = L::SX, NQP = L::NQP, NPP = L::NPP;
   constexpr unsigned ROW = NpNf * 4u;    // bytes of one element's state
   const float fdt = sh.fdt;
   const int4 *__restrict__ Dids = o.Dids;
   const float4 *__restrict__ Dvals = o.Dvals;
-  const int *__restrict__ EEL_ids4 = o.EEL_ids4;
-  const float *__restrict__ EEL_vals = o.EEL_vals;
-  const int *__restrict__ L0_ids = o.L0_ids;
-  const float *__restrict__ L0_vals = o.L0_vals;
   float *W1 = lds;                       // [TB * W1]
   float *W2 = W1 + TB * L::W1;           // [TB * W2]
-  float *sR = W2 + TB * L::W2;           // [TB][RS]
-  float *sG = sR + TB * RS;              // [TB][9]
+  float *sG = W2 + TB * L::W2;           // [TB][9]
   float *sF = sG + TB * 9;               // [TB][16]
   int *sSlot = (int *)(sF + TB * 16);
-  float *sQ = W1;                        // phases 1-4
-  float4 *sL = (float4 *)W1;             // phases 5-6
-  float *S = W1;                         // phases 7-8
+  float *sQ = W1;                        // phases 1-2
+  float *sB = W1;                        // phases 3-5: face flux [KB][TB]
+  float *S = W1;                         // phases 6-7
   float4 *sV = (float4 *)W2;             // phases 2-3
-  float2 *sFl = (float2 *)W2;            // phases 4-5
+  float *sR = W2;                        // phases 4-8: rhs [TB][RS]
   const int lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const size_t base = (size_t)k0 * NpNf;
   const unsigned tbytes = (unsigned)nk * ROW;
@@ -172,17 +171,17 @@ __device__ __forceinline__ void tile_shot(const FusedMesh &m, const FusedOps &o,
   const brsrc rO = mkrsrc(sh.Qout + base, tbytes);
   const brsrc rC = mkrsrc(sh.c2q + (size_t)k0 * Nq, (unsigned)nk * Nq * 4u);
   const brsrc rT = mkrsrc(sh.Qin, (unsigned)m.K * ROW);   // neighbour traces
-  // face-node mapping (phases 2, 4, 5)
+  // face-node mapping (phases 2, 3)
   const int g3 = tid / NfpNf, m3 = tid - g3 * NfpNf;
   const bool on3 = g3 < L::NG3;
   const int f3 = m3 / Nfp;
-  // node mapping (phases 2, 3, 6)
+  // node mapping (phases 2, 3, 4)
   const int gv = tid / L::GS, nv = tid - gv * L::GS;
   const bool onv = gv < L::NG && nv < Np;
   // source: elements are sorted, so a tile outside [first, last] has none (uniform test)
   const bool has_src = src.nsrc > 0 && k0 + nk - 1 >= src.first && k0 <= src.last;
 
-  // ---- 1: tile loads and register prefetch
+  // ---- 1: tile loads; register prefetch of res, Q_in rows, c2, neighbour traces
   {
     constexpr unsigned QB = TB * ROW, QIT = (QB + 4095) / 4096;
     constexpr bool QFIT = QIT * 4096 <= TB * L::W1 * 4u;
@@ -192,16 +191,12 @@ __device__ __forceinline__ void tile_shot(const FusedMesh &m, const FusedOps &o,
       const unsigned off = tid * 16u + it * 4096u;
       if (QFIT || off < QB) sQ4[off / 16] = bld4(rQ, off);
     }
-    if (tid < TB * 3) sR[(tid / 3) * RS + NpNf + tid % 3] = 0.f;   // K padding of the MFMA
     if (has_src && tid < TB) sSlot[tid] = tid < nk ? src_slot2(src, k0 + tid) : -1;
   }
   float qreg[L::PF ? TB : 1], rreg[L::PF ? TB : 1];
   if constexpr (L::PF) {
 #pragma unroll
-    for (int e = 0; e < TB; ++e) {
-      qreg[e] = bld(rQ, tid * 4u + e * ROW);
-      rreg[e] = bld(rR, tid * 4u + e * ROW);
-    }
+    for (int e = 0; e < TB; ++e) rreg[e] = bld(rR, tid * 4u + e * ROW);
   }
   float c2r[L::QTW][4];
   {
@@ -214,9 +209,17 @@ __device__ __forceinline__ void tile_shot(const FusedMesh &m, const FusedOps &o,
       for (int r = 0; r < 4; ++r) c2r[t][r] = qt < L::NQT ? bld(rC, co + (qt * 16 + r) * 4u) : 0.f;
     }
   }
+  float tr[L::EPT3][4];
+#pragma unroll
+  for (int i = 0; i < L::EPT3; ++i) {
+    const unsigned to = (unsigned)vm[i] * 4u;
+#pragma unroll
+    for (int f = 0; f < 4; ++f) tr[i][f] = bld(rT, to + f * Np * 4u);
+  }
   __syncthreads();
 
-  // ---- 2: (p, Ur, Us, Ut) per node; neighbour traces into registers
+  // ---- 2: (p, Ur, Us, Ut) per node; upwind flux at face nodes into registers.  The p and U
+  // fluxes are (phi, -phi) with phi = (pjump - Unjump) Fscale / 2, so one value per face node.
   if (onv) {
 #pragma unroll
     for (int i = 0; i < L::EPTV; ++i) {
@@ -231,64 +234,12 @@ __device__ __forceinline__ void tile_shot(const FusedMesh &m, const FusedOps &o,
       }
     }
   }
-  float tr[L::EPT3][4];
-#pragma unroll
-  for (int i = 0; i < L::EPT3; ++i) {
-    const unsigned to = (unsigned)vm[i] * 4u;
-#pragma unroll
-    for (int f = 0; f < 4; ++f) tr[i][f] = bld(rT, to + f * Np * 4u);
-  }
-  __syncthreads();
-
-  // ---- 3: volume terms (operand addresses fixed per thread, element offsets immediate)
-  if (onv) {
-    const float4 dv = Dvals[nv];
-    const int4 D1 = Dids[nv], D2 = Dids[Np + nv], D3 = Dids[2 * Np + nv], D4 = Dids[3 * Np + nv];
-    const float4 *Vb = sV + gv * Np;
-    const float4 *pa1 = Vb + D1.x, *pa2 = Vb + D2.x, *pa3 = Vb + D3.x, *pa4 = Vb + D4.x;
-    const float4 *pb1 = Vb + D1.y, *pb2 = Vb + D2.y, *pb3 = Vb + D3.y, *pb4 = Vb + D4.y;
-    const float4 *pc1 = Vb + D1.z, *pc2 = Vb + D2.z, *pc3 = Vb + D3.z, *pc4 = Vb + D4.z;
-    const float4 *pd1 = Vb + D1.w, *pd2 = Vb + D2.w, *pd3 = Vb + D3.w, *pd4 = Vb + D4.w;
-    float *rb = sR + gv * RS + nv;
-    const float *Gb = sG + 9 * gv;
-#pragma unroll
-    for (int i = 0; i < L::EPTV; ++i) {
-      const int e = gv + L::NG * i;
-      if (L::NG * L::EPTV == TB || e < TB) {
-        const int o4 = i * L::NG * Np;
-        const float4 a1 = pa1[o4], a2 = pa2[o4], a3 = pa3[o4], a4 = pa4[o4];
-        const float4 b1 = pb1[o4], b2 = pb2[o4], b3 = pb3[o4], b4 = pb4[o4];
-        const float4 c1 = pc1[o4], c2 = pc2[o4], c3 = pc3[o4], c4 = pc4[o4];
-        const float4 d1 = pd1[o4], d2 = pd2[o4], d3 = pd3[o4], d4 = pd4[o4];
-        const float p1 = dv.x * a1.x + dv.y * a2.x + dv.z * a3.x + dv.w * a4.x;
-        const float p2 = dv.x * b1.x + dv.y * b2.x + dv.z * b3.x + dv.w * b4.x;
-        const float p3 = dv.x * c1.x + dv.y * c2.x + dv.z * c3.x + dv.w * c4.x;
-        const float p4 = dv.x * d1.x + dv.y * d2.x + dv.z * d3.x + dv.w * d4.x;
-        float dU1 = dv.x * (b1.y + c1.z + d1.w);
-        dU1 += dv.y * (b2.y + c2.z + d2.w);
-        dU1 += dv.z * (b3.y + c3.z + d3.w);
-        dU1 += dv.w * (b4.y + c4.z + d4.w);
-        float dU2 = dv.x * (a1.y + a1.z + a1.w);
-        dU2 += dv.y * (a2.y + a2.z + a2.w);
-        dU2 += dv.z * (a3.y + a3.z + a3.w);
-        dU2 += dv.w * (a4.y + a4.z + a4.w);
-        const float dpdr = .5f * (p2 - p1), dpds = .5f * (p3 - p1), dpdt = .5f * (p4 - p1);
-        const float *G = Gb + 9 * L::NG * i;
-        float *r = rb + L::NG * i * RS;
-        r[0] = -.5f * (dU1 - dU2);
-        r[Np] = -(G[0] * dpdr + G[3] * dpds + G[6] * dpdt);
-        r[2 * Np] = -(G[1] * dpdr + G[4] * dpds + G[7] * dpdt);
-        r[3 * Np] = -(G[2] * dpdr + G[5] * dpds + G[8] * dpdt);
-      }
-    }
-  }
-  __syncthreads();
-
-  // ---- 4: upwind flux (own trace from the Q_in tile, neighbour trace from registers)
+  float phi[L::EPT3];
   if (on3) {
 #pragma unroll
     for (int i = 0; i < L::EPT3; ++i) {
       const int e = g3 + L::NG3 * i;
+      phi[i] = 0.f;
       if (L::NG3 * L::EPT3 == TB || e < TB) {
         const float *s = sQ + e * NpNf;
         const float *F = sF + 16 * e + 4 * f3;
@@ -302,78 +253,128 @@ __device__ __forceinline__ void tile_shot(const FusedMesh &m, const FusedOps &o,
           pjump = tr[i][0] - pM;
           Unjump = (tr[i][1] - uM) * F[1] + (tr[i][2] - vM) * F[2] + (tr[i][3] - wM) * F[3];
         }
-        sFl[e * NfpNf + m3] = make_float2(.5f * (pjump - Unjump) * F[0], .5f * (Unjump - pjump) * F[0]);
+        phi[i] = .5f * (pjump - Unjump) * F[0];
       }
     }
   }
   __syncthreads();
 
-  // ---- 5: L0 per face; store (p, U nx, U ny, U nz) so the lift needs no face lookup
+  // ---- 3: face flux -> LDS [face node][element]; volume terms into registers
   if (on3) {
-    const int nt = m3 - f3 * Nfp;
-    float Lv[L0_nnz];
-    const float2 *Xp[L0_nnz];
-    const float2 *Xb = sFl + g3 * NfpNf + f3 * Nfp;
-#pragma unroll
-    for (int j = 0; j < L0_nnz; ++j) {
-      Lv[j] = L0_vals[nt + j * Nfp];
-      Xp[j] = Xb + L0_ids[nt + j * Nfp];
-    }
 #pragma unroll
     for (int i = 0; i < L::EPT3; ++i) {
       const int e = g3 + L::NG3 * i;
-      if (L::NG3 * L::EPT3 == TB || e < TB) {
-        const int o2 = i * L::NG3 * NfpNf;
-        float v1 = 0.f, v2 = 0.f;
+      if (L::NG3 * L::EPT3 == TB || e < TB) sB[m3 * TB + e] = phi[i];
+    }
+  }
+  if (tid < (KB - NfpNf) * TB) sB[NfpNf * TB + tid] = 0.f;   // K padding of the last face
+  float vol[L::EPTV][4];
+  if (onv) {
+    const float4 dv = Dvals[nv];
+    const int4 D1 = Dids[nv], D2 = Dids[Np + nv], D3 = Dids[2 * Np + nv], D4 = Dids[3 * Np + nv];
+    const float4 *Vb = sV + gv * Np;
+    const float4 *pa1 = Vb + D1.x, *pa2 = Vb + D2.x, *pa3 = Vb + D3.x, *pa4 = Vb + D4.x;
+    const float4 *pb1 = Vb + D1.y, *pb2 = Vb + D2.y, *pb3 = Vb + D3.y, *pb4 = Vb + D4.y;
+    const float4 *pc1 = Vb + D1.z, *pc2 = Vb + D2.z, *pc3 = Vb + D3.z, *pc4 = Vb + D4.z;
+    const float4 *pd1 = Vb + D1.w, *pd2 = Vb + D2.w, *pd3 = Vb + D3.w, *pd4 = Vb + D4.w;
+    const float *Gb = sG + 9 * gv;
 #pragma unroll
-        for (int j = 0; j < L0_nnz; ++j) {
-          const float2 x = Xp[j][o2];
-          v1 += Lv[j] * x.x;
-          v2 += Lv[j] * x.y;
-        }
-        const float *F = sF + 16 * e + 4 * f3;
-        sL[e * NfpNf + m3] = make_float4(v1, v2 * F[1], v2 * F[2], v2 * F[3]);
-      }
+    for (int i = 0; i < L::EPTV; ++i) {
+      const int o4 = i * L::NG * Np;
+      const float4 a1 = pa1[o4], a2 = pa2[o4], a3 = pa3[o4], a4 = pa4[o4];
+      const float4 b1 = pb1[o4], b2 = pb2[o4], b3 = pb3[o4], b4 = pb4[o4];
+      const float4 c1 = pc1[o4], c2 = pc2[o4], c3 = pc3[o4], c4 = pc4[o4];
+      const float4 d1 = pd1[o4], d2 = pd2[o4], d3 = pd3[o4], d4 = pd4[o4];
+      const float p1 = dv.x * a1.x + dv.y * a2.x + dv.z * a3.x + dv.w * a4.x;
+      const float p2 = dv.x * b1.x + dv.y * b2.x + dv.z * b3.x + dv.w * b4.x;
+      const float p3 = dv.x * c1.x + dv.y * c2.x + dv.z * c3.x + dv.w * c4.x;
+      const float p4 = dv.x * d1.x + dv.y * d2.x + dv.z * d3.x + dv.w * d4.x;
+      float dU1 = dv.x * (b1.y + c1.z + d1.w);
+      dU1 += dv.y * (b2.y + c2.z + d2.w);
+      dU1 += dv.z * (b3.y + c3.z + d3.w);
+      dU1 += dv.w * (b4.y + c4.z + d4.w);
+      float dU2 = dv.x * (a1.y + a1.z + a1.w);
+      dU2 += dv.y * (a2.y + a2.z + a2.w);
+      dU2 += dv.z * (a3.y + a3.z + a3.w);
+      dU2 += dv.w * (a4.y + a4.z + a4.w);
+      const float dpdr = .5f * (p2 - p1), dpds = .5f * (p3 - p1), dpdt = .5f * (p4 - p1);
+      const float *G = Gb + 9 * L::NG * i;
+      vol[i][0] = -.5f * (dU1 - dU2);
+      vol[i][1] = -(G[0] * dpdr + G[3] * dpds + G[6] * dpdt);
+      vol[i][2] = -(G[1] * dpdr + G[4] * dpds + G[7] * dpdt);
+      vol[i][3] = -(G[2] * dpdr + G[5] * dpds + G[8] * dpdt);
     }
   }
   __syncthreads();
 
-  // ---- 6: EEL lift into rhs (operand addresses fixed per thread)
+  // ---- 4: volume terms -> rhs (the (p, U) tile is no longer read)
   if (onv) {
-    float Ev[EEL_nnz];
-    const float4 *Ep[EEL_nnz];
-    const float *Lb = (const float *)(sL + gv * NfpNf);
-#pragma unroll
-    for (int j = 0; j < EEL_nnz; ++j) {
-      Ev[j] = EEL_vals[nv + j * Np];
-      Ep[j] = (const float4 *)(Lb + EEL_ids4[nv + j * Np]);
-    }
     float *rb = sR + gv * RS + nv;
 #pragma unroll
     for (int i = 0; i < L::EPTV; ++i) {
       const int e = gv + L::NG * i;
       if (L::NG * L::EPTV == TB || e < TB) {
-        const int o4 = i * L::NG * NfpNf;
-        float v1 = 0.f, v2 = 0.f, v3 = 0.f, v4 = 0.f;
-#pragma unroll
-        for (int j = 0; j < EEL_nnz; ++j) {
-          const float4 x = Ep[j][o4];
-          v1 += Ev[j] * x.x;
-          v2 += Ev[j] * x.y;
-          v3 += Ev[j] * x.z;
-          v4 += Ev[j] * x.w;
-        }
         float *r = rb + L::NG * i * RS;
-        r[0] += v1;
-        r[Np] += v2;
-        r[2 * Np] += v3;
-        r[3 * Np] += v4;
+        r[0] = vol[i][0];
+        r[Np] = vol[i][1];
+        r[2 * Np] = vol[i][2];
+        r[3 * Np] = vol[i][3];
+      }
+    }
+  }
+  if (tid < TB * 3) sR[(tid / 3) * RS + NpNf + tid % 3] = 0.f;   // K padding of the MFMA
+  __syncthreads();
+
+  // ---- 5: lift on MFMA, one product per face (M = nodes, N = 16 elements, K = face nodes):
+  // P_f = LIFT_f phi_f;  rhs_p += sum_f P_f,  rhs_U -= sum_f n_f P_f
+  {
+    const int col = lane & 15, kr = lane >> 4;
+    const brsrc rA = mkrsrc(o.LiftPad, 4u * KF * NPP * 4u);
+    const unsigned ao = (unsigned)(kr * NPP + col) * 4u;
+    const float *Bp = sB + kr * TB + col;
+    const float *Fe = sF + 16 * col;
+    for (int mt = wave; mt < L::NPT; mt += 4) {
+      f4 acc[4];
+#pragma unroll
+      for (int f = 0; f < 4; ++f) {
+        acc[f] = f4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int kt = 0; kt < KF / 4; ++kt) {
+          const float a = bld_s(rA, ao, (unsigned)((f * KF + 4 * kt) * NPP + mt * 16) * 4u);
+          acc[f] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, Bp[(f * Nfp + 4 * kt) * TB], acc[f], 0, 0, 0);
+        }
+      }
+      float nx[4], ny[4], nz[4];
+#pragma unroll
+      for (int f = 0; f < 4; ++f) {
+        nx[f] = Fe[4 * f + 1];
+        ny[f] = Fe[4 * f + 2];
+        nz[f] = Fe[4 * f + 3];
+      }
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int n = mt * 16 + 4 * kr + r;
+        if (n < Np) {
+          float dp = 0.f, du = 0.f, dv = 0.f, dw = 0.f;
+#pragma unroll
+          for (int f = 0; f < 4; ++f) {
+            dp += acc[f][r];
+            du += nx[f] * acc[f][r];
+            dv += ny[f] * acc[f][r];
+            dw += nz[f] * acc[f][r];
+          }
+          float *rr = sR + col * RS + n;
+          rr[0] += dp;
+          rr[Np] -= du;
+          rr[2 * Np] -= dv;
+          rr[3 * Np] -= dw;
+        }
       }
     }
   }
   __syncthreads();
 
-  // ---- 7: S = (Vq . rhs) x c2 on MFMA (A from the zero-padded Vq, fixed offsets)
+  // ---- 6: S = (Vq . rhs) x c2 on MFMA (A from the zero-padded Vq, fixed offsets)
   {
     const int col = lane & 15, kr = lane >> 4;
     const float *Bp = sR + col * RS + kr;
@@ -413,7 +414,11 @@ __device__ __forceinline__ void tile_shot(const FusedMesh &m, const FusedOps &o,
   }
   __syncthreads();
 
-  // ---- 8: rhs = Pq . S on MFMA
+  // ---- 7: rhs = Pq . S on MFMA; Q_in rows for phase 8 read again (cached since phase 1)
+  if constexpr (L::PF) {
+#pragma unroll
+    for (int e = 0; e < TB; ++e) qreg[e] = bld(rQ, tid * 4u + e * ROW);
+  }
   {
     const int col = lane & 15, kr = lane >> 4;
     const brsrc rA = mkrsrc(o.PqPad, (unsigned)NQP * NPP * 4u);
@@ -442,7 +447,7 @@ __device__ __forceinline__ void tile_shot(const FusedMesh &m, const FusedOps &o,
   }
   __syncthreads();
 
-  // ---- 9: LSRK stage: res = fa res + fdt rhs (+ source); Q_out = Q_in + fb res
+  // ---- 8: LSRK stage: res = fa res + fdt rhs (+ source); Q_out = Q_in + fb res
   const float ft = sh.ft;
   if constexpr (L::PF) {
     if (tid < NpNf) {

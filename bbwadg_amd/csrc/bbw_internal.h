@@ -66,6 +66,9 @@ struct bbw_ctx {
   float *PqTPad = nullptr, *VqTPad = nullptr;
   // EEL column ids as float4 slots and L0 ids with the face offset folded in, for the fused kernel
   int *EEL_ids4 = nullptr;
+  // the lift EEL . blkdiag(L0, L0, L0, L0) per face, zero-padded for the MFMA kernels:
+  // LiftPad [4 faces][KF = 4*ceil(Nfp/4) face nodes][NPP] (node contiguous)
+  float *LiftPad = nullptr;
   double *Vq64 = nullptr, *wq64 = nullptr;
   double rk4a[5] = {0}, rk4b[5] = {0}, rk4c[6] = {0};
   bool have_ops = false;
