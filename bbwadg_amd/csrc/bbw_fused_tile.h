@@ -30,17 +30,21 @@ struct FusedLayout {
   static constexpr int Np = Dims<N>::Np, Nfp = Dims<N>::Nfp, NfpNf = Dims<N>::NfpNf;
   static constexpr int Nq = Dims<N>::Nq;
   static constexpr int TB = 16;                  // elements per tile = MFMA N
-  static constexpr int RS = 4 * Np + 3;          // odd rhs row stride; covers the K padding
   static constexpr int NQT = (Nq + 15) / 16, NQP = NQT * 16;
   static constexpr int NPT = (Np + 15) / 16, NPP = NPT * 16;
   static constexpr int KTV = (Np + 3) / 4, KTP = NQP / 4;
+  static constexpr int NP = PROJ_U ? 4 : 1;
+  // rhs row stride: 4 fields plus the K padding the Vq product reads, rounded up to 2 mod 32
+  // so that the MFMA operand reads (16 rows x 2 consecutive words per 32 lanes) hit 32 banks
+  static constexpr int RS0 = 4 * Np > (NP - 1) * Np + 4 * KTV ? 4 * Np : (NP - 1) * Np + 4 * KTV;
+  static constexpr int RS = RS0 + ((34 - RS0 % 32) % 32);
   static constexpr int KF = 4 * ((Nfp + 3) / 4); // face nodes per face, padded for MFMA K
   static constexpr int KB = NfpNf + 4;           // rows of the face-flux tile (zero tail)
-  static constexpr int NP = PROJ_U ? 4 : 1;
+  static constexpr int BS = TB + 1;              // its row stride: column writes spread over banks
   static constexpr int SX = NP * TB;             // S row length
   static constexpr int cmax(int a, int b) { return a > b ? a : b; }
-  // W1: Q_in tile [TB][4Np] | face flux [KB][TB] | S [NQP][SX]
-  static constexpr int W1 = cmax(cmax(4 * Np, KB), NQP * NP);
+  // W1: Q_in tile [TB][4Np] | face flux [KB][BS] | S [NQP][SX]
+  static constexpr int W1 = cmax(cmax(4 * Np, (KB * BS + TB - 1) / TB), NQP * NP);
   // W2: (p, Ur, Us, Ut) float4 [TB][Np] | rhs [TB][RS]
   static constexpr int W2 = cmax(4 * Np, RS);
   static constexpr int QTW = (NQT + 3) / 4;     // q tiles per wave
@@ -147,7 +151,7 @@ __device__ __forceinline__ void tile_shot(const FusedMesh &m, const FusedOps &o,
                                           int fid3, const int tid) {
   using L = FusedLayout<N, PROJ_U>;
   constexpr int Np = L::Np, Nfp = L::Nfp, NfpNf = L::NfpNf, Nq = L::Nq, TB = L::TB, RS = L::RS;
-  constexpr int NpNf = 4 * Np, KF = L::KF, KB = L::KB;
+  constexpr int NpNf = 4 * Np, KF = L::KF, KB = L::KB, BS = L::BS;
   constexpr int NP = L::NP, SX = L::SX, NQP = L::NQP, NPP = L::NPP;
   constexpr unsigned ROW = NpNf * 4u;    // bytes of one element's state
   const float fdt = sh.fdt;
@@ -159,7 +163,7 @@ __device__ __forceinline__ void tile_shot(const FusedMesh &m, const FusedOps &o,
   float *sF = sG + TB * 9;               // [TB][16]
   int *sSlot = (int *)(sF + TB * 16);
   float *sQ = W1;                        // phases 1-2
-  float *sB = W1;                        // phases 3-5: face flux [KB][TB]
+  float *sB = W1;                        // phases 3-5: face flux [KB][BS]
   float *S = W1;                         // phases 6-7
   float4 *sV = (float4 *)W2;             // phases 2-3
   float *sR = W2;                        // phases 4-8: rhs [TB][RS]
@@ -264,10 +268,10 @@ __device__ __forceinline__ void tile_shot(const FusedMesh &m, const FusedOps &o,
 #pragma unroll
     for (int i = 0; i < L::EPT3; ++i) {
       const int e = g3 + L::NG3 * i;
-      if (L::NG3 * L::EPT3 == TB || e < TB) sB[m3 * TB + e] = phi[i];
+      if (L::NG3 * L::EPT3 == TB || e < TB) sB[m3 * BS + e] = phi[i];
     }
   }
-  if (tid < (KB - NfpNf) * TB) sB[NfpNf * TB + tid] = 0.f;   // K padding of the last face
+  if (tid < (KB - NfpNf) * BS) sB[NfpNf * BS + tid] = 0.f;   // K padding of the last face
   float vol[L::EPTV][4];
   if (onv) {
     const float4 dv = Dvals[nv];
@@ -322,7 +326,9 @@ __device__ __forceinline__ void tile_shot(const FusedMesh &m, const FusedOps &o,
       }
     }
   }
-  if (tid < TB * 3) sR[(tid / 3) * RS + NpNf + tid % 3] = 0.f;   // K padding of the MFMA
+  constexpr int KPAD = L::RS0 - NpNf;      // rhs words past the 4 fields that the Vq product reads
+  static_assert(TB * KPAD <= 256, "one padding word per thread");
+  if (KPAD > 0 && tid < TB * KPAD) sR[(tid / KPAD) * RS + NpNf + tid % KPAD] = 0.f;
   __syncthreads();
 
   // ---- 5: lift on MFMA, one product per face (M = nodes, N = 16 elements, K = face nodes):
@@ -331,7 +337,7 @@ __device__ __forceinline__ void tile_shot(const FusedMesh &m, const FusedOps &o,
     const int col = lane & 15, kr = lane >> 4;
     const brsrc rA = mkrsrc(o.LiftPad, 4u * KF * NPP * 4u);
     const unsigned ao = (unsigned)(kr * NPP + col) * 4u;
-    const float *Bp = sB + kr * TB + col;
+    const float *Bp = sB + kr * BS + col;
     const float *Fe = sF + 16 * col;
     for (int mt = wave; mt < L::NPT; mt += 4) {
       f4 acc[4];
@@ -341,7 +347,7 @@ __device__ __forceinline__ void tile_shot(const FusedMesh &m, const FusedOps &o,
 #pragma unroll
         for (int kt = 0; kt < KF / 4; ++kt) {
           const float a = bld_s(rA, ao, (unsigned)((f * KF + 4 * kt) * NPP + mt * 16) * 4u);
-          acc[f] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, Bp[(f * Nfp + 4 * kt) * TB], acc[f], 0, 0, 0);
+          acc[f] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, Bp[(f * Nfp + 4 * kt) * BS], acc[f], 0, 0, 0);
         }
       }
       float nx[4], ny[4], nz[4];
