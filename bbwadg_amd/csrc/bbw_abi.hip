@@ -5,6 +5,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <functional>
 #include <vector>
 
 #include "bbw_internal.h"
@@ -156,7 +157,7 @@ void bbw_destroy(bbw_ctx *c) {
   if (c->stream) hipStreamSynchronize(c->stream);
   reset_graph(c);
   dfree(c->Dids); dfree(c->Dvals); dfree(c->Fmask); dfree(c->EEL_ids); dfree(c->L0_ids);
-  dfree(c->EEL_vals); dfree(c->L0_vals); dfree(c->Vq); dfree(c->Pq); dfree(c->Vq64); dfree(c->VqPad); dfree(c->PqPad); dfree(c->EEL_ids4); dfree(c->LiftPad); dfree(c->PqTPad); dfree(c->VqTPad);
+  dfree(c->EEL_vals); dfree(c->L0_vals); dfree(c->Vq); dfree(c->Pq); dfree(c->Vq64); dfree(c->VqPad); dfree(c->PqPad); dfree(c->EEL_ids4); dfree(c->LiftPad); dfree(c->VqBlk); dfree(c->PqBlk); dfree(c->LiftBlk); dfree(c->PqTPad); dfree(c->VqTPad);
   dfree(c->DTids); dfree(c->DTvals); dfree(c->EELT_ids); dfree(c->EELT_vals); dfree(c->L0T_ids);
   dfree(c->L0T_vals); dfree(c->FmaskInv); dfree(c->mapP); dfree(c->lQ); dfree(c->lres);
   dfree(c->Zadj); dfree(c->Hadj); dfree(c->grad); dfree(c->Ybuf); dfree(c->ckpt);
@@ -325,6 +326,31 @@ int bbw_upload_operators(bbw_ctx *c, const bbw_ops *o) {
             lp[((size_t)f * KF + t) * NPP + n] = (float)lift[(size_t)n * NfpNf + f * Nfp + t];
       dfree(c->LiftPad);
       if ((rc = upload(c, &c->LiftPad, lp.data(), lp.size()))) return rc;
+      // lane-ordered A operand blocks (see bbw_ctx::VqBlk)
+      auto blocks = [](int M, int K, const std::function<float(int, int)> &A, std::vector<float> &out) {
+        const int MT = (M + 15) / 16, KT = (K + 3) / 4, KB = (KT + 3) / 4;
+        out.assign((size_t)MT * KB * 256, 0.f);
+        for (int t = 0; t < MT; ++t)
+          for (int b = 0; b < KB; ++b)
+            for (int lane = 0; lane < 64; ++lane)
+              for (int sb = 0; sb < 4; ++sb) {
+                const int i = 16 * t + lane % 16, k = 4 * (4 * b + sb) + lane / 16;
+                if (i < M && k < K) out[(((size_t)t * KB + b) * 64 + lane) * 4 + sb] = A(i, k);
+              }
+      };
+      std::vector<float> vb, pb, lb, lf;
+      blocks(Nq, Np, [&](int q, int j) { return o->Vq[q + (size_t)j * Nq]; }, vb);
+      blocks(Np, Nq, [&](int n, int q) { return o->Pq[n + (size_t)q * Np]; }, pb);
+      for (int f = 0; f < 4; ++f) {
+        blocks(Np, Nfp, [&](int n, int t) { return (float)lift[(size_t)n * NfpNf + f * Nfp + t]; }, lf);
+        lb.insert(lb.end(), lf.begin(), lf.end());
+      }
+      dfree(c->VqBlk);
+      dfree(c->PqBlk);
+      dfree(c->LiftBlk);
+      if ((rc = upload(c, &c->VqBlk, vb.data(), vb.size()))) return rc;
+      if ((rc = upload(c, &c->PqBlk, pb.data(), pb.size()))) return rc;
+      if ((rc = upload(c, &c->LiftBlk, lb.data(), lb.size()))) return rc;
     }
   }
   std::vector<double> v64((size_t)Nq * Np);

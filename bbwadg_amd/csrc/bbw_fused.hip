@@ -57,6 +57,9 @@ FusedOps fused_ops(const bbw_ctx *c) {
   o.VqPad = c->VqPad;
   o.PqPad = c->PqPad;
   o.LiftPad = c->LiftPad;
+  o.VqBlk = c->VqBlk;
+  o.PqBlk = c->PqBlk;
+  o.LiftBlk = c->LiftBlk;
   return o;
 }
 

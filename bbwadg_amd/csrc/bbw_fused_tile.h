@@ -67,6 +67,7 @@ struct FusedOps {
   const float *VqPad;
   const float *PqPad;
   const float *LiftPad;
+  const float *VqBlk, *PqBlk, *LiftBlk;   // lane-ordered A operands (bbw_ctx::VqBlk)
 };
 
 struct FusedMesh {
@@ -110,6 +111,10 @@ __device__ __forceinline__ float4 bld4(brsrc r, unsigned off) {
 // scalar offset (which the range check does not cover)
 __device__ __forceinline__ float bld_s(brsrc r, unsigned voff, unsigned soff) {
   return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
+}
+// 4 consecutive K-steps of an MFMA A operand (lane-ordered blocks, always in range)
+__device__ __forceinline__ f4 bld4_s(brsrc r, unsigned voff, unsigned soff) {
+  return __builtin_bit_cast(f4, __builtin_amdgcn_raw_buffer_load_b128(r, voff, soff, 0));
 }
 __device__ __forceinline__ void bst(brsrc r, float v, unsigned off) {
   __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, off, 0, 0);
@@ -335,8 +340,8 @@ __device__ __forceinline__ void tile_shot(const FusedMesh &m, const FusedOps &o,
   // P_f = LIFT_f phi_f;  rhs_p += sum_f P_f,  rhs_U -= sum_f n_f P_f
   {
     const int col = lane & 15, kr = lane >> 4;
-    const brsrc rA = mkrsrc(o.LiftPad, 4u * KF * NPP * 4u);
-    const unsigned ao = (unsigned)(kr * NPP + col) * 4u;
+    constexpr int KTL = KF / 4, KBL = (KTL + 3) / 4;
+    const brsrc rA = mkrsrc(o.LiftBlk, 4u * L::NPT * KBL * 1024u);
     const float *Bp = sB + kr * BS + col;
     const float *Fe = sF + 16 * col;
     for (int mt = wave; mt < L::NPT; mt += 4) {
@@ -345,9 +350,14 @@ __device__ __forceinline__ void tile_shot(const FusedMesh &m, const FusedOps &o,
       for (int f = 0; f < 4; ++f) {
         acc[f] = f4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-        for (int kt = 0; kt < KF / 4; ++kt) {
-          const float a = bld_s(rA, ao, (unsigned)((f * KF + 4 * kt) * NPP + mt * 16) * 4u);
-          acc[f] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, Bp[(f * Nfp + 4 * kt) * BS], acc[f], 0, 0, 0);
+        for (int b = 0; b < KBL; ++b) {
+          const f4 a = bld4_s(rA, lane * 16u, (unsigned)((f * L::NPT + mt) * KBL + b) * 1024u);
+#pragma unroll
+          for (int sb = 0; sb < 4; ++sb) {
+            const int kt = 4 * b + sb;
+            if (kt < KTL)
+              acc[f] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[sb], Bp[(f * Nfp + 4 * kt) * BS], acc[f], 0, 0, 0);
+          }
         }
       }
       float nx[4], ny[4], nz[4];
@@ -384,7 +394,8 @@ __device__ __forceinline__ void tile_shot(const FusedMesh &m, const FusedOps &o,
   {
     const int col = lane & 15, kr = lane >> 4;
     const float *Bp = sR + col * RS + kr;
-    const brsrc rA = mkrsrc(o.VqPad, 4u * L::KTV * NQP * 4u);
+    constexpr int KBV = (L::KTV + 3) / 4;
+    const brsrc rA = mkrsrc(o.VqBlk, (unsigned)L::NQT * KBV * 1024u);
     const brsrc rY = mkrsrc(sh.yout ? sh.yout + (size_t)k0 * Nq : nullptr, sh.yout ? (unsigned)nk * Nq * 4u : 0u);
 #pragma unroll
     for (int t = 0; t < L::QTW; ++t) {
@@ -393,13 +404,18 @@ __device__ __forceinline__ void tile_shot(const FusedMesh &m, const FusedOps &o,
       f4 acc[NP];
 #pragma unroll
       for (int b = 0; b < NP; ++b) acc[b] = f4{0.f, 0.f, 0.f, 0.f};
-      const unsigned ao = (unsigned)(kr * NQP + col) * 4u;
 #pragma unroll
-      for (int kt = 0; kt < L::KTV; ++kt) {
-        const float a = bld_s(rA, ao, (unsigned)(kt * 4 * NQP + qt * 16) * 4u);
+      for (int kb = 0; kb < KBV; ++kb) {
+        const f4 a = bld4_s(rA, lane * 16u, (unsigned)(qt * KBV + kb) * 1024u);
 #pragma unroll
-        for (int b = 0; b < NP; ++b)
-          acc[b] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, Bp[b * Np + kt * 4], acc[b], 0, 0, 0);
+        for (int sb = 0; sb < 4; ++sb) {
+          const int kt = 4 * kb + sb;
+          if (kt < L::KTV) {
+#pragma unroll
+            for (int b = 0; b < NP; ++b)
+              acc[b] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[sb], Bp[b * Np + kt * 4], acc[b], 0, 0, 0);
+          }
+        }
       }
       if (sh.yout) {
         // Vq . rhs_p at the cubature points, kept for the adjoint's c2 gradient
@@ -427,19 +443,25 @@ __device__ __forceinline__ void tile_shot(const FusedMesh &m, const FusedOps &o,
   }
   {
     const int col = lane & 15, kr = lane >> 4;
-    const brsrc rA = mkrsrc(o.PqPad, (unsigned)NQP * NPP * 4u);
+    constexpr int KBP = (L::KTP + 3) / 4;
+    const brsrc rA = mkrsrc(o.PqBlk, (unsigned)L::NPT * KBP * 1024u);
     for (int mt = wave; mt < L::NPT; mt += 4) {
       f4 acc[NP];
 #pragma unroll
       for (int b = 0; b < NP; ++b) acc[b] = f4{0.f, 0.f, 0.f, 0.f};
-      const unsigned ao = (unsigned)(kr * NPP + col) * 4u;
       const float *Bp = S + kr * SX + col;
-#pragma unroll 8
-      for (int kt = 0; kt < L::KTP; ++kt) {
-        const float a = bld_s(rA, ao, (unsigned)(kt * 4 * NPP + mt * 16) * 4u);
 #pragma unroll
-        for (int b = 0; b < NP; ++b)
-          acc[b] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, Bp[kt * 4 * SX + b * 16], acc[b], 0, 0, 0);
+      for (int kb = 0; kb < KBP; ++kb) {
+        const f4 a = bld4_s(rA, lane * 16u, (unsigned)(mt * KBP + kb) * 1024u);
+#pragma unroll
+        for (int sb = 0; sb < 4; ++sb) {
+          const int kt = 4 * kb + sb;
+          if (kt < L::KTP) {
+#pragma unroll
+            for (int b = 0; b < NP; ++b)
+              acc[b] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[sb], Bp[kt * 4 * SX + b * 16], acc[b], 0, 0, 0);
+          }
+        }
       }
 #pragma unroll
       for (int r = 0; r < 4; ++r) {

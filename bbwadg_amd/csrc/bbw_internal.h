@@ -69,6 +69,12 @@ struct bbw_ctx {
   // the lift EEL . blkdiag(L0, L0, L0, L0) per face, zero-padded for the MFMA kernels:
   // LiftPad [4 faces][KF = 4*ceil(Nfp/4) face nodes][NPP] (node contiguous)
   float *LiftPad = nullptr;
+  // the MFMA A operands of the fused kernel in lane order: for M tile t and K-step block b,
+  // 64 lanes x 4 consecutive K-steps, value A[16 t + (lane % 16)][4 (4 b + s) + lane / 16] at
+  // float ((t * KB + b) * 64 + lane) * 4 + s (zero outside A).  VqBlk: A = Vq (M = cubature
+  // points, K = nodes); PqBlk: A = Pq (M = nodes, K = cubature points); LiftBlk: per face f,
+  // A = LIFT_f (M = nodes, K = face nodes), face blocks of NPT * KB tiles
+  float *VqBlk = nullptr, *PqBlk = nullptr, *LiftBlk = nullptr;
   double *Vq64 = nullptr, *wq64 = nullptr;
   double rk4a[5] = {0}, rk4b[5] = {0}, rk4c[6] = {0};
   bool have_ops = false;
