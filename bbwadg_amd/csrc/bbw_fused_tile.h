@@ -334,29 +334,55 @@ __device__ __forceinline__ void tile_shot(const FusedMesh &m, const FusedOps &o,
   constexpr int KPAD = L::RS0 - NpNf;      // rhs words past the 4 fields that the Vq product reads
   static_assert(TB * KPAD <= 256, "one padding word per thread");
   if (KPAD > 0 && tid < TB * KPAD) sR[(tid / KPAD) * RS + NpNf + tid % KPAD] = 0.f;
+  // MFMA A operands are fetched one phase ahead of their products (for the wave's first M tile)
+  constexpr int KTL = KF / 4, KBL = (KTL + 3) / 4;
+  constexpr int KBV = (L::KTV + 3) / 4, KBP = (L::KTP + 3) / 4;
+  const brsrc rAL = mkrsrc(o.LiftBlk, 4u * L::NPT * KBL * 1024u);
+  const brsrc rAV = mkrsrc(o.VqBlk, (unsigned)L::NQT * KBV * 1024u);
+  const brsrc rAP = mkrsrc(o.PqBlk, (unsigned)L::NPT * KBP * 1024u);
+  f4 aL[4][KBL];
+  if (wave < L::NPT) {
+#pragma unroll
+    for (int f = 0; f < 4; ++f)
+#pragma unroll
+      for (int b = 0; b < KBL; ++b)
+        aL[f][b] = bld4_s(rAL, lane * 16u, (unsigned)((f * L::NPT + wave) * KBL + b) * 1024u);
+  }
   __syncthreads();
 
   // ---- 5: lift on MFMA, one product per face (M = nodes, N = 16 elements, K = face nodes):
   // P_f = LIFT_f phi_f;  rhs_p += sum_f P_f,  rhs_U -= sum_f n_f P_f
+  f4 aV[L::QTW][KBV];
+#pragma unroll
+  for (int t = 0; t < L::QTW; ++t)
+    if (wave + 4 * t < L::NQT) {
+#pragma unroll
+      for (int kb = 0; kb < KBV; ++kb)
+        aV[t][kb] = bld4_s(rAV, lane * 16u, (unsigned)((wave + 4 * t) * KBV + kb) * 1024u);
+    }
   {
     const int col = lane & 15, kr = lane >> 4;
-    constexpr int KTL = KF / 4, KBL = (KTL + 3) / 4;
-    const brsrc rA = mkrsrc(o.LiftBlk, 4u * L::NPT * KBL * 1024u);
     const float *Bp = sB + kr * BS + col;
     const float *Fe = sF + 16 * col;
     for (int mt = wave; mt < L::NPT; mt += 4) {
+      if (mt != wave) {
+#pragma unroll
+        for (int f = 0; f < 4; ++f)
+#pragma unroll
+          for (int b = 0; b < KBL; ++b)
+            aL[f][b] = bld4_s(rAL, lane * 16u, (unsigned)((f * L::NPT + mt) * KBL + b) * 1024u);
+      }
       f4 acc[4];
 #pragma unroll
       for (int f = 0; f < 4; ++f) {
         acc[f] = f4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
         for (int b = 0; b < KBL; ++b) {
-          const f4 a = bld4_s(rA, lane * 16u, (unsigned)((f * L::NPT + mt) * KBL + b) * 1024u);
 #pragma unroll
           for (int sb = 0; sb < 4; ++sb) {
             const int kt = 4 * b + sb;
             if (kt < KTL)
-              acc[f] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[sb], Bp[(f * Nfp + 4 * kt) * BS], acc[f], 0, 0, 0);
+              acc[f] = __builtin_amdgcn_mfma_f32_16x16x4f32(aL[f][b][sb], Bp[(f * Nfp + 4 * kt) * BS], acc[f], 0, 0, 0);
           }
         }
       }
@@ -391,11 +417,14 @@ __device__ __forceinline__ void tile_shot(const FusedMesh &m, const FusedOps &o,
   __syncthreads();
 
   // ---- 6: S = (Vq . rhs) x c2 on MFMA (A from the zero-padded Vq, fixed offsets)
+  f4 aP[KBP];
+  if (wave < L::NPT) {
+#pragma unroll
+    for (int kb = 0; kb < KBP; ++kb) aP[kb] = bld4_s(rAP, lane * 16u, (unsigned)(wave * KBP + kb) * 1024u);
+  }
   {
     const int col = lane & 15, kr = lane >> 4;
     const float *Bp = sR + col * RS + kr;
-    constexpr int KBV = (L::KTV + 3) / 4;
-    const brsrc rA = mkrsrc(o.VqBlk, (unsigned)L::NQT * KBV * 1024u);
     const brsrc rY = mkrsrc(sh.yout ? sh.yout + (size_t)k0 * Nq : nullptr, sh.yout ? (unsigned)nk * Nq * 4u : 0u);
 #pragma unroll
     for (int t = 0; t < L::QTW; ++t) {
@@ -406,14 +435,13 @@ __device__ __forceinline__ void tile_shot(const FusedMesh &m, const FusedOps &o,
       for (int b = 0; b < NP; ++b) acc[b] = f4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
       for (int kb = 0; kb < KBV; ++kb) {
-        const f4 a = bld4_s(rA, lane * 16u, (unsigned)(qt * KBV + kb) * 1024u);
 #pragma unroll
         for (int sb = 0; sb < 4; ++sb) {
           const int kt = 4 * kb + sb;
           if (kt < L::KTV) {
 #pragma unroll
             for (int b = 0; b < NP; ++b)
-              acc[b] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[sb], Bp[b * Np + kt * 4], acc[b], 0, 0, 0);
+              acc[b] = __builtin_amdgcn_mfma_f32_16x16x4f32(aV[t][kb][sb], Bp[b * Np + kt * 4], acc[b], 0, 0, 0);
           }
         }
       }
@@ -443,23 +471,24 @@ __device__ __forceinline__ void tile_shot(const FusedMesh &m, const FusedOps &o,
   }
   {
     const int col = lane & 15, kr = lane >> 4;
-    constexpr int KBP = (L::KTP + 3) / 4;
-    const brsrc rA = mkrsrc(o.PqBlk, (unsigned)L::NPT * KBP * 1024u);
     for (int mt = wave; mt < L::NPT; mt += 4) {
+      if (mt != wave) {
+#pragma unroll
+        for (int kb = 0; kb < KBP; ++kb) aP[kb] = bld4_s(rAP, lane * 16u, (unsigned)(mt * KBP + kb) * 1024u);
+      }
       f4 acc[NP];
 #pragma unroll
       for (int b = 0; b < NP; ++b) acc[b] = f4{0.f, 0.f, 0.f, 0.f};
       const float *Bp = S + kr * SX + col;
 #pragma unroll
       for (int kb = 0; kb < KBP; ++kb) {
-        const f4 a = bld4_s(rA, lane * 16u, (unsigned)(mt * KBP + kb) * 1024u);
 #pragma unroll
         for (int sb = 0; sb < 4; ++sb) {
           const int kt = 4 * kb + sb;
           if (kt < L::KTP) {
 #pragma unroll
             for (int b = 0; b < NP; ++b)
-              acc[b] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[sb], Bp[kt * 4 * SX + b * 16], acc[b], 0, 0, 0);
+              acc[b] = __builtin_amdgcn_mfma_f32_16x16x4f32(aP[kb][sb], Bp[kt * 4 * SX + b * 16], acc[b], 0, 0, 0);
           }
         }
       }
