@@ -129,6 +129,11 @@ int bbw_create(const bbw_config *cfg, bbw_ctx **out) {
     return BBW_ERR_HIP;
   }
   c->stream = c->own_stream;
+  if (hipDeviceGetAttribute(&c->ncu, hipDeviceAttributeMultiprocessorCount, c->dev) != hipSuccess || c->ncu <= 0) {
+    hipStreamDestroy(c->own_stream);
+    delete c;
+    return BBW_ERR_HIP;
+  }
   const size_t n = nstate(c);
   int rc = dalloc(c, &c->Q[0], n);
   if (!rc) rc = dalloc(c, &c->Q[1], n);

@@ -51,21 +51,26 @@ __device__ __forceinline__ T *sp(T *p) {
   return (T *)(GT *)(((unsigned long long)hi << 32) | lo);
 }
 
+#ifndef BBW_FUSED_WG_PER_CU
+#define BBW_FUSED_WG_PER_CU 3
+#endif
+
+// Persistent over work items: items blockIdx.x, + gridDim.x, ..., each with its shots in
+// order on the same tile (geometry and vmapP entries loaded once per item).
 template <int N>
-// three waves per SIMD (three workgroups per CU, as LDS allows): caps registers at 168
-__global__ __launch_bounds__(256, 3) void k_fused_batch(int nitems, const BatchItem *__restrict__ items,
-                                                     const FusedMesh *__restrict__ meshes,
-                                                     const BatchShotDev *__restrict__ shots,
-                                                     const int *__restrict__ list, FusedOps o,
-                                                     int stage, int par, float fa, float fb,
-                                                     const int *__restrict__ step) {
+__global__ __launch_bounds__(256, BBW_FUSED_WG_PER_CU) void k_fused_batch(
+    int nitems, const BatchItem *__restrict__ items, const FusedMesh *__restrict__ meshes,
+    const BatchShotDev *__restrict__ shots, const int *__restrict__ list, FusedOps o, int stage,
+    int par, float fa, float fb, const int *__restrict__ step) {
   using L = FusedLayout<N, false>;
   extern __shared__ float lds[];
-  const int st = *step;
+  const int stp = *step;
+  const int fid3 = face_node<N, false>(o, threadIdx.x);
 #pragma unroll 1
   for (int w = blockIdx.x; w < nitems; w += gridDim.x) {
-    // thread index laundered per item: address arithmetic derived from it is redone per tile
-    // instead of being hoisted out of the persistent loop and kept live (it would spill)
+    // thread index re-derived per item and shot: address arithmetic is redone per tile instead
+    // of being hoisted out of the loops and kept live across them (it would not fit the
+    // register budget)
     int tid;
     asm volatile("v_mov_b32 %0, %1" : "=v"(tid) : "v"((int)threadIdx.x));
     const int mesh = su(items[w].mesh), k0 = su(items[w].k0);
@@ -76,8 +81,8 @@ __global__ __launch_bounds__(256, 3) void k_fused_batch(int nitems, const BatchI
     m.fgeo = sp(meshes[mesh].fgeo);
     m.vmapP = sp(meshes[mesh].vmapP);
     const int nk = min(L::TB, m.K - k0);
-    int vm[L::EPT3], fid3;
-    tile_geometry<N, false>(m, o, k0, nk, lds, vm, fid3, tid);
+    int vm[L::EPT3];
+    vm_load<N, false>(vm, m, k0, nk, tid);
 #pragma unroll 1
     for (int j = 0; j < nshot; ++j) {
       int tj;
@@ -97,9 +102,14 @@ __global__ __launch_bounds__(256, 3) void k_fused_batch(int nitems, const BatchI
       src.coef = sp(sd->src.coef);
 #pragma unroll
       for (int f = 0; f < 9; ++f) src.w[f] = sf(sd->src.w[f]);
-      sh.ft = src.nsrc ? sf(sp(sd->ftab)[st * 5 + stage]) : 0.f;
+      sh.ft = src.nsrc ? sf(sp(sd->ftab)[stp * 5 + stage]) : 0.f;
       sh.fdt = sf(sd->dt);
-      tile_shot<N, false>(m, o, sh, src, fa, fb, k0, nk, lds, vm, fid3, tj);
+      TileStage<N, false> st;
+      stage_load<N, false>(st, m, sh.Qin, k0, nk, tj);
+      stage_store<N, false>(st, lds + L::OQ, lds + L::OG, tj);
+      __syncthreads();
+      tile_shot<N, false>(m, o, sh, src, fa, fb, k0, nk, lds, lds + L::OQ, lds + L::OG, vm, fid3, tj,
+                          NoHook{}, NoHook{}, NoHook{});
     }
   }
 }

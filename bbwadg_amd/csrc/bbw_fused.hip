@@ -22,6 +22,8 @@
 //
 // v_mfma_f32_16x16x4f32 lane layout (checked on gfx950): A[i][k] from lane i + 16k,
 // B[k][j] from lane j + 16k, D[4*(l/16) + r][l%16] in register r of lane l.
+#include <algorithm>
+
 #include "bbw_fused_tile.h"
 
 namespace bbw {
@@ -33,16 +35,21 @@ namespace bbw {
 
 template <int N, bool PROJ_U>
 __global__ __launch_bounds__(256, BBW_FUSED_WG_PER_CU) void k_fused_mfma(FusedMesh m, FusedOps o, FusedShot sh,
-                                                    SrcArgs src, float fa, float fb,
-                                                    const float *ftime_dev) {
+                                                                         SrcArgs src, float fa, float fb,
+                                                                         const float *ftime_dev) {
   using L = FusedLayout<N, PROJ_U>;
   extern __shared__ float lds[];
-  const int k0 = blockIdx.x * L::TB;
-  const int nk = min(L::TB, m.K - k0);
+  const int tid = threadIdx.x;
+  const int k0 = blockIdx.x * L::TB, nk = min(L::TB, m.K - k0);
   if (ftime_dev) sh.ft = *ftime_dev;
-  int vm[L::EPT3], fid3;
-  tile_geometry<N, PROJ_U>(m, o, k0, nk, lds, vm, fid3, threadIdx.x);
-  tile_shot<N, PROJ_U>(m, o, sh, src, fa, fb, k0, nk, lds, vm, fid3, threadIdx.x);
+  TileStage<N, PROJ_U> st;
+  stage_load<N, PROJ_U>(st, m, sh.Qin, k0, nk, tid);
+  int vm[L::EPT3];
+  vm_load<N, PROJ_U>(vm, m, k0, nk, tid);
+  stage_store<N, PROJ_U>(st, lds + L::OQ, lds + L::OG, tid);
+  __syncthreads();
+  tile_shot<N, PROJ_U>(m, o, sh, src, fa, fb, k0, nk, lds, lds + L::OQ, lds + L::OG, vm,
+                       face_node<N, PROJ_U>(o, tid), tid, NoHook{}, NoHook{}, NoHook{});
 }
 
 FusedOps fused_ops(const bbw_ctx *c) {
@@ -70,6 +77,7 @@ static hipError_t fused_mfma_n(bbw_ctx *c, const float *Qin, float *Qout, float 
   using L = FusedLayout<N, P>;
   const size_t sh = sizeof(float) * L::floats;
   const int grid = (c->K + L::TB - 1) / L::TB;
+  if ((unsigned long long)c->K * 16 * L::Np >= (1ull << 32)) return hipErrorInvalidValue;
   if (sh > 65536) {
     hipError_t e = hipFuncSetAttribute((const void *)k_fused_mfma<N, P>,
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh);

@@ -43,16 +43,20 @@ struct FusedLayout {
   static constexpr int BS = TB + 1;              // its row stride: column writes spread over banks
   static constexpr int SX = NP * TB;             // S row length
   static constexpr int cmax(int a, int b) { return a > b ? a : b; }
-  // W1: Q_in tile [TB][4Np] | face flux [KB][BS] | S [NQP][SX]
-  static constexpr int W1 = cmax(cmax(4 * Np, (KB * BS + TB - 1) / TB), NQP * NP);
+  // W1: face flux [KB][BS] | S [NQP][SX]
+  static constexpr int W1 = cmax((KB * BS + TB - 1) / TB, NQP * NP);
   // W2: (p, Ur, Us, Ut) float4 [TB][Np] | rhs [TB][RS]
   static constexpr int W2 = cmax(4 * Np, RS);
   static constexpr int QTW = (NQT + 3) / 4;     // q tiles per wave
   static constexpr int GS = ((Np + 63) / 64) * 64, NG = 256 / GS;        // node groups
   static constexpr int NG3 = 256 / NfpNf, EPT3 = (TB + NG3 - 1) / NG3;   // face-node groups
   static constexpr int EPTV = (TB + NG - 1) / NG;
-  static constexpr bool PF = 4 * Np <= 256;    // one state row per thread: prefetch res, Q_in
-  static constexpr size_t floats = (size_t)TB * (W1 + W2 + 9 + 16) + TB;
+  static constexpr bool PF = 4 * Np <= 256;    // one state row per thread: prefetch res
+  // Q_in tile [TB][4Np] (read in phases 2 and 8) and geometry (vgeo [TB][9], fgeo [TB][16])
+  static constexpr int QT = TB * 4 * Np, GT = TB * 25;
+  static constexpr unsigned QB = QT * 4u, QIT = (QB + 4095) / 4096;  // 16-byte words per thread
+  static constexpr int OQ = 0, OW1 = QT, OW2 = OW1 + TB * W1, OG = OW2 + TB * W2, OSL = OG + GT;
+  static constexpr size_t floats = (size_t)OSL + TB;
 };
 
 // Operator, mesh and per-shot arguments of one tile.
@@ -120,40 +124,77 @@ __device__ __forceinline__ void bst(brsrc r, float v, unsigned off) {
   __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, off, 0, 0);
 }
 
-// Geometry of a tile: vgeo/fgeo into LDS, the thread's vmapP entries and face-node id into
-// registers.  Shared by every shot processed on the same tile.  Entries of elements past the
-// tile's end read as 0 (a valid trace index; their results are never stored).
+// A tile's Q_in rows and geometry on their way to LDS through registers (words past the
+// tile's end read as 0).
 template <int N, bool PROJ_U>
-__device__ __forceinline__ void tile_geometry(const FusedMesh &m, const FusedOps &o, int k0, int nk,
-                                              float *lds, int (&vm)[FL<N, PROJ_U>::EPT3],
-                                              int &fid3, const int tid) {
+struct TileStage {
+  float4 q[FL<N, PROJ_U>::QIT];
+  float g[2];
+};
+
+template <int N, bool PROJ_U>
+__device__ __forceinline__ void stage_load(TileStage<N, PROJ_U> &st, const FusedMesh &m, const float *Qin,
+                                           int k0, int nk, const int tid) {
   using L = FL<N, PROJ_U>;
-  constexpr int NfpNf = L::NfpNf, TB = L::TB;
-  static_assert(TB * 16 <= 256, "one geometry word per thread");
-  float *sG = lds + TB * (L::W1 + L::W2);
-  float *sF = sG + TB * 9;
+  static_assert(L::TB * 16 <= 256, "one geometry word per thread");
+  const brsrc rQ = mkrsrc(Qin + (size_t)k0 * 4 * L::Np, (unsigned)nk * L::Np * 16u);
+#pragma unroll
+  for (unsigned it = 0; it < L::QIT; ++it) st.q[it] = bld4(rQ, tid * 16u + it * 4096u);
   const brsrc rg = mkrsrc(m.vgeo + (size_t)k0 * 9, (unsigned)nk * 36u);
   const brsrc rf = mkrsrc(m.fgeo + (size_t)k0 * 16, (unsigned)nk * 64u);
+  st.g[0] = bld(rg, tid * 4u);
+  st.g[1] = bld(rf, tid * 4u);
+}
+
+template <int N, bool PROJ_U>
+__device__ __forceinline__ void stage_store(const TileStage<N, PROJ_U> &st, float *sQ, float *sG,
+                                            const int tid) {
+  using L = FL<N, PROJ_U>;
+  float4 *sQ4 = (float4 *)sQ;
+#pragma unroll
+  for (unsigned it = 0; it < L::QIT; ++it) {
+    const unsigned off = tid * 16u + it * 4096u;
+    if (L::QIT * 4096u == L::QB || off < L::QB) sQ4[off / 16] = st.q[it];
+  }
+  if (tid < L::TB * 9) sG[tid] = st.g[0];
+  sG[L::TB * 9 + tid] = st.g[1];
+}
+
+// The thread's vmapP entries of a tile (face-node mapping of the flux phase), vm[EPT3].
+template <int N, bool PROJ_U>
+__device__ __forceinline__ void vm_load(int *vm, const FusedMesh &m, int k0, int nk, const int tid) {
+  using L = FL<N, PROJ_U>;
+  constexpr int NfpNf = L::NfpNf;
   const brsrc rv = mkrsrc(m.vmapP + (size_t)k0 * NfpNf, (unsigned)nk * NfpNf * 4u);
-  if (tid < TB * 9) sG[tid] = bld(rg, tid * 4u);
-  if (tid < TB * 16) sF[tid] = bld(rf, tid * 4u);
   const int g3 = tid / NfpNf, m3 = tid - g3 * NfpNf;
-  const bool on3 = g3 < L::NG3;
-  fid3 = on3 ? o.Fmask[m3] : 0;
   const unsigned vo = (unsigned)(g3 * NfpNf + m3) * 4u;
 #pragma unroll
   for (int i = 0; i < L::EPT3; ++i) vm[i] = bldi(rv, vo + i * L::NG3 * NfpNf * 4u);
 }
 
-// One fused LSRK stage of one shot on one tile (phases 1-8 of bbw_fused.hip).  Ends with a
-// barrier so the caller may process another shot on the same tile.  Elements past the tile's
-// end are computed on zeros and never stored.  Requires K * 4 * Np * 4 < 2^32 (32-bit trace
-// offsets).
+// The thread's face node (flux phase): its node id in the element.
 template <int N, bool PROJ_U>
+__device__ __forceinline__ int face_node(const FusedOps &o, const int tid) {
+  using L = FL<N, PROJ_U>;
+  const int g3 = tid / L::NfpNf, m3 = tid - g3 * L::NfpNf;
+  return g3 < L::NG3 ? o.Fmask[m3] : 0;
+}
+
+struct NoHook {
+  __device__ void operator()() const {}
+};
+
+// One fused LSRK stage of one shot on one tile (phases 1-8 of bbw_fused.hip), with the tile's
+// Q_in rows already in LDS at sQ and its geometry at sG.  at_start runs after the phase-1 loads
+// are issued, at_mid at the start of phase 6 and at_end before the final barrier: the caller's
+// prefetch of the next tile goes there.  Ends with a barrier.  Elements past the tile's end are
+// computed on zeros and never stored.  Requires K * 4 * Np * 4 < 2^32 (32-bit trace offsets).
+template <int N, bool PROJ_U, class HS, class HM, class HE>
 __device__ __forceinline__ void tile_shot(const FusedMesh &m, const FusedOps &o, const FusedShot &sh,
                                           const SrcArgs &src, float fa, float fb, int k0, int nk,
-                                          float *lds, const int (&vm)[FL<N, PROJ_U>::EPT3],
-                                          int fid3, const int tid) {
+                                          float *lds, const float *sQ, const float *sG,
+                                          const int (&vm)[FL<N, PROJ_U>::EPT3], int fid3,
+                                          const int tid, HS &&at_start, HM &&at_mid, HE &&at_end) {
   using L = FusedLayout<N, PROJ_U>;
   constexpr int Np = L::Np, Nfp = L::Nfp, NfpNf = L::NfpNf, Nq = L::Nq, TB = L::TB, RS = L::RS;
   constexpr int NpNf = 4 * Np, KF = L::KF, KB = L::KB, BS = L::BS;
@@ -162,12 +203,10 @@ __device__ __forceinline__ void tile_shot(const FusedMesh &m, const FusedOps &o,
   const float fdt = sh.fdt;
   const int4 *__restrict__ Dids = o.Dids;
   const float4 *__restrict__ Dvals = o.Dvals;
-  float *W1 = lds;                       // [TB * W1]
-  float *W2 = W1 + TB * L::W1;           // [TB * W2]
-  float *sG = W2 + TB * L::W2;           // [TB][9]
-  float *sF = sG + TB * 9;               // [TB][16]
-  int *sSlot = (int *)(sF + TB * 16);
-  float *sQ = W1;                        // phases 1-2
+  float *W1 = lds + L::OW1;              // [TB * W1]
+  float *W2 = lds + L::OW2;              // [TB * W2]
+  const float *sF = sG + TB * 9;         // [TB][16]
+  int *sSlot = (int *)(lds + L::OSL);
   float *sB = W1;                        // phases 3-5: face flux [KB][BS]
   float *S = W1;                         // phases 6-7
   float4 *sV = (float4 *)W2;             // phases 2-3
@@ -175,7 +214,6 @@ __device__ __forceinline__ void tile_shot(const FusedMesh &m, const FusedOps &o,
   const int lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const size_t base = (size_t)k0 * NpNf;
   const unsigned tbytes = (unsigned)nk * ROW;
-  const brsrc rQ = mkrsrc(sh.Qin + base, tbytes);
   const brsrc rR = mkrsrc(sh.res + base, tbytes);
   const brsrc rO = mkrsrc(sh.Qout + base, tbytes);
   const brsrc rC = mkrsrc(sh.c2q + (size_t)k0 * Nq, (unsigned)nk * Nq * 4u);
@@ -190,19 +228,9 @@ __device__ __forceinline__ void tile_shot(const FusedMesh &m, const FusedOps &o,
   // source: elements are sorted, so a tile outside [first, last] has none (uniform test)
   const bool has_src = src.nsrc > 0 && k0 + nk - 1 >= src.first && k0 <= src.last;
 
-  // ---- 1: tile loads; register prefetch of res, Q_in rows, c2, neighbour traces
-  {
-    constexpr unsigned QB = TB * ROW, QIT = (QB + 4095) / 4096;
-    constexpr bool QFIT = QIT * 4096 <= TB * L::W1 * 4u;
-    float4 *sQ4 = (float4 *)sQ;
-#pragma unroll
-    for (unsigned it = 0; it < QIT; ++it) {
-      const unsigned off = tid * 16u + it * 4096u;
-      if (QFIT || off < QB) sQ4[off / 16] = bld4(rQ, off);
-    }
-    if (has_src && tid < TB) sSlot[tid] = tid < nk ? src_slot2(src, k0 + tid) : -1;
-  }
-  float qreg[L::PF ? TB : 1], rreg[L::PF ? TB : 1];
+  // ---- 1: register prefetch of res, c2 and the neighbour traces (Q_in tile already in LDS)
+  if (has_src && tid < TB) sSlot[tid] = tid < nk ? src_slot2(src, k0 + tid) : -1;
+  float rreg[L::PF ? TB : 1];
   if constexpr (L::PF) {
 #pragma unroll
     for (int e = 0; e < TB; ++e) rreg[e] = bld(rR, tid * 4u + e * ROW);
@@ -225,6 +253,7 @@ __device__ __forceinline__ void tile_shot(const FusedMesh &m, const FusedOps &o,
 #pragma unroll
     for (int f = 0; f < 4; ++f) tr[i][f] = bld(rT, to + f * Np * 4u);
   }
+  at_start();
   __syncthreads();
 
   // ---- 2: (p, Ur, Us, Ut) per node; upwind flux at face nodes into registers.  The p and U
@@ -417,6 +446,7 @@ __device__ __forceinline__ void tile_shot(const FusedMesh &m, const FusedOps &o,
   __syncthreads();
 
   // ---- 6: S = (Vq . rhs) x c2 on MFMA (A from the zero-padded Vq, fixed offsets)
+  at_mid();
   f4 aP[KBP];
   if (wave < L::NPT) {
 #pragma unroll
@@ -464,11 +494,7 @@ __device__ __forceinline__ void tile_shot(const FusedMesh &m, const FusedOps &o,
   }
   __syncthreads();
 
-  // ---- 7: rhs = Pq . S on MFMA; Q_in rows for phase 8 read again (cached since phase 1)
-  if constexpr (L::PF) {
-#pragma unroll
-    for (int e = 0; e < TB; ++e) qreg[e] = bld(rQ, tid * 4u + e * ROW);
-  }
+  // ---- 7: rhs = Pq . S on MFMA
   {
     const int col = lane & 15, kr = lane >> 4;
     for (int mt = wave; mt < L::NPT; mt += 4) {
@@ -513,7 +539,7 @@ __device__ __forceinline__ void tile_shot(const FusedMesh &m, const FusedOps &o,
         for (int e = 0; e < TB; ++e) {
           const float x = fa * rreg[e] + fdt * sR[e * RS + tid];
           bst(rR, x, tid * 4u + e * ROW);
-          bst(rO, qreg[e] + fb * x, tid * 4u + e * ROW);
+          bst(rO, sQ[e * NpNf + tid] + fb * x, tid * 4u + e * ROW);
         }
       } else {
         const int fr = tid / Np, nr = tid - fr * Np;
@@ -525,7 +551,7 @@ __device__ __forceinline__ void tile_shot(const FusedMesh &m, const FusedOps &o,
           if (slot >= 0) v += wsel * ft * src.coef[(size_t)slot * Np + nr];
           const float x = fa * rreg[e] + fdt * v;
           bst(rR, x, tid * 4u + e * ROW);
-          bst(rO, qreg[e] + fb * x, tid * 4u + e * ROW);
+          bst(rO, sQ[e * NpNf + tid] + fb * x, tid * 4u + e * ROW);
         }
       }
     }
@@ -541,10 +567,11 @@ __device__ __forceinline__ void tile_shot(const FusedMesh &m, const FusedOps &o,
         const unsigned g = rr * 4u + e * ROW;
         const float x = fa * bld(rR, g) + fdt * v;
         bst(rR, x, g);
-        bst(rO, bld(rQ, g) + fb * x, g);
+        bst(rO, sQ[e * NpNf + rr] + fb * x, g);
       }
     }
   }
+  at_end();
   __syncthreads();
 }
 

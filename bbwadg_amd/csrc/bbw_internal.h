@@ -48,6 +48,7 @@ struct Graph {
 
 struct bbw_ctx {
   int N = 0, Np = 0, Nfp = 0, Nq = 0, Nf = 0, K = 0, dev = 0;
+  int ncu = 0;  // compute units of the device
   int KblkV = 0, KblkS = 0, KblkU = 0, KblkF = 0;
   int fused_impl = 1;  // BBW_FUSED_VECTOR or BBW_FUSED_MFMA
   int adj_impl = 1;    // 0: per-element adjoint kernels, 1: tiled MFMA adjoint kernels
