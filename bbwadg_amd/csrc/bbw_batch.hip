@@ -55,19 +55,18 @@ __device__ __forceinline__ T *sp(T *p) {
 #define BBW_FUSED_WG_PER_CU 3
 #endif
 
-// Persistent over work items: items blockIdx.x, + gridDim.x, ..., each with its shots in
-// order on the same tile (geometry and vmapP entries loaded once per item).
-template <int N>
-__global__ __launch_bounds__(256, BBW_FUSED_WG_PER_CU) void k_fused_batch(
+// Work items blockIdx.x, + gridDim.x, ... (PERSIST; otherwise one item per workgroup, which
+// keeps the register count of the single-shot kernel), each with its shots in order on the
+// same tile (the shots after the first read the tile's geometry and vmapP entries from cache).
+template <int N, bool PERSIST>
+__global__ __launch_bounds__(256, PERSIST ? 3 : 4) void k_fused_batch(
     int nitems, const BatchItem *__restrict__ items, const FusedMesh *__restrict__ meshes,
     const BatchShotDev *__restrict__ shots, const int *__restrict__ list, FusedOps o, int stage,
     int par, float fa, float fb, const int *__restrict__ step) {
   using L = FusedLayout<N, false>;
   extern __shared__ float lds[];
   const int stp = *step;
-  const int fid3 = face_node<N, false>(o, threadIdx.x);
-#pragma unroll 1
-  for (int w = blockIdx.x; w < nitems; w += gridDim.x) {
+  auto item = [&](int w) {
     // thread index re-derived per item and shot: address arithmetic is redone per tile instead
     // of being hoisted out of the loops and kept live across them (it would not fit the
     // register budget)
@@ -81,12 +80,15 @@ __global__ __launch_bounds__(256, BBW_FUSED_WG_PER_CU) void k_fused_batch(
     m.fgeo = sp(meshes[mesh].fgeo);
     m.vmapP = sp(meshes[mesh].vmapP);
     const int nk = min(L::TB, m.K - k0);
-    int vm[L::EPT3];
-    vm_load<N, false>(vm, m, k0, nk, tid);
 #pragma unroll 1
     for (int j = 0; j < nshot; ++j) {
       int tj;
       asm volatile("v_mov_b32 %0, %1" : "=v"(tj) : "v"(tid));
+      // vmapP entries per shot (the shots of an item read them from cache; keeping them live
+      // across the shot loop would not fit the register budget)
+      int vm[L::EPT3];
+      vm_load<N, false>(vm, m, k0, nk, tj);
+      const int fid3 = face_node<N, false>(o, tj);
       const BatchShotDev *sd = shots + su(list[shot0 + j]);
       FusedShot sh;
       sh.c2q = sp(sd->c2q);
@@ -111,6 +113,12 @@ __global__ __launch_bounds__(256, BBW_FUSED_WG_PER_CU) void k_fused_batch(
       tile_shot<N, false>(m, o, sh, src, fa, fb, k0, nk, lds, lds + L::OQ, lds + L::OG, vm, fid3, tj,
                           NoHook{}, NoHook{}, NoHook{});
     }
+  };
+  if constexpr (PERSIST) {
+#pragma unroll 1
+    for (int w = blockIdx.x; w < nitems; w += gridDim.x) item(w);
+  } else {
+    item(blockIdx.x);
   }
 }
 
@@ -178,10 +186,17 @@ hipError_t launch_batch_n(bbw_batch *b, int nitems, const BatchItem *items, cons
                           float fb, const int *step, int grid) {
   using L = FusedLayout<N, false>;
   const size_t sh = sizeof(float) * L::floats;
-  if (sh > 65536)
-    hipFuncSetAttribute((const void *)k_fused_batch<N>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh);
-  hipLaunchKernelGGL(k_fused_batch<N>, dim3(grid), dim3(256), sh, b->ops->stream, nitems, items,
-                     meshes, shots, list, fused_ops(b->ops), stage, par, fa, fb, step);
+  if (grid < nitems) {
+    if (sh > 65536)
+      hipFuncSetAttribute((const void *)k_fused_batch<N, true>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh);
+    hipLaunchKernelGGL((k_fused_batch<N, true>), dim3(grid), dim3(256), sh, b->ops->stream, nitems, items,
+                       meshes, shots, list, fused_ops(b->ops), stage, par, fa, fb, step);
+  } else {
+    if (sh > 65536)
+      hipFuncSetAttribute((const void *)k_fused_batch<N, false>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh);
+    hipLaunchKernelGGL((k_fused_batch<N, false>), dim3(nitems), dim3(256), sh, b->ops->stream, nitems, items,
+                       meshes, shots, list, fused_ops(b->ops), stage, par, fa, fb, step);
+  }
   return hipGetLastError();
 }
 
