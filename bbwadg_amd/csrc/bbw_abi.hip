@@ -162,7 +162,7 @@ void bbw_destroy(bbw_ctx *c) {
   if (c->stream) hipStreamSynchronize(c->stream);
   reset_graph(c);
   dfree(c->Dids); dfree(c->Dvals); dfree(c->Fmask); dfree(c->EEL_ids); dfree(c->L0_ids);
-  dfree(c->EEL_vals); dfree(c->L0_vals); dfree(c->Vq); dfree(c->Pq); dfree(c->Vq64); dfree(c->VqPad); dfree(c->PqPad); dfree(c->EEL_ids4); dfree(c->LiftPad); dfree(c->VqBlk); dfree(c->PqBlk); dfree(c->LiftBlk); dfree(c->PqTPad); dfree(c->VqTPad);
+  dfree(c->EEL_vals); dfree(c->L0_vals); dfree(c->Vq); dfree(c->Pq); dfree(c->Vq64); dfree(c->VqPad); dfree(c->PqPad); dfree(c->EEL_ids4); dfree(c->LiftPad); dfree(c->VqBlk); dfree(c->PqBlk); dfree(c->LiftBlk); dfree(c->PqTBlk); dfree(c->VqTBlk); dfree(c->LiftTBlk); dfree(c->PqTPad); dfree(c->VqTPad);
   dfree(c->DTids); dfree(c->DTvals); dfree(c->EELT_ids); dfree(c->EELT_vals); dfree(c->L0T_ids);
   dfree(c->L0T_vals); dfree(c->FmaskInv); dfree(c->mapP); dfree(c->lQ); dfree(c->lres);
   dfree(c->Zadj); dfree(c->Hadj); dfree(c->grad); dfree(c->Ybuf); dfree(c->ckpt);
@@ -356,6 +356,19 @@ int bbw_upload_operators(bbw_ctx *c, const bbw_ops *o) {
       if ((rc = upload(c, &c->VqBlk, vb.data(), vb.size()))) return rc;
       if ((rc = upload(c, &c->PqBlk, pb.data(), pb.size()))) return rc;
       if ((rc = upload(c, &c->LiftBlk, lb.data(), lb.size()))) return rc;
+      std::vector<float> ptb, vtb, ltb;
+      blocks(Nq, Np, [&](int q, int n) { return o->Pq[n + (size_t)q * Np]; }, ptb);
+      blocks(Np, Nq, [&](int n, int q) { return o->Vq[q + (size_t)n * Nq]; }, vtb);
+      for (int f = 0; f < 4; ++f) {
+        blocks(Nfp, Np, [&](int t, int n) { return (float)lift[(size_t)n * NfpNf + f * Nfp + t]; }, lf);
+        ltb.insert(ltb.end(), lf.begin(), lf.end());
+      }
+      dfree(c->PqTBlk);
+      dfree(c->VqTBlk);
+      dfree(c->LiftTBlk);
+      if ((rc = upload(c, &c->PqTBlk, ptb.data(), ptb.size()))) return rc;
+      if ((rc = upload(c, &c->VqTBlk, vtb.data(), vtb.size()))) return rc;
+      if ((rc = upload(c, &c->LiftTBlk, ltb.data(), ltb.size()))) return rc;
     }
   }
   std::vector<double> v64((size_t)Nq * Np);

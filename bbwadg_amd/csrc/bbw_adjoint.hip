@@ -11,7 +11,7 @@
 //   k_adj_apply  A^T z: volume transpose (D_m^T rows) + own and neighbour flux transposes
 //                gathered through Fmask^-1 and mapP; lQ += dt A^T z
 //   k_adj_inject receiver residuals into lQ_p (B_r^T (p_r - d_r))
-#include "bbw_internal.h"
+#include "bbw_fused_tile.h"
 
 namespace bbw {
 
@@ -259,135 +259,212 @@ template <int N>
 struct AdjLayout {
   static constexpr int Np = Dims<N>::Np, Nfp = Dims<N>::Nfp, NfpNf = Dims<N>::NfpNf;
   static constexpr int Nq = Dims<N>::Nq, TB = 16;
-  static constexpr int RS = 4 * Np + 3;
   static constexpr int NQT = (Nq + 15) / 16, NQP = 16 * NQT, NPT = (Np + 15) / 16, NPP = 16 * NPT;
   static constexpr int KTV = (Np + 3) / 4, KTP = NQP / 4, QTW = (NQT + 3) / 4;
+  static constexpr int KBV = (KTV + 3) / 4, KBP = (KTP + 3) / 4;
+  static constexpr int FMT = (Nfp + 15) / 16;      // face-node M tiles per face
+  static constexpr int NPR = 4 * FMT, PRW = (NPR + 3) / 4;   // (face, M tile) pairs; per wave
+  // g / z rows [TB][RS]: 4 fields, stride 2 mod 32 for the MFMA operand reads
+  static constexpr int RS0 = 4 * Np, RS = RS0 + ((34 - RS0 % 32) % 32);
+  static constexpr int KW = 4 * KTV, WS = TB + 1;  // w_f rows (K padded) and row stride
   static constexpr int NG3 = 256 / NfpNf, EPT3 = (TB + NG3 - 1) / NG3;
   static constexpr int GS = ((Np + 63) / 64) * 64, NG = 256 / GS, EPTV = (TB + NG - 1) / NG;
-  // sg/sz [TB][RS] | S [NQP][16] | sT float4 [TB][NfpNf] | sF [TB][16]
-  static constexpr size_t floats1 = (size_t)TB * RS + NQP * 16 + (size_t)TB * 4 * NfpNf + TB * 16;
+  static constexpr bool PF = 4 * Np <= 256;
+  static constexpr int cmax(int a, int b) { return a > b ? a : b; }
+  // sg [TB][RS] | S [NQP][16] or w [4][KW][WS] | sF [TB][16]
+  static constexpr int U = cmax(NQP * 16, 4 * KW * WS);
+  static constexpr size_t floats1 = (size_t)TB * RS + U + TB * 16;
   // sZ4 float4 [TB][Np] | sH [TB][NfpNf] | sG [TB][9] | sF [TB][16]
   static constexpr size_t floats2 = (size_t)TB * 4 * Np + (size_t)TB * NfpNf + TB * 9 + TB * 16;
 };
 
-// g = lres + b lQ; lres <- a g; t = Pq^T g_p; grad += dt t y; z_p = Vq^T (c2 t); z_u = g_u -> Z;
-// h = 1/2 Fscale (L0^T EEL^T z_p - n . L0^T EEL^T z_u) -> H
+// One reverse stage, element-local part, on 16-element tiles:
+//   g = lres + b lQ; lres <- a g; t = Pq^T g_p (MFMA); grad += dt t y; z_p = Vq^T (c2 t) (MFMA);
+//   z_u = g_u -> Z;  h_f = 1/2 Fscale_f LIFT_f^T (z_p - n_f . z_u) per face (MFMA) -> H
+// (LIFT_f^T = L0^T EEL^T restricted to face f, applied densely on the matrix cores).
 template <int N>
-__global__ __launch_bounds__(256) void k_adj1(
-    int K, const float *__restrict__ PqTPad, const float *__restrict__ VqTPad,
-    const float *__restrict__ c2q, const float *__restrict__ Y, const float *__restrict__ fgeo,
-    const int *__restrict__ EELT_ids, const float *__restrict__ EELT_vals, int WE,
-    const int *__restrict__ L0T_ids, const float *__restrict__ L0T_vals, int WL, float fa,
-    float fb, float fdt, const float *__restrict__ lQ, float *__restrict__ lres,
-    float *__restrict__ Z, float *__restrict__ H, float *__restrict__ grad) {
+__global__ __launch_bounds__(256) void k_adj1(int K, FusedOps o, const float *__restrict__ c2q,
+                                              const float *__restrict__ Y,
+                                              const float *__restrict__ fgeo, float fa, float fb,
+                                              float fdt, const float *__restrict__ lQ,
+                                              float *__restrict__ lres, float *__restrict__ Z,
+                                              float *__restrict__ H, float *__restrict__ grad,
+                                              const float *__restrict__ PqTBlk,
+                                              const float *__restrict__ VqTBlk,
+                                              const float *__restrict__ LiftTBlk) {
   using L = AdjLayout<N>;
   constexpr int Np = L::Np, Nfp = L::Nfp, NfpNf = L::NfpNf, Nq = L::Nq, TB = L::TB, RS = L::RS;
-  constexpr int NpNf = 4 * Np, NQP = L::NQP, NPP = L::NPP;
+  constexpr int NpNf = 4 * Np, KTV = L::KTV, KTP = L::KTP, KBV = L::KBV, KBP = L::KBP;
+  constexpr int KW = L::KW, WS = L::WS, FMT = L::FMT;
+  constexpr unsigned ROW = NpNf * 4u;
   extern __shared__ float lds[];
   float *sg = lds;                        // [TB][RS]
-  float *S = sg + TB * RS;                // [NQP][16]
-  float4 *sT = (float4 *)(S + NQP * 16);  // [TB][NfpNf]
-  float *sF = (float *)(sT + TB * NfpNf); // [TB][16]
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int k0 = blockIdx.x * TB;
-  const int nk = min(TB, K - k0);
+  float *S = sg + TB * RS;                // [NQP][16]   (phases 2-3)
+  float *sW = S;                          // [4][KW][WS] (phases 4-5)
+  float *sF = S + L::U;                   // [TB][16]
+  const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int col = lane & 15, kr = lane >> 4;
+  const int k0 = blockIdx.x * TB, nk = min(TB, K - k0);
   const size_t base = (size_t)k0 * NpNf;
-  for (int i = tid; i < TB * RS; i += 256) sg[i] = 0.f;
-  for (int i = tid; i < nk * 16; i += 256) sF[i] = fgeo[(size_t)k0 * 16 + i];
-  __syncthreads();
-  for (int rr = tid; rr < NpNf; rr += 256)
-    for (int e = 0; e < nk; ++e) {
-      const size_t gi = base + (size_t)e * NpNf + rr;
-      const float g = lres[gi] + fb * lQ[gi];
-      lres[gi] = fa * g;
-      sg[e * RS + rr] = g;
+  const unsigned tbytes = (unsigned)nk * ROW, qbytes = (unsigned)nk * Nq * 4u;
+  const brsrc rQl = mkrsrc(lQ + base, tbytes), rRl = mkrsrc(lres + base, tbytes);
+  const brsrc rZ = mkrsrc(Z + base, tbytes);
+  const brsrc rH = mkrsrc(H + (size_t)k0 * NfpNf, (unsigned)nk * NfpNf * 4u);
+  const brsrc rC = mkrsrc(c2q + (size_t)k0 * Nq, qbytes), rY = mkrsrc(Y + (size_t)k0 * Nq, qbytes);
+  const brsrc rGd = mkrsrc(grad + (size_t)k0 * Nq, qbytes);
+  const brsrc rPT = mkrsrc(PqTBlk, (unsigned)L::NQT * KBV * 1024u);
+  const brsrc rVT = mkrsrc(VqTBlk, (unsigned)L::NPT * KBP * 1024u);
+  const brsrc rLT = mkrsrc(LiftTBlk, 4u * FMT * KBV * 1024u);
+
+  // ---- 1: g = lres + b lQ -> LDS; lres <- a g; fgeo; c2, y, grad in the MFMA output layout
+  sF[tid] = bld(mkrsrc(fgeo + (size_t)k0 * 16, (unsigned)nk * 64u), tid * 4u);
+  if constexpr (L::PF) {
+#pragma unroll
+    for (int e = 0; e < TB; ++e) {
+      const float g = bld(rRl, tid * 4u + e * ROW) + fb * bld(rQl, tid * 4u + e * ROW);
+      if (tid < NpNf) {
+        sg[e * RS + tid] = g;
+        bst(rRl, fa * g, tid * 4u + e * ROW);
+      }
     }
+  } else {
+    for (int rr = tid; rr < NpNf; rr += 256)
+      for (int e = 0; e < TB; ++e) {
+        const float g = bld(rRl, rr * 4u + e * ROW) + fb * bld(rQl, rr * 4u + e * ROW);
+        sg[e * RS + rr] = g;
+        bst(rRl, fa * g, rr * 4u + e * ROW);
+      }
+  }
+  float c2r[L::QTW][4], yr[L::QTW][4], gr[L::QTW][4];
+  f4 aT[L::QTW][KBV];
+  const unsigned co = (unsigned)(col * Nq + 4 * kr) * 4u;
+#pragma unroll
+  for (int t = 0; t < L::QTW; ++t) {
+    const int qt = wave + 4 * t;
+    if (qt < L::NQT) {
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        c2r[t][r] = bld(rC, co + (qt * 16 + r) * 4u);
+        yr[t][r] = bld(rY, co + (qt * 16 + r) * 4u);
+        gr[t][r] = bld(rGd, co + (qt * 16 + r) * 4u);
+      }
+#pragma unroll
+      for (int kb = 0; kb < KBV; ++kb) aT[t][kb] = bld4_s(rPT, lane * 16u, (unsigned)(qt * KBV + kb) * 1024u);
+    }
+  }
   __syncthreads();
-  // t = Pq^T g_p (M = cubature points, N = elements, K = nodes); gradient; S = c2 t
+
+  // ---- 2: t = Pq^T g_p (M = cubature points, N = elements, K = nodes); grad; S = c2 t
+  f4 aV[KBP];
+  if (wave < L::NPT) {
+#pragma unroll
+    for (int kb = 0; kb < KBP; ++kb) aV[kb] = bld4_s(rVT, lane * 16u, (unsigned)(wave * KBP + kb) * 1024u);
+  }
   {
-    const int col = lane & 15, kr = lane >> 4;
     const float *Bp = sg + col * RS + kr;
 #pragma unroll
     for (int t = 0; t < L::QTW; ++t) {
       const int qt = wave + 4 * t;
       if (qt >= L::NQT) break;
-      f4a acc = {0.f, 0.f, 0.f, 0.f};
-      const float *Ap = PqTPad + kr * NQP + qt * 16 + col;
+      f4 acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-      for (int kt = 0; kt < L::KTV; ++kt)
-        acc = __builtin_amdgcn_mfma_f32_16x16x4f32(Ap[kt * 4 * NQP], Bp[kt * 4], acc, 0, 0, 0);
+      for (int kb = 0; kb < KBV; ++kb)
+#pragma unroll
+        for (int sb = 0; sb < 4; ++sb)
+          if (4 * kb + sb < KTV)
+            acc = __builtin_amdgcn_mfma_f32_16x16x4f32(aT[t][kb][sb], Bp[(4 * kb + sb) * 4], acc, 0, 0, 0);
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int q = qt * 16 + 4 * kr + r;
-        float sv = 0.f;
-        if (q < Nq && col < nk) {
-          const size_t id = (size_t)(k0 + col) * Nq + q;
-          grad[id] += fdt * acc[r] * Y[id];
-          sv = c2q[id] * acc[r];
-        }
-        S[q * 16 + col] = sv;
+        if (q < Nq) bst(rGd, gr[t][r] + fdt * acc[r] * yr[t][r], co + (qt * 16 + r) * 4u);
+        S[q * 16 + col] = c2r[t][r] * acc[r];
       }
     }
   }
   __syncthreads();
-  // z_p = Vq^T S (M = nodes, K = cubature points), written over g_p
-  {
-    const int col = lane & 15, kr = lane >> 4;
-    for (int mt = wave; mt < L::NPT; mt += 4) {
-      f4a acc = {0.f, 0.f, 0.f, 0.f};
-      const float *Ap = VqTPad + kr * NPP + mt * 16 + col;
-      const float *Bp = S + kr * 16 + col;
-#pragma unroll 8
-      for (int kt = 0; kt < L::KTP; ++kt)
-        acc = __builtin_amdgcn_mfma_f32_16x16x4f32(Ap[kt * 4 * NPP], Bp[kt * 64], acc, 0, 0, 0);
+
+  // ---- 3: z_p = Vq^T S (M = nodes, K = cubature points), over g_p
+  for (int mt = wave; mt < L::NPT; mt += 4) {
+    if (mt != wave) {
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int n = mt * 16 + 4 * kr + r;
-        if (n < Np) sg[col * RS + n] = acc[r];
-      }
+      for (int kb = 0; kb < KBP; ++kb) aV[kb] = bld4_s(rVT, lane * 16u, (unsigned)(mt * KBP + kb) * 1024u);
+    }
+    f4 acc = {0.f, 0.f, 0.f, 0.f};
+    const float *Bp = S + kr * 16 + col;
+#pragma unroll
+    for (int kb = 0; kb < KBP; ++kb)
+#pragma unroll
+      for (int sb = 0; sb < 4; ++sb)
+        if (4 * kb + sb < KTP)
+          acc = __builtin_amdgcn_mfma_f32_16x16x4f32(aV[kb][sb], Bp[(4 * kb + sb) * 64], acc, 0, 0, 0);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int n = mt * 16 + 4 * kr + r;
+      if (n < Np) sg[col * RS + n] = acc[r];
     }
   }
   __syncthreads();
-  // Z out; EEL^T z per face node (4 fields) -> sT
-  for (int rr = tid; rr < NpNf; rr += 256)
-    for (int e = 0; e < nk; ++e) Z[base + (size_t)e * NpNf + rr] = sg[e * RS + rr];
+
+  // ---- 4: Z out; w_f = z_p - n_f . z_u per face -> LDS [face][node][element]
+  if constexpr (L::PF) {
+    if (tid < NpNf) {
+#pragma unroll
+      for (int e = 0; e < TB; ++e) bst(rZ, sg[e * RS + tid], tid * 4u + e * ROW);
+    }
+  } else {
+    for (int rr = tid; rr < NpNf; rr += 256)
+      for (int e = 0; e < TB; ++e) bst(rZ, sg[e * RS + rr], rr * 4u + e * ROW);
+  }
   {
-    const int g3 = tid / NfpNf, m3 = tid - g3 * NfpNf;
-    if (g3 < L::NG3) {
-      for (int e = g3; e < nk; e += L::NG3) {
-        const float *z = sg + e * RS;
-        float a0 = 0.f, a1 = 0.f, a2 = 0.f, a3 = 0.f;
-        for (int j = 0; j < WE; ++j) {
-          const int cc = EELT_ids[m3 + j * NfpNf];
-          const float v = EELT_vals[m3 + j * NfpNf];
-          a0 += v * z[cc];
-          a1 += v * z[Np + cc];
-          a2 += v * z[2 * Np + cc];
-          a3 += v * z[3 * Np + cc];
+    const int gv = tid / L::GS, nv = tid - gv * L::GS;
+    if (gv < L::NG && nv < KW) {
+#pragma unroll
+      for (int i = 0; i < L::EPTV; ++i) {
+        const int e = gv + L::NG * i;
+        if (L::NG * L::EPTV == TB || e < TB) {
+          const float *z = sg + e * RS;
+          const bool on = nv < Np;
+          const float zp = on ? z[nv] : 0.f, zu = on ? z[Np + nv] : 0.f;
+          const float zv = on ? z[2 * Np + nv] : 0.f, zw = on ? z[3 * Np + nv] : 0.f;
+#pragma unroll
+          for (int f = 0; f < 4; ++f) {
+            const float *F = sF + 16 * e + 4 * f;
+            sW[(f * KW + nv) * WS + e] = zp - (F[1] * zu + F[2] * zv + F[3] * zw);
+          }
         }
-        sT[e * NfpNf + m3] = make_float4(a0, a1, a2, a3);
       }
     }
   }
+  f4 aL[L::PRW][KBV];
+#pragma unroll
+  for (int j = 0; j < L::PRW; ++j) {
+    const int pr = wave + 4 * j;
+    if (pr < L::NPR) {
+#pragma unroll
+      for (int kb = 0; kb < KBV; ++kb) aL[j][kb] = bld4_s(rLT, lane * 16u, (unsigned)(pr * KBV + kb) * 1024u);
+    }
+  }
   __syncthreads();
-  {
-    const int g3 = tid / NfpNf, m3 = tid - g3 * NfpNf;
-    if (g3 < L::NG3) {
-      const int f = m3 / Nfp, nn = m3 - f * Nfp;
-      for (int e = g3; e < nk; e += L::NG3) {
-        const float4 *t = sT + e * NfpNf + f * Nfp;
-        float gp = 0.f, gu = 0.f, gv = 0.f, gw = 0.f;
-        for (int j = 0; j < WL; ++j) {
-          const float v = L0T_vals[nn + j * Nfp];
-          const float4 x = t[L0T_ids[nn + j * Nfp]];
-          gp += v * x.x;
-          gu += v * x.y;
-          gv += v * x.z;
-          gw += v * x.w;
-        }
-        const float *F = sF + 16 * e + 4 * f;
-        H[(size_t)(k0 + e) * NfpNf + m3] = .5f * F[0] * (gp - (F[1] * gu + F[2] * gv + F[3] * gw));
-      }
+
+  // ---- 5: h_f = 1/2 Fscale_f LIFT_f^T w_f (M = face nodes, N = elements, K = nodes) -> H
+#pragma unroll
+  for (int j = 0; j < L::PRW; ++j) {
+    const int pr = wave + 4 * j;
+    if (pr >= L::NPR) break;
+    const int f = pr / FMT, mt = pr - f * FMT;
+    f4 acc = {0.f, 0.f, 0.f, 0.f};
+    const float *Bp = sW + (f * KW + kr) * WS + col;
+#pragma unroll
+    for (int kb = 0; kb < KBV; ++kb)
+#pragma unroll
+      for (int sb = 0; sb < 4; ++sb)
+        if (4 * kb + sb < KTV)
+          acc = __builtin_amdgcn_mfma_f32_16x16x4f32(aL[j][kb][sb], Bp[(4 * kb + sb) * 4 * WS], acc, 0, 0, 0);
+    const float hs = .5f * sF[16 * col + 4 * f];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int m = mt * 16 + 4 * kr + r;
+      if (m < Nfp) bst(rH, hs * acc[r], (unsigned)(col * NfpNf + f * Nfp + m) * 4u);
     }
   }
 }
@@ -509,9 +586,10 @@ static hipError_t adj_tiled_n(bbw_ctx *c, const float *Y, float fa, float fb, fl
     hipFuncSetAttribute((const void *)k_adj1<N>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)s1);
   if (s2 > 65536)
     hipFuncSetAttribute((const void *)k_adj2<N>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)s2);
-  hipLaunchKernelGGL(k_adj1<N>, dim3(grid), dim3(256), s1, c->stream, c->K, c->PqTPad, c->VqTPad,
-                     c->c2q, Y, c->fgeo, c->EELT_ids, c->EELT_vals, c->WE, c->L0T_ids, c->L0T_vals,
-                     c->WL, fa, fb, fdt, c->lQ, c->lres, c->Zadj, c->Hadj, c->grad);
+  if ((unsigned long long)c->K * 16 * L::Np >= (1ull << 32)) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_adj1<N>, dim3(grid), dim3(256), s1, c->stream, c->K, fused_ops(c), c->c2q, Y,
+                     c->fgeo, fa, fb, fdt, c->lQ, c->lres, c->Zadj, c->Hadj, c->grad, c->PqTBlk,
+                     c->VqTBlk, c->LiftTBlk);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(k_adj2<N>, dim3(grid), dim3(256), s2, c->stream, c->K, c->vgeo, c->fgeo,

@@ -76,6 +76,10 @@ struct bbw_ctx {
   // points, K = nodes); PqBlk: A = Pq (M = nodes, K = cubature points); LiftBlk: per face f,
   // A = LIFT_f (M = nodes, K = face nodes), face blocks of NPT * KB tiles
   float *VqBlk = nullptr, *PqBlk = nullptr, *LiftBlk = nullptr;
+  // the same for the adjoint stage: PqTBlk: A = Pq^T (M = cubature points, K = nodes);
+  // VqTBlk: A = Vq^T (M = nodes, K = cubature points); LiftTBlk: per face f, A = LIFT_f^T
+  // (M = face nodes, K = nodes), face blocks of ceil(Nfp/16) * KB tiles
+  float *PqTBlk = nullptr, *VqTBlk = nullptr, *LiftTBlk = nullptr;
   double *Vq64 = nullptr, *wq64 = nullptr;
   double rk4a[5] = {0}, rk4b[5] = {0}, rk4c[6] = {0};
   bool have_ops = false;
