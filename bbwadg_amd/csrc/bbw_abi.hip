@@ -760,7 +760,7 @@ int bbw_adjoint_gradient(bbw_ctx *c, int nsteps, double dt, double t0, const flo
   if (rc) return rc;
   if (!c->mapP) return fail(c, BBW_ERR_STATE, "adjoint maps not uploaded");
   const size_t n = nstate(c), nq = (size_t)c->K * c->Nq, nh = (size_t)c->K * 4 * c->Nfp;
-  const size_t ystep = 5 * nq;  // floats of Vq (A Q)_p per step
+  const size_t ystep = 5 * (size_t)c->K * c->Np;  // floats of (A Q)_p per step (5 stages)
   if ((rc = dalloc(c, &c->lQ, n)) || (rc = dalloc(c, &c->lres, n)) || (rc = dalloc(c, &c->Zadj, n)) ||
       (rc = dalloc(c, &c->Hadj, nh)) || (rc = dalloc(c, &c->grad, nq)))
     return rc;
@@ -807,7 +807,7 @@ int bbw_adjoint_gradient(bbw_ctx *c, int nsteps, double dt, double t0, const flo
     for (int i = 0; i < 5; ++i) {
       hipError_t e = launch_fused_mfma(c, c->Q[c->cur], c->Q[1 - c->cur], (float)c->rk4a[i],
                                        (float)c->rk4b[i], (float)dt, nullptr, stage_time(step, i),
-                                       0, Y + (size_t)i * nq);
+                                       0, Y + (size_t)i * c->K * c->Np);
       if (e != hipSuccess) return hipfail(c, e, "forward stage with Vq (A Q)_p capture");
       c->cur = 1 - c->cur;
     }
@@ -817,8 +817,9 @@ int bbw_adjoint_gradient(bbw_ctx *c, int nsteps, double dt, double t0, const flo
     hipError_t e = launch_adj_inject(c, c->resid + (size_t)step * c->nrec);
     if (e != hipSuccess) return hipfail(c, e, "receiver injection");
     for (int i = 4; i >= 0; --i) {
-      e = c->adj_impl ? launch_adj_stage_tiled(c, Y + (size_t)i * nq, (float)c->rk4a[i], (float)c->rk4b[i], (float)dt)
-                      : launch_adj_stage(c, Y + (size_t)i * nq, (float)c->rk4a[i], (float)c->rk4b[i], (float)dt);
+      const float *Yi = Y + (size_t)i * c->K * c->Np;
+      e = c->adj_impl ? launch_adj_stage_tiled(c, Yi, (float)c->rk4a[i], (float)c->rk4b[i], (float)dt)
+                      : launch_adj_stage(c, Yi, (float)c->rk4a[i], (float)c->rk4b[i], (float)dt);
       if (e != hipSuccess) return hipfail(c, e, "adjoint stage");
     }
     return BBW_OK;

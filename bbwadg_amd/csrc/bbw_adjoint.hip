@@ -4,7 +4,7 @@
 //     r = W A Q + s(t);  res' = a res + dt r;  Q' = Q + b res'
 // Reverse stage, given the adjoints lQ (of Q') and lres (of res'):
 //     g = lres + b lQ;  lres <- a g;  z = W^T g;  grad_c2 += dt (Pq^T g_p) * y;
-//     lQ <- lQ + dt A^T z,          y = Vq (A Q)_p at the stage input (saved by the forward)
+//     lQ <- lQ + dt A^T z,          y = Vq (A Q)_p at the stage input ((A Q)_p saved by the forward)
 // Kernels:
 //   k_adj_wadg   g, lres update, t = Pq^T g_p, gradient, z_p = Vq^T (c2 t), z_u = g_u -> Z
 //   k_adj_lift   h = 1/2 Fscale (L0^T EEL^T z_p - n . L0^T EEL^T z_u) per face node -> H
@@ -40,10 +40,14 @@ __global__ __launch_bounds__(256) void k_adj_wadg(int K, int KB, const float *__
   for (int w = threadIdx.x; w < nk * Nq; w += blockDim.x) {
     const int e = w / Nq, q = w - e * Nq;
     const float *g = sg + e * Nf * Np;
-    float t = 0.f;
-    for (int n = 0; n < Np; ++n) t += Pq[n + q * Np] * g[n];
+    const float *u = Y + (size_t)(k0 + e) * Np;
+    float t = 0.f, y = 0.f;
+    for (int n = 0; n < Np; ++n) {
+      t += Pq[n + q * Np] * g[n];
+      y += Vq[q + n * Nq] * u[n];
+    }
     const size_t id = (size_t)(k0 + e) * Nq + q;
-    grad[id] += fdt * t * Y[id];
+    grad[id] += fdt * t * y;
     st[e * Nq + q] = c2q[id] * t;
   }
   __syncthreads();
@@ -273,7 +277,9 @@ struct AdjLayout {
   static constexpr int cmax(int a, int b) { return a > b ? a : b; }
   // sg [TB][RS] | S [NQP][16] or w [4][KW][WS] | sF [TB][16]
   static constexpr int U = cmax(NQP * 16, 4 * KW * WS);
-  static constexpr size_t floats1 = (size_t)TB * RS + U + TB * 16;
+  // (A Q)_p rows [TB][RU]: K padded to KU, stride 2 mod 32
+  static constexpr int KU = 4 * KTV, RU = KU + ((34 - KU % 32) % 32);
+  static constexpr size_t floats1 = (size_t)TB * RS + U + TB * 16 + (size_t)TB * RU;
   // sZ4 float4 [TB][Np] | sH [TB][NfpNf] | sG [TB][9] | sF [TB][16]
   static constexpr size_t floats2 = (size_t)TB * 4 * Np + (size_t)TB * NfpNf + TB * 9 + TB * 16;
 };
@@ -302,6 +308,7 @@ __global__ __launch_bounds__(256) void k_adj1(int K, FusedOps o, const float *__
   float *S = sg + TB * RS;                // [NQP][16]   (phases 2-3)
   float *sW = S;                          // [4][KW][WS] (phases 4-5)
   float *sF = S + L::U;                   // [TB][16]
+  float *sU = sF + TB * 16;               // [TB][RU] (A Q)_p of the forward stage
   const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int col = lane & 15, kr = lane >> 4;
   const int k0 = blockIdx.x * TB, nk = min(TB, K - k0);
@@ -310,7 +317,9 @@ __global__ __launch_bounds__(256) void k_adj1(int K, FusedOps o, const float *__
   const brsrc rQl = mkrsrc(lQ + base, tbytes), rRl = mkrsrc(lres + base, tbytes);
   const brsrc rZ = mkrsrc(Z + base, tbytes);
   const brsrc rH = mkrsrc(H + (size_t)k0 * NfpNf, (unsigned)nk * NfpNf * 4u);
-  const brsrc rC = mkrsrc(c2q + (size_t)k0 * Nq, qbytes), rY = mkrsrc(Y + (size_t)k0 * Nq, qbytes);
+  const brsrc rC = mkrsrc(c2q + (size_t)k0 * Nq, qbytes);
+  const brsrc rY = mkrsrc(Y + (size_t)k0 * Np, (unsigned)nk * Np * 4u);
+  const brsrc rVq = mkrsrc(o.VqBlk, (unsigned)L::NQT * KBV * 1024u);
   const brsrc rGd = mkrsrc(grad + (size_t)k0 * Nq, qbytes);
   const brsrc rPT = mkrsrc(PqTBlk, (unsigned)L::NQT * KBV * 1024u);
   const brsrc rVT = mkrsrc(VqTBlk, (unsigned)L::NPT * KBP * 1024u);
@@ -335,8 +344,13 @@ __global__ __launch_bounds__(256) void k_adj1(int K, FusedOps o, const float *__
         bst(rRl, fa * g, rr * 4u + e * ROW);
       }
   }
-  float c2r[L::QTW][4], yr[L::QTW][4], gr[L::QTW][4];
-  f4 aT[L::QTW][KBV];
+  for (int i = tid; i < TB * Np; i += 256) {
+    const int e = i / Np;
+    sU[e * L::RU + (i - e * Np)] = bld(rY, i * 4u);
+  }
+  if (tid < TB * (L::KU - Np)) sU[(tid / (L::KU - Np)) * L::RU + Np + tid % (L::KU - Np)] = 0.f;
+  float c2r[L::QTW][4], gr[L::QTW][4];
+  f4 aT[L::QTW][KBV], aY[L::QTW][KBV];
   const unsigned co = (unsigned)(col * Nq + 4 * kr) * 4u;
 #pragma unroll
   for (int t = 0; t < L::QTW; ++t) {
@@ -345,16 +359,19 @@ __global__ __launch_bounds__(256) void k_adj1(int K, FusedOps o, const float *__
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         c2r[t][r] = bld(rC, co + (qt * 16 + r) * 4u);
-        yr[t][r] = bld(rY, co + (qt * 16 + r) * 4u);
         gr[t][r] = bld(rGd, co + (qt * 16 + r) * 4u);
       }
 #pragma unroll
-      for (int kb = 0; kb < KBV; ++kb) aT[t][kb] = bld4_s(rPT, lane * 16u, (unsigned)(qt * KBV + kb) * 1024u);
+      for (int kb = 0; kb < KBV; ++kb) {
+        aT[t][kb] = bld4_s(rPT, lane * 16u, (unsigned)(qt * KBV + kb) * 1024u);
+        aY[t][kb] = bld4_s(rVq, lane * 16u, (unsigned)(qt * KBV + kb) * 1024u);
+      }
     }
   }
   __syncthreads();
 
-  // ---- 2: t = Pq^T g_p (M = cubature points, N = elements, K = nodes); grad; S = c2 t
+  // ---- 2: t = Pq^T g_p and y = Vq (A Q)_p (M = cubature points, N = elements, K = nodes);
+  // grad += dt t y; S = c2 t
   f4 aV[KBP];
   if (wave < L::NPT) {
 #pragma unroll
@@ -362,21 +379,24 @@ __global__ __launch_bounds__(256) void k_adj1(int K, FusedOps o, const float *__
   }
   {
     const float *Bp = sg + col * RS + kr;
+    const float *Bu = sU + col * L::RU + kr;
 #pragma unroll
     for (int t = 0; t < L::QTW; ++t) {
       const int qt = wave + 4 * t;
       if (qt >= L::NQT) break;
-      f4 acc = {0.f, 0.f, 0.f, 0.f};
+      f4 acc = {0.f, 0.f, 0.f, 0.f}, ay = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
       for (int kb = 0; kb < KBV; ++kb)
 #pragma unroll
         for (int sb = 0; sb < 4; ++sb)
-          if (4 * kb + sb < KTV)
+          if (4 * kb + sb < KTV) {
             acc = __builtin_amdgcn_mfma_f32_16x16x4f32(aT[t][kb][sb], Bp[(4 * kb + sb) * 4], acc, 0, 0, 0);
+            ay = __builtin_amdgcn_mfma_f32_16x16x4f32(aY[t][kb][sb], Bu[(4 * kb + sb) * 4], ay, 0, 0, 0);
+          }
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int q = qt * 16 + 4 * kr + r;
-        if (q < Nq) bst(rGd, gr[t][r] + fdt * acc[r] * yr[t][r], co + (qt * 16 + r) * 4u);
+        if (q < Nq) bst(rGd, gr[t][r] + fdt * acc[r] * ay[r], co + (qt * 16 + r) * 4u);
         S[q * 16 + col] = c2r[t][r] * acc[r];
       }
     }
@@ -470,7 +490,8 @@ __global__ __launch_bounds__(256) void k_adj1(int K, FusedOps o, const float *__
 }
 
 // lQ += dt A^T z: volume transpose from (z_p, Zr, Zs, Zt) float4 per node, flux transpose from
-// own H and the neighbour's H through mapP
+// own H and the neighbour's H through mapP.  Every global read (Z rows, lQ rows, mapP entries,
+// neighbour H) is issued before the first barrier.
 template <int N>
 __global__ __launch_bounds__(256) void k_adj2(int K, const float *__restrict__ vgeo,
                                               const float *__restrict__ fgeo,
@@ -483,32 +504,52 @@ __global__ __launch_bounds__(256) void k_adj2(int K, const float *__restrict__ v
                                               float *__restrict__ lQ) {
   using L = AdjLayout<N>;
   constexpr int Np = L::Np, Nfp = L::Nfp, NfpNf = L::NfpNf, TB = L::TB, NpNf = 4 * Np;
+  constexpr unsigned ROW = NpNf * 4u;
   extern __shared__ float lds[];
   float4 *sZ = (float4 *)lds;              // [TB][Np]
   float *sH = (float *)(sZ + TB * Np);     // [TB][NfpNf]
   float *sG = sH + TB * NfpNf;             // [TB][9]
   float *sF = sG + TB * 9;                 // [TB][16]
   const int tid = threadIdx.x;
-  const int k0 = blockIdx.x * TB;
-  const int nk = min(TB, K - k0);
+  const int k0 = blockIdx.x * TB, nk = min(TB, K - k0);
   const size_t base = (size_t)k0 * NpNf;
-  for (int i = tid; i < nk * 9; i += 256) sG[i] = vgeo[(size_t)k0 * 9 + i];
-  for (int i = tid; i < nk * 16; i += 256) sF[i] = fgeo[(size_t)k0 * 16 + i];
-  for (int i = tid; i < nk * NfpNf; i += 256) sH[i] = H[(size_t)k0 * NfpNf + i];
-  __syncthreads();
+  const brsrc rZ = mkrsrc(Z + base, (unsigned)nk * ROW), rL = mkrsrc(lQ + base, (unsigned)nk * ROW);
+  const brsrc rHt = mkrsrc(H + (size_t)k0 * NfpNf, (unsigned)nk * NfpNf * 4u);
+  const brsrc rHa = mkrsrc(H, (unsigned)K * NfpNf * 4u);
+  const brsrc rM = mkrsrc(mapP + (size_t)k0 * NfpNf, (unsigned)nk * NfpNf * 4u);
+  if (tid < TB * 9) sG[tid] = bld(mkrsrc(vgeo + (size_t)k0 * 9, (unsigned)nk * 36u), tid * 4u);
+  sF[tid] = bld(mkrsrc(fgeo + (size_t)k0 * 16, (unsigned)nk * 64u), tid * 4u);
+  for (int i = tid; i < TB * NfpNf; i += 256) sH[i] = bld(rHt, i * 4u);
   const int gv = tid / L::GS, nv = tid - gv * L::GS;
   const bool on = gv < L::NG && nv < Np;
+  float zr[L::EPTV][4], lq[L::EPTV][4], hn[L::EPTV][3];
+  int mp[L::EPTV][3], slot[3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) slot[a] = on ? FmaskInv[3 * nv + a] : -1;
+#pragma unroll
+  for (int i = 0; i < L::EPTV; ++i) {
+    const int e = gv + L::NG * i;
+#pragma unroll
+    for (int f = 0; f < 4; ++f) {
+      zr[i][f] = bld(rZ, (unsigned)(e * NpNf + f * Np + nv) * 4u);
+      lq[i][f] = bld(rL, (unsigned)(e * NpNf + f * Np + nv) * 4u);
+    }
+#pragma unroll
+    for (int a = 0; a < 3; ++a) mp[i][a] = bldi(rM, (unsigned)(e * NfpNf + (slot[a] < 0 ? 0 : slot[a])) * 4u);
+  }
+#pragma unroll
+  for (int i = 0; i < L::EPTV; ++i)
+#pragma unroll
+    for (int a = 0; a < 3; ++a) hn[i][a] = bld(rHa, (unsigned)mp[i][a] * 4u);
+  __syncthreads();
   if (on) {
 #pragma unroll
     for (int i = 0; i < L::EPTV; ++i) {
       const int e = gv + L::NG * i;
-      if (e < nk) {
-        const float *z = Z + base + (size_t)e * NpNf;
-        const float *G = sG + 9 * e;
-        const float u = z[Np + nv], v = z[2 * Np + nv], w = z[3 * Np + nv];
-        sZ[e * Np + nv] = make_float4(z[nv], G[0] * u + G[1] * v + G[2] * w,
-                                      G[3] * u + G[4] * v + G[5] * w, G[6] * u + G[7] * v + G[8] * w);
-      }
+      const float *G = sG + 9 * e;
+      const float u = zr[i][1], v = zr[i][2], w = zr[i][3];
+      sZ[e * Np + nv] = make_float4(zr[i][0], G[0] * u + G[1] * v + G[2] * w,
+                                    G[3] * u + G[4] * v + G[5] * w, G[6] * u + G[7] * v + G[8] * w);
     }
   }
   __syncthreads();
@@ -520,14 +561,9 @@ __global__ __launch_bounds__(256) void k_adj2(int K, const float *__restrict__ v
     di[m] = DTids[m * Np + nv];
     dv[m] = DTvals[m * Np + nv];
   }
-  int slot[3];
-#pragma unroll
-  for (int a = 0; a < 3; ++a) slot[a] = FmaskInv[3 * nv + a];
 #pragma unroll
   for (int i = 0; i < L::EPTV; ++i) {
     const int e = gv + L::NG * i;
-    if (e >= nk) break;
-    const int k = k0 + e;
     const float4 *V = sZ + e * Np;
     float dp[4], dz[4];
 #pragma unroll
@@ -554,26 +590,24 @@ __global__ __launch_bounds__(256) void k_adj2(int K, const float *__restrict__ v
     for (int a = 0; a < 3; ++a) {
       const int r = slot[a];
       if (r < 0) continue;
-      const size_t own = (size_t)k * NfpNf + r;
+      const int own = (k0 + e) * NfpNf + r;
       const float h = sH[e * NfpNf + r];
-      const int mp = mapP[own];
       const float *F = sF + 16 * e + 4 * (r / Nfp);
-      if ((size_t)mp == own) {
+      if (mp[i][a] == own) {
         op -= 2.f * h;
       } else {
-        const float hn = H[mp];
-        op += hn - h;
-        const float hs = h + hn;
+        op += hn[i][a] - h;
+        const float hs = h + hn[i][a];
         ou += hs * F[1];
         ov += hs * F[2];
         ow += hs * F[3];
       }
     }
-    float *o = lQ + base + (size_t)e * NpNf + nv;
-    o[0] += fdt * op;
-    o[Np] += fdt * ou;
-    o[2 * Np] += fdt * ov;
-    o[3 * Np] += fdt * ow;
+    const unsigned o = (unsigned)(e * NpNf + nv) * 4u;
+    bst(rL, lq[i][0] + fdt * op, o);
+    bst(rL, lq[i][1] + fdt * ou, o + Np * 4u);
+    bst(rL, lq[i][2] + fdt * ov, o + 2 * Np * 4u);
+    bst(rL, lq[i][3] + fdt * ow, o + 3 * Np * 4u);
   }
 }
 

@@ -84,7 +84,7 @@ struct FusedShot {
   const float *c2q;
   const float *Qin;
   float *Qout, *res;
-  float *yout;   // Vq (A Q)_p for the adjoint, or null
+  float *yout;   // (A Q)_p in BB coefficients [K][Np] for the adjoint, or null
   float ft;      // source time function at this stage
   float fdt;     // time step
 };
@@ -447,6 +447,12 @@ __device__ __forceinline__ void tile_shot(const FusedMesh &m, const FusedOps &o,
 
   // ---- 6: S = (Vq . rhs) x c2 on MFMA (A from the zero-padded Vq, fixed offsets)
   at_mid();
+  if (sh.yout && tid < Np) {
+    // (A Q)_p before the projection, kept for the adjoint's c2 gradient
+    const brsrc rY = mkrsrc(sh.yout + (size_t)k0 * Np, (unsigned)nk * Np * 4u);
+#pragma unroll
+    for (int e = 0; e < TB; ++e) bst(rY, sR[e * RS + tid], (unsigned)(e * Np + tid) * 4u);
+  }
   f4 aP[KBP];
   if (wave < L::NPT) {
 #pragma unroll
@@ -455,7 +461,6 @@ __device__ __forceinline__ void tile_shot(const FusedMesh &m, const FusedOps &o,
   {
     const int col = lane & 15, kr = lane >> 4;
     const float *Bp = sR + col * RS + kr;
-    const brsrc rY = mkrsrc(sh.yout ? sh.yout + (size_t)k0 * Nq : nullptr, sh.yout ? (unsigned)nk * Nq * 4u : 0u);
 #pragma unroll
     for (int t = 0; t < L::QTW; ++t) {
       const int qt = wave + 4 * t;
@@ -473,14 +478,6 @@ __device__ __forceinline__ void tile_shot(const FusedMesh &m, const FusedOps &o,
             for (int b = 0; b < NP; ++b)
               acc[b] = __builtin_amdgcn_mfma_f32_16x16x4f32(aV[t][kb][sb], Bp[b * Np + kt * 4], acc[b], 0, 0, 0);
           }
-        }
-      }
-      if (sh.yout) {
-        // Vq . rhs_p at the cubature points, kept for the adjoint's c2 gradient
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          const int q = qt * 16 + 4 * kr + r;
-          if (q < Nq) bst(rY, acc[0][r], (unsigned)(col * Nq + q) * 4u);
         }
       }
 #pragma unroll
