@@ -289,7 +289,7 @@ struct AdjLayout {
 //   z_u = g_u -> Z;  h_f = 1/2 Fscale_f LIFT_f^T (z_p - n_f . z_u) per face (MFMA) -> H
 // (LIFT_f^T = L0^T EEL^T restricted to face f, applied densely on the matrix cores).
 template <int N>
-__global__ __launch_bounds__(256) void k_adj1(int K, FusedOps o, const float *__restrict__ c2q,
+__global__ __launch_bounds__(256, 4) void k_adj1(int K, FusedOps o, const float *__restrict__ c2q,
                                               const float *__restrict__ Y,
                                               const float *__restrict__ fgeo, float fa, float fb,
                                               float fdt, const float *__restrict__ lQ,
