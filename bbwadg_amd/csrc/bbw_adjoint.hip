@@ -270,16 +270,18 @@ struct AdjLayout {
   static constexpr int NPR = 4 * FMT, PRW = (NPR + 3) / 4;   // (face, M tile) pairs; per wave
   // g / z rows [TB][RS]: 4 fields, stride 2 mod 32 for the MFMA operand reads
   static constexpr int RS0 = 4 * Np, RS = RS0 + ((34 - RS0 % 32) % 32);
-  static constexpr int KW = 4 * KTV, WS = TB + 1;  // w_f rows (K padded) and row stride
   static constexpr int NG3 = 256 / NfpNf, EPT3 = (TB + NG3 - 1) / NG3;
   static constexpr int GS = ((Np + 63) / 64) * 64, NG = 256 / GS, EPTV = (TB + NG - 1) / NG;
   static constexpr bool PF = 4 * Np <= 256;
   static constexpr int cmax(int a, int b) { return a > b ? a : b; }
-  // sg [TB][RS] | S [NQP][16] or w [4][KW][WS] | sF [TB][16]
-  static constexpr int U = cmax(NQP * 16, 4 * KW * WS);
+  // sg [TB][RS] | S [NQP][16] | sF [TB][16]
+  static constexpr int U = NQP * 16;
   // (A Q)_p rows [TB][RU]: K padded to KU, stride 2 mod 32
   static constexpr int KU = 4 * KTV, RU = KU + ((34 - KU % 32) % 32);
-  static constexpr size_t floats1 = (size_t)TB * RS + U + TB * 16 + (size_t)TB * RU;
+  // c2 and gradient tiles [TB * Nq] each (contiguous copies of the global rows); the H tile
+  // [TB][NfpNf] reuses them
+  static constexpr int CG = cmax(2 * TB * Nq, TB * NfpNf);
+  static constexpr size_t floats1 = (size_t)TB * RS + U + TB * 16 + (size_t)TB * RU + CG;
   // sZ4 float4 [TB][Np] | sH [TB][NfpNf] | sG [TB][9] | sF [TB][16]
   static constexpr size_t floats2 = (size_t)TB * 4 * Np + (size_t)TB * NfpNf + TB * 9 + TB * 16;
 };
@@ -289,7 +291,7 @@ struct AdjLayout {
 //   z_u = g_u -> Z;  h_f = 1/2 Fscale_f LIFT_f^T (z_p - n_f . z_u) per face (MFMA) -> H
 // (LIFT_f^T = L0^T EEL^T restricted to face f, applied densely on the matrix cores).
 template <int N>
-__global__ __launch_bounds__(256, 4) void k_adj1(int K, FusedOps o, const float *__restrict__ c2q,
+__global__ __launch_bounds__(256, N <= 5 ? 4 : 2) void k_adj1(int K, FusedOps o, const float *__restrict__ c2q,
                                               const float *__restrict__ Y,
                                               const float *__restrict__ fgeo, float fa, float fb,
                                               float fdt, const float *__restrict__ lQ,
@@ -301,14 +303,16 @@ __global__ __launch_bounds__(256, 4) void k_adj1(int K, FusedOps o, const float 
   using L = AdjLayout<N>;
   constexpr int Np = L::Np, Nfp = L::Nfp, NfpNf = L::NfpNf, Nq = L::Nq, TB = L::TB, RS = L::RS;
   constexpr int NpNf = 4 * Np, KTV = L::KTV, KTP = L::KTP, KBV = L::KBV, KBP = L::KBP;
-  constexpr int KW = L::KW, WS = L::WS, FMT = L::FMT;
+  constexpr int FMT = L::FMT;
   constexpr unsigned ROW = NpNf * 4u;
   extern __shared__ float lds[];
   float *sg = lds;                        // [TB][RS]
   float *S = sg + TB * RS;                // [NQP][16]   (phases 2-3)
-  float *sW = S;                          // [4][KW][WS] (phases 4-5)
   float *sF = S + L::U;                   // [TB][16]
   float *sU = sF + TB * 16;               // [TB][RU] (A Q)_p of the forward stage
+  float *sC = sU + TB * L::RU;            // [TB * Nq] c2        (phases 1-2)
+  float *sGr = sC + TB * Nq;              // [TB * Nq] gradient  (phases 1-3)
+  float *sHt = sC;                        // [TB][NfpNf] H       (phases 4-5)
   const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int col = lane & 15, kr = lane >> 4;
   const int k0 = blockIdx.x * TB, nk = min(TB, K - k0);
@@ -349,18 +353,16 @@ __global__ __launch_bounds__(256, 4) void k_adj1(int K, FusedOps o, const float 
     sU[e * L::RU + (i - e * Np)] = bld(rY, i * 4u);
   }
   if (tid < TB * (L::KU - Np)) sU[(tid / (L::KU - Np)) * L::RU + Np + tid % (L::KU - Np)] = 0.f;
-  float c2r[L::QTW][4], gr[L::QTW][4];
+  // c2 and gradient rows of the tile: contiguous in memory, copied as such
+  for (int i = tid; i < TB * Nq; i += 256) {
+    sC[i] = bld(rC, i * 4u);
+    sGr[i] = bld(rGd, i * 4u);
+  }
   f4 aT[L::QTW][KBV], aY[L::QTW][KBV];
-  const unsigned co = (unsigned)(col * Nq + 4 * kr) * 4u;
 #pragma unroll
   for (int t = 0; t < L::QTW; ++t) {
     const int qt = wave + 4 * t;
     if (qt < L::NQT) {
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        c2r[t][r] = bld(rC, co + (qt * 16 + r) * 4u);
-        gr[t][r] = bld(rGd, co + (qt * 16 + r) * 4u);
-      }
 #pragma unroll
       for (int kb = 0; kb < KBV; ++kb) {
         aT[t][kb] = bld4_s(rPT, lane * 16u, (unsigned)(qt * KBV + kb) * 1024u);
@@ -396,14 +398,28 @@ __global__ __launch_bounds__(256, 4) void k_adj1(int K, FusedOps o, const float 
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int q = qt * 16 + 4 * kr + r;
-        if (q < Nq) bst(rGd, gr[t][r] + fdt * acc[r] * ay[r], co + (qt * 16 + r) * 4u);
-        S[q * 16 + col] = c2r[t][r] * acc[r];
+        float sv = 0.f;
+        if (q < Nq) {
+          sGr[col * Nq + q] += fdt * acc[r] * ay[r];
+          sv = sC[col * Nq + q] * acc[r];
+        }
+        S[q * 16 + col] = sv;
       }
     }
   }
   __syncthreads();
 
-  // ---- 3: z_p = Vq^T S (M = nodes, K = cubature points), over g_p
+  // ---- 3: z_p = Vq^T S (M = nodes, K = cubature points), over g_p; gradient rows out
+  f4 aL[L::PRW][KBV];
+#pragma unroll
+  for (int j = 0; j < L::PRW; ++j) {
+    const int pr = wave + 4 * j;
+    if (pr < L::NPR) {
+#pragma unroll
+      for (int kb = 0; kb < KBV; ++kb) aL[j][kb] = bld4_s(rLT, lane * 16u, (unsigned)(pr * KBV + kb) * 1024u);
+    }
+  }
+  for (int i = tid; i < TB * Nq; i += 256) bst(rGd, sGr[i], i * 4u);
   for (int mt = wave; mt < L::NPT; mt += 4) {
     if (mt != wave) {
 #pragma unroll
@@ -425,7 +441,8 @@ __global__ __launch_bounds__(256, 4) void k_adj1(int K, FusedOps o, const float 
   }
   __syncthreads();
 
-  // ---- 4: Z out; w_f = z_p - n_f . z_u per face -> LDS [face][node][element]
+  // ---- 4: Z out; h_f = 1/2 Fscale_f LIFT_f^T (z_p - n_f . z_u) per face on MFMA (M = face
+  // nodes, N = elements, K = nodes; the B operand is formed from z in LDS) -> H tile in LDS
   if constexpr (L::PF) {
     if (tid < NpNf) {
 #pragma unroll
@@ -436,57 +453,38 @@ __global__ __launch_bounds__(256, 4) void k_adj1(int K, FusedOps o, const float 
       for (int e = 0; e < TB; ++e) bst(rZ, sg[e * RS + rr], rr * 4u + e * ROW);
   }
   {
-    const int gv = tid / L::GS, nv = tid - gv * L::GS;
-    if (gv < L::NG && nv < KW) {
+    const float *zc = sg + col * RS + kr;
+    const float *Fe = sF + 16 * col;
 #pragma unroll
-      for (int i = 0; i < L::EPTV; ++i) {
-        const int e = gv + L::NG * i;
-        if (L::NG * L::EPTV == TB || e < TB) {
-          const float *z = sg + e * RS;
-          const bool on = nv < Np;
-          const float zp = on ? z[nv] : 0.f, zu = on ? z[Np + nv] : 0.f;
-          const float zv = on ? z[2 * Np + nv] : 0.f, zw = on ? z[3 * Np + nv] : 0.f;
+    for (int j = 0; j < L::PRW; ++j) {
+      const int pr = wave + 4 * j;
+      if (pr >= L::NPR) break;
+      const int f = pr / FMT, mt = pr - f * FMT;
+      const float nx = Fe[4 * f + 1], ny = Fe[4 * f + 2], nz = Fe[4 * f + 3];
+      f4 acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-          for (int f = 0; f < 4; ++f) {
-            const float *F = sF + 16 * e + 4 * f;
-            sW[(f * KW + nv) * WS + e] = zp - (F[1] * zu + F[2] * zv + F[3] * zw);
+      for (int kb = 0; kb < KBV; ++kb)
+#pragma unroll
+        for (int sb = 0; sb < 4; ++sb) {
+          const int kt = 4 * kb + sb;
+          if (kt < KTV) {
+            const int n = 4 * kt;
+            float w = zc[n] - (nx * zc[Np + n] + ny * zc[2 * Np + n] + nz * zc[3 * Np + n]);
+            if (n + 3 >= Np && n + kr >= Np) w = 0.f;   // K padding past the last node
+            acc = __builtin_amdgcn_mfma_f32_16x16x4f32(aL[j][kb][sb], w, acc, 0, 0, 0);
           }
         }
+      const float hs = .5f * Fe[4 * f];
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int m = mt * 16 + 4 * kr + r;
+        if (m < Nfp) sHt[col * NfpNf + f * Nfp + m] = hs * acc[r];
       }
     }
   }
-  f4 aL[L::PRW][KBV];
-#pragma unroll
-  for (int j = 0; j < L::PRW; ++j) {
-    const int pr = wave + 4 * j;
-    if (pr < L::NPR) {
-#pragma unroll
-      for (int kb = 0; kb < KBV; ++kb) aL[j][kb] = bld4_s(rLT, lane * 16u, (unsigned)(pr * KBV + kb) * 1024u);
-    }
-  }
   __syncthreads();
-
-  // ---- 5: h_f = 1/2 Fscale_f LIFT_f^T w_f (M = face nodes, N = elements, K = nodes) -> H
-#pragma unroll
-  for (int j = 0; j < L::PRW; ++j) {
-    const int pr = wave + 4 * j;
-    if (pr >= L::NPR) break;
-    const int f = pr / FMT, mt = pr - f * FMT;
-    f4 acc = {0.f, 0.f, 0.f, 0.f};
-    const float *Bp = sW + (f * KW + kr) * WS + col;
-#pragma unroll
-    for (int kb = 0; kb < KBV; ++kb)
-#pragma unroll
-      for (int sb = 0; sb < 4; ++sb)
-        if (4 * kb + sb < KTV)
-          acc = __builtin_amdgcn_mfma_f32_16x16x4f32(aL[j][kb][sb], Bp[(4 * kb + sb) * 4 * WS], acc, 0, 0, 0);
-    const float hs = .5f * sF[16 * col + 4 * f];
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      const int m = mt * 16 + 4 * kr + r;
-      if (m < Nfp) bst(rH, hs * acc[r], (unsigned)(col * NfpNf + f * Nfp + m) * 4u);
-    }
-  }
+  // ---- 5: H rows out (contiguous)
+  for (int i = tid; i < TB * NfpNf; i += 256) bst(rH, sHt[i], i * 4u);
 }
 
 // lQ += dt A^T z: volume transpose from (z_p, Zr, Zs, Zt) float4 per node, flux transpose from
