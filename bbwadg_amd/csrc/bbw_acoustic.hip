@@ -622,6 +622,7 @@ hipError_t launch_surface(bbw_ctx *c) {
 
 hipError_t launch_update_wadg(bbw_ctx *c, float fa, float fb, float fdt, const float *ftd,
                               float ft) {
+  if (c->update_impl == BBW_UPDATE_MFMA) return launch_update_wadg_mfma(c, fa, fb, fdt, ftd, ft);
   switch (c->N) {
     case 1: return update_wadg_n<1>(c, fa, fb, fdt, ftd, ft);
     case 2: return update_wadg_n<2>(c, fa, fb, fdt, ftd, ft);

@@ -37,3 +37,19 @@ def test_bad_config_rejected_without_gpu_work():
     cfg = _lib.Config(N=5, Nfields=4, K=10, Nq=94)
     assert lib.bbw_create(ctypes.byref(cfg), ctypes.byref(h)) == _lib.BBW_ERR_ARG
     assert lib.bbw_create(None, ctypes.byref(h)) == _lib.BBW_ERR_ARG
+
+
+def test_header_constants_match_bindings():
+    """Every option, mode and implementation constant of the header has the same value in the
+    Python bindings."""
+    text = open(os.path.join(ROOT, "include", "bbwadg.h")).read()
+    defs = dict((k, int(v)) for k, v in re.findall(r"^#define BBW_(\w+) \(?(-?\d+)\)?", text, re.M))
+    checked = 0
+    for k, v in defs.items():
+        if hasattr(_lib, k):
+            assert getattr(_lib, k) == v, k
+            checked += 1
+        elif hasattr(_lib, "BBW_" + k):
+            assert getattr(_lib, "BBW_" + k) == v, k
+            checked += 1
+    assert checked >= 12

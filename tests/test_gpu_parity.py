@@ -55,10 +55,12 @@ def test_surface(N):
         assert rel(got, ref) < tol
 
 
-@pytest.mark.parametrize("N", [3, 4, 5])
-def test_update_wadg(N):
-    print(f"case update_wadg N={N}", flush=True)
+@pytest.mark.parametrize("impl", [_lib.UPDATE_VECTOR, _lib.UPDATE_MFMA])
+@pytest.mark.parametrize("N", [2, 3, 4, 5, 6])
+def test_update_wadg(N, impl):
+    print(f"case update_wadg N={N} impl={impl}", flush=True)
     P, s = make(N, 3)
+    s.set_option(_lib.OPT_UPDATE_IMPL, impl)
     Q, R, res = P.random_state(), P.random_state(), P.random_state()
     s.WaveSetData3d(Q)
     s.set_rhs(R)
