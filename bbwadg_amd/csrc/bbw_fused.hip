@@ -71,7 +71,7 @@ struct UpdLayout {
 };
 
 template <int N>
-__global__ __launch_bounds__(256, 4) void k_update_wadg_mfma(int K, FusedOps o, const float *__restrict__ c2q,
+__global__ __launch_bounds__(256, N <= 4 ? 4 : N == 5 ? 3 : 2) void k_update_wadg_mfma(int K, FusedOps o, const float *__restrict__ c2q,
                                                              const float *__restrict__ Jq, float fa,
                                                              float fb, float fdt, SrcArgs src,
                                                              const float *ftime_dev, float ftime,
