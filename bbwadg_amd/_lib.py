@@ -33,9 +33,9 @@ EXPORTS = [
     "bbw_batch_get_state", "bbw_batch_reset", "bbw_batch_last_ms", "bbw_batch_set_step_cap",
 ]
 OPT_FUSED_IMPL, OPT_KBLK_V, OPT_KBLK_S, OPT_KBLK_U, OPT_KBLK_F, OPT_ADJ_CACHE_MB, OPT_ADJ_IMPL = 0, 1, 2, 3, 4, 5, 6
-OPT_UPDATE_IMPL = 7
+OPT_PLAIN_IMPL = 7
 FUSED_VECTOR, FUSED_MFMA = 0, 1
-UPDATE_VECTOR, UPDATE_MFMA = 0, 1
+PLAIN_FLAT, PLAIN_TILED = 0, 1
 
 
 class BBWLibraryError(RuntimeError):

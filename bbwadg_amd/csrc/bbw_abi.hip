@@ -877,10 +877,9 @@ int bbw_set_option(bbw_ctx *c, int option, int value) {
     case BBW_OPT_KBLK_F: c->KblkF = value > 0 ? value : 0; break;
     case BBW_OPT_ADJ_CACHE_MB: c->ycache_mb = value; break;
     case BBW_OPT_ADJ_IMPL: c->adj_impl = value ? 1 : 0; break;
-    case BBW_OPT_UPDATE_IMPL:
-      if (value != BBW_UPDATE_VECTOR && value != BBW_UPDATE_MFMA) return fail(c, BBW_ERR_ARG, "bad update impl");
-      if (value == BBW_UPDATE_MFMA && c->Nf != 4) return fail(c, BBW_ERR_ARG, "MFMA update is acoustic only");
-      c->update_impl = value;
+    case BBW_OPT_PLAIN_IMPL:
+      if (value != BBW_PLAIN_FLAT && value != BBW_PLAIN_TILED) return fail(c, BBW_ERR_ARG, "bad plain impl");
+      c->plain_impl = value;
       break;
     default: return fail(c, BBW_ERR_ARG, "unknown option");
   }

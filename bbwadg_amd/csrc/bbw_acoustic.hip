@@ -597,6 +597,7 @@ static hipError_t energy_n(bbw_ctx *c, double *partial, int *nblocks) {
 bool order_supported(int N) { return N >= 1 && N <= 6; }
 
 hipError_t launch_volume(bbw_ctx *c) {
+  if (c->plain_impl == BBW_PLAIN_TILED) return launch_volume_tiled(c);
   switch (c->N) {
     case 1: return volume_n<1>(c);
     case 2: return volume_n<2>(c);
@@ -609,6 +610,7 @@ hipError_t launch_volume(bbw_ctx *c) {
 }
 
 hipError_t launch_surface(bbw_ctx *c) {
+  if (c->plain_impl == BBW_PLAIN_TILED) return launch_surface_tiled(c);
   switch (c->N) {
     case 1: return surface_n<1>(c);
     case 2: return surface_n<2>(c);
@@ -622,7 +624,7 @@ hipError_t launch_surface(bbw_ctx *c) {
 
 hipError_t launch_update_wadg(bbw_ctx *c, float fa, float fb, float fdt, const float *ftd,
                               float ft) {
-  if (c->update_impl == BBW_UPDATE_MFMA) return launch_update_wadg_mfma(c, fa, fb, fdt, ftd, ft);
+  if (c->plain_impl == BBW_PLAIN_TILED) return launch_update_wadg_mfma(c, fa, fb, fdt, ftd, ft);
   switch (c->N) {
     case 1: return update_wadg_n<1>(c, fa, fb, fdt, ftd, ft);
     case 2: return update_wadg_n<2>(c, fa, fb, fdt, ftd, ft);

@@ -277,6 +277,278 @@ hipError_t launch_update_wadg_mfma(bbw_ctx *c, float fa, float fb, float fdt, co
   return hipErrorInvalidValue;
 }
 
+// Plain volume and surface kernels on 16-element tiles (the three-launch path's
+// rk_volume_bern, cpp/okl/WaveKernels.okl:344-474, and rk_surface_bern, :479-647): the same
+// buffer addressing and per-thread operand addresses as the fused stage.  The surface kernel
+// keeps the reference's lift: L0 per face, then EEL, accumulated into rhs.
+template <int N>
+struct PlainLayout {
+  static constexpr int Np = Dims<N>::Np, Nfp = Dims<N>::Nfp, NfpNf = Dims<N>::NfpNf, TB = 16;
+  static constexpr int NpNf = 4 * Np, QT = TB * NpNf;
+  static constexpr unsigned QB = QT * 4u, QIT = (QB + 4095) / 4096;
+  static constexpr int GS = ((Np + 63) / 64) * 64, NG = 256 / GS, EPTV = (TB + NG - 1) / NG;
+  static constexpr int NG3 = 256 / NfpNf, EPT3 = (TB + NG3 - 1) / NG3;
+  static constexpr int cmax(int a, int b) { return a > b ? a : b; }
+  // volume: Q tile [TB][4Np] | (p, Ur, Us, Ut) [TB][Np] float4 | vgeo [TB][9]
+  static constexpr size_t vfloats = (size_t)QT + QT + TB * 9;
+  // surface: Q tile [TB][4Np] then lifted face data float4 [TB][NfpNf] | phi [TB][NfpNf] | fgeo
+  static constexpr size_t sfloats = (size_t)cmax(QT, 4 * TB * NfpNf) + TB * NfpNf + TB * 16;
+};
+
+template <int N>
+__device__ __forceinline__ void plain_q_tile(float *sQ, const float *Q, int k0, int nk, int tid) {
+  using L = PlainLayout<N>;
+  const brsrc rQ = mkrsrc(Q + (size_t)k0 * L::NpNf, (unsigned)nk * L::NpNf * 4u);
+  float4 *sQ4 = (float4 *)sQ;
+#pragma unroll
+  for (unsigned it = 0; it < L::QIT; ++it) {
+    const unsigned off = tid * 16u + it * 4096u;
+    if (L::QIT * 4096u == L::QB || off < L::QB) sQ4[off / 16] = bld4(rQ, off);
+  }
+}
+
+template <int N>
+__global__ __launch_bounds__(256) void k_volume_t(int K, const float *__restrict__ vgeo,
+                                                  const int4 *__restrict__ Dids,
+                                                  const float4 *__restrict__ Dvals,
+                                                  const float *__restrict__ Q, float *__restrict__ rhs) {
+  using L = PlainLayout<N>;
+  constexpr int Np = L::Np, NpNf = L::NpNf, TB = L::TB;
+  extern __shared__ float lds[];
+  float *sQ = lds;
+  float4 *sV = (float4 *)(lds + L::QT);
+  float *sG = lds + 2 * L::QT;
+  const int tid = threadIdx.x;
+  const int k0 = blockIdx.x * TB, nk = min(TB, K - k0);
+  plain_q_tile<N>(sQ, Q, k0, nk, tid);
+  if (tid < TB * 9) sG[tid] = bld(mkrsrc(vgeo + (size_t)k0 * 9, (unsigned)nk * 36u), tid * 4u);
+  __syncthreads();
+  const int gv = tid / L::GS, nv = tid - gv * L::GS;
+  const bool onv = gv < L::NG && nv < Np;
+  if (onv) {
+#pragma unroll
+    for (int i = 0; i < L::EPTV; ++i) {
+      const int e = gv + L::NG * i;
+      const float *q = sQ + e * NpNf;
+      const float *G = sG + 9 * e;
+      const float uu = q[Np + nv], vv = q[2 * Np + nv], ww = q[3 * Np + nv];
+      sV[e * Np + nv] = make_float4(q[nv], uu * G[0] + vv * G[1] + ww * G[2],
+                                    uu * G[3] + vv * G[4] + ww * G[5], uu * G[6] + vv * G[7] + ww * G[8]);
+    }
+  }
+  __syncthreads();
+  if (!onv) return;
+  const brsrc rO = mkrsrc(rhs + (size_t)k0 * NpNf, (unsigned)nk * NpNf * 4u);
+  const float4 dv = Dvals[nv];
+  const int4 D1 = Dids[nv], D2 = Dids[Np + nv], D3 = Dids[2 * Np + nv], D4 = Dids[3 * Np + nv];
+  const float4 *Vb = sV + gv * Np;
+  const float4 *pa1 = Vb + D1.x, *pa2 = Vb + D2.x, *pa3 = Vb + D3.x, *pa4 = Vb + D4.x;
+  const float4 *pb1 = Vb + D1.y, *pb2 = Vb + D2.y, *pb3 = Vb + D3.y, *pb4 = Vb + D4.y;
+  const float4 *pc1 = Vb + D1.z, *pc2 = Vb + D2.z, *pc3 = Vb + D3.z, *pc4 = Vb + D4.z;
+  const float4 *pd1 = Vb + D1.w, *pd2 = Vb + D2.w, *pd3 = Vb + D3.w, *pd4 = Vb + D4.w;
+#pragma unroll
+  for (int i = 0; i < L::EPTV; ++i) {
+    const int e = gv + L::NG * i;
+    const int o4 = i * L::NG * Np;
+    const float4 a1 = pa1[o4], a2 = pa2[o4], a3 = pa3[o4], a4 = pa4[o4];
+    const float4 b1 = pb1[o4], b2 = pb2[o4], b3 = pb3[o4], b4 = pb4[o4];
+    const float4 c1 = pc1[o4], c2 = pc2[o4], c3 = pc3[o4], c4 = pc4[o4];
+    const float4 d1 = pd1[o4], d2 = pd2[o4], d3 = pd3[o4], d4 = pd4[o4];
+    const float p1 = dv.x * a1.x + dv.y * a2.x + dv.z * a3.x + dv.w * a4.x;
+    const float p2 = dv.x * b1.x + dv.y * b2.x + dv.z * b3.x + dv.w * b4.x;
+    const float p3 = dv.x * c1.x + dv.y * c2.x + dv.z * c3.x + dv.w * c4.x;
+    const float p4 = dv.x * d1.x + dv.y * d2.x + dv.z * d3.x + dv.w * d4.x;
+    float dU1 = dv.x * (b1.y + c1.z + d1.w);
+    dU1 += dv.y * (b2.y + c2.z + d2.w);
+    dU1 += dv.z * (b3.y + c3.z + d3.w);
+    dU1 += dv.w * (b4.y + c4.z + d4.w);
+    float dU2 = dv.x * (a1.y + a1.z + a1.w);
+    dU2 += dv.y * (a2.y + a2.z + a2.w);
+    dU2 += dv.z * (a3.y + a3.z + a3.w);
+    dU2 += dv.w * (a4.y + a4.z + a4.w);
+    const float dpdr = .5f * (p2 - p1), dpds = .5f * (p3 - p1), dpdt = .5f * (p4 - p1);
+    const float *G = sG + 9 * e;
+    const unsigned o = (unsigned)(e * NpNf + nv) * 4u;
+    bst(rO, -.5f * (dU1 - dU2), o);
+    bst(rO, -(G[0] * dpdr + G[3] * dpds + G[6] * dpdt), o + Np * 4u);
+    bst(rO, -(G[1] * dpdr + G[4] * dpds + G[7] * dpdt), o + 2 * Np * 4u);
+    bst(rO, -(G[2] * dpdr + G[5] * dpds + G[8] * dpdt), o + 3 * Np * 4u);
+  }
+}
+
+template <int N>
+__global__ __launch_bounds__(256) void k_surface_t(int K, const float *__restrict__ fgeo,
+                                                   const int *__restrict__ Fmask,
+                                                   const int *__restrict__ vmapP,
+                                                   const int *__restrict__ EEL_ids4,
+                                                   const float *__restrict__ EEL_vals,
+                                                   const int *__restrict__ L0_ids,
+                                                   const float *__restrict__ L0_vals,
+                                                   const float *__restrict__ Q, float *__restrict__ rhs) {
+  using L = PlainLayout<N>;
+  constexpr int Np = L::Np, Nfp = L::Nfp, NfpNf = L::NfpNf, NpNf = L::NpNf, TB = L::TB;
+  constexpr int EEL_nnz = Dims<N>::EEL_nnz, L0_nnz = Dims<N>::L0_nnz;
+  extern __shared__ float lds[];
+  float *sQ = lds;                                        // [TB][4Np]       (phases 1-2)
+  float4 *sL = (float4 *)lds;                             // [TB][NfpNf]     (phases 3-4)
+  float *sPhi = lds + L::cmax(L::QT, 4 * TB * NfpNf);     // [TB][NfpNf]
+  float *sF = sPhi + TB * NfpNf;                          // [TB][16]
+  const int tid = threadIdx.x;
+  const int k0 = blockIdx.x * TB, nk = min(TB, K - k0);
+  const brsrc rO = mkrsrc(rhs + (size_t)k0 * NpNf, (unsigned)nk * NpNf * 4u);
+  const brsrc rT = mkrsrc(Q, (unsigned)K * NpNf * 4u);
+  // ---- 1: Q tile, fgeo; the thread's vmapP entries, neighbour traces and rhs rows
+  plain_q_tile<N>(sQ, Q, k0, nk, tid);
+  sF[tid] = bld(mkrsrc(fgeo + (size_t)k0 * 16, (unsigned)nk * 64u), tid * 4u);
+  const int g3 = tid / NfpNf, m3 = tid - g3 * NfpNf;
+  const bool on3 = g3 < L::NG3;
+  const int f3 = m3 / Nfp;
+  const int fid3 = on3 ? Fmask[m3] : 0;
+  const FusedMesh mm{K, nullptr, nullptr, vmapP};
+  int vm[L::EPT3];
+  vm_load<N, false>(vm, mm, k0, nk, tid);
+  float tr[L::EPT3][4];
+#pragma unroll
+  for (int i = 0; i < L::EPT3; ++i)
+#pragma unroll
+    for (int f = 0; f < 4; ++f) tr[i][f] = bld(rT, (unsigned)vm[i] * 4u + f * Np * 4u);
+  const int gv = tid / L::GS, nv = tid - gv * L::GS;
+  const bool onv = gv < L::NG && nv < Np;
+  float ro[L::EPTV][4];
+#pragma unroll
+  for (int i = 0; i < L::EPTV; ++i)
+#pragma unroll
+    for (int f = 0; f < 4; ++f) ro[i][f] = bld(rO, (unsigned)((gv + L::NG * i) * NpNf + f * Np + nv) * 4u);
+  __syncthreads();
+  // ---- 2: upwind flux phi = (pjump - Unjump) Fscale / 2 (the U flux is -phi)
+  if (on3) {
+#pragma unroll
+    for (int i = 0; i < L::EPT3; ++i) {
+      const int e = g3 + L::NG3 * i;
+      if (L::NG3 * L::EPT3 == TB || e < TB) {
+        const float *q = sQ + e * NpNf;
+        const float *F = sF + 16 * e + 4 * f3;
+        const float pM = q[fid3], uM = q[Np + fid3], vM = q[2 * Np + fid3], wM = q[3 * Np + fid3];
+        float pjump, Unjump;
+        if (vm[i] == fid3 + (k0 + e) * NpNf) {
+          pjump = -2.f * pM;
+          Unjump = 0.f;
+        } else {
+          pjump = tr[i][0] - pM;
+          Unjump = (tr[i][1] - uM) * F[1] + (tr[i][2] - vM) * F[2] + (tr[i][3] - wM) * F[3];
+        }
+        sPhi[e * NfpNf + m3] = .5f * (pjump - Unjump) * F[0];
+      }
+    }
+  }
+  __syncthreads();
+  // ---- 3: L0 per face; stored as (p, U nx, U ny, U nz) for the EEL rows
+  if (on3) {
+    const int nt = m3 - f3 * Nfp;
+    float Lv[L0_nnz];
+    const float *Xp[L0_nnz];
+    const float *Xb = sPhi + g3 * NfpNf + f3 * Nfp;
+#pragma unroll
+    for (int j = 0; j < L0_nnz; ++j) {
+      Lv[j] = L0_vals[nt + j * Nfp];
+      Xp[j] = Xb + L0_ids[nt + j * Nfp];
+    }
+#pragma unroll
+    for (int i = 0; i < L::EPT3; ++i) {
+      const int e = g3 + L::NG3 * i;
+      if (L::NG3 * L::EPT3 == TB || e < TB) {
+        float v = 0.f;
+#pragma unroll
+        for (int j = 0; j < L0_nnz; ++j) v += Lv[j] * Xp[j][i * L::NG3 * NfpNf];
+        const float *F = sF + 16 * e + 4 * f3;
+        sL[e * NfpNf + m3] = make_float4(v, -v * F[1], -v * F[2], -v * F[3]);
+      }
+    }
+  }
+  __syncthreads();
+  // ---- 4: EEL rows, accumulated into rhs
+  if (!onv) return;
+  float Ev[EEL_nnz];
+  const float4 *Ep[EEL_nnz];
+  const float *Lb = (const float *)(sL + gv * NfpNf);
+#pragma unroll
+  for (int j = 0; j < EEL_nnz; ++j) {
+    Ev[j] = EEL_vals[nv + j * Np];
+    Ep[j] = (const float4 *)(Lb + EEL_ids4[nv + j * Np]);
+  }
+#pragma unroll
+  for (int i = 0; i < L::EPTV; ++i) {
+    const int e = gv + L::NG * i;
+    float v1 = 0.f, v2 = 0.f, v3 = 0.f, v4 = 0.f;
+#pragma unroll
+    for (int j = 0; j < EEL_nnz; ++j) {
+      const float4 x = Ep[j][i * L::NG * NfpNf];
+      v1 += Ev[j] * x.x;
+      v2 += Ev[j] * x.y;
+      v3 += Ev[j] * x.z;
+      v4 += Ev[j] * x.w;
+    }
+    const unsigned o = (unsigned)(e * NpNf + nv) * 4u;
+    bst(rO, ro[i][0] + v1, o);
+    bst(rO, ro[i][1] + v2, o + Np * 4u);
+    bst(rO, ro[i][2] + v3, o + 2 * Np * 4u);
+    bst(rO, ro[i][3] + v4, o + 3 * Np * 4u);
+  }
+}
+
+template <int N>
+static hipError_t volume_t_n(bbw_ctx *c) {
+  using L = PlainLayout<N>;
+  const size_t sh = sizeof(float) * L::vfloats;
+  if ((unsigned long long)c->K * 16 * L::Np >= (1ull << 32)) return hipErrorInvalidValue;
+  if (sh > 65536) {
+    hipError_t e = hipFuncSetAttribute((const void *)k_volume_t<N>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh);
+    if (e != hipSuccess) return e;
+  }
+  hipLaunchKernelGGL(k_volume_t<N>, dim3((c->K + L::TB - 1) / L::TB), dim3(256), sh, c->stream, c->K,
+                     c->vgeo, c->Dids, c->Dvals, c->Q[c->cur], c->rhs);
+  return hipGetLastError();
+}
+
+template <int N>
+static hipError_t surface_t_n(bbw_ctx *c) {
+  using L = PlainLayout<N>;
+  const size_t sh = sizeof(float) * L::sfloats;
+  if ((unsigned long long)c->K * 16 * L::Np >= (1ull << 32)) return hipErrorInvalidValue;
+  if (sh > 65536) {
+    hipError_t e = hipFuncSetAttribute((const void *)k_surface_t<N>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh);
+    if (e != hipSuccess) return e;
+  }
+  hipLaunchKernelGGL(k_surface_t<N>, dim3((c->K + L::TB - 1) / L::TB), dim3(256), sh, c->stream, c->K,
+                     c->fgeo, c->Fmask, c->vmapP, c->EEL_ids4, c->EEL_vals, c->L0_ids, c->L0_vals,
+                     c->Q[c->cur], c->rhs);
+  return hipGetLastError();
+}
+
+hipError_t launch_volume_tiled(bbw_ctx *c) {
+  switch (c->N) {
+    case 1: return volume_t_n<1>(c);
+    case 2: return volume_t_n<2>(c);
+    case 3: return volume_t_n<3>(c);
+    case 4: return volume_t_n<4>(c);
+    case 5: return volume_t_n<5>(c);
+    case 6: return volume_t_n<6>(c);
+  }
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_surface_tiled(bbw_ctx *c) {
+  switch (c->N) {
+    case 1: return surface_t_n<1>(c);
+    case 2: return surface_t_n<2>(c);
+    case 3: return surface_t_n<3>(c);
+    case 4: return surface_t_n<4>(c);
+    case 5: return surface_t_n<5>(c);
+    case 6: return surface_t_n<6>(c);
+  }
+  return hipErrorInvalidValue;
+}
+
 FusedOps fused_ops(const bbw_ctx *c) {
   FusedOps o;
   o.Fmask = c->Fmask;

@@ -52,7 +52,7 @@ struct bbw_ctx {
   int KblkV = 0, KblkS = 0, KblkU = 0, KblkF = 0;
   int fused_impl = 1;  // BBW_FUSED_VECTOR or BBW_FUSED_MFMA
   int adj_impl = 1;    // 0: per-element adjoint kernels, 1: tiled MFMA adjoint kernels
-  int update_impl = 1; // plain WADG update: 0 vector kernel, 1 MFMA kernel
+  int plain_impl = 1;  // plain kernels: 0 flat per-element kernels (Kblk*), 1 16-element tiles
   hipStream_t own_stream = nullptr, stream = nullptr;
   std::string err;
 
@@ -149,6 +149,8 @@ hipError_t launch_update_wadg(bbw_ctx *c, float fa, float fb, float fdt, const f
 hipError_t launch_update(bbw_ctx *c, float fa, float fb, float fdt);
 hipError_t launch_update_wadg_mfma(bbw_ctx *c, float fa, float fb, float fdt, const float *ftime_dev,
                                    float ftime);
+hipError_t launch_volume_tiled(bbw_ctx *c);
+hipError_t launch_surface_tiled(bbw_ctx *c);
 hipError_t launch_fused(bbw_ctx *c, const float *Qin, float *Qout, float fa, float fb, float fdt,
                         const float *ftime_dev, float ftime, int project_velocity);
 hipError_t launch_fused_mfma(bbw_ctx *c, const float *Qin, float *Qout, float fa, float fb,

@@ -25,10 +25,12 @@ def oracle_ops(P, dtype):
     return orc.OracleOps(P.ops, dtype)
 
 
-@pytest.mark.parametrize("N", [3, 4, 5])
-def test_volume(N):
-    print(f"case volume N={N}", flush=True)
+@pytest.mark.parametrize("impl", [_lib.PLAIN_FLAT, _lib.PLAIN_TILED])
+@pytest.mark.parametrize("N", [2, 3, 4, 5, 6])
+def test_volume(N, impl):
+    print(f"case volume N={N} impl={impl}", flush=True)
     P, s = make(N, 3)
+    s.set_option(_lib.OPT_PLAIN_IMPL, impl)
     Q = P.random_state()
     s.WaveSetData3d(Q)
     s.volume()
@@ -39,10 +41,12 @@ def test_volume(N):
         assert rel(got, ref) < tol
 
 
-@pytest.mark.parametrize("N", [3, 4, 5])
-def test_surface(N):
-    print(f"case surface N={N}", flush=True)
+@pytest.mark.parametrize("impl", [_lib.PLAIN_FLAT, _lib.PLAIN_TILED])
+@pytest.mark.parametrize("N", [2, 3, 4, 5, 6])
+def test_surface(N, impl):
+    print(f"case surface N={N} impl={impl}", flush=True)
     P, s = make(N, 3)
+    s.set_option(_lib.OPT_PLAIN_IMPL, impl)
     Q = P.random_state()
     R0 = P.random_state()
     s.WaveSetData3d(Q)
@@ -55,12 +59,12 @@ def test_surface(N):
         assert rel(got, ref) < tol
 
 
-@pytest.mark.parametrize("impl", [_lib.UPDATE_VECTOR, _lib.UPDATE_MFMA])
+@pytest.mark.parametrize("impl", [_lib.PLAIN_FLAT, _lib.PLAIN_TILED])
 @pytest.mark.parametrize("N", [2, 3, 4, 5, 6])
 def test_update_wadg(N, impl):
     print(f"case update_wadg N={N} impl={impl}", flush=True)
     P, s = make(N, 3)
-    s.set_option(_lib.OPT_UPDATE_IMPL, impl)
+    s.set_option(_lib.OPT_PLAIN_IMPL, impl)
     Q, R, res = P.random_state(), P.random_state(), P.random_state()
     s.WaveSetData3d(Q)
     s.set_rhs(R)
