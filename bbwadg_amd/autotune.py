@@ -2,8 +2,8 @@
 ``cpp/runBlkTimings:4-24`` and the Kblk argv of ``cpp/main.cpp:43-52``).
 
 * ``tune_single(solver, dt)``  times the fused implementations (vector kernel with several
-  elements-per-block, MFMA kernel with its 16-element tile) and the plain kernels' Kblk, keeps
-  the fastest and sets it on the solver;
+  elements-per-block, MFMA kernel with its 16-element tile) and the plain kernels (tiled, or
+  flat with several Kblk), keeps the fastest and sets it on the solver;
 * ``tune_batch(batch)``       times shots-per-tile x persistent blocks-per-CU on a batch and
   returns the fastest (the batch is reset after every trial).
 
@@ -25,8 +25,9 @@ def _time_steps(solver, dt, nsteps, mode):
 
 
 def tune_single(solver, dt, nsteps=10, kblk_vector=(4, 8, 16), kblk_plain=(1, 2, 4, 8)):
-    """Pick the fused implementation and the plain kernels' elements-per-block; returns a dict
-    of the trials and the choice (also applied to the solver).  The state is restored."""
+    """Pick the fused implementation and the plain kernels (16-element tiles, or the flat
+    kernels with their elements-per-block); returns a dict of the trials and the choice (also
+    applied to the solver).  The state is restored."""
     Q0, res0 = solver.WaveGetData3d(), solver.get_res()
     trials = []
     for kb in kblk_vector:
@@ -41,13 +42,19 @@ def tune_single(solver, dt, nsteps=10, kblk_vector=(4, 8, 16), kblk_plain=(1, 2,
     if impl == "vector":
         solver.set_option(_lib.OPT_KBLK_F, kb)
     plain = []
+    solver.set_option(_lib.OPT_PLAIN_IMPL, _lib.PLAIN_TILED)
+    plain.append(("tiled", _time_steps(solver, dt, nsteps, MODE_PLAIN)))
+    solver.set_option(_lib.OPT_PLAIN_IMPL, _lib.PLAIN_FLAT)
     for kb in kblk_plain:
         for opt in (_lib.OPT_KBLK_V, _lib.OPT_KBLK_S, _lib.OPT_KBLK_U):
             solver.set_option(opt, kb)
         plain.append((kb, _time_steps(solver, dt, nsteps, MODE_PLAIN)))
     best_plain = min(plain, key=lambda t: t[1])
-    for opt in (_lib.OPT_KBLK_V, _lib.OPT_KBLK_S, _lib.OPT_KBLK_U):
-        solver.set_option(opt, best_plain[0])
+    if best_plain[0] == "tiled":
+        solver.set_option(_lib.OPT_PLAIN_IMPL, _lib.PLAIN_TILED)
+    else:
+        for opt in (_lib.OPT_KBLK_V, _lib.OPT_KBLK_S, _lib.OPT_KBLK_U):
+            solver.set_option(opt, best_plain[0])
     solver.WaveSetData3d(Q0)
     solver.set_res(res0)
     return {"fused_trials": trials, "fused_choice": best[0], "plain_trials": plain,

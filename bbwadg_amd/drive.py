@@ -3,8 +3,9 @@ reference's ``runBlkTimings`` / ``profileKernels`` runs, ``cpp/runBlkTimings:4-2
 
     python -m bbwadg_amd.drive --mesh 32 --N 5 --steps 10 --mode fused --impl mfma
 
-Builds a seeded cube mesh with Fourier c^2, runs ``--steps`` LSRK45 steps in the chosen
-mode and prints one line with the time per stage.
+Builds a seeded cube mesh with Fourier c^2 (or, with ``--elastic``, seeded Lame fields and
+9 fields on the plain path), runs ``--steps`` LSRK45 steps in the chosen mode and prints one
+line with the time per stage.
 """
 import argparse
 import time
@@ -29,6 +30,20 @@ def setup(n, N, seed=1, **kw):
     return s
 
 
+def setup_elastic(n, N, seed=1):
+    ops = O.build_ref_ops(N)
+    m = M.cube_mesh(n)
+    M.face_pair(m)
+    dm = M.device_arrays(m, ops, 9)
+    xq, yq, zq = media.cubature_points(m, ops)
+    lam, mu, rho = media.lame_fourier(seed)
+    s = WaveSolver(m, N, nfields=9, ops=ops, dmesh=dm)
+    s.InitElastic(1.0 / rho(xq, yq, zq), lam(xq, yq, zq), mu(xq, yq, zq))
+    rng = np.random.default_rng(seed)
+    s.WaveSetData3d(rng.uniform(-1, 1, (m.K, 9, ops.Np)).astype(np.float32))
+    return s
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--mesh", type=int, default=32)
@@ -37,6 +52,7 @@ def main():
     ap.add_argument("--mode", choices=["fused", "plain"], default="fused")
     ap.add_argument("--impl", choices=["mfma", "vector"], default="mfma")
     ap.add_argument("--project-velocity", action="store_true")
+    ap.add_argument("--elastic", action="store_true", help="elastic (9 fields), plain stage")
     ap.add_argument("--copy-bytes", type=int, default=0,
                     help="instead: one device-to-device copy of this many bytes (counter calibration)")
     args = ap.parse_args()
@@ -50,9 +66,13 @@ def main():
         torch.cuda.synchronize()
         print(f"copied {n * 4} bytes", flush=True)
         return
-    s = setup(args.mesh, args.N)
-    s.set_option(_lib.OPT_FUSED_IMPL, _lib.FUSED_MFMA if args.impl == "mfma" else _lib.FUSED_VECTOR)
-    mode = MODE_FUSED if args.mode == "fused" else MODE_PLAIN
+    if args.elastic:
+        s = setup_elastic(args.mesh, args.N)
+        mode = MODE_PLAIN
+    else:
+        s = setup(args.mesh, args.N)
+        s.set_option(_lib.OPT_FUSED_IMPL, _lib.FUSED_MFMA if args.impl == "mfma" else _lib.FUSED_VECTOR)
+        mode = MODE_FUSED if args.mode == "fused" else MODE_PLAIN
     dt = 0.5 / args.mesh / args.N ** 2
     s.step(2, dt, mode=mode, project_velocity=args.project_velocity)
     s.sync()
@@ -60,8 +80,9 @@ def main():
     s.step(args.steps, dt, mode=mode, project_velocity=args.project_velocity)
     s.sync()
     el = time.perf_counter() - t0
-    dofs = s.K * s.ops.Np * 4 * 5 * args.steps
-    print(f"K={s.K} N={args.N} mode={args.mode} impl={args.impl} steps={args.steps} "
+    dofs = s.K * s.ops.Np * s.nfields * 5 * args.steps
+    print(f"K={s.K} N={args.N} fields={s.nfields} mode={'plain' if mode == MODE_PLAIN else 'fused'} "
+          f"impl={args.impl} steps={args.steps} "
           f"ms/stage={el / (5 * args.steps) * 1e3:.4f} DOF-stage/s={dofs / el:.4e}", flush=True)
 
 

@@ -205,3 +205,18 @@ def test_source_and_receivers_match_oracle():
                record=lambda st, Qs: rec.append(np.einsum("rj,rj->r", B, Qs[relems, 0])))
     assert rel_l2(got, ref) < 1e-5
     assert rel_l2(tr, np.array(rec)) < 1e-5
+
+
+def test_tune_single_restores_state_and_applies_choice():
+    """The single-shot autotuner times every fused and plain variant, applies the fastest and
+    leaves the state as it was."""
+    from bbwadg_amd import autotune
+    print("case tune_single", flush=True)
+    P, s = make(3, 3)
+    Q, res = P.random_state(), P.random_state()
+    s.WaveSetData3d(Q)
+    s.set_res(res)
+    r = autotune.tune_single(s, P.dt_cfl(), nsteps=2, kblk_vector=(4,), kblk_plain=(2,))
+    assert {t[0] for t in r["plain_trials"]} == {"tiled", 2}
+    assert len(r["fused_trials"]) == 2
+    assert np.array_equal(s.WaveGetData3d(), Q) and np.array_equal(s.get_res(), res)
