@@ -304,6 +304,7 @@ hipError_t launch_surface_elas(bbw_ctx *c) {
 
 hipError_t launch_update_elas(bbw_ctx *c, float fa, float fb, float fdt, const float *ftd,
                               float ft) {
+  if (c->plain_impl == BBW_PLAIN_TILED) return launch_update_elas_mfma(c, fa, fb, fdt, ftd, ft);
   switch (c->N) {
     case 1: return update_elas_n<1>(c, fa, fb, fdt, ftd, ft);
     case 2: return update_elas_n<2>(c, fa, fb, fdt, ftd, ft);

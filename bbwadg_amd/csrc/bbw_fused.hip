@@ -549,6 +549,224 @@ hipError_t launch_surface_tiled(bbw_ctx *c) {
   return hipErrorInvalidValue;
 }
 
+// Elastic WADG update on MFMA (rk_update_elas, elas_cpp/okl/ElasKernelsWADG.okl:286-433), per
+// 16-element tile: the 9 fields go through Vq in three groups of three, whose cubature-point
+// maps are independent (velocities x 1/rho; normal stresses x (lambda, mu); shear stresses
+// x mu), then back through Pq; LSRK stage with the source.
+template <int N>
+struct ElasUpdLayout {
+  static constexpr int Np = Dims<N>::Np, Nq = Dims<N>::Nq, TB = 16, NpNf = 9 * Np;
+  static constexpr int NQT = (Nq + 15) / 16, NQP = 16 * NQT, NPT = (Np + 15) / 16;
+  static constexpr int KTV = (Np + 3) / 4, KTP = NQP / 4, QTW = (NQT + 3) / 4;
+  static constexpr int KBV = (KTV + 3) / 4, KBP = (KTP + 3) / 4;
+  static constexpr int RS0 = 9 * Np > 8 * Np + 4 * KTV ? 9 * Np : 8 * Np + 4 * KTV;
+  static constexpr int RS = RS0 + ((34 - RS0 % 32) % 32);
+  // rhs rows [TB][RS] | S [3 fields][NQP][16] | source slots [TB]
+  static constexpr size_t floats = (size_t)TB * RS + 3 * NQP * 16 + TB;
+};
+
+template <int N>
+__global__ __launch_bounds__(256, N <= 4 ? 4 : N == 5 ? 3 : 2) void k_update_elas_mfma(
+    int K, FusedOps o, const float *__restrict__ rhoinvq, const float *__restrict__ lambdaq,
+    const float *__restrict__ muq, SrcArgs src, const float *ftime_dev, float ftime, float fa,
+    float fb, float fdt, const float *__restrict__ rhs, float *__restrict__ res, float *__restrict__ Q) {
+  using L = ElasUpdLayout<N>;
+  constexpr int Np = L::Np, Nq = L::Nq, TB = L::TB, NpNf = L::NpNf, RS = L::RS, NQP = L::NQP;
+  constexpr int KTV = L::KTV, KTP = L::KTP, KBV = L::KBV, KBP = L::KBP;
+  constexpr unsigned ROW = NpNf * 4u;
+  extern __shared__ float lds[];
+  float *sR = lds;                         // [TB][RS]
+  float *S = sR + TB * RS;                 // [3][NQP][16]
+  int *sSlot = (int *)(S + 3 * NQP * 16);  // [TB]
+  const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int col = lane & 15, kr = lane >> 4;
+  const int k0 = blockIdx.x * TB, nk = min(TB, K - k0);
+  const size_t base = (size_t)k0 * NpNf;
+  const unsigned tbytes = (unsigned)nk * ROW, qbytes = (unsigned)nk * Nq * 4u;
+  const brsrc rRh = mkrsrc(rhs + base, tbytes), rRs = mkrsrc(res + base, tbytes);
+  const brsrc rQ = mkrsrc(Q + base, tbytes);
+  const brsrc rRho = mkrsrc(rhoinvq + (size_t)k0 * Nq, qbytes), rLam = mkrsrc(lambdaq + (size_t)k0 * Nq, qbytes);
+  const brsrc rMu = mkrsrc(muq + (size_t)k0 * Nq, qbytes);
+  const brsrc rAV = mkrsrc(o.VqBlk, (unsigned)L::NQT * KBV * 1024u);
+  const brsrc rAP = mkrsrc(o.PqBlk, (unsigned)L::NPT * KBP * 1024u);
+  const bool has_src = src.nsrc > 0 && k0 + nk - 1 >= src.first && k0 <= src.last;
+
+  // ---- 1: rhs tile -> LDS; materials at the cubature points (MFMA output layout); Vq, Pq blocks
+  for (int rr = tid; rr < NpNf; rr += 256)
+#pragma unroll
+    for (int e = 0; e < TB; ++e) sR[e * RS + rr] = bld(rRh, rr * 4u + e * ROW);
+  constexpr int KPAD = L::RS0 - NpNf;
+  static_assert(TB * KPAD <= 256, "one padding word per thread");
+  if (KPAD > 0 && tid < TB * KPAD) sR[(tid / KPAD) * RS + NpNf + tid % KPAD] = 0.f;
+  if (has_src && tid < TB) sSlot[tid] = tid < nk ? src_slot2(src, k0 + tid) : -1;
+  float rho[L::QTW][4], lam[L::QTW][4], mu[L::QTW][4];
+  f4 aV[L::QTW][KBV], aP[KBP];
+  const unsigned co = (unsigned)(col * Nq + 4 * kr) * 4u;
+#pragma unroll
+  for (int t = 0; t < L::QTW; ++t) {
+    const int qt = wave + 4 * t;
+    if (qt < L::NQT) {
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        rho[t][r] = bld(rRho, co + (qt * 16 + r) * 4u);
+        lam[t][r] = bld(rLam, co + (qt * 16 + r) * 4u);
+        mu[t][r] = bld(rMu, co + (qt * 16 + r) * 4u);
+      }
+#pragma unroll
+      for (int kb = 0; kb < KBV; ++kb) aV[t][kb] = bld4_s(rAV, lane * 16u, (unsigned)(qt * KBV + kb) * 1024u);
+    }
+  }
+  if (wave < L::NPT) {
+#pragma unroll
+    for (int kb = 0; kb < KBP; ++kb) aP[kb] = bld4_s(rAP, lane * 16u, (unsigned)(wave * KBP + kb) * 1024u);
+  }
+  __syncthreads();
+
+#pragma unroll 1
+  for (int g = 0; g < 3; ++g) {
+    // ---- 2: S_b = map_g(Vq . rhs_{3g+b}) (M = cubature points, N = elements, K = nodes)
+    {
+      const float *Bp = sR + col * RS + 3 * g * Np + kr;
+#pragma unroll
+      for (int t = 0; t < L::QTW; ++t) {
+        const int qt = wave + 4 * t;
+        if (qt >= L::NQT) break;
+        f4 acc[3];
+#pragma unroll
+        for (int b = 0; b < 3; ++b) acc[b] = f4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int kb = 0; kb < KBV; ++kb)
+#pragma unroll
+          for (int sb = 0; sb < 4; ++sb) {
+            const int kt = 4 * kb + sb;
+            if (kt < KTV) {
+#pragma unroll
+              for (int b = 0; b < 3; ++b)
+                acc[b] = __builtin_amdgcn_mfma_f32_16x16x4f32(aV[t][kb][sb], Bp[b * Np + kt * 4], acc[b], 0, 0, 0);
+            }
+          }
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int q = qt * 16 + 4 * kr + r;
+          float s0, s1, s2;
+          if (g == 0) {
+            s0 = rho[t][r] * acc[0][r];
+            s1 = rho[t][r] * acc[1][r];
+            s2 = rho[t][r] * acc[2][r];
+          } else if (g == 1) {
+            const float A = 2.f * mu[t][r] + lam[t][r], l = lam[t][r];
+            s0 = A * acc[0][r] + l * acc[1][r] + l * acc[2][r];
+            s1 = A * acc[1][r] + l * acc[0][r] + l * acc[2][r];
+            s2 = A * acc[2][r] + l * acc[0][r] + l * acc[1][r];
+          } else {
+            s0 = mu[t][r] * acc[0][r];
+            s1 = mu[t][r] * acc[1][r];
+            s2 = mu[t][r] * acc[2][r];
+          }
+          const bool in = q < Nq;
+          S[q * 16 + col] = in ? s0 : 0.f;
+          S[(NQP + q) * 16 + col] = in ? s1 : 0.f;
+          S[(2 * NQP + q) * 16 + col] = in ? s2 : 0.f;
+        }
+      }
+    }
+    __syncthreads();
+    // ---- 3: rhs_{3g+b} = Pq . S_b (M = nodes, K = cubature points), over the group's rows
+    for (int mt = wave; mt < L::NPT; mt += 4) {
+      if (mt != wave) {
+#pragma unroll
+        for (int kb = 0; kb < KBP; ++kb) aP[kb] = bld4_s(rAP, lane * 16u, (unsigned)(mt * KBP + kb) * 1024u);
+      }
+      f4 acc[3];
+#pragma unroll
+      for (int b = 0; b < 3; ++b) acc[b] = f4{0.f, 0.f, 0.f, 0.f};
+      const float *Bp = S + kr * 16 + col;
+#pragma unroll
+      for (int kb = 0; kb < KBP; ++kb)
+#pragma unroll
+        for (int sb = 0; sb < 4; ++sb) {
+          const int kt = 4 * kb + sb;
+          if (kt < KTP) {
+#pragma unroll
+            for (int b = 0; b < 3; ++b)
+              acc[b] = __builtin_amdgcn_mfma_f32_16x16x4f32(aP[kb][sb], Bp[(b * NQP + kt * 4) * 16], acc[b], 0, 0, 0);
+          }
+        }
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int n = mt * 16 + 4 * kr + r;
+        if (n < Np) {
+#pragma unroll
+          for (int b = 0; b < 3; ++b) sR[col * RS + (3 * g + b) * Np + n] = acc[b][r];
+        }
+      }
+      if (mt != wave && wave < L::NPT) {
+#pragma unroll
+        for (int kb = 0; kb < KBP; ++kb) aP[kb] = bld4_s(rAP, lane * 16u, (unsigned)(wave * KBP + kb) * 1024u);
+      }
+    }
+    __syncthreads();
+  }
+
+  // ---- 4: LSRK stage: res = fa res + fdt (rhs + source); Q += fb res
+  const float ft = ftime_dev ? *ftime_dev : ftime;
+  for (int rr = tid; rr < NpNf; rr += 256) {
+    const int fr = rr / Np, nr = rr - fr * Np;
+    float wsel = 0.f;
+#pragma unroll
+    for (int f = 0; f < 9; ++f) wsel = fr == f ? src.w[f] : wsel;
+#pragma unroll
+    for (int e = 0; e < TB; ++e) {
+      float v = sR[e * RS + rr];
+      if (has_src) {
+        const int slot = sSlot[e];
+        if (slot >= 0) v += wsel * ft * src.coef[(size_t)slot * Np + nr];
+      }
+      const unsigned gi = rr * 4u + e * ROW;
+      const float x = fa * bld(rRs, gi) + fdt * v;
+      bst(rRs, x, gi);
+      bst(rQ, bld(rQ, gi) + fb * x, gi);
+    }
+  }
+}
+
+template <int N>
+static hipError_t update_elas_mfma_n(bbw_ctx *c, float fa, float fb, float fdt, const float *ftd,
+                                     float ft, const SrcArgs &s) {
+  using L = ElasUpdLayout<N>;
+  const size_t sh = sizeof(float) * L::floats;
+  if ((unsigned long long)c->K * 36 * L::Np >= (1ull << 32)) return hipErrorInvalidValue;
+  if (sh > 65536) {
+    hipError_t e = hipFuncSetAttribute((const void *)k_update_elas_mfma<N>,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh);
+    if (e != hipSuccess) return e;
+  }
+  hipLaunchKernelGGL(k_update_elas_mfma<N>, dim3((c->K + L::TB - 1) / L::TB), dim3(256), sh, c->stream,
+                     c->K, fused_ops(c), c->rhoinv, c->lambda, c->mu, s, ftd, ft, fa, fb, fdt, c->rhs,
+                     c->res, c->Q[c->cur]);
+  return hipGetLastError();
+}
+
+hipError_t launch_update_elas_mfma(bbw_ctx *c, float fa, float fb, float fdt, const float *ftd,
+                                   float ft) {
+  SrcArgs s;
+  s.nsrc = c->nsrc;
+  s.first = c->src_first;
+  s.last = c->src_last;
+  s.elems = c->src_elems;
+  s.coef = c->src_coef;
+  for (int f = 0; f < 9; ++f) s.w[f] = c->src_w[f];
+  switch (c->N) {
+    case 1: return update_elas_mfma_n<1>(c, fa, fb, fdt, ftd, ft, s);
+    case 2: return update_elas_mfma_n<2>(c, fa, fb, fdt, ftd, ft, s);
+    case 3: return update_elas_mfma_n<3>(c, fa, fb, fdt, ftd, ft, s);
+    case 4: return update_elas_mfma_n<4>(c, fa, fb, fdt, ftd, ft, s);
+    case 5: return update_elas_mfma_n<5>(c, fa, fb, fdt, ftd, ft, s);
+    case 6: return update_elas_mfma_n<6>(c, fa, fb, fdt, ftd, ft, s);
+  }
+  return hipErrorInvalidValue;
+}
+
 FusedOps fused_ops(const bbw_ctx *c) {
   FusedOps o;
   o.Fmask = c->Fmask;

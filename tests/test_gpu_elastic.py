@@ -3,7 +3,7 @@ against the elastic oracle on the same seeded inputs; tolerances as the acoustic
 import numpy as np
 import pytest
 
-from bbwadg_amd import media, mesh as M, operators as O
+from bbwadg_amd import _lib, media, mesh as M, operators as O
 from bbwadg_amd.solver import WaveSolver, MODE_PLAIN
 import oracle as orc
 from cases import rel, rel_l2
@@ -24,10 +24,12 @@ def make(N, n, seed=0, tau=(1.0, 1.0)):
     return ops, m, dm, mats, s
 
 
-@pytest.mark.parametrize("N", [3, 4])
-def test_elastic_kernels(N):
-    print(f"case elastic kernels N={N}", flush=True)
+@pytest.mark.parametrize("impl", [_lib.PLAIN_FLAT, _lib.PLAIN_TILED])
+@pytest.mark.parametrize("N", [2, 3, 4, 5, 6])
+def test_elastic_kernels(N, impl):
+    print(f"case elastic kernels N={N} impl={impl}", flush=True)
     ops, m, dm, mats, s = make(N, 3, tau=(0.8, 1.2))
+    s.set_option(_lib.OPT_PLAIN_IMPL, impl)
     rng = np.random.default_rng(N)
     Q = rng.uniform(-1, 1, (m.K, 9, ops.Np)).astype(np.float32)
     R0 = rng.uniform(-1, 1, Q.shape).astype(np.float32)
