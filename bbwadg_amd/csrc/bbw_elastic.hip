@@ -291,6 +291,7 @@ hipError_t launch_volume_elas(bbw_ctx *c) {
 }
 
 hipError_t launch_surface_elas(bbw_ctx *c) {
+  if (c->plain_impl == BBW_PLAIN_TILED) return launch_surface_elas_tiled(c);
   switch (c->N) {
     case 1: return surface_elas_n<1>(c);
     case 2: return surface_elas_n<2>(c);

@@ -767,6 +767,206 @@ hipError_t launch_update_elas_mfma(bbw_ctx *c, float fa, float fb, float fdt, co
   return hipErrorInvalidValue;
 }
 
+// Elastic surface kernel on 16-element tiles (rk_surface_elas,
+// elas_cpp/okl/ElasKernelsWADG.okl:128-284, with the BB lift): central flux with tau_v / tau_s
+// penalties (Q+ = 0 on the boundary) for the 9 fields in registers, then L0 per face and EEL
+// for three fields at a time, accumulated into rhs.
+template <int N>
+struct ElasSurfLayout {
+  static constexpr int Np = Dims<N>::Np, Nfp = Dims<N>::Nfp, NfpNf = Dims<N>::NfpNf, TB = 16;
+  static constexpr int RowF = 9 * Np, QT = TB * RowF;
+  static constexpr unsigned QB = QT * 4u, QIT = (QB + 4095) / 4096;
+  static constexpr int NG3 = 256 / NfpNf, EPT3 = (TB + NG3 - 1) / NG3;
+  static constexpr int GS = ((Np + 63) / 64) * 64, NG = 256 / GS, EPTV = (TB + NG - 1) / NG;
+  static constexpr int GW = 3 * TB * NfpNf;      // one field group's face data
+  static constexpr int W = QT > 2 * GW ? QT : 2 * GW;
+  static constexpr size_t floats = (size_t)W + TB * 16;
+};
+
+template <int N>
+__global__ __launch_bounds__(256) void k_surface_elas_t(int K, const float *__restrict__ fgeo,
+                                                        const int *__restrict__ Fmask,
+                                                        const int *__restrict__ vmapP,
+                                                        const int *__restrict__ EEL_ids,
+                                                        const float *__restrict__ EEL_vals,
+                                                        const int *__restrict__ L0_ids,
+                                                        const float *__restrict__ L0_vals,
+                                                        float tau_v, float tau_s,
+                                                        const float *__restrict__ Q, float *__restrict__ rhs) {
+  using L = ElasSurfLayout<N>;
+  constexpr int Np = L::Np, Nfp = L::Nfp, NfpNf = L::NfpNf, TB = L::TB, RowF = L::RowF;
+  constexpr int EEL_nnz = Dims<N>::EEL_nnz, L0_nnz = Dims<N>::L0_nnz;
+  extern __shared__ float lds[];
+  float *sQ = lds;                 // [TB][9][Np]            (phases 1-2)
+  float *sFl = lds;                // [TB][3][NfpNf] flux    (group phases)
+  float *sT = lds + L::GW;         // [TB][3][NfpNf] L0 flux (group phases)
+  float *sF = lds + L::W;          // [TB][16]
+  const int tid = threadIdx.x;
+  const int k0 = blockIdx.x * TB, nk = min(TB, K - k0);
+  const brsrc rO = mkrsrc(rhs + (size_t)k0 * RowF, (unsigned)nk * RowF * 4u);
+  const brsrc rT = mkrsrc(Q, (unsigned)K * RowF * 4u);
+  // ---- 1: Q tile, fgeo, the thread's vmapP entries and neighbour traces (9 fields)
+  {
+    const brsrc rQ = mkrsrc(Q + (size_t)k0 * RowF, (unsigned)nk * RowF * 4u);
+    float4 *sQ4 = (float4 *)sQ;
+#pragma unroll
+    for (unsigned it = 0; it < L::QIT; ++it) {
+      const unsigned off = tid * 16u + it * 4096u;
+      if (L::QIT * 4096u == L::QB || off < L::QB) sQ4[off / 16] = bld4(rQ, off);
+    }
+  }
+  sF[tid] = bld(mkrsrc(fgeo + (size_t)k0 * 16, (unsigned)nk * 64u), tid * 4u);
+  const int g3 = tid / NfpNf, m3 = tid - g3 * NfpNf;
+  const bool on3 = g3 < L::NG3;
+  const int f3 = m3 / Nfp;
+  const int fid3 = on3 ? Fmask[m3] : 0;
+  int vm[L::EPT3];
+  {
+    const brsrc rv = mkrsrc(vmapP + (size_t)k0 * NfpNf, (unsigned)nk * NfpNf * 4u);
+#pragma unroll
+    for (int i = 0; i < L::EPT3; ++i) vm[i] = bldi(rv, (unsigned)((g3 + L::NG3 * i) * NfpNf + m3) * 4u);
+  }
+  float fl[L::EPT3][9];
+#pragma unroll
+  for (int i = 0; i < L::EPT3; ++i)
+#pragma unroll
+    for (int c = 0; c < 9; ++c) fl[i][c] = bld(rT, (unsigned)(vm[i] + c * Np) * 4u);
+  __syncthreads();
+  // ---- 2: flux per face node (into the same registers)
+  if (on3) {
+#pragma unroll
+    for (int i = 0; i < L::EPT3; ++i) {
+      const int e = g3 + L::NG3 * i;
+      if (L::NG3 * L::EPT3 == TB || e < TB) {
+        const float *q = sQ + e * RowF + fid3;
+        const float *F = sF + 16 * e + 4 * f3;
+        const float Fs = .5f * F[0], nx = F[1], ny = F[2], nz = F[3];
+        const bool isB = vm[i] == fid3 + (k0 + e) * RowF;
+        float dQ[9];
+#pragma unroll
+        for (int c = 0; c < 9; ++c) dQ[c] = (isB ? 0.f : fl[i][c]) - q[c * Np];
+        float fc[9];
+        fc[0] = dQ[3] * nx + dQ[8] * ny + dQ[7] * nz;
+        fc[1] = dQ[8] * nx + dQ[4] * ny + dQ[6] * nz;
+        fc[2] = dQ[7] * nx + dQ[6] * ny + dQ[5] * nz;
+        fc[3] = dQ[0] * nx;
+        fc[4] = dQ[1] * ny;
+        fc[5] = dQ[2] * nz;
+        fc[6] = dQ[2] * ny + dQ[1] * nz;
+        fc[7] = dQ[2] * nx + dQ[0] * nz;
+        fc[8] = dQ[1] * nx + dQ[0] * ny;
+        fl[i][0] = Fs * (fc[0] + tau_v * (fc[3] * nx + fc[8] * ny + fc[7] * nz));
+        fl[i][1] = Fs * (fc[1] + tau_v * (fc[8] * nx + fc[4] * ny + fc[6] * nz));
+        fl[i][2] = Fs * (fc[2] + tau_v * (fc[7] * nx + fc[6] * ny + fc[5] * nz));
+        fl[i][3] = Fs * (fc[3] + tau_s * (fc[0] * nx));
+        fl[i][4] = Fs * (fc[4] + tau_s * (fc[1] * ny));
+        fl[i][5] = Fs * (fc[5] + tau_s * (fc[2] * nz));
+        fl[i][6] = Fs * (fc[6] + tau_s * (fc[2] * ny + fc[1] * nz));
+        fl[i][7] = Fs * (fc[7] + tau_s * (fc[2] * nx + fc[0] * nz));
+        fl[i][8] = Fs * (fc[8] + tau_s * (fc[1] * nx + fc[0] * ny));
+      }
+    }
+  }
+  __syncthreads();
+  const int gv = tid / L::GS, nv = tid - gv * L::GS;
+  const bool onv = gv < L::NG && nv < Np;
+  const int nt = m3 - f3 * Nfp;
+#pragma unroll
+  for (int g = 0; g < 3; ++g) {
+    // ---- 3: the group's flux -> LDS
+    if (on3) {
+#pragma unroll
+      for (int i = 0; i < L::EPT3; ++i) {
+        const int e = g3 + L::NG3 * i;
+        if (L::NG3 * L::EPT3 == TB || e < TB) {
+#pragma unroll
+          for (int b = 0; b < 3; ++b) sFl[(e * 3 + b) * NfpNf + m3] = fl[i][3 * g + b];
+        }
+      }
+    }
+    __syncthreads();
+    // ---- 4: L0 per face
+    if (on3) {
+      float Lv[L0_nnz];
+      const float *Xp[L0_nnz];
+      const float *Xb = sFl + g3 * 3 * NfpNf + f3 * Nfp;
+#pragma unroll
+      for (int j = 0; j < L0_nnz; ++j) {
+        Lv[j] = L0_vals[nt + j * Nfp];
+        Xp[j] = Xb + L0_ids[nt + j * Nfp];
+      }
+#pragma unroll
+      for (int i = 0; i < L::EPT3; ++i) {
+        const int e = g3 + L::NG3 * i;
+        if (L::NG3 * L::EPT3 == TB || e < TB) {
+#pragma unroll
+          for (int b = 0; b < 3; ++b) {
+            float v = 0.f;
+#pragma unroll
+            for (int j = 0; j < L0_nnz; ++j) v += Lv[j] * Xp[j][(i * L::NG3 * 3 + b) * NfpNf];
+            sT[(e * 3 + b) * NfpNf + m3] = v;
+          }
+        }
+      }
+    }
+    __syncthreads();
+    // ---- 5: EEL rows, accumulated into the group's rhs rows
+    if (onv) {
+      float Ev[EEL_nnz];
+      const float *Ep[EEL_nnz];
+      const float *Tb = sT + gv * 3 * NfpNf;
+#pragma unroll
+      for (int j = 0; j < EEL_nnz; ++j) {
+        Ev[j] = EEL_vals[nv + j * Np];
+        Ep[j] = Tb + EEL_ids[nv + j * Np];
+      }
+      float ro[L::EPTV][3];
+#pragma unroll
+      for (int i = 0; i < L::EPTV; ++i)
+#pragma unroll
+        for (int b = 0; b < 3; ++b)
+          ro[i][b] = bld(rO, (unsigned)((gv + L::NG * i) * RowF + (3 * g + b) * Np + nv) * 4u);
+#pragma unroll
+      for (int i = 0; i < L::EPTV; ++i) {
+#pragma unroll
+        for (int b = 0; b < 3; ++b) {
+          float v = 0.f;
+#pragma unroll
+          for (int j = 0; j < EEL_nnz; ++j) v += Ev[j] * Ep[j][(i * L::NG * 3 + b) * NfpNf];
+          bst(rO, ro[i][b] + v, (unsigned)((gv + L::NG * i) * RowF + (3 * g + b) * Np + nv) * 4u);
+        }
+      }
+    }
+  }
+}
+
+template <int N>
+static hipError_t surface_elas_t_n(bbw_ctx *c) {
+  using L = ElasSurfLayout<N>;
+  const size_t sh = sizeof(float) * L::floats;
+  if ((unsigned long long)c->K * 36 * L::Np >= (1ull << 32)) return hipErrorInvalidValue;
+  if (sh > 65536) {
+    hipError_t e = hipFuncSetAttribute((const void *)k_surface_elas_t<N>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh);
+    if (e != hipSuccess) return e;
+  }
+  hipLaunchKernelGGL(k_surface_elas_t<N>, dim3((c->K + L::TB - 1) / L::TB), dim3(256), sh, c->stream, c->K,
+                     c->fgeo, c->Fmask, c->vmapP, c->EEL_ids, c->EEL_vals, c->L0_ids, c->L0_vals,
+                     c->tau_v, c->tau_s, c->Q[c->cur], c->rhs);
+  return hipGetLastError();
+}
+
+hipError_t launch_surface_elas_tiled(bbw_ctx *c) {
+  switch (c->N) {
+    case 1: return surface_elas_t_n<1>(c);
+    case 2: return surface_elas_t_n<2>(c);
+    case 3: return surface_elas_t_n<3>(c);
+    case 4: return surface_elas_t_n<4>(c);
+    case 5: return surface_elas_t_n<5>(c);
+    case 6: return surface_elas_t_n<6>(c);
+  }
+  return hipErrorInvalidValue;
+}
+
 FusedOps fused_ops(const bbw_ctx *c) {
   FusedOps o;
   o.Fmask = c->Fmask;
