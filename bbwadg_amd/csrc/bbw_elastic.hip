@@ -279,6 +279,7 @@ static hipError_t update_elas_n(bbw_ctx *c, float fa, float fb, float fdt, const
 }
 
 hipError_t launch_volume_elas(bbw_ctx *c) {
+  if (c->plain_impl == BBW_PLAIN_TILED) return launch_volume_elas_tiled(c);
   switch (c->N) {
     case 1: return volume_elas_n<1>(c);
     case 2: return volume_elas_n<2>(c);

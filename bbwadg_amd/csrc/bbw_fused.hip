@@ -967,6 +967,144 @@ hipError_t launch_surface_elas_tiled(bbw_ctx *c) {
   return hipErrorInvalidValue;
 }
 
+// Elastic volume kernel on 16-element tiles (rk_volume_elas, elas_cpp/okl/ElasKernelsWADG.okl
+// :25-124, with the 4-nnz BB derivative operators): the tile's 9 fields are repacked per node as
+// (v1 v2 v3 sxx) (syy szz syz sxz) sxy, so each of the 16 operator entries costs three LDS reads.
+template <int N>
+struct ElasVolLayout {
+  static constexpr int Np = Dims<N>::Np, TB = 16, RowF = 9 * Np, QT = TB * RowF;
+  static constexpr unsigned QB = QT * 4u, QIT = (QB + 4095) / 4096;
+  static constexpr int GS = ((Np + 63) / 64) * 64, NG = 256 / GS, EPTV = (TB + NG - 1) / NG;
+  // Q tile [TB][9][Np], then packed [TB][Np] float4 | [TB][Np] float4 | [TB][Np] float | vgeo
+  static constexpr size_t floats = (size_t)QT + TB * 9;
+};
+
+template <int N>
+__global__ __launch_bounds__(256) void k_volume_elas_t(int K, const float *__restrict__ vgeo,
+                                                       const int4 *__restrict__ Dids,
+                                                       const float4 *__restrict__ Dvals,
+                                                       const float *__restrict__ Q, float *__restrict__ rhs) {
+  using L = ElasVolLayout<N>;
+  constexpr int Np = L::Np, TB = L::TB, RowF = L::RowF;
+  extern __shared__ float lds[];
+  float *sQ = lds;
+  float4 *sA = (float4 *)lds;                 // [TB][Np] (v1 v2 v3 sxx)
+  float4 *sB = sA + TB * Np;                  // [TB][Np] (syy szz syz sxz)
+  float *sC = (float *)(sB + TB * Np);        // [TB][Np] sxy
+  float *sG = lds + L::QT;                    // [TB][9]
+  const int tid = threadIdx.x;
+  const int k0 = blockIdx.x * TB, nk = min(TB, K - k0);
+  {
+    const brsrc rQ = mkrsrc(Q + (size_t)k0 * RowF, (unsigned)nk * RowF * 4u);
+    float4 *sQ4 = (float4 *)sQ;
+#pragma unroll
+    for (unsigned it = 0; it < L::QIT; ++it) {
+      const unsigned off = tid * 16u + it * 4096u;
+      if (L::QIT * 4096u == L::QB || off < L::QB) sQ4[off / 16] = bld4(rQ, off);
+    }
+  }
+  if (tid < TB * 9) sG[tid] = bld(mkrsrc(vgeo + (size_t)k0 * 9, (unsigned)nk * 36u), tid * 4u);
+  __syncthreads();
+  const int gv = tid / L::GS, nv = tid - gv * L::GS;
+  const bool onv = gv < L::NG && nv < Np;
+  float qv[L::EPTV][9];
+  if (onv) {
+#pragma unroll
+    for (int i = 0; i < L::EPTV; ++i)
+#pragma unroll
+      for (int c = 0; c < 9; ++c) qv[i][c] = sQ[(gv + L::NG * i) * RowF + c * Np + nv];
+  }
+  __syncthreads();
+  if (onv) {
+#pragma unroll
+    for (int i = 0; i < L::EPTV; ++i) {
+      const int e = gv + L::NG * i;
+      sA[e * Np + nv] = make_float4(qv[i][0], qv[i][1], qv[i][2], qv[i][3]);
+      sB[e * Np + nv] = make_float4(qv[i][4], qv[i][5], qv[i][6], qv[i][7]);
+      sC[e * Np + nv] = qv[i][8];
+    }
+  }
+  __syncthreads();
+  if (!onv) return;
+  const brsrc rO = mkrsrc(rhs + (size_t)k0 * RowF, (unsigned)nk * RowF * 4u);
+  const float4 dv = Dvals[nv];
+  int id[4][4];
+  {
+    const int4 D1 = Dids[nv], D2 = Dids[Np + nv], D3 = Dids[2 * Np + nv], D4 = Dids[3 * Np + nv];
+    // id[m][j]: column of operator m (1..4), non-zero j
+    id[0][0] = D1.x; id[0][1] = D2.x; id[0][2] = D3.x; id[0][3] = D4.x;
+    id[1][0] = D1.y; id[1][1] = D2.y; id[1][2] = D3.y; id[1][3] = D4.y;
+    id[2][0] = D1.z; id[2][1] = D2.z; id[2][2] = D3.z; id[2][3] = D4.z;
+    id[3][0] = D1.w; id[3][1] = D2.w; id[3][2] = D3.w; id[3][3] = D4.w;
+  }
+  const float dw[4] = {dv.x, dv.y, dv.z, dv.w};
+#pragma unroll
+  for (int i = 0; i < L::EPTV; ++i) {
+    const int e = gv + L::NG * i;
+    const float4 *A = sA + e * Np, *B = sB + e * Np;
+    const float *C = sC + e * Np;
+    float pm[4][9];
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+#pragma unroll
+      for (int c = 0; c < 9; ++c) pm[m][c] = 0.f;
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const float4 a = A[id[m][j]], b = B[id[m][j]];
+        const float cc = C[id[m][j]];
+        pm[m][0] += dw[j] * a.x; pm[m][1] += dw[j] * a.y; pm[m][2] += dw[j] * a.z; pm[m][3] += dw[j] * a.w;
+        pm[m][4] += dw[j] * b.x; pm[m][5] += dw[j] * b.y; pm[m][6] += dw[j] * b.z; pm[m][7] += dw[j] * b.w;
+        pm[m][8] += dw[j] * cc;
+      }
+    }
+    const float *G = sG + 9 * e;
+    float Qx[9], Qy[9], Qz[9];
+#pragma unroll
+    for (int c = 0; c < 9; ++c) {
+      const float dr = .5f * (pm[1][c] - pm[0][c]), ds = .5f * (pm[2][c] - pm[0][c]), dt = .5f * (pm[3][c] - pm[0][c]);
+      Qx[c] = G[0] * dr + G[3] * ds + G[6] * dt;
+      Qy[c] = G[1] * dr + G[4] * ds + G[7] * dt;
+      Qz[c] = G[2] * dr + G[5] * ds + G[8] * dt;
+    }
+    const unsigned o = (unsigned)(e * RowF + nv) * 4u;
+    bst(rO, Qx[3] + Qy[8] + Qz[7], o);
+    bst(rO, Qx[8] + Qy[4] + Qz[6], o + 1 * Np * 4u);
+    bst(rO, Qx[7] + Qy[6] + Qz[5], o + 2 * Np * 4u);
+    bst(rO, Qx[0], o + 3 * Np * 4u);
+    bst(rO, Qy[1], o + 4 * Np * 4u);
+    bst(rO, Qz[2], o + 5 * Np * 4u);
+    bst(rO, Qy[2] + Qz[1], o + 6 * Np * 4u);
+    bst(rO, Qx[2] + Qz[0], o + 7 * Np * 4u);
+    bst(rO, Qx[1] + Qy[0], o + 8 * Np * 4u);
+  }
+}
+
+template <int N>
+static hipError_t volume_elas_t_n(bbw_ctx *c) {
+  using L = ElasVolLayout<N>;
+  const size_t sh = sizeof(float) * L::floats;
+  if ((unsigned long long)c->K * 36 * L::Np >= (1ull << 32)) return hipErrorInvalidValue;
+  if (sh > 65536) {
+    hipError_t e = hipFuncSetAttribute((const void *)k_volume_elas_t<N>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh);
+    if (e != hipSuccess) return e;
+  }
+  hipLaunchKernelGGL(k_volume_elas_t<N>, dim3((c->K + L::TB - 1) / L::TB), dim3(256), sh, c->stream, c->K,
+                     c->vgeo, c->Dids, c->Dvals, c->Q[c->cur], c->rhs);
+  return hipGetLastError();
+}
+
+hipError_t launch_volume_elas_tiled(bbw_ctx *c) {
+  switch (c->N) {
+    case 1: return volume_elas_t_n<1>(c);
+    case 2: return volume_elas_t_n<2>(c);
+    case 3: return volume_elas_t_n<3>(c);
+    case 4: return volume_elas_t_n<4>(c);
+    case 5: return volume_elas_t_n<5>(c);
+    case 6: return volume_elas_t_n<6>(c);
+  }
+  return hipErrorInvalidValue;
+}
+
 FusedOps fused_ops(const bbw_ctx *c) {
   FusedOps o;
   o.Fmask = c->Fmask;

@@ -151,6 +151,7 @@ hipError_t launch_update_wadg_mfma(bbw_ctx *c, float fa, float fb, float fdt, co
                                    float ftime);
 hipError_t launch_volume_tiled(bbw_ctx *c);
 hipError_t launch_surface_elas_tiled(bbw_ctx *c);
+hipError_t launch_volume_elas_tiled(bbw_ctx *c);
 hipError_t launch_update_elas_mfma(bbw_ctx *c, float fa, float fb, float fdt, const float *ftime_dev,
                                    float ftime);
 hipError_t launch_surface_tiled(bbw_ctx *c);

@@ -185,8 +185,7 @@ double bbw_batch_last_ms(const bbw_batch *b); /* device time of the last bbw_bat
 #define BBW_OPT_ADJ_IMPL 6 /* adjoint kernels: 0 per-element, 1 tiled MFMA (default) */
 #define BBW_OPT_PLAIN_IMPL 7 /* plain kernels (bbw_volume, bbw_surface, bbw_update_wadg): */
 #define BBW_PLAIN_FLAT 0     /* per-element loops over Kblk* elements per block */
-#define BBW_PLAIN_TILED 1    /* default: 16-element tiles, WADG projections on MFMA (acoustic: volume,
-                                surface and update; elastic: update) */
+#define BBW_PLAIN_TILED 1    /* default: 16-element tiles, WADG projections on MFMA */
 #define BBW_FUSED_VECTOR 0
 #define BBW_FUSED_MFMA 1
 int bbw_set_option(bbw_ctx *ctx, int option, int value);
