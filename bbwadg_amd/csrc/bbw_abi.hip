@@ -76,13 +76,18 @@ void reset_graph(bbw_ctx *c) {
 }
 
 // One LSRK stage; ftd (device) or ft (host value) is the source time function value.
-int do_stage(bbw_ctx *c, int i, double dt, const float *ftd, float ft, int mode, int projU) {
+// more_steps: another step of the same call follows, so when a_0 = 0 the residual left by the
+// last stage is never read (the next stage scales it by a_0) and the fused stage skips its store.
+int do_stage(bbw_ctx *c, int i, double dt, const float *ftd, float ft, int mode, int projU,
+             bool more_steps) {
   const float fa = (float)c->rk4a[i], fb = (float)c->rk4b[i], fdt = (float)dt;
   hipError_t e;
   if (c->Nf == 4) {
     if (mode == BBW_MODE_FUSED) {
+      const int store_res = !(i == 4 && more_steps && c->rk4a[0] == 0.0);
       e = c->fused_impl == BBW_FUSED_MFMA
-              ? launch_fused_mfma(c, c->Q[c->cur], c->Q[1 - c->cur], fa, fb, fdt, ftd, ft, projU)
+              ? launch_fused_mfma(c, c->Q[c->cur], c->Q[1 - c->cur], fa, fb, fdt, ftd, ft, projU,
+                                  nullptr, store_res)
               : launch_fused(c, c->Q[c->cur], c->Q[1 - c->cur], fa, fb, fdt, ftd, ft, projU);
       if (e != hipSuccess) return hipfail(c, e, "fused stage launch");
       c->cur = 1 - c->cur;
@@ -597,7 +602,8 @@ int bbw_step(bbw_ctx *c, int nsteps, double dt, double t0, int mode, int flags) 
       int crc = BBW_OK;
       for (int j = 0; j < G && !crc; ++j) {
         for (int i = 0; i < 5 && !crc; ++i)
-          crc = do_stage(c, i, dt, has_src ? c->ftime_win + j * 5 + i : nullptr, 0.f, mode, projU);
+          crc = do_stage(c, i, dt, has_src ? c->ftime_win + j * 5 + i : nullptr, 0.f, mode, projU,
+                         j + 1 < G);
         if (!crc && c->nrec) {
           hipError_t e = launch_record(c, c->Q[c->cur], rec_win + (size_t)j * c->nrec);
           if (e != hipSuccess) crc = hipfail(c, e, "record launch");
@@ -639,7 +645,9 @@ int bbw_step(bbw_ctx *c, int nsteps, double dt, double t0, int mode, int flags) 
   }
   for (; s < nsteps; ++s) {
     for (int i = 0; i < 5; ++i)
-      if ((rc = do_stage(c, i, dt, nullptr, has_src ? ft[(size_t)s * 5 + i] : 0.f, mode, projU))) return rc;
+      if ((rc = do_stage(c, i, dt, nullptr, has_src ? ft[(size_t)s * 5 + i] : 0.f, mode, projU,
+                         s + 1 < nsteps)))
+        return rc;
     if (c->nrec) {
       hipError_t e = launch_record(c, c->Q[c->cur], c->rec_trace + (size_t)c->rec_steps * c->nrec);
       if (e != hipSuccess) return hipfail(c, e, "record launch");

@@ -62,7 +62,7 @@ template <int N, bool PERSIST>
 __global__ __launch_bounds__(256, PERSIST ? 3 : 4) void k_fused_batch(
     int nitems, const BatchItem *__restrict__ items, const FusedMesh *__restrict__ meshes,
     const BatchShotDev *__restrict__ shots, const int *__restrict__ list, FusedOps o, int stage,
-    int par, float fa, float fb, const int *__restrict__ step) {
+    int par, float fa, float fb, int store_res, const int *__restrict__ step) {
   using L = FusedLayout<N, false>;
   extern __shared__ float lds[];
   const int stp = *step;
@@ -106,6 +106,7 @@ __global__ __launch_bounds__(256, PERSIST ? 3 : 4) void k_fused_batch(
       for (int f = 0; f < 9; ++f) src.w[f] = sf(sd->src.w[f]);
       sh.ft = src.nsrc ? sf(sp(sd->ftab)[stp * 5 + stage]) : 0.f;
       sh.fdt = sf(sd->dt);
+      sh.store_res = store_res;
       TileStage<N, false> st;
       stage_load<N, false>(st, m, sh.Qin, k0, nk, tj);
       stage_store<N, false>(st, lds + L::OQ, lds + L::OG, tj);
@@ -183,33 +184,33 @@ double ricker_b(double t, double f0) {
 template <int N>
 hipError_t launch_batch_n(bbw_batch *b, int nitems, const BatchItem *items, const FusedMesh *meshes,
                           const BatchShotDev *shots, const int *list, int stage, int par, float fa,
-                          float fb, const int *step, int grid) {
+                          float fb, int store_res, const int *step, int grid) {
   using L = FusedLayout<N, false>;
   const size_t sh = sizeof(float) * L::floats;
   if (grid < nitems) {
     if (sh > 65536)
       hipFuncSetAttribute((const void *)k_fused_batch<N, true>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh);
     hipLaunchKernelGGL((k_fused_batch<N, true>), dim3(grid), dim3(256), sh, b->ops->stream, nitems, items,
-                       meshes, shots, list, fused_ops(b->ops), stage, par, fa, fb, step);
+                       meshes, shots, list, fused_ops(b->ops), stage, par, fa, fb, store_res, step);
   } else {
     if (sh > 65536)
       hipFuncSetAttribute((const void *)k_fused_batch<N, false>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh);
     hipLaunchKernelGGL((k_fused_batch<N, false>), dim3(nitems), dim3(256), sh, b->ops->stream, nitems, items,
-                       meshes, shots, list, fused_ops(b->ops), stage, par, fa, fb, step);
+                       meshes, shots, list, fused_ops(b->ops), stage, par, fa, fb, store_res, step);
   }
   return hipGetLastError();
 }
 
 hipError_t launch_batch(bbw_batch *b, int nitems, const BatchItem *items, const FusedMesh *meshes,
                         const BatchShotDev *shots, const int *list, int stage, int par, float fa,
-                        float fb, const int *step, int grid) {
+                        float fb, int store_res, const int *step, int grid) {
   switch (b->ops->N) {
-    case 1: return launch_batch_n<1>(b, nitems, items, meshes, shots, list, stage, par, fa, fb, step, grid);
-    case 2: return launch_batch_n<2>(b, nitems, items, meshes, shots, list, stage, par, fa, fb, step, grid);
-    case 3: return launch_batch_n<3>(b, nitems, items, meshes, shots, list, stage, par, fa, fb, step, grid);
-    case 4: return launch_batch_n<4>(b, nitems, items, meshes, shots, list, stage, par, fa, fb, step, grid);
-    case 5: return launch_batch_n<5>(b, nitems, items, meshes, shots, list, stage, par, fa, fb, step, grid);
-    case 6: return launch_batch_n<6>(b, nitems, items, meshes, shots, list, stage, par, fa, fb, step, grid);
+    case 1: return launch_batch_n<1>(b, nitems, items, meshes, shots, list, stage, par, fa, fb, store_res, step, grid);
+    case 2: return launch_batch_n<2>(b, nitems, items, meshes, shots, list, stage, par, fa, fb, store_res, step, grid);
+    case 3: return launch_batch_n<3>(b, nitems, items, meshes, shots, list, stage, par, fa, fb, store_res, step, grid);
+    case 4: return launch_batch_n<4>(b, nitems, items, meshes, shots, list, stage, par, fa, fb, store_res, step, grid);
+    case 5: return launch_batch_n<5>(b, nitems, items, meshes, shots, list, stage, par, fa, fb, store_res, step, grid);
+    case 6: return launch_batch_n<6>(b, nitems, items, meshes, shots, list, stage, par, fa, fb, store_res, step, grid);
   }
   return hipErrorInvalidValue;
 }
@@ -434,8 +435,11 @@ int bbw_batch_run(bbw_batch *b, int shots_per_tile, int blocks_per_cu, int use_g
     auto one_step = [&](int parity) -> hipError_t {
       int p = parity;
       for (int i = 0; i < 5; ++i) {
+        // a shot's residual is not part of its readable state: after the last stage it is dead
+        // whenever a_0 = 0 (the next step's first stage scales it by 0)
+        const int store_res = !(i == 4 && c->rk4a[0] == 0.0);
         hipError_t e = launch_batch(b, ph.nitems, ditems + ph.item0, dmesh, dshot, dlist, i, p,
-                                    (float)c->rk4a[i], (float)c->rk4b[i], dstep, grid);
+                                    (float)c->rk4a[i], (float)c->rk4b[i], store_res, dstep, grid);
         if (e != hipSuccess) return e;
         p ^= 1;
       }

@@ -98,15 +98,23 @@ __global__ __launch_bounds__(256, N <= 4 ? 4 : N == 5 ? 3 : 2) void k_update_wad
   const brsrc rAP = mkrsrc(o.PqBlk, (unsigned)L::NPT * KBP * 1024u);
   const bool has_src = src.nsrc > 0 && k0 + nk - 1 >= src.first && k0 <= src.last;
 
-  // ---- 1: rhs tile -> LDS; res and Q rows, c2 and 1/J (MFMA output layout), Vq blocks
+  // ---- 1: rhs tile -> LDS; res and Q rows, c2 and 1/J (MFMA output layout), Vq blocks.
+  // With fa = 0 (the first LSRK45 stage) res is not read: fa res = 0 for any finite res.
+  const bool read_res = fa != 0.f;
   float rreg[L::PF ? TB : 1], qreg[L::PF ? TB : 1];
   if constexpr (L::PF) {
 #pragma unroll
     for (int e = 0; e < TB; ++e) {
       const float v = bld(rRh, tid * 4u + e * ROW);
       if (tid < NpNf) sR[e * RS + tid] = v;
-      rreg[e] = bld(rRs, tid * 4u + e * ROW);
       qreg[e] = bld(rQ, tid * 4u + e * ROW);
+    }
+    if (read_res) {
+#pragma unroll
+      for (int e = 0; e < TB; ++e) rreg[e] = bld(rRs, tid * 4u + e * ROW);
+    } else {
+#pragma unroll
+      for (int e = 0; e < TB; ++e) rreg[e] = 0.f;
     }
   } else {
     for (int rr = tid; rr < NpNf; rr += 256)
@@ -232,7 +240,7 @@ __global__ __launch_bounds__(256, N <= 4 ? 4 : N == 5 ? 3 : 2) void k_update_wad
           if (slot >= 0) v += w4(src, fr) * ft * src.coef[(size_t)slot * Np + nr];
         }
         const unsigned g = rr * 4u + e * ROW;
-        const float x = fa * bld(rRs, g) + fdt * v;
+        const float x = (read_res ? fa * bld(rRs, g) : 0.f) + fdt * v;
         bst(rRs, x, g);
         bst(rQ, bld(rQ, g) + fb * x, g);
       }
@@ -708,8 +716,10 @@ __global__ __launch_bounds__(256, N <= 4 ? 4 : N == 5 ? 3 : 2) void k_update_ela
     __syncthreads();
   }
 
-  // ---- 4: LSRK stage: res = fa res + fdt (rhs + source); Q += fb res
+  // ---- 4: LSRK stage: res = fa res + fdt (rhs + source); Q += fb res (res is not read when
+  // fa = 0, the first LSRK45 stage)
   const float ft = ftime_dev ? *ftime_dev : ftime;
+  const bool read_res = fa != 0.f;
   for (int rr = tid; rr < NpNf; rr += 256) {
     const int fr = rr / Np, nr = rr - fr * Np;
     float wsel = 0.f;
@@ -723,7 +733,7 @@ __global__ __launch_bounds__(256, N <= 4 ? 4 : N == 5 ? 3 : 2) void k_update_ela
         if (slot >= 0) v += wsel * ft * src.coef[(size_t)slot * Np + nr];
       }
       const unsigned gi = rr * 4u + e * ROW;
-      const float x = fa * bld(rRs, gi) + fdt * v;
+      const float x = (read_res ? fa * bld(rRs, gi) : 0.f) + fdt * v;
       bst(rRs, x, gi);
       bst(rQ, bld(rQ, gi) + fb * x, gi);
     }
@@ -1126,7 +1136,7 @@ FusedOps fused_ops(const bbw_ctx *c) {
 template <int N, bool P>
 static hipError_t fused_mfma_n(bbw_ctx *c, const float *Qin, float *Qout, float fa, float fb,
                                float fdt, const float *ftd, float ft, const SrcArgs &s,
-                               float *yout) {
+                               float *yout, int store_res) {
   using L = FusedLayout<N, P>;
   const size_t sh = sizeof(float) * L::floats;
   const int grid = (c->K + L::TB - 1) / L::TB;
@@ -1137,7 +1147,7 @@ static hipError_t fused_mfma_n(bbw_ctx *c, const float *Qin, float *Qout, float 
     if (e != hipSuccess) return e;
   }
   FusedMesh m{c->K, c->vgeo, c->fgeo, c->vmapP};
-  FusedShot shot{c->c2q, Qin, Qout, c->res, yout, ft, fdt};
+  FusedShot shot{c->c2q, Qin, Qout, c->res, yout, ft, fdt, store_res};
   hipLaunchKernelGGL((k_fused_mfma<N, P>), dim3(grid), dim3(256), sh, c->stream, m, fused_ops(c),
                      shot, s, fa, fb, ftd);
   return hipGetLastError();
@@ -1146,13 +1156,14 @@ static hipError_t fused_mfma_n(bbw_ctx *c, const float *Qin, float *Qout, float 
 template <int N>
 static hipError_t fused_mfma_sel(bbw_ctx *c, const float *Qin, float *Qout, float fa, float fb,
                                  float fdt, const float *ftd, float ft, int projU,
-                                 const SrcArgs &s, float *yout) {
-  return projU ? fused_mfma_n<N, true>(c, Qin, Qout, fa, fb, fdt, ftd, ft, s, yout)
-               : fused_mfma_n<N, false>(c, Qin, Qout, fa, fb, fdt, ftd, ft, s, yout);
+                                 const SrcArgs &s, float *yout, int store_res) {
+  return projU ? fused_mfma_n<N, true>(c, Qin, Qout, fa, fb, fdt, ftd, ft, s, yout, store_res)
+               : fused_mfma_n<N, false>(c, Qin, Qout, fa, fb, fdt, ftd, ft, s, yout, store_res);
 }
 
 hipError_t launch_fused_mfma(bbw_ctx *c, const float *Qin, float *Qout, float fa, float fb,
-                             float fdt, const float *ftd, float ft, int projU, float *yout) {
+                             float fdt, const float *ftd, float ft, int projU, float *yout,
+                             int store_res) {
   SrcArgs s;
   s.nsrc = c->nsrc;
   s.first = c->src_first;
@@ -1161,12 +1172,12 @@ hipError_t launch_fused_mfma(bbw_ctx *c, const float *Qin, float *Qout, float fa
   s.coef = c->src_coef;
   for (int f = 0; f < 9; ++f) s.w[f] = c->src_w[f];
   switch (c->N) {
-    case 1: return fused_mfma_sel<1>(c, Qin, Qout, fa, fb, fdt, ftd, ft, projU, s, yout);
-    case 2: return fused_mfma_sel<2>(c, Qin, Qout, fa, fb, fdt, ftd, ft, projU, s, yout);
-    case 3: return fused_mfma_sel<3>(c, Qin, Qout, fa, fb, fdt, ftd, ft, projU, s, yout);
-    case 4: return fused_mfma_sel<4>(c, Qin, Qout, fa, fb, fdt, ftd, ft, projU, s, yout);
-    case 5: return fused_mfma_sel<5>(c, Qin, Qout, fa, fb, fdt, ftd, ft, projU, s, yout);
-    case 6: return fused_mfma_sel<6>(c, Qin, Qout, fa, fb, fdt, ftd, ft, projU, s, yout);
+    case 1: return fused_mfma_sel<1>(c, Qin, Qout, fa, fb, fdt, ftd, ft, projU, s, yout, store_res);
+    case 2: return fused_mfma_sel<2>(c, Qin, Qout, fa, fb, fdt, ftd, ft, projU, s, yout, store_res);
+    case 3: return fused_mfma_sel<3>(c, Qin, Qout, fa, fb, fdt, ftd, ft, projU, s, yout, store_res);
+    case 4: return fused_mfma_sel<4>(c, Qin, Qout, fa, fb, fdt, ftd, ft, projU, s, yout, store_res);
+    case 5: return fused_mfma_sel<5>(c, Qin, Qout, fa, fb, fdt, ftd, ft, projU, s, yout, store_res);
+    case 6: return fused_mfma_sel<6>(c, Qin, Qout, fa, fb, fdt, ftd, ft, projU, s, yout, store_res);
   }
   return hipErrorInvalidValue;
 }

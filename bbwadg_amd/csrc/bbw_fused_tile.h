@@ -87,6 +87,8 @@ struct FusedShot {
   float *yout;   // (A Q)_p in BB coefficients [K][Np] for the adjoint, or null
   float ft;      // source time function at this stage
   float fdt;     // time step
+  int store_res; // 0: res is dead after this stage and is not stored (the last LSRK45 stage of a
+                 // step that another step follows: the next stage multiplies it by a_0 = 0)
 };
 
 template <int N, bool PROJ_U>
@@ -228,12 +230,19 @@ __device__ __forceinline__ void tile_shot(const FusedMesh &m, const FusedOps &o,
   // source: elements are sorted, so a tile outside [first, last] has none (uniform test)
   const bool has_src = src.nsrc > 0 && k0 + nk - 1 >= src.first && k0 <= src.last;
 
-  // ---- 1: register prefetch of res, c2 and the neighbour traces (Q_in tile already in LDS)
+  // ---- 1: register prefetch of res, c2 and the neighbour traces (Q_in tile already in LDS).
+  // With fa = 0 (the first LSRK45 stage) res is not read: fa res = 0 for any finite res.
   if (has_src && tid < TB) sSlot[tid] = tid < nk ? src_slot2(src, k0 + tid) : -1;
+  const bool read_res = fa != 0.f;
   float rreg[L::PF ? TB : 1];
   if constexpr (L::PF) {
+    if (read_res) {
 #pragma unroll
-    for (int e = 0; e < TB; ++e) rreg[e] = bld(rR, tid * 4u + e * ROW);
+      for (int e = 0; e < TB; ++e) rreg[e] = bld(rR, tid * 4u + e * ROW);
+    } else {
+#pragma unroll
+      for (int e = 0; e < TB; ++e) rreg[e] = 0.f;
+    }
   }
   float c2r[L::QTW][4];
   {
@@ -535,7 +544,7 @@ __device__ __forceinline__ void tile_shot(const FusedMesh &m, const FusedOps &o,
 #pragma unroll
         for (int e = 0; e < TB; ++e) {
           const float x = fa * rreg[e] + fdt * sR[e * RS + tid];
-          bst(rR, x, tid * 4u + e * ROW);
+          if (sh.store_res) bst(rR, x, tid * 4u + e * ROW);
           bst(rO, sQ[e * NpNf + tid] + fb * x, tid * 4u + e * ROW);
         }
       } else {
@@ -547,7 +556,7 @@ __device__ __forceinline__ void tile_shot(const FusedMesh &m, const FusedOps &o,
           const int slot = sSlot[e];
           if (slot >= 0) v += wsel * ft * src.coef[(size_t)slot * Np + nr];
           const float x = fa * rreg[e] + fdt * v;
-          bst(rR, x, tid * 4u + e * ROW);
+          if (sh.store_res) bst(rR, x, tid * 4u + e * ROW);
           bst(rO, sQ[e * NpNf + tid] + fb * x, tid * 4u + e * ROW);
         }
       }
@@ -562,8 +571,8 @@ __device__ __forceinline__ void tile_shot(const FusedMesh &m, const FusedOps &o,
           if (slot >= 0) v += w4(src, fr) * ft * src.coef[(size_t)slot * Np + nr];
         }
         const unsigned g = rr * 4u + e * ROW;
-        const float x = fa * bld(rR, g) + fdt * v;
-        bst(rR, x, g);
+        const float x = (read_res ? fa * bld(rR, g) : 0.f) + fdt * v;
+        if (sh.store_res) bst(rR, x, g);
         bst(rO, sQ[e * NpNf + rr] + fb * x, g);
       }
     }

@@ -159,7 +159,7 @@ hipError_t launch_fused(bbw_ctx *c, const float *Qin, float *Qout, float fa, flo
                         const float *ftime_dev, float ftime, int project_velocity);
 hipError_t launch_fused_mfma(bbw_ctx *c, const float *Qin, float *Qout, float fa, float fb,
                              float fdt, const float *ftime_dev, float ftime, int project_velocity,
-                             float *yout = nullptr);
+                             float *yout = nullptr, int store_res = 1);
 hipError_t launch_adj_stage(bbw_ctx *c, const float *Y, float fa, float fb, float fdt);
 hipError_t launch_adj_stage_tiled(bbw_ctx *c, const float *Y, float fa, float fb, float fdt);
 hipError_t launch_adj_inject(bbw_ctx *c, const float *resid_row);

@@ -220,3 +220,56 @@ def test_tune_single_restores_state_and_applies_choice():
     assert {t[0] for t in r["plain_trials"]} == {"tiled", 2}
     assert len(r["fused_trials"]) == 2
     assert np.array_equal(s.WaveGetData3d(), Q) and np.array_equal(s.get_res(), res)
+
+
+@pytest.mark.parametrize("kind", ["fused", "plain_tiled", "plain_flat"])
+def test_first_stage_does_not_depend_on_res(kind):
+    """a_0 = 0: the first LSRK45 stage computes res = dt rhs whatever res held (the kernels skip
+    reading it), against the FP64 oracle that multiplies the old res by 0."""
+    print(f"case first stage {kind}", flush=True)
+    P, s = make(5, 3)
+    assert P.ops.rk4a[0] == 0.0
+    Q, res = P.random_state(), 1e3 * P.random_state()
+    fa, fb, fdt = np.float32(P.ops.rk4a[0]), np.float32(P.ops.rk4b[0]), np.float32(1e-2)
+    s.WaveSetData3d(Q); s.set_res(res)
+    oo = oracle_ops(P, np.float64)
+    rhs = np.zeros(Q.shape)
+    orc.volume(oo, P.K, P.dm.vgeo, Q.astype(np.float64), rhs)
+    orc.surface(oo, P.K, P.dm.fgeo, P.dm.vmapP, Q.astype(np.float64), rhs)
+    if kind == "fused":
+        s.stage_fused(fa, fb, fdt)
+        nproj = 1
+    else:
+        s.set_option(_lib.OPT_PLAIN_IMPL, _lib.PLAIN_TILED if kind == "plain_tiled" else _lib.PLAIN_FLAT)
+        s.set_rhs(rhs.astype(np.float32))
+        s.update_wadg(fa, fb, fdt)
+        nproj = 4
+    rQ, rres = Q.astype(np.float64).copy(), res.astype(np.float64).copy()
+    orc.update_wadg(oo, P.K, P.c2q, fa, fb, fdt, rhs, rres, rQ, nproj=nproj)
+    assert rel(s.get_res(), rres) < 2e-5
+    assert rel(s.WaveGetData3d(), rQ) < 2e-5
+
+
+@pytest.mark.parametrize("graph,nsteps", [(False, 7), (True, 10), (True, 23)])
+def test_run_leaves_reference_residual(graph, nsteps):
+    """Inside a multi-step call the fused stage drops the dead store of res after the last stage
+    of a step; the residual left at the end of the call is still the reference's."""
+    print(f"case residual after run graph={graph} nsteps={nsteps}", flush=True)
+    P, s = make(4, 2)
+    Q0 = P.random_state()
+    dt = P.dt_cfl()
+    s.WaveSetData3d(Q0); s.set_res(P.random_state())
+    s.step(nsteps, dt, mode=MODE_FUSED, graph=graph)
+    got_Q, got_res = s.WaveGetData3d(), s.get_res()
+    # one step per call stores res after every step: same arithmetic, so bitwise equal
+    s.WaveSetData3d(Q0); s.set_res(P.random_state())
+    for _ in range(nsteps):
+        s.step(1, dt, mode=MODE_FUSED, graph=False)
+    assert np.array_equal(s.WaveGetData3d(), got_Q) and np.array_equal(s.get_res(), got_res)
+    ref = Q0.astype(np.float64).copy()
+    _, rres = orc.rk_run(oracle_ops(P, np.float64), P.K, P.dm.vgeo, P.dm.fgeo, P.dm.vmapP, P.c2q,
+                         ref, dt, nsteps, nproj=1)
+    assert rel_l2(got_Q, ref) < 1e-5
+    # res is an increment (dt-scaled right-hand side): FP32 rounding relative to it is larger
+    # than relative to the state; a stale residual would be off by O(1)
+    assert rel_l2(got_res, rres) < 1e-4
