@@ -186,16 +186,66 @@ struct NoHook {
   __device__ void operator()() const {}
 };
 
+// The register-resident inputs of a tile's stage (phase 1): res rows, c2 in the MFMA output
+// layout and the neighbour traces of the thread's face nodes.  Issued before the tile's Q_in
+// rows are stored to LDS, so the vmapP -> trace chain overlaps the tile loads.
+template <int N, bool PROJ_U>
+struct TilePre {
+  float rreg[FL<N, PROJ_U>::PF ? FL<N, PROJ_U>::TB : 1];
+  float c2r[FL<N, PROJ_U>::QTW][4];
+  float tr[FL<N, PROJ_U>::EPT3][4];
+};
+
+// Issues the loads of TilePre.  With fa = 0 (the first LSRK45 stage) res is not read: fa res = 0
+// for any finite res.  Requires K * 4 * Np * 4 < 2^32 (32-bit trace offsets).
+template <int N, bool PROJ_U>
+__device__ __forceinline__ void tile_prefetch(TilePre<N, PROJ_U> &pf, const FusedMesh &m,
+                                              const FusedShot &sh, float fa, int k0, int nk,
+                                              const int (&vm)[FL<N, PROJ_U>::EPT3], const int tid) {
+  using L = FusedLayout<N, PROJ_U>;
+  constexpr int Np = L::Np, Nq = L::Nq, TB = L::TB;
+  constexpr unsigned ROW = 4 * Np * 4u;
+  const int lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  if constexpr (L::PF) {
+    const brsrc rR = mkrsrc(sh.res + (size_t)k0 * 4 * Np, (unsigned)nk * ROW);
+    if (fa != 0.f) {
+#pragma unroll
+      for (int e = 0; e < TB; ++e) pf.rreg[e] = bld(rR, tid * 4u + e * ROW);
+    } else {
+#pragma unroll
+      for (int e = 0; e < TB; ++e) pf.rreg[e] = 0.f;
+    }
+  }
+  // rows q >= Nq read a neighbouring value; the padded Vq rows make them 0 in S
+  const brsrc rC = mkrsrc(sh.c2q + (size_t)k0 * Nq, (unsigned)nk * Nq * 4u);
+  const unsigned co = (unsigned)((lane & 15) * Nq + 4 * (lane >> 4)) * 4u;
+#pragma unroll
+  for (int t = 0; t < L::QTW; ++t) {
+    const int qt = wave + 4 * t;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) pf.c2r[t][r] = qt < L::NQT ? bld(rC, co + (qt * 16 + r) * 4u) : 0.f;
+  }
+  // the traces last: their addresses wait for vmapP, the loads above do not
+  const brsrc rT = mkrsrc(sh.Qin, (unsigned)m.K * ROW);
+#pragma unroll
+  for (int i = 0; i < L::EPT3; ++i) {
+    const unsigned to = (unsigned)vm[i] * 4u;
+#pragma unroll
+    for (int f = 0; f < 4; ++f) pf.tr[i][f] = bld(rT, to + f * Np * 4u);
+  }
+}
+
 // One fused LSRK stage of one shot on one tile (phases 1-8 of bbw_fused.hip), with the tile's
-// Q_in rows already in LDS at sQ and its geometry at sG.  at_start runs after the phase-1 loads
-// are issued, at_mid at the start of phase 6 and at_end before the final barrier: the caller's
-// prefetch of the next tile goes there.  Ends with a barrier.  Elements past the tile's end are
-// computed on zeros and never stored.  Requires K * 4 * Np * 4 < 2^32 (32-bit trace offsets).
+// Q_in rows already in LDS at sQ, its geometry at sG and the register inputs in pf
+// (tile_prefetch).  at_start runs after phase 1, at_mid at the start of phase 6 and at_end
+// before the final barrier.  Ends with a barrier.  Elements past the tile's end are computed on
+// zeros and never stored.
 template <int N, bool PROJ_U, class HS, class HM, class HE>
 __device__ __forceinline__ void tile_shot(const FusedMesh &m, const FusedOps &o, const FusedShot &sh,
                                           const SrcArgs &src, float fa, float fb, int k0, int nk,
                                           float *lds, const float *sQ, const float *sG,
-                                          const int (&vm)[FL<N, PROJ_U>::EPT3], int fid3,
+                                          const int (&vm)[FL<N, PROJ_U>::EPT3],
+                                          const TilePre<N, PROJ_U> &pf, int fid3,
                                           const int tid, HS &&at_start, HM &&at_mid, HE &&at_end) {
   using L = FusedLayout<N, PROJ_U>;
   constexpr int Np = L::Np, Nfp = L::Nfp, NfpNf = L::NfpNf, Nq = L::Nq, TB = L::TB, RS = L::RS;
@@ -218,8 +268,9 @@ __device__ __forceinline__ void tile_shot(const FusedMesh &m, const FusedOps &o,
   const unsigned tbytes = (unsigned)nk * ROW;
   const brsrc rR = mkrsrc(sh.res + base, tbytes);
   const brsrc rO = mkrsrc(sh.Qout + base, tbytes);
-  const brsrc rC = mkrsrc(sh.c2q + (size_t)k0 * Nq, (unsigned)nk * Nq * 4u);
-  const brsrc rT = mkrsrc(sh.Qin, (unsigned)m.K * ROW);   // neighbour traces
+  const float(&rreg)[L::PF ? TB : 1] = pf.rreg;
+  const float(&c2r)[L::QTW][4] = pf.c2r;
+  const float(&tr)[L::EPT3][4] = pf.tr;
   // face-node mapping (phases 2, 3)
   const int g3 = tid / NfpNf, m3 = tid - g3 * NfpNf;
   const bool on3 = g3 < L::NG3;
@@ -230,40 +281,11 @@ __device__ __forceinline__ void tile_shot(const FusedMesh &m, const FusedOps &o,
   // source: elements are sorted, so a tile outside [first, last] has none (uniform test)
   const bool has_src = src.nsrc > 0 && k0 + nk - 1 >= src.first && k0 <= src.last;
 
-  // ---- 1: register prefetch of res, c2 and the neighbour traces (Q_in tile already in LDS).
-  // With fa = 0 (the first LSRK45 stage) res is not read: fa res = 0 for any finite res.
+  // ---- 1: source slots of the tile's elements (read in phase 8, after several barriers); the
+  // register inputs (res, c2, traces) were issued by tile_prefetch before the Q_in tile's barrier
   if (has_src && tid < TB) sSlot[tid] = tid < nk ? src_slot2(src, k0 + tid) : -1;
   const bool read_res = fa != 0.f;
-  float rreg[L::PF ? TB : 1];
-  if constexpr (L::PF) {
-    if (read_res) {
-#pragma unroll
-      for (int e = 0; e < TB; ++e) rreg[e] = bld(rR, tid * 4u + e * ROW);
-    } else {
-#pragma unroll
-      for (int e = 0; e < TB; ++e) rreg[e] = 0.f;
-    }
-  }
-  float c2r[L::QTW][4];
-  {
-    // rows q >= Nq read a neighbouring value; the padded Vq rows make them 0 in S
-    const unsigned co = (unsigned)((lane & 15) * Nq + 4 * (lane >> 4)) * 4u;
-#pragma unroll
-    for (int t = 0; t < L::QTW; ++t) {
-      const int qt = wave + 4 * t;
-#pragma unroll
-      for (int r = 0; r < 4; ++r) c2r[t][r] = qt < L::NQT ? bld(rC, co + (qt * 16 + r) * 4u) : 0.f;
-    }
-  }
-  float tr[L::EPT3][4];
-#pragma unroll
-  for (int i = 0; i < L::EPT3; ++i) {
-    const unsigned to = (unsigned)vm[i] * 4u;
-#pragma unroll
-    for (int f = 0; f < 4; ++f) tr[i][f] = bld(rT, to + f * Np * 4u);
-  }
   at_start();
-  __syncthreads();
 
   // ---- 2: (p, Ur, Us, Ut) per node; upwind flux at face nodes into registers.  The p and U
   // fluxes are (phi, -phi) with phi = (pjump - Unjump) Fscale / 2, so one value per face node.
