@@ -2,6 +2,7 @@
 // The step loop restates Wave_RK / RK_step_WADG_subelem (cpp/src/WaveOKL3d.cpp:2014-2109,
 // elas_cpp/src/WaveOKL3d.cpp:1605-1693) on one HIP stream, optionally replayed from a
 // captured HIP graph.
+#include <array>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -168,7 +169,7 @@ void bbw_destroy(bbw_ctx *c) {
   reset_graph(c);
   dfree(c->Dids); dfree(c->Dvals); dfree(c->Fmask); dfree(c->EEL_ids); dfree(c->L0_ids);
   dfree(c->EEL_vals); dfree(c->L0_vals); dfree(c->Vq); dfree(c->Pq); dfree(c->Vq64); dfree(c->VqPad); dfree(c->PqPad); dfree(c->EEL_ids4); dfree(c->LiftPad); dfree(c->VqBlk); dfree(c->PqBlk); dfree(c->LiftBlk); dfree(c->PqTBlk); dfree(c->VqTBlk); dfree(c->LiftTBlk); dfree(c->PqTPad); dfree(c->VqTPad);
-  dfree(c->DTids); dfree(c->DTvals); dfree(c->EELT_ids); dfree(c->EELT_vals); dfree(c->L0T_ids);
+  dfree(c->DTids); dfree(c->DTvals); dfree(c->Sids); dfree(c->Eids); dfree(c->Evals); dfree(c->EELT_ids); dfree(c->EELT_vals); dfree(c->L0T_ids);
   dfree(c->L0T_vals); dfree(c->FmaskInv); dfree(c->mapP); dfree(c->lQ); dfree(c->lres);
   dfree(c->Zadj); dfree(c->Hadj); dfree(c->grad); dfree(c->Ybuf); dfree(c->ckpt);
   dfree(c->resid); dfree(c->Q0save); dfree(c->Ycache);
@@ -268,6 +269,71 @@ int bbw_upload_operators(bbw_ctx *c, const bbw_ops *o) {
     dfree(c->DTvals);
     if ((rc = upload(c, &c->DTids, (const int4 *)dti.data(), (size_t)4 * Np))) return rc;
     if ((rc = upload(c, &c->DTvals, (const float4 *)dtv.data(), (size_t)4 * Np))) return rc;
+    // The Bernstein derivative in factored form for the fused stage: D_m u [a] =
+    // sum_j a_j u[a - e_j + e_m] = (elevation of the shifted coefficients u[b + e_m], |b| = N - 1).
+    // Multi-indices (i, j, k, l) in the reference's order (l outer, then k, then j; i = N - j - k - l;
+    // cpp/src/Basis.cpp:539-566); e_m adds 1 to component m.  The uploaded D must be exactly this
+    // operator (checked), which is what StartUp3d builds from VB \ GradBernTet.
+    {
+      const int N = c->N, Np1 = N * (N + 1) * (N + 2) / 6;
+      auto order = [](int M, std::vector<std::array<int, 4>> &ix) {
+        ix.clear();
+        for (int l = 0; l <= M; ++l)
+          for (int k = 0; k <= M - l; ++k)
+            for (int j = 0; j <= M - l - k; ++j) ix.push_back({M - j - k - l, j, k, l});
+      };
+      std::vector<std::array<int, 4>> aN, aM;
+      order(N, aN);
+      order(N - 1, aM);
+      auto find = [](const std::vector<std::array<int, 4>> &ix, const std::array<int, 4> &a) {
+        for (size_t n = 0; n < ix.size(); ++n)
+          if (ix[n] == a) return (int)n;
+        return -1;
+      };
+      std::vector<int> sid(4 * (size_t)Np1), eid(4 * (size_t)Np);
+      std::vector<float> ev(4 * (size_t)Np);
+      for (int b = 0; b < Np1; ++b)
+        for (int m = 0; m < 4; ++m) {
+          std::array<int, 4> a = aM[b];
+          ++a[m];
+          sid[4 * b + m] = find(aN, a);
+        }
+      for (int a = 0; a < Np; ++a) {
+        int first = -1;
+        for (int j = 0; j < 4; ++j)
+          if (aN[a][j] > 0 && first < 0) {
+            std::array<int, 4> b = aN[a];
+            --b[j];
+            first = find(aM, b);
+          }
+        for (int j = 0; j < 4; ++j) {
+          std::array<int, 4> b = aN[a];
+          --b[j];
+          eid[4 * a + j] = aN[a][j] > 0 ? find(aM, b) : first;   // zero weight: any valid node
+          ev[4 * a + j] = (float)aN[a][j];
+        }
+      }
+      for (int m = 0; m < 4; ++m)
+        for (int a = 0; a < Np; ++a)
+          for (int g = 0; g < Np; ++g) {
+            double want = 0.0;
+            for (int j = 0; j < 4; ++j) {
+              if (aN[a][j] == 0) continue;
+              std::array<int, 4> t = aN[a];
+              --t[j];
+              ++t[m];
+              if (find(aN, t) == g) want += aN[a][j];
+            }
+            if (std::fabs(D[((size_t)m * Np + a) * Np + g] - want) > 1e-4 * N)
+              return fail(c, BBW_ERR_ARG, "D_ids/Dvals are not the Bernstein derivative operators of order N");
+          }
+      dfree(c->Sids);
+      dfree(c->Eids);
+      dfree(c->Evals);
+      if ((rc = upload(c, &c->Sids, (const int4 *)sid.data(), (size_t)Np1))) return rc;
+      if ((rc = upload(c, &c->Eids, (const int4 *)eid.data(), (size_t)Np))) return rc;
+      if ((rc = upload(c, &c->Evals, (const float4 *)ev.data(), (size_t)Np))) return rc;
+    }
     const int NfpNf = 4 * Nfp;
     auto transpose_sparse = [&](const int32_t *ids, const float *vals, int rows, int cols, int nnz,
                                 std::vector<int> &tid, std::vector<float> &tval, int &width) {

@@ -1133,6 +1133,9 @@ FusedOps fused_ops(const bbw_ctx *c) {
   o.VqBlk = c->VqBlk;
   o.PqBlk = c->PqBlk;
   o.LiftBlk = c->LiftBlk;
+  o.Sids = c->Sids;
+  o.Eids = c->Eids;
+  o.Evals = c->Evals;
   return o;
 }
 

@@ -45,12 +45,16 @@ struct FusedLayout {
   static constexpr int cmax(int a, int b) { return a > b ? a : b; }
   // W1: face flux [KB][BS] | S [NQP][SX]
   static constexpr int W1 = cmax((KB * BS + TB - 1) / TB, NQP * NP);
-  // W2: (p, Ur, Us, Ut) float4 [TB][Np] | rhs [TB][RS]
+  // W2: volume step-1 values F float4 [TB][Np1] | rhs [TB][RS]
   static constexpr int W2 = cmax(4 * Np, RS);
   static constexpr int QTW = (NQT + 3) / 4;     // q tiles per wave
   static constexpr int GS = ((Np + 63) / 64) * 64, NG = 256 / GS;        // node groups
   static constexpr int NG3 = 256 / NfpNf, EPT3 = (TB + NG3 - 1) / NG3;   // face-node groups
   static constexpr int EPTV = (TB + NG - 1) / NG;
+  // degree N-1 nodes of the factored derivative (volume step 1): groups of 32 or 64 lanes
+  static constexpr int Np1 = N * (N + 1) * (N + 2) / 6;
+  static constexpr int GS1 = Np1 <= 32 ? 32 : 64, NG1 = 256 / GS1, EPT1 = TB / NG1;
+  static_assert(Np1 <= 64 && NG1 * EPT1 == TB, "volume step 1 covers the tile");
   static constexpr bool PF = 4 * Np <= 256;    // one state row per thread: prefetch res
   // Q_in tile [TB][4Np] (read in phases 2 and 8) and geometry (vgeo [TB][9], fgeo [TB][16])
   static constexpr int QT = TB * 4 * Np, GT = TB * 25;
@@ -72,6 +76,9 @@ struct FusedOps {
   const float *PqPad;
   const float *LiftPad;
   const float *VqBlk, *PqBlk, *LiftBlk;   // lane-ordered A operands (bbw_ctx::VqBlk)
+  const int4 *Sids;     // [Np1]: degree-N nodes b + e_1 .. b + e_4 of degree-(N-1) node b
+  const int4 *Eids;     // [Np]: degree-(N-1) nodes a - e_1 .. a - e_4 of degree-N node a
+  const float4 *Evals;  // [Np]: (a_1, a_2, a_3, a_4), the elevation weights
 };
 
 struct FusedMesh {
@@ -253,15 +260,12 @@ __device__ __forceinline__ void tile_shot(const FusedMesh &m, const FusedOps &o,
   constexpr int NP = L::NP, SX = L::SX, NQP = L::NQP, NPP = L::NPP;
   constexpr unsigned ROW = NpNf * 4u;    // bytes of one element's state
   const float fdt = sh.fdt;
-  const int4 *__restrict__ Dids = o.Dids;
-  const float4 *__restrict__ Dvals = o.Dvals;
   float *W1 = lds + L::OW1;              // [TB * W1]
   float *W2 = lds + L::OW2;              // [TB * W2]
   const float *sF = sG + TB * 9;         // [TB][16]
   int *sSlot = (int *)(lds + L::OSL);
   float *sB = W1;                        // phases 3-5: face flux [KB][BS]
   float *S = W1;                         // phases 6-7
-  float4 *sV = (float4 *)W2;             // phases 2-3
   float *sR = W2;                        // phases 4-8: rhs [TB][RS]
   const int lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const size_t base = (size_t)k0 * NpNf;
@@ -287,22 +291,8 @@ __device__ __forceinline__ void tile_shot(const FusedMesh &m, const FusedOps &o,
   const bool read_res = fa != 0.f;
   at_start();
 
-  // ---- 2: (p, Ur, Us, Ut) per node; upwind flux at face nodes into registers.  The p and U
-  // fluxes are (phi, -phi) with phi = (pjump - Unjump) Fscale / 2, so one value per face node.
-  if (onv) {
-#pragma unroll
-    for (int i = 0; i < L::EPTV; ++i) {
-      const int e = gv + L::NG * i;
-      if (L::NG * L::EPTV == TB || e < TB) {
-        const float *s = sQ + e * NpNf;
-        const float *G = sG + 9 * e;
-        const float uu = s[Np + nv], vv = s[2 * Np + nv], ww = s[3 * Np + nv];
-        sV[e * Np + nv] = make_float4(s[nv], uu * G[0] + vv * G[1] + ww * G[2],
-                                      uu * G[3] + vv * G[4] + ww * G[5],
-                                      uu * G[6] + vv * G[7] + ww * G[8]);
-      }
-    }
-  }
+  // ---- 2: upwind flux at face nodes into registers.  The p and U fluxes are (phi, -phi) with
+  // phi = (pjump - Unjump) Fscale / 2, so one value per face node.
   float phi[L::EPT3];
   if (on3) {
 #pragma unroll
@@ -326,9 +316,14 @@ __device__ __forceinline__ void tile_shot(const FusedMesh &m, const FusedOps &o,
       }
     }
   }
-  __syncthreads();
 
-  // ---- 3: face flux -> LDS [face node][element]; volume terms into registers
+  // ---- 3: face flux -> LDS [face node][element]; volume step 1 from the Q_in tile.
+  // The derivative along barycentric m is D_m u [a] = sum_j a_j u[a - e_j + e_m]: the shift to
+  // degree N - 1, u[b + e_m], then elevation.  The volume terms are linear in the shifted values,
+  // so step 1 forms, at each degree-(N-1) node b, F[b] = (H, X, Y, Z) with
+  //   H = -(Ur[b+e2] + Us[b+e3] + Ut[b+e4] - (Ur + Us + Ut)[b+e1]) / 2     (Ur = U . grad r, ...)
+  //   (X, Y, Z) = -G^T ((p[b+e2] - p[b+e1]) / 2, (p[b+e3] - p[b+e1]) / 2, (p[b+e4] - p[b+e1]) / 2)
+  // and step 2 (phase 3b) elevates: vol[a] = sum_j a_j F[a - e_j].
   if (on3) {
 #pragma unroll
     for (int i = 0; i < L::EPT3; ++i) {
@@ -337,46 +332,49 @@ __device__ __forceinline__ void tile_shot(const FusedMesh &m, const FusedOps &o,
     }
   }
   if (tid < (KB - NfpNf) * BS) sB[NfpNf * BS + tid] = 0.f;   // K padding of the last face
-  float vol[L::EPTV][4];
-  if (onv) {
-    const float4 dv = Dvals[nv];
-    const int4 D1 = Dids[nv], D2 = Dids[Np + nv], D3 = Dids[2 * Np + nv], D4 = Dids[3 * Np + nv];
-    const float4 *Vb = sV + gv * Np;
-    const float4 *pa1 = Vb + D1.x, *pa2 = Vb + D2.x, *pa3 = Vb + D3.x, *pa4 = Vb + D4.x;
-    const float4 *pb1 = Vb + D1.y, *pb2 = Vb + D2.y, *pb3 = Vb + D3.y, *pb4 = Vb + D4.y;
-    const float4 *pc1 = Vb + D1.z, *pc2 = Vb + D2.z, *pc3 = Vb + D3.z, *pc4 = Vb + D4.z;
-    const float4 *pd1 = Vb + D1.w, *pd2 = Vb + D2.w, *pd3 = Vb + D3.w, *pd4 = Vb + D4.w;
-    const float *Gb = sG + 9 * gv;
+  float4 *sF1 = (float4 *)W2;                                // [TB][Np1]: F
+  {
+    const int g1 = tid / L::GS1, n1 = tid - g1 * L::GS1;
+    if (n1 < L::Np1) {
+      const int4 S = o.Sids[n1];
 #pragma unroll
-    for (int i = 0; i < L::EPTV; ++i) {
-      const int o4 = i * L::NG * Np;
-      const float4 a1 = pa1[o4], a2 = pa2[o4], a3 = pa3[o4], a4 = pa4[o4];
-      const float4 b1 = pb1[o4], b2 = pb2[o4], b3 = pb3[o4], b4 = pb4[o4];
-      const float4 c1 = pc1[o4], c2 = pc2[o4], c3 = pc3[o4], c4 = pc4[o4];
-      const float4 d1 = pd1[o4], d2 = pd2[o4], d3 = pd3[o4], d4 = pd4[o4];
-      const float p1 = dv.x * a1.x + dv.y * a2.x + dv.z * a3.x + dv.w * a4.x;
-      const float p2 = dv.x * b1.x + dv.y * b2.x + dv.z * b3.x + dv.w * b4.x;
-      const float p3 = dv.x * c1.x + dv.y * c2.x + dv.z * c3.x + dv.w * c4.x;
-      const float p4 = dv.x * d1.x + dv.y * d2.x + dv.z * d3.x + dv.w * d4.x;
-      float dU1 = dv.x * (b1.y + c1.z + d1.w);
-      dU1 += dv.y * (b2.y + c2.z + d2.w);
-      dU1 += dv.z * (b3.y + c3.z + d3.w);
-      dU1 += dv.w * (b4.y + c4.z + d4.w);
-      float dU2 = dv.x * (a1.y + a1.z + a1.w);
-      dU2 += dv.y * (a2.y + a2.z + a2.w);
-      dU2 += dv.z * (a3.y + a3.z + a3.w);
-      dU2 += dv.w * (a4.y + a4.z + a4.w);
-      const float dpdr = .5f * (p2 - p1), dpds = .5f * (p3 - p1), dpdt = .5f * (p4 - p1);
-      const float *G = Gb + 9 * L::NG * i;
-      vol[i][0] = -.5f * (dU1 - dU2);
-      vol[i][1] = -(G[0] * dpdr + G[3] * dpds + G[6] * dpdt);
-      vol[i][2] = -(G[1] * dpdr + G[4] * dpds + G[7] * dpdt);
-      vol[i][3] = -(G[2] * dpdr + G[5] * dpds + G[8] * dpdt);
+      for (int i = 0; i < L::EPT1; ++i) {
+        const int e = g1 + L::NG1 * i;
+        const float *q = sQ + e * NpNf;
+        const float *G = sG + 9 * e;
+        const float p1 = q[S.x], p2 = q[S.y], p3 = q[S.z], p4 = q[S.w];
+        const float ur = q[Np + S.y] * G[0] + q[2 * Np + S.y] * G[1] + q[3 * Np + S.y] * G[2];
+        const float us = q[Np + S.z] * G[3] + q[2 * Np + S.z] * G[4] + q[3 * Np + S.z] * G[5];
+        const float ut = q[Np + S.w] * G[6] + q[2 * Np + S.w] * G[7] + q[3 * Np + S.w] * G[8];
+        const float u1 = q[Np + S.x] * (G[0] + G[3] + G[6]) + q[2 * Np + S.x] * (G[1] + G[4] + G[7]) +
+                         q[3 * Np + S.x] * (G[2] + G[5] + G[8]);
+        const float a = .5f * (p2 - p1), b = .5f * (p3 - p1), cc = .5f * (p4 - p1);
+        sF1[e * L::Np1 + n1] = make_float4(-.5f * ((ur + us + ut) - u1), -(G[0] * a + G[3] * b + G[6] * cc),
+                                           -(G[1] * a + G[4] * b + G[7] * cc), -(G[2] * a + G[5] * b + G[8] * cc));
+      }
     }
   }
   __syncthreads();
 
-  // ---- 4: volume terms -> rhs (the (p, U) tile is no longer read)
+  // ---- 3b: volume step 2 (elevation) into registers: vol[a] = sum_j a_j F[a - e_j]
+  float vol[L::EPTV][4];
+  if (onv) {
+    const int4 E = o.Eids[nv];
+    const float4 ev = o.Evals[nv];
+    const float4 *Fb = sF1 + gv * L::Np1;
+#pragma unroll
+    for (int i = 0; i < L::EPTV; ++i) {
+      const int o4 = i * L::NG * L::Np1;
+      const float4 f1 = Fb[E.x + o4], f2 = Fb[E.y + o4], f3 = Fb[E.z + o4], f4 = Fb[E.w + o4];
+      vol[i][0] = ev.x * f1.x + ev.y * f2.x + ev.z * f3.x + ev.w * f4.x;
+      vol[i][1] = ev.x * f1.y + ev.y * f2.y + ev.z * f3.y + ev.w * f4.y;
+      vol[i][2] = ev.x * f1.z + ev.y * f2.z + ev.z * f3.z + ev.w * f4.z;
+      vol[i][3] = ev.x * f1.w + ev.y * f2.w + ev.z * f3.w + ev.w * f4.w;
+    }
+  }
+  __syncthreads();
+
+  // ---- 4: volume terms -> rhs (F is no longer read)
   if (onv) {
     float *rb = sR + gv * RS + nv;
 #pragma unroll

@@ -121,6 +121,10 @@ struct bbw_ctx {
   // adjoint: transposed operators (D_m^T rows [4][Np] int4/float4, EEL^T [WE][4Nfp], L0^T
   // [WL][Nfp]), node -> face-node slots [Np][3], face-node neighbour map [K][4Nfp], buffers
   int4 *DTids = nullptr;
+  // factored Bernstein derivative (fused stage): shift [Np1] int4, elevation ids [Np] int4 and
+  // weights [Np] float4
+  int4 *Sids = nullptr, *Eids = nullptr;
+  float4 *Evals = nullptr;
   float4 *DTvals = nullptr;
   int *EELT_ids = nullptr, *L0T_ids = nullptr, *FmaskInv = nullptr, *mapP = nullptr;
   float *EELT_vals = nullptr, *L0T_vals = nullptr;
