@@ -3,6 +3,8 @@
 Restates the mesh layer of the reference:
 
 * ``read_gmsh``       -- GMSH 2.2 ASCII reader, ``ReadGmsh3d`` (``cpp/src/Mesh3d.cpp:9-200``);
+* ``write_gmsh_node_data`` -- per-element nodal values as a GMSH ``$ElementNodeData`` view,
+  ``writeVisToGMSH`` (``cpp/src/WaveOKL3d.cpp:2560-2642``);
 * ``cube_mesh``       -- structured n^3 cubes x 6 tetrahedra on [lo, hi]^3 (new: the
   reference reads cube meshes from ``cpp/meshes/cube*.msh``);
 * ``face_pair``       -- EToE/EToF by sorted vertex triples, ``FacePair3d``
@@ -39,6 +41,7 @@ class Mesh:
     EToV: np.ndarray  # (K, 4) int64
     EToE: np.ndarray = None
     EToF: np.ndarray = None
+    gmsh_ids: np.ndarray = None  # (K,) element numbers of the tets in the GMSH file (EToGmshE)
 
     @property
     def K(self):
@@ -111,7 +114,7 @@ def read_gmsh(path):
     with f:
         lines = f.read().splitlines()
     VX = VY = VZ = None
-    tets = []
+    tets, gids = [], []
     i = 0
     while i < len(lines):
         line = lines[i]
@@ -128,6 +131,7 @@ def read_gmsh(path):
                 if parts[1] == 4:
                     ntags = parts[2]
                     tets.append(parts[3 + ntags:3 + ntags + 4])
+                    gids.append(parts[0])
             i += 2 + ne
             continue
         i += 1
@@ -135,7 +139,70 @@ def read_gmsh(path):
         raise MeshError(f"no tetrahedra in '{path}'")
     EToV = np.array(tets, dtype=np.int64) - 1
     EToV = _orient(VX, VY, VZ, EToV)
-    return Mesh(VX=VX, VY=VY, VZ=VZ, EToV=EToV)
+    return Mesh(VX=VX, VY=VY, VZ=VZ, EToV=EToV, gmsh_ids=np.array(gids, dtype=np.int64))
+
+
+def write_gmsh_node_data(path, mesh, r, s, t, values, name="Field 0", time=0.0, step=0):
+    """Write per-element nodal values as a GMSH 2.2 ``$ElementNodeData`` view with its own
+    interpolation scheme (``writeVisToGMSH``, cpp/src/WaveOKL3d.cpp:2560-2642).
+
+    ``values`` (K, Np) are the field at the reference nodes (r, s, t) of every element (the
+    reference writes Q after WaveGetData3d, i.e. nodal values).  The scheme's monomials are
+    u^a v^b w^c in u = (r+1)/2, v = (s+1)/2, w = (t+1)/2 with a+b+c <= N (a outer, c inner), and
+    its coefficient matrix is the inverse of their Vandermonde matrix at the nodes, so GMSH
+    reproduces the nodal values.  Elements are numbered as in the GMSH file the mesh came from
+    (``EToGmshE``), else 1..K.
+    """
+    values = np.asarray(values, dtype=np.float64)
+    K, Np = values.shape
+    N = 0
+    while (N + 1) * (N + 2) * (N + 3) // 6 < Np:
+        N += 1
+    if (N + 1) * (N + 2) * (N + 3) // 6 != Np or K != mesh.K or len(r) != Np:
+        raise MeshError("values must be (K, Np) at the Np reference nodes of an order-N tetrahedron")
+    monom = np.array([(a, b, c) for a in range(N + 1) for b in range(N + 1) for c in range(N + 1)
+                      if a + b + c <= N], dtype=np.int64)
+    u, v, w = (np.asarray(r) + 1) / 2, (np.asarray(s) + 1) / 2, (np.asarray(t) + 1) / 2
+    vdm = (u[None, :] ** monom[:, 0:1]) * (v[None, :] ** monom[:, 1:2]) * (w[None, :] ** monom[:, 2:3])
+    coeff = np.linalg.inv(vdm)
+    ids = mesh.gmsh_ids if mesh.gmsh_ids is not None else np.arange(1, K + 1)
+    with open(path, "w") as f:
+        f.write("$MeshFormat\n2.2 0 8\n$EndMeshFormat\n")
+        f.write('$InterpolationScheme\n"MyInterpScheme"\n1\n5 2\n')
+        f.write(f"{Np} {Np}\n")
+        for row in coeff:
+            f.write(" ".join(repr(float(x)) for x in row) + " \n")
+        f.write(f"{Np} 3\n")
+        for row in monom:
+            f.write(" ".join(str(int(x)) for x in row) + " \n")
+        f.write("$EndInterpolationScheme\n")
+        f.write(f'$ElementNodeData\n2\n"{name}"\n"MyInterpScheme"\n1\n{time!r}\n3\n{int(step)}\n1\n{K}\n')
+        for k in range(K):
+            f.write(f"{int(ids[k])} {Np} " + " ".join(repr(float(x)) for x in values[k]) + "\n")
+        f.write("$EndElementNodeData\n")
+
+
+def read_gmsh_node_data(path):
+    """Read back a view written by ``write_gmsh_node_data``: (element numbers, values (K, Np),
+    interpolation coefficient matrix, monomial exponents)."""
+    with open(path) as f:
+        lines = f.read().splitlines()
+    i = lines.index("$InterpolationScheme")
+    nr, nc = (int(x) for x in lines[i + 4].split())
+    coeff = np.array([[float(x) for x in lines[i + 5 + a].split()] for a in range(nr)])
+    j = i + 5 + nr
+    nm = int(lines[j].split()[0])
+    monom = np.array([[int(x) for x in lines[j + 1 + a].split()] for a in range(nm)])
+    d = lines.index("$ElementNodeData")
+    ntags = int(lines[d + 1])
+    nreal = int(lines[d + 2 + ntags])
+    nint_at = d + 3 + ntags + nreal
+    nint = int(lines[nint_at])
+    K = int(lines[nint_at + nint])
+    rows = [lines[nint_at + nint + 1 + k].split() for k in range(K)]
+    ids = np.array([int(rw[0]) for rw in rows])
+    vals = np.array([[float(x) for x in rw[2:]] for rw in rows])
+    return ids, vals, coeff, monom
 
 
 def face_pair(mesh):

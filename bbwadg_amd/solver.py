@@ -156,6 +156,14 @@ class WaveSolver:
     def to_nodal(self, Q):
         return np.einsum("ij,kfj->kfi", self.ops.VB, Q)
 
+    def writeVisToGMSH(self, fileName, Q=None, iField=0, time=0.0, step=0):
+        """Field iField of Q (BB coefficients; default the current state) at the nodes, as a GMSH
+        ``$ElementNodeData`` view (``writeVisToGMSH``, cpp/src/WaveOKL3d.cpp:2560-2642)."""
+        Q = self.WaveGetData3d() if Q is None else np.asarray(Q)
+        nodal = self.to_nodal(Q)[:, iField]
+        meshmod.write_gmsh_node_data(fileName, self.mesh, self.ops.r, self.ops.s, self.ops.t, nodal,
+                                     name=f"Field {iField}", time=time, step=step)
+
     # -- kernels -------------------------------------------------------------------
     def volume(self):
         check(self.ctx, self.lib.bbw_volume(self.ctx))

@@ -273,3 +273,41 @@ def test_run_leaves_reference_residual(graph, nsteps):
     # res is an increment (dt-scaled right-hand side): FP32 rounding relative to it is larger
     # than relative to the state; a stale residual would be off by O(1)
     assert rel_l2(got_res, rres) < 1e-4
+
+
+@pytest.mark.parametrize("mode", [MODE_PLAIN, MODE_FUSED])
+def test_unstructured_reference_mesh(mode):
+    """The reference's cube3.msh (196 tets, unstructured, every face orientation and partial last
+    tile): cavity start, Fourier c2, N = 4, 25 steps against the FP64 oracle; the GMSH view of the
+    result holds the nodal values."""
+    import os
+    from bbwadg_amd import media, mesh as M, operators as O
+    from cases import cavity
+    print(f"case unstructured cube3.msh mode={mode}", flush=True)
+    N = 4
+    m = M.read_gmsh(os.path.join(os.path.dirname(__file__), "golden", "cube3.msh"))
+    M.face_pair(m)
+    ops = O.build_ref_ops(N)
+    dm = M.device_arrays(m, ops, 4)
+    xq, yq, zq = media.cubature_points(m, ops)
+    c2q = media.c2_fourier(2)(xq, yq, zq).astype(np.float32)
+    x, y, z = M.affine_points(m, ops.r, ops.s, ops.t)
+    Q0 = np.zeros((m.K, 4, ops.Np))
+    Q0[:, 0] = (ops.invVB @ cavity(x, y, z, 0.0)).T
+    Q0 = Q0.astype(np.float32)
+    dt, nsteps = 0.25 / (N ** 2 * dm.FscaleMax), 25
+    s = WaveSolver(m, N, ops=ops, dmesh=dm)
+    s.InitWADG_subelem(c2q)
+    s.WaveSetData3d(Q0)
+    s.step(nsteps, dt, mode=mode, graph=True)
+    got = s.WaveGetData3d()
+    ref = Q0.astype(np.float64).copy()
+    orc.rk_run(orc.OracleOps(ops, np.float64), m.K, dm.vgeo, dm.fgeo, dm.vmapP, c2q, ref, dt, nsteps)
+    assert rel_l2(got, ref) < 1e-5
+    path = os.path.join(os.environ.get("TMPDIR", "/tmp"), f"bbw_view_{os.getpid()}_{mode}.msh")
+    s.writeVisToGMSH(path, Q=got, iField=0)
+    ids, vals, _, _ = M.read_gmsh_node_data(path)
+    os.remove(path)
+    assert list(ids) == list(m.gmsh_ids)
+    assert np.allclose(vals, s.to_nodal(got)[:, 0], rtol=0, atol=1e-6)
+    s.close()
