@@ -311,3 +311,34 @@ def test_unstructured_reference_mesh(mode):
     assert list(ids) == list(m.gmsh_ids)
     assert np.allclose(vals, s.to_nodal(got)[:, 0], rtol=0, atol=1e-6)
     s.close()
+
+
+@pytest.mark.parametrize("kind", ["fused", "plain_tiled", "plain_flat"])
+@pytest.mark.parametrize("N", [3, 5])
+def test_wadg_update_against_exact_bb_product(N, kind):
+    """Second oracle for the WADG projection: with c2 linear the quadrature update must equal the
+    exact BB product projection P_N(c2 p) (oracle/bb_exact.py, the rk_update_BBWADG algorithm)."""
+    import bb_exact as X
+    print(f"case exact BB-WADG N={N} {kind}", flush=True)
+    P = Problem(N, 3, seed=2)
+    c2q = 1.0 + 0.3 * P.xq - 0.2 * P.yq + 0.25 * P.zq
+    s = WaveSolver(P.mesh, N, ops=P.ops, dmesh=P.dm)
+    s.InitWADG_subelem(c2q)
+    Q, res = P.random_state(), P.random_state()
+    fa, fb, fdt = np.float32(P.ops.rk4a[3]), np.float32(P.ops.rk4b[3]), np.float32(1e-2)
+    oo = oracle_ops(P, np.float64)
+    rhs = np.zeros(Q.shape)
+    orc.volume(oo, P.K, P.dm.vgeo, Q.astype(np.float64), rhs)
+    orc.surface(oo, P.K, P.dm.fgeo, P.dm.vmapP, Q.astype(np.float64), rhs)
+    s.WaveSetData3d(Q); s.set_res(res)
+    if kind == "fused":
+        s.stage_fused(fa, fb, fdt)
+    else:
+        s.set_option(_lib.OPT_PLAIN_IMPL, _lib.PLAIN_TILED if kind == "plain_tiled" else _lib.PLAIN_FLAT)
+        s.set_rhs(rhs.astype(np.float32))
+        s.update_wadg(fa, fb, fdt)
+    rQ, rres = X.update_bbwadg(P.ops, X.c2_bb(P.ops, s.c2q), fa, fb, fdt, rhs,
+                               res.astype(np.float64), Q.astype(np.float64))
+    assert rel(s.get_res(), rres) < 2e-5
+    assert rel(s.WaveGetData3d(), rQ) < 2e-5
+    s.close()
