@@ -64,6 +64,20 @@ double ricker(double t, double f0) {
   return (1.0 - 2.0 * a * a) * std::exp(-a * a);
 }
 
+// MFMA A operand in lane order (see bbw_ctx::VqBlk): M x K matrix A, 16-row tiles, blocks of four
+// K-steps of 4, 64 lanes x 4 floats per block, zero outside A.
+void lane_blocks(int M, int K, const std::function<float(int, int)> &A, std::vector<float> &out) {
+  const int MT = (M + 15) / 16, KT = (K + 3) / 4, KB = (KT + 3) / 4;
+  out.assign((size_t)MT * KB * 256, 0.f);
+  for (int t = 0; t < MT; ++t)
+    for (int b = 0; b < KB; ++b)
+      for (int lane = 0; lane < 64; ++lane)
+        for (int sb = 0; sb < 4; ++sb) {
+          const int i = 16 * t + lane % 16, k = 4 * (4 * b + sb) + lane / 16;
+          if (i < M && k < K) out[(((size_t)t * KB + b) * 64 + lane) * 4 + sb] = A(i, k);
+        }
+}
+
 int check_ready(bbw_ctx *c) {
   if (!c->have_ops) return fail(c, BBW_ERR_STATE, "operators not uploaded");
   if (!c->have_mesh) return fail(c, BBW_ERR_STATE, "mesh not uploaded");
@@ -169,6 +183,7 @@ void bbw_destroy(bbw_ctx *c) {
   reset_graph(c);
   dfree(c->Dids); dfree(c->Dvals); dfree(c->Fmask); dfree(c->EEL_ids); dfree(c->L0_ids);
   dfree(c->EEL_vals); dfree(c->L0_vals); dfree(c->Vq); dfree(c->Pq); dfree(c->Vq64); dfree(c->VqPad); dfree(c->PqPad); dfree(c->EEL_ids4); dfree(c->LiftPad); dfree(c->VqBlk); dfree(c->PqBlk); dfree(c->LiftBlk); dfree(c->PqTBlk); dfree(c->VqTBlk); dfree(c->LiftTBlk); dfree(c->PqTPad); dfree(c->VqTPad);
+  dfree(c->bb_CN); dfree(c->bb_PBlk); dfree(c->c2bb);
   dfree(c->DTids); dfree(c->DTvals); dfree(c->Sids); dfree(c->Eids); dfree(c->Evals); dfree(c->EELT_ids); dfree(c->EELT_vals); dfree(c->L0T_ids);
   dfree(c->L0T_vals); dfree(c->FmaskInv); dfree(c->mapP); dfree(c->lQ); dfree(c->lres);
   dfree(c->Zadj); dfree(c->Hadj); dfree(c->grad); dfree(c->Ybuf); dfree(c->ckpt);
@@ -403,17 +418,7 @@ int bbw_upload_operators(bbw_ctx *c, const bbw_ops *o) {
       dfree(c->LiftPad);
       if ((rc = upload(c, &c->LiftPad, lp.data(), lp.size()))) return rc;
       // lane-ordered A operand blocks (see bbw_ctx::VqBlk)
-      auto blocks = [](int M, int K, const std::function<float(int, int)> &A, std::vector<float> &out) {
-        const int MT = (M + 15) / 16, KT = (K + 3) / 4, KB = (KT + 3) / 4;
-        out.assign((size_t)MT * KB * 256, 0.f);
-        for (int t = 0; t < MT; ++t)
-          for (int b = 0; b < KB; ++b)
-            for (int lane = 0; lane < 64; ++lane)
-              for (int sb = 0; sb < 4; ++sb) {
-                const int i = 16 * t + lane % 16, k = 4 * (4 * b + sb) + lane / 16;
-                if (i < M && k < K) out[(((size_t)t * KB + b) * 64 + lane) * 4 + sb] = A(i, k);
-              }
-      };
+      auto blocks = lane_blocks;
       std::vector<float> vb, pb, lb, lf;
       blocks(Nq, Np, [&](int q, int j) { return o->Vq[q + (size_t)j * Nq]; }, vb);
       blocks(Np, Nq, [&](int n, int q) { return o->Pq[n + (size_t)q * Np]; }, pb);
@@ -605,6 +610,87 @@ int bbw_update_wadg(bbw_ctx *c, float fa, float fb, float fdt, float ftime) {
   hipError_t e = c->Nf == 4 ? launch_update_wadg(c, fa, fb, fdt, nullptr, ftime)
                             : launch_update_elas(c, fa, fb, fdt, nullptr, ftime);
   return e == hipSuccess ? BBW_OK : hipfail(c, e, "update launch");
+}
+
+int bbw_upload_bbwadg(bbw_ctx *c, const int32_t *subtet_ids, const double *CNscale,
+                      const double *invC2Nscale, const int32_t *Ei_ids4, const double *Ei_vals4,
+                      const double *cj, const float *c2_bb) {
+  if (!c || !subtet_ids || !CNscale || !invC2Nscale || !Ei_ids4 || !Ei_vals4 || !cj || !c2_bb)
+    return BBW_ERR_ARG;
+  if (c->Nf != 4) return fail(c, BBW_ERR_UNSUPPORTED, "the exact BB-WADG update is acoustic");
+  const int N = c->N, Np = c->Np, N2 = 2 * N, N2p = np_of(N2);
+  auto npd = [](int d) { return (d + 1) * (d + 2) * (d + 3) / 6; };
+  // the kernel forms the degree-2N index of a + b arithmetically, Np(2N) - Np(2N - l) +
+  // k (2N - l + 1) - k (k - 1) / 2 + j for (j, k, l) = (j, k, l)_a + (j, k, l)_b in the reference
+  // order (l outer, k, j inner); subtet_ids must be that table
+  {
+    std::vector<std::array<int, 3>> jkl;
+    for (int l = 0; l <= N; ++l)
+      for (int k = 0; k <= N - l; ++k)
+        for (int j = 0; j <= N - l - k; ++j) jkl.push_back({j, k, l});
+    for (int a = 0; a < Np; ++a)
+      for (int b = 0; b < Np; ++b) {
+        const int j = jkl[a][0] + jkl[b][0], k = jkl[a][1] + jkl[b][1], l = jkl[a][2] + jkl[b][2];
+        const int g = N2p - npd(N2 - l) + k * (N2 - l + 1) - k * (k - 1) / 2 + j;
+        if (subtet_ids[(size_t)a * Np + b] != g)
+          return fail(c, BBW_ERR_ARG, "subtet_ids is not the degree-2N index of a + b");
+      }
+  }
+  for (int i = 0; i < N2; ++i)
+    for (int r = 0; r < npd(i + 1); ++r)
+      for (int w = 0; w < 4; ++w) {
+        const int id = Ei_ids4[((size_t)i * N2p + r) * 4 + w];
+        if (id < 0 || id >= npd(i)) return fail(c, BBW_ERR_ARG, "Ei_ids4 out of range");
+      }
+  // dense E_{a -> b} = E_{b-1} ... E_a from the one-step sparse elevations, FP64
+  auto elev = [&](int a, int b) {
+    std::vector<double> M((size_t)npd(a) * npd(a), 0.0);
+    for (int i = 0; i < npd(a); ++i) M[(size_t)i * npd(a) + i] = 1.0;
+    for (int d = a; d < b; ++d) {
+      const int rows = npd(d + 1), cols = npd(a);
+      std::vector<double> R((size_t)rows * cols, 0.0);
+      for (int r = 0; r < rows; ++r)
+        for (int w = 0; w < 4; ++w) {
+          const size_t o = ((size_t)d * N2p + r) * 4 + w;
+          const double v = Ei_vals4[o];
+          if (v == 0.0) continue;
+          const int m = Ei_ids4[o];
+          for (int cc = 0; cc < cols; ++cc) R[(size_t)r * cols + cc] += v * M[(size_t)m * cols + cc];
+        }
+      M.swap(R);
+    }
+    return M;  // npd(b) x npd(a), row-major
+  };
+  // P' = (sum_j c_j E_{N-j}^N (E_{N-j}^2N)^T) diag(1/C(2N,g)), Np x N2p
+  std::vector<double> P((size_t)Np * N2p, 0.0);
+  for (int j = 0; j <= N; ++j) {
+    const int a = N - j, na = npd(a);
+    const std::vector<double> EN = elev(a, N), E2 = elev(a, N2);
+    for (int i = 0; i < Np; ++i)
+      for (int g = 0; g < N2p; ++g) {
+        double acc = 0.0;
+        for (int t = 0; t < na; ++t) acc += EN[(size_t)i * na + t] * E2[(size_t)g * na + t];
+        P[(size_t)i * N2p + g] += cj[j] * acc;
+      }
+  }
+  std::vector<float> pb, cn(Np);
+  lane_blocks(Np, N2p, [&](int i, int g) { return (float)(P[(size_t)i * N2p + g] * invC2Nscale[g]); }, pb);
+  for (int i = 0; i < Np; ++i) cn[i] = (float)CNscale[i];
+  int rc;
+  dfree(c->bb_CN);
+  dfree(c->bb_PBlk);
+  dfree(c->c2bb);
+  if ((rc = upload(c, &c->bb_CN, cn.data(), cn.size()))) return rc;
+  if ((rc = upload(c, &c->bb_PBlk, pb.data(), pb.size()))) return rc;
+  if ((rc = upload(c, &c->c2bb, c2_bb, (size_t)c->K * Np))) return rc;
+  return BBW_OK;
+}
+
+int bbw_update_bbwadg(bbw_ctx *c) {
+  if (!c) return BBW_ERR_ARG;
+  if (!c->bb_PBlk || !c->c2bb) return fail(c, BBW_ERR_STATE, "exact BB-WADG operators not uploaded");
+  hipError_t e = launch_update_bbwadg(c);
+  return e == hipSuccess ? BBW_OK : hipfail(c, e, "exact BB-WADG launch");
 }
 
 int bbw_update(bbw_ctx *c, float fa, float fb, float fdt) {

@@ -1118,6 +1118,147 @@ hipError_t launch_volume_elas_tiled(bbw_ctx *c) {
   return hipErrorInvalidValue;
 }
 
+// Exact Bernstein-Bezier WADG pressure update (rk_update_BBWADG,
+// cpp/okl/WaveKernelsBBWADG.okl:14-321): rhs_p <- P_N(c2 rhs_p), c2 in degree-N BB coefficients.
+// One workgroup per 16-element tile:
+//   1  rows C(N,a) c2_a and C(N,b) p_b -> LDS; the degree-2N product tile f [g][element] zeroed
+//   2  f_g = sum_{a+b=g} (C(N,a) c2_a)(C(N,b) p_b), four elements per wave with lanes over b: for
+//      each a the lanes' targets g = a + b are distinct, so the LDS accumulation is race free
+//      (the 1/C(2N,g) scaling is folded into the projection)
+//   3  r = P' f on MFMA 16x16x4 (M = nodes, N = 16 elements, K = degree-2N coefficients),
+//      P' = sum_j c_j E_{N-j}^N (E_{N-j}^2N)^T diag(1/C(2N,g)) as lane-ordered blocks
+//   4  r -> rhs_p rows through LDS
+template <int N>
+struct BBLayout {
+  static constexpr int Np = Dims<N>::Np, N2p = (2 * N + 1) * (2 * N + 2) * (2 * N + 3) / 6, TB = 16;
+  static constexpr int KT = (N2p + 3) / 4, K2 = 4 * KT, KB = (KT + 3) / 4, NPT = (Np + 15) / 16;
+  static constexpr int FS = TB + 1;  // f row stride: element columns + 1 (bank spread)
+  // sp [TB][Np] | sc [TB][Np] | f [K2][FS]
+  static constexpr size_t floats = (size_t)2 * TB * Np + (size_t)K2 * FS;
+};
+
+template <int N>
+__global__ __launch_bounds__(256) void k_update_bbwadg(int K, const float *__restrict__ CN,
+                                                       const float *__restrict__ PBlk,
+                                                       const float *__restrict__ c2bb,
+                                                       float *__restrict__ rhs) {
+  using L = BBLayout<N>;
+  constexpr int Np = L::Np, TB = L::TB, FS = L::FS, K2 = L::K2, KT = L::KT, KB = L::KB;
+  extern __shared__ float lds[];
+  float *sp = lds, *sc = sp + TB * Np, *f = sc + TB * Np;
+  const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int k0 = blockIdx.x * TB, nk = min(TB, K - k0);
+  // ---- 1
+  for (int i = tid; i < K2 * FS; i += 256) f[i] = 0.f;
+  for (int i = tid; i < TB * Np; i += 256) {
+    const int e = i / Np, n = i - e * Np;
+    const float cn = CN[n];
+    const bool in = e < nk;
+    sp[i] = in ? cn * rhs[(size_t)(k0 + e) * 4 * Np + n] : 0.f;
+    sc[i] = in ? cn * c2bb[(size_t)(k0 + e) * Np + n] : 0.f;
+  }
+  __syncthreads();
+  // ---- 2: the wave's elements wave + 4 i (i < 4) together, lanes over b (and b + 64 when
+  // Np > 64).  Multi-indices (i, j, k, l) in the reference order (l outer, k, j inner); the
+  // degree-2N index of (j, k, l) is Np(2N) - Np(2N - l) + k (2N - l + 1) - k (k - 1) / 2 + j, so for
+  // fixed (k_a, l_a) it steps by one with j_a.  The lanes' targets a + b are distinct for each a
+  // and only this wave updates its element columns, so the read-add-write updates need no
+  // atomics (LDS float atomics measured 12x slower).
+  {
+    constexpr int M = 2 * N, NpM = L::N2p;
+    auto npd = [](int d) { return (d + 1) * (d + 2) * (d + 3) / 6; };
+#pragma unroll
+    for (int b0 = 0; b0 < Np; b0 += 64) {
+      const int b = b0 + lane;
+      const bool on = b < Np;
+      // (j, k, l) of node b
+      int lb = 0, kb = 0, rem = on ? b : 0;
+      while (rem >= (N - lb + 1) * (N - lb + 2) / 2) { rem -= (N - lb + 1) * (N - lb + 2) / 2; ++lb; }
+      while (rem >= N - lb - kb + 1) { rem -= N - lb - kb + 1; ++kb; }
+      const int jb = rem;
+      float pb[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) pb[i] = on ? sp[(wave + 4 * i) * Np + b] : 0.f;
+      int a = 0;
+      for (int la = 0; la <= N; ++la) {
+        const int l = la + lb;
+        for (int ka = 0; ka <= N - la; ++ka) {
+          const int k = ka + kb;
+          const int g0 = NpM - npd(M - l) + k * (M - l + 1) - k * (k - 1) / 2 + jb;
+          for (int ja = 0; ja <= N - la - ka; ++ja, ++a) {
+            float *fg = f + (g0 + ja) * FS + wave;
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+              if (on) fg[4 * i] += sc[(wave + 4 * i) * Np + a] * pb[i];
+            }
+            // other lanes update this lane's next targets: keep each step's LDS reads after the
+            // previous step's writes (the compiler sees one lane, whose own addresses differ)
+            asm volatile("" ::: "memory");
+          }
+        }
+      }
+    }
+  }
+  __syncthreads();
+  // ---- 3: r = P' f; rows -> sp (its last reads were before the barrier)
+  {
+    const brsrc rP = mkrsrc(PBlk, (unsigned)L::NPT * KB * 1024u);
+    const int col = lane & 15, kr = lane >> 4;
+    const float *Bp = f + kr * FS + col;
+    for (int mt = wave; mt < L::NPT; mt += 4) {
+      // four independent accumulation chains (K-steps 4 kb + sb), summed at the end
+      f4 acc[4];
+#pragma unroll
+      for (int sb = 0; sb < 4; ++sb) acc[sb] = f4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll 2
+      for (int kb = 0; kb < KB; ++kb) {
+        const f4 a4 = bld4_s(rP, lane * 16u, (unsigned)(mt * KB + kb) * 1024u);
+#pragma unroll
+        for (int sb = 0; sb < 4; ++sb) {
+          const int kt = 4 * kb + sb;
+          if (kt < KT) acc[sb] = __builtin_amdgcn_mfma_f32_16x16x4f32(a4[sb], Bp[kt * 4 * FS], acc[sb], 0, 0, 0);
+        }
+      }
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int n = mt * 16 + 4 * kr + r;
+        if (n < Np) sp[col * Np + n] = (acc[0][r] + acc[1][r]) + (acc[2][r] + acc[3][r]);
+      }
+    }
+  }
+  __syncthreads();
+  // ---- 4
+  for (int i = tid; i < nk * Np; i += 256) {
+    const int e = i / Np, n = i - e * Np;
+    rhs[(size_t)(k0 + e) * 4 * Np + n] = sp[i];
+  }
+}
+
+template <int N>
+static hipError_t update_bbwadg_n(bbw_ctx *c) {
+  using L = BBLayout<N>;
+  const size_t sh = sizeof(float) * L::floats;
+  if (sh > 65536) {
+    hipError_t e = hipFuncSetAttribute((const void *)k_update_bbwadg<N>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh);
+    if (e != hipSuccess) return e;
+  }
+  hipLaunchKernelGGL((k_update_bbwadg<N>), dim3((c->K + L::TB - 1) / L::TB), dim3(256), sh, c->stream, c->K,
+                     c->bb_CN, c->bb_PBlk, c->c2bb, c->rhs);
+  return hipGetLastError();
+}
+
+hipError_t launch_update_bbwadg(bbw_ctx *c) {
+  switch (c->N) {
+    case 1: return update_bbwadg_n<1>(c);
+    case 2: return update_bbwadg_n<2>(c);
+    case 3: return update_bbwadg_n<3>(c);
+    case 4: return update_bbwadg_n<4>(c);
+    case 5: return update_bbwadg_n<5>(c);
+    case 6: return update_bbwadg_n<6>(c);
+  }
+  return hipErrorInvalidValue;
+}
+
 FusedOps fused_ops(const bbw_ctx *c) {
   FusedOps o;
   o.Fmask = c->Fmask;

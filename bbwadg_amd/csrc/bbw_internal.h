@@ -120,6 +120,10 @@ struct bbw_ctx {
 
   // adjoint: transposed operators (D_m^T rows [4][Np] int4/float4, EEL^T [WE][4Nfp], L0^T
   // [WL][Nfp]), node -> face-node slots [Np][3], face-node neighbour map [K][4Nfp], buffers
+  // exact BB-WADG update (bbw_upload_bbwadg): C(N,a) [Np], the folded projection
+  // P diag(1/C(2N,g)) as lane-ordered MFMA A blocks (M = Np, K = N2p) and c2 in BB
+  // coefficients [K][Np]
+  float *bb_CN = nullptr, *bb_PBlk = nullptr, *c2bb = nullptr;
   int4 *DTids = nullptr;
   // factored Bernstein derivative (fused stage): shift [Np1] int4, elevation ids [Np] int4 and
   // weights [Np] float4
@@ -170,6 +174,7 @@ hipError_t launch_adj_inject(bbw_ctx *c, const float *resid_row);
 hipError_t launch_energy(bbw_ctx *c, double *E_dev_partial, int *nblocks);
 hipError_t launch_reduce(bbw_ctx *c, const double *partial, int n, double *out);
 hipError_t launch_record(bbw_ctx *c, const float *Q, float *row);
+hipError_t launch_update_bbwadg(bbw_ctx *c);
 
 hipError_t launch_volume_elas(bbw_ctx *c);
 hipError_t launch_surface_elas(bbw_ctx *c);

@@ -53,6 +53,9 @@ def main():
     ap.add_argument("--impl", choices=["mfma", "vector"], default="mfma")
     ap.add_argument("--project-velocity", action="store_true")
     ap.add_argument("--elastic", action="store_true", help="elastic (9 fields), plain stage")
+    ap.add_argument("--test-rk", action="store_true",
+                    help="instead: time the WADG update variants per launch (test_RK): the plain "
+                         "MFMA update (all four fields + LSRK) and the exact BB-WADG pressure update")
     ap.add_argument("--copy-bytes", type=int, default=0,
                     help="instead: one device-to-device copy of this many bytes (counter calibration)")
     args = ap.parse_args()
@@ -65,6 +68,22 @@ def main():
         b.copy_(a)
         torch.cuda.synchronize()
         print(f"copied {n * 4} bytes", flush=True)
+        return
+    if args.test_rk:
+        s = setup(args.mesh, args.N)
+        xq, yq, zq = s.cubature_points()
+        s.InitBBWADG(media.c2_fourier(1)(xq, yq, zq))
+        s.set_rhs(np.random.default_rng(2).uniform(-1, 1, (s.K, 4, s.ops.Np)).astype(np.float32))
+        for name, fn in (("update_wadg (MFMA, 4 fields + LSRK)", lambda: s.update_wadg(0.5, 0.1, 1e-4)),
+                         ("update_bbwadg (exact BB product, pressure)", s.update_bbwadg)):
+            fn()
+            s.sync()
+            t0 = time.perf_counter()
+            for _ in range(args.steps):
+                fn()
+            s.sync()
+            el = (time.perf_counter() - t0) / args.steps
+            print(f"K={s.K} N={args.N} {name}: {el * 1e6:.1f} us per launch", flush=True)
         return
     if args.elastic:
         s = setup_elastic(args.mesh, args.N)

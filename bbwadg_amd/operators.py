@@ -250,3 +250,63 @@ def packed_device_ops(ops):
         "Vq": np.ascontiguousarray(ops.Vq.T, dtype=f32),   # column-major Nq x Np
         "Pq": np.ascontiguousarray(ops.Pq.T, dtype=f32),   # column-major Np x Nq
     }
+
+
+def bbwadg_exact_ops(N):
+    """Operators of the exact Bernstein-Bezier WADG update (``rk_update_BBWADG``), restating the
+    BB branch of ``InitWADG_subelem`` (``cpp/src/WaveOKL3d.cpp:488-690``):
+
+    * ``subtet_ids`` [Np][Np]: degree-2N index of the multi-index sum a + b (the BB product
+      rule B^N_a B^N_b = C(N,a) C(N,b) / C(2N,a+b) B^2N_{a+b}); symmetric;
+    * ``CNscale`` [Np] = C(N, a), ``invC2Nscale`` [N2p] = 1 / C(2N, g) (multinomials);
+    * ``Ei_ids4``/``Ei_vals4`` [2N][N2p][4]: the one-step degree elevations E_i (degree i ->
+      i + 1, i = 0 .. 2N-1) as fixed-width sparse rows (at most 4 non-zeros, value g_m / (i+1)
+      at the column of g - e_m), rows beyond Np(i+1) zero; the reference obtains them as
+      VB_{i+1} \\ VB_i at the degree-(i+1) nodes;
+    * ``cj`` [N+1]: P_{N <- 2N} = sum_j c_j E_{N-j}^N (E_{N-j}^{2N})^T, from the reference's
+      (N+1) x (N+1) system L c = b (d = 3).
+    """
+    from math import factorial
+
+    def mult(M, a):
+        c = factorial(M)
+        for x in a:
+            c //= factorial(int(x))
+        return c
+
+    def fr(a, b):
+        r = 1
+        for x in range(b + 1, a + 1):
+            r *= x
+        return r
+
+    mi = basis.tet_multi_indices(N)
+    Np = len(mi)
+    m2 = basis.tet_multi_indices(2 * N)
+    N2p = len(m2)
+    idx2 = {tuple(x): i for i, x in enumerate(m2)}
+    subtet = np.array([[idx2[tuple(a + b)] for b in mi] for a in mi], dtype=np.int32)
+    CN = np.array([mult(N, a) for a in mi], dtype=np.float64)
+    invC2N = np.array([1.0 / mult(2 * N, g) for g in m2], dtype=np.float64)
+    Eids = np.zeros((2 * N, N2p, 4), dtype=np.int32)
+    Evals = np.zeros((2 * N, N2p, 4), dtype=np.float64)
+    for i in range(2 * N):
+        lo = {tuple(x): n for n, x in enumerate(basis.tet_multi_indices(i))}
+        for row, g in enumerate(basis.tet_multi_indices(i + 1)):
+            ents = []
+            for m in range(4):
+                if g[m] > 0:
+                    a = g.copy()
+                    a[m] -= 1
+                    ents.append((lo[tuple(a)], g[m] / (i + 1)))
+            for w, (col, v) in enumerate(sorted(ents)):
+                Eids[i, row, w], Evals[i, row, w] = col, v
+    L = np.zeros((N + 1, N + 1))
+    for j in range(N + 1):
+        Nj = N - j
+        for i in range(Nj + 1):
+            L[i, j] = fr(Nj, Nj - i) / fr(Nj + i + 3, Nj)
+    b = np.array([fr(2 * N, 2 * N - i) / fr(2 * N + i + 3, 2 * N) for i in range(N + 1)])
+    cj = np.linalg.solve(L, b)
+    return dict(subtet_ids=subtet, CNscale=CN, invC2Nscale=invC2N, Ei_ids4=Eids, Ei_vals4=Evals,
+                cj=cj, Np=Np, N2p=N2p)

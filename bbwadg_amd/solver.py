@@ -88,6 +88,23 @@ class WaveSolver:
         Jq = None if Jq is None else f32(Jq)
         check(self.ctx, self.lib.bbw_upload_media(self.ctx, ptr(self.c2q), ptr(Jq)))
 
+    def InitBBWADG(self, c2q):
+        """Operators and c2 for the exact Bernstein-Bezier WADG update (the BB branch of
+        ``InitWADG_subelem``, cpp/src/WaveOKL3d.cpp:486-690): c2 in degree-N BB coefficients per
+        element, c2_bb = Pq c2q (FP64), with the product and projection tables."""
+        d = operators.bbwadg_exact_ops(self.N)
+        self.c2bb = f32(np.asarray(c2q, dtype=np.float64) @ self.ops.Pq.T)
+        self._bb_keep = {k: (i32(v) if v.dtype.kind == "i" else f64(v)) for k, v in d.items()
+                         if isinstance(v, np.ndarray)}
+        k = self._bb_keep
+        check(self.ctx, self.lib.bbw_upload_bbwadg(self.ctx, ptr(k["subtet_ids"]), ptr(k["CNscale"]),
+                                                   ptr(k["invC2Nscale"]), ptr(k["Ei_ids4"]),
+                                                   ptr(k["Ei_vals4"]), ptr(k["cj"]), ptr(self.c2bb)))
+
+    def update_bbwadg(self):
+        """rhs_p <- P_N(c2 rhs_p) in place (``rk_update_BBWADG``)."""
+        check(self.ctx, self.lib.bbw_update_bbwadg(self.ctx))
+
     def InitElastic(self, rhoinvq, lamq, muq, tau_v=1.0, tau_s=1.0):
         self.rhoinvq, self.lamq, self.muq = f32(rhoinvq), f32(lamq), f32(muq)
         check(self.ctx, self.lib.bbw_upload_elastic_media(self.ctx, ptr(self.rhoinvq), ptr(self.lamq),

@@ -130,6 +130,22 @@ int bbw_update(bbw_ctx *ctx, float fa, float fb, float fdt);
  * velocity fields through Pq*Vq as rk_update_WADG does, 0 uses Pq*Vq = I. */
 int bbw_stage_fused(bbw_ctx *ctx, float fa, float fb, float fdt, float ftime, int project_velocity);
 
+/* Exact Bernstein-Bezier WADG pressure update, ≈ rk_update_BBWADG
+ * (cpp/okl/WaveKernelsBBWADG.okl:14-321, driven by test_RK, cpp/src/WaveOKL3d.cpp:2288-2296):
+ * rhs_p <- P_N(c2 rhs_p) in place on the context's rhsQ, with c2 given by its degree-N BB
+ * coefficients and P_N the L2 projection of the degree-2N product.  The operators are the
+ * reference's (InitWADG_subelem BB branch, cpp/src/WaveOKL3d.cpp:486-690), FP64 on the host:
+ * subtet_ids [Np][Np] degree-2N index of a + b; CNscale [Np] = C(N,a); invC2Nscale [N2p] =
+ * 1/C(2N,g); Ei_ids4 / Ei_vals4 [2N][N2p][4] one-step elevations E_i (degree i -> i+1) as
+ * 4-wide sparse rows; cj [N+1] of P = sum_j c_j E_{N-j}^N (E_{N-j}^{2N})^T; c2_bb [K][Np]
+ * (= Pq c2q per element).  N2p = (2N+1)(2N+2)(2N+3)/6.  The projection is folded into one
+ * dense Np x N2p matrix at upload (applied on MFMA); subtet_ids is checked against the index
+ * formula the kernel evaluates. */
+int bbw_upload_bbwadg(bbw_ctx *ctx, const int32_t *subtet_ids, const double *CNscale,
+                      const double *invC2Nscale, const int32_t *Ei_ids4, const double *Ei_vals4,
+                      const double *cj, const float *c2_bb);
+int bbw_update_bbwadg(bbw_ctx *ctx);
+
 /* ≈ Wave_RK inner loop (cpp/src/WaveOKL3d.cpp:2029-2080): nsteps LSRK45 steps of size dt
  * from time t0; stage coefficients cast to float as the reference does.  mode: BBW_MODE_*.
  * flags bit 0: project velocity in the fused path; bit 1: capture steps in a HIP graph. */

@@ -342,3 +342,21 @@ def test_wadg_update_against_exact_bb_product(N, kind):
     assert rel(s.get_res(), rres) < 2e-5
     assert rel(s.WaveGetData3d(), rQ) < 2e-5
     s.close()
+
+
+@pytest.mark.parametrize("N", [1, 2, 3, 4, 5, 6])
+def test_exact_bbwadg_update(N):
+    """bbw_update_bbwadg (rk_update_BBWADG) against the exact BB-WADG oracle: rhs_p <- P_N(c2 rhs_p)
+    with the Fourier c2 in BB coefficients; the velocity rows are not touched."""
+    import bb_exact as X
+    print(f"case exact BB-WADG kernel N={N}", flush=True)
+    P, s = make(N, 3)
+    s.InitBBWADG(P.c2q)
+    R = P.random_state()
+    s.set_rhs(R)
+    s.update_bbwadg()
+    got = s.get_rhs()
+    ref = X.wadg_exact(P.ops, s.c2bb.astype(np.float64), R[:, 0].astype(np.float64))
+    assert rel(got[:, 0], ref) < 2e-5
+    assert np.array_equal(got[:, 1:], R[:, 1:])
+    s.close()
