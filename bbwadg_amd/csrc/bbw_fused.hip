@@ -243,7 +243,7 @@ __global__ __launch_bounds__(256, N <= 4 ? 4 : N == 5 ? 3 : 2) void k_update_wad
           if (slot >= 0) v += w4(src, fr) * ft * src.coef[(size_t)slot * Np + nr];
         }
         const unsigned g = rr * 4u + e * ROW;
-        const float x = (read_res ? fa * bld(rRs, g) : 0.f) + fdt * v;
+        const float x = fa * bld(rRs, g) + fdt * v;
         bst(rRs, x, g);
         bst(rQ, bld(rQ, g) + fb * x, g);
       }
@@ -719,10 +719,8 @@ __global__ __launch_bounds__(256, N <= 4 ? 4 : N == 5 ? 3 : 2) void k_update_ela
     __syncthreads();
   }
 
-  // ---- 4: LSRK stage: res = fa res + fdt (rhs + source); Q += fb res (res is not read when
-  // fa = 0, the first LSRK45 stage)
+  // ---- 4: LSRK stage: res = fa res + fdt (rhs + source); Q += fb res
   const float ft = ftime_dev ? *ftime_dev : ftime;
-  const bool read_res = fa != 0.f;
   for (int rr = tid; rr < NpNf; rr += 256) {
     const int fr = rr / Np, nr = rr - fr * Np;
     float wsel = 0.f;
@@ -736,7 +734,7 @@ __global__ __launch_bounds__(256, N <= 4 ? 4 : N == 5 ? 3 : 2) void k_update_ela
         if (slot >= 0) v += wsel * ft * src.coef[(size_t)slot * Np + nr];
       }
       const unsigned gi = rr * 4u + e * ROW;
-      const float x = (read_res ? fa * bld(rRs, gi) : 0.f) + fdt * v;
+      const float x = fa * bld(rRs, gi) + fdt * v;
       bst(rRs, x, gi);
       bst(rQ, bld(rQ, gi) + fb * x, gi);
     }

@@ -288,7 +288,6 @@ __device__ __forceinline__ void tile_shot(const FusedMesh &m, const FusedOps &o,
   // ---- 1: source slots of the tile's elements (read in phase 8, after several barriers); the
   // register inputs (res, c2, traces) were issued by tile_prefetch before the Q_in tile's barrier
   if (has_src && tid < TB) sSlot[tid] = tid < nk ? src_slot2(src, k0 + tid) : -1;
-  const bool read_res = fa != 0.f;
   at_start();
 
   // ---- 2: upwind flux at face nodes into registers.  The p and U fluxes are (phi, -phi) with
@@ -591,7 +590,7 @@ __device__ __forceinline__ void tile_shot(const FusedMesh &m, const FusedOps &o,
           if (slot >= 0) v += w4(src, fr) * ft * src.coef[(size_t)slot * Np + nr];
         }
         const unsigned g = rr * 4u + e * ROW;
-        const float x = (read_res ? fa * bld(rR, g) : 0.f) + fdt * v;
+        const float x = fa * bld(rR, g) + fdt * v;
         if (sh.store_res) bst(rR, x, g);
         bst(rO, sQ[e * NpNf + rr] + fb * x, g);
       }
