@@ -184,7 +184,7 @@ void bbw_destroy(bbw_ctx *c) {
   dfree(c->Dids); dfree(c->Dvals); dfree(c->Fmask); dfree(c->EEL_ids); dfree(c->L0_ids);
   dfree(c->EEL_vals); dfree(c->L0_vals); dfree(c->Vq); dfree(c->Pq); dfree(c->Vq64); dfree(c->VqPad); dfree(c->PqPad); dfree(c->EEL_ids4); dfree(c->LiftPad); dfree(c->VqBlk); dfree(c->PqBlk); dfree(c->LiftBlk); dfree(c->PqTBlk); dfree(c->VqTBlk); dfree(c->LiftTBlk); dfree(c->PqTPad); dfree(c->VqTPad);
   dfree(c->bb_CN); dfree(c->bb_PBlk); dfree(c->c2bb);
-  dfree(c->DTids); dfree(c->DTvals); dfree(c->Sids); dfree(c->Eids); dfree(c->Evals); dfree(c->EELT_ids); dfree(c->EELT_vals); dfree(c->L0T_ids);
+  dfree(c->DTids); dfree(c->DTvals); dfree(c->Sids); dfree(c->Eids); dfree(c->Evals); dfree(c->Swts); dfree(c->EELT_ids); dfree(c->EELT_vals); dfree(c->L0T_ids);
   dfree(c->L0T_vals); dfree(c->FmaskInv); dfree(c->mapP); dfree(c->lQ); dfree(c->lres);
   dfree(c->Zadj); dfree(c->Hadj); dfree(c->grad); dfree(c->Ybuf); dfree(c->ckpt);
   dfree(c->resid); dfree(c->Q0save); dfree(c->Ycache);
@@ -306,12 +306,13 @@ int bbw_upload_operators(bbw_ctx *c, const bbw_ops *o) {
         return -1;
       };
       std::vector<int> sid(4 * (size_t)Np1), eid(4 * (size_t)Np);
-      std::vector<float> ev(4 * (size_t)Np);
+      std::vector<float> ev(4 * (size_t)Np), sw(4 * (size_t)Np1);
       for (int b = 0; b < Np1; ++b)
         for (int m = 0; m < 4; ++m) {
           std::array<int, 4> a = aM[b];
           ++a[m];
           sid[4 * b + m] = find(aN, a);
+          sw[4 * b + m] = (float)(aM[b][m] + 1);
         }
       for (int a = 0; a < Np; ++a) {
         int first = -1;
@@ -345,6 +346,8 @@ int bbw_upload_operators(bbw_ctx *c, const bbw_ops *o) {
       dfree(c->Sids);
       dfree(c->Eids);
       dfree(c->Evals);
+      dfree(c->Swts);
+      if ((rc = upload(c, &c->Swts, (const float4 *)sw.data(), (size_t)Np1))) return rc;
       if ((rc = upload(c, &c->Sids, (const int4 *)sid.data(), (size_t)Np1))) return rc;
       if ((rc = upload(c, &c->Eids, (const int4 *)eid.data(), (size_t)Np))) return rc;
       if ((rc = upload(c, &c->Evals, (const float4 *)ev.data(), (size_t)Np))) return rc;

@@ -272,6 +272,10 @@ struct AdjLayout {
   static constexpr int RS0 = 4 * Np, RS = RS0 + ((34 - RS0 % 32) % 32);
   static constexpr int NG3 = 256 / NfpNf, EPT3 = (TB + NG3 - 1) / NG3;
   static constexpr int GS = ((Np + 63) / 64) * 64, NG = 256 / GS, EPTV = (TB + NG - 1) / NG;
+  // degree N-1 nodes of the factored transposed derivative (k_adj2 step 1)
+  static constexpr int Np1 = N * (N + 1) * (N + 2) / 6;
+  static constexpr int GS1 = Np1 <= 32 ? 32 : 64, NG1 = 256 / GS1, EPT1 = TB / NG1;
+  static_assert(Np1 <= 64 && NG1 * EPT1 == TB && 5 * Np1 <= 4 * Np, "step 1 covers the tile and fits sZ");
   static constexpr bool PF = 4 * Np <= 256;
   static constexpr int cmax(int a, int b) { return a > b ? a : b; }
   // sg [TB][RS] | S [NQP][16] | sF [TB][16]
@@ -493,8 +497,10 @@ __global__ __launch_bounds__(256, N <= 5 ? 4 : 2) void k_adj1(int K, FusedOps o,
 template <int N>
 __global__ __launch_bounds__(256) void k_adj2(int K, const float *__restrict__ vgeo,
                                               const float *__restrict__ fgeo,
-                                              const int4 *__restrict__ DTids,
-                                              const float4 *__restrict__ DTvals,
+                                              const int4 *__restrict__ Sids,
+                                              const float4 *__restrict__ Swts,
+                                              const int4 *__restrict__ Eids,
+                                              const float4 *__restrict__ Evals,
                                               const int *__restrict__ FmaskInv,
                                               const int *__restrict__ mapP,
                                               const float *__restrict__ Z,
@@ -551,33 +557,55 @@ __global__ __launch_bounds__(256) void k_adj2(int K, const float *__restrict__ v
     }
   }
   __syncthreads();
-  if (!on) return;
-  int4 di[4];
-  float4 dv[4];
+  // Volume transpose in factored form.  D_m = E . Shift_m (times N), so
+  // D_m^T v [c] = [c_m >= 1] T[c - e_m] with T = N E^T v, T[b] = sum_j (b_j + 1) v[b + e_j] at the
+  // degree-(N-1) nodes b.  Step 1 forms T of p, Ur + Us + Ut, Ur, Us and Ut from the (p, U) tile;
+  // step 2 reads it at c - e_m.
+  float4 ta[L::EPT1];
+  float tb[L::EPT1];
+  const int g1 = tid / L::GS1, n1 = tid - g1 * L::GS1;
+  if (n1 < L::Np1) {
+    const int4 S = Sids[n1];
+    const float4 wt = Swts[n1];
 #pragma unroll
-  for (int m = 0; m < 4; ++m) {
-    di[m] = DTids[m * Np + nv];
-    dv[m] = DTvals[m * Np + nv];
+    for (int i = 0; i < L::EPT1; ++i) {
+      const float4 *V = sZ + (g1 + L::NG1 * i) * Np;
+      const float4 q1 = V[S.x], q2 = V[S.y], q3 = V[S.z], q4 = V[S.w];
+      const float ur = wt.x * q1.y + wt.y * q2.y + wt.z * q3.y + wt.w * q4.y;
+      const float us = wt.x * q1.z + wt.y * q2.z + wt.z * q3.z + wt.w * q4.z;
+      const float ut = wt.x * q1.w + wt.y * q2.w + wt.z * q3.w + wt.w * q4.w;
+      ta[i] = make_float4(wt.x * q1.x + wt.y * q2.x + wt.z * q3.x + wt.w * q4.x, ur + us + ut, ur, us);
+      tb[i] = ut;
+    }
   }
+  __syncthreads();
+  float4 *sT = sZ;                                     // [TB][Np1] (p, Ur + Us + Ut, Ur, Us)
+  float *sTw = (float *)(sT + TB * L::Np1);            // [TB][Np1] Ut
+  if (n1 < L::Np1) {
+#pragma unroll
+    for (int i = 0; i < L::EPT1; ++i) {
+      const int e = g1 + L::NG1 * i;
+      sT[e * L::Np1 + n1] = ta[i];
+      sTw[e * L::Np1 + n1] = tb[i];
+    }
+  }
+  __syncthreads();
+  if (!on) return;
+  const int4 E = Eids[nv];
+  const float4 ev = Evals[nv];
+  const float mx = ev.x > 0.f ? 1.f : 0.f, my = ev.y > 0.f ? 1.f : 0.f;
+  const float mz = ev.z > 0.f ? 1.f : 0.f, mw = ev.w > 0.f ? 1.f : 0.f;
 #pragma unroll
   for (int i = 0; i < L::EPTV; ++i) {
     const int e = gv + L::NG * i;
-    const float4 *V = sZ + e * Np;
+    const float4 *T = sT + e * L::Np1;
+    const float4 t1 = T[E.x], t2 = T[E.y], t3 = T[E.z], t4 = T[E.w];
+    const float t4w = sTw[e * L::Np1 + E.w];
     float dp[4], dz[4];
-#pragma unroll
-    for (int m = 0; m < 4; ++m) {
-      const float4 a = V[di[m].x], b = V[di[m].y], c = V[di[m].z], d = V[di[m].w];
-      dp[m] = dv[m].x * a.x + dv[m].y * b.x + dv[m].z * c.x + dv[m].w * d.x;
-      if (m == 0)
-        dz[0] = dv[0].x * (a.y + a.z + a.w) + dv[0].y * (b.y + b.z + b.w) +
-                dv[0].z * (c.y + c.z + c.w) + dv[0].w * (d.y + d.z + d.w);
-      else if (m == 1)
-        dz[1] = dv[1].x * a.y + dv[1].y * b.y + dv[1].z * c.y + dv[1].w * d.y;
-      else if (m == 2)
-        dz[2] = dv[2].x * a.z + dv[2].y * b.z + dv[2].z * c.z + dv[2].w * d.z;
-      else
-        dz[3] = dv[3].x * a.w + dv[3].y * b.w + dv[3].z * c.w + dv[3].w * d.w;
-    }
+    dp[0] = mx * t1.x; dz[0] = mx * t1.y;
+    dp[1] = my * t2.x; dz[1] = my * t2.z;
+    dp[2] = mz * t3.x; dz[2] = mz * t3.w;
+    dp[3] = mw * t4.x; dz[3] = mw * t4w;
     const float *G = sG + 9 * e;
     float op = -.5f * (dz[1] + dz[2] + dz[3] - dz[0]);
     const float wr = .5f * (dp[1] - dp[0]), ws = .5f * (dp[2] - dp[0]), wt = .5f * (dp[3] - dp[0]);
@@ -625,7 +653,8 @@ static hipError_t adj_tiled_n(bbw_ctx *c, const float *Y, float fa, float fb, fl
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(k_adj2<N>, dim3(grid), dim3(256), s2, c->stream, c->K, c->vgeo, c->fgeo,
-                     c->DTids, c->DTvals, c->FmaskInv, c->mapP, c->Zadj, c->Hadj, fdt, c->lQ);
+                     c->Sids, c->Swts, c->Eids, c->Evals, c->FmaskInv, c->mapP, c->Zadj, c->Hadj, fdt,
+                     c->lQ);
   return hipGetLastError();
 }
 

@@ -129,6 +129,7 @@ struct bbw_ctx {
   // weights [Np] float4
   int4 *Sids = nullptr, *Eids = nullptr;
   float4 *Evals = nullptr;
+  float4 *Swts = nullptr;  // [Np1]: (b_1 + 1, .., b_4 + 1), weights of N E^T (adjoint volume)
   float4 *DTvals = nullptr;
   int *EELT_ids = nullptr, *L0T_ids = nullptr, *FmaskInv = nullptr, *mapP = nullptr;
   float *EELT_vals = nullptr, *L0T_vals = nullptr;
