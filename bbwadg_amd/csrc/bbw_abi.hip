@@ -963,14 +963,17 @@ int bbw_adjoint_gradient(bbw_ctx *c, int nsteps, double dt, double t0, const flo
   }
   if ((rc = save_state(c))) return rc;
 
+  const bool a0_zero = c->rk4a[0] == 0.0;
   auto stage_time = [&](int step, int i) -> float {
     return c->nsrc ? (float)ricker(t0 + step * dt + c->rk4c[i] * dt, c->src_f0) : 0.f;
   };
   auto fwd_step_y = [&](int step, float *Y) -> int {
     for (int i = 0; i < 5; ++i) {
+      // the residual left by the last stage is dead when a_0 = 0 (the state is restored and the
+      // residual cleared after the gradient)
       hipError_t e = launch_fused_mfma(c, c->Q[c->cur], c->Q[1 - c->cur], (float)c->rk4a[i],
                                        (float)c->rk4b[i], (float)dt, nullptr, stage_time(step, i),
-                                       0, Y + (size_t)i * c->K * c->Np);
+                                       0, Y + (size_t)i * c->K * c->Np, !(i == 4 && a0_zero));
       if (e != hipSuccess) return hipfail(c, e, "forward stage with Vq (A Q)_p capture");
       c->cur = 1 - c->cur;
     }
@@ -981,7 +984,10 @@ int bbw_adjoint_gradient(bbw_ctx *c, int nsteps, double dt, double t0, const flo
     if (e != hipSuccess) return hipfail(c, e, "receiver injection");
     for (int i = 4; i >= 0; --i) {
       const float *Yi = Y + (size_t)i * c->K * c->Np;
-      e = c->adj_impl ? launch_adj_stage_tiled(c, Yi, (float)c->rk4a[i], (float)c->rk4b[i], (float)dt)
+      // with a_0 = 0 the reverse of stage 0 sets lres to 0 (dead store), so the reverse of stage 4,
+      // which always follows it or the zeroed start, reads zeros
+      const int lres_io = a0_zero ? (i == 4 ? 0 : 1) | (i == 0 ? 0 : 2) : 3;
+      e = c->adj_impl ? launch_adj_stage_tiled(c, Yi, (float)c->rk4a[i], (float)c->rk4b[i], (float)dt, lres_io)
                       : launch_adj_stage(c, Yi, (float)c->rk4a[i], (float)c->rk4b[i], (float)dt);
       if (e != hipSuccess) return hipfail(c, e, "adjoint stage");
     }

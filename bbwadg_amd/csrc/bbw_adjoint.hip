@@ -11,6 +11,8 @@
 //   k_adj_apply  A^T z: volume transpose (D_m^T rows) + own and neighbour flux transposes
 //                gathered through Fmask^-1 and mapP; lQ += dt A^T z
 //   k_adj_inject receiver residuals into lQ_p (B_r^T (p_r - d_r))
+#include <type_traits>
+
 #include "bbw_fused_tile.h"
 
 namespace bbw {
@@ -298,7 +300,7 @@ template <int N>
 __global__ __launch_bounds__(256, N <= 5 ? 4 : 2) void k_adj1(int K, FusedOps o, const float *__restrict__ c2q,
                                               const float *__restrict__ Y,
                                               const float *__restrict__ fgeo, float fa, float fb,
-                                              float fdt, const float *__restrict__ lQ,
+                                              float fdt, int lres_io, const float *__restrict__ lQ,
                                               float *__restrict__ lres, float *__restrict__ Z,
                                               float *__restrict__ H, float *__restrict__ grad,
                                               const float *__restrict__ PqTBlk,
@@ -333,25 +335,38 @@ __global__ __launch_bounds__(256, N <= 5 ? 4 : 2) void k_adj1(int K, FusedOps o,
   const brsrc rVT = mkrsrc(VqTBlk, (unsigned)L::NPT * KBP * 1024u);
   const brsrc rLT = mkrsrc(LiftTBlk, 4u * FMT * KBV * 1024u);
 
-  // ---- 1: g = lres + b lQ -> LDS; lres <- a g; fgeo; c2, y, grad in the MFMA output layout
+  // ---- 1: g = lres + b lQ -> LDS; lres <- a g; fgeo; c2, y, grad in the MFMA output layout.
+  // lres_io bit 0: read lres (else it is 0); bit 1: store a g (else it is dead)
   sF[tid] = bld(mkrsrc(fgeo + (size_t)k0 * 16, (unsigned)nk * 64u), tid * 4u);
-  if constexpr (L::PF) {
+  const bool st_lres = (lres_io & 2) != 0;
+  auto load_g = [&](auto rd) {
+    if constexpr (L::PF) {
+      float gv[TB];
 #pragma unroll
-    for (int e = 0; e < TB; ++e) {
-      const float g = bld(rRl, tid * 4u + e * ROW) + fb * bld(rQl, tid * 4u + e * ROW);
-      if (tid < NpNf) {
-        sg[e * RS + tid] = g;
-        bst(rRl, fa * g, tid * 4u + e * ROW);
-      }
-    }
-  } else {
-    for (int rr = tid; rr < NpNf; rr += 256)
       for (int e = 0; e < TB; ++e) {
-        const float g = bld(rRl, rr * 4u + e * ROW) + fb * bld(rQl, rr * 4u + e * ROW);
-        sg[e * RS + rr] = g;
-        bst(rRl, fa * g, rr * 4u + e * ROW);
+        const float lr = decltype(rd)::value ? bld(rRl, tid * 4u + e * ROW) : 0.f;
+        gv[e] = lr + fb * bld(rQl, tid * 4u + e * ROW);
       }
-  }
+      if (tid < NpNf) {
+#pragma unroll
+        for (int e = 0; e < TB; ++e) sg[e * RS + tid] = gv[e];
+        if (st_lres) {
+#pragma unroll
+          for (int e = 0; e < TB; ++e) bst(rRl, fa * gv[e], tid * 4u + e * ROW);
+        }
+      }
+    } else {
+      for (int rr = tid; rr < NpNf; rr += 256)
+        for (int e = 0; e < TB; ++e) {
+          const float lr = decltype(rd)::value ? bld(rRl, rr * 4u + e * ROW) : 0.f;
+          const float g = lr + fb * bld(rQl, rr * 4u + e * ROW);
+          sg[e * RS + rr] = g;
+          if (st_lres) bst(rRl, fa * g, rr * 4u + e * ROW);
+        }
+    }
+  };
+  if (lres_io & 1) load_g(std::true_type{});
+  else load_g(std::false_type{});
   for (int i = tid; i < TB * Np; i += 256) {
     const int e = i / Np;
     sU[e * L::RU + (i - e * Np)] = bld(rY, i * 4u);
@@ -638,7 +653,7 @@ __global__ __launch_bounds__(256) void k_adj2(int K, const float *__restrict__ v
 }
 
 template <int N>
-static hipError_t adj_tiled_n(bbw_ctx *c, const float *Y, float fa, float fb, float fdt) {
+static hipError_t adj_tiled_n(bbw_ctx *c, const float *Y, float fa, float fb, float fdt, int lres_io) {
   using L = AdjLayout<N>;
   const int grid = (c->K + L::TB - 1) / L::TB;
   const size_t s1 = sizeof(float) * L::floats1, s2 = sizeof(float) * L::floats2;
@@ -648,7 +663,7 @@ static hipError_t adj_tiled_n(bbw_ctx *c, const float *Y, float fa, float fb, fl
     hipFuncSetAttribute((const void *)k_adj2<N>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)s2);
   if ((unsigned long long)c->K * 16 * L::Np >= (1ull << 32)) return hipErrorInvalidValue;
   hipLaunchKernelGGL(k_adj1<N>, dim3(grid), dim3(256), s1, c->stream, c->K, fused_ops(c), c->c2q, Y,
-                     c->fgeo, fa, fb, fdt, c->lQ, c->lres, c->Zadj, c->Hadj, c->grad, c->PqTBlk,
+                     c->fgeo, fa, fb, fdt, lres_io, c->lQ, c->lres, c->Zadj, c->Hadj, c->grad, c->PqTBlk,
                      c->VqTBlk, c->LiftTBlk);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
@@ -658,14 +673,14 @@ static hipError_t adj_tiled_n(bbw_ctx *c, const float *Y, float fa, float fb, fl
   return hipGetLastError();
 }
 
-hipError_t launch_adj_stage_tiled(bbw_ctx *c, const float *Y, float fa, float fb, float fdt) {
+hipError_t launch_adj_stage_tiled(bbw_ctx *c, const float *Y, float fa, float fb, float fdt, int lres_io) {
   switch (c->N) {
-    case 1: return adj_tiled_n<1>(c, Y, fa, fb, fdt);
-    case 2: return adj_tiled_n<2>(c, Y, fa, fb, fdt);
-    case 3: return adj_tiled_n<3>(c, Y, fa, fb, fdt);
-    case 4: return adj_tiled_n<4>(c, Y, fa, fb, fdt);
-    case 5: return adj_tiled_n<5>(c, Y, fa, fb, fdt);
-    case 6: return adj_tiled_n<6>(c, Y, fa, fb, fdt);
+    case 1: return adj_tiled_n<1>(c, Y, fa, fb, fdt, lres_io);
+    case 2: return adj_tiled_n<2>(c, Y, fa, fb, fdt, lres_io);
+    case 3: return adj_tiled_n<3>(c, Y, fa, fb, fdt, lres_io);
+    case 4: return adj_tiled_n<4>(c, Y, fa, fb, fdt, lres_io);
+    case 5: return adj_tiled_n<5>(c, Y, fa, fb, fdt, lres_io);
+    case 6: return adj_tiled_n<6>(c, Y, fa, fb, fdt, lres_io);
   }
   return hipErrorInvalidValue;
 }

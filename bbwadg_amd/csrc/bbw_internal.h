@@ -170,7 +170,8 @@ hipError_t launch_fused_mfma(bbw_ctx *c, const float *Qin, float *Qout, float fa
                              float fdt, const float *ftime_dev, float ftime, int project_velocity,
                              float *yout = nullptr, int store_res = 1);
 hipError_t launch_adj_stage(bbw_ctx *c, const float *Y, float fa, float fb, float fdt);
-hipError_t launch_adj_stage_tiled(bbw_ctx *c, const float *Y, float fa, float fb, float fdt);
+// lres_io bit 0: read the adjoint residual (else it is zero); bit 1: store it (else it is dead)
+hipError_t launch_adj_stage_tiled(bbw_ctx *c, const float *Y, float fa, float fb, float fdt, int lres_io = 3);
 hipError_t launch_adj_inject(bbw_ctx *c, const float *resid_row);
 hipError_t launch_energy(bbw_ctx *c, double *E_dev_partial, int *nblocks);
 hipError_t launch_reduce(bbw_ctx *c, const double *partial, int n, double *out);
