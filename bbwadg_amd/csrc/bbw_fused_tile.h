@@ -559,13 +559,10 @@ __device__ __forceinline__ void tile_shot(const FusedMesh &m, const FusedOps &o,
   const float ft = sh.ft;
   if constexpr (L::PF) {
     if (tid < NpNf) {
+      float x[TB];
       if (!has_src) {
 #pragma unroll
-        for (int e = 0; e < TB; ++e) {
-          const float x = fa * rreg[e] + fdt * sR[e * RS + tid];
-          if (sh.store_res) bst(rR, x, tid * 4u + e * ROW);
-          bst(rO, sQ[e * NpNf + tid] + fb * x, tid * 4u + e * ROW);
-        }
+        for (int e = 0; e < TB; ++e) x[e] = fa * rreg[e] + fdt * sR[e * RS + tid];
       } else {
         const int fr = tid / Np, nr = tid - fr * Np;
         const float wsel = w4(src, fr);
@@ -574,10 +571,14 @@ __device__ __forceinline__ void tile_shot(const FusedMesh &m, const FusedOps &o,
           float v = sR[e * RS + tid];
           const int slot = sSlot[e];
           if (slot >= 0) v += wsel * ft * src.coef[(size_t)slot * Np + nr];
-          const float x = fa * rreg[e] + fdt * v;
-          if (sh.store_res) bst(rR, x, tid * 4u + e * ROW);
-          bst(rO, sQ[e * NpNf + tid] + fb * x, tid * 4u + e * ROW);
+          x[e] = fa * rreg[e] + fdt * v;
         }
+      }
+#pragma unroll
+      for (int e = 0; e < TB; ++e) bst(rO, sQ[e * NpNf + tid] + fb * x[e], tid * 4u + e * ROW);
+      if (sh.store_res) {
+#pragma unroll
+        for (int e = 0; e < TB; ++e) bst(rR, x[e], tid * 4u + e * ROW);
       }
     }
   } else {
