@@ -68,7 +68,10 @@ def cpu_baseline(ops, dm, c2q, K, sample_elems, threads):
     """Oracle FP32 (OpenMP) over the first ``sample_elems`` elements for one LSRK step."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     os.environ["OMP_NUM_THREADS"] = str(threads)
+    import ctypes
     import oracle as orc
+    orc.lib()
+    ctypes.CDLL("libgomp.so.1").omp_set_num_threads(int(threads))
     oo = orc.OracleOps(ops, np.float32)
     Ks = min(sample_elems, K)
     rng = np.random.default_rng(0)
@@ -85,6 +88,20 @@ def cpu_baseline(ops, dm, c2q, K, sample_elems, threads):
                         np.float32(1e-4), rhs, res, Q)
     el = time.perf_counter() - t0
     return Ks * ops.Np * 4 * 5 / el, el, Ks
+
+
+def _host_cpu():
+    """CPU model and logical CPU count of this host (for the CPU baseline line)."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return f"{model}, {os.cpu_count()} logical CPUs"
 
 
 def adjoint_ratio(nsteps, mesh_n=32, N=5, ckpt=50):
@@ -254,9 +271,13 @@ def main():
     if not args.no_cpu:
         threads = max(1, min(os.cpu_count() or 1, 16))
         rate, el, Ks = cpu_baseline(ops, dm, c2q, K, args.cpu_sample, threads)
+        rate1, el1, Ks1 = cpu_baseline(ops, dm, c2q, K, max(1, min(args.cpu_sample, K) // 16), 1)
         cpu = {"value": rate, "unit": "DOF-stage/s", "cores": threads, "kind": "port",
                "sample": f"oracle FP32 C restatement (OpenMP), 1 LSRK45 step (5 stages) over the "
-                         f"first {Ks} of {K} elements of the same mesh/order, {el:.1f} s"}
+                         f"first {Ks} of {K} elements of the same mesh/order, {el:.1f} s",
+               "value_1_thread": rate1,
+               "sample_1_thread": f"the same on 1 thread over the first {Ks1} elements, {el1:.1f} s",
+               "host": _host_cpu()}
 
     out = {
         "metric": "DOF-stage updates/s N=5 FP32, batched vs plain; adjoint-grad/forward time ratio",

@@ -90,7 +90,7 @@ def test_update_plain():
 
 
 @pytest.mark.parametrize("impl", [_lib.FUSED_VECTOR, _lib.FUSED_MFMA])
-@pytest.mark.parametrize("N,proj", [(3, False), (4, True), (5, False), (5, True), (6, False), (2, True)])
+@pytest.mark.parametrize("N,proj", [(1, False), (3, False), (4, True), (5, False), (5, True), (6, False), (2, True)])
 def test_fused_stage(N, proj, impl):
     print(f"case fused N={N} proj={proj} impl={impl}", flush=True)
     P, s = make(N, 3)
