@@ -286,8 +286,13 @@ struct AdjLayout {
   static constexpr int KU = 4 * KTV, RU = KU + ((34 - KU % 32) % 32);
   // c2 and gradient tiles [TB * Nq] each (contiguous copies of the global rows); the H tile
   // [TB][NfpNf] reuses them
-  static constexpr int CG = cmax(2 * TB * Nq, TB * NfpNf);
-  static constexpr size_t floats1 = (size_t)TB * RS + U + TB * 16 + (size_t)TB * RU + CG;
+  // (padded to whole 256-word rounds, so every thread stores every word it loads: a conditional
+  // store lets the compiler sink its load, serialising the rounds)
+  static constexpr int CQ = ((TB * Nq + 255) / 256) * 256;
+  static constexpr int CG = cmax(2 * CQ, TB * NfpNf);
+  static constexpr int NIU = (TB * Np + 255) / 256, NIC = CQ / 256;
+  // ... | one scratch word for the (A Q)_p words past the tile
+  static constexpr size_t floats1 = (size_t)TB * RS + U + TB * 16 + (size_t)TB * RU + CG + 1;
   // sZ4 float4 [TB][Np] | sH [TB][NfpNf] | sG [TB][9] | sF [TB][16]
   static constexpr size_t floats2 = (size_t)TB * 4 * Np + (size_t)TB * NfpNf + TB * 9 + TB * 16;
 };
@@ -317,7 +322,8 @@ __global__ __launch_bounds__(256, N <= 5 ? 4 : 2) void k_adj1(int K, FusedOps o,
   float *sF = S + L::U;                   // [TB][16]
   float *sU = sF + TB * 16;               // [TB][RU] (A Q)_p of the forward stage
   float *sC = sU + TB * L::RU;            // [TB * Nq] c2        (phases 1-2)
-  float *sGr = sC + TB * Nq;              // [TB * Nq] gradient  (phases 1-3)
+  float *sGr = sC + L::CQ;                // [TB * Nq] gradient  (phases 1-3)
+  float *sScr = sC + L::CG;               // scratch word
   float *sHt = sC;                        // [TB][NfpNf] H       (phases 4-5)
   const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int col = lane & 15, kr = lane >> 4;
@@ -367,15 +373,27 @@ __global__ __launch_bounds__(256, N <= 5 ? 4 : 2) void k_adj1(int K, FusedOps o,
   };
   if (lres_io & 1) load_g(std::true_type{});
   else load_g(std::false_type{});
-  for (int i = tid; i < TB * Np; i += 256) {
-    const int e = i / Np;
-    sU[e * L::RU + (i - e * Np)] = bld(rY, i * 4u);
+  // (A Q)_p rows, c2 and gradient rows of the tile (the last two contiguous in memory and copied
+  // as such): every load issued before the first store
+  float yv[L::NIU], cv[L::NIC], gvv[L::NIC];
+#pragma unroll
+  for (int j = 0; j < L::NIU; ++j) yv[j] = bld(rY, (unsigned)(tid + 256 * j) * 4u);
+#pragma unroll
+  for (int j = 0; j < L::NIC; ++j) {
+    cv[j] = bld(rC, (unsigned)(tid + 256 * j) * 4u);
+    gvv[j] = bld(rGd, (unsigned)(tid + 256 * j) * 4u);
+  }
+#pragma unroll
+  for (int j = 0; j < L::NIU; ++j) {
+    const int i = tid + 256 * j, e = i / Np;
+    float *dst = i < TB * Np ? sU + e * L::RU + (i - e * Np) : sScr;
+    *dst = yv[j];
   }
   if (tid < TB * (L::KU - Np)) sU[(tid / (L::KU - Np)) * L::RU + Np + tid % (L::KU - Np)] = 0.f;
-  // c2 and gradient rows of the tile: contiguous in memory, copied as such
-  for (int i = tid; i < TB * Nq; i += 256) {
-    sC[i] = bld(rC, i * 4u);
-    sGr[i] = bld(rGd, i * 4u);
+#pragma unroll
+  for (int j = 0; j < L::NIC; ++j) {
+    sC[tid + 256 * j] = cv[j];
+    sGr[tid + 256 * j] = gvv[j];
   }
   f4 aT[L::QTW][KBV], aY[L::QTW][KBV];
 #pragma unroll
