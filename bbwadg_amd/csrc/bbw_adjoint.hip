@@ -293,8 +293,10 @@ struct AdjLayout {
   static constexpr int NIU = (TB * Np + 255) / 256, NIC = CQ / 256;
   // ... | one scratch word for the (A Q)_p words past the tile
   static constexpr size_t floats1 = (size_t)TB * RS + U + TB * 16 + (size_t)TB * RU + CG + 1;
-  // sZ4 float4 [TB][Np] | sH [TB][NfpNf] | sG [TB][9] | sF [TB][16]
-  static constexpr size_t floats2 = (size_t)TB * 4 * Np + (size_t)TB * NfpNf + TB * 9 + TB * 16;
+  // sZ4 float4 [TB][Np] | sH [TB][NfpNf] padded to whole 256-word rounds | sG [TB][9] in a
+  // 256-word slot | sF [TB][16]
+  static constexpr int HIT = (TB * NfpNf + 255) / 256;
+  static constexpr size_t floats2 = (size_t)TB * 4 * Np + (size_t)HIT * 256 + 256 + TB * 16;
 };
 
 // One reverse stage, element-local part, on 16-element tiles:
@@ -544,9 +546,9 @@ __global__ __launch_bounds__(256) void k_adj2(int K, const float *__restrict__ v
   constexpr unsigned ROW = NpNf * 4u;
   extern __shared__ float lds[];
   float4 *sZ = (float4 *)lds;              // [TB][Np]
-  float *sH = (float *)(sZ + TB * Np);     // [TB][NfpNf]
-  float *sG = sH + TB * NfpNf;             // [TB][9]
-  float *sF = sG + TB * 9;                 // [TB][16]
+  float *sH = (float *)(sZ + TB * Np);     // [TB][NfpNf], HIT x 256 words
+  float *sG = sH + L::HIT * 256;           // [TB][9], 256 words
+  float *sF = sG + 256;                    // [TB][16]
   const int tid = threadIdx.x;
   const int k0 = blockIdx.x * TB, nk = min(TB, K - k0);
   const size_t base = (size_t)k0 * NpNf;
@@ -554,9 +556,13 @@ __global__ __launch_bounds__(256) void k_adj2(int K, const float *__restrict__ v
   const brsrc rHt = mkrsrc(H + (size_t)k0 * NfpNf, (unsigned)nk * NfpNf * 4u);
   const brsrc rHa = mkrsrc(H, (unsigned)K * NfpNf * 4u);
   const brsrc rM = mkrsrc(mapP + (size_t)k0 * NfpNf, (unsigned)nk * NfpNf * 4u);
-  if (tid < TB * 9) sG[tid] = bld(mkrsrc(vgeo + (size_t)k0 * 9, (unsigned)nk * 36u), tid * 4u);
-  sF[tid] = bld(mkrsrc(fgeo + (size_t)k0 * 16, (unsigned)nk * 64u), tid * 4u);
-  for (int i = tid; i < TB * NfpNf; i += 256) sH[i] = bld(rHt, i * 4u);
+  // The tile loads go to registers first and to LDS after the last of them is issued, with
+  // unconditional stores into whole 256-word rounds (words past the tile hold zeros, never read).
+  float hreg[L::HIT];
+#pragma unroll
+  for (int r = 0; r < L::HIT; ++r) hreg[r] = bld(rHt, (unsigned)(tid + 256 * r) * 4u);
+  const float gval = bld(mkrsrc(vgeo + (size_t)k0 * 9, (unsigned)nk * 36u), tid * 4u);
+  const float fval = bld(mkrsrc(fgeo + (size_t)k0 * 16, (unsigned)nk * 64u), tid * 4u);
   const int gv = tid / L::GS, nv = tid - gv * L::GS;
   const bool on = gv < L::NG && nv < Np;
   float zr[L::EPTV][4], lq[L::EPTV][4], hn[L::EPTV][3];
@@ -574,6 +580,10 @@ __global__ __launch_bounds__(256) void k_adj2(int K, const float *__restrict__ v
 #pragma unroll
     for (int a = 0; a < 3; ++a) mp[i][a] = bldi(rM, (unsigned)(e * NfpNf + (slot[a] < 0 ? 0 : slot[a])) * 4u);
   }
+#pragma unroll
+  for (int r = 0; r < L::HIT; ++r) sH[tid + 256 * r] = hreg[r];
+  sG[tid] = gval;
+  sF[tid] = fval;
 #pragma unroll
   for (int i = 0; i < L::EPTV; ++i)
 #pragma unroll

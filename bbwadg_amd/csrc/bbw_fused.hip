@@ -790,7 +790,8 @@ struct ElasSurfLayout {
   static constexpr int NG3 = 256 / NfpNf, EPT3 = (TB + NG3 - 1) / NG3;
   static constexpr int GS = ((Np + 63) / 64) * 64, NG = 256 / GS, EPTV = (TB + NG - 1) / NG;
   static constexpr int GW = 3 * TB * NfpNf;      // one field group's face data
-  static constexpr int W = QT > 2 * GW ? QT : 2 * GW;
+  static constexpr int QTP = (int)QIT * 1024;   // the Q tile padded to whole 4 KB rounds
+  static constexpr int W = QTP > 2 * GW ? QTP : 2 * GW;
   static constexpr size_t floats = (size_t)W + TB * 16;
 };
 
@@ -816,17 +817,10 @@ __global__ __launch_bounds__(256) void k_surface_elas_t(int K, const float *__re
   const int k0 = blockIdx.x * TB, nk = min(TB, K - k0);
   const brsrc rO = mkrsrc(rhs + (size_t)k0 * RowF, (unsigned)nk * RowF * 4u);
   const brsrc rT = mkrsrc(Q, (unsigned)K * RowF * 4u);
-  // ---- 1: Q tile, fgeo, the thread's vmapP entries and neighbour traces (9 fields)
-  {
-    const brsrc rQ = mkrsrc(Q + (size_t)k0 * RowF, (unsigned)nk * RowF * 4u);
-    float4 *sQ4 = (float4 *)sQ;
-#pragma unroll
-    for (unsigned it = 0; it < L::QIT; ++it) {
-      const unsigned off = tid * 16u + it * 4096u;
-      if (L::QIT * 4096u == L::QB || off < L::QB) sQ4[off / 16] = bld4(rQ, off);
-    }
-  }
-  sF[tid] = bld(mkrsrc(fgeo + (size_t)k0 * 16, (unsigned)nk * 64u), tid * 4u);
+  // ---- 1: the thread's face node and vmapP entries, the Q tile, fgeo, then the neighbour
+  // traces (9 fields).  Every load is issued before the tile's LDS stores, which are
+  // unconditional (the region holds whole 4 KB rounds): a conditional store lets the compiler
+  // sink its load and serialise the rounds.
   const int g3 = tid / NfpNf, m3 = tid - g3 * NfpNf;
   const bool on3 = g3 < L::NG3;
   const int f3 = m3 / Nfp;
@@ -837,11 +831,25 @@ __global__ __launch_bounds__(256) void k_surface_elas_t(int K, const float *__re
 #pragma unroll
     for (int i = 0; i < L::EPT3; ++i) vm[i] = bldi(rv, (unsigned)((g3 + L::NG3 * i) * NfpNf + m3) * 4u);
   }
+  float4 qreg[L::QIT];
+  {
+    const brsrc rQ = mkrsrc(Q + (size_t)k0 * RowF, (unsigned)nk * RowF * 4u);
+#pragma unroll
+    for (unsigned it = 0; it < L::QIT; ++it) qreg[it] = bld4(rQ, tid * 16u + it * 4096u);
+  }
+  const float fgv = bld(mkrsrc(fgeo + (size_t)k0 * 16, (unsigned)nk * 64u), tid * 4u);
   float fl[L::EPT3][9];
 #pragma unroll
   for (int i = 0; i < L::EPT3; ++i)
 #pragma unroll
     for (int c = 0; c < 9; ++c) fl[i][c] = bld(rT, (unsigned)(vm[i] + c * Np) * 4u);
+  {
+    static_assert(L::QIT * 1024 <= L::W, "the tile region holds whole rounds");
+    float4 *sQ4 = (float4 *)sQ;
+#pragma unroll
+    for (unsigned it = 0; it < L::QIT; ++it) sQ4[tid + it * 256u] = qreg[it];
+  }
+  sF[tid] = fgv;
   __syncthreads();
   // ---- 2: flux per face node (into the same registers)
   if (on3) {
@@ -986,8 +994,10 @@ struct ElasVolLayout {
   static constexpr int Np = Dims<N>::Np, TB = 16, RowF = 9 * Np, QT = TB * RowF;
   static constexpr unsigned QB = QT * 4u, QIT = (QB + 4095) / 4096;
   static constexpr int GS = ((Np + 63) / 64) * 64, NG = 256 / GS, EPTV = (TB + NG - 1) / NG;
-  // Q tile [TB][9][Np], then packed [TB][Np] float4 | [TB][Np] float4 | [TB][Np] float | vgeo
-  static constexpr size_t floats = (size_t)QT + TB * 9;
+  // Q tile [TB][9][Np] (padded to whole 4 KB rounds), then packed [TB][Np] float4 | [TB][Np]
+  // float4 | [TB][Np] float in the same region; vgeo in a 256-word slot
+  static constexpr int QTP = (int)QIT * 1024;
+  static constexpr size_t floats = (size_t)QTP + 256;
 };
 
 template <int N>
@@ -1002,19 +1012,22 @@ __global__ __launch_bounds__(256) void k_volume_elas_t(int K, const float *__res
   float4 *sA = (float4 *)lds;                 // [TB][Np] (v1 v2 v3 sxx)
   float4 *sB = sA + TB * Np;                  // [TB][Np] (syy szz syz sxz)
   float *sC = (float *)(sB + TB * Np);        // [TB][Np] sxy
-  float *sG = lds + L::QT;                    // [TB][9]
+  float *sG = lds + L::QTP;                   // [TB][9] in a 256-word slot
   const int tid = threadIdx.x;
   const int k0 = blockIdx.x * TB, nk = min(TB, K - k0);
   {
+    // every load before the LDS stores, which are unconditional (whole rounds; words past the
+    // tile and past vgeo's TB x 9 are written but never read)
     const brsrc rQ = mkrsrc(Q + (size_t)k0 * RowF, (unsigned)nk * RowF * 4u);
+    float4 qreg[L::QIT];
+#pragma unroll
+    for (unsigned it = 0; it < L::QIT; ++it) qreg[it] = bld4(rQ, tid * 16u + it * 4096u);
+    const float gv0 = bld(mkrsrc(vgeo + (size_t)k0 * 9, (unsigned)nk * 36u), tid * 4u);
     float4 *sQ4 = (float4 *)sQ;
 #pragma unroll
-    for (unsigned it = 0; it < L::QIT; ++it) {
-      const unsigned off = tid * 16u + it * 4096u;
-      if (L::QIT * 4096u == L::QB || off < L::QB) sQ4[off / 16] = bld4(rQ, off);
-    }
+    for (unsigned it = 0; it < L::QIT; ++it) sQ4[tid + it * 256u] = qreg[it];
+    sG[tid] = gv0;
   }
-  if (tid < TB * 9) sG[tid] = bld(mkrsrc(vgeo + (size_t)k0 * 9, (unsigned)nk * 36u), tid * 4u);
   __syncthreads();
   const int gv = tid / L::GS, nv = tid - gv * L::GS;
   const bool onv = gv < L::NG && nv < Np;
@@ -1131,8 +1144,9 @@ struct BBLayout {
   static constexpr int Np = Dims<N>::Np, N2p = (2 * N + 1) * (2 * N + 2) * (2 * N + 3) / 6, TB = 16;
   static constexpr int KT = (N2p + 3) / 4, K2 = 4 * KT, KB = (KT + 3) / 4, NPT = (Np + 15) / 16;
   static constexpr int FS = TB + 1;  // f row stride: element columns + 1 (bank spread)
-  // sp [TB][Np] | sc [TB][Np] | f [K2][FS]
-  static constexpr size_t floats = (size_t)2 * TB * Np + (size_t)K2 * FS;
+  static constexpr int PR = (TB * Np + 255) / 256;  // 256-thread rounds of the row loads
+  // sp [TB][Np] | sc [TB][Np] (each PR x 256 words) | f [K2][FS]
+  static constexpr size_t floats = (size_t)2 * PR * 256 + (size_t)K2 * FS;
 };
 
 template <int N>
@@ -1143,17 +1157,28 @@ __global__ __launch_bounds__(256) void k_update_bbwadg(int K, const float *__res
   using L = BBLayout<N>;
   constexpr int Np = L::Np, TB = L::TB, FS = L::FS, K2 = L::K2, KT = L::KT, KB = L::KB;
   extern __shared__ float lds[];
-  float *sp = lds, *sc = sp + TB * Np, *f = sc + TB * Np;
+  float *sp = lds, *sc = sp + L::PR * 256, *f = sc + L::PR * 256;
   const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int k0 = blockIdx.x * TB, nk = min(TB, K - k0);
-  // ---- 1
-  for (int i = tid; i < K2 * FS; i += 256) f[i] = 0.f;
-  for (int i = tid; i < TB * Np; i += 256) {
-    const int e = i / Np, n = i - e * Np;
-    const float cn = CN[n];
-    const bool in = e < nk;
-    sp[i] = in ? cn * rhs[(size_t)(k0 + e) * 4 * Np + n] : 0.f;
-    sc[i] = in ? cn * c2bb[(size_t)(k0 + e) * Np + n] : 0.f;
+  // ---- 1: all row loads issued before the LDS stores (whole rounds, unconditional; rows of
+  // elements past the mesh and words past the tile read as zero through the range checks)
+  {
+    const brsrc rp = mkrsrc(rhs + (size_t)k0 * 4 * Np, (unsigned)nk * 16u * Np);
+    const brsrc rc = mkrsrc(c2bb + (size_t)k0 * Np, (unsigned)nk * 4u * Np);
+    float pv[L::PR], cv[L::PR], cn[L::PR];
+#pragma unroll
+    for (int r = 0; r < L::PR; ++r) {
+      const int i = tid + 256 * r, e = i / Np, n = i - e * Np;
+      pv[r] = bld(rp, (unsigned)(e * 4 * Np + n) * 4u);
+      cv[r] = bld(rc, (unsigned)i * 4u);
+      cn[r] = CN[n];
+    }
+    for (int i = tid; i < K2 * FS; i += 256) f[i] = 0.f;
+#pragma unroll
+    for (int r = 0; r < L::PR; ++r) {
+      sp[tid + 256 * r] = cn[r] * pv[r];
+      sc[tid + 256 * r] = cn[r] * cv[r];
+    }
   }
   __syncthreads();
   // ---- 2: the wave's elements wave + 4 i (i < 4) together, lanes over b (and b + 64 when
