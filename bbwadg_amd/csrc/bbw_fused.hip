@@ -69,8 +69,8 @@ struct UpdLayout {
   static constexpr int RS0 = 4 * Np > 3 * Np + 4 * KTV ? 4 * Np : 3 * Np + 4 * KTV;
   static constexpr int RS = RS0 + ((34 - RS0 % 32) % 32);
   static constexpr bool PF = NpNf <= 256;
-  // rhs rows [TB][RS] | S [4 fields][NQP][16] | source slots [TB]
-  static constexpr size_t floats = (size_t)TB * RS + 4 * NQP * 16 + TB;
+  // rhs rows [TB][RS] | S [4 fields][NQP][16] | source slots [TB] | one scratch word
+  static constexpr size_t floats = (size_t)TB * RS + 4 * NQP * 16 + TB + 1;
 };
 
 template <int N>
@@ -89,6 +89,7 @@ __global__ __launch_bounds__(256, N <= 4 ? 4 : N == 5 ? 3 : 2) void k_update_wad
   float *sR = lds;                         // [TB][RS]
   float *S = sR + TB * RS;                 // [4][NQP][16]
   int *sSlot = (int *)(S + 4 * NQP * 16);  // [TB]
+  float *sScr = (float *)(sSlot + TB);     // written by lanes past a row, never read
   const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int col = lane & 15, kr = lane >> 4;
   const int k0 = blockIdx.x * TB, nk = min(TB, K - k0);
@@ -106,10 +107,13 @@ __global__ __launch_bounds__(256, N <= 4 ? 4 : N == 5 ? 3 : 2) void k_update_wad
   const bool read_res = fa != 0.f;
   float rreg[L::PF ? TB : 1], qreg[L::PF ? TB : 1];
   if constexpr (L::PF) {
+    // every row load is issued before the LDS stores, which are unconditional: lanes past the
+    // row (tid >= NpNf) store to the scratch word.  A store under a lane branch lets the
+    // compiler sink its load into the branch and wait for each row in turn.
+    float hreg[TB];
 #pragma unroll
     for (int e = 0; e < TB; ++e) {
-      const float v = bld(rRh, tid * 4u + e * ROW);
-      if (tid < NpNf) sR[e * RS + tid] = v;
+      hreg[e] = bld(rRh, tid * 4u + e * ROW);
       qreg[e] = bld(rQ, tid * 4u + e * ROW);
     }
     if (read_res) {
@@ -119,6 +123,10 @@ __global__ __launch_bounds__(256, N <= 4 ? 4 : N == 5 ? 3 : 2) void k_update_wad
 #pragma unroll
       for (int e = 0; e < TB; ++e) rreg[e] = 0.f;
     }
+    float *dst = tid < NpNf ? sR + tid : sScr;
+    const int step = tid < NpNf ? RS : 0;
+#pragma unroll
+    for (int e = 0; e < TB; ++e) dst[e * step] = hreg[e];
   } else {
     for (int rr = tid; rr < NpNf; rr += 256)
       for (int e = 0; e < TB; ++e) sR[e * RS + rr] = bld(rRh, rr * 4u + e * ROW);
@@ -297,25 +305,39 @@ struct PlainLayout {
   static constexpr int Np = Dims<N>::Np, Nfp = Dims<N>::Nfp, NfpNf = Dims<N>::NfpNf, TB = 16;
   static constexpr int NpNf = 4 * Np, QT = TB * NpNf;
   static constexpr unsigned QB = QT * 4u, QIT = (QB + 4095) / 4096;
+  static constexpr int QTP = (int)QIT * 1024;  // the Q tile padded to whole 4 KB rounds
   static constexpr int GS = ((Np + 63) / 64) * 64, NG = 256 / GS, EPTV = (TB + NG - 1) / NG;
   static constexpr int NG3 = 256 / NfpNf, EPT3 = (TB + NG3 - 1) / NG3;
   static constexpr int cmax(int a, int b) { return a > b ? a : b; }
-  // volume: Q tile [TB][4Np] | (p, Ur, Us, Ut) [TB][Np] float4 | vgeo [TB][9]
-  static constexpr size_t vfloats = (size_t)QT + QT + TB * 9;
-  // surface: Q tile [TB][4Np] then lifted face data float4 [TB][NfpNf] | phi [TB][NfpNf] | fgeo
-  static constexpr size_t sfloats = (size_t)cmax(QT, 4 * TB * NfpNf) + TB * NfpNf + TB * 16;
+  // volume: Q tile [TB][4Np] (QTP words) | (p, Ur, Us, Ut) [TB][Np] float4 | vgeo [TB][9] in a
+  // 256-word slot
+  static constexpr size_t vfloats = (size_t)QTP + QT + 256;
+  // surface: Q tile [TB][4Np] (QTP words) then lifted face data float4 [TB][NfpNf] | phi
+  // [TB][NfpNf] | fgeo
+  static constexpr size_t sfloats = (size_t)cmax(QTP, 4 * TB * NfpNf) + TB * NfpNf + TB * 16;
+};
+
+// The Q tile in registers (QIT 4 KB rounds of float4 per workgroup); stored to LDS after the
+// kernel's other loads are issued, with unconditional stores (the LDS tile holds whole rounds;
+// words past the tile hold zeros through the range check and are never read).
+template <int N>
+struct PlainQ {
+  float4 v[PlainLayout<N>::QIT];
 };
 
 template <int N>
-__device__ __forceinline__ void plain_q_tile(float *sQ, const float *Q, int k0, int nk, int tid) {
+__device__ __forceinline__ void plain_q_load(PlainQ<N> &t, const float *Q, int k0, int nk, int tid) {
   using L = PlainLayout<N>;
   const brsrc rQ = mkrsrc(Q + (size_t)k0 * L::NpNf, (unsigned)nk * L::NpNf * 4u);
+#pragma unroll
+  for (unsigned it = 0; it < L::QIT; ++it) t.v[it] = bld4(rQ, tid * 16u + it * 4096u);
+}
+
+template <int N>
+__device__ __forceinline__ void plain_q_store(const PlainQ<N> &t, float *sQ, int tid) {
   float4 *sQ4 = (float4 *)sQ;
 #pragma unroll
-  for (unsigned it = 0; it < L::QIT; ++it) {
-    const unsigned off = tid * 16u + it * 4096u;
-    if (L::QIT * 4096u == L::QB || off < L::QB) sQ4[off / 16] = bld4(rQ, off);
-  }
+  for (unsigned it = 0; it < PlainLayout<N>::QIT; ++it) sQ4[tid + it * 256u] = t.v[it];
 }
 
 template <int N>
@@ -327,12 +349,17 @@ __global__ __launch_bounds__(256) void k_volume_t(int K, const float *__restrict
   constexpr int Np = L::Np, NpNf = L::NpNf, TB = L::TB;
   extern __shared__ float lds[];
   float *sQ = lds;
-  float4 *sV = (float4 *)(lds + L::QT);
-  float *sG = lds + 2 * L::QT;
+  float4 *sV = (float4 *)(lds + L::QTP);
+  float *sG = lds + L::QTP + L::QT;
   const int tid = threadIdx.x;
   const int k0 = blockIdx.x * TB, nk = min(TB, K - k0);
-  plain_q_tile<N>(sQ, Q, k0, nk, tid);
-  if (tid < TB * 9) sG[tid] = bld(mkrsrc(vgeo + (size_t)k0 * 9, (unsigned)nk * 36u), tid * 4u);
+  {
+    PlainQ<N> qt;
+    plain_q_load<N>(qt, Q, k0, nk, tid);
+    const float gval = bld(mkrsrc(vgeo + (size_t)k0 * 9, (unsigned)nk * 36u), tid * 4u);
+    plain_q_store<N>(qt, sQ, tid);
+    sG[tid] = gval;
+  }
   __syncthreads();
   const int gv = tid / L::GS, nv = tid - gv * L::GS;
   const bool onv = gv < L::NG && nv < Np;
@@ -402,15 +429,17 @@ __global__ __launch_bounds__(256) void k_surface_t(int K, const float *__restric
   extern __shared__ float lds[];
   float *sQ = lds;                                        // [TB][4Np]       (phases 1-2)
   float4 *sL = (float4 *)lds;                             // [TB][NfpNf]     (phases 3-4)
-  float *sPhi = lds + L::cmax(L::QT, 4 * TB * NfpNf);     // [TB][NfpNf]
+  float *sPhi = lds + L::cmax(L::QTP, 4 * TB * NfpNf);    // [TB][NfpNf]
   float *sF = sPhi + TB * NfpNf;                          // [TB][16]
   const int tid = threadIdx.x;
   const int k0 = blockIdx.x * TB, nk = min(TB, K - k0);
   const brsrc rO = mkrsrc(rhs + (size_t)k0 * NpNf, (unsigned)nk * NpNf * 4u);
   const brsrc rT = mkrsrc(Q, (unsigned)K * NpNf * 4u);
-  // ---- 1: Q tile, fgeo; the thread's vmapP entries, neighbour traces and rhs rows
-  plain_q_tile<N>(sQ, Q, k0, nk, tid);
-  sF[tid] = bld(mkrsrc(fgeo + (size_t)k0 * 16, (unsigned)nk * 64u), tid * 4u);
+  // ---- 1: Q tile, fgeo; the thread's vmapP entries, neighbour traces and rhs rows (all loads
+  // issued before the LDS stores)
+  PlainQ<N> qt;
+  plain_q_load<N>(qt, Q, k0, nk, tid);
+  const float fval = bld(mkrsrc(fgeo + (size_t)k0 * 16, (unsigned)nk * 64u), tid * 4u);
   const int g3 = tid / NfpNf, m3 = tid - g3 * NfpNf;
   const bool on3 = g3 < L::NG3;
   const int f3 = m3 / Nfp;
@@ -430,6 +459,8 @@ __global__ __launch_bounds__(256) void k_surface_t(int K, const float *__restric
   for (int i = 0; i < L::EPTV; ++i)
 #pragma unroll
     for (int f = 0; f < 4; ++f) ro[i][f] = bld(rO, (unsigned)((gv + L::NG * i) * NpNf + f * Np + nv) * 4u);
+  plain_q_store<N>(qt, sQ, tid);
+  sF[tid] = fval;
   __syncthreads();
   // ---- 2: upwind flux phi = (pjump - Unjump) Fscale / 2 (the U flux is -phi)
   if (on3) {
