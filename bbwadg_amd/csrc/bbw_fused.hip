@@ -73,6 +73,36 @@ struct UpdLayout {
   static constexpr size_t floats = (size_t)TB * RS + 4 * NQP * 16 + TB + 1;
 };
 
+// A tile's rows [TB][NW] (row stride ROW bytes in HBM) held in registers between their loads
+// and their LDS stores (rows of stride RS), so the kernel's other loads can be issued in
+// between and every store is unconditional.  Lanes past a row read through an out-of-range
+// offset (0, no memory traffic) and store to a scratch word: a store under a lane branch lets
+// the compiler sink its load into the branch and wait for each round in turn.
+template <int NW, int TB>
+struct TileRows {
+  static constexpr int RR = (NW + 255) / 256;
+  float h[RR][TB];
+  __device__ __forceinline__ void load(brsrc r, unsigned ROW, int tid) {
+#pragma unroll
+    for (int k = 0; k < RR; ++k) {
+      const int rr = tid + 256 * k;
+#pragma unroll
+      for (int e = 0; e < TB; ++e) h[k][e] = bld(r, rr < NW ? (unsigned)rr * 4u + e * ROW : 0x80000000u);
+    }
+  }
+  template <int RS>
+  __device__ __forceinline__ void store(float *sR, float *sScr, int tid) const {
+#pragma unroll
+    for (int k = 0; k < RR; ++k) {
+      const int rr = tid + 256 * k;
+      float *dst = rr < NW ? sR + rr : sScr;
+      const int step = rr < NW ? RS : 0;
+#pragma unroll
+      for (int e = 0; e < TB; ++e) dst[e * step] = h[k][e];
+    }
+  }
+};
+
 template <int N>
 __global__ __launch_bounds__(256, N <= 4 ? 4 : N == 5 ? 3 : 2) void k_update_wadg_mfma(int K, FusedOps o, const float *__restrict__ c2q,
                                                              const float *__restrict__ Jq, float fa,
@@ -104,16 +134,14 @@ __global__ __launch_bounds__(256, N <= 4 ? 4 : N == 5 ? 3 : 2) void k_update_wad
 
   // ---- 1: rhs tile -> LDS; res and Q rows, c2 and 1/J (MFMA output layout), Vq blocks.
   // With fa = 0 (the first LSRK45 stage) res is not read: fa res = 0 for any finite res.
+  // The rhs rows are stored to LDS after every load of the phase is issued.
   const bool read_res = fa != 0.f;
   float rreg[L::PF ? TB : 1], qreg[L::PF ? TB : 1];
+  TileRows<NpNf, TB> rows;
   if constexpr (L::PF) {
-    // every row load is issued before the LDS stores, which are unconditional: lanes past the
-    // row (tid >= NpNf) store to the scratch word.  A store under a lane branch lets the
-    // compiler sink its load into the branch and wait for each row in turn.
-    float hreg[TB];
 #pragma unroll
     for (int e = 0; e < TB; ++e) {
-      hreg[e] = bld(rRh, tid * 4u + e * ROW);
+      rows.h[0][e] = bld(rRh, tid * 4u + e * ROW);
       qreg[e] = bld(rQ, tid * 4u + e * ROW);
     }
     if (read_res) {
@@ -123,13 +151,8 @@ __global__ __launch_bounds__(256, N <= 4 ? 4 : N == 5 ? 3 : 2) void k_update_wad
 #pragma unroll
       for (int e = 0; e < TB; ++e) rreg[e] = 0.f;
     }
-    float *dst = tid < NpNf ? sR + tid : sScr;
-    const int step = tid < NpNf ? RS : 0;
-#pragma unroll
-    for (int e = 0; e < TB; ++e) dst[e * step] = hreg[e];
   } else {
-    for (int rr = tid; rr < NpNf; rr += 256)
-      for (int e = 0; e < TB; ++e) sR[e * RS + rr] = bld(rRh, rr * 4u + e * ROW);
+    rows.load(rRh, ROW, tid);
   }
   constexpr int KPAD = L::RS0 - NpNf;
   static_assert(TB * KPAD <= 256, "one padding word per thread");
@@ -151,6 +174,7 @@ __global__ __launch_bounds__(256, N <= 4 ? 4 : N == 5 ? 3 : 2) void k_update_wad
       for (int kb = 0; kb < KBV; ++kb) aV[t][kb] = bld4_s(rAV, lane * 16u, (unsigned)(qt * KBV + kb) * 1024u);
     }
   }
+  rows.template store<RS>(sR, sScr, tid);
   __syncthreads();
 
   // ---- 2: S_f = (Vq . rhs_f) x c2/J (p) or 1/J (M = cubature points, N = elements, K = nodes)
@@ -603,8 +627,8 @@ struct ElasUpdLayout {
   static constexpr int KBV = (KTV + 3) / 4, KBP = (KTP + 3) / 4;
   static constexpr int RS0 = 9 * Np > 8 * Np + 4 * KTV ? 9 * Np : 8 * Np + 4 * KTV;
   static constexpr int RS = RS0 + ((34 - RS0 % 32) % 32);
-  // rhs rows [TB][RS] | S [3 fields][NQP][16] | source slots [TB]
-  static constexpr size_t floats = (size_t)TB * RS + 3 * NQP * 16 + TB;
+  // rhs rows [TB][RS] | S [3 fields][NQP][16] | source slots [TB] | one scratch word
+  static constexpr size_t floats = (size_t)TB * RS + 3 * NQP * 16 + TB + 1;
 };
 
 template <int N>
@@ -620,6 +644,7 @@ __global__ __launch_bounds__(256, N <= 4 ? 4 : N == 5 ? 3 : 2) void k_update_ela
   float *sR = lds;                         // [TB][RS]
   float *S = sR + TB * RS;                 // [3][NQP][16]
   int *sSlot = (int *)(S + 3 * NQP * 16);  // [TB]
+  float *sScr = (float *)(sSlot + TB);     // written by lanes past a row, never read
   const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int col = lane & 15, kr = lane >> 4;
   const int k0 = blockIdx.x * TB, nk = min(TB, K - k0);
@@ -633,14 +658,10 @@ __global__ __launch_bounds__(256, N <= 4 ? 4 : N == 5 ? 3 : 2) void k_update_ela
   const brsrc rAP = mkrsrc(o.PqBlk, (unsigned)L::NPT * KBP * 1024u);
   const bool has_src = src.nsrc > 0 && k0 + nk - 1 >= src.first && k0 <= src.last;
 
-  // ---- 1: rhs tile -> LDS; materials at the cubature points (MFMA output layout); Vq, Pq blocks
-  for (int rr = tid; rr < NpNf; rr += 256)
-#pragma unroll
-    for (int e = 0; e < TB; ++e) sR[e * RS + rr] = bld(rRh, rr * 4u + e * ROW);
-  constexpr int KPAD = L::RS0 - NpNf;
-  static_assert(TB * KPAD <= 256, "one padding word per thread");
-  if (KPAD > 0 && tid < TB * KPAD) sR[(tid / KPAD) * RS + NpNf + tid % KPAD] = 0.f;
-  if (has_src && tid < TB) sSlot[tid] = tid < nk ? src_slot2(src, k0 + tid) : -1;
+  // ---- 1: rhs rows, materials at the cubature points (MFMA output layout), Vq and Pq blocks;
+  // then the rows -> LDS
+  TileRows<NpNf, TB> rows;
+  rows.load(rRh, ROW, tid);
   float rho[L::QTW][4], lam[L::QTW][4], mu[L::QTW][4];
   f4 aV[L::QTW][KBV], aP[KBP];
   const unsigned co = (unsigned)(col * Nq + 4 * kr) * 4u;
@@ -662,6 +683,11 @@ __global__ __launch_bounds__(256, N <= 4 ? 4 : N == 5 ? 3 : 2) void k_update_ela
 #pragma unroll
     for (int kb = 0; kb < KBP; ++kb) aP[kb] = bld4_s(rAP, lane * 16u, (unsigned)(wave * KBP + kb) * 1024u);
   }
+  rows.template store<RS>(sR, sScr, tid);
+  constexpr int KPAD = L::RS0 - NpNf;
+  static_assert(TB * KPAD <= 256, "one padding word per thread");
+  if (KPAD > 0 && tid < TB * KPAD) sR[(tid / KPAD) * RS + NpNf + tid % KPAD] = 0.f;
+  if (has_src && tid < TB) sSlot[tid] = tid < nk ? src_slot2(src, k0 + tid) : -1;
   __syncthreads();
 
 #pragma unroll 1
