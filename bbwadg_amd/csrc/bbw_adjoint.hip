@@ -344,39 +344,34 @@ __global__ __launch_bounds__(256, N <= 5 ? 4 : 2) void k_adj1(int K, FusedOps o,
   const brsrc rLT = mkrsrc(LiftTBlk, 4u * FMT * KBV * 1024u);
 
   // ---- 1: g = lres + b lQ -> LDS; lres <- a g; fgeo; c2, y, grad in the MFMA output layout.
-  // lres_io bit 0: read lres (else it is 0); bit 1: store a g (else it is dead)
-  sF[tid] = bld(mkrsrc(fgeo + (size_t)k0 * 16, (unsigned)nk * 64u), tid * 4u);
+  // lres_io bit 0: read lres (else it is 0); bit 1: store a g (else it is dead).  Every load of
+  // the phase is issued before the first store; the stores are unconditional (lanes past a row
+  // read and write through an out-of-range offset, which returns 0 and drops the write, and
+  // store to LDS at the scratch word).
+  const float fval = bld(mkrsrc(fgeo + (size_t)k0 * 16, (unsigned)nk * 64u), tid * 4u);
   const bool st_lres = (lres_io & 2) != 0;
-  auto load_g = [&](auto rd) {
-    if constexpr (L::PF) {
-      float gv[TB];
+  const unsigned gof = tid < NpNf ? tid * 4u : 0x80000000u;
+  // the raw rows stay in registers until the stores: g formed right after its loads would make
+  // the later loads of the phase wait for them
+  float lrv[L::PF ? TB : 1], lqv[L::PF ? TB : 1];
+  if constexpr (L::PF) {
+    const unsigned rof = (lres_io & 1) ? gof : 0x80000000u;
 #pragma unroll
-      for (int e = 0; e < TB; ++e) {
-        const float lr = decltype(rd)::value ? bld(rRl, tid * 4u + e * ROW) : 0.f;
-        gv[e] = lr + fb * bld(rQl, tid * 4u + e * ROW);
-      }
-      if (tid < NpNf) {
-#pragma unroll
-        for (int e = 0; e < TB; ++e) sg[e * RS + tid] = gv[e];
-        if (st_lres) {
-#pragma unroll
-          for (int e = 0; e < TB; ++e) bst(rRl, fa * gv[e], tid * 4u + e * ROW);
-        }
-      }
-    } else {
-      for (int rr = tid; rr < NpNf; rr += 256)
-        for (int e = 0; e < TB; ++e) {
-          const float lr = decltype(rd)::value ? bld(rRl, rr * 4u + e * ROW) : 0.f;
-          const float g = lr + fb * bld(rQl, rr * 4u + e * ROW);
-          sg[e * RS + rr] = g;
-          if (st_lres) bst(rRl, fa * g, rr * 4u + e * ROW);
-        }
+    for (int e = 0; e < TB; ++e) {
+      lrv[e] = bld(rRl, rof + e * ROW);
+      lqv[e] = bld(rQl, gof + e * ROW);
     }
-  };
-  if (lres_io & 1) load_g(std::true_type{});
-  else load_g(std::false_type{});
+  } else {
+    for (int rr = tid; rr < NpNf; rr += 256)
+      for (int e = 0; e < TB; ++e) {
+        const float lr = (lres_io & 1) ? bld(rRl, rr * 4u + e * ROW) : 0.f;
+        const float g = lr + fb * bld(rQl, rr * 4u + e * ROW);
+        sg[e * RS + rr] = g;
+        if (st_lres) bst(rRl, fa * g, rr * 4u + e * ROW);
+      }
+  }
   // (A Q)_p rows, c2 and gradient rows of the tile (the last two contiguous in memory and copied
-  // as such): every load issued before the first store
+  // as such), then the operand blocks of phase 2
   float yv[L::NIU], cv[L::NIC], gvv[L::NIC];
 #pragma unroll
   for (int j = 0; j < L::NIU; ++j) yv[j] = bld(rY, (unsigned)(tid + 256 * j) * 4u);
@@ -384,18 +379,6 @@ __global__ __launch_bounds__(256, N <= 5 ? 4 : 2) void k_adj1(int K, FusedOps o,
   for (int j = 0; j < L::NIC; ++j) {
     cv[j] = bld(rC, (unsigned)(tid + 256 * j) * 4u);
     gvv[j] = bld(rGd, (unsigned)(tid + 256 * j) * 4u);
-  }
-#pragma unroll
-  for (int j = 0; j < L::NIU; ++j) {
-    const int i = tid + 256 * j, e = i / Np;
-    float *dst = i < TB * Np ? sU + e * L::RU + (i - e * Np) : sScr;
-    *dst = yv[j];
-  }
-  if (tid < TB * (L::KU - Np)) sU[(tid / (L::KU - Np)) * L::RU + Np + tid % (L::KU - Np)] = 0.f;
-#pragma unroll
-  for (int j = 0; j < L::NIC; ++j) {
-    sC[tid + 256 * j] = cv[j];
-    sGr[tid + 256 * j] = gvv[j];
   }
   f4 aT[L::QTW][KBV], aY[L::QTW][KBV];
 #pragma unroll
@@ -408,6 +391,32 @@ __global__ __launch_bounds__(256, N <= 5 ? 4 : 2) void k_adj1(int K, FusedOps o,
         aY[t][kb] = bld4_s(rVq, lane * 16u, (unsigned)(qt * KBV + kb) * 1024u);
       }
     }
+  }
+  sF[tid] = fval;
+  if constexpr (L::PF) {
+    float gv[TB];
+#pragma unroll
+    for (int e = 0; e < TB; ++e) gv[e] = lrv[e] + fb * lqv[e];
+    float *dst = tid < NpNf ? sg + tid : sScr;
+    const int step = tid < NpNf ? RS : 0;
+#pragma unroll
+    for (int e = 0; e < TB; ++e) dst[e * step] = gv[e];
+    if (st_lres) {
+#pragma unroll
+      for (int e = 0; e < TB; ++e) bst(rRl, fa * gv[e], gof + e * ROW);
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < L::NIU; ++j) {
+    const int i = tid + 256 * j, e = i / Np;
+    float *dst = i < TB * Np ? sU + e * L::RU + (i - e * Np) : sScr;
+    *dst = yv[j];
+  }
+  if (tid < TB * (L::KU - Np)) sU[(tid / (L::KU - Np)) * L::RU + Np + tid % (L::KU - Np)] = 0.f;
+#pragma unroll
+  for (int j = 0; j < L::NIC; ++j) {
+    sC[tid + 256 * j] = cv[j];
+    sGr[tid + 256 * j] = gvv[j];
   }
   __syncthreads();
 
@@ -556,19 +565,22 @@ __global__ __launch_bounds__(256) void k_adj2(int K, const float *__restrict__ v
   const brsrc rHt = mkrsrc(H + (size_t)k0 * NfpNf, (unsigned)nk * NfpNf * 4u);
   const brsrc rHa = mkrsrc(H, (unsigned)K * NfpNf * 4u);
   const brsrc rM = mkrsrc(mapP + (size_t)k0 * NfpNf, (unsigned)nk * NfpNf * 4u);
-  // The tile loads go to registers first and to LDS after the last of them is issued, with
-  // unconditional stores into whole 256-word rounds (words past the tile hold zeros, never read).
+  // The node's face slots first (the mapP gather depends on them, and a wait for them then
+  // covers nothing issued later).  The tile loads go to registers and to LDS after the last of
+  // them is issued, with unconditional stores into whole 256-word rounds (words past the tile
+  // hold zeros, never read).
+  const int gv = tid / L::GS, nv = tid - gv * L::GS;
+  const bool on = gv < L::NG && nv < Np;
+  int slot[3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) slot[a] = on ? FmaskInv[3 * nv + a] : -1;
   float hreg[L::HIT];
 #pragma unroll
   for (int r = 0; r < L::HIT; ++r) hreg[r] = bld(rHt, (unsigned)(tid + 256 * r) * 4u);
   const float gval = bld(mkrsrc(vgeo + (size_t)k0 * 9, (unsigned)nk * 36u), tid * 4u);
   const float fval = bld(mkrsrc(fgeo + (size_t)k0 * 16, (unsigned)nk * 64u), tid * 4u);
-  const int gv = tid / L::GS, nv = tid - gv * L::GS;
-  const bool on = gv < L::NG && nv < Np;
   float zr[L::EPTV][4], lq[L::EPTV][4], hn[L::EPTV][3];
-  int mp[L::EPTV][3], slot[3];
-#pragma unroll
-  for (int a = 0; a < 3; ++a) slot[a] = on ? FmaskInv[3 * nv + a] : -1;
+  int mp[L::EPTV][3];
 #pragma unroll
   for (int i = 0; i < L::EPTV; ++i) {
     const int e = gv + L::NG * i;
