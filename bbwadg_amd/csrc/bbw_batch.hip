@@ -110,7 +110,7 @@ __global__ __launch_bounds__(256, PERSIST ? 3 : 4) void k_fused_batch(
       TileStage<N, false> st;
       stage_load<N, false>(st, m, sh.Qin, k0, nk, tj);
       TilePre<N, false> pf;
-      tile_prefetch<N, false>(pf, m, sh, fa, k0, nk, vm, tj);
+      tile_prefetch<N, false>(pf, m, o, sh, fa, k0, nk, vm, tj);
       stage_store<N, false>(st, lds + L::OQ, lds + L::OG, tj);
       __syncthreads();
       tile_shot<N, false>(m, o, sh, src, fa, fb, k0, nk, lds, lds + L::OQ, lds + L::OG, vm, pf, fid3, tj,

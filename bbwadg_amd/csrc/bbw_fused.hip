@@ -48,7 +48,7 @@ __global__ __launch_bounds__(256, BBW_FUSED_WG_PER_CU) void k_fused_mfma(FusedMe
   TileStage<N, PROJ_U> st;
   stage_load<N, PROJ_U>(st, m, sh.Qin, k0, nk, tid);
   TilePre<N, PROJ_U> pf;
-  tile_prefetch<N, PROJ_U>(pf, m, sh, fa, k0, nk, vm, tid);
+  tile_prefetch<N, PROJ_U>(pf, m, o, sh, fa, k0, nk, vm, tid);
   stage_store<N, PROJ_U>(st, lds + L::OQ, lds + L::OG, tid);
   __syncthreads();
   tile_shot<N, PROJ_U>(m, o, sh, src, fa, fb, k0, nk, lds, lds + L::OQ, lds + L::OG, vm, pf,

@@ -193,11 +193,15 @@ struct NoHook {
   __device__ void operator()() const {}
 };
 
-// The register-resident inputs of a tile's stage (phase 1): res rows, c2 in the MFMA output
-// layout and the neighbour traces of the thread's face nodes.  Issued before the tile's Q_in
-// rows are stored to LDS, so the vmapP -> trace chain overlaps the tile loads.
+// The register-resident inputs of a tile's stage (phase 1): the thread's rows of the volume
+// shift/elevation tables, res rows, c2 in the MFMA output layout and the neighbour traces of
+// the thread's face nodes.  Issued before the tile's Q_in rows are stored to LDS, so the
+// vmapP -> trace chain overlaps the tile loads and no phase waits on a table load after its
+// barrier.
 template <int N, bool PROJ_U>
 struct TilePre {
+  int4 S, E;
+  float4 ev;
   float rreg[FL<N, PROJ_U>::PF ? FL<N, PROJ_U>::TB : 1];
   float c2r[FL<N, PROJ_U>::QTW][4];
   float tr[FL<N, PROJ_U>::EPT3][4];
@@ -206,13 +210,21 @@ struct TilePre {
 // Issues the loads of TilePre.  With fa = 0 (the first LSRK45 stage) res is not read: fa res = 0
 // for any finite res.  Requires K * 4 * Np * 4 < 2^32 (32-bit trace offsets).
 template <int N, bool PROJ_U>
-__device__ __forceinline__ void tile_prefetch(TilePre<N, PROJ_U> &pf, const FusedMesh &m,
+__device__ __forceinline__ void tile_prefetch(TilePre<N, PROJ_U> &pf, const FusedMesh &m, const FusedOps &o,
                                               const FusedShot &sh, float fa, int k0, int nk,
                                               const int (&vm)[FL<N, PROJ_U>::EPT3], const int tid) {
   using L = FusedLayout<N, PROJ_U>;
   constexpr int Np = L::Np, Nq = L::Nq, TB = L::TB;
   constexpr unsigned ROW = 4 * Np * 4u;
   const int lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  {
+    // rows of the thread's degree-(N-1) node (phase 3) and node (phase 3b); lanes past them
+    // read row 0 and do not use it
+    const int n1 = tid % L::GS1, nv = tid % L::GS;
+    pf.S = o.Sids[n1 < L::Np1 ? n1 : 0];
+    pf.E = o.Eids[nv < Np ? nv : 0];
+    pf.ev = o.Evals[nv < Np ? nv : 0];
+  }
   if constexpr (L::PF) {
     const brsrc rR = mkrsrc(sh.res + (size_t)k0 * 4 * Np, (unsigned)nk * ROW);
     if (fa != 0.f) {
@@ -335,7 +347,7 @@ __device__ __forceinline__ void tile_shot(const FusedMesh &m, const FusedOps &o,
   {
     const int g1 = tid / L::GS1, n1 = tid - g1 * L::GS1;
     if (n1 < L::Np1) {
-      const int4 S = o.Sids[n1];
+      const int4 S = pf.S;
 #pragma unroll
       for (int i = 0; i < L::EPT1; ++i) {
         const int e = g1 + L::NG1 * i;
@@ -358,8 +370,8 @@ __device__ __forceinline__ void tile_shot(const FusedMesh &m, const FusedOps &o,
   // ---- 3b: volume step 2 (elevation) into registers: vol[a] = sum_j a_j F[a - e_j]
   float vol[L::EPTV][4];
   if (onv) {
-    const int4 E = o.Eids[nv];
-    const float4 ev = o.Evals[nv];
+    const int4 E = pf.E;
+    const float4 ev = pf.ev;
     const float4 *Fb = sF1 + gv * L::Np1;
 #pragma unroll
     for (int i = 0; i < L::EPTV; ++i) {
