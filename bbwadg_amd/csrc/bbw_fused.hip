@@ -377,6 +377,13 @@ __global__ __launch_bounds__(256) void k_volume_t(int K, const float *__restrict
   float *sG = lds + L::QTP + L::QT;
   const int tid = threadIdx.x;
   const int k0 = blockIdx.x * TB, nk = min(TB, K - k0);
+  const int gv = tid / L::GS, nv = tid - gv * L::GS;
+  const bool onv = gv < L::NG && nv < Np;
+  // the node's derivative rows are loaded with the tile, not after the barriers (lanes past
+  // the nodes read row 0 and return before using it)
+  const int nr = nv < Np ? nv : 0;
+  const float4 dv = Dvals[nr];
+  const int4 D1 = Dids[nr], D2 = Dids[Np + nr], D3 = Dids[2 * Np + nr], D4 = Dids[3 * Np + nr];
   {
     PlainQ<N> qt;
     plain_q_load<N>(qt, Q, k0, nk, tid);
@@ -385,8 +392,6 @@ __global__ __launch_bounds__(256) void k_volume_t(int K, const float *__restrict
     sG[tid] = gval;
   }
   __syncthreads();
-  const int gv = tid / L::GS, nv = tid - gv * L::GS;
-  const bool onv = gv < L::NG && nv < Np;
   if (onv) {
 #pragma unroll
     for (int i = 0; i < L::EPTV; ++i) {
@@ -401,8 +406,6 @@ __global__ __launch_bounds__(256) void k_volume_t(int K, const float *__restrict
   __syncthreads();
   if (!onv) return;
   const brsrc rO = mkrsrc(rhs + (size_t)k0 * NpNf, (unsigned)nk * NpNf * 4u);
-  const float4 dv = Dvals[nv];
-  const int4 D1 = Dids[nv], D2 = Dids[Np + nv], D3 = Dids[2 * Np + nv], D4 = Dids[3 * Np + nv];
   const float4 *Vb = sV + gv * Np;
   const float4 *pa1 = Vb + D1.x, *pa2 = Vb + D2.x, *pa3 = Vb + D3.x, *pa4 = Vb + D4.x;
   const float4 *pb1 = Vb + D1.y, *pb2 = Vb + D2.y, *pb3 = Vb + D3.y, *pb4 = Vb + D4.y;
@@ -1072,6 +1075,13 @@ __global__ __launch_bounds__(256) void k_volume_elas_t(int K, const float *__res
   float *sG = lds + L::QTP;                   // [TB][9] in a 256-word slot
   const int tid = threadIdx.x;
   const int k0 = blockIdx.x * TB, nk = min(TB, K - k0);
+  const int gv = tid / L::GS, nv = tid - gv * L::GS;
+  const bool onv = gv < L::NG && nv < Np;
+  // the node's derivative rows are loaded with the tile, not after the barriers (lanes past
+  // the nodes read row 0 and return before using it)
+  const int nr = nv < Np ? nv : 0;
+  const float4 dv = Dvals[nr];
+  const int4 D1 = Dids[nr], D2 = Dids[Np + nr], D3 = Dids[2 * Np + nr], D4 = Dids[3 * Np + nr];
   {
     // every load before the LDS stores, which are unconditional (whole rounds; words past the
     // tile and past vgeo's TB x 9 are written but never read)
@@ -1086,8 +1096,6 @@ __global__ __launch_bounds__(256) void k_volume_elas_t(int K, const float *__res
     sG[tid] = gv0;
   }
   __syncthreads();
-  const int gv = tid / L::GS, nv = tid - gv * L::GS;
-  const bool onv = gv < L::NG && nv < Np;
   float qv[L::EPTV][9];
   if (onv) {
 #pragma unroll
@@ -1108,10 +1116,8 @@ __global__ __launch_bounds__(256) void k_volume_elas_t(int K, const float *__res
   __syncthreads();
   if (!onv) return;
   const brsrc rO = mkrsrc(rhs + (size_t)k0 * RowF, (unsigned)nk * RowF * 4u);
-  const float4 dv = Dvals[nv];
   int id[4][4];
   {
-    const int4 D1 = Dids[nv], D2 = Dids[Np + nv], D3 = Dids[2 * Np + nv], D4 = Dids[3 * Np + nv];
     // id[m][j]: column of operator m (1..4), non-zero j
     id[0][0] = D1.x; id[0][1] = D2.x; id[0][2] = D3.x; id[0][3] = D4.x;
     id[1][0] = D1.y; id[1][1] = D2.y; id[1][2] = D3.y; id[1][3] = D4.y;
