@@ -103,6 +103,48 @@ struct TileRows {
   }
 };
 
+// LSRK stage over a tile whose rhs rows [TB][NW] sit in LDS (stride RS): res = fa res +
+// fdt (rhs + source), Q += fb res, for rows of NW words (fields of Np nodes) in HBM with row
+// stride ROW bytes.  Every res and Q load of the thread is issued before the first store; lanes
+// past a row use an out-of-range offset (the loads read 0, the stores are dropped).  wfield(f)
+// is the source weight of field f.
+template <int NW, int RS, int TB, int Np, class WF>
+__device__ __forceinline__ void lsrk_rows(brsrc rRs, brsrc rQ, unsigned ROW, const float *sR,
+                                          const int *sSlot, bool has_src, const SrcArgs &src,
+                                          float ft, float fa, float fb, float fdt, int tid, WF &&wfield) {
+  constexpr int RR = (NW + 255) / 256;
+  float rv[RR][TB], qv[RR][TB];
+#pragma unroll
+  for (int k = 0; k < RR; ++k) {
+    const int rr = tid + 256 * k;
+#pragma unroll
+    for (int e = 0; e < TB; ++e) {
+      const unsigned g = rr < NW ? (unsigned)rr * 4u + e * ROW : 0x80000000u;
+      rv[k][e] = bld(rRs, g);
+      qv[k][e] = bld(rQ, g);
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < RR; ++k) {
+    const int rr = tid + 256 * k;
+    const int rc = rr < NW ? rr : 0;   // LDS reads of lanes past the row stay in the tile
+    const int fr = rc / Np, nr = rc - fr * Np;
+    const float wsel = wfield(fr);
+#pragma unroll
+    for (int e = 0; e < TB; ++e) {
+      float v = sR[e * RS + rc];
+      if (has_src) {
+        const int slot = sSlot[e];
+        if (slot >= 0) v += wsel * ft * src.coef[(size_t)slot * Np + nr];
+      }
+      const unsigned g = rr < NW ? (unsigned)rr * 4u + e * ROW : 0x80000000u;
+      const float x = fa * rv[k][e] + fdt * v;
+      bst(rRs, x, g);
+      bst(rQ, qv[k][e] + fb * x, g);
+    }
+  }
+}
+
 template <int N>
 __global__ __launch_bounds__(256, N <= 4 ? 4 : N == 5 ? 3 : 2) void k_update_wadg_mfma(int K, FusedOps o, const float *__restrict__ c2q,
                                                              const float *__restrict__ Jq, float fa,
@@ -266,20 +308,8 @@ __global__ __launch_bounds__(256, N <= 4 ? 4 : N == 5 ? 3 : 2) void k_update_wad
       }
     }
   } else {
-    for (int rr = tid; rr < NpNf; rr += 256) {
-      const int fr = rr / Np, nr = rr - fr * Np;
-      for (int e = 0; e < nk; ++e) {
-        float v = sR[e * RS + rr];
-        if (has_src) {
-          const int slot = sSlot[e];
-          if (slot >= 0) v += w4(src, fr) * ft * src.coef[(size_t)slot * Np + nr];
-        }
-        const unsigned g = rr * 4u + e * ROW;
-        const float x = fa * bld(rRs, g) + fdt * v;
-        bst(rRs, x, g);
-        bst(rQ, bld(rQ, g) + fb * x, g);
-      }
-    }
+    lsrk_rows<NpNf, RS, TB, Np>(rRs, rQ, ROW, sR, sSlot, has_src, src, ft, fa, fb, fdt, tid,
+                                [&](int f) { return w4(src, f); });
   }
 }
 
@@ -781,24 +811,12 @@ __global__ __launch_bounds__(256, N <= 4 ? 4 : N == 5 ? 3 : 2) void k_update_ela
 
   // ---- 4: LSRK stage: res = fa res + fdt (rhs + source); Q += fb res
   const float ft = ftime_dev ? *ftime_dev : ftime;
-  for (int rr = tid; rr < NpNf; rr += 256) {
-    const int fr = rr / Np, nr = rr - fr * Np;
+  lsrk_rows<NpNf, RS, TB, Np>(rRs, rQ, ROW, sR, sSlot, has_src, src, ft, fa, fb, fdt, tid, [&](int fr) {
     float wsel = 0.f;
 #pragma unroll
     for (int f = 0; f < 9; ++f) wsel = fr == f ? src.w[f] : wsel;
-#pragma unroll
-    for (int e = 0; e < TB; ++e) {
-      float v = sR[e * RS + rr];
-      if (has_src) {
-        const int slot = sSlot[e];
-        if (slot >= 0) v += wsel * ft * src.coef[(size_t)slot * Np + nr];
-      }
-      const unsigned gi = rr * 4u + e * ROW;
-      const float x = fa * bld(rRs, gi) + fdt * v;
-      bst(rRs, x, gi);
-      bst(rQ, bld(rQ, gi) + fb * x, gi);
-    }
-  }
+    return wsel;
+  });
 }
 
 template <int N>
