@@ -857,9 +857,10 @@ hipError_t launch_update_elas_mfma(bbw_ctx *c, float fa, float fb, float fdt, co
 }
 
 // Elastic surface kernel on 16-element tiles (rk_surface_elas,
-// elas_cpp/okl/ElasKernelsWADG.okl:128-284, with the BB lift): central flux with tau_v / tau_s
-// penalties (Q+ = 0 on the boundary) for the 9 fields in registers, then L0 per face and EEL
-// for three fields at a time, accumulated into rhs.
+// elas_cpp/okl/ElasKernelsWADG.okl:128-284, whose lift is a dense Np x 4Nfp product): central
+// flux with tau_v / tau_s penalties (Q+ = 0 on the boundary) for the 9 fields in registers,
+// then, three fields at a time, the BB lift LIFT = EEL blkdiag(L0) on MFMA as one product per
+// face (M = nodes, N = 16 elements, K = face nodes), accumulated into rhs.
 template <int N>
 struct ElasSurfLayout {
   static constexpr int Np = Dims<N>::Np, Nfp = Dims<N>::Nfp, NfpNf = Dims<N>::NfpNf, TB = 16;
@@ -867,29 +868,31 @@ struct ElasSurfLayout {
   static constexpr unsigned QB = QT * 4u, QIT = (QB + 4095) / 4096;
   static constexpr int NG3 = 256 / NfpNf, EPT3 = (TB + NG3 - 1) / NG3;
   static constexpr int GS = ((Np + 63) / 64) * 64, NG = 256 / GS, EPTV = (TB + NG - 1) / NG;
-  static constexpr int GW = 3 * TB * NfpNf;      // one field group's face data
-  static constexpr int QTP = (int)QIT * 1024;   // the Q tile padded to whole 4 KB rounds
-  static constexpr int W = QTP > 2 * GW ? QTP : 2 * GW;
+  static constexpr int KF = 4 * ((Nfp + 3) / 4), KTL = KF / 4, KBL = (KTL + 3) / 4;
+  static constexpr int NPT = (Np + 15) / 16;
+  static constexpr int KB = NfpNf + 4, BS = TB + 1;  // flux rows (zero tail) and row stride
+  static constexpr int GB = 3 * KB * BS;             // a group's flux, B operand layout
+  static constexpr int GO = 3 * TB * Np;             // a group's lifted values [3][TB][Np]
+  static constexpr int QTP = (int)QIT * 1024;        // the Q tile padded to whole 4 KB rounds
+  static constexpr int W = QTP > GB + GO ? QTP : GB + GO;
   static constexpr size_t floats = (size_t)W + TB * 16;
+  static_assert(3 * (KB - NfpNf) * BS <= 256, "one tail word per thread");
 };
 
 template <int N>
 __global__ __launch_bounds__(256) void k_surface_elas_t(int K, const float *__restrict__ fgeo,
                                                         const int *__restrict__ Fmask,
                                                         const int *__restrict__ vmapP,
-                                                        const int *__restrict__ EEL_ids,
-                                                        const float *__restrict__ EEL_vals,
-                                                        const int *__restrict__ L0_ids,
-                                                        const float *__restrict__ L0_vals,
+                                                        const float *__restrict__ LiftBlk,
                                                         float tau_v, float tau_s,
                                                         const float *__restrict__ Q, float *__restrict__ rhs) {
   using L = ElasSurfLayout<N>;
   constexpr int Np = L::Np, Nfp = L::Nfp, NfpNf = L::NfpNf, TB = L::TB, RowF = L::RowF;
-  constexpr int EEL_nnz = Dims<N>::EEL_nnz, L0_nnz = Dims<N>::L0_nnz;
+  constexpr int KB = L::KB, BS = L::BS, KTL = L::KTL, KBL = L::KBL, NPT = L::NPT;
   extern __shared__ float lds[];
-  float *sQ = lds;                 // [TB][9][Np]            (phases 1-2)
-  float *sFl = lds;                // [TB][3][NfpNf] flux    (group phases)
-  float *sT = lds + L::GW;         // [TB][3][NfpNf] L0 flux (group phases)
+  float *sQ = lds;                 // [TB][9][Np]                  (phases 1-2)
+  float *sB = lds;                 // [3][KB][BS] group flux       (group phases)
+  float *sO = lds + L::GB;         // [3][TB][Np] lifted values    (group phases)
   float *sF = lds + L::W;          // [TB][16]
   const int tid = threadIdx.x;
   const int k0 = blockIdx.x * TB, nk = min(TB, K - k0);
@@ -967,56 +970,74 @@ __global__ __launch_bounds__(256) void k_surface_elas_t(int K, const float *__re
   __syncthreads();
   const int gv = tid / L::GS, nv = tid - gv * L::GS;
   const bool onv = gv < L::NG && nv < Np;
-  const int nt = m3 - f3 * Nfp;
+  const int lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int col = lane & 15, kr = lane >> 4;
+  // lane-ordered LIFT_f blocks of the wave's first node tile (every group uses them)
+  const brsrc rAL = mkrsrc(LiftBlk, 4u * NPT * KBL * 1024u);
+  f4 aL[4][KBL];
+  auto load_lift = [&](int mt) {
+#pragma unroll
+    for (int f = 0; f < 4; ++f)
+#pragma unroll
+      for (int b = 0; b < KBL; ++b) aL[f][b] = bld4_s(rAL, lane * 16u, (unsigned)((f * NPT + mt) * KBL + b) * 1024u);
+  };
+  if (wave < NPT) load_lift(wave);
+  // unrolled: the group selects fl[.][3 g + b] at compile time (a run-time index would move
+  // the flux registers to scratch memory)
 #pragma unroll
   for (int g = 0; g < 3; ++g) {
-    // ---- 3: the group's flux -> LDS
+    // ---- 3: the group's flux -> LDS as MFMA B operands (rows = face nodes, columns =
+    // elements), with zero rows past the last face
     if (on3) {
 #pragma unroll
       for (int i = 0; i < L::EPT3; ++i) {
         const int e = g3 + L::NG3 * i;
         if (L::NG3 * L::EPT3 == TB || e < TB) {
 #pragma unroll
-          for (int b = 0; b < 3; ++b) sFl[(e * 3 + b) * NfpNf + m3] = fl[i][3 * g + b];
+          for (int b = 0; b < 3; ++b) sB[(b * KB + m3) * BS + e] = fl[i][3 * g + b];
         }
       }
     }
+    if (tid < 3 * (KB - NfpNf) * BS) {
+      const int b = tid / ((KB - NfpNf) * BS), j = tid - b * ((KB - NfpNf) * BS);
+      sB[(b * KB + NfpNf) * BS + j] = 0.f;
+    }
     __syncthreads();
-    // ---- 4: L0 per face
-    if (on3) {
-      float Lv[L0_nnz];
-      const float *Xp[L0_nnz];
-      const float *Xb = sFl + g3 * 3 * NfpNf + f3 * Nfp;
+    // ---- 4: lift on MFMA: rhs_c += sum_f LIFT_f flux_{c,f} for the group's three fields c
+    {
+      const float *Bp = sB + kr * BS + col;
+      for (int mt = wave; mt < NPT; mt += 4) {
+        // the registers hold the wave's first tile, except after a wave's second tile
+        if (mt != wave || (g > 0 && NPT > wave + 4)) load_lift(mt);
+        f4 acc[3];
 #pragma unroll
-      for (int j = 0; j < L0_nnz; ++j) {
-        Lv[j] = L0_vals[nt + j * Nfp];
-        Xp[j] = Xb + L0_ids[nt + j * Nfp];
-      }
+        for (int b = 0; b < 3; ++b) {
+          acc[b] = f4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-      for (int i = 0; i < L::EPT3; ++i) {
-        const int e = g3 + L::NG3 * i;
-        if (L::NG3 * L::EPT3 == TB || e < TB) {
+          for (int f = 0; f < 4; ++f)
 #pragma unroll
-          for (int b = 0; b < 3; ++b) {
-            float v = 0.f;
+            for (int kb = 0; kb < KBL; ++kb)
 #pragma unroll
-            for (int j = 0; j < L0_nnz; ++j) v += Lv[j] * Xp[j][(i * L::NG3 * 3 + b) * NfpNf];
-            sT[(e * 3 + b) * NfpNf + m3] = v;
+              for (int sb = 0; sb < 4; ++sb) {
+                const int kt = 4 * kb + sb;
+                if (kt < KTL)
+                  acc[b] = __builtin_amdgcn_mfma_f32_16x16x4f32(aL[f][kb][sb], Bp[(b * KB + f * Nfp + 4 * kt) * BS],
+                                                                acc[b], 0, 0, 0);
+              }
+        }
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int n = mt * 16 + 4 * kr + r;
+          if (n < Np) {
+#pragma unroll
+            for (int b = 0; b < 3; ++b) sO[(b * TB + col) * Np + n] = acc[b][r];
           }
         }
       }
     }
     __syncthreads();
-    // ---- 5: EEL rows, accumulated into the group's rhs rows
+    // ---- 5: the group's rhs rows += lifted values (lanes over nodes)
     if (onv) {
-      float Ev[EEL_nnz];
-      const float *Ep[EEL_nnz];
-      const float *Tb = sT + gv * 3 * NfpNf;
-#pragma unroll
-      for (int j = 0; j < EEL_nnz; ++j) {
-        Ev[j] = EEL_vals[nv + j * Np];
-        Ep[j] = Tb + EEL_ids[nv + j * Np];
-      }
       float ro[L::EPTV][3];
 #pragma unroll
       for (int i = 0; i < L::EPTV; ++i)
@@ -1025,13 +1046,10 @@ __global__ __launch_bounds__(256) void k_surface_elas_t(int K, const float *__re
           ro[i][b] = bld(rO, (unsigned)((gv + L::NG * i) * RowF + (3 * g + b) * Np + nv) * 4u);
 #pragma unroll
       for (int i = 0; i < L::EPTV; ++i) {
+        const int e = gv + L::NG * i;
 #pragma unroll
-        for (int b = 0; b < 3; ++b) {
-          float v = 0.f;
-#pragma unroll
-          for (int j = 0; j < EEL_nnz; ++j) v += Ev[j] * Ep[j][(i * L::NG * 3 + b) * NfpNf];
-          bst(rO, ro[i][b] + v, (unsigned)((gv + L::NG * i) * RowF + (3 * g + b) * Np + nv) * 4u);
-        }
+        for (int b = 0; b < 3; ++b)
+          bst(rO, ro[i][b] + sO[(b * TB + e) * Np + nv], (unsigned)(e * RowF + (3 * g + b) * Np + nv) * 4u);
       }
     }
   }
@@ -1047,8 +1065,7 @@ static hipError_t surface_elas_t_n(bbw_ctx *c) {
     if (e != hipSuccess) return e;
   }
   hipLaunchKernelGGL(k_surface_elas_t<N>, dim3((c->K + L::TB - 1) / L::TB), dim3(256), sh, c->stream, c->K,
-                     c->fgeo, c->Fmask, c->vmapP, c->EEL_ids, c->EEL_vals, c->L0_ids, c->L0_vals,
-                     c->tau_v, c->tau_s, c->Q[c->cur], c->rhs);
+                     c->fgeo, c->Fmask, c->vmapP, c->LiftBlk, c->tau_v, c->tau_s, c->Q[c->cur], c->rhs);
   return hipGetLastError();
 }
 
