@@ -9,7 +9,7 @@ error of p and u against the exact standing mode (cpp/main_subelem.cpp:3-5 for p
 u_t = -grad p).  Full-size workloads whose FP64 oracle run would take minutes are checked over
 a few steps at their full size and over a long run on a smaller mesh of the same generator.
 
-    python tests/accuracy_report.py [--json gpurun_out/accuracy.json] [--skip-large]
+    python tests/accuracy_report.py [--json accuracy.json] [--skip-large]
 
 Test infrastructure (it runs the oracle); prints one markdown row per check as it finishes.
 """
