@@ -574,6 +574,13 @@ __global__ __launch_bounds__(256) void k_adj2(int K, const float *__restrict__ v
   int slot[3];
 #pragma unroll
   for (int a = 0; a < 3; ++a) slot[a] = on ? FmaskInv[3 * nv + a] : -1;
+  // the thread's rows of the shift and elevation tables (volume transpose), loaded with the
+  // tile rather than after the barriers; lanes past the nodes read row 0 and do not use it
+  const int g1 = tid / L::GS1, n1 = tid - g1 * L::GS1;
+  const int4 S = Sids[n1 < L::Np1 ? n1 : 0];
+  const float4 wt = Swts[n1 < L::Np1 ? n1 : 0];
+  const int4 E = Eids[nv < Np ? nv : 0];
+  const float4 ev = Evals[nv < Np ? nv : 0];
   float hreg[L::HIT];
 #pragma unroll
   for (int r = 0; r < L::HIT; ++r) hreg[r] = bld(rHt, (unsigned)(tid + 256 * r) * 4u);
@@ -618,10 +625,7 @@ __global__ __launch_bounds__(256) void k_adj2(int K, const float *__restrict__ v
   // step 2 reads it at c - e_m.
   float4 ta[L::EPT1];
   float tb[L::EPT1];
-  const int g1 = tid / L::GS1, n1 = tid - g1 * L::GS1;
   if (n1 < L::Np1) {
-    const int4 S = Sids[n1];
-    const float4 wt = Swts[n1];
 #pragma unroll
     for (int i = 0; i < L::EPT1; ++i) {
       const float4 *V = sZ + (g1 + L::NG1 * i) * Np;
@@ -646,8 +650,6 @@ __global__ __launch_bounds__(256) void k_adj2(int K, const float *__restrict__ v
   }
   __syncthreads();
   if (!on) return;
-  const int4 E = Eids[nv];
-  const float4 ev = Evals[nv];
   const float mx = ev.x > 0.f ? 1.f : 0.f, my = ev.y > 0.f ? 1.f : 0.f;
   const float mz = ev.z > 0.f ? 1.f : 0.f, mw = ev.w > 0.f ? 1.f : 0.f;
 #pragma unroll
