@@ -42,6 +42,7 @@ __global__ __launch_bounds__(256, BBW_FUSED_WG_PER_CU) void k_fused_mfma(FusedMe
   const int tid = threadIdx.x;
   const int k0 = blockIdx.x * L::TB, nk = min(L::TB, m.K - k0);
   if (ftime_dev) sh.ft = *ftime_dev;
+  const int fid3 = face_node<N, PROJ_U>(o, tid);
   // vmapP first: the trace gather that depends on it is issued behind the tile loads
   int vm[L::EPT3];
   vm_load<N, PROJ_U>(vm, m, k0, nk, tid);
@@ -51,8 +52,8 @@ __global__ __launch_bounds__(256, BBW_FUSED_WG_PER_CU) void k_fused_mfma(FusedMe
   tile_prefetch<N, PROJ_U>(pf, m, o, sh, fa, k0, nk, vm, tid);
   stage_store<N, PROJ_U>(st, lds + L::OQ, lds + L::OG, tid);
   __syncthreads();
-  tile_shot<N, PROJ_U>(m, o, sh, src, fa, fb, k0, nk, lds, lds + L::OQ, lds + L::OG, vm, pf,
-                       face_node<N, PROJ_U>(o, tid), tid, NoHook{}, NoHook{}, NoHook{});
+  tile_shot<N, PROJ_U>(m, o, sh, src, fa, fb, k0, nk, lds, lds + L::OQ, lds + L::OG, vm, pf, fid3, tid,
+                       NoHook{}, NoHook{}, NoHook{});
 }
 
 // Plain WADG update on MFMA (the three-launch path's rk_update_WADG,

@@ -185,8 +185,10 @@ __device__ __forceinline__ void vm_load(int *vm, const FusedMesh &m, int k0, int
 template <int N, bool PROJ_U>
 __device__ __forceinline__ int face_node(const FusedOps &o, const int tid) {
   using L = FL<N, PROJ_U>;
+  // unconditional (m3 < NfpNf for every thread; threads past the face-node groups do not use
+  // it), so that the load is issued with the tile's loads and not behind the first barrier
   const int g3 = tid / L::NfpNf, m3 = tid - g3 * L::NfpNf;
-  return g3 < L::NG3 ? o.Fmask[m3] : 0;
+  return o.Fmask[m3];
 }
 
 struct NoHook {
