@@ -381,14 +381,16 @@ __global__ __launch_bounds__(256, N <= 5 ? 4 : 2) void k_adj1(int K, FusedOps o,
     gvv[j] = bld(rGd, (unsigned)(tid + 256 * j) * 4u);
   }
   f4 aT[L::QTW][KBV], aY[L::QTW][KBV];
+  // (under the wave test: loading the last tile unconditionally instead, as the MFMA updates
+  // do, made this kernel slower: 214 against 205 us per stage on 196608 tets, N = 5)
 #pragma unroll
   for (int t = 0; t < L::QTW; ++t) {
     const int qt = wave + 4 * t;
     if (qt < L::NQT) {
 #pragma unroll
       for (int kb = 0; kb < KBV; ++kb) {
-        aT[t][kb] = bld4_s(rPT, lane * 16u, (unsigned)(qt * KBV + kb) * 1024u);
-        aY[t][kb] = bld4_s(rVq, lane * 16u, (unsigned)(qt * KBV + kb) * 1024u);
+        aT[t][kb] = bld_blk<KTV>(rPT, lane * 16u, (unsigned)(qt * KBV + kb) * 1024u, kb);
+        aY[t][kb] = bld_blk<KTV>(rVq, lane * 16u, (unsigned)(qt * KBV + kb) * 1024u, kb);
       }
     }
   }
@@ -423,9 +425,10 @@ __global__ __launch_bounds__(256, N <= 5 ? 4 : 2) void k_adj1(int K, FusedOps o,
   // ---- 2: t = Pq^T g_p and y = Vq (A Q)_p (M = cubature points, N = elements, K = nodes);
   // grad += dt t y; S = c2 t
   f4 aV[KBP];
-  if (wave < L::NPT) {
+  {
+    const int mt0 = wave < L::NPT ? wave : L::NPT - 1;
 #pragma unroll
-    for (int kb = 0; kb < KBP; ++kb) aV[kb] = bld4_s(rVT, lane * 16u, (unsigned)(wave * KBP + kb) * 1024u);
+    for (int kb = 0; kb < KBP; ++kb) aV[kb] = bld_blk<KTP>(rVT, lane * 16u, (unsigned)(mt0 * KBP + kb) * 1024u, kb);
   }
   {
     const float *Bp = sg + col * RS + kr;
@@ -464,14 +467,14 @@ __global__ __launch_bounds__(256, N <= 5 ? 4 : 2) void k_adj1(int K, FusedOps o,
     const int pr = wave + 4 * j;
     if (pr < L::NPR) {
 #pragma unroll
-      for (int kb = 0; kb < KBV; ++kb) aL[j][kb] = bld4_s(rLT, lane * 16u, (unsigned)(pr * KBV + kb) * 1024u);
+      for (int kb = 0; kb < KBV; ++kb) aL[j][kb] = bld_blk<KTV>(rLT, lane * 16u, (unsigned)(pr * KBV + kb) * 1024u, kb);
     }
   }
   for (int i = tid; i < TB * Nq; i += 256) bst(rGd, sGr[i], i * 4u);
   for (int mt = wave; mt < L::NPT; mt += 4) {
     if (mt != wave) {
 #pragma unroll
-      for (int kb = 0; kb < KBP; ++kb) aV[kb] = bld4_s(rVT, lane * 16u, (unsigned)(mt * KBP + kb) * 1024u);
+      for (int kb = 0; kb < KBP; ++kb) aV[kb] = bld_blk<KTP>(rVT, lane * 16u, (unsigned)(mt * KBP + kb) * 1024u, kb);
     }
     f4 acc = {0.f, 0.f, 0.f, 0.f};
     const float *Bp = S + kr * 16 + col;

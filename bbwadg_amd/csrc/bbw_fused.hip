@@ -204,27 +204,29 @@ __global__ __launch_bounds__(256, N <= 4 ? 4 : N == 5 ? 3 : 2) void k_update_wad
   float c2r[L::QTW][4], jr[L::QTW][4];
   f4 aV[L::QTW][KBV];
   const unsigned co = (unsigned)(col * Nq + 4 * kr) * 4u;
+  // unconditional (a wave past the last tile loads the last one and does not use it): loads
+  // under a wave test leave the compiler unsure how many are in flight where the paths join,
+  // and its waits then cover later prefetches as well
 #pragma unroll
   for (int t = 0; t < L::QTW; ++t) {
-    const int qt = wave + 4 * t;
-    if (qt < L::NQT) {
+    const int qt = wave + 4 * t < L::NQT ? wave + 4 * t : L::NQT - 1;
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        c2r[t][r] = bld(rC, co + (qt * 16 + r) * 4u);
-        jr[t][r] = Jq ? 1.f / bld(rJ, co + (qt * 16 + r) * 4u) : 1.f;
-      }
-#pragma unroll
-      for (int kb = 0; kb < KBV; ++kb) aV[t][kb] = bld4_s(rAV, lane * 16u, (unsigned)(qt * KBV + kb) * 1024u);
+    for (int r = 0; r < 4; ++r) {
+      c2r[t][r] = bld(rC, co + (qt * 16 + r) * 4u);
+      jr[t][r] = Jq ? 1.f / bld(rJ, co + (qt * 16 + r) * 4u) : 1.f;
     }
+#pragma unroll
+    for (int kb = 0; kb < KBV; ++kb) aV[t][kb] = bld_blk<KTV>(rAV, lane * 16u, (unsigned)(qt * KBV + kb) * 1024u, kb);
   }
   rows.template store<RS>(sR, sScr, tid);
   __syncthreads();
 
   // ---- 2: S_f = (Vq . rhs_f) x c2/J (p) or 1/J (M = cubature points, N = elements, K = nodes)
   f4 aP[KBP];
-  if (wave < L::NPT) {
+  {
+    const int mt0 = wave < L::NPT ? wave : L::NPT - 1;
 #pragma unroll
-    for (int kb = 0; kb < KBP; ++kb) aP[kb] = bld4_s(rAP, lane * 16u, (unsigned)(wave * KBP + kb) * 1024u);
+    for (int kb = 0; kb < KBP; ++kb) aP[kb] = bld_blk<KTP>(rAP, lane * 16u, (unsigned)(mt0 * KBP + kb) * 1024u, kb);
   }
   {
     const float *Bp = sR + col * RS + kr;
@@ -262,7 +264,7 @@ __global__ __launch_bounds__(256, N <= 4 ? 4 : N == 5 ? 3 : 2) void k_update_wad
   for (int mt = wave; mt < L::NPT; mt += 4) {
     if (mt != wave) {
 #pragma unroll
-      for (int kb = 0; kb < KBP; ++kb) aP[kb] = bld4_s(rAP, lane * 16u, (unsigned)(mt * KBP + kb) * 1024u);
+      for (int kb = 0; kb < KBP; ++kb) aP[kb] = bld_blk<KTP>(rAP, lane * 16u, (unsigned)(mt * KBP + kb) * 1024u, kb);
     }
     f4 acc[4];
 #pragma unroll
@@ -699,23 +701,23 @@ __global__ __launch_bounds__(256, N <= 4 ? 4 : N == 5 ? 3 : 2) void k_update_ela
   float rho[L::QTW][4], lam[L::QTW][4], mu[L::QTW][4];
   f4 aV[L::QTW][KBV], aP[KBP];
   const unsigned co = (unsigned)(col * Nq + 4 * kr) * 4u;
+  // unconditional, clamped to the last tile (as in the acoustic update)
 #pragma unroll
   for (int t = 0; t < L::QTW; ++t) {
-    const int qt = wave + 4 * t;
-    if (qt < L::NQT) {
+    const int qt = wave + 4 * t < L::NQT ? wave + 4 * t : L::NQT - 1;
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        rho[t][r] = bld(rRho, co + (qt * 16 + r) * 4u);
-        lam[t][r] = bld(rLam, co + (qt * 16 + r) * 4u);
-        mu[t][r] = bld(rMu, co + (qt * 16 + r) * 4u);
-      }
-#pragma unroll
-      for (int kb = 0; kb < KBV; ++kb) aV[t][kb] = bld4_s(rAV, lane * 16u, (unsigned)(qt * KBV + kb) * 1024u);
+    for (int r = 0; r < 4; ++r) {
+      rho[t][r] = bld(rRho, co + (qt * 16 + r) * 4u);
+      lam[t][r] = bld(rLam, co + (qt * 16 + r) * 4u);
+      mu[t][r] = bld(rMu, co + (qt * 16 + r) * 4u);
     }
-  }
-  if (wave < L::NPT) {
 #pragma unroll
-    for (int kb = 0; kb < KBP; ++kb) aP[kb] = bld4_s(rAP, lane * 16u, (unsigned)(wave * KBP + kb) * 1024u);
+    for (int kb = 0; kb < KBV; ++kb) aV[t][kb] = bld_blk<KTV>(rAV, lane * 16u, (unsigned)(qt * KBV + kb) * 1024u, kb);
+  }
+  {
+    const int mt0 = wave < L::NPT ? wave : L::NPT - 1;
+#pragma unroll
+    for (int kb = 0; kb < KBP; ++kb) aP[kb] = bld_blk<KTP>(rAP, lane * 16u, (unsigned)(mt0 * KBP + kb) * 1024u, kb);
   }
   rows.template store<RS>(sR, sScr, tid);
   constexpr int KPAD = L::RS0 - NpNf;
@@ -777,7 +779,7 @@ __global__ __launch_bounds__(256, N <= 4 ? 4 : N == 5 ? 3 : 2) void k_update_ela
     for (int mt = wave; mt < L::NPT; mt += 4) {
       if (mt != wave) {
 #pragma unroll
-        for (int kb = 0; kb < KBP; ++kb) aP[kb] = bld4_s(rAP, lane * 16u, (unsigned)(mt * KBP + kb) * 1024u);
+        for (int kb = 0; kb < KBP; ++kb) aP[kb] = bld_blk<KTP>(rAP, lane * 16u, (unsigned)(mt * KBP + kb) * 1024u, kb);
       }
       f4 acc[3];
 #pragma unroll
@@ -804,7 +806,7 @@ __global__ __launch_bounds__(256, N <= 4 ? 4 : N == 5 ? 3 : 2) void k_update_ela
       }
       if (mt != wave && wave < L::NPT) {
 #pragma unroll
-        for (int kb = 0; kb < KBP; ++kb) aP[kb] = bld4_s(rAP, lane * 16u, (unsigned)(wave * KBP + kb) * 1024u);
+        for (int kb = 0; kb < KBP; ++kb) aP[kb] = bld_blk<KTP>(rAP, lane * 16u, (unsigned)(wave * KBP + kb) * 1024u, kb);
       }
     }
     __syncthreads();
@@ -980,9 +982,9 @@ __global__ __launch_bounds__(256) void k_surface_elas_t(int K, const float *__re
 #pragma unroll
     for (int f = 0; f < 4; ++f)
 #pragma unroll
-      for (int b = 0; b < KBL; ++b) aL[f][b] = bld4_s(rAL, lane * 16u, (unsigned)((f * NPT + mt) * KBL + b) * 1024u);
+      for (int b = 0; b < KBL; ++b) aL[f][b] = bld_blk<KTL>(rAL, lane * 16u, (unsigned)((f * NPT + mt) * KBL + b) * 1024u, b);
   };
-  if (wave < NPT) load_lift(wave);
+  load_lift(wave < NPT ? wave : NPT - 1);   // unconditional: see the MFMA updates
   // unrolled: the group selects fl[.][3 g + b] at compile time (a run-time index would move
   // the flux registers to scratch memory)
 #pragma unroll

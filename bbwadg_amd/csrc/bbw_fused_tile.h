@@ -129,6 +129,29 @@ __device__ __forceinline__ float bld_s(brsrc r, unsigned voff, unsigned soff) {
 __device__ __forceinline__ f4 bld4_s(brsrc r, unsigned voff, unsigned soff) {
   return __builtin_bit_cast(f4, __builtin_amdgcn_raw_buffer_load_b128(r, voff, soff, 0));
 }
+typedef unsigned u2v __attribute__((ext_vector_type(2)));
+typedef unsigned u3v __attribute__((ext_vector_type(3)));
+// Block kb of an operand with KT K-steps.  In the last block only the KT mod 4 K-steps that
+// the products use are loaded and the others are zero constants: a load into registers that
+// are never read lets the compiler reuse them while the load is in flight, and the reuse then
+// waits for that load and every load issued before it (the next phase's prefetch included).
+// For fully unrolled block loops only (kb a compile-time constant): in a partly unrolled loop
+// the choice becomes a run-time branch, and the product loop of the exact BB-WADG update
+// compiled that way gave wrong results at N = 3, so it keeps the 128-bit load.
+template <int KT>
+__device__ __forceinline__ f4 bld_blk(brsrc r, unsigned voff, unsigned soff, int kb) {
+  constexpr int KB = (KT + 3) / 4, REM = KT - 4 * (KB - 1);
+  if (REM == 4 || kb != KB - 1) return bld4_s(r, voff, soff);
+  if constexpr (REM == 1) {
+    return f4{bld_s(r, voff, soff), 0.f, 0.f, 0.f};
+  } else if constexpr (REM == 2) {
+    const u2v v = __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0);
+    return f4{__uint_as_float(v[0]), __uint_as_float(v[1]), 0.f, 0.f};
+  } else {
+    const u3v v = __builtin_amdgcn_raw_buffer_load_b96(r, voff, soff, 0);
+    return f4{__uint_as_float(v[0]), __uint_as_float(v[1]), __uint_as_float(v[2]), 0.f};
+  }
+}
 __device__ __forceinline__ void bst(brsrc r, float v, unsigned off) {
   __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, off, 0, 0);
 }
@@ -411,13 +434,17 @@ __device__ __forceinline__ void tile_shot(const FusedMesh &m, const FusedOps &o,
   const brsrc rAL = mkrsrc(o.LiftBlk, 4u * L::NPT * KBL * 1024u);
   const brsrc rAV = mkrsrc(o.VqBlk, (unsigned)L::NQT * KBV * 1024u);
   const brsrc rAP = mkrsrc(o.PqBlk, (unsigned)L::NPT * KBP * 1024u);
+  // The prefetches are unconditional (a wave past the last tile loads the last one and does not
+  // use it): loads under a wave test leave the compiler unsure how many are in flight where the
+  // paths join, and its waits then cover the prefetches as well.
   f4 aL[4][KBL];
-  if (wave < L::NPT) {
+  {
+    const int mt0 = wave < L::NPT ? wave : L::NPT - 1;
 #pragma unroll
     for (int f = 0; f < 4; ++f)
 #pragma unroll
       for (int b = 0; b < KBL; ++b)
-        aL[f][b] = bld4_s(rAL, lane * 16u, (unsigned)((f * L::NPT + wave) * KBL + b) * 1024u);
+        aL[f][b] = bld_blk<KTL>(rAL, lane * 16u, (unsigned)((f * L::NPT + mt0) * KBL + b) * 1024u, b);
   }
   __syncthreads();
 
@@ -425,12 +452,12 @@ __device__ __forceinline__ void tile_shot(const FusedMesh &m, const FusedOps &o,
   // P_f = LIFT_f phi_f;  rhs_p += sum_f P_f,  rhs_U -= sum_f n_f P_f
   f4 aV[L::QTW][KBV];
 #pragma unroll
-  for (int t = 0; t < L::QTW; ++t)
-    if (wave + 4 * t < L::NQT) {
+  for (int t = 0; t < L::QTW; ++t) {
+    const int qt = wave + 4 * t < L::NQT ? wave + 4 * t : L::NQT - 1;
 #pragma unroll
-      for (int kb = 0; kb < KBV; ++kb)
-        aV[t][kb] = bld4_s(rAV, lane * 16u, (unsigned)((wave + 4 * t) * KBV + kb) * 1024u);
-    }
+    for (int kb = 0; kb < KBV; ++kb)
+      aV[t][kb] = bld_blk<L::KTV>(rAV, lane * 16u, (unsigned)(qt * KBV + kb) * 1024u, kb);
+  }
   {
     const int col = lane & 15, kr = lane >> 4;
     const float *Bp = sB + kr * BS + col;
@@ -441,7 +468,7 @@ __device__ __forceinline__ void tile_shot(const FusedMesh &m, const FusedOps &o,
         for (int f = 0; f < 4; ++f)
 #pragma unroll
           for (int b = 0; b < KBL; ++b)
-            aL[f][b] = bld4_s(rAL, lane * 16u, (unsigned)((f * L::NPT + mt) * KBL + b) * 1024u);
+            aL[f][b] = bld_blk<KTL>(rAL, lane * 16u, (unsigned)((f * L::NPT + mt) * KBL + b) * 1024u, b);
       }
       f4 acc[4];
 #pragma unroll
@@ -496,9 +523,10 @@ __device__ __forceinline__ void tile_shot(const FusedMesh &m, const FusedOps &o,
     for (int e = 0; e < TB; ++e) bst(rY, sR[e * RS + tid], (unsigned)(e * Np + tid) * 4u);
   }
   f4 aP[KBP];
-  if (wave < L::NPT) {
+  {
+    const int mt0 = wave < L::NPT ? wave : L::NPT - 1;
 #pragma unroll
-    for (int kb = 0; kb < KBP; ++kb) aP[kb] = bld4_s(rAP, lane * 16u, (unsigned)(wave * KBP + kb) * 1024u);
+    for (int kb = 0; kb < KBP; ++kb) aP[kb] = bld_blk<L::KTP>(rAP, lane * 16u, (unsigned)(mt0 * KBP + kb) * 1024u, kb);
   }
   {
     const int col = lane & 15, kr = lane >> 4;
@@ -539,7 +567,7 @@ __device__ __forceinline__ void tile_shot(const FusedMesh &m, const FusedOps &o,
     for (int mt = wave; mt < L::NPT; mt += 4) {
       if (mt != wave) {
 #pragma unroll
-        for (int kb = 0; kb < KBP; ++kb) aP[kb] = bld4_s(rAP, lane * 16u, (unsigned)(mt * KBP + kb) * 1024u);
+        for (int kb = 0; kb < KBP; ++kb) aP[kb] = bld_blk<L::KTP>(rAP, lane * 16u, (unsigned)(mt * KBP + kb) * 1024u, kb);
       }
       f4 acc[NP];
 #pragma unroll
