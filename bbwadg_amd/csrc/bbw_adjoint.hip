@@ -568,40 +568,50 @@ __global__ __launch_bounds__(256) void k_adj2(int K, const float *__restrict__ v
   const brsrc rHt = mkrsrc(H + (size_t)k0 * NfpNf, (unsigned)nk * NfpNf * 4u);
   const brsrc rHa = mkrsrc(H, (unsigned)K * NfpNf * 4u);
   const brsrc rM = mkrsrc(mapP + (size_t)k0 * NfpNf, (unsigned)nk * NfpNf * 4u);
-  // The node's face slots first (the mapP gather depends on them, and a wait for them then
-  // covers nothing issued later).  The tile loads go to registers and to LDS after the last of
-  // them is issued, with unconditional stores into whole 256-word rounds (words past the tile
-  // hold zeros, never read).
+  // Loads in the order of first use, fixed by scheduling barriers (the memory counter is
+  // in order, so waiting for a load also waits for every load issued before it; left to
+  // itself the compiler issued the Z rows, which the first phase after the barrier reads,
+  // last): the node's face slots (the mapP gather depends on them), the Z rows, the volume
+  // tables, the H tile and geometry (stored to LDS before the barrier), the mapP entries, the
+  // lQ rows (read at the end).  The tile goes to LDS with unconditional stores into whole
+  // 256-word rounds (words past the tile hold zeros, never read).
   const int gv = tid / L::GS, nv = tid - gv * L::GS;
   const bool on = gv < L::NG && nv < Np;
   int slot[3];
 #pragma unroll
   for (int a = 0; a < 3; ++a) slot[a] = on ? FmaskInv[3 * nv + a] : -1;
-  // the thread's rows of the shift and elevation tables (volume transpose), loaded with the
-  // tile rather than after the barriers; lanes past the nodes read row 0 and do not use it
+  float zr[L::EPTV][4], lq[L::EPTV][4], hn[L::EPTV][3];
+  int mp[L::EPTV][3];
+#pragma unroll
+  for (int i = 0; i < L::EPTV; ++i)
+#pragma unroll
+    for (int f = 0; f < 4; ++f) zr[i][f] = bld(rZ, (unsigned)((gv + L::NG * i) * NpNf + f * Np + nv) * 4u);
+  __builtin_amdgcn_sched_barrier(0);
+  // the thread's rows of the shift and elevation tables (lanes past the nodes read row 0 and
+  // do not use it)
   const int g1 = tid / L::GS1, n1 = tid - g1 * L::GS1;
   const int4 S = Sids[n1 < L::Np1 ? n1 : 0];
   const float4 wt = Swts[n1 < L::Np1 ? n1 : 0];
   const int4 E = Eids[nv < Np ? nv : 0];
   const float4 ev = Evals[nv < Np ? nv : 0];
+  __builtin_amdgcn_sched_barrier(0);
   float hreg[L::HIT];
 #pragma unroll
   for (int r = 0; r < L::HIT; ++r) hreg[r] = bld(rHt, (unsigned)(tid + 256 * r) * 4u);
   const float gval = bld(mkrsrc(vgeo + (size_t)k0 * 9, (unsigned)nk * 36u), tid * 4u);
   const float fval = bld(mkrsrc(fgeo + (size_t)k0 * 16, (unsigned)nk * 64u), tid * 4u);
-  float zr[L::EPTV][4], lq[L::EPTV][4], hn[L::EPTV][3];
-  int mp[L::EPTV][3];
+  __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-  for (int i = 0; i < L::EPTV; ++i) {
-    const int e = gv + L::NG * i;
+  for (int i = 0; i < L::EPTV; ++i)
 #pragma unroll
-    for (int f = 0; f < 4; ++f) {
-      zr[i][f] = bld(rZ, (unsigned)(e * NpNf + f * Np + nv) * 4u);
-      lq[i][f] = bld(rL, (unsigned)(e * NpNf + f * Np + nv) * 4u);
-    }
+    for (int a = 0; a < 3; ++a)
+      mp[i][a] = bldi(rM, (unsigned)((gv + L::NG * i) * NfpNf + (slot[a] < 0 ? 0 : slot[a])) * 4u);
+  __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-    for (int a = 0; a < 3; ++a) mp[i][a] = bldi(rM, (unsigned)(e * NfpNf + (slot[a] < 0 ? 0 : slot[a])) * 4u);
-  }
+  for (int i = 0; i < L::EPTV; ++i)
+#pragma unroll
+    for (int f = 0; f < 4; ++f) lq[i][f] = bld(rL, (unsigned)((gv + L::NG * i) * NpNf + f * Np + nv) * 4u);
+  __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
   for (int r = 0; r < L::HIT; ++r) sH[tid + 256 * r] = hreg[r];
   sG[tid] = gval;
