@@ -1085,41 +1085,53 @@ hipError_t launch_surface_elas_tiled(bbw_ctx *c) {
 }
 
 // Elastic volume kernel on 16-element tiles (rk_volume_elas, elas_cpp/okl/ElasKernelsWADG.okl
-// :25-124, with the 4-nnz BB derivative operators): the tile's 9 fields are repacked per node as
-// (v1 v2 v3 sxx) (syy szz syz sxz) sxy, so each of the 16 operator entries costs three LDS reads.
+// :25-124, with the BB derivative operators) in the factored form of the fused acoustic stage.
+// D_m u[a] = sum_j a_j u[a - e_j + e_m], and the nine outputs (div sigma and the six strain
+// rates; the update applies the stiffness) are linear in the D_m u, so
+// each output is the elevation out[a] = sum_j a_j F[a - e_j] of values F formed at the
+// degree-(N-1) nodes b from the shifted coefficients u[b + e_m] (step 1, 36 scalar LDS reads per
+// b), followed by 12 LDS reads per node (step 2), against 48 (16 per operator entry of the
+// unfactored 4-nnz rows) before.
 template <int N>
 struct ElasVolLayout {
   static constexpr int Np = Dims<N>::Np, TB = 16, RowF = 9 * Np, QT = TB * RowF;
   static constexpr unsigned QB = QT * 4u, QIT = (QB + 4095) / 4096;
   static constexpr int GS = ((Np + 63) / 64) * 64, NG = 256 / GS, EPTV = (TB + NG - 1) / NG;
-  // Q tile [TB][9][Np] (padded to whole 4 KB rounds), then packed [TB][Np] float4 | [TB][Np]
-  // float4 | [TB][Np] float in the same region; vgeo in a 256-word slot
+  static constexpr int Np1 = N * (N + 1) * (N + 2) / 6;
+  static constexpr int GS1 = Np1 <= 32 ? 32 : 64, NG1 = 256 / GS1, EPT1 = TB / NG1;
+  static_assert(Np1 <= 64 && NG1 * EPT1 == TB, "step 1 covers the tile");
+  // Q tile [TB][9][Np] (padded to whole 4 KB rounds), then in the same region F as
+  // [TB][Np1] float4 | [TB][Np1] float4 | [TB][Np1] float; vgeo in a 256-word slot
   static constexpr int QTP = (int)QIT * 1024;
+  static_assert(9 * TB * Np1 <= QTP, "F fits the tile region");
   static constexpr size_t floats = (size_t)QTP + 256;
 };
 
 template <int N>
 __global__ __launch_bounds__(256) void k_volume_elas_t(int K, const float *__restrict__ vgeo,
-                                                       const int4 *__restrict__ Dids,
-                                                       const float4 *__restrict__ Dvals,
+                                                       const int4 *__restrict__ Sids,
+                                                       const int4 *__restrict__ Eids,
+                                                       const float4 *__restrict__ Evals,
                                                        const float *__restrict__ Q, float *__restrict__ rhs) {
   using L = ElasVolLayout<N>;
-  constexpr int Np = L::Np, TB = L::TB, RowF = L::RowF;
+  constexpr int Np = L::Np, TB = L::TB, RowF = L::RowF, Np1 = L::Np1;
   extern __shared__ float lds[];
-  float *sQ = lds;
-  float4 *sA = (float4 *)lds;                 // [TB][Np] (v1 v2 v3 sxx)
-  float4 *sB = sA + TB * Np;                  // [TB][Np] (syy szz syz sxz)
-  float *sC = (float *)(sB + TB * Np);        // [TB][Np] sxy
-  float *sG = lds + L::QTP;                   // [TB][9] in a 256-word slot
+  float *sQ = lds;                             // [TB][9][Np]     (step 1)
+  float4 *sFa = (float4 *)lds;                 // [TB][Np1] F0-3  (step 2)
+  float4 *sFb = sFa + TB * Np1;                // [TB][Np1] F4-7
+  float *sFc = (float *)(sFb + TB * Np1);      // [TB][Np1] F8
+  float *sG = lds + L::QTP;                    // [TB][9] in a 256-word slot
   const int tid = threadIdx.x;
   const int k0 = blockIdx.x * TB, nk = min(TB, K - k0);
   const int gv = tid / L::GS, nv = tid - gv * L::GS;
   const bool onv = gv < L::NG && nv < Np;
-  // the node's derivative rows are loaded with the tile, not after the barriers (lanes past
-  // the nodes read row 0 and return before using it)
-  const int nr = nv < Np ? nv : 0;
-  const float4 dv = Dvals[nr];
-  const int4 D1 = Dids[nr], D2 = Dids[Np + nr], D3 = Dids[2 * Np + nr], D4 = Dids[3 * Np + nr];
+  const int g1 = tid / L::GS1, n1 = tid - g1 * L::GS1;
+  const bool on1 = n1 < Np1;
+  // the thread's rows of the shift and elevation tables, loaded with the tile (lanes past the
+  // nodes read row 0 and do not use it)
+  const int4 S = Sids[on1 ? n1 : 0];
+  const int4 E = Eids[nv < Np ? nv : 0];
+  const float4 ev = Evals[nv < Np ? nv : 0];
   {
     // every load before the LDS stores, which are unconditional (whole rounds; words past the
     // tile and past vgeo's TB x 9 are written but never read)
@@ -1134,73 +1146,67 @@ __global__ __launch_bounds__(256) void k_volume_elas_t(int K, const float *__res
     sG[tid] = gv0;
   }
   __syncthreads();
-  float qv[L::EPTV][9];
-  if (onv) {
+  // ---- step 1: F at the degree-(N-1) nodes b from u[b + e_m] (m = 1..4), into registers
+  float F[L::EPT1][9];
+  if (on1) {
 #pragma unroll
-    for (int i = 0; i < L::EPTV; ++i)
+    for (int i = 0; i < L::EPT1; ++i) {
+      const int e = g1 + L::NG1 * i;
+      const float *q = sQ + e * RowF;
+      const float *G = sG + 9 * e;
+      float Qx[9], Qy[9], Qz[9];
 #pragma unroll
-      for (int c = 0; c < 9; ++c) qv[i][c] = sQ[(gv + L::NG * i) * RowF + c * Np + nv];
+      for (int c = 0; c < 9; ++c) {
+        const float *qc = q + c * Np;
+        const float s0 = qc[S.x], s1 = qc[S.y], s2 = qc[S.z], s3 = qc[S.w];
+        const float dr = .5f * (s1 - s0), ds = .5f * (s2 - s0), dt = .5f * (s3 - s0);
+        Qx[c] = G[0] * dr + G[3] * ds + G[6] * dt;
+        Qy[c] = G[1] * dr + G[4] * ds + G[7] * dt;
+        Qz[c] = G[2] * dr + G[5] * ds + G[8] * dt;
+      }
+      F[i][0] = Qx[3] + Qy[8] + Qz[7];
+      F[i][1] = Qx[8] + Qy[4] + Qz[6];
+      F[i][2] = Qx[7] + Qy[6] + Qz[5];
+      F[i][3] = Qx[0];
+      F[i][4] = Qy[1];
+      F[i][5] = Qz[2];
+      F[i][6] = Qy[2] + Qz[1];
+      F[i][7] = Qx[2] + Qz[0];
+      F[i][8] = Qx[1] + Qy[0];
+    }
   }
   __syncthreads();
-  if (onv) {
+  if (on1) {
 #pragma unroll
-    for (int i = 0; i < L::EPTV; ++i) {
-      const int e = gv + L::NG * i;
-      sA[e * Np + nv] = make_float4(qv[i][0], qv[i][1], qv[i][2], qv[i][3]);
-      sB[e * Np + nv] = make_float4(qv[i][4], qv[i][5], qv[i][6], qv[i][7]);
-      sC[e * Np + nv] = qv[i][8];
+    for (int i = 0; i < L::EPT1; ++i) {
+      const int e = g1 + L::NG1 * i;
+      sFa[e * Np1 + n1] = make_float4(F[i][0], F[i][1], F[i][2], F[i][3]);
+      sFb[e * Np1 + n1] = make_float4(F[i][4], F[i][5], F[i][6], F[i][7]);
+      sFc[e * Np1 + n1] = F[i][8];
     }
   }
   __syncthreads();
   if (!onv) return;
+  // ---- step 2: out[a] = sum_j a_j F[a - e_j]; rows out (lanes over nodes)
   const brsrc rO = mkrsrc(rhs + (size_t)k0 * RowF, (unsigned)nk * RowF * 4u);
-  int id[4][4];
-  {
-    // id[m][j]: column of operator m (1..4), non-zero j
-    id[0][0] = D1.x; id[0][1] = D2.x; id[0][2] = D3.x; id[0][3] = D4.x;
-    id[1][0] = D1.y; id[1][1] = D2.y; id[1][2] = D3.y; id[1][3] = D4.y;
-    id[2][0] = D1.z; id[2][1] = D2.z; id[2][2] = D3.z; id[2][3] = D4.z;
-    id[3][0] = D1.w; id[3][1] = D2.w; id[3][2] = D3.w; id[3][3] = D4.w;
-  }
-  const float dw[4] = {dv.x, dv.y, dv.z, dv.w};
 #pragma unroll
   for (int i = 0; i < L::EPTV; ++i) {
     const int e = gv + L::NG * i;
-    const float4 *A = sA + e * Np, *B = sB + e * Np;
-    const float *C = sC + e * Np;
-    float pm[4][9];
-#pragma unroll
-    for (int m = 0; m < 4; ++m) {
-#pragma unroll
-      for (int c = 0; c < 9; ++c) pm[m][c] = 0.f;
-#pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        const float4 a = A[id[m][j]], b = B[id[m][j]];
-        const float cc = C[id[m][j]];
-        pm[m][0] += dw[j] * a.x; pm[m][1] += dw[j] * a.y; pm[m][2] += dw[j] * a.z; pm[m][3] += dw[j] * a.w;
-        pm[m][4] += dw[j] * b.x; pm[m][5] += dw[j] * b.y; pm[m][6] += dw[j] * b.z; pm[m][7] += dw[j] * b.w;
-        pm[m][8] += dw[j] * cc;
-      }
-    }
-    const float *G = sG + 9 * e;
-    float Qx[9], Qy[9], Qz[9];
-#pragma unroll
-    for (int c = 0; c < 9; ++c) {
-      const float dr = .5f * (pm[1][c] - pm[0][c]), ds = .5f * (pm[2][c] - pm[0][c]), dt = .5f * (pm[3][c] - pm[0][c]);
-      Qx[c] = G[0] * dr + G[3] * ds + G[6] * dt;
-      Qy[c] = G[1] * dr + G[4] * ds + G[7] * dt;
-      Qz[c] = G[2] * dr + G[5] * ds + G[8] * dt;
-    }
+    const float4 *Fa = sFa + e * Np1, *Fb = sFb + e * Np1;
+    const float *Fc = sFc + e * Np1;
+    const float4 a1 = Fa[E.x], a2 = Fa[E.y], a3 = Fa[E.z], a4 = Fa[E.w];
+    const float4 b1 = Fb[E.x], b2 = Fb[E.y], b3 = Fb[E.z], b4 = Fb[E.w];
+    const float c1 = Fc[E.x], c2 = Fc[E.y], c3 = Fc[E.z], c4 = Fc[E.w];
     const unsigned o = (unsigned)(e * RowF + nv) * 4u;
-    bst(rO, Qx[3] + Qy[8] + Qz[7], o);
-    bst(rO, Qx[8] + Qy[4] + Qz[6], o + 1 * Np * 4u);
-    bst(rO, Qx[7] + Qy[6] + Qz[5], o + 2 * Np * 4u);
-    bst(rO, Qx[0], o + 3 * Np * 4u);
-    bst(rO, Qy[1], o + 4 * Np * 4u);
-    bst(rO, Qz[2], o + 5 * Np * 4u);
-    bst(rO, Qy[2] + Qz[1], o + 6 * Np * 4u);
-    bst(rO, Qx[2] + Qz[0], o + 7 * Np * 4u);
-    bst(rO, Qx[1] + Qy[0], o + 8 * Np * 4u);
+    bst(rO, ev.x * a1.x + ev.y * a2.x + ev.z * a3.x + ev.w * a4.x, o);
+    bst(rO, ev.x * a1.y + ev.y * a2.y + ev.z * a3.y + ev.w * a4.y, o + 1 * Np * 4u);
+    bst(rO, ev.x * a1.z + ev.y * a2.z + ev.z * a3.z + ev.w * a4.z, o + 2 * Np * 4u);
+    bst(rO, ev.x * a1.w + ev.y * a2.w + ev.z * a3.w + ev.w * a4.w, o + 3 * Np * 4u);
+    bst(rO, ev.x * b1.x + ev.y * b2.x + ev.z * b3.x + ev.w * b4.x, o + 4 * Np * 4u);
+    bst(rO, ev.x * b1.y + ev.y * b2.y + ev.z * b3.y + ev.w * b4.y, o + 5 * Np * 4u);
+    bst(rO, ev.x * b1.z + ev.y * b2.z + ev.z * b3.z + ev.w * b4.z, o + 6 * Np * 4u);
+    bst(rO, ev.x * b1.w + ev.y * b2.w + ev.z * b3.w + ev.w * b4.w, o + 7 * Np * 4u);
+    bst(rO, ev.x * c1 + ev.y * c2 + ev.z * c3 + ev.w * c4, o + 8 * Np * 4u);
   }
 }
 
@@ -1214,7 +1220,7 @@ static hipError_t volume_elas_t_n(bbw_ctx *c) {
     if (e != hipSuccess) return e;
   }
   hipLaunchKernelGGL(k_volume_elas_t<N>, dim3((c->K + L::TB - 1) / L::TB), dim3(256), sh, c->stream, c->K,
-                     c->vgeo, c->Dids, c->Dvals, c->Q[c->cur], c->rhs);
+                     c->vgeo, c->Sids, c->Eids, c->Evals, c->Q[c->cur], c->rhs);
   return hipGetLastError();
 }
 
