@@ -726,7 +726,7 @@ __global__ __launch_bounds__(256, N <= 4 ? 4 : N == 5 ? 3 : 2) void k_update_ela
   if (has_src && tid < TB) sSlot[tid] = tid < nk ? src_slot2(src, k0 + tid) : -1;
   __syncthreads();
 
-#pragma unroll 1
+#pragma unroll
   for (int g = 0; g < 3; ++g) {
     // ---- 2: S_b = map_g(Vq . rhs_{3g+b}) (M = cubature points, N = elements, K = nodes)
     {
