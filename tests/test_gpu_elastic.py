@@ -25,7 +25,7 @@ def make(N, n, seed=0, tau=(1.0, 1.0)):
 
 
 @pytest.mark.parametrize("impl", [_lib.PLAIN_FLAT, _lib.PLAIN_TILED])
-@pytest.mark.parametrize("N", [2, 3, 4, 5, 6])
+@pytest.mark.parametrize("N", [1, 2, 3, 4, 5, 6])
 def test_elastic_kernels(N, impl):
     print(f"case elastic kernels N={N} impl={impl}", flush=True)
     ops, m, dm, mats, s = make(N, 3, tau=(0.8, 1.2))

@@ -26,7 +26,7 @@ def oracle_ops(P, dtype):
 
 
 @pytest.mark.parametrize("impl", [_lib.PLAIN_FLAT, _lib.PLAIN_TILED])
-@pytest.mark.parametrize("N", [2, 3, 4, 5, 6])
+@pytest.mark.parametrize("N", [1, 2, 3, 4, 5, 6])
 def test_volume(N, impl):
     print(f"case volume N={N} impl={impl}", flush=True)
     P, s = make(N, 3)
@@ -42,7 +42,7 @@ def test_volume(N, impl):
 
 
 @pytest.mark.parametrize("impl", [_lib.PLAIN_FLAT, _lib.PLAIN_TILED])
-@pytest.mark.parametrize("N", [2, 3, 4, 5, 6])
+@pytest.mark.parametrize("N", [1, 2, 3, 4, 5, 6])
 def test_surface(N, impl):
     print(f"case surface N={N} impl={impl}", flush=True)
     P, s = make(N, 3)
@@ -60,7 +60,7 @@ def test_surface(N, impl):
 
 
 @pytest.mark.parametrize("impl", [_lib.PLAIN_FLAT, _lib.PLAIN_TILED])
-@pytest.mark.parametrize("N", [2, 3, 4, 5, 6])
+@pytest.mark.parametrize("N", [1, 2, 3, 4, 5, 6])
 def test_update_wadg(N, impl):
     print(f"case update_wadg N={N} impl={impl}", flush=True)
     P, s = make(N, 3)
