@@ -7,7 +7,9 @@ FP32.  A step is one LSRK45 step (5 stages) over the whole mesh, inputs resident
 
 Reported: the fused path (one kernel per stage, HIP-graph replay) as ``value``; the plain
 three-launch path of the reference (volume, surface, WADG update) beside it; the
-roofline of the fused stage kernel; the CPU oracle timed on a bounded sample.
+roofline of the fused stage kernel; the adjoint-gradient / forward ratio; the batched
+multi-shot run; the elastic configuration's stage rate; the CPU oracle timed on a bounded
+sample.
 
     python bench.py [--steps K] [--warmup W] [--gpus 1] [--mesh 48] [--N 5]
 """
@@ -139,6 +141,39 @@ def adjoint_ratio(nsteps, mesh_n=32, N=5, ckpt=50):
             "misfit_agree": abs(Jf - Jg) / max(Jf, 1e-300)}
 
 
+def elastic_throughput(nsteps, mesh_n=32, N=4):
+    """Elastic velocity-stress configuration (32^3 x 6 tets, N = 4, 9 fields, seeded Lame and
+    density fields, random start, the reference's elastic time step): the three-launch stage
+    (the elastic product path, tiled kernels), nsteps steps from HIP graphs, timed with HIP
+    events on the solver's stream."""
+    import torch
+    from bbwadg_amd.drive import setup_elastic
+    from bbwadg_amd.solver import MODE_PLAIN
+    s = setup_elastic(mesh_n, N)
+    stream = torch.cuda.Stream()
+    s.set_stream(stream.cuda_stream)
+    dt = s.dt_ref   # the reference's elastic time step estimate
+    s.step(2, dt, mode=MODE_PLAIN)
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    s.step(nsteps, dt, mode=MODE_PLAIN)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1)
+    Q = s.WaveGetData3d()
+    if not np.all(np.isfinite(Q)):
+        raise RuntimeError("non-finite elastic state after the timed run")
+    dofs = s.K * s.ops.Np * 9 * 5 * nsteps
+    K = s.K
+    s.close()
+    return {"workload": f"elastic velocity-stress, {mesh_n}^3x6={K} tets, N={N}, 9 fields, seeded "
+                        "Lame/density fields, three-launch stage + HIP graph",
+            "value": dofs / (ms / 1e3), "unit": "DOF-stage/s", "steps": nsteps, "dt": dt,
+            "ms_per_stage": ms / (5 * nsteps)}
+
+
 def batched_throughput(step_scale=1.0, tune_steps=20):
     """Batched configuration (64 shots on 16^3..40^3 x 6 meshes, N = 5, durations 300-1200
     steps): one persistent launch per stage with the autotuned shots-per-tile and
@@ -202,6 +237,8 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=1 << 30)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--adjoint-steps", type=int, default=800)
+    ap.add_argument("--elastic-steps", type=int, default=50,
+                    help="steps of the elastic configuration (0 skips it)")
     ap.add_argument("--batched-scale", type=float, default=1.0,
                     help="duration scale of the batched configuration (0 skips it)")
     args = ap.parse_args()
@@ -222,7 +259,10 @@ def main():
     s = WaveSolver(m, N, ops=ops, dmesh=dm)
     s.InitWADG_subelem(c2q)
     s.set_source(selems, scoef, f0=10.0, amp=1.0)
-    stream = torch.cuda.current_stream()
+    # A stream of our own (default priority) for the solver, so that the HIP events below are
+    # recorded on the stream the kernels are launched on.  torch's default stream has handle 0,
+    # which bbw_set_stream takes as "the context's own stream".
+    stream = torch.cuda.Stream()
     s.set_stream(stream.cuda_stream)
     Q0 = np.zeros((K, 4, ops.Np), dtype=np.float32)
     s.WaveSetData3d(Q0)
@@ -265,6 +305,7 @@ def main():
 
     s.close()
     adj = adjoint_ratio(args.adjoint_steps) if args.adjoint_steps > 0 else None
+    elas = elastic_throughput(args.elastic_steps) if args.elastic_steps > 0 else None
     bat = batched_throughput(args.batched_scale) if args.batched_scale > 0 else None
 
     cpu = None
@@ -302,6 +343,7 @@ def main():
         "adjoint_over_forward": adj["ratio"] if adj else None,
         "adjoint": adj,
         "batched": bat,
+        "elastic": elas,
         "energy_after": E,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
