@@ -464,11 +464,11 @@ __global__ __launch_bounds__(256, N <= 5 ? 4 : 2) void k_adj1(int K, FusedOps o,
   f4 aL[L::PRW][KBV];
 #pragma unroll
   for (int j = 0; j < L::PRW; ++j) {
-    const int pr = wave + 4 * j;
-    if (pr < L::NPR) {
+    // unconditional, clamped to the last row tile (a wave past it loads that one and does not
+    // use it): at N = 5 every wave has both tiles, and without the test the waits are exact
+    const int pr = wave + 4 * j < L::NPR ? wave + 4 * j : L::NPR - 1;
 #pragma unroll
-      for (int kb = 0; kb < KBV; ++kb) aL[j][kb] = bld_blk<KTV>(rLT, lane * 16u, (unsigned)(pr * KBV + kb) * 1024u, kb);
-    }
+    for (int kb = 0; kb < KBV; ++kb) aL[j][kb] = bld_blk<KTV>(rLT, lane * 16u, (unsigned)(pr * KBV + kb) * 1024u, kb);
   }
   for (int i = tid; i < TB * Nq; i += 256) bst(rGd, sGr[i], i * 4u);
   for (int mt = wave; mt < L::NPT; mt += 4) {
