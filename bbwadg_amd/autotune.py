@@ -27,8 +27,8 @@ def _time_steps(solver, dt, nsteps, mode):
 def tune_single(solver, dt, nsteps=10, kblk_vector=(4, 8, 16), kblk_plain=(1, 2, 4, 8)):
     """Pick the fused implementation and the plain kernels (16-element tiles, or the flat
     kernels with their elements-per-block); returns a dict of the trials and the choice (also
-    applied to the solver).  The state is restored."""
-    Q0, res0 = solver.WaveGetData3d(), solver.get_res()
+    applied to the solver).  The state, residual and solver time are restored."""
+    Q0, res0, time0 = solver.WaveGetData3d(), solver.get_res(), solver.time
     trials = []
     for kb in kblk_vector:
         solver.set_option(_lib.OPT_FUSED_IMPL, _lib.FUSED_VECTOR)
@@ -57,6 +57,7 @@ def tune_single(solver, dt, nsteps=10, kblk_vector=(4, 8, 16), kblk_plain=(1, 2,
             solver.set_option(opt, best_plain[0])
     solver.WaveSetData3d(Q0)
     solver.set_res(res0)
+    solver.time = time0
     return {"fused_trials": trials, "fused_choice": best[0], "plain_trials": plain,
             "plain_choice": best_plain[0]}
 

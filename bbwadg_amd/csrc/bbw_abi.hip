@@ -192,7 +192,9 @@ void bbw_destroy(bbw_ctx *c) {
   dfree(c->c2q); dfree(c->Jq); dfree(c->rhoinv); dfree(c->lambda); dfree(c->mu);
   dfree(c->Q[0]); dfree(c->Q[1]); dfree(c->res); dfree(c->rhs); dfree(c->src_elems);
   dfree(c->src_coef); dfree(c->rec_elems); dfree(c->rec_basis); dfree(c->rec_trace);
-  dfree(c->partial); dfree(c->ftime_win);
+  dfree(c->partial); dfree(c->ftime_win); dfree(c->ft_all);
+  if (c->ft_host) hipHostFree(c->ft_host);
+  if (c->ft_evt) hipEventDestroy(c->ft_evt);
   if (c->own_stream) hipStreamDestroy(c->own_stream);
   delete c;
 }
@@ -740,12 +742,17 @@ int bbw_step(bbw_ctx *c, int nsteps, double dt, double t0, int mode, int flags) 
 
   int s = 0;
   if (use_graph && nsteps >= G) {
-    // window buffers (graph nodes read their stage values and write receiver rows there)
-    if (c->ftime_win_len < G * 5) {
-      dfree(c->ftime_win);
-      if ((rc = dalloc(c, &c->ftime_win, (size_t)G * 5 + (size_t)G * (c->nrec > 0 ? c->nrec : 1)))) return rc;
-      c->ftime_win_len = G * 5;
+    // window buffer: the graph nodes read their stage values and write receiver rows there.  It
+    // is sized for the current receiver count; a larger one (receivers set after a graph step)
+    // reallocates it and recaptures the graph, which holds its addresses.
+    const size_t win = (size_t)G * 5 + (size_t)G * (c->nrec > 0 ? c->nrec : 1);
+    if (c->ftime_win_cap < win) {
+      HIPCHK(c, hipStreamSynchronize(c->stream));
       reset_graph(c);
+      dfree(c->ftime_win);
+      c->ftime_win_cap = 0;
+      if ((rc = dalloc(c, &c->ftime_win, win))) return rc;
+      c->ftime_win_cap = win;
     }
     float *rec_win = c->ftime_win + G * 5;
     const int parity = mode == BBW_MODE_FUSED ? c->cur : 0;
@@ -777,14 +784,35 @@ int bbw_step(bbw_ctx *c, int nsteps, double dt, double t0, int mode, int flags) 
       if (e != hipSuccess) return hipfail(c, e, "hipGraphInstantiate");
       g.steps = G; g.mode = mode; g.flags = flags; g.parity = parity; g.dt = dt;
     }
-    float *ft_dev = nullptr;
     if (has_src) {
-      if ((rc = dalloc(c, &ft_dev, ft.size()))) return rc;
-      HIPCHK(c, hipMemcpyAsync(ft_dev, ft.data(), ft.size() * sizeof(float), hipMemcpyHostToDevice, c->stream));
+      // the call's stage values go to the device once, through a persistent pinned staging
+      // buffer; the buffers grow only when a call has more steps than every earlier one
+      const size_t nft = ft.size();
+      if (!c->ft_evt) HIPCHK(c, hipEventCreateWithFlags(&c->ft_evt, hipEventDisableTiming));
+      if (c->ft_evt_pending) {
+        HIPCHK(c, hipEventSynchronize(c->ft_evt));  // the last copy out of ft_host has ended
+        c->ft_evt_pending = false;
+      }
+      if (c->ft_all_cap < nft || c->ft_host_cap < nft) {
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        dfree(c->ft_all);
+        c->ft_all_cap = 0;
+        if ((rc = dalloc(c, &c->ft_all, nft))) return rc;
+        c->ft_all_cap = nft;
+        if (c->ft_host) hipHostFree(c->ft_host);
+        c->ft_host = nullptr;
+        c->ft_host_cap = 0;
+        HIPCHK(c, hipHostMalloc((void **)&c->ft_host, nft * sizeof(float), hipHostMallocDefault));
+        c->ft_host_cap = nft;
+      }
+      std::memcpy(c->ft_host, ft.data(), nft * sizeof(float));
+      HIPCHK(c, hipMemcpyAsync(c->ft_all, c->ft_host, nft * sizeof(float), hipMemcpyHostToDevice, c->stream));
+      HIPCHK(c, hipEventRecord(c->ft_evt, c->stream));
+      c->ft_evt_pending = true;
     }
     for (; s + G <= nsteps; s += G) {
       if (has_src)
-        HIPCHK(c, hipMemcpyAsync(c->ftime_win, ft_dev + (size_t)s * 5, sizeof(float) * G * 5,
+        HIPCHK(c, hipMemcpyAsync(c->ftime_win, c->ft_all + (size_t)s * 5, sizeof(float) * G * 5,
                                  hipMemcpyDeviceToDevice, c->stream));
       HIPCHK(c, hipGraphLaunch(g.exec, c->stream));
       if (c->nrec) {
@@ -792,10 +820,6 @@ int bbw_step(bbw_ctx *c, int nsteps, double dt, double t0, int mode, int flags) 
                                  sizeof(float) * G * c->nrec, hipMemcpyDeviceToDevice, c->stream));
         c->rec_steps += G;
       }
-    }
-    if (ft_dev) {
-      HIPCHK(c, hipStreamSynchronize(c->stream));
-      hipFree(ft_dev);
     }
   }
   for (; s < nsteps; ++s) {

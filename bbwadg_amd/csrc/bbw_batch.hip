@@ -265,6 +265,9 @@ int bbw_batch_add_mesh(bbw_batch *b, int K, const float *vgeo, const float *fgeo
   if (!b || K <= 0 || !vgeo || !fgeo || !vmapP || !mesh_id) return BBW_ERR_ARG;
   const int Np = b->ops->Np, NfpNf = 4 * b->ops->Nfp;
   const long long lim = (long long)K * 4 * Np;
+  // the stage kernel addresses a shot's state (K * 4 * Np floats) through 32-bit byte offsets
+  if ((unsigned long long)K * 16 * Np >= (1ull << 32))
+    return bfail(b, BBW_ERR_ARG, "mesh too large: K * 4 * Np * 4 bytes must stay below 2^32");
   for (size_t i = 0; i < (size_t)K * NfpNf; ++i)
     if (vmapP[i] < 0 || vmapP[i] >= lim || (vmapP[i] % (4LL * Np)) >= Np)
       return bfail(b, BBW_ERR_ARG, "vmapP entry out of range");

@@ -142,10 +142,17 @@ struct bbw_ctx {
   size_t Ycache_len = 0;
   int ycache_mb = 0;  // device memory for Ycache in MiB: 0 = automatic, < 0 = none
 
-  // graph cache and per-window stage data
+  // graph cache and per-window stage data: ftime_win holds the window's stage values
+  // [G][5] followed by its receiver rows [G][max(nrec, 1)]; ftime_win_cap is its length in floats
   bbw::Graph graph;
-  float *ftime_win = nullptr;  // [steps_in_window * 5]
-  int ftime_win_len = 0;
+  float *ftime_win = nullptr;
+  size_t ftime_win_cap = 0;
+  // stage values of a whole bbw_step call: device copy, its pinned host staging buffer and the
+  // event that marks the end of the last host-to-device copy from it (capacities in floats)
+  float *ft_all = nullptr, *ft_host = nullptr;
+  size_t ft_all_cap = 0, ft_host_cap = 0;
+  hipEvent_t ft_evt = nullptr;
+  bool ft_evt_pending = false;
 };
 
 namespace bbw {

@@ -28,6 +28,21 @@ from . import _lib, media, mesh as meshmod, operators
 from ._lib import MODE_FUSED, MODE_PLAIN, check, f32, f64, i32, ptr
 
 
+def l2_error(ops, cub, J, Q, uex, time, field=0):
+    """(L2err, relL2err) of field ``field`` of the BB coefficients Q [K][Nf][Np] against
+    uex(x, y, z, t) at the WADG cubature ``cub`` = (xq, yq, zq), each (K, Nq): the sums
+    sum_K J_K sum_q w_q (u_q - uex_q)^2 and sum_K J_K sum_q w_q uex_q^2 of ``compute_error``
+    (cpp/src/WaveOKL3d.cpp:1683-1726), FP64.  The relative error is sqrt(err2) / sqrt(norm2);
+    the reference takes one more square root of it (:1722)."""
+    xq, yq, zq = cub
+    uq = np.asarray(Q, dtype=np.float64)[:, field] @ ops.Vq.T
+    ue = uex(xq, yq, zq, time)
+    w = ops.wq[None, :] * np.asarray(J, dtype=np.float64)[:, None]
+    e2 = float(np.sum((uq - ue) ** 2 * w))
+    n2 = float(np.sum(ue ** 2 * w))
+    return np.sqrt(e2), (np.sqrt(e2) / np.sqrt(n2) if n2 > 0 else np.inf)
+
+
 class WaveSolver:
     def __init__(self, mesh, N, nfields=4, KblkV=0, KblkS=0, KblkU=0, KblkF=0, device=0,
                  ops=None, dmesh=None):
@@ -228,13 +243,7 @@ class WaveSolver:
     def compute_error(self, time, uex, Q=None, field=0):
         """(L2err, relL2err) of one field at the WADG cubature (``compute_error``)."""
         Q = self.WaveGetData3d() if Q is None else Q
-        xq, yq, zq = self.cubature_points()
-        uq = np.asarray(Q, dtype=np.float64)[:, field] @ self.ops.Vq.T
-        ue = uex(xq, yq, zq, time)
-        w = self.ops.wq[None, :] * self.dmesh.J[:, None]
-        e2 = float(np.sum((uq - ue) ** 2 * w))
-        n2 = float(np.sum(ue ** 2 * w))
-        return np.sqrt(e2), np.sqrt(e2) / np.sqrt(n2)
+        return l2_error(self.ops, self.cubature_points(), self.dmesh.J, Q, uex, time, field)
 
     def set_option(self, option, value):
         check(self.ctx, self.lib.bbw_set_option(self.ctx, int(option), int(value)))

@@ -18,8 +18,9 @@ reference's OKL kernels in FP64 and FP32) and adds, in numpy FP64:
   relative error computed as sqrt(err2)/sqrt(norm2) (the reference takes an
   extra square root, ``:1722``).
 
-Parity pins for each function are in ``tests/test_oracle.py`` and
-``tests/test_operators.py``; see the module docstrings there.
+Parity pins for each function are in ``tests/test_oracle.py`` (drivers, energies,
+errors, elastic update), ``tests/test_operators.py`` (operators) and
+``tests/test_elastic_oracle.py`` (elastic right-hand side); see the module docstrings there.
 """
 import ctypes
 import os
@@ -129,13 +130,21 @@ def ricker(t, f0):
     return (1.0 - 2.0 * a * a) * np.exp(-a * a)
 
 
+def stage_time(t, c_i, dt, src_time):
+    """Time at which a stage evaluates the source: t + c_i dt ("stage"), or the step start t
+    for every stage ("step", as the reference's elastic driver does,
+    ``elas_cpp/src/WaveOKL3d.cpp:1675-1680``)."""
+    return t + c_i * dt if src_time == "stage" else t
+
+
 def rk_run(oo, K, vgeo, fgeo, vmapP, c2q, Q, dt, nsteps, t0=0.0, nproj=4, Jq=None,
-           src=None, f0=None, amp=1.0, res=None, record=None):
+           src=None, f0=None, amp=1.0, res=None, record=None, src_time="stage"):
     """Advance Q (modified in place) by ``nsteps`` LSRK45 steps of the acoustic WADG scheme.
 
     Each stage: rhs = volume(Q); rhs += surface(Q); WADG update (``RK_step_WADG_subelem``).
     ``src`` (K, Np) BB coefficients are added to the p right-hand side with
-    amplitude amp*ricker(t + c_i dt, f0).  ``record(step, Q)`` is called after each step.
+    amplitude amp*ricker(stage_time(t, c_i, dt, src_time), f0).  ``record(step, Q)`` is called
+    after each step.
     """
     wd = oo.dtype
     rhs = np.zeros_like(Q)
@@ -147,7 +156,7 @@ def rk_run(oo, K, vgeo, fgeo, vmapP, c2q, Q, dt, nsteps, t0=0.0, nproj=4, Jq=Non
             fa, fb, fdt = wd.type(oo.rk4a[i]), wd.type(oo.rk4b[i]), wd.type(dt)
             volume(oo, K, vgeo, Q, rhs)
             surface(oo, K, fgeo, vmapP, Q, rhs)
-            ft = 0.0 if src is None else amp * ricker(t + oo.rk4c[i] * dt, f0)
+            ft = 0.0 if src is None else amp * ricker(stage_time(t, oo.rk4c[i], dt, src_time), f0)
             update_wadg(oo, K, c2q, fa, fb, fdt, rhs, res, Q, Jq=Jq, nproj=nproj, ftime=ft, src=src)
         if record is not None:
             record(step, Q)
@@ -155,7 +164,8 @@ def rk_run(oo, K, vgeo, fgeo, vmapP, c2q, Q, dt, nsteps, t0=0.0, nproj=4, Jq=Non
 
 
 def rk_run_elas(oo, K, vgeo, fgeo, vmapP, rhoinvq, lamq, muq, Q, dt, nsteps, t0=0.0,
-                src=None, srcw=None, f0=None, amp=1.0, tau_v=1.0, tau_s=1.0, res=None):
+                src=None, srcw=None, f0=None, amp=1.0, tau_v=1.0, tau_s=1.0, res=None,
+                src_time="stage"):
     wd = oo.dtype
     rhs = np.zeros_like(Q)
     if res is None:
@@ -166,7 +176,7 @@ def rk_run_elas(oo, K, vgeo, fgeo, vmapP, rhoinvq, lamq, muq, Q, dt, nsteps, t0=
             fa, fb, fdt = wd.type(oo.rk4a[i]), wd.type(oo.rk4b[i]), wd.type(dt)
             volume(oo, K, vgeo, Q, rhs, nfields=9)
             surface(oo, K, fgeo, vmapP, Q, rhs, nfields=9, tau_v=tau_v, tau_s=tau_s)
-            ft = 0.0 if src is None else amp * ricker(t + oo.rk4c[i] * dt, f0)
+            ft = 0.0 if src is None else amp * ricker(stage_time(t, oo.rk4c[i], dt, src_time), f0)
             update_elas(oo, K, rhoinvq, lamq, muq, fa, fb, fdt, rhs, res, Q, ftime=ft, src=src, srcw=srcw)
     return Q, res
 
@@ -194,11 +204,12 @@ def energy_wadg(ops, J, c2q, Q):
 
 
 def l2_error(ops, xq, yq, zq, J, Q, exact, time, field=0, nfields=4):
-    """(L2err, relL2err) of one field against exact(x, y, z, t) at the WADG cubature."""
+    """(L2err, relL2err) of one field against exact(x, y, z, t) at the WADG cubature points
+    xq, yq, zq, each (K, Nq)."""
     K, Np = len(J), ops.Np
     Qr = np.asarray(Q, dtype=np.float64).reshape(K, nfields, Np)
     uq = Qr[:, field] @ ops.Vq.T  # (K, Nq)
-    ue = exact(xq.T, yq.T, zq.T, time)
+    ue = exact(xq, yq, zq, time)
     w = ops.wq[None, :] * J[:, None]
     err2 = float(np.sum((uq - ue) ** 2 * w))
     nrm2 = float(np.sum(ue ** 2 * w))

@@ -1,0 +1,291 @@
+"""Pins the oracle's drivers and diagnostics (oracle/oracle.py) with checks that share none of
+their steps, each chosen so that a plausible mistake fails it:
+
+* ``rk_run`` (``Wave_RK`` / ``RK_step_WADG_subelem``, cpp/src/WaveOKL3d.cpp:2014-2109): the
+  cavity mode on [-0.5, 0.5]^3 (cpp/main_subelem.cpp:3-5) against its exact solution, with the
+  observed order N + 1 under mesh refinement at N = 3, 4, 5; fourth order in time with a
+  Ricker source evaluated at the stage times (a source evaluated at the step start for every
+  stage, the reference's elastic driver, drops to first order and fails the same check);
+* ``energy_quadrature`` / ``energy_wadg``: closed-form values for states whose integrals are
+  known exactly (constant and linear fields, c^2 constant per element), and monotone decay of
+  the WADG weighted-mass energy (matlab/WADG/check_conservation_loss.m:51-54) under the
+  upwind flux with a Fourier c^2; an energy that weights p by c^2 instead of 1/c^2 fails;
+* ``l2_error`` and the product's ``solver.l2_error`` (``compute_error``,
+  cpp/src/WaveOKL3d.cpp:1683-1726): zero for a field of degree <= N, 1/12 for u = 0 against x;
+* ``update_elas`` (``rk_update_elas``, elas_cpp/okl/ElasKernelsWADG.okl:286-433): media
+  constant per element with lambda != mu != 1/rho make the WADG update an exact scaling (the
+  isotropic stiffness of :384-392); lambda and mu swapped fail it; the semi-discrete energy
+  rate <Q, T rhs(Q)>_E equals the media-free <Q, rhs(Q)>_M and is negative (swapped media break
+  the identity); the fully discrete elastic energy decays at lambda = 2, mu = 1, rho = 0.5.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from bbwadg_amd import media, mesh as M, operators as O
+from bbwadg_amd import solver as S
+import oracle as orc
+from cases import cavity
+from test_elastic_oracle import elastic_energy
+
+
+def _setup(N, n, nfields=4):
+    ops = O.build_ref_ops(N)
+    m = M.cube_mesh(n)
+    M.face_pair(m)
+    dm = M.device_arrays(m, ops, nfields)
+    return ops, m, dm
+
+
+def _interp(ops, m, f):
+    x, y, z = M.affine_points(m, ops.r, ops.s, ops.t)
+    return (ops.invVB @ f(x, y, z)).T
+
+
+def cavity_error(N, n, T=0.2, dt_scale=0.5):
+    ops, m, dm = _setup(N, n)
+    Q = np.zeros((m.K, 4, ops.Np))
+    Q[:, 0] = _interp(ops, m, lambda x, y, z: cavity(x, y, z, 0.0))
+    c2q = np.ones((m.K, ops.Nq))
+    nsteps = int(math.ceil(T / (dt_scale * (1.0 / n) / N ** 2)))
+    orc.rk_run(orc.OracleOps(ops, np.float64), m.K, dm.vgeo, dm.fgeo, dm.vmapP, c2q, Q, T / nsteps,
+               nsteps, nproj=1)
+    return orc.l2_error(ops, *media.cubature_points(m, ops), dm.J, Q, cavity, T)
+
+
+@pytest.mark.parametrize("N,meshes", [(3, (4, 8)), (4, (4, 8)), (5, (2, 4))])
+def test_rk_run_cavity_converges_at_order_N_plus_1(N, meshes):
+    (e1, r1), (e2, r2) = (cavity_error(N, n) for n in meshes)
+    rate = math.log2(e1 / e2)
+    assert rate > N + 1 - 0.3, (N, e1, e2, rate)
+    # the error against the exact standing wave is small in absolute terms
+    assert r2 < {3: 2e-4, 4: 2e-5, 5: 2e-5}[N], r2
+
+
+def _source_run(nsteps, T, src_time, N=2, n=2):
+    ops, m, dm = _setup(N, n)
+    c2q = media.c2_fourier(1)(*media.cubature_points(m, ops))
+    el, co = media.gaussian_source(m, ops, (0.1, -0.05, 0.02), a=5.0)
+    src = np.zeros((m.K, ops.Np))
+    src[el] = co
+    Q = np.zeros((m.K, 4, ops.Np))
+    orc.rk_run(orc.OracleOps(ops, np.float64), m.K, dm.vgeo, dm.fgeo, dm.vmapP, c2q, Q, T / nsteps,
+               nsteps, src=src, f0=10.0, amp=1.0, src_time=src_time)
+    return Q
+
+
+def _time_ratios(src_time, T=0.08, counts=(8, 16, 32, 64)):
+    sols = [_source_run(ns, T, src_time) for ns in counts]
+    d = [np.linalg.norm(sols[i] - sols[i + 1]) for i in range(len(sols) - 1)]
+    return [d[i] / d[i + 1] for i in range(len(d) - 1)]
+
+
+def test_rk_run_source_is_fourth_order_in_time():
+    ratios = _time_ratios("stage")
+    assert all(14.0 < r < 19.0 for r in ratios), ratios   # 2^4 = 16
+
+
+def test_planted_step_start_source_fails_the_time_order_check():
+    # the same driver with the source at the step start for all five stages is first order
+    ratios = _time_ratios("step")
+    assert not all(14.0 < r < 19.0 for r in ratios), ratios
+    assert max(ratios) < 4.0, ratios
+
+
+def _tet_x2_integrals(m):
+    """Exact integral of x^2 over each tetrahedron: V/10 (sum_i x_i^2 + sum_{i<j} x_i x_j)."""
+    X = m.GX
+    J = M.jacobian(m.GX, m.GY, m.GZ)
+    V = np.abs(J) * 4.0 / 3.0
+    s2 = (X ** 2).sum(axis=1)
+    sij = sum(X[:, i] * X[:, j] for i in range(4) for j in range(i + 1, 4))
+    return V / 10.0 * (s2 + sij)
+
+
+@pytest.mark.parametrize("N", [2, 3, 5])
+def test_energies_closed_form(N):
+    ops, m, dm = _setup(N, 2)
+    K = m.K
+    # p = 1, u = 0, c^2 = c0: E = 1/2 |domain| / c0
+    c0 = 0.8
+    c2q = np.full((K, ops.Nq), c0)
+    Q = np.zeros((K, 4, ops.Np))
+    Q[:, 0] = 1.0   # BB coefficients of the constant 1
+    for E in (orc.energy_quadrature(ops, dm.J, c2q, Q), orc.energy_wadg(ops, dm.J, c2q, Q)):
+        assert abs(E - 0.5 / c0) < 1e-12
+    # p = 0, u = (x, y, z): E = 1/2 * 3 * int x^2 = 1/8 on [-0.5, 0.5]^3
+    Q[:] = 0.0
+    for f, g in ((1, lambda x, y, z: x), (2, lambda x, y, z: y), (3, lambda x, y, z: z)):
+        Q[:, f] = _interp(ops, m, g)
+    for E in (orc.energy_quadrature(ops, dm.J, c2q, Q), orc.energy_wadg(ops, dm.J, c2q, Q)):
+        assert abs(E - 0.125) < 1e-12
+    # p = x, c^2 = c_K constant per element: E = 1/2 sum_K int_K x^2 / c_K
+    rng = np.random.default_rng(N)
+    cK = rng.uniform(0.6, 1.4, K)
+    c2q = np.repeat(cK[:, None], ops.Nq, axis=1)
+    Q[:] = 0.0
+    Q[:, 0] = _interp(ops, m, lambda x, y, z: x)
+    want = 0.5 * float(np.sum(_tet_x2_integrals(m) / cK))
+    for E in (orc.energy_quadrature(ops, dm.J, c2q, Q), orc.energy_wadg(ops, dm.J, c2q, Q)):
+        assert abs(E - want) < 1e-12 * want
+
+
+def test_planted_c2_weighting_fails_the_energy_closed_form():
+    ops, m, dm = _setup(3, 2)
+    c0 = 0.8
+    Q = np.zeros((m.K, 4, ops.Np))
+    Q[:, 0] = 1.0
+    # weighting p^2 by c^2 instead of 1/c^2 (passing 1/c^2 for c^2)
+    E = orc.energy_quadrature(ops, dm.J, np.full((m.K, ops.Nq), 1.0 / c0), Q)
+    assert abs(E - 0.5 / c0) > 0.1
+
+
+def test_wadg_energy_decays_under_upwind_flux():
+    N, n = 3, 3
+    ops, m, dm = _setup(N, n)
+    c2q = media.c2_fourier(2)(*media.cubature_points(m, ops))
+    bump = lambda x, y, z: np.exp(-20 * ((x - 0.1) ** 2 + y ** 2 + z ** 2))
+    Q = np.zeros((m.K, 4, ops.Np))
+    Q[:, 0] = _interp(ops, m, bump)
+    Q[:, 1] = 0.5 * Q[:, 0]
+    oo = orc.OracleOps(ops, np.float64)
+    dt = 0.5 * (1.0 / n) / N ** 2
+    res = np.zeros_like(Q)
+    E = [orc.energy_wadg(ops, dm.J, c2q, Q)]
+    for _ in range(20):
+        orc.rk_run(oo, m.K, dm.vgeo, dm.fgeo, dm.vmapP, c2q, Q, dt, 1, res=res)
+        E.append(orc.energy_wadg(ops, dm.J, c2q, Q))
+    E = np.array(E)
+    assert np.all(np.diff(E) < 0), np.diff(E)
+    assert E[-1] > 0.9 * E[0]
+
+
+@pytest.mark.parametrize("N", [2, 3, 4])
+def test_l2_error_closed_forms(N):
+    ops, m, dm = _setup(N, 2)
+    cub = media.cubature_points(m, ops)
+    poly = lambda x, y, z, t: 1.0 + 2.0 * x - y * z + x * x
+    Q = np.zeros((m.K, 4, ops.Np))
+    Q[:, 0] = _interp(ops, m, lambda x, y, z: poly(x, y, z, 0.0))
+    for err, rel in (orc.l2_error(ops, *cub, dm.J, Q, poly, 0.0),
+                     S.l2_error(ops, cub, dm.J, Q, poly, 0.0)):
+        assert err < 1e-12 and rel < 1e-12
+    Q[:] = 0.0
+    lin = lambda x, y, z, t: x
+    for err, rel in (orc.l2_error(ops, *cub, dm.J, Q, lin, 0.0), S.l2_error(ops, cub, dm.J, Q, lin, 0.0)):
+        assert abs(err - math.sqrt(1.0 / 12.0)) < 1e-13 and abs(rel - 1.0) < 1e-13
+    # the field index selects the field
+    Q[:, 2] = _interp(ops, m, lambda x, y, z: x)
+    assert S.l2_error(ops, cub, dm.J, Q, lin, 0.0, field=2)[0] < 1e-12
+    assert orc.l2_error(ops, *cub, dm.J, Q, lin, 0.0, field=2)[0] < 1e-12
+
+
+def _elastic_scaled(rhs, rhoinv, lam, mu):
+    """The isotropic stiffness applied to a right-hand side with coefficients constant per
+    element: v / rho; sigma_ii = (2 mu + lambda) e_ii + lambda (e_jj + e_kk); shear = mu gamma."""
+    out = np.empty_like(rhs)
+    r, l, u = rhoinv[:, None], lam[:, None], mu[:, None]
+    out[:, 0:3] = r[:, :, None] * rhs[:, 0:3]
+    e = rhs[:, 3:6]
+    tr = e.sum(axis=1)
+    for i in range(3):
+        out[:, 3 + i] = 2.0 * u * e[:, i] + l * tr
+    out[:, 6:9] = u[:, :, None] * rhs[:, 6:9]
+    return out
+
+
+def _elastic_update(ops, rhoinv, lam, mu, rhs, res, Q, fa, fb, fdt, swap=False):
+    Nq = ops.Nq
+    rq, lq, mq = (np.repeat(a[:, None], Nq, axis=1) for a in (rhoinv, lam, mu))
+    if swap:
+        lq, mq = mq, lq
+    res, Q = res.copy(), Q.copy()
+    orc.update_elas(orc.OracleOps(ops, np.float64), len(rhoinv), rq, lq, mq, fa, fb, fdt, rhs.copy(), res, Q)
+    return res, Q
+
+
+@pytest.mark.parametrize("N", [2, 4])
+def test_elastic_update_exact_scaling(N):
+    ops = O.build_ref_ops(N)
+    K = 7
+    rng = np.random.default_rng(N)
+    rhoinv, lam, mu = rng.uniform(1.5, 2.5, K), rng.uniform(2.0, 3.0, K), rng.uniform(0.5, 1.0, K)
+    rhs, res, Q = (rng.standard_normal((K, 9, ops.Np)) for _ in range(3))
+    fa, fb, fdt = -0.41, 0.37, 0.01
+    res1, Q1 = _elastic_update(ops, rhoinv, lam, mu, rhs, res, Q, fa, fb, fdt)
+    want_res = fa * res + fdt * _elastic_scaled(rhs, rhoinv, lam, mu)
+    assert np.abs(res1 - want_res).max() < 1e-12 * np.abs(want_res).max()
+    assert np.abs(Q1 - (Q + fb * want_res)).max() < 1e-12 * np.abs(Q).max()
+    # lambda and mu swapped: the check fails
+    res2, _ = _elastic_update(ops, rhoinv, lam, mu, rhs, res, Q, fa, fb, fdt, swap=True)
+    assert np.abs(res2 - want_res).max() > 1e-3 * np.abs(want_res).max()
+
+
+def _energy_inner(ops, J, rho, lam, mu, Q, R):
+    """The elastic energy inner product sum_K J sum_q w_q (rho v.r_v + sigma . C^-1 r_sigma) of
+    two coefficient states at the cubature points (C^-1 the isotropic compliance with
+    engineering shear)."""
+    Qq = np.einsum("qj,kfj->kfq", ops.Vq, Q)
+    Rq = np.einsum("qj,kfj->kfq", ops.Vq, R)
+    kin = rho * (Qq[:, 0:3] * Rq[:, 0:3]).sum(axis=1)
+    trR = Rq[:, 3:6].sum(axis=1)
+    cn = [(Rq[:, 3 + i] - lam / (2 * mu + 3 * lam) * trR) / (2 * mu) for i in range(3)]
+    pot = sum(Qq[:, 3 + i] * cn[i] for i in range(3)) + (Qq[:, 6:9] * Rq[:, 6:9]).sum(axis=1) / mu
+    return float(np.sum(J[:, None] * ops.wq[None, :] * (kin + pot)))
+
+
+@pytest.mark.parametrize("swap", [False, True])
+def test_elastic_semidiscrete_energy_rate(swap):
+    """dE/dt = <Q, T rhs(Q)>_E, with T the WADG update's media map, equals the unweighted
+    <Q, rhs(Q)>_M (the stiffness of the update cancels the compliance of the energy) and is
+    negative (penalty flux).  lambda and mu swapped in the update break the identity."""
+    N = 3
+    ops, m, dm = _setup(N, 2, nfields=9)
+    K = m.K
+    rng = np.random.default_rng(9)
+    cK = rng.uniform(0.5, 1.0, (3, K))
+    rhoinv, lam, mu = (np.repeat(1.0 / c[:, None] if i == 0 else 2.0 * c[:, None] if i == 1 else c[:, None],
+                                 ops.Nq, axis=1) for i, c in enumerate(cK))
+    oo = orc.OracleOps(ops, np.float64)
+    Q = rng.standard_normal((K, 9, ops.Np))
+    rhs = np.zeros_like(Q)
+    orc.volume(oo, K, dm.vgeo, Q, rhs, nfields=9)
+    orc.surface(oo, K, dm.fgeo, dm.vmapP, Q, rhs, nfields=9)
+    plain = float(np.einsum("kfi,ij,kfj,k->", Q, ops.MB, rhs, dm.J))
+    assert plain < 0
+    Tr, Qc = np.zeros_like(Q), Q.copy()
+    orc.update_elas(oo, K, rhoinv, mu if swap else lam, lam if swap else mu, 0.0, 0.0, 1.0, rhs.copy(), Tr, Qc)
+    rate = _energy_inner(ops, dm.J, 1.0 / rhoinv, lam, mu, Q, Tr)
+    if swap:
+        assert abs(rate - plain) > 1e-3 * abs(plain)
+    else:
+        assert abs(rate - plain) < 1e-10 * abs(plain)
+
+
+def test_elastic_energy_decays_lame_2_1_rho_half():
+    """Energy decay of the fully discrete scheme at lambda = 2, mu = 1, rho = 0.5, over windows
+    of four steps (the LSRK45 step is not strongly stable, so single steps may raise the energy
+    of a dissipative system slightly; the semi-discrete rate is pinned above)."""
+    N, n = 3, 3
+    ops, m, dm = _setup(N, n, nfields=9)
+    K = m.K
+    rhoinv = np.full((K, ops.Nq), 2.0)
+    lam = np.full((K, ops.Nq), 2.0)
+    mu = np.full((K, ops.Nq), 1.0)
+    rng = np.random.default_rng(5)
+    bump = lambda x, y, z: np.exp(-30 * (x ** 2 + (y - 0.1) ** 2 + z ** 2))
+    B = _interp(ops, m, bump)
+    Q = np.stack([rng.uniform(-1, 1) * B for _ in range(9)], axis=1)
+    oo = orc.OracleOps(ops, np.float64)
+    # elastic time step estimate of the reference (elas_cpp/src/WaveOKL3d.cpp:1296-1297) with
+    # the fastest wave speed sqrt((2 mu + lambda) / rho) = sqrt(8)
+    dt = 0.25 / ((N + 1) * (N + 3) / 3.0 * dm.FscaleMax) / math.sqrt(8.0)
+    res = np.zeros_like(Q)
+    E = [elastic_energy(ops, dm.J, rhoinv, lam, mu, Q)]
+    for _ in range(8):
+        orc.rk_run_elas(oo, K, dm.vgeo, dm.fgeo, dm.vmapP, rhoinv, lam, mu, Q, dt, 4, res=res)
+        E.append(elastic_energy(ops, dm.J, rhoinv, lam, mu, Q))
+    E = np.array(E)
+    assert np.all(np.diff(E) < 0), np.diff(E)
+    assert E[-1] > 0.3 * E[0]
