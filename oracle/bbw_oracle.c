@@ -37,7 +37,7 @@ void orc_volume_##SFX(int K, int Np, const R *vgeo, const int *Dids, const R *Dv
                       const R *Q, R *rhsQ)                                               \
 {                                                                                        \
   const int Nf = 4;                                                                      \
-  _Pragma("omp parallel for schedule(static)")                                           \
+  _Pragma("omp parallel for schedule(static) if(K >= 2048)")                                           \
   for (int k = 0; k < K; ++k) {                                                          \
     R sp[512], sUr[512], sUs[512], sUt[512];                                             \
     const R *G = vgeo + 9 * k;                                                           \
@@ -89,7 +89,7 @@ void orc_surface_##SFX(int K, int Np, int Nfp, const R *fgeo, const int *Fmask, 
                        const R *Q, R *rhsQ)                                              \
 {                                                                                        \
   const int Nf = 4, NfpNf = NFACES * Nfp;                                                \
-  _Pragma("omp parallel for schedule(static)")                                           \
+  _Pragma("omp parallel for schedule(static) if(K >= 2048)")                                           \
   for (int k = 0; k < K; ++k) {                                                          \
     R s_pflux[256], s_Uflux[256], s_ptmp[256], s_Utmp[256], s_nxyz[16];                  \
     for (int f = 0; f < NFACES; ++f) {                                                   \
@@ -152,7 +152,7 @@ void orc_update_wadg_##SFX(int K, int Np, int Nq, int nproj, const R *Vq, const 
                            const R *src, const R *rhsQ, R *resQ, R *Q)                   \
 {                                                                                        \
   const int Nf = 4;                                                                      \
-  _Pragma("omp parallel for schedule(static)")                                           \
+  _Pragma("omp parallel for schedule(static) if(K >= 2048)")                                           \
   for (int k = 0; k < K; ++k) {                                                          \
     R sq[4][256], r[4][256];                                                             \
     for (int q = 0; q < Nq; ++q) {                                                       \
@@ -190,7 +190,7 @@ void orc_update_wadg_##SFX(int K, int Np, int Nq, int nproj, const R *Vq, const 
                                                                                          \
 void orc_update_##SFX(long n, R fa, R fb, R fdt, const R *rhsQ, R *resQ, R *Q)           \
 {                                                                                        \
-  _Pragma("omp parallel for schedule(static)")                                           \
+  _Pragma("omp parallel for schedule(static) if(n >= 65536)")                                           \
   for (long i = 0; i < n; ++i) {                                                         \
     R res = resQ[i];                                                                     \
     res = fa * res + fdt * rhsQ[i];                                                      \
@@ -218,7 +218,7 @@ void orc_volume_elas_##SFX(int K, int Np, const R *vgeo, const int *Dids,       
                            const R *Dvals, const R *Q, R *rhsQ)                          \
 {                                                                                        \
   const int Nf = 9;                                                                      \
-  _Pragma("omp parallel for schedule(static)")                                           \
+  _Pragma("omp parallel for schedule(static) if(K >= 2048)")                                           \
   for (int k = 0; k < K; ++k) {                                                          \
     const R *G = vgeo + 9 * k;                                                           \
     const R *Qk = Q + (long)k * Nf * Np;                                                 \
@@ -258,7 +258,7 @@ void orc_surface_elas_##SFX(int K, int Np, int Nfp, const R *fgeo, const int *Fm
                             int L0_nnz, R tau_v, R tau_s, const R *Q, R *rhsQ)           \
 {                                                                                        \
   const int Nf = 9, NfpNf = NFACES * Nfp;                                                \
-  _Pragma("omp parallel for schedule(static)")                                           \
+  _Pragma("omp parallel for schedule(static) if(K >= 2048)")                                           \
   for (int k = 0; k < K; ++k) {                                                          \
     R flux[9][256], tmp[9][256];                                                         \
     for (int i = 0; i < NfpNf; ++i) {                                                    \
@@ -324,7 +324,7 @@ void orc_update_elas_##SFX(int K, int Np, int Nq, const R *Vq, const R *Pq,     
                            const R *rhsQ, R *resQ, R *Q)                                 \
 {                                                                                        \
   const int Nf = 9;                                                                      \
-  _Pragma("omp parallel for schedule(static)")                                           \
+  _Pragma("omp parallel for schedule(static) if(K >= 2048)")                                           \
   for (int k = 0; k < K; ++k) {                                                          \
     R sQ[9][256];                                                                        \
     for (int i = 0; i < Nq; ++i) {                                                       \

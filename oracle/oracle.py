@@ -14,6 +14,8 @@ reference's OKL kernels in FP64 and FP32) and adds, in numpy FP64:
 * ``energy_quadrature`` -- 1/2 sum_K J sum_q w_q (p_q^2 / c^2_q + |u_q|^2);
 * ``energy_wadg``       -- the exact WADG energy 1/2 sum_K [p^T M (M_{c^2})^-1 M p + u^T M u]
   (weighted mass of ``matlab/WADG/check_conservation_loss.m:51-54``);
+* ``energy_quadrature_elastic`` / ``energy_wadg_elastic`` -- the same two forms for the
+  elastic fields (rho |v|^2 + sigma : C^-1 sigma; weighted mass of the elastic WADG update);
 * ``l2_error``          -- ``compute_error`` (``cpp/src/WaveOKL3d.cpp:1683-1726``) with the
   relative error computed as sqrt(err2)/sqrt(norm2) (the reference takes an
   extra square root, ``:1722``).
@@ -201,6 +203,57 @@ def energy_wadg(ops, J, c2q, Q):
     Ep = np.sum(Mp * y, axis=1) * J
     Eu = np.einsum("kfi,ij,kfj->k", Qr[:, 1:], MB, Qr[:, 1:]) * J
     return 0.5 * float(np.sum(Ep + Eu))
+
+
+def elastic_density(ops, rhoinvq, lamq, muq, Uq):
+    """rho |v|^2 + sigma : C^-1 sigma at the cubature points (C^-1 the isotropic compliance;
+    shear stresses with engineering strains, sigma_yz = mu gamma_yz)."""
+    rho = 1.0 / np.asarray(rhoinvq, dtype=np.float64)
+    lam = np.asarray(lamq, dtype=np.float64)
+    mu = np.asarray(muq, dtype=np.float64)
+    kin = rho * (Uq[:, 0] ** 2 + Uq[:, 1] ** 2 + Uq[:, 2] ** 2)
+    tr = Uq[:, 3] + Uq[:, 4] + Uq[:, 5]
+    normal = (Uq[:, 3] ** 2 + Uq[:, 4] ** 2 + Uq[:, 5] ** 2 - lam / (2 * mu + 3 * lam) * tr ** 2) / (2 * mu)
+    shear = (Uq[:, 6] ** 2 + Uq[:, 7] ** 2 + Uq[:, 8] ** 2) / mu
+    return kin + normal + shear
+
+
+def energy_quadrature_elastic(ops, J, rhoinvq, lamq, muq, Q):
+    """1/2 sum_K J sum_q w_q (rho |v|^2 + sigma : C^-1 sigma), FP64."""
+    K = len(J)
+    Uq = np.einsum("qj,kfj->kfq", ops.Vq, np.asarray(Q, dtype=np.float64).reshape(K, 9, -1))
+    dens = elastic_density(ops, rhoinvq, lamq, muq, Uq)
+    return 0.5 * float(np.sum(np.asarray(J)[:, None] * ops.wq[None, :] * dens))
+
+
+def energy_wadg_elastic(ops, J, rhoinvq, lamq, muq, Q):
+    """Weighted-mass energy of the elastic WADG scheme: the update applies
+    T = Pq diag(m) Vq (m = 1/rho on v, the isotropic stiffness on the normal stresses, mu on the
+    shear stresses), so the scheme's mass matrix is M A_m^-1 M with A_m = Vq^T W m Vq and
+    E = 1/2 sum_K J (MB x)^T A_m^-1 (MB x) over the velocity, normal-stress and shear-stress
+    blocks (dense FP64 solves per element)."""
+    K, Np = len(J), ops.Np
+    Qr = np.asarray(Q, dtype=np.float64).reshape(K, 9, Np)
+    V, w = ops.Vq, ops.wq
+    MB = V.T @ (w[:, None] * V)
+    rhoinv, lam, mu = (np.asarray(a, dtype=np.float64) for a in (rhoinvq, lamq, muq))
+    E = 0.0
+    for k in range(K):
+        def scalar_block(c, fields):
+            A = V.T @ ((w * c)[:, None] * V)
+            B = MB @ Qr[k, fields].T
+            return float(np.sum(B * np.linalg.solve(A, B)))
+        e = scalar_block(rhoinv[k], [0, 1, 2]) + scalar_block(mu[k], [6, 7, 8])
+        # normal block: C = lambda 1 1^T + 2 mu I couples the three normal stresses per point
+        A = np.zeros((3 * Np, 3 * Np))
+        for a in range(3):
+            for b in range(3):
+                cab = lam[k] + (2 * mu[k] if a == b else 0.0)
+                A[a * Np:(a + 1) * Np, b * Np:(b + 1) * Np] = V.T @ ((w * cab)[:, None] * V)
+        B = np.concatenate([MB @ Qr[k, 3 + a] for a in range(3)])
+        e += float(B @ np.linalg.solve(A, B))
+        E += J[k] * e
+    return 0.5 * E
 
 
 def l2_error(ops, xq, yq, zq, J, Q, exact, time, field=0, nfields=4):

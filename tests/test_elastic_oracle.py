@@ -35,16 +35,9 @@ def test_bb_elastic_rhs_equals_nodal(N):
 
 
 def elastic_energy(ops, J, rhoinvq, lamq, muq, Q):
-    """1/2 sum_K J sum_q w_q (rho |v|^2 + sigma : C^-1 sigma) at the WADG cubature."""
-    K = len(J)
-    Uq = np.einsum("qj,kfj->kfq", ops.Vq, np.asarray(Q, dtype=np.float64).reshape(K, 9, -1))
-    rho = 1.0 / rhoinvq
-    kin = rho * (Uq[:, 0] ** 2 + Uq[:, 1] ** 2 + Uq[:, 2] ** 2)
-    # isotropic compliance: eps = (sigma - lam/(2mu+3lam) tr(sigma) I) / (2 mu)
-    tr = Uq[:, 3] + Uq[:, 4] + Uq[:, 5]
-    dev = Uq[:, 3] ** 2 + Uq[:, 4] ** 2 + Uq[:, 5] ** 2 - lamq / (2 * muq + 3 * lamq) * tr ** 2
-    pot = (dev + 2 * (Uq[:, 6] ** 2 + Uq[:, 7] ** 2 + Uq[:, 8] ** 2)) / (2 * muq)
-    return 0.5 * float(np.sum(J[:, None] * ops.wq[None, :] * (kin + pot)))
+    """1/2 sum_K J sum_q w_q (rho |v|^2 + sigma : C^-1 sigma) at the WADG cubature (the
+    oracle's quadrature form; pinned in tests/test_oracle.py)."""
+    return orc.energy_quadrature_elastic(ops, J, rhoinvq, lamq, muq, Q)
 
 
 def test_elastic_energy_decays_constant_media():

@@ -47,8 +47,10 @@ def test_gradient_matches_fp64_host_adjoint():
         Jr += Ji
         gr += gi
     assert J > 0
-    assert abs(J - Jr) < 1e-4 * Jr
-    assert rel_l2(g, gr) < 1e-3
+    assert abs(J - Jr) < 1e-5 * Jr
+    err = rel_l2(g, gr)
+    print(f"adjoint N=3 gradient vs FP64 host adjoint: rel L2 {err:.2e}", flush=True)
+    assert err < 3e-5   # measured 3.1e-6 on MI355X
 
 
 def test_checkpoint_interval_is_exact_and_fd_agrees():

@@ -360,3 +360,26 @@ def test_exact_bbwadg_update(N):
     assert rel(got[:, 0], ref) < 2e-5
     assert np.array_equal(got[:, 1:], R[:, 1:])
     s.close()
+
+
+def test_receivers_set_after_graph_steps():
+    """Graph steps without receivers, then receivers set on the same solver: the graph's window
+    grows with the receiver count (its rows are written by captured nodes) and the traces equal
+    an eager run's."""
+    from bbwadg_amd import media
+    print("case receivers after graph steps", flush=True)
+    P, s = make(3, 3)
+    Q0 = P.random_state()
+    dt = P.dt_cfl()
+    s.WaveSetData3d(Q0); s.set_res(np.zeros_like(Q0))
+    s.step(10, dt, mode=MODE_FUSED, graph=True)          # captures a graph with no receiver rows
+    relems, B = media.receivers(P.mesh, P.ops, media.surface_receivers(5, 40))
+    traces = []
+    for graph in (True, False):
+        s.set_receivers(relems, B, 20)
+        s.WaveSetData3d(Q0); s.set_res(np.zeros_like(Q0))
+        s.step(20, dt, mode=MODE_FUSED, graph=graph)
+        traces.append(s.get_traces(20))
+    assert np.array_equal(traces[0], traces[1])
+    assert np.abs(traces[0]).max() > 0
+    s.close()
