@@ -182,7 +182,7 @@ void bbw_destroy(bbw_ctx *c) {
   if (c->stream) hipStreamSynchronize(c->stream);
   reset_graph(c);
   dfree(c->Dids); dfree(c->Dvals); dfree(c->Fmask); dfree(c->EEL_ids); dfree(c->L0_ids);
-  dfree(c->EEL_vals); dfree(c->L0_vals); dfree(c->Vq); dfree(c->Pq); dfree(c->Vq64); dfree(c->VqPad); dfree(c->PqPad); dfree(c->EEL_ids4); dfree(c->LiftPad); dfree(c->VqBlk); dfree(c->PqBlk); dfree(c->LiftBlk); dfree(c->PqTBlk); dfree(c->VqTBlk); dfree(c->LiftTBlk); dfree(c->PqTPad); dfree(c->VqTPad);
+  dfree(c->EEL_vals); dfree(c->L0_vals); dfree(c->Vq); dfree(c->Pq); dfree(c->Vq64); dfree(c->Pq64); dfree(c->VqPad); dfree(c->PqPad); dfree(c->EEL_ids4); dfree(c->LiftPad); dfree(c->VqBlk); dfree(c->PqBlk); dfree(c->LiftBlk); dfree(c->PqTBlk); dfree(c->VqTBlk); dfree(c->LiftTBlk); dfree(c->PqTPad); dfree(c->VqTPad);
   dfree(c->bb_CN); dfree(c->bb_PBlk); dfree(c->c2bb);
   dfree(c->DTids); dfree(c->DTvals); dfree(c->Sids); dfree(c->Eids); dfree(c->Evals); dfree(c->Swts); dfree(c->EELT_ids); dfree(c->EELT_vals); dfree(c->L0T_ids);
   dfree(c->L0T_vals); dfree(c->FmaskInv); dfree(c->mapP); dfree(c->lQ); dfree(c->lres);
@@ -454,7 +454,42 @@ int bbw_upload_operators(bbw_ctx *c, const bbw_ops *o) {
   }
   std::vector<double> v64((size_t)Nq * Np);
   for (size_t i = 0; i < v64.size(); ++i) v64[i] = o->Vq64 ? o->Vq64[i] : (double)o->Vq[i];
+  // FP64 projection for the energy reductions: Pq64 = MB^-1 Vq^T W with MB = Vq^T W Vq (the
+  // mass matrix, exact for the degree 2N + 1 rule), by a Cholesky factorisation of MB
+  std::vector<double> p64((size_t)Np * Nq);
+  {
+    std::vector<double> L((size_t)Np * Np, 0.0);
+    for (int i = 0; i < Np; ++i)
+      for (int j = 0; j <= i; ++j) {
+        double m = 0.0;
+        for (int q = 0; q < Nq; ++q) m += v64[q + (size_t)i * Nq] * o->wq[q] * v64[q + (size_t)j * Nq];
+        for (int t = 0; t < j; ++t) m -= L[(size_t)i * Np + t] * L[(size_t)j * Np + t];
+        if (i == j) {
+          if (!(m > 0.0)) return fail(c, BBW_ERR_ARG, "Vq^T W Vq is not positive definite");
+          L[(size_t)i * Np + i] = std::sqrt(m);
+        } else {
+          L[(size_t)i * Np + j] = m / L[(size_t)j * Np + j];
+        }
+      }
+    std::vector<double> x(Np);
+    for (int q = 0; q < Nq; ++q) {
+      for (int i = 0; i < Np; ++i) x[i] = v64[q + (size_t)i * Nq] * o->wq[q];
+      for (int i = 0; i < Np; ++i) {
+        for (int t = 0; t < i; ++t) x[i] -= L[(size_t)i * Np + t] * x[t];
+        x[i] /= L[(size_t)i * Np + i];
+      }
+      for (int i = Np - 1; i >= 0; --i) {
+        for (int t = i + 1; t < Np; ++t) x[i] -= L[(size_t)t * Np + i] * x[t];
+        x[i] /= L[(size_t)i * Np + i];
+      }
+      for (int i = 0; i < Np; ++i) p64[i + (size_t)q * Np] = x[i];
+    }
+  }
+  dfree(c->Vq64);
+  dfree(c->Pq64);
+  dfree(c->wq64);
   if ((rc = upload(c, &c->Vq64, v64.data(), v64.size()))) return rc;
+  if ((rc = upload(c, &c->Pq64, p64.data(), p64.size()))) return rc;
   if ((rc = upload(c, &c->wq64, o->wq, (size_t)Nq))) return rc;
   std::memcpy(c->rk4a, o->rk4a, 5 * sizeof(double));
   std::memcpy(c->rk4b, o->rk4b, 5 * sizeof(double));
@@ -726,14 +761,17 @@ int bbw_step(bbw_ctx *c, int nsteps, double dt, double t0, int mode, int flags) 
     return fail(c, BBW_ERR_UNSUPPORTED, "fused stage is acoustic only");
   if (c->nrec && c->rec_steps + nsteps > c->rec_max_steps)
     return fail(c, BBW_ERR_ARG, "receiver trace buffer too short");
+  const bool prepare = (flags & 4) != 0;
+  flags &= 3;
   const int projU = flags & 1;
   const bool use_graph = (flags & 2) != 0;
   const int G = 10;  // steps per captured graph (even: the fused ping-pong parity is restored)
+  if (prepare && !use_graph) return fail(c, BBW_ERR_ARG, "graph preparation needs the graph flag");
 
   // stage time-function values for all steps (host FP64 -> float)
   const bool has_src = c->nsrc > 0;
   std::vector<float> ft;
-  if (has_src) {
+  if (has_src && !prepare) {
     ft.resize((size_t)nsteps * 5);
     for (int s = 0; s < nsteps; ++s)
       for (int i = 0; i < 5; ++i)
@@ -741,7 +779,7 @@ int bbw_step(bbw_ctx *c, int nsteps, double dt, double t0, int mode, int flags) 
   }
 
   int s = 0;
-  if (use_graph && nsteps >= G) {
+  if (use_graph && (nsteps >= G || prepare)) {
     // window buffer: the graph nodes read their stage values and write receiver rows there.  It
     // is sized for the current receiver count; a larger one (receivers set after a graph step)
     // reallocates it and recaptures the graph, which holds its addresses.
@@ -787,7 +825,7 @@ int bbw_step(bbw_ctx *c, int nsteps, double dt, double t0, int mode, int flags) 
     if (has_src) {
       // the call's stage values go to the device once, through a persistent pinned staging
       // buffer; the buffers grow only when a call has more steps than every earlier one
-      const size_t nft = ft.size();
+      const size_t nft = (size_t)nsteps * 5;
       if (!c->ft_evt) HIPCHK(c, hipEventCreateWithFlags(&c->ft_evt, hipEventDisableTiming));
       if (c->ft_evt_pending) {
         HIPCHK(c, hipEventSynchronize(c->ft_evt));  // the last copy out of ft_host has ended
@@ -805,6 +843,15 @@ int bbw_step(bbw_ctx *c, int nsteps, double dt, double t0, int mode, int flags) 
         HIPCHK(c, hipHostMalloc((void **)&c->ft_host, nft * sizeof(float), hipHostMallocDefault));
         c->ft_host_cap = nft;
       }
+    }
+    if (prepare) {
+      // graph and buffers ready for a call of nsteps steps with these arguments; no step is run
+      HIPCHK(c, hipGraphUpload(g.exec, c->stream));
+      HIPCHK(c, hipStreamSynchronize(c->stream));
+      return BBW_OK;
+    }
+    if (has_src) {
+      const size_t nft = ft.size();
       std::memcpy(c->ft_host, ft.data(), nft * sizeof(float));
       HIPCHK(c, hipMemcpyAsync(c->ft_all, c->ft_host, nft * sizeof(float), hipMemcpyHostToDevice, c->stream));
       HIPCHK(c, hipEventRecord(c->ft_evt, c->stream));
@@ -836,26 +883,21 @@ int bbw_step(bbw_ctx *c, int nsteps, double dt, double t0, int mode, int flags) 
   return BBW_OK;
 }
 
-int bbw_energy(bbw_ctx *c, double *E) {
+static int energy_common(bbw_ctx *c, int kind, double *E) {
   if (!c || !E) return BBW_ERR_ARG;
   int rc = check_ready(c);
   if (rc) return rc;
-  if (c->Nf != 4) return fail(c, BBW_ERR_UNSUPPORTED, "energy is acoustic only");
-  int nb = 0;
-  launch_energy(c, nullptr, &nb);
-  if (c->npartial < nb + 1) {
-    dfree(c->partial);
-    if ((rc = dalloc(c, &c->partial, (size_t)nb + 1))) return rc;
-    c->npartial = nb + 1;
-  }
-  hipError_t e = launch_energy(c, c->partial, &nb);
+  if ((rc = dalloc(c, &c->partial, (size_t)BBW_ENERGY_BLOCKS + 1))) return rc;
+  hipError_t e = launch_energy(c, kind, c->partial, c->partial + BBW_ENERGY_BLOCKS);
   if (e != hipSuccess) return hipfail(c, e, "energy launch");
-  e = launch_reduce(c, c->partial, nb, c->partial + nb);
-  if (e != hipSuccess) return hipfail(c, e, "reduce launch");
-  HIPCHK(c, hipMemcpyAsync(E, c->partial + nb, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipMemcpyAsync(E, c->partial + BBW_ENERGY_BLOCKS, sizeof(double), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   return BBW_OK;
 }
+
+int bbw_energy(bbw_ctx *c, double *E) { return energy_common(c, BBW_ENERGY_QUADRATURE, E); }
+
+int bbw_energy_wadg(bbw_ctx *c, double *E) { return energy_common(c, BBW_ENERGY_WADG, E); }
 
 int bbw_upload_adjoint_maps(bbw_ctx *c, const int32_t *mapP) {
   if (!c || !mapP) return BBW_ERR_ARG;

@@ -452,53 +452,6 @@ __global__ __launch_bounds__(256) void k_fused(int K, int KB, const float *__res
   }
 }
 
-// ------------------------------------------------------------------ energy (FP64)
-template <int N>
-__global__ __launch_bounds__(256) void k_energy(int K, int KB, const double *__restrict__ Vq64,
-                                                const double *__restrict__ wq,
-                                                const double *__restrict__ J,
-                                                const float *__restrict__ c2q,
-                                                const float *__restrict__ Q,
-                                                double *__restrict__ partial) {
-  constexpr int Np = Dims<N>::Np, Nq = Dims<N>::Nq, Nf = 4;
-  __shared__ double red[256];
-  const int k0 = blockIdx.x * KB;
-  const int nk = min(KB, K - k0);
-  double acc = 0.0;
-  for (int w = threadIdx.x; w < nk * Nq; w += blockDim.x) {
-    const int e = w / Nq, q = w - e * Nq, k = k0 + e;
-    const float *Qk = Q + (size_t)k * Nf * Np;
-    double u[4] = {0, 0, 0, 0};
-    for (int j = 0; j < Np; ++j) {
-      const double v = Vq64[q + j * Nq];
-#pragma unroll
-      for (int f = 0; f < 4; ++f) u[f] += v * (double)Qk[f * Np + j];
-    }
-    acc += J[k] * wq[q] * (u[0] * u[0] / (double)c2q[(size_t)k * Nq + q] + u[1] * u[1] + u[2] * u[2] + u[3] * u[3]);
-  }
-  red[threadIdx.x] = acc;
-  __syncthreads();
-  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
-    if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) partial[blockIdx.x] = 0.5 * red[0];
-}
-
-__global__ __launch_bounds__(256) void k_reduce(const double *__restrict__ partial, int n,
-                                                double *__restrict__ out) {
-  __shared__ double red[256];
-  double acc = 0.0;
-  for (int i = threadIdx.x; i < n; i += blockDim.x) acc += partial[i];
-  red[threadIdx.x] = acc;
-  __syncthreads();
-  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
-    if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) *out = red[0];
-}
-
 __global__ void k_record(int nrec, int Np, int NfNp, const int *__restrict__ elems,
                          const float *__restrict__ basis, const float *__restrict__ Q,
                          float *__restrict__ row) {
@@ -582,18 +535,6 @@ static hipError_t fused_n(bbw_ctx *c, const float *Qin, float *Qout, float fa, f
   return hipGetLastError();
 }
 
-template <int N>
-static hipError_t energy_n(bbw_ctx *c, double *partial, int *nblocks) {
-  constexpr int Nq = Dims<N>::Nq;
-  const int KB = (1024 + Nq - 1) / Nq;
-  const int grid = (c->K + KB - 1) / KB;
-  *nblocks = grid;
-  if (!partial) return hipSuccess;
-  hipLaunchKernelGGL(k_energy<N>, dim3(grid), dim3(256), 0, c->stream, c->K, KB, c->Vq64,
-                     c->wq64, c->J, c->c2q, c->Q[c->cur], partial);
-  return hipGetLastError();
-}
-
 bool order_supported(int N) { return N >= 1 && N <= 6; }
 
 hipError_t launch_volume(bbw_ctx *c) {
@@ -655,23 +596,6 @@ hipError_t launch_fused(bbw_ctx *c, const float *Qin, float *Qout, float fa, flo
     case 6: return fused_n<6>(c, Qin, Qout, fa, fb, fdt, ftd, ft, projU);
   }
   return hipErrorInvalidValue;
-}
-
-hipError_t launch_energy(bbw_ctx *c, double *partial, int *nblocks) {
-  switch (c->N) {
-    case 1: return energy_n<1>(c, partial, nblocks);
-    case 2: return energy_n<2>(c, partial, nblocks);
-    case 3: return energy_n<3>(c, partial, nblocks);
-    case 4: return energy_n<4>(c, partial, nblocks);
-    case 5: return energy_n<5>(c, partial, nblocks);
-    case 6: return energy_n<6>(c, partial, nblocks);
-  }
-  return hipErrorInvalidValue;
-}
-
-hipError_t launch_reduce(bbw_ctx *c, const double *partial, int n, double *out) {
-  hipLaunchKernelGGL(k_reduce, dim3(1), dim3(256), 0, c->stream, partial, n, out);
-  return hipGetLastError();
 }
 
 hipError_t launch_record(bbw_ctx *c, const float *Q, float *row) {

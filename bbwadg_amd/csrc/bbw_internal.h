@@ -29,6 +29,10 @@ inline int nq_of(int N) {
   return (N >= 1 && N <= 6) ? t[N] : -1;
 }
 
+// Energy reductions: kinds and the fixed number of partial sums (one per block of the grid).
+enum { BBW_ENERGY_QUADRATURE = 0, BBW_ENERGY_WADG = 1 };
+constexpr int BBW_ENERGY_BLOCKS = 4096;
+
 // Source description passed to the update kernels.
 struct SrcArgs {
   int nsrc;
@@ -82,6 +86,7 @@ struct bbw_ctx {
   // (M = face nodes, K = nodes), face blocks of ceil(Nfp/16) * KB tiles
   float *PqTBlk = nullptr, *VqTBlk = nullptr, *LiftTBlk = nullptr;
   double *Vq64 = nullptr, *wq64 = nullptr;
+  double *Pq64 = nullptr;  // (Vq64^T W Vq64)^-1 Vq64^T W, column-major Np x Nq (energy reductions)
   double rk4a[5] = {0}, rk4b[5] = {0}, rk4c[6] = {0};
   bool have_ops = false;
 
@@ -114,9 +119,8 @@ struct bbw_ctx {
   int *rec_elems = nullptr;
   float *rec_basis = nullptr, *rec_trace = nullptr;
 
-  // energy scratch
+  // energy reductions: BBW_ENERGY_BLOCKS partial sums and the result
   double *partial = nullptr;
-  int npartial = 0;
 
   // adjoint: transposed operators (D_m^T rows [4][Np] int4/float4, EEL^T [WE][4Nfp], L0^T
   // [WL][Nfp]), node -> face-node slots [Np][3], face-node neighbour map [K][4Nfp], buffers
@@ -180,8 +184,10 @@ hipError_t launch_adj_stage(bbw_ctx *c, const float *Y, float fa, float fb, floa
 // lres_io bit 0: read the adjoint residual (else it is zero); bit 1: store it (else it is dead)
 hipError_t launch_adj_stage_tiled(bbw_ctx *c, const float *Y, float fa, float fb, float fdt, int lres_io = 3);
 hipError_t launch_adj_inject(bbw_ctx *c, const float *resid_row);
-hipError_t launch_energy(bbw_ctx *c, double *E_dev_partial, int *nblocks);
-hipError_t launch_reduce(bbw_ctx *c, const double *partial, int n, double *out);
+// energy of the current state (kind BBW_ENERGY_*): partial sums in partial[BBW_ENERGY_BLOCKS],
+// the total in *out (device memory)
+hipError_t launch_energy(bbw_ctx *c, int kind, double *partial, double *out);
+int energy_blocks(const bbw_ctx *c, int kind);   // partial sums the reduction writes
 hipError_t launch_record(bbw_ctx *c, const float *Q, float *row);
 hipError_t launch_update_bbwadg(bbw_ctx *c);
 

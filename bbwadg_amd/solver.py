@@ -224,6 +224,12 @@ class WaveSolver:
         check(self.ctx, self.lib.bbw_step(self.ctx, int(nsteps), float(dt), float(t0), int(mode), flags))
         self.time = t0 + nsteps * dt
 
+    def prepare_graph(self, nsteps, dt, mode=MODE_FUSED, project_velocity=False):
+        """Capture, instantiate and upload the step graph for these arguments and size the
+        stage-time buffers for a call of ``nsteps`` steps, without stepping."""
+        flags = (1 if project_velocity else 0) | 2 | 4
+        check(self.ctx, self.lib.bbw_step(self.ctx, int(nsteps), float(dt), float(self.time), int(mode), flags))
+
     def Wave_RK(self, FinalTime, dt, mode=MODE_PLAIN, **kw):
         """Steps of size dt to FinalTime, the last one shortened (``Wave_RK``)."""
         nfull = int(np.floor(FinalTime / dt + 1e-12))
@@ -235,9 +241,13 @@ class WaveSolver:
     def sync(self):
         check(self.ctx, self.lib.bbw_sync(self.ctx))
 
-    def energy(self):
+    def energy(self, form="quadrature"):
+        """FP64 energy of the current state: ``"quadrature"`` (p^2/c^2 + |u|^2, elastic
+        rho |v|^2 + sigma : C^-1 sigma at the cubature points) or ``"wadg"`` (the weighted-mass
+        energy the WADG scheme dissipates)."""
         E = ctypes.c_double()
-        check(self.ctx, self.lib.bbw_energy(self.ctx, ctypes.byref(E)))
+        fn = {"quadrature": self.lib.bbw_energy, "wadg": self.lib.bbw_energy_wadg}[form]
+        check(self.ctx, fn(self.ctx, ctypes.byref(E)))
         return E.value
 
     def compute_error(self, time, uex, Q=None, field=0):

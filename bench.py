@@ -154,7 +154,8 @@ def elastic_throughput(nsteps, mesh_n=32, N=4):
     stream = torch.cuda.Stream()
     s.set_stream(stream.cuda_stream)
     dt = s.dt_ref   # the reference's elastic time step estimate
-    s.step(GRAPH_STEPS, dt, mode=MODE_PLAIN)   # graph capture and instantiation
+    s.step(2, dt, mode=MODE_PLAIN)
+    s.prepare_graph(nsteps, dt, mode=MODE_PLAIN)   # graph capture and instantiation
     torch.cuda.synchronize()
     e0 = torch.cuda.Event(enable_timing=True)
     e1 = torch.cuda.Event(enable_timing=True)
@@ -287,10 +288,11 @@ def main():
         torch.cuda.synchronize()
         return e0.elapsed_time(e1)
 
-    # the warm-up runs at least one graph's worth of steps in the timed mode, so the graph is
-    # captured and instantiated (and the stage-time buffers allocated) before the timed region
-    warm = max(args.warmup, GRAPH_STEPS)
+    # warm-up steps, then the step graph is captured, instantiated and uploaded and the
+    # stage-time buffers sized for the timed call, so the timed region holds steps only
+    warm = args.warmup
     s.step(warm, dt, mode=MODE_FUSED, graph=True, t0=0.0)
+    s.prepare_graph(args.steps, dt, mode=MODE_FUSED)
     ms = timed(args.steps, MODE_FUSED)
     dof_stage = K * ops.Np * 4 * 5
     value = dof_stage * args.steps / (ms / 1e3)
@@ -303,6 +305,7 @@ def main():
     s.WaveSetData3d(Q0)
     s.set_res(np.zeros_like(Q0))
     s.step(warm, dt, mode=MODE_PLAIN, graph=True, t0=0.0)
+    s.prepare_graph(args.plain_steps, dt, mode=MODE_PLAIN)
     ms_plain = timed(args.plain_steps, MODE_PLAIN)
     plain_value = dof_stage * args.plain_steps / (ms_plain / 1e3)
 

@@ -148,11 +148,25 @@ int bbw_update_bbwadg(bbw_ctx *ctx);
 
 /* ≈ Wave_RK inner loop (cpp/src/WaveOKL3d.cpp:2029-2080): nsteps LSRK45 steps of size dt
  * from time t0; stage coefficients cast to float as the reference does.  mode: BBW_MODE_*.
- * flags bit 0: project velocity in the fused path; bit 1: capture steps in a HIP graph. */
+ * flags bit 0: project velocity in the fused path; bit 1: capture steps in a HIP graph;
+ * bit 2 (with bit 1, no reference counterpart): prepare only -- capture, instantiate and upload
+ * the graph for these arguments and the current state buffer, and size the stage-time buffers
+ * for a call of nsteps steps, without running a step (so that a later call's time excludes it). */
+#define BBW_STEP_PROJECT_U 1
+#define BBW_STEP_GRAPH 2
+#define BBW_STEP_PREPARE 4
 int bbw_step(bbw_ctx *ctx, int nsteps, double dt, double t0, int mode, int flags);
 
-/* FP64 energy 1/2 sum_K J sum_q w_q (p_q^2 / c2_q + |u_q|^2) (acoustic), reduced on device. */
+/* FP64 energies of the current state, reduced on the device (no C++ counterpart in the
+ * reference; the weighted-mass form follows matlab/WADG/check_conservation_loss.m:51-54).
+ *   bbw_energy       quadrature form 1/2 sum_K J sum_q w_q e_q with e = p^2 / c2 + |u|^2
+ *                    (acoustic) or rho |v|^2 + sigma : C^-1 sigma (elastic, isotropic compliance)
+ *   bbw_energy_wadg  the energy the WADG scheme dissipates: 1/2 sum_K J (MB x)^T A_m^-1 (MB x)
+ *                    per block of fields, A_m = Vq^T W m Vq with the block's media m (c2 on p,
+ *                    1 on u; 1/rho on v, the isotropic stiffness on the normal stresses, mu on
+ *                    the shear stresses), solved per element by conjugate gradients in FP64 */
 int bbw_energy(bbw_ctx *ctx, double *E);
+int bbw_energy_wadg(bbw_ctx *ctx, double *E);
 
 /* Discrete adjoint (no reference counterpart; acoustic, fused forward with Pq Vq = I on the
  * velocity).  Misfit J = 1/2 sum_n sum_r (p_r(Q_{n+1}) - d_obs[n][r])^2 over nsteps steps from the

@@ -383,3 +383,28 @@ def test_receivers_set_after_graph_steps():
     assert np.array_equal(traces[0], traces[1])
     assert np.abs(traces[0]).max() > 0
     s.close()
+
+
+def test_prepared_graph_steps_equal_eager_steps():
+    """bbw_step's prepare flag captures and uploads the graph (and sizes the stage-time buffers)
+    without stepping; the steps that follow equal eager steps bitwise, with a source."""
+    from bbwadg_amd import media
+    print("case prepared graph", flush=True)
+    P, s = make(3, 3)
+    el, co = media.gaussian_source(P.mesh, P.ops, (0.1, 0.0, -0.1), a=10.0)
+    s.set_source(el, co, f0=8.0, amp=2.0)
+    Q0 = P.random_state()
+    dt = P.dt_cfl()
+    mid, end = [], []
+    for graph in (False, True):
+        s.WaveSetData3d(Q0); s.set_res(np.zeros_like(Q0))
+        s.time = 0.0
+        s.step(3, dt, mode=MODE_FUSED, graph=graph)
+        mid.append(s.WaveGetData3d())
+        if graph:
+            s.prepare_graph(20, dt, mode=MODE_FUSED)
+            assert np.array_equal(s.WaveGetData3d(), mid[-1])   # prepare does not step
+        s.step(20, dt, mode=MODE_FUSED, graph=graph)
+        end.append(s.WaveGetData3d())
+    assert np.array_equal(mid[0], mid[1]) and np.array_equal(end[0], end[1])
+    s.close()
