@@ -27,6 +27,7 @@ EXPORTS = [
     "bbw_set_source", "bbw_set_receivers", "bbw_get_traces", "bbw_set_state", "bbw_get_state",
     "bbw_set_res", "bbw_get_res", "bbw_set_rhs", "bbw_get_rhs", "bbw_volume", "bbw_surface",
     "bbw_update_wadg", "bbw_update", "bbw_stage_fused", "bbw_step", "bbw_energy", "bbw_energy_wadg",
+    "bbw_energy_noalloc",
     "bbw_device_ptr", "bbw_set_option", "bbw_upload_adjoint_maps", "bbw_misfit",
     "bbw_adjoint_gradient", "bbw_batch_create", "bbw_batch_destroy", "bbw_batch_last_error",
     "bbw_batch_upload_operators", "bbw_batch_add_mesh", "bbw_batch_add_shot", "bbw_batch_run",
@@ -107,6 +108,7 @@ def load():
         "bbw_step": ([P, I, D, D, I, I], I),
         "bbw_energy": ([P, ctypes.POINTER(D)], I),
         "bbw_energy_wadg": ([P, ctypes.POINTER(D)], I),
+        "bbw_energy_noalloc": ([P, ctypes.POINTER(D)], I),
         "bbw_device_ptr": ([P, I, ctypes.POINTER(P)], I),
         "bbw_set_option": ([P, I, I], I),
         "bbw_upload_adjoint_maps": ([P, P], I),

@@ -149,6 +149,11 @@ int bbw_create(const bbw_config *cfg, bbw_ctx **out) {
     return BBW_ERR_HIP;
   }
   c->stream = c->own_stream;
+  if (energy_scratch_init(c->dev) != hipSuccess) {
+    hipStreamDestroy(c->own_stream);
+    delete c;
+    return BBW_ERR_HIP;
+  }
   if (hipDeviceGetAttribute(&c->ncu, hipDeviceAttributeMultiprocessorCount, c->dev) != hipSuccess || c->ncu <= 0) {
     hipStreamDestroy(c->own_stream);
     delete c;
@@ -898,6 +903,23 @@ static int energy_common(bbw_ctx *c, int kind, double *E) {
 int bbw_energy(bbw_ctx *c, double *E) { return energy_common(c, BBW_ENERGY_QUADRATURE, E); }
 
 int bbw_energy_wadg(bbw_ctx *c, double *E) { return energy_common(c, BBW_ENERGY_WADG, E); }
+
+int bbw_energy_noalloc(bbw_ctx *c, double *E) {
+  if (!c || !E) return BBW_ERR_ARG;
+  int rc = check_ready(c);
+  if (rc) return rc;
+  double *scr = energy_scratch(c->dev);
+  if (!scr) return fail(c, BBW_ERR_STATE, "energy scratch array not set up on this device");
+  // the reduction writes one partial sum per block of a grid capped at BBW_ENERGY_BLOCKS (so any
+  // K fits), and the array holds that many and the result
+  if (energy_blocks(c, BBW_ENERGY_QUADRATURE) > BBW_ENERGY_BLOCKS)
+    return fail(c, BBW_ERR_ARG, "energy reduction needs more partial sums than the scratch array holds");
+  hipError_t e = launch_energy(c, BBW_ENERGY_QUADRATURE, scr, scr + BBW_ENERGY_BLOCKS);
+  if (e != hipSuccess) return hipfail(c, e, "energy launch");
+  HIPCHK(c, hipMemcpyAsync(E, scr + BBW_ENERGY_BLOCKS, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return BBW_OK;
+}
 
 int bbw_upload_adjoint_maps(bbw_ctx *c, const int32_t *mapP) {
   if (!c || !mapP) return BBW_ERR_ARG;

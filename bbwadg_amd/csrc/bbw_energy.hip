@@ -16,6 +16,8 @@
 // Both reductions write one partial sum per block of a fixed grid (at most BBW_ENERGY_BLOCKS),
 // which one block then adds in a fixed order: the result does not depend on the launch, and the
 // partial-sum buffer has a fixed size whatever K is.
+#include <mutex>
+
 #include "bbw_internal.h"
 
 namespace bbw {
@@ -302,6 +304,40 @@ static hipError_t energy_n(bbw_ctx *c, int kind, double *partial, double *out) {
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(k_reduce, dim3(1), dim3(256), 0, c->stream, partial, grid, out);
   return hipGetLastError();
+}
+
+// Partial sums and result of bbw_energy_noalloc: a file-scope device array, one per GPU, held
+// until the process ends.  Host code never passes its name to a kernel (on the host it is a
+// stand-in, not a device address); it uses the address hipGetSymbolAddress returned for each
+// device, recorded at context creation.
+static __device__ __align__(256) double scratch[BBW_ENERGY_BLOCKS + 1];
+
+namespace {
+constexpr int MAX_DEVICES = 64;
+std::mutex scratch_mu;
+double *scratch_ptr[MAX_DEVICES] = {};
+}  // namespace
+
+hipError_t energy_scratch_init(int dev) {
+  if (dev < 0 || dev >= MAX_DEVICES) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lock(scratch_mu);
+  if (scratch_ptr[dev]) return hipSuccess;
+  int cur = -1;
+  hipError_t e = hipGetDevice(&cur);
+  if (e != hipSuccess) return e;
+  if (cur != dev && (e = hipSetDevice(dev)) != hipSuccess) return e;
+  void *p = nullptr;
+  e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (cur != dev) hipSetDevice(cur);
+  if (e != hipSuccess) return e;
+  scratch_ptr[dev] = (double *)p;
+  return hipSuccess;
+}
+
+double *energy_scratch(int dev) {
+  if (dev < 0 || dev >= MAX_DEVICES) return nullptr;
+  std::lock_guard<std::mutex> lock(scratch_mu);
+  return scratch_ptr[dev];
 }
 
 hipError_t launch_energy(bbw_ctx *c, int kind, double *partial, double *out) {

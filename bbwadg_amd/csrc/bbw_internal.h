@@ -188,6 +188,10 @@ hipError_t launch_adj_inject(bbw_ctx *c, const float *resid_row);
 // the total in *out (device memory)
 hipError_t launch_energy(bbw_ctx *c, int kind, double *partial, double *out);
 int energy_blocks(const bbw_ctx *c, int kind);   // partial sums the reduction writes
+// the device address of the energy scratch array (BBW_ENERGY_BLOCKS + 1 doubles) on device dev:
+// looked up once per device by energy_scratch_init (called by bbw_create), null before
+hipError_t energy_scratch_init(int dev);
+double *energy_scratch(int dev);
 hipError_t launch_record(bbw_ctx *c, const float *Q, float *row);
 hipError_t launch_update_bbwadg(bbw_ctx *c);
 

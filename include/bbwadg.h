@@ -167,6 +167,13 @@ int bbw_step(bbw_ctx *ctx, int nsteps, double dt, double t0, int mode, int flags
  *                    the shear stresses), solved per element by conjugate gradients in FP64 */
 int bbw_energy(bbw_ctx *ctx, double *E);
 int bbw_energy_wadg(bbw_ctx *ctx, double *E);
+/* bbw_energy without allocation.  No counterpart in the reference: it is for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc).  The reduction's partial sums and result go to a file-scope device
+ * array of BBW_ENERGY_BLOCKS + 1 = 4097 doubles (32 KiB), one per GPU, held until the process
+ * ends; the grid is capped at that many blocks, so any K fits.  Every call on a GPU shares that
+ * GPU's array: calls that overlap on one GPU (on different streams or threads) race. */
+int bbw_energy_noalloc(bbw_ctx *ctx, double *E);
 
 /* Discrete adjoint (no reference counterpart; acoustic, fused forward with Pq Vq = I on the
  * velocity).  Misfit J = 1/2 sum_n sum_r (p_r(Q_{n+1}) - d_obs[n][r])^2 over nsteps steps from the
