@@ -10,7 +10,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libbbwadg_hip.so")
-SOURCES = ["bbw_abi.hip", "bbw_acoustic.hip", "bbw_fused.hip", "bbw_elastic.hip", "bbw_adjoint.hip", "bbw_batch.hip", "bbw_energy.hip"]
+SOURCES = ["bbw_abi.hip", "bbw_acoustic.hip", "bbw_fused.hip", "bbw_elastic.hip", "bbw_adjoint.hip", "bbw_batch.hip", "bbw_energy.hip",
+           "bbw_fused_elas.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [

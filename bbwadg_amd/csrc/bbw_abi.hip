@@ -114,6 +114,13 @@ int do_stage(bbw_ctx *c, int i, double dt, const float *ftd, float ft, int mode,
       return hipfail(c, e, "update launch");
     return BBW_OK;
   }
+  if (mode == BBW_MODE_FUSED) {
+    const int store_res = !(i == 4 && more_steps && c->rk4a[0] == 0.0);
+    e = launch_fused_elas(c, c->Q[c->cur], c->Q[1 - c->cur], fa, fb, fdt, ftd, ft, store_res);
+    if (e != hipSuccess) return hipfail(c, e, "fused elastic stage launch");
+    c->cur = 1 - c->cur;
+    return BBW_OK;
+  }
   if ((e = launch_volume_elas(c)) != hipSuccess) return hipfail(c, e, "elastic volume launch");
   if ((e = launch_surface_elas(c)) != hipSuccess) return hipfail(c, e, "elastic surface launch");
   if ((e = launch_update_elas(c, fa, fb, fdt, ftd, ft)) != hipSuccess)
@@ -748,7 +755,12 @@ int bbw_stage_fused(bbw_ctx *c, float fa, float fb, float fdt, float ftime, int 
   if (!c) return BBW_ERR_ARG;
   int rc = check_ready(c);
   if (rc) return rc;
-  if (c->Nf != 4) return fail(c, BBW_ERR_UNSUPPORTED, "fused stage is acoustic only");
+  if (c->Nf == 9) {
+    hipError_t e = launch_fused_elas(c, c->Q[c->cur], c->Q[1 - c->cur], fa, fb, fdt, nullptr, ftime);
+    if (e != hipSuccess) return hipfail(c, e, "fused elastic stage launch");
+    c->cur = 1 - c->cur;
+    return BBW_OK;
+  }
   hipError_t e = c->fused_impl == BBW_FUSED_MFMA
                      ? launch_fused_mfma(c, c->Q[c->cur], c->Q[1 - c->cur], fa, fb, fdt, nullptr, ftime, projU)
                      : launch_fused(c, c->Q[c->cur], c->Q[1 - c->cur], fa, fb, fdt, nullptr, ftime, projU);
@@ -762,8 +774,6 @@ int bbw_step(bbw_ctx *c, int nsteps, double dt, double t0, int mode, int flags) 
     return BBW_ERR_ARG;
   int rc = check_ready(c);
   if (rc) return rc;
-  if (mode == BBW_MODE_FUSED && c->Nf != 4)
-    return fail(c, BBW_ERR_UNSUPPORTED, "fused stage is acoustic only");
   if (c->nrec && c->rec_steps + nsteps > c->rec_max_steps)
     return fail(c, BBW_ERR_ARG, "receiver trace buffer too short");
   const bool prepare = (flags & 4) != 0;

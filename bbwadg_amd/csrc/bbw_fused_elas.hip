@@ -1,0 +1,589 @@
+// Fused elastic stage for gfx950: one launch per LSRK stage of the velocity-stress system
+// (rk_volume_elas + rk_surface_elas + rk_update_elas, elas_cpp/okl/ElasKernelsWADG.okl:25-433,
+// with the Bernstein-Bezier operators), reading Q_in and writing Q_out, so the right-hand side
+// never goes through HBM.
+//
+// One workgroup (4 waves) owns a tile of TB = 16 consecutive elements (the N width of the FP32
+// 16x16x4 MFMA).  Fields: v1 v2 v3 sxx syy szz syz sxz sxy.
+//   1  loads: Q_in tile -> LDS (kept to the end), geometry, vmapP, then in registers: the
+//      neighbour traces of the thread's face nodes, the media at the cubature points (MFMA
+//      output layout), the residual rows of the thread's LSRK rows and the shift/elevation
+//      table rows
+//   2  penalty flux of the 9 fields at the thread's face nodes (Q+ = 0 on the boundary), in
+//      registers: the 3 velocity fluxes and the 3-vector w that determines the 6 stress fluxes
+//   then for each group g of three fields (velocities; normal stresses; shear stresses), whose
+//   cubature-point maps are independent (1/rho; the isotropic stiffness; mu):
+//   3  volume step 1: F_g at the degree-(N-1) nodes b from the shifted coefficients u[b + e_m]
+//      of the fields the group's outputs differentiate (div sigma for v; the strain rates of v
+//      for the stresses); the group's flux -> LDS as MFMA B operands
+//   4  volume step 2: rhs_g[a] = sum_j a_j F_g[a - e_j] (elevation) -> LDS
+//   5  lift on MFMA: rhs_g += LIFT_f flux_f per face (M = nodes, N = elements, K = face nodes);
+//      the six stress fluxes are the symmetric part of n (x) w with one 3-vector w per face node
+//      and the normal constant on a face, so the stresses lift w (3 products per face, shared by
+//      both stress groups) and apply the normals afterwards
+//   6  S = map_g(Vq rhs_g) on MFMA (M = cubature points, K = nodes)
+//   7  rhs_g = Pq S on MFMA (M = nodes, K = cubature points)
+//   8  LSRK for the group's rows: res = fa res + fdt (rhs + source); Q_out = Q_in + fb res
+// The derivative in factored form: D_m u[a] = sum_j a_j u[a - e_j + e_m], and every output is
+// linear in the D_m u, so the volume terms are an elevation of values formed at the degree-(N-1)
+// nodes.  The lift is LIFT = EEL blkdiag(L0) applied densely (as rk_surface_elas lifts densely).
+//
+// v_mfma_f32_16x16x4f32 lane layout: A[i][k] from lane i + 16k, B[k][j] from lane j + 16k,
+// D[4*(l/16) + r][l%16] in register r of lane l.
+#include "bbw_fused_tile.h"
+
+namespace bbw {
+
+template <int N>
+struct ElasFusedLayout {
+  static constexpr int Np = Dims<N>::Np, Nfp = Dims<N>::Nfp, NfpNf = Dims<N>::NfpNf, Nq = Dims<N>::Nq;
+  static constexpr int TB = 16, RowF = 9 * Np;
+  static constexpr int NQT = (Nq + 15) / 16, NQP = NQT * 16, NPT = (Np + 15) / 16;
+  static constexpr int KTV = (Np + 3) / 4, KTP = NQP / 4, QTW = (NQT + 3) / 4;
+  static constexpr int KBV = (KTV + 3) / 4, KBP = (KTP + 3) / 4;
+  static constexpr int KF = 4 * ((Nfp + 3) / 4), KTL = KF / 4, KBL = (KTL + 3) / 4;
+  static constexpr int MTW = (NPT + 3) / 4;   // node tiles per wave in the lift and Pq products
+  // a group's rhs rows: 3 fields of Np plus the K padding the Vq product reads past the third,
+  // row stride rounded up to 2 mod 32 (MFMA operand reads: 16 rows x 2 consecutive words)
+  static constexpr int RG0 = 3 * Np > 2 * Np + 4 * KTV ? 3 * Np : 2 * Np + 4 * KTV;
+  static constexpr int RG = RG0 + ((34 - RG0 % 32) % 32);
+  static constexpr int KB = NfpNf + 4, BS = TB + 1;  // flux rows (zero tail) and their stride
+  // thread mappings
+  static constexpr int NG3 = 256 / NfpNf, EPT3 = (TB + NG3 - 1) / NG3;   // face nodes
+  static constexpr int GS = ((Np + 63) / 64) * 64, NG = 256 / GS, EPTV = (TB + NG - 1) / NG;  // nodes
+  static constexpr int Np1 = N * (N + 1) * (N + 2) / 6;
+  static constexpr int GS1 = Np1 <= 32 ? 32 : 64, NG1 = 256 / GS1, EPT1 = TB / NG1;   // degree N-1 nodes
+  static_assert(Np1 <= 64 && NG1 * EPT1 == TB, "volume step 1 covers the tile");
+  // LSRK rows of a group: 3 Np rows x TB elements; HE element halves when 2 * 3 Np <= 256
+  static constexpr int LR = 3 * Np, HE = 2 * LR <= 256 ? 2 : 1, EPL = TB / HE;
+  static_assert(LR <= 256, "one LSRK row per thread");
+  static constexpr bool PFR = HE == 2;   // residual rows prefetched into registers
+  // LDS: Q_in tile padded to whole 4 KB rounds | geometry vgeo [TB][9], fgeo [TB][16] |
+  // F float4 [TB][Np1] | rhs_g [TB][RG] | W: flux [3][KB][BS] or S [3][NQP][16] | slots [TB]
+  static constexpr unsigned QB = TB * RowF * 4u, QIT = (QB + 4095) / 4096;
+  static constexpr int QTP = (int)QIT * 1024;
+  static constexpr int OQ = 0, OG = QTP, OF = OG + TB * 25, OR = OF + 4 * TB * Np1;
+  static constexpr int WSZ = 3 * KB * BS > 3 * NQP * 16 ? 3 * KB * BS : 3 * NQP * 16;
+  static constexpr int OW = OR + TB * RG, OSL = OW + WSZ;
+  static constexpr size_t floats = (size_t)OSL + TB;
+  static_assert(3 * (KB - NfpNf) * BS <= 256, "one flux tail word per thread");
+  static_assert(TB * (RG0 - 3 * Np) <= 256, "one rhs padding word per thread");
+};
+
+struct ElasMedia {
+  const float *rhoinv, *lambda, *mu;
+  float tau_v, tau_s;
+};
+
+template <int N>
+__global__ __launch_bounds__(256, N <= 4 ? 3 : N == 5 ? 2 : 1) void k_fused_elas(FusedMesh m, FusedOps o, ElasMedia md, const float *Qin,
+                                                        float *Qout, float *res, SrcArgs src, float fa, float fb,
+                                                        float fdt, float ft, const float *ftime_dev, int store_res) {
+  using L = ElasFusedLayout<N>;
+  constexpr int Np = L::Np, Nfp = L::Nfp, NfpNf = L::NfpNf, Nq = L::Nq, TB = L::TB, RowF = L::RowF;
+  constexpr int RG = L::RG, KB = L::KB, BS = L::BS, NQP = L::NQP, NPT = L::NPT, Np1 = L::Np1;
+  constexpr int KTV = L::KTV, KTP = L::KTP, KBV = L::KBV, KBP = L::KBP, KTL = L::KTL, KBL = L::KBL;
+  constexpr unsigned ROW = RowF * 4u;
+  extern __shared__ float lds[];
+  float *sQ = lds + L::OQ;                 // [TB][RowF]
+  float *sG = lds + L::OG;                 // vgeo [TB][9]
+  float *sFg = sG + TB * 9;                // fgeo [TB][16]
+  float4 *sF = (float4 *)(lds + L::OF);    // [TB][Np1]
+  float *sR = lds + L::OR;                 // [TB][RG]
+  float *sB = lds + L::OW;                 // [3][KB][BS]  (phases 3-5)
+  float *S = lds + L::OW;                  // [3][NQP][16] (phases 6-7)
+  int *sSlot = (int *)(lds + L::OSL);
+  const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int col = lane & 15, kr = lane >> 4;
+  const int k0 = blockIdx.x * TB, nk = min(TB, m.K - k0);
+  const unsigned tbytes = (unsigned)nk * ROW;
+  if (ftime_dev) ft = *ftime_dev;
+  const bool has_src = src.nsrc > 0 && k0 + nk - 1 >= src.first && k0 <= src.last;
+
+  // ---- 1: loads.  vmapP first (the trace gather waits for it), the Q_in tile and geometry,
+  // the table rows, media, residual rows, then the traces; the LDS stores after all of them.
+  const int g3 = tid / NfpNf, m3 = tid - g3 * NfpNf;
+  const bool on3 = g3 < L::NG3;
+  const int f3 = m3 / Nfp;
+  const int fid3 = o.Fmask[m3 < NfpNf ? m3 : 0];
+  int vm[L::EPT3];
+  {
+    const brsrc rv = mkrsrc(m.vmapP + (size_t)k0 * NfpNf, (unsigned)nk * NfpNf * 4u);
+#pragma unroll
+    for (int i = 0; i < L::EPT3; ++i) vm[i] = bldi(rv, (unsigned)((g3 + L::NG3 * i) * NfpNf + m3) * 4u);
+  }
+  float4 qreg[L::QIT];
+  {
+    const brsrc rQ = mkrsrc(Qin + (size_t)k0 * RowF, tbytes);
+#pragma unroll
+    for (unsigned it = 0; it < L::QIT; ++it) qreg[it] = bld4(rQ, tid * 16u + it * 4096u);
+  }
+  const float gv0 = bld(mkrsrc(m.vgeo + (size_t)k0 * 9, (unsigned)nk * 36u), tid * 4u);
+  const float gf0 = bld(mkrsrc(m.fgeo + (size_t)k0 * 16, (unsigned)nk * 64u), tid * 4u);
+  const int gv = tid / L::GS, nv = tid - gv * L::GS;
+  const bool onv = gv < L::NG && nv < Np;
+  const int g1 = tid / L::GS1, n1 = tid - g1 * L::GS1;
+  const bool on1 = n1 < Np1;
+  const int4 Sr = o.Sids[on1 ? n1 : 0];
+  const int4 Er = o.Eids[nv < Np ? nv : 0];
+  const float4 evr = o.Evals[nv < Np ? nv : 0];
+  // media in the MFMA output layout of the Vq product (rows q >= Nq read a neighbouring value;
+  // the padded Vq rows make them 0)
+  float rho[L::QTW][4], lam[L::QTW][4], mu[L::QTW][4];
+  {
+    const unsigned qbytes = (unsigned)nk * Nq * 4u;
+    const brsrc rRho = mkrsrc(md.rhoinv + (size_t)k0 * Nq, qbytes);
+    const brsrc rLam = mkrsrc(md.lambda + (size_t)k0 * Nq, qbytes);
+    const brsrc rMu = mkrsrc(md.mu + (size_t)k0 * Nq, qbytes);
+    const unsigned co = (unsigned)(col * Nq + 4 * kr) * 4u;
+#pragma unroll
+    for (int t = 0; t < L::QTW; ++t) {
+      const int qt = wave + 4 * t < L::NQT ? wave + 4 * t : L::NQT - 1;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        rho[t][r] = bld(rRho, co + (qt * 16 + r) * 4u);
+        lam[t][r] = bld(rLam, co + (qt * 16 + r) * 4u);
+        mu[t][r] = bld(rMu, co + (qt * 16 + r) * 4u);
+      }
+    }
+  }
+  // LSRK rows: row lr of the group (field 3g + lr / Np, node lr % Np), elements he * EPL ..
+  const int lr = tid % L::LR, he = tid / L::LR;
+  const bool onl = he < L::HE;
+  const brsrc rRes = mkrsrc(res + (size_t)k0 * RowF, tbytes);
+  const brsrc rOut = mkrsrc(Qout + (size_t)k0 * RowF, tbytes);
+  float rreg[L::PFR ? 3 : 1][L::PFR ? L::EPL : 1];
+  if constexpr (L::PFR) {
+#pragma unroll
+    for (int g = 0; g < 3; ++g)
+#pragma unroll
+      for (int i = 0; i < L::EPL; ++i) {
+        const unsigned off = onl ? (unsigned)((he * L::EPL + i) * RowF + 3 * g * Np + lr) * 4u : 0x80000000u;
+        rreg[g][i] = fa != 0.f ? bld(rRes, off) : 0.f;
+      }
+  }
+  // neighbour traces of the 9 fields (then the flux, in the same registers)
+  float fl[L::EPT3][9];
+  {
+    const brsrc rT = mkrsrc(Qin, (unsigned)m.K * ROW);
+#pragma unroll
+    for (int i = 0; i < L::EPT3; ++i)
+#pragma unroll
+      for (int c = 0; c < 9; ++c) fl[i][c] = bld(rT, (unsigned)(vm[i] + c * Np) * 4u);
+  }
+  {
+    float4 *sQ4 = (float4 *)sQ;
+#pragma unroll
+    for (unsigned it = 0; it < L::QIT; ++it) sQ4[tid + it * 256u] = qreg[it];
+  }
+  if (tid < TB * 9) sG[tid] = gv0;
+  sFg[tid] = gf0;
+  // zero K padding of the rhs rows (read by the Vq product, never written otherwise)
+  constexpr int KPAD = L::RG0 - 3 * Np;
+  if (KPAD > 0 && tid < TB * KPAD) sR[(tid / KPAD) * RG + 3 * Np + tid % KPAD] = 0.f;
+  if (has_src && tid < TB) sSlot[tid] = tid < nk ? src_slot2(src, k0 + tid) : -1;
+  __syncthreads();
+
+  // ---- 2: penalty flux at the face nodes (central flux + tau_v / tau_s penalties, Q+ = 0 on
+  // the boundary), the same formula as rk_surface_elas
+  if (on3) {
+#pragma unroll
+    for (int i = 0; i < L::EPT3; ++i) {
+      const int e = g3 + L::NG3 * i;
+      if (L::NG3 * L::EPT3 == TB || e < TB) {
+        const float *q = sQ + e * RowF + fid3;
+        const float *F = sFg + 16 * e + 4 * f3;
+        const float Fs = .5f * F[0], nx = F[1], ny = F[2], nz = F[3];
+        const bool isB = vm[i] == fid3 + (k0 + e) * RowF;
+        float dQ[9];
+#pragma unroll
+        for (int c = 0; c < 9; ++c) dQ[c] = (isB ? 0.f : fl[i][c]) - q[c * Np];
+        float fc[9];
+        fc[0] = dQ[3] * nx + dQ[8] * ny + dQ[7] * nz;
+        fc[1] = dQ[8] * nx + dQ[4] * ny + dQ[6] * nz;
+        fc[2] = dQ[7] * nx + dQ[6] * ny + dQ[5] * nz;
+        fc[3] = dQ[0] * nx;
+        fc[4] = dQ[1] * ny;
+        fc[5] = dQ[2] * nz;
+        fc[6] = dQ[2] * ny + dQ[1] * nz;
+        fc[7] = dQ[2] * nx + dQ[0] * nz;
+        fc[8] = dQ[1] * nx + dQ[0] * ny;
+        // velocity fluxes z = Fs (sigma jump . n + tau_v A_n^T A_n v jump)
+        fl[i][0] = Fs * (fc[0] + md.tau_v * (fc[3] * nx + fc[8] * ny + fc[7] * nz));
+        fl[i][1] = Fs * (fc[1] + md.tau_v * (fc[8] * nx + fc[4] * ny + fc[6] * nz));
+        fl[i][2] = Fs * (fc[2] + md.tau_v * (fc[7] * nx + fc[6] * ny + fc[5] * nz));
+        // stress fluxes: the symmetric part of n (x) w with w = Fs (v jump + tau_s sigma jump . n)
+        // (sxx: nx w_x, syz: ny w_z + nz w_y, ...); only w is kept, and the normals, constant
+        // on a face, are applied after the lift
+        fl[i][3] = Fs * (dQ[0] + md.tau_s * fc[0]);
+        fl[i][4] = Fs * (dQ[1] + md.tau_s * fc[1]);
+        fl[i][5] = Fs * (dQ[2] + md.tau_s * fc[2]);
+      }
+    }
+  }
+
+  // MFMA A operands: lift blocks of the wave's first node tile (every group uses them), the Vq
+  // blocks of the wave's cubature tiles and the Pq blocks of its first node tile.  The loads
+  // are unconditional (clamped to the last tile): loads under a wave test leave the compiler
+  // unsure how many are in flight where the paths join.
+  const brsrc rAL = mkrsrc(o.LiftBlk, 4u * NPT * KBL * 1024u);
+  const brsrc rAV = mkrsrc(o.VqBlk, (unsigned)L::NQT * KBV * 1024u);
+  const brsrc rAP = mkrsrc(o.PqBlk, (unsigned)NPT * KBP * 1024u);
+  const int mt0 = wave < NPT ? wave : NPT - 1;
+  f4 aL[4][KBL], aV[L::QTW][KBV], aP[KBP];
+  auto load_lift = [&](int mt) {
+#pragma unroll
+    for (int f = 0; f < 4; ++f)
+#pragma unroll
+      for (int b = 0; b < KBL; ++b)
+        aL[f][b] = bld_blk<KTL>(rAL, lane * 16u, (unsigned)((f * NPT + mt) * KBL + b) * 1024u, b);
+  };
+  auto load_pq = [&](int mt) {
+#pragma unroll
+    for (int kb = 0; kb < KBP; ++kb) aP[kb] = bld_blk<KTP>(rAP, lane * 16u, (unsigned)(mt * KBP + kb) * 1024u, kb);
+  };
+  load_lift(mt0);
+#pragma unroll
+  for (int t = 0; t < L::QTW; ++t) {
+    const int qt = wave + 4 * t < L::NQT ? wave + 4 * t : L::NQT - 1;
+#pragma unroll
+    for (int kb = 0; kb < KBV; ++kb) aV[t][kb] = bld_blk<KTV>(rAV, lane * 16u, (unsigned)(qt * KBV + kb) * 1024u, kb);
+  }
+  load_pq(mt0);
+
+  float shr[L::MTW][3][4];   // shear-stress lift sums of the wave's node tiles (lift phase)
+  // unrolled: the group selects flux registers and media maps at compile time
+#pragma unroll
+  for (int g = 0; g < 3; ++g) {
+    // ---- 3: volume step 1 (F_g) and the group's flux as MFMA B operands
+    if (on1) {
+#pragma unroll
+      for (int i = 0; i < L::EPT1; ++i) {
+        const int e = g1 + L::NG1 * i;
+        const float *q = sQ + e * RowF;
+        const float *G = sG + 9 * e;
+        // gradients of the fields the group differentiates: (x, y, z) derivatives from
+        // (d/dr, d/ds, d/dt) = ((u[b+e2] - u[b+e1]) / 2, (u[b+e3] - u[b+e1]) / 2, (u[b+e4] - u[b+e1]) / 2)
+        auto grad = [&](int c, float &dx, float &dy, float &dz) {
+          const float *qc = q + c * Np;
+          const float s0 = qc[Sr.x], s1 = qc[Sr.y], s2 = qc[Sr.z], s3 = qc[Sr.w];
+          const float dr = .5f * (s1 - s0), ds = .5f * (s2 - s0), dt = .5f * (s3 - s0);
+          dx = G[0] * dr + G[3] * ds + G[6] * dt;
+          dy = G[1] * dr + G[4] * ds + G[7] * dt;
+          dz = G[2] * dr + G[5] * ds + G[8] * dt;
+        };
+        float F0, F1, F2;
+        if (g == 0) {
+          // div sigma: (sxx_x + sxy_y + sxz_z, sxy_x + syy_y + syz_z, sxz_x + syz_y + szz_z)
+          float x3, y3, z3, x4, y4, z4, x5, y5, z5, x6, y6, z6, x7, y7, z7, x8, y8, z8;
+          grad(3, x3, y3, z3);
+          grad(4, x4, y4, z4);
+          grad(5, x5, y5, z5);
+          grad(6, x6, y6, z6);
+          grad(7, x7, y7, z7);
+          grad(8, x8, y8, z8);
+          F0 = x3 + y8 + z7;
+          F1 = x8 + y4 + z6;
+          F2 = x7 + y6 + z5;
+        } else {
+          float x0, y0, z0, x1, y1, z1, x2, y2, z2;
+          grad(0, x0, y0, z0);
+          grad(1, x1, y1, z1);
+          grad(2, x2, y2, z2);
+          if (g == 1) {   // normal strain rates
+            F0 = x0;
+            F1 = y1;
+            F2 = z2;
+          } else {        // engineering shear strain rates (yz, xz, xy)
+            F0 = y2 + z1;
+            F1 = x2 + z0;
+            F2 = x1 + y0;
+          }
+        }
+        sF[e * Np1 + n1] = make_float4(F0, F1, F2, 0.f);
+      }
+    }
+    if (g < 2 && on3) {
+#pragma unroll
+      for (int i = 0; i < L::EPT3; ++i) {
+        const int e = g3 + L::NG3 * i;
+        if (L::NG3 * L::EPT3 == TB || e < TB) {
+#pragma unroll
+          for (int b = 0; b < 3; ++b) sB[(b * KB + m3) * BS + e] = fl[i][3 * g + b];
+        }
+      }
+    }
+    if (g < 2 && tid < 3 * (KB - NfpNf) * BS) {
+      const int b = tid / ((KB - NfpNf) * BS), j = tid - b * ((KB - NfpNf) * BS);
+      sB[(b * KB + NfpNf) * BS + j] = 0.f;
+    }
+    __syncthreads();
+
+    // ---- 4: volume step 2 (elevation) -> rhs_g
+    if (onv) {
+#pragma unroll
+      for (int i = 0; i < L::EPTV; ++i) {
+        const int e = gv + L::NG * i;
+        if (L::NG * L::EPTV == TB || e < TB) {
+          const float4 *Fb = sF + e * Np1;
+          const float4 f1 = Fb[Er.x], f2 = Fb[Er.y], f3v = Fb[Er.z], f4v = Fb[Er.w];
+          float *r = sR + e * RG + nv;
+          r[0] = evr.x * f1.x + evr.y * f2.x + evr.z * f3v.x + evr.w * f4v.x;
+          r[Np] = evr.x * f1.y + evr.y * f2.y + evr.z * f3v.y + evr.w * f4v.y;
+          r[2 * Np] = evr.x * f1.z + evr.y * f2.z + evr.z * f3v.z + evr.w * f4v.z;
+        }
+      }
+    }
+    __syncthreads();
+
+    // ---- 5: lift on MFMA (M = nodes, N = elements, K = face nodes).  Velocities:
+    // rhs_c += sum_f LIFT_f z_{c,f}.  Stresses: the per-face products P_{f,c} = LIFT_f w_{c,f} are
+    // formed once (in the normal-stress group) and combined with the face normals:
+    // sxx += sum_f nx_f P_{f,x}, ..., syz += sum_f (ny_f P_{f,z} + nz_f P_{f,y}), ...; the shear
+    // sums wait in registers for the shear group.
+    {
+      const float *Bp = sB + kr * BS + col;
+#pragma unroll
+      for (int j = 0; j < L::MTW; ++j) {
+        const int mt = wave + 4 * j;
+        if (mt >= NPT) break;
+        if (g == 0) {
+          // the registers hold the wave's first tile, except after a wave's second tile
+          if (j > 0) load_lift(mt);
+          f4 acc[3];
+#pragma unroll
+          for (int b = 0; b < 3; ++b) {
+            acc[b] = f4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int f = 0; f < 4; ++f)
+#pragma unroll
+              for (int kb = 0; kb < KBL; ++kb)
+#pragma unroll
+                for (int sb = 0; sb < 4; ++sb) {
+                  const int kt = 4 * kb + sb;
+                  if (kt < KTL)
+                    acc[b] = __builtin_amdgcn_mfma_f32_16x16x4f32(aL[f][kb][sb], Bp[(b * KB + f * Nfp + 4 * kt) * BS],
+                                                                  acc[b], 0, 0, 0);
+                }
+          }
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            const int n = mt * 16 + 4 * kr + r;
+            if (n < Np) {
+#pragma unroll
+              for (int b = 0; b < 3; ++b) sR[col * RG + b * Np + n] += acc[b][r];
+            }
+          }
+          if (j > 0) load_lift(wave);   // the stress lift starts from the first tile
+        } else if (g == 1) {
+          if (j > 0) load_lift(mt);
+          f4 acc[4][3];
+#pragma unroll
+          for (int f = 0; f < 4; ++f)
+#pragma unroll
+            for (int b = 0; b < 3; ++b) {
+              acc[f][b] = f4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+              for (int kb = 0; kb < KBL; ++kb)
+#pragma unroll
+                for (int sb = 0; sb < 4; ++sb) {
+                  const int kt = 4 * kb + sb;
+                  if (kt < KTL)
+                    acc[f][b] = __builtin_amdgcn_mfma_f32_16x16x4f32(aL[f][kb][sb], Bp[(b * KB + f * Nfp + 4 * kt) * BS],
+                                                                     acc[f][b], 0, 0, 0);
+                }
+            }
+          float nx[4], ny[4], nz[4];
+#pragma unroll
+          for (int f = 0; f < 4; ++f) {
+            nx[f] = sFg[16 * col + 4 * f + 1];
+            ny[f] = sFg[16 * col + 4 * f + 2];
+            nz[f] = sFg[16 * col + 4 * f + 3];
+          }
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            float xx = 0.f, yy = 0.f, zz = 0.f, yz = 0.f, xz = 0.f, xy = 0.f;
+#pragma unroll
+            for (int f = 0; f < 4; ++f) {
+              const float px = acc[f][0][r], py = acc[f][1][r], pz = acc[f][2][r];
+              xx += nx[f] * px;
+              yy += ny[f] * py;
+              zz += nz[f] * pz;
+              yz += ny[f] * pz + nz[f] * py;
+              xz += nx[f] * pz + nz[f] * px;
+              xy += nx[f] * py + ny[f] * px;
+            }
+            shr[j][0][r] = yz;
+            shr[j][1][r] = xz;
+            shr[j][2][r] = xy;
+            const int n = mt * 16 + 4 * kr + r;
+            if (n < Np) {
+              float *rr = sR + col * RG + n;
+              rr[0] += xx;
+              rr[Np] += yy;
+              rr[2 * Np] += zz;
+            }
+          }
+        } else {
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            const int n = mt * 16 + 4 * kr + r;
+            if (n < Np) {
+#pragma unroll
+              for (int b = 0; b < 3; ++b) sR[col * RG + b * Np + n] += shr[j][b][r];
+            }
+          }
+        }
+      }
+    }
+    __syncthreads();
+
+    // ---- 6: S_b = map_g(Vq rhs_{3g+b}) (M = cubature points, N = elements, K = nodes)
+    {
+      const float *Bp = sR + col * RG + kr;
+#pragma unroll
+      for (int t = 0; t < L::QTW; ++t) {
+        const int qt = wave + 4 * t;
+        if (qt >= L::NQT) break;
+        f4 acc[3];
+#pragma unroll
+        for (int b = 0; b < 3; ++b) acc[b] = f4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int kb = 0; kb < KBV; ++kb)
+#pragma unroll
+          for (int sb = 0; sb < 4; ++sb) {
+            const int kt = 4 * kb + sb;
+            if (kt < KTV) {
+#pragma unroll
+              for (int b = 0; b < 3; ++b)
+                acc[b] = __builtin_amdgcn_mfma_f32_16x16x4f32(aV[t][kb][sb], Bp[b * Np + kt * 4], acc[b], 0, 0, 0);
+            }
+          }
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int q = qt * 16 + 4 * kr + r;
+          float s0, s1, s2;
+          if (g == 0) {
+            s0 = rho[t][r] * acc[0][r];
+            s1 = rho[t][r] * acc[1][r];
+            s2 = rho[t][r] * acc[2][r];
+          } else if (g == 1) {
+            const float A = 2.f * mu[t][r] + lam[t][r], l = lam[t][r];
+            s0 = A * acc[0][r] + l * acc[1][r] + l * acc[2][r];
+            s1 = A * acc[1][r] + l * acc[0][r] + l * acc[2][r];
+            s2 = A * acc[2][r] + l * acc[0][r] + l * acc[1][r];
+          } else {
+            s0 = mu[t][r] * acc[0][r];
+            s1 = mu[t][r] * acc[1][r];
+            s2 = mu[t][r] * acc[2][r];
+          }
+          const bool in = q < Nq;
+          S[q * 16 + col] = in ? s0 : 0.f;
+          S[(NQP + q) * 16 + col] = in ? s1 : 0.f;
+          S[(2 * NQP + q) * 16 + col] = in ? s2 : 0.f;
+        }
+      }
+    }
+    __syncthreads();
+
+    // ---- 7: rhs_{3g+b} = Pq S_b (M = nodes, K = cubature points)
+    for (int mt = wave; mt < NPT; mt += 4) {
+      if (mt != wave) load_pq(mt);
+      f4 acc[3];
+#pragma unroll
+      for (int b = 0; b < 3; ++b) acc[b] = f4{0.f, 0.f, 0.f, 0.f};
+      const float *Bp = S + kr * 16 + col;
+#pragma unroll
+      for (int kb = 0; kb < KBP; ++kb)
+#pragma unroll
+        for (int sb = 0; sb < 4; ++sb) {
+          const int kt = 4 * kb + sb;
+          if (kt < KTP) {
+#pragma unroll
+            for (int b = 0; b < 3; ++b)
+              acc[b] = __builtin_amdgcn_mfma_f32_16x16x4f32(aP[kb][sb], Bp[(b * NQP + kt * 4) * 16], acc[b], 0, 0, 0);
+          }
+        }
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int n = mt * 16 + 4 * kr + r;
+        if (n < Np) {
+#pragma unroll
+          for (int b = 0; b < 3; ++b) sR[col * RG + b * Np + n] = acc[b][r];
+        }
+      }
+      if (mt != wave && g < 2) load_pq(mt0);   // the next group starts from the first tile
+    }
+    __syncthreads();
+
+    // ---- 8: LSRK for the group's rows: res = fa res + fdt (rhs + source); Q_out = Q_in + fb res
+    if (onl) {
+      const int fr = 3 * g + lr / Np, nr = lr - (lr / Np) * Np;
+      float wsel = 0.f;
+#pragma unroll
+      for (int f = 0; f < 9; ++f) wsel = fr == f ? src.w[f] : wsel;
+      float rv[L::EPL];
+#pragma unroll
+      for (int i = 0; i < L::EPL; ++i) {
+        const unsigned off = (unsigned)((he * L::EPL + i) * RowF + 3 * g * Np + lr) * 4u;
+        if constexpr (L::PFR) rv[i] = rreg[g][i];
+        else rv[i] = fa != 0.f ? bld(rRes, off) : 0.f;
+      }
+#pragma unroll
+      for (int i = 0; i < L::EPL; ++i) {
+        const int e = he * L::EPL + i;
+        float v = sR[e * RG + lr];
+        if (has_src) {
+          const int slot = sSlot[e];
+          if (slot >= 0) v += wsel * ft * src.coef[(size_t)slot * Np + nr];
+        }
+        const float x = fa * rv[i] + fdt * v;
+        const unsigned off = (unsigned)(e * RowF + 3 * g * Np + lr) * 4u;
+        bst(rOut, sQ[e * RowF + 3 * g * Np + lr] + fb * x, off);
+        if (store_res) bst(rRes, x, off);
+      }
+    }
+    // the next group's phase 3 writes F and the flux region, which nothing reads now; its
+    // phase 4 writes rhs_g after the barrier that ends phase 3
+  }
+}
+
+template <int N>
+static hipError_t fused_elas_n(bbw_ctx *c, const float *Qin, float *Qout, float fa, float fb, float fdt,
+                               const float *ftd, float ft, const SrcArgs &s, int store_res) {
+  using L = ElasFusedLayout<N>;
+  const size_t sh = sizeof(float) * L::floats;
+  if ((unsigned long long)c->K * 36 * L::Np >= (1ull << 32)) return hipErrorInvalidValue;
+  if (sh > 160 * 1024) return hipErrorInvalidValue;
+  if (sh > 65536) {
+    hipError_t e = hipFuncSetAttribute((const void *)k_fused_elas<N>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh);
+    if (e != hipSuccess) return e;
+  }
+  FusedMesh m{c->K, c->vgeo, c->fgeo, c->vmapP};
+  ElasMedia md{c->rhoinv, c->lambda, c->mu, c->tau_v, c->tau_s};
+  hipLaunchKernelGGL(k_fused_elas<N>, dim3((c->K + L::TB - 1) / L::TB), dim3(256), sh, c->stream, m, fused_ops(c), md,
+                     Qin, Qout, c->res, s, fa, fb, fdt, ft, ftd, store_res);
+  return hipGetLastError();
+}
+
+hipError_t launch_fused_elas(bbw_ctx *c, const float *Qin, float *Qout, float fa, float fb, float fdt,
+                             const float *ftd, float ft, int store_res) {
+  SrcArgs s;
+  s.nsrc = c->nsrc;
+  s.first = c->src_first;
+  s.last = c->src_last;
+  s.elems = c->src_elems;
+  s.coef = c->src_coef;
+  for (int f = 0; f < 9; ++f) s.w[f] = c->src_w[f];
+  switch (c->N) {
+    case 1: return fused_elas_n<1>(c, Qin, Qout, fa, fb, fdt, ftd, ft, s, store_res);
+    case 2: return fused_elas_n<2>(c, Qin, Qout, fa, fb, fdt, ftd, ft, s, store_res);
+    case 3: return fused_elas_n<3>(c, Qin, Qout, fa, fb, fdt, ftd, ft, s, store_res);
+    case 4: return fused_elas_n<4>(c, Qin, Qout, fa, fb, fdt, ftd, ft, s, store_res);
+    case 5: return fused_elas_n<5>(c, Qin, Qout, fa, fb, fdt, ftd, ft, s, store_res);
+    case 6: return fused_elas_n<6>(c, Qin, Qout, fa, fb, fdt, ftd, ft, s, store_res);
+  }
+  return hipErrorInvalidValue;
+}
+
+}  // namespace bbw

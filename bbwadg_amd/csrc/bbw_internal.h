@@ -180,6 +180,9 @@ hipError_t launch_fused(bbw_ctx *c, const float *Qin, float *Qout, float fa, flo
 hipError_t launch_fused_mfma(bbw_ctx *c, const float *Qin, float *Qout, float fa, float fb,
                              float fdt, const float *ftime_dev, float ftime, int project_velocity,
                              float *yout = nullptr, int store_res = 1);
+// fused elastic stage: Qin -> Qout, res updated (stored unless store_res = 0)
+hipError_t launch_fused_elas(bbw_ctx *c, const float *Qin, float *Qout, float fa, float fb, float fdt,
+                             const float *ftime_dev, float ftime, int store_res = 1);
 hipError_t launch_adj_stage(bbw_ctx *c, const float *Y, float fa, float fb, float fdt);
 // lres_io bit 0: read the adjoint residual (else it is zero); bit 1: store it (else it is dead)
 hipError_t launch_adj_stage_tiled(bbw_ctx *c, const float *Y, float fa, float fb, float fdt, int lres_io = 3);

@@ -4,7 +4,7 @@ reference's ``runBlkTimings`` / ``profileKernels`` runs, ``cpp/runBlkTimings:4-2
     python -m bbwadg_amd.drive --mesh 32 --N 5 --steps 10 --mode fused --impl mfma
 
 Builds a seeded cube mesh with Fourier c^2 (or, with ``--elastic``, seeded Lame fields and
-9 fields on the plain path), runs ``--steps`` LSRK45 steps in the chosen mode and prints one
+9 fields), runs ``--steps`` LSRK45 steps in the chosen mode and prints one
 line with the time per stage.
 """
 import argparse
@@ -52,7 +52,7 @@ def main():
     ap.add_argument("--mode", choices=["fused", "plain"], default="fused")
     ap.add_argument("--impl", choices=["mfma", "vector"], default="mfma")
     ap.add_argument("--project-velocity", action="store_true")
-    ap.add_argument("--elastic", action="store_true", help="elastic (9 fields), plain stage")
+    ap.add_argument("--elastic", action="store_true", help="elastic (9 fields), fused or plain stage")
     ap.add_argument("--test-rk", action="store_true",
                     help="instead: time the WADG update variants per launch (test_RK): the plain "
                          "MFMA update (all four fields + LSRK) and the exact BB-WADG pressure update")
@@ -87,7 +87,7 @@ def main():
         return
     if args.elastic:
         s = setup_elastic(args.mesh, args.N)
-        mode = MODE_PLAIN
+        mode = MODE_FUSED if args.mode == "fused" else MODE_PLAIN
     else:
         s = setup(args.mesh, args.N)
         s.set_option(_lib.OPT_FUSED_IMPL, _lib.FUSED_MFMA if args.impl == "mfma" else _lib.FUSED_VECTOR)

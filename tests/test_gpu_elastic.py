@@ -74,3 +74,57 @@ def test_elastic_run_with_moment_source():
                     *[a.astype(np.float32) for a in mats], ref, dt, 30, src=src, srcw=w, f0=f0, amp=amp)
     assert np.abs(ref).max() > 0
     assert rel_l2(s.WaveGetData3d(), ref) < 1e-5
+
+
+@pytest.mark.parametrize("N", [1, 2, 3, 4, 5, 6])
+def test_fused_elastic_stage(N):
+    """One fused elastic stage (volume + penalty surface + Lame WADG update + LSRK in one launch)
+    against the FP64 oracle's three kernels on the same FP32 inputs."""
+    print(f"case fused elastic stage N={N}", flush=True)
+    ops, m, dm, mats, s = make(N, 3, tau=(0.8, 1.2))
+    rng = np.random.default_rng(10 + N)
+    Q = rng.uniform(-1, 1, (m.K, 9, ops.Np)).astype(np.float32)
+    res = rng.uniform(-1, 1, Q.shape).astype(np.float32)
+    s.WaveSetData3d(Q)
+    s.set_res(res)
+    fa, fb, fdt = np.float32(ops.rk4a[2]), np.float32(ops.rk4b[2]), np.float32(1e-2)
+    s.stage_fused(fa, fb, fdt)
+    oo = orc.OracleOps(ops, np.float64)
+    rhs = np.zeros(Q.shape)
+    Q64 = Q.astype(np.float64)
+    orc.volume(oo, m.K, dm.vgeo, Q64, rhs, nfields=9)
+    orc.surface(oo, m.K, dm.fgeo, dm.vmapP, Q64, rhs, nfields=9, tau_v=0.8, tau_s=1.2)
+    rQ, rres = Q64.copy(), res.astype(np.float64).copy()
+    orc.update_elas(oo, m.K, *[a.astype(np.float32) for a in mats], fa, fb, fdt, rhs, rres, rQ)
+    assert rel(s.get_res(), rres) < 2e-5
+    assert rel(s.WaveGetData3d(), rQ) < 2e-5
+    s.close()
+
+
+@pytest.mark.parametrize("N", [3, 4])
+def test_fused_elastic_run_with_moment_source(N):
+    """Fused elastic steps with the explosive moment-tensor source: graph replay bitwise equal to
+    eager stepping, within 1e-5 of the FP64 oracle and of the three-launch path."""
+    print(f"case fused elastic run N={N}", flush=True)
+    ops, m, dm, mats, s = make(N, 3)
+    elems, coef = media.gaussian_source(m, ops, (0.02, -0.03, 0.05), a=15.0)
+    w = np.array([0, 0, 0, 1, 1, 1, 0, 0, 0], dtype=np.float64)
+    f0, amp, nsteps = 10.0, 5.0, 23
+    s.set_source(elems, coef, f0=f0, amp=amp, weights=w)
+    Q0 = (0.1 * np.random.default_rng(N).uniform(-1, 1, (m.K, 9, ops.Np))).astype(np.float32)
+    dt = s.dt_ref
+    out = {}
+    for name, mode, graph in (("fused_graph", _lib.MODE_FUSED, True), ("fused_eager", _lib.MODE_FUSED, False),
+                              ("plain", MODE_PLAIN, True)):
+        s.WaveSetData3d(Q0)
+        s.set_res(np.zeros_like(Q0))
+        s.step(nsteps, dt, mode=mode, graph=graph, t0=0.0)
+        out[name] = s.WaveGetData3d()
+    assert np.array_equal(out["fused_graph"], out["fused_eager"])
+    src = np.zeros((m.K, ops.Np)); src[elems] = coef
+    ref = Q0.astype(np.float64).copy()
+    orc.rk_run_elas(orc.OracleOps(ops, np.float64), m.K, dm.vgeo, dm.fgeo, dm.vmapP,
+                    *[a.astype(np.float32) for a in mats], ref, dt, nsteps, src=src, srcw=w, f0=f0, amp=amp)
+    assert rel_l2(out["fused_graph"], ref) < 1e-5
+    assert rel_l2(out["fused_graph"], out["plain"]) < 1e-5
+    s.close()
