@@ -5,7 +5,7 @@ reference's WADG LSRK45 path (``cpp/src/WaveOKL3d.cpp:2014-2109``).
 
     inv = AcousticInversion(solver, shots, rec_points, nsteps, dt, ckpt_every=50)
     J, g = inv.misfit_and_gradient(c2q, d_obs)          # numpy, summed over shots
-    J = C2Misfit.apply(c2q_tensor, inv, d_obs)          # torch.autograd.Function
+    J = c2_misfit(c2q_tensor, inv, d_obs)               # torch.autograd.Function, on the GPU
     J.backward(); c2q_tensor.grad                       # dJ/dc2q at the cubature points
 
 Each shot is a source (element ids, BB coefficients, Ricker f0, amplitude).  All shots
@@ -38,6 +38,7 @@ class AcousticInversion:
         mapP = i32(meshmod.build_mapP(solver.mesh, solver.ops))
         check(solver.ctx, solver.lib.bbw_upload_adjoint_maps(solver.ctx, ptr(mapP)))
         self.Q0 = np.zeros((solver.K, 4, solver.ops.Np), dtype=np.float32)
+        self._stream = None
 
     def _set_shot(self, shot):
         self.s.set_source(shot.elems, shot.coef, shot.f0, shot.amp, shot.weights)
@@ -86,18 +87,70 @@ class AcousticInversion:
         return J, g
 
 
+    # -- device-resident path (torch tensors on the GPU) -------------------------------------
+    def torch_stream(self):
+        """The torch stream (default priority) the solver's work goes to on the tensor path."""
+        import torch
+        if self._stream is None:
+            self._stream = torch.cuda.Stream()
+            self.s.set_stream(self._stream.cuda_stream)
+        return self._stream
+
+    def misfit_and_gradient_device(self, c2q, d_obs, need_grad=True):
+        """c2q: (K, Nq) float32 CUDA tensor; d_obs: (nshots, nsteps, nrec) float32 CUDA tensor.
+        Returns (J, grad) with grad a (K, Nq) float32 CUDA tensor (None when need_grad is false:
+        then only the forward misfit runs).  Everything is enqueued on ``torch_stream()``, which
+        the caller orders against its own stream; J is reduced in FP64 on the device."""
+        import torch
+        lib, ctx = self.s.lib, self.s.ctx
+        st = self.torch_stream()
+        with torch.cuda.stream(st):
+            c2q = c2q.detach().to(torch.float32).contiguous()
+            d_obs = d_obs.detach().to(torch.float32).contiguous()
+            check(ctx, lib.bbw_upload_media_device(ctx, ctypes.c_void_p(c2q.data_ptr())))
+            g = torch.zeros_like(c2q) if need_grad else None
+            gs = torch.empty_like(c2q) if need_grad else None
+            J = 0.0
+            for sh, d in zip(self.shots, d_obs):
+                self.s.set_source(sh.elems, sh.coef, sh.f0, sh.amp, sh.weights)
+                check(ctx, lib.bbw_set_state(ctx, None))
+                v = ctypes.c_double()
+                dp = ctypes.c_void_p(d.data_ptr())
+                if need_grad:
+                    check(ctx, lib.bbw_adjoint_gradient_device(ctx, self.nsteps, self.dt, self.t0, dp,
+                                                               self.ckpt_every, ctypes.c_void_p(gs.data_ptr()),
+                                                               ctypes.byref(v)))
+                    g += gs
+                else:
+                    check(ctx, lib.bbw_misfit_device(ctx, self.nsteps, self.dt, self.t0, dp, ctypes.byref(v)))
+                J += v.value
+        return J, g
+
+
 def _torch_function():
     import torch
 
     class C2Misfit(torch.autograd.Function):
-        """J(c2q) = sum over shots of 1/2 ||receiver traces - d_obs||^2; backward returns the
-        adjoint gradient (computed with the forward, checkpoint-and-recompute inside)."""
+        """J(c2q) = sum over shots of 1/2 ||receiver traces - d_obs||^2 for a CUDA tensor c2q;
+        backward returns the adjoint gradient, computed with the forward (checkpoint and
+        recompute inside) only when c2q needs a gradient.  c2q, the observations and the
+        gradient stay on the GPU; the work is ordered after, and its results before, the
+        caller's current stream."""
 
         @staticmethod
         def forward(ctx, c2q, inv, d_obs):
-            c2 = c2q.detach().to("cpu", torch.float32).numpy()
-            J, g = inv.misfit_and_gradient(c2, d_obs)
-            ctx.save_for_backward(torch.as_tensor(g, dtype=c2q.dtype, device=c2q.device))
+            if not c2q.is_cuda:
+                raise ValueError("c2_misfit takes c2q on the GPU")
+            need = bool(ctx.needs_input_grad[0])
+            cur = torch.cuda.current_stream(c2q.device)
+            d = torch.as_tensor(d_obs, device=c2q.device)   # on the caller's stream
+            st = inv.torch_stream()
+            st.wait_stream(cur)
+            J, g = inv.misfit_and_gradient_device(c2q, d, need)
+            cur.wait_stream(st)
+            if need:
+                g.record_stream(cur)
+                ctx.save_for_backward(g.to(c2q.dtype))
             return torch.tensor(J, dtype=torch.float64, device=c2q.device)
 
         @staticmethod

@@ -204,7 +204,7 @@ void bbw_destroy(bbw_ctx *c) {
   dfree(c->c2q); dfree(c->Jq); dfree(c->rhoinv); dfree(c->lambda); dfree(c->mu);
   dfree(c->Q[0]); dfree(c->Q[1]); dfree(c->res); dfree(c->rhs); dfree(c->src_elems);
   dfree(c->src_coef); dfree(c->rec_elems); dfree(c->rec_basis); dfree(c->rec_trace);
-  dfree(c->partial); dfree(c->ftime_win); dfree(c->ft_all);
+  dfree(c->partial); dfree(c->ftime_win); dfree(c->ft_all); dfree(c->dobs);
   if (c->ft_host) hipHostFree(c->ft_host);
   if (c->ft_evt) hipEventDestroy(c->ft_evt);
   if (c->own_stream) hipStreamDestroy(c->own_stream);
@@ -549,6 +549,18 @@ int bbw_upload_media(bbw_ctx *c, const float *c2q, const float *Jq) {
   return BBW_OK;
 }
 
+int bbw_upload_media_device(bbw_ctx *c, const float *c2q_dev) {
+  if (!c || !c2q_dev) return BBW_ERR_ARG;
+  if (c->Nf != 4) return fail(c, BBW_ERR_ARG, "acoustic media on an elastic context");
+  const size_t n = (size_t)c->K * c->Nq;
+  int rc = dalloc(c, &c->c2q, n);
+  if (rc) return rc;
+  HIPCHK(c, hipMemcpyAsync(c->c2q, c2q_dev, n * sizeof(float), hipMemcpyDeviceToDevice, c->stream));
+  dfree(c->Jq);
+  c->have_media = true;
+  return BBW_OK;
+}
+
 int bbw_upload_elastic_media(bbw_ctx *c, const float *rhoinvq, const float *lambdaq,
                              const float *muq, float tau_v, float tau_s) {
   if (!c || !rhoinvq || !lambdaq || !muq) return BBW_ERR_ARG;
@@ -634,7 +646,14 @@ static int copy_state(bbw_ctx *c, float *dev, const float *hin, float *hout) {
   return BBW_OK;
 }
 
-int bbw_set_state(bbw_ctx *c, const float *Q) { return c ? copy_state(c, c->Q[c->cur], Q, nullptr) : BBW_ERR_ARG; }
+int bbw_set_state(bbw_ctx *c, const float *Q) {
+  if (!c) return BBW_ERR_ARG;
+  if (!Q) {
+    HIPCHK(c, hipMemsetAsync(c->Q[c->cur], 0, nstate(c) * sizeof(float), c->stream));
+    return BBW_OK;
+  }
+  return copy_state(c, c->Q[c->cur], Q, nullptr);
+}
 int bbw_get_state(bbw_ctx *c, float *Q) { return c ? copy_state(c, c->Q[c->cur], nullptr, Q) : BBW_ERR_ARG; }
 int bbw_set_res(bbw_ctx *c, const float *r) { return c ? copy_state(c, c->res, r, nullptr) : BBW_ERR_ARG; }
 int bbw_get_res(bbw_ctx *c, float *r) { return c ? copy_state(c, c->res, nullptr, r) : BBW_ERR_ARG; }
@@ -942,21 +961,44 @@ int bbw_upload_adjoint_maps(bbw_ctx *c, const int32_t *mapP) {
 
 namespace {
 
-// Forward nsteps from the current state, recording receivers, keeping a copy of the state at
-// the start of every checkpoint segment (ckpt may be null).  Returns traces on the host.
-int forward_record(bbw_ctx *c, int nsteps, double dt, double t0, int C, std::vector<float> &tr) {
-  int rc;
+// Forward nsteps from the current state, recording receivers into the trace buffer.
+int forward_record(bbw_ctx *c, int nsteps, double dt, double t0) {
   c->rec_steps = 0;
-  const size_t n = nstate(c);
-  for (int s0 = 0; s0 < nsteps; s0 += C) {
-    const int ns = nsteps - s0 < C ? nsteps - s0 : C;
-    if (c->ckpt)
-      HIPCHK(c, hipMemcpyAsync(c->ckpt + (size_t)(s0 / C) * n, c->Q[c->cur], n * sizeof(float),
-                               hipMemcpyDeviceToDevice, c->stream));
-    if ((rc = bbw_step(c, ns, dt, t0 + s0 * dt, BBW_MODE_FUSED, 2))) return rc;
+  return bbw_step(c, nsteps, dt, t0, BBW_MODE_FUSED, 2);
+}
+
+// Observations on the device: a caller's device array, or a host array copied into the
+// context's buffer.
+int device_obs(bbw_ctx *c, int nsteps, const float *d_obs, bool on_device, const float **d_dev) {
+  if (on_device) {
+    *d_dev = d_obs;
+    return BBW_OK;
   }
-  tr.resize((size_t)nsteps * c->nrec);
-  return bbw_get_traces(c, tr.data(), nsteps);
+  const size_t n = (size_t)nsteps * c->nrec;
+  if (c->dobs_cap < n) {
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    dfree(c->dobs);
+    c->dobs_cap = 0;
+    int rc = dalloc(c, &c->dobs, n);
+    if (rc) return rc;
+    c->dobs_cap = n;
+  }
+  HIPCHK(c, hipMemcpyAsync(c->dobs, d_obs, n * sizeof(float), hipMemcpyHostToDevice, c->stream));
+  *d_dev = c->dobs;
+  return BBW_OK;
+}
+
+// J = 1/2 sum (trace - d)^2 over nsteps rows of the trace buffer, FP64 on the device; with
+// resid, the residuals (FP32) for the adjoint's receiver injection.  *J is written once the
+// stream reaches it (the callers synchronize before returning).
+int device_misfit(bbw_ctx *c, int nsteps, const float *d_dev, float *resid, double *J) {
+  int rc = dalloc(c, &c->partial, (size_t)BBW_ENERGY_BLOCKS + 1);
+  if (rc) return rc;
+  hipError_t e = launch_misfit(c, c->rec_trace, d_dev, (long long)nsteps * c->nrec, resid, c->partial,
+                               c->partial + BBW_ENERGY_BLOCKS);
+  if (e != hipSuccess) return hipfail(c, e, "misfit launch");
+  HIPCHK(c, hipMemcpyAsync(J, c->partial + BBW_ENERGY_BLOCKS, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  return BBW_OK;
 }
 
 int check_inverse(bbw_ctx *c, int nsteps, const float *d_obs) {
@@ -986,36 +1028,20 @@ int restore_state(bbw_ctx *c) {
   return BBW_OK;
 }
 
-double misfit_host(const std::vector<float> &tr, const float *d, size_t n, std::vector<float> *resid) {
-  double J = 0.0;
-  if (resid) resid->resize(n);
-  for (size_t i = 0; i < n; ++i) {
-    const double r = (double)tr[i] - (double)d[i];
-    J += 0.5 * r * r;
-    if (resid) (*resid)[i] = (float)r;
-  }
-  return J;
-}
-
-}  // namespace
-
-int bbw_misfit(bbw_ctx *c, int nsteps, double dt, double t0, const float *d_obs, double *J) {
+int misfit_impl(bbw_ctx *c, int nsteps, double dt, double t0, const float *d_obs, bool obs_dev, double *J) {
   if (!c || !J) return BBW_ERR_ARG;
   int rc = check_inverse(c, nsteps, d_obs);
   if (rc) return rc;
+  const float *d_dev = nullptr;
+  if ((rc = device_obs(c, nsteps, d_obs, obs_dev, &d_dev))) return rc;
   if ((rc = save_state(c))) return rc;
-  float *ck = c->ckpt;
-  c->ckpt = nullptr;
-  std::vector<float> tr;
-  rc = forward_record(c, nsteps, dt, t0, nsteps, tr);
-  c->ckpt = ck;
-  if (rc) return rc;
-  *J = misfit_host(tr, d_obs, tr.size(), nullptr);
+  if ((rc = forward_record(c, nsteps, dt, t0))) return rc;
+  if ((rc = device_misfit(c, nsteps, d_dev, nullptr, J))) return rc;
   return restore_state(c);
 }
 
-int bbw_adjoint_gradient(bbw_ctx *c, int nsteps, double dt, double t0, const float *d_obs,
-                         int ckpt_every, float *grad_out, double *J) {
+int gradient_impl(bbw_ctx *c, int nsteps, double dt, double t0, const float *d_obs, bool obs_dev,
+                  int ckpt_every, float *grad_out, bool grad_dev, double *J) {
   if (!c || !grad_out || !J || ckpt_every <= 0) return BBW_ERR_ARG;
   int rc = check_inverse(c, nsteps, d_obs);
   if (rc) return rc;
@@ -1106,10 +1132,9 @@ int bbw_adjoint_gradient(bbw_ctx *c, int nsteps, double dt, double t0, const flo
     if (e != hipSuccess) return hipfail(c, e, "record launch");
     c->rec_steps += 1;
   }
-  std::vector<float> tr((size_t)nsteps * c->nrec), resid;
-  if ((rc = bbw_get_traces(c, tr.data(), nsteps))) return rc;
-  *J = misfit_host(tr, d_obs, tr.size(), &resid);
-  HIPCHK(c, hipMemcpyAsync(c->resid, resid.data(), resid.size() * sizeof(float), hipMemcpyHostToDevice, c->stream));
+  const float *d_dev = nullptr;
+  if ((rc = device_obs(c, nsteps, d_obs, obs_dev, &d_dev))) return rc;
+  if ((rc = device_misfit(c, nsteps, d_dev, c->resid, J))) return rc;
 
   // 2. reverse: cached steps first, then checkpointed segments (recomputed with capture)
   HIPCHK(c, hipMemsetAsync(c->lQ, 0, n * sizeof(float), c->stream));
@@ -1127,8 +1152,29 @@ int bbw_adjoint_gradient(bbw_ctx *c, int nsteps, double dt, double t0, const flo
     for (int j = ns - 1; j >= 0; --j)
       if ((rc = adj_step(s0 + j, c->Ybuf + (size_t)j * ystep))) return rc;
   }
-  HIPCHK(c, hipMemcpyAsync(grad_out, c->grad, nq * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipMemcpyAsync(grad_out, c->grad, nq * sizeof(float),
+                           grad_dev ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, c->stream));
   return restore_state(c);
+}
+
+}  // namespace
+
+int bbw_misfit(bbw_ctx *c, int nsteps, double dt, double t0, const float *d_obs, double *J) {
+  return misfit_impl(c, nsteps, dt, t0, d_obs, false, J);
+}
+
+int bbw_misfit_device(bbw_ctx *c, int nsteps, double dt, double t0, const float *d_obs_dev, double *J) {
+  return misfit_impl(c, nsteps, dt, t0, d_obs_dev, true, J);
+}
+
+int bbw_adjoint_gradient(bbw_ctx *c, int nsteps, double dt, double t0, const float *d_obs,
+                         int ckpt_every, float *grad_out, double *J) {
+  return gradient_impl(c, nsteps, dt, t0, d_obs, false, ckpt_every, grad_out, false, J);
+}
+
+int bbw_adjoint_gradient_device(bbw_ctx *c, int nsteps, double dt, double t0, const float *d_obs_dev,
+                                int ckpt_every, float *grad_dev, double *J) {
+  return gradient_impl(c, nsteps, dt, t0, d_obs_dev, true, ckpt_every, grad_dev, true, J);
 }
 
 int bbw_set_option(bbw_ctx *c, int option, int value) {

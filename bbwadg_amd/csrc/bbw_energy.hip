@@ -258,6 +258,25 @@ __global__ __launch_bounds__(64) void k_energy_wadg(int K, const double *__restr
   if (lane == 0) partial[blockIdx.x] = 0.5 * acc;
 }
 
+__global__ __launch_bounds__(256) void k_misfit(const float *__restrict__ tr, const float *__restrict__ d,
+                                                long long n, float *__restrict__ resid,
+                                                double *__restrict__ partial) {
+  __shared__ double red[256];
+  double acc = 0.0;
+  for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long long)gridDim.x * 256) {
+    const double r = (double)tr[i] - (double)d[i];
+    if (resid) resid[i] = (float)r;
+    acc += 0.5 * r * r;
+  }
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+}
+
 __global__ __launch_bounds__(256) void k_reduce(const double *__restrict__ partial, int n,
                                                 double *__restrict__ out) {
   __shared__ double red[256];
@@ -338,6 +357,17 @@ double *energy_scratch(int dev) {
   if (dev < 0 || dev >= MAX_DEVICES) return nullptr;
   std::lock_guard<std::mutex> lock(scratch_mu);
   return scratch_ptr[dev];
+}
+
+hipError_t launch_misfit(bbw_ctx *c, const float *tr, const float *d, long long n, float *resid,
+                         double *partial, double *out) {
+  const long long want = (n + 255) / 256;
+  const int grid = (int)(want < 1 ? 1 : want < BBW_ENERGY_BLOCKS ? want : BBW_ENERGY_BLOCKS);
+  hipLaunchKernelGGL(k_misfit, dim3(grid), dim3(256), 0, c->stream, tr, d, n, resid, partial);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_reduce, dim3(1), dim3(256), 0, c->stream, partial, grid, out);
+  return hipGetLastError();
 }
 
 hipError_t launch_energy(bbw_ctx *c, int kind, double *partial, double *out) {

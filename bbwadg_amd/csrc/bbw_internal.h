@@ -140,6 +140,8 @@ struct bbw_ctx {
   int WE = 0, WL = 0;
   float *lQ = nullptr, *lres = nullptr, *Zadj = nullptr, *Hadj = nullptr, *grad = nullptr;
   float *Ybuf = nullptr, *ckpt = nullptr, *resid = nullptr, *Q0save = nullptr;
+  float *dobs = nullptr;   // observations copied from the host [nsteps][nrec]
+  size_t dobs_cap = 0;
   size_t Ybuf_len = 0, ckpt_len = 0, resid_len = 0;
   // Vq (A Q)_p of the final steps kept from the first forward pass (no recompute for them)
   float *Ycache = nullptr;
@@ -191,6 +193,10 @@ hipError_t launch_adj_inject(bbw_ctx *c, const float *resid_row);
 // the total in *out (device memory)
 hipError_t launch_energy(bbw_ctx *c, int kind, double *partial, double *out);
 int energy_blocks(const bbw_ctx *c, int kind);   // partial sums the reduction writes
+// receiver misfit 1/2 sum (tr - d)^2 over n values, FP64, partial sums in
+// partial[BBW_ENERGY_BLOCKS] and the total in *out (device); resid (or null) gets tr - d
+hipError_t launch_misfit(bbw_ctx *c, const float *tr, const float *d, long long n, float *resid,
+                         double *partial, double *out);
 // the device address of the energy scratch array (BBW_ENERGY_BLOCKS + 1 doubles) on device dev:
 // looked up once per device by energy_scratch_init (called by bbw_create), null before
 hipError_t energy_scratch_init(int dev);

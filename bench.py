@@ -38,13 +38,13 @@ def stage_bytes(N, Nf=4, Nm=1):
     return 4 * (4 * Nf * Np + Nf * 4 * Nfp + 25 + 4 * Nfp + Nm * Nq)
 
 
-def run_bytes(N, nsteps, graph_steps=10, Nf=4):
+def run_bytes(N, nsteps, graph_steps=10, Nf=4, Nm=1):
     """Average compulsory bytes per element-stage over one bbw_step call of nsteps steps: the
     first LSRK45 stage (a_0 = 0) reads no res, and the last stage of a step that another step of
     the same captured graph (of graph_steps steps) or call follows stores no res."""
     Np = (N + 1) * (N + 2) * (N + 3) // 6
     row = 4 * Nf * Np
-    full = stage_bytes(N, Nf)
+    full = stage_bytes(N, Nf, Nm)
     chunks = [graph_steps] * (nsteps // graph_steps) if nsteps >= graph_steps else []
     rest = nsteps - sum(chunks)
     if rest:
@@ -144,36 +144,49 @@ def adjoint_ratio(nsteps, mesh_n=32, N=5, ckpt=50):
 
 def elastic_throughput(nsteps, mesh_n=32, N=4):
     """Elastic velocity-stress configuration (32^3 x 6 tets, N = 4, 9 fields, seeded Lame and
-    density fields, random start, the reference's elastic time step): the three-launch stage
-    (the elastic product path, tiled kernels), nsteps steps from HIP graphs, timed with HIP
-    events on the solver's stream."""
+    density fields, random start, the reference's elastic time step): the fused stage (one
+    launch per stage) and the three-launch stage, nsteps steps each from HIP graphs, timed with
+    HIP events on the solver's stream; the roofline fraction prices the fused rate at the
+    compulsory bytes of an elastic element-stage."""
     import torch
     from bbwadg_amd.drive import setup_elastic
-    from bbwadg_amd.solver import MODE_PLAIN
+    from bbwadg_amd.solver import MODE_FUSED, MODE_PLAIN
     s = setup_elastic(mesh_n, N)
     stream = torch.cuda.Stream()
     s.set_stream(stream.cuda_stream)
     dt = s.dt_ref   # the reference's elastic time step estimate
-    s.step(2, dt, mode=MODE_PLAIN)
-    s.prepare_graph(nsteps, dt, mode=MODE_PLAIN)   # graph capture and instantiation
-    torch.cuda.synchronize()
-    e0 = torch.cuda.Event(enable_timing=True)
-    e1 = torch.cuda.Event(enable_timing=True)
-    e0.record(stream)
-    s.step(nsteps, dt, mode=MODE_PLAIN)
-    e1.record(stream)
-    torch.cuda.synchronize()
-    ms = e0.elapsed_time(e1)
-    Q = s.WaveGetData3d()
-    if not np.all(np.isfinite(Q)):
-        raise RuntimeError("non-finite elastic state after the timed run")
+    Q0 = s.WaveGetData3d()
+    out = {}
+    for name, mode in (("fused", MODE_FUSED), ("plain", MODE_PLAIN)):
+        s.WaveSetData3d(Q0)
+        s.set_res(np.zeros_like(Q0))
+        s.step(2, dt, mode=mode)
+        s.prepare_graph(nsteps, dt, mode=mode)   # graph capture and instantiation
+        torch.cuda.synchronize()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        s.step(nsteps, dt, mode=mode)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        out[name] = e0.elapsed_time(e1)
+        if not np.all(np.isfinite(s.WaveGetData3d())):
+            raise RuntimeError(f"non-finite elastic state after the timed {name} run")
     dofs = s.K * s.ops.Np * 9 * 5 * nsteps
     K = s.K
     s.close()
+    ms_stage = out["fused"] / (5 * nsteps)
+    Bst = run_bytes(N, nsteps, Nf=9, Nm=3)
     return {"workload": f"elastic velocity-stress, {mesh_n}^3x6={K} tets, N={N}, 9 fields, seeded "
-                        "Lame/density fields, three-launch stage + HIP graph",
-            "value": dofs / (ms / 1e3), "unit": "DOF-stage/s", "steps": nsteps, "dt": dt,
-            "ms_per_stage": ms / (5 * nsteps)}
+                        "Lame/density fields, fused stage + HIP graph",
+            "value": dofs / (out["fused"] / 1e3), "unit": "DOF-stage/s", "steps": nsteps, "dt": dt,
+            "ms_per_stage": ms_stage,
+            "plain_value": dofs / (out["plain"] / 1e3), "plain_ms_per_stage": out["plain"] / (5 * nsteps),
+            "fused_over_plain": out["plain"] / out["fused"],
+            "bytes_per_element_stage": Bst, "bytes_per_element_stage_full": stage_bytes(N, 9, 3),
+            "achieved_GBs": Bst * K / (ms_stage / 1e3) / 1e9,
+            "frac_of_hbm_peak": Bst * K / (ms_stage / 1e3) / 1e9 / HBM_PEAK_GBS,
+            "frac_of_stage_roofline": stage_bytes(N, 9, 3) * K / (ms_stage / 1e3) / 1e9 / HBM_PEAK_GBS}
 
 
 def batched_throughput(step_scale=1.0, tune_steps=20):

@@ -91,6 +91,9 @@ int bbw_upload_mesh(bbw_ctx *ctx, const float *vgeo, const float *fgeo, const in
                     const double *J);
 /* ≈ c_c2q / c_Jq_reduced uploads of InitWADG_subelem (:906-909).  Jq may be NULL (1/J = 1). */
 int bbw_upload_media(bbw_ctx *ctx, const float *c2q, const float *Jq);
+/* The same from device memory (no reference counterpart): c2q_dev [K][Nq] is copied on the
+ * context's stream (Jq = 1). */
+int bbw_upload_media_device(bbw_ctx *ctx, const float *c2q_dev);
 /* Elastic media at cubature (elas_cpp/src/WaveOKL3d.cpp:445-511): rhoinv (the kernel's rhoq
  * multiplies), lambda, mu, each [K][Nq]; penalties tau_v, tau_s (:516-517). */
 int bbw_upload_elastic_media(bbw_ctx *ctx, const float *rhoinvq, const float *lambdaq,
@@ -106,7 +109,8 @@ int bbw_set_receivers(bbw_ctx *ctx, int nrec, const int32_t *elems, const float 
                       int max_steps);
 int bbw_get_traces(bbw_ctx *ctx, float *out, int nsteps);
 
-/* ≈ WaveSetData3d / WaveGetData3d device copies (:2527-2537).  Q in BB coefficients. */
+/* ≈ WaveSetData3d / WaveGetData3d device copies (:2527-2537).  Q in BB coefficients.
+ * bbw_set_state(ctx, NULL) sets the state to zero on the device. */
 int bbw_set_state(bbw_ctx *ctx, const float *Q);
 int bbw_get_state(bbw_ctx *ctx, float *Q);
 int bbw_set_res(bbw_ctx *ctx, const float *res);
@@ -184,6 +188,12 @@ int bbw_upload_adjoint_maps(bbw_ctx *ctx, const int32_t *mapP); /* [K][4Nfp] fac
 int bbw_misfit(bbw_ctx *ctx, int nsteps, double dt, double t0, const float *d_obs, double *J);
 int bbw_adjoint_gradient(bbw_ctx *ctx, int nsteps, double dt, double t0, const float *d_obs,
                          int ckpt_every, float *grad, double *J);
+/* The same with the observations [nsteps][nrec] and the gradient [K][Nq] in device memory, all
+ * work on the context's stream (for callers that keep both on the GPU, e.g. torch tensors); J
+ * is reduced in FP64 on the device in all four and returned on the host. */
+int bbw_misfit_device(bbw_ctx *ctx, int nsteps, double dt, double t0, const float *d_obs_dev, double *J);
+int bbw_adjoint_gradient_device(bbw_ctx *ctx, int nsteps, double dt, double t0, const float *d_obs_dev,
+                                int ckpt_every, float *grad_dev, double *J);
 
 /* Batched multi-shot acoustic runs (no reference counterpart): many shots on several meshes,
  * uneven durations, one persistent launch per LSRK stage for all active shots, HIP-graph
