@@ -36,6 +36,7 @@ EXPORTS = [
 ]
 OPT_FUSED_IMPL, OPT_KBLK_V, OPT_KBLK_S, OPT_KBLK_U, OPT_KBLK_F, OPT_ADJ_CACHE_MB, OPT_ADJ_IMPL = 0, 1, 2, 3, 4, 5, 6
 OPT_PLAIN_IMPL = 7
+OPT_SRC_STEP_TIME = 8
 FUSED_VECTOR, FUSED_MFMA = 0, 1
 PLAIN_FLAT, PLAIN_TILED = 0, 1
 

@@ -53,11 +53,18 @@ int upload(bbw_ctx *c, T **dst, const T *src, size_t n) {
 
 template <typename T>
 void dfree(T *&p) {
-  if (p) hipFree((void *)p);
+  if (p) (void)hipFree((void *)p);
   p = nullptr;
 }
 
 size_t nstate(const bbw_ctx *c) { return (size_t)c->K * c->Nf * c->Np; }
+
+// time at which stage i of the step starting at t evaluates the source: t + c_i dt, or t itself
+// for every stage when the context reproduces the reference's elastic driver
+// (elas_cpp/src/WaveOKL3d.cpp:1675-1680 evaluates the pulse at the step's start time)
+double stage_time(const bbw_ctx *c, double t, int i, double dt) {
+  return c->src_step_time ? t : t + c->rk4c[i] * dt;
+}
 
 double ricker(double t, double f0) {
   const double a = M_PI * f0 * (t - 1.0 / f0);
@@ -86,7 +93,7 @@ int check_ready(bbw_ctx *c) {
 }
 
 void reset_graph(bbw_ctx *c) {
-  if (c->graph.exec) hipGraphExecDestroy(c->graph.exec);
+  if (c->graph.exec) (void)hipGraphExecDestroy(c->graph.exec);
   c->graph = Graph();
 }
 
@@ -157,12 +164,12 @@ int bbw_create(const bbw_config *cfg, bbw_ctx **out) {
   }
   c->stream = c->own_stream;
   if (energy_scratch_init(c->dev) != hipSuccess) {
-    hipStreamDestroy(c->own_stream);
+    (void)hipStreamDestroy(c->own_stream);
     delete c;
     return BBW_ERR_HIP;
   }
   if (hipDeviceGetAttribute(&c->ncu, hipDeviceAttributeMultiprocessorCount, c->dev) != hipSuccess || c->ncu <= 0) {
-    hipStreamDestroy(c->own_stream);
+    (void)hipStreamDestroy(c->own_stream);
     delete c;
     return BBW_ERR_HIP;
   }
@@ -190,8 +197,8 @@ int bbw_create(const bbw_config *cfg, bbw_ctx **out) {
 
 void bbw_destroy(bbw_ctx *c) {
   if (!c) return;
-  hipSetDevice(c->dev);
-  if (c->stream) hipStreamSynchronize(c->stream);
+  (void)hipSetDevice(c->dev);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
   reset_graph(c);
   dfree(c->Dids); dfree(c->Dvals); dfree(c->Fmask); dfree(c->EEL_ids); dfree(c->L0_ids);
   dfree(c->EEL_vals); dfree(c->L0_vals); dfree(c->Vq); dfree(c->Pq); dfree(c->Vq64); dfree(c->Pq64); dfree(c->VqPad); dfree(c->PqPad); dfree(c->EEL_ids4); dfree(c->LiftPad); dfree(c->VqBlk); dfree(c->PqBlk); dfree(c->LiftBlk); dfree(c->PqTBlk); dfree(c->VqTBlk); dfree(c->LiftTBlk); dfree(c->PqTPad); dfree(c->VqTPad);
@@ -205,9 +212,9 @@ void bbw_destroy(bbw_ctx *c) {
   dfree(c->Q[0]); dfree(c->Q[1]); dfree(c->res); dfree(c->rhs); dfree(c->src_elems);
   dfree(c->src_coef); dfree(c->rec_elems); dfree(c->rec_basis); dfree(c->rec_trace);
   dfree(c->partial); dfree(c->ftime_win); dfree(c->ft_all); dfree(c->dobs);
-  if (c->ft_host) hipHostFree(c->ft_host);
-  if (c->ft_evt) hipEventDestroy(c->ft_evt);
-  if (c->own_stream) hipStreamDestroy(c->own_stream);
+  if (c->ft_host) (void)hipHostFree(c->ft_host);
+  if (c->ft_evt) (void)hipEventDestroy(c->ft_evt);
+  if (c->own_stream) (void)hipStreamDestroy(c->own_stream);
   delete c;
 }
 
@@ -809,7 +816,7 @@ int bbw_step(bbw_ctx *c, int nsteps, double dt, double t0, int mode, int flags) 
     ft.resize((size_t)nsteps * 5);
     for (int s = 0; s < nsteps; ++s)
       for (int i = 0; i < 5; ++i)
-        ft[(size_t)s * 5 + i] = (float)ricker(t0 + s * dt + c->rk4c[i] * dt, c->src_f0);
+        ft[(size_t)s * 5 + i] = (float)ricker(stage_time(c, t0 + s * dt, i, dt), c->src_f0);
   }
 
   int s = 0;
@@ -847,12 +854,12 @@ int bbw_step(bbw_ctx *c, int nsteps, double dt, double t0, int mode, int flags) 
       hipError_t e = hipStreamEndCapture(c->stream, &graph);
       c->cur = cur0;
       if (crc) {
-        if (graph) hipGraphDestroy(graph);
+        if (graph) (void)hipGraphDestroy(graph);
         return crc;
       }
       if (e != hipSuccess) return hipfail(c, e, "hipStreamEndCapture");
       e = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
-      hipGraphDestroy(graph);
+      (void)hipGraphDestroy(graph);
       if (e != hipSuccess) return hipfail(c, e, "hipGraphInstantiate");
       g.steps = G; g.mode = mode; g.flags = flags; g.parity = parity; g.dt = dt;
     }
@@ -871,7 +878,7 @@ int bbw_step(bbw_ctx *c, int nsteps, double dt, double t0, int mode, int flags) 
         c->ft_all_cap = 0;
         if ((rc = dalloc(c, &c->ft_all, nft))) return rc;
         c->ft_all_cap = nft;
-        if (c->ft_host) hipHostFree(c->ft_host);
+        if (c->ft_host) (void)hipHostFree(c->ft_host);
         c->ft_host = nullptr;
         c->ft_host_cap = 0;
         HIPCHK(c, hipHostMalloc((void **)&c->ft_host, nft * sizeof(float), hipHostMallocDefault));
@@ -1088,15 +1095,15 @@ int gradient_impl(bbw_ctx *c, int nsteps, double dt, double t0, const float *d_o
   if ((rc = save_state(c))) return rc;
 
   const bool a0_zero = c->rk4a[0] == 0.0;
-  auto stage_time = [&](int step, int i) -> float {
-    return c->nsrc ? (float)ricker(t0 + step * dt + c->rk4c[i] * dt, c->src_f0) : 0.f;
+  auto stage_value = [&](int step, int i) -> float {
+    return c->nsrc ? (float)ricker(stage_time(c, t0 + step * dt, i, dt), c->src_f0) : 0.f;
   };
   auto fwd_step_y = [&](int step, float *Y) -> int {
     for (int i = 0; i < 5; ++i) {
       // the residual left by the last stage is dead when a_0 = 0 (the state is restored and the
       // residual cleared after the gradient)
       hipError_t e = launch_fused_mfma(c, c->Q[c->cur], c->Q[1 - c->cur], (float)c->rk4a[i],
-                                       (float)c->rk4b[i], (float)dt, nullptr, stage_time(step, i),
+                                       (float)c->rk4b[i], (float)dt, nullptr, stage_value(step, i),
                                        0, Y + (size_t)i * c->K * c->Np, !(i == 4 && a0_zero));
       if (e != hipSuccess) return hipfail(c, e, "forward stage with Vq (A Q)_p capture");
       c->cur = 1 - c->cur;
@@ -1194,6 +1201,7 @@ int bbw_set_option(bbw_ctx *c, int option, int value) {
       if (value != BBW_PLAIN_FLAT && value != BBW_PLAIN_TILED) return fail(c, BBW_ERR_ARG, "bad plain impl");
       c->plain_impl = value;
       break;
+    case BBW_OPT_SRC_STEP_TIME: c->src_step_time = value ? 1 : 0; break;
     default: return fail(c, BBW_ERR_ARG, "unknown option");
   }
   reset_graph(c);

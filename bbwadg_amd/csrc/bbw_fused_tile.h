@@ -135,9 +135,13 @@ typedef unsigned u3v __attribute__((ext_vector_type(3)));
 // the products use are loaded and the others are zero constants: a load into registers that
 // are never read lets the compiler reuse them while the load is in flight, and the reuse then
 // waits for that load and every load issued before it (the next phase's prefetch included).
-// For fully unrolled block loops only (kb a compile-time constant): in a partly unrolled loop
-// the choice becomes a run-time branch, and the product loop of the exact BB-WADG update
-// compiled that way gave wrong results at N = 3, so it keeps the 128-bit load.
+// For fully unrolled block loops only (kb a compile-time constant).  In a partly unrolled loop
+// the choice becomes a run-time branch; the exact BB-WADG update's product loop (unrolled by 2,
+// four accumulator chains, the K-steps past KT skipped) compiled that way at N = 3 with this
+// compiler has, on the path that skips accumulator 1's last MFMA, four v_accvgpr_mov that copy
+// one component of another accumulator into all four of accumulator 1, which loses its sum (the
+// wrong results seen at N = 3).  With the 128-bit load in every block the copy is not emitted,
+// so that loop keeps it; tests/test_codegen.py checks the shipped kernels' code for the pattern.
 template <int KT>
 __device__ __forceinline__ f4 bld_blk(brsrc r, unsigned voff, unsigned soff, int kb) {
   constexpr int KB = (KT + 3) / 4, REM = KT - 4 * (KB - 1);

@@ -113,6 +113,7 @@ struct bbw_ctx {
   float *src_coef = nullptr;
   float src_w[9] = {0};
   double src_f0 = 0, src_amp = 0;
+  int src_step_time = 0;   // 1: every stage evaluates the source at its step's start time
 
   // receivers
   int nrec = 0, rec_max_steps = 0, rec_steps = 0;

@@ -71,6 +71,30 @@ def lame_fourier(seed, nterms=6, amp=0.1):
     return lam, mu, rho
 
 
+def element_average(ops, mq):
+    """The cubature-weighted element average sum_q w_q m_q / sum_q w_q of a field at the WADG
+    cubature (K, Nq), repeated at every point: the reference's smoothing of mu
+    (elas_cpp/src/WaveOKL3d.cpp:501-506)."""
+    mq = np.asarray(mq, dtype=np.float64)
+    avg = (mq @ ops.wq) / ops.wq.sum()
+    return np.repeat(avg[:, None], mq.shape[1], axis=1)
+
+
+def elastic_reference_media(m, ops, weight, average_mu=True):
+    """(1/rho, lambda, mu) at the WADG cubature as the reference's elastic driver builds them
+    (elas_cpp/src/WaveOKL3d.cpp:453-506): rho = 1; lambda = 1, mu = 1 + weight(x) for z >= 0;
+    lambda = 2, mu = 2 + weight(x) for z < 0 (isotropic, discontinuous across z = 0); mu then
+    averaged per element (``average_mu``)."""
+    xq, yq, zq = cubature_points(m, ops)
+    w = weight(xq, yq, zq)
+    below = zq < 0
+    lam = np.where(below, 2.0, 1.0)
+    mu = np.where(below, 2.0, 1.0) + w
+    if average_mu:
+        mu = element_average(ops, mu)
+    return np.ones_like(lam), lam, mu
+
+
 def cubature_points(m, ops):
     """Physical WADG cubature coordinates, each (K, Nq)."""
     xq, yq, zq = meshmod.affine_points(m, ops.rq, ops.sq, ops.tq)

@@ -120,10 +120,20 @@ class WaveSolver:
         """rhs_p <- P_N(c2 rhs_p) in place (``rk_update_BBWADG``)."""
         check(self.ctx, self.lib.bbw_update_bbwadg(self.ctx))
 
-    def InitElastic(self, rhoinvq, lamq, muq, tau_v=1.0, tau_s=1.0):
+    def InitElastic(self, rhoinvq, lamq, muq, tau_v=1.0, tau_s=1.0, average_mu=False):
+        """Elastic media at the WADG cubature (``InitWADG_subelem`` of elas_cpp): 1/rho, lambda,
+        mu, each (K, Nq); ``average_mu`` replaces mu by its element averages as the reference
+        does (elas_cpp/src/WaveOKL3d.cpp:501-506)."""
+        if average_mu:
+            muq = media.element_average(self.ops, muq)
         self.rhoinvq, self.lamq, self.muq = f32(rhoinvq), f32(lamq), f32(muq)
         check(self.ctx, self.lib.bbw_upload_elastic_media(self.ctx, ptr(self.rhoinvq), ptr(self.lamq),
                                                           ptr(self.muq), tau_v, tau_s))
+
+    def set_source_step_time(self, on=True):
+        """Evaluate the source at each step's start time for all five stages (the reference's
+        elastic driver, elas_cpp/src/WaveOKL3d.cpp:1675-1680) instead of at the stage times."""
+        self.set_option(_lib.OPT_SRC_STEP_TIME, 1 if on else 0)
 
     def set_source(self, elems, coef, f0, amp=1.0, weights=None):
         w = np.zeros(9)

@@ -235,6 +235,10 @@ double bbw_batch_last_ms(const bbw_batch *b); /* device time of the last bbw_bat
 #define BBW_PLAIN_TILED 1    /* default: 16-element tiles, WADG projections on MFMA */
 #define BBW_FUSED_VECTOR 0
 #define BBW_FUSED_MFMA 1
+/* source time: 0 (default) evaluates the Ricker pulse at each stage's time t + c_i dt; 1 at the
+ * step's start time for all five stages, as the reference's elastic driver does
+ * (elas_cpp/src/WaveOKL3d.cpp:1675-1680; first order in time for the source term) */
+#define BBW_OPT_SRC_STEP_TIME 8
 int bbw_set_option(bbw_ctx *ctx, int option, int value);
 
 /* Device addresses of the context's buffers (for copies into caller-owned device memory):

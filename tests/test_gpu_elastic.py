@@ -128,3 +128,42 @@ def test_fused_elastic_run_with_moment_source(N):
     assert rel_l2(out["fused_graph"], ref) < 1e-5
     assert rel_l2(out["fused_graph"], out["plain"]) < 1e-5
     s.close()
+
+
+@pytest.mark.parametrize("mode", [_lib.MODE_FUSED, MODE_PLAIN])
+def test_reference_elastic_driver_switches(mode):
+    """The reference's elastic run (elas_cpp/src/WaveOKL3d.cpp:453-506, 1675-1693): its media
+    (lambda, mu jump across z = 0, mu averaged per element), the smoothed source at (0, 0, 0.1)
+    on v_x with amplitude 1e4, the Ricker pulse evaluated at each step's start for all five
+    stages; 20 steps against the FP64 oracle with the same step-start source time."""
+    print(f"case reference elastic driver mode={mode}", flush=True)
+    N = 3
+    ops = O.build_ref_ops(N)
+    m = M.cube_mesh(4)
+    M.face_pair(m)
+    dm = M.device_arrays(m, ops, 9)
+    weight = lambda x, y, z: 0.1 * np.sin(3 * x) * np.cos(2 * y)
+    mats = tuple(np.asarray(a, dtype=np.float32) for a in media.elastic_reference_media(m, ops, weight))
+    s = WaveSolver(m, N, nfields=9, ops=ops, dmesh=dm)
+    s.InitElastic(*mats)
+    elems, coef = media.gaussian_source(m, ops, (0.0, 0.0, 0.1), a=100.0)
+    w = np.array([1, 0, 0, 0, 0, 0, 0, 0, 0], dtype=np.float64)
+    s.set_source(elems, coef, f0=10.0, amp=1e4, weights=w)
+    s.set_source_step_time(True)
+    Q0 = np.zeros((m.K, 9, ops.Np), dtype=np.float32)
+    s.WaveSetData3d(Q0)
+    dt, nsteps = s.dt_ref, 20
+    s.step(nsteps, dt, mode=mode, graph=True, t0=0.05)
+    got = s.WaveGetData3d()
+    src = np.zeros((m.K, ops.Np)); src[elems] = coef
+    ref = Q0.astype(np.float64)
+    orc.rk_run_elas(orc.OracleOps(ops, np.float64), m.K, dm.vgeo, dm.fgeo, dm.vmapP, *mats, ref, dt, nsteps,
+                    t0=0.05, src=src, srcw=w, f0=10.0, amp=1e4, src_time="step")
+    assert np.abs(ref).max() > 0
+    assert rel_l2(got, ref) < 1e-5
+    # the stage-time source differs from it at first order in dt
+    ref2 = Q0.astype(np.float64)
+    orc.rk_run_elas(orc.OracleOps(ops, np.float64), m.K, dm.vgeo, dm.fgeo, dm.vmapP, *mats, ref2, dt, nsteps,
+                    t0=0.05, src=src, srcw=w, f0=10.0, amp=1e4)
+    assert rel_l2(got, ref2) > 1e-4
+    s.close()

@@ -35,3 +35,27 @@ def test_c2_range():
     x = np.random.default_rng(0).uniform(-0.5, 0.5, (3, 10000))
     c = g(*x)
     assert c.min() >= 0.6 and c.max() <= 1.4
+
+
+def test_element_average_and_reference_elastic_media():
+    """media.element_average is the cubature-weighted mean (exact for a linear field: the mean
+    of a linear function over a tetrahedron is its value at the centroid); the reference's
+    elastic media (elas_cpp/src/WaveOKL3d.cpp:453-506) are rho = 1, (lambda, mu) = (1, 1 + w)
+    above z = 0 and (2, 2 + w) below, mu averaged per element."""
+    from bbwadg_amd import operators as O
+    ops = O.build_ref_ops(3)
+    m = M.cube_mesh(2)
+    xq, yq, zq = media.cubature_points(m, ops)
+    f = 1.0 + 2.0 * xq - yq + 0.5 * zq
+    avg = media.element_average(ops, f)
+    cx, cy, cz = (G.mean(axis=1) for G in (m.GX, m.GY, m.GZ))
+    assert np.allclose(avg, (1.0 + 2.0 * cx - cy + 0.5 * cz)[:, None], atol=1e-13)
+    w = lambda x, y, z: 0.1 * np.sin(3 * x) * np.cos(2 * y)
+    rinv, lam, mu = media.elastic_reference_media(m, ops, w, average_mu=False)
+    below = zq < 0
+    assert np.all(rinv == 1.0)
+    assert np.array_equal(lam, np.where(below, 2.0, 1.0))
+    assert np.allclose(mu, np.where(below, 2.0, 1.0) + w(xq, yq, zq))
+    _, _, mu_avg = media.elastic_reference_media(m, ops, w)
+    assert np.allclose(mu_avg, media.element_average(ops, mu))
+    assert np.allclose(mu_avg.std(axis=1), 0.0)
