@@ -32,13 +32,13 @@ EXPORTS = [
     "bbw_adjoint_gradient", "bbw_batch_create", "bbw_batch_destroy", "bbw_batch_last_error",
     "bbw_batch_upload_operators", "bbw_batch_add_mesh", "bbw_batch_add_shot", "bbw_batch_run",
     "bbw_batch_get_state", "bbw_batch_reset", "bbw_batch_last_ms", "bbw_batch_set_step_cap",
-    "bbw_upload_bbwadg", "bbw_update_bbwadg",
+    "bbw_upload_bbwadg", "bbw_update_bbwadg", "bbw_upload_slice_lift",
 ]
 OPT_FUSED_IMPL, OPT_KBLK_V, OPT_KBLK_S, OPT_KBLK_U, OPT_KBLK_F, OPT_ADJ_CACHE_MB, OPT_ADJ_IMPL = 0, 1, 2, 3, 4, 5, 6
 OPT_PLAIN_IMPL = 7
 OPT_SRC_STEP_TIME = 8
 FUSED_VECTOR, FUSED_MFMA = 0, 1
-PLAIN_FLAT, PLAIN_TILED = 0, 1
+PLAIN_FLAT, PLAIN_TILED, PLAIN_SLICE = 0, 1, 2
 
 
 class BBWLibraryError(RuntimeError):
@@ -114,6 +114,7 @@ def load():
         "bbw_set_option": ([P, I, I], I),
         "bbw_upload_adjoint_maps": ([P, P], I),
         "bbw_upload_bbwadg": ([P, P, P, P, P, P, P, P], I),
+        "bbw_upload_slice_lift": ([P, P, P, P, P], I),
         "bbw_update_bbwadg": ([P], I),
         "bbw_misfit": ([P, I, D, D, P, ctypes.POINTER(D)], I),
         "bbw_misfit_device": ([P, I, D, D, P, ctypes.POINTER(D)], I),

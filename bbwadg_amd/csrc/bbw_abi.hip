@@ -89,6 +89,15 @@ int check_ready(bbw_ctx *c) {
   if (!c->have_ops) return fail(c, BBW_ERR_STATE, "operators not uploaded");
   if (!c->have_mesh) return fail(c, BBW_ERR_STATE, "mesh not uploaded");
   if (!c->have_media) return fail(c, BBW_ERR_STATE, "media not uploaded");
+  if (c->plain_impl == BBW_PLAIN_SLICE && !c->have_slice)
+    return fail(c, BBW_ERR_STATE, "slice-lift tables not uploaded");
+  return BBW_OK;
+}
+
+// paths whose kernels are instantiated for N <= BBW_MAX_ORDER_TILED only
+int check_tiled_order(bbw_ctx *c, const char *what) {
+  if (c->N > BBW_MAX_ORDER_TILED)
+    return fail(c, BBW_ERR_UNSUPPORTED, std::string(what) + " is compiled for N <= 6");
   return BBW_OK;
 }
 
@@ -144,6 +153,7 @@ int bbw_create(const bbw_config *cfg, bbw_ctx **out) {
   *out = nullptr;
   if (!order_supported(cfg->N)) return BBW_ERR_UNSUPPORTED;
   if (cfg->Nfields != 4 && cfg->Nfields != 9) return BBW_ERR_UNSUPPORTED;
+  if (cfg->Nfields == 9 && cfg->N > BBW_MAX_ORDER_TILED) return BBW_ERR_UNSUPPORTED;
   if (cfg->K <= 0 || cfg->Nq != nq_of(cfg->N)) return BBW_ERR_ARG;
   if ((double)cfg->K * cfg->Nfields * np_of(cfg->N) >= 2147483647.0) return BBW_ERR_ARG;
   bbw_ctx *c = new bbw_ctx();
@@ -158,6 +168,7 @@ int bbw_create(const bbw_config *cfg, bbw_ctx **out) {
   c->KblkS = cfg->KblkS;
   c->KblkU = cfg->KblkU;
   c->KblkF = cfg->KblkF;
+  if (c->N > BBW_MAX_ORDER_TILED) c->plain_impl = BBW_PLAIN_SLICE;
   if (hipSetDevice(c->dev) != hipSuccess || hipStreamCreateWithFlags(&c->own_stream, hipStreamNonBlocking) != hipSuccess) {
     delete c;
     return BBW_ERR_HIP;
@@ -203,6 +214,7 @@ void bbw_destroy(bbw_ctx *c) {
   dfree(c->Dids); dfree(c->Dvals); dfree(c->Fmask); dfree(c->EEL_ids); dfree(c->L0_ids);
   dfree(c->EEL_vals); dfree(c->L0_vals); dfree(c->Vq); dfree(c->Pq); dfree(c->Vq64); dfree(c->Pq64); dfree(c->VqPad); dfree(c->PqPad); dfree(c->EEL_ids4); dfree(c->LiftPad); dfree(c->VqBlk); dfree(c->PqBlk); dfree(c->LiftBlk); dfree(c->PqTBlk); dfree(c->VqTBlk); dfree(c->LiftTBlk); dfree(c->PqTPad); dfree(c->VqTPad);
   dfree(c->bb_CN); dfree(c->bb_PBlk); dfree(c->c2bb);
+  dfree(c->slice_inv); dfree(c->red_ids); dfree(c->red_vals); dfree(c->cEL);
   dfree(c->DTids); dfree(c->DTvals); dfree(c->Sids); dfree(c->Eids); dfree(c->Evals); dfree(c->Swts); dfree(c->EELT_ids); dfree(c->EELT_vals); dfree(c->L0T_ids);
   dfree(c->L0T_vals); dfree(c->FmaskInv); dfree(c->mapP); dfree(c->lQ); dfree(c->lres);
   dfree(c->Zadj); dfree(c->Hadj); dfree(c->grad); dfree(c->Ybuf); dfree(c->ckpt);
@@ -476,7 +488,7 @@ int bbw_upload_operators(bbw_ctx *c, const bbw_ops *o) {
   // FP64 projection for the energy reductions: Pq64 = MB^-1 Vq^T W with MB = Vq^T W Vq (the
   // mass matrix, exact for the degree 2N + 1 rule), by a Cholesky factorisation of MB
   std::vector<double> p64((size_t)Np * Nq);
-  {
+  if (c->N <= BBW_MAX_ORDER_TILED) {
     std::vector<double> L((size_t)Np * Np, 0.0);
     for (int i = 0; i < Np; ++i)
       for (int j = 0; j <= i; ++j) {
@@ -514,6 +526,55 @@ int bbw_upload_operators(bbw_ctx *c, const bbw_ops *o) {
   std::memcpy(c->rk4b, o->rk4b, 5 * sizeof(double));
   std::memcpy(c->rk4c, o->rk4c, 6 * sizeof(double));
   c->have_ops = true;
+  reset_graph(c);
+  return BBW_OK;
+}
+
+int bbw_upload_slice_lift(bbw_ctx *c, const int32_t *slice_ids, const int32_t *red_ids,
+                          const float *red_vals, const float *cEL) {
+  if (!c || !slice_ids || !red_ids || !red_vals || !cEL) return BBW_ERR_ARG;
+  if (c->Nf != 4) return fail(c, BBW_ERR_UNSUPPORTED, "the slice-by-slice lift is acoustic");
+  if (!c->have_ops) return fail(c, BBW_ERR_STATE, "upload the operators first (Fmask)");
+  const int N = c->N, Np = c->Np, Nfp = c->Nfp;
+  std::vector<int> fm((size_t)4 * Nfp);
+  HIPCHK(c, hipMemcpy(fm.data(), c->Fmask, fm.size() * sizeof(int), hipMemcpyDeviceToHost));
+  // inverse table: node n, face f -> (row of n in f's slices) | (slice << 16); every face's
+  // rows must be a permutation of the nodes, slice 0 the face in Fmask order
+  std::vector<int> inv((size_t)4 * Np, -1);
+  for (int f = 0; f < 4; ++f) {
+    int sk = 0;
+    for (int s = 0; s <= N; ++s) {
+      const int nn = (N - s + 1) * (N - s + 2) / 2;
+      for (int r = 0; r < nn; ++r, ++sk) {
+        const int n = slice_ids[4 * sk + f];
+        if (n < 0 || n >= Np || inv[4 * n + f] >= 0)
+          return fail(c, BBW_ERR_ARG, "slice_ids: a face's rows are not a permutation of the nodes");
+        if (s == 0 && n != fm[(size_t)f * Nfp + r])
+          return fail(c, BBW_ERR_ARG, "slice_ids: slice 0 must be the face nodes in Fmask order");
+        inv[4 * n + f] = sk | (s << 16);
+      }
+    }
+  }
+  // reduction rows of slice s index slice s - 1
+  int row = 0;
+  for (int s = 1; s <= N; ++s) {
+    const int nn = (N - s + 1) * (N - s + 2) / 2, nprev = (N - s + 2) * (N - s + 3) / 2;
+    for (int r = 0; r < nn; ++r, ++row)
+      for (int j = 0; j < 3; ++j)
+        if (red_ids[3 * row + j] < 0 || red_ids[3 * row + j] >= nprev)
+          return fail(c, BBW_ERR_ARG, "red_ids out of range of the previous slice");
+  }
+  dfree(c->slice_inv);
+  dfree(c->red_ids);
+  dfree(c->red_vals);
+  dfree(c->cEL);
+  c->have_slice = false;
+  int rc;
+  if ((rc = upload(c, (int **)&c->slice_inv, inv.data(), inv.size()))) return rc;
+  if ((rc = upload(c, &c->red_ids, red_ids, (size_t)3 * (Np - Nfp)))) return rc;
+  if ((rc = upload(c, &c->red_vals, red_vals, (size_t)3 * (Np - Nfp)))) return rc;
+  if ((rc = upload(c, &c->cEL, cEL, (size_t)N + 1))) return rc;
+  c->have_slice = true;
   reset_graph(c);
   return BBW_OK;
 }
@@ -696,6 +757,7 @@ int bbw_upload_bbwadg(bbw_ctx *c, const int32_t *subtet_ids, const double *CNsca
   if (!c || !subtet_ids || !CNscale || !invC2Nscale || !Ei_ids4 || !Ei_vals4 || !cj || !c2_bb)
     return BBW_ERR_ARG;
   if (c->Nf != 4) return fail(c, BBW_ERR_UNSUPPORTED, "the exact BB-WADG update is acoustic");
+  if (int rc = check_tiled_order(c, "the exact BB-WADG update")) return rc;
   const int N = c->N, Np = c->Np, N2 = 2 * N, N2p = np_of(N2);
   auto npd = [](int d) { return (d + 1) * (d + 2) * (d + 3) / 6; };
   // the kernel forms the degree-2N index of a + b arithmetically, Np(2N) - Np(2N - l) +
@@ -780,7 +842,7 @@ int bbw_update(bbw_ctx *c, float fa, float fb, float fdt) {
 int bbw_stage_fused(bbw_ctx *c, float fa, float fb, float fdt, float ftime, int projU) {
   if (!c) return BBW_ERR_ARG;
   int rc = check_ready(c);
-  if (rc) return rc;
+  if (rc || (rc = check_tiled_order(c, "the fused stage"))) return rc;
   if (c->Nf == 9) {
     hipError_t e = launch_fused_elas(c, c->Q[c->cur], c->Q[1 - c->cur], fa, fb, fdt, nullptr, ftime);
     if (e != hipSuccess) return hipfail(c, e, "fused elastic stage launch");
@@ -800,6 +862,7 @@ int bbw_step(bbw_ctx *c, int nsteps, double dt, double t0, int mode, int flags) 
     return BBW_ERR_ARG;
   int rc = check_ready(c);
   if (rc) return rc;
+  if (mode == BBW_MODE_FUSED && (rc = check_tiled_order(c, "the fused stage"))) return rc;
   if (c->nrec && c->rec_steps + nsteps > c->rec_max_steps)
     return fail(c, BBW_ERR_ARG, "receiver trace buffer too short");
   const bool prepare = (flags & 4) != 0;
@@ -927,7 +990,7 @@ int bbw_step(bbw_ctx *c, int nsteps, double dt, double t0, int mode, int flags) 
 static int energy_common(bbw_ctx *c, int kind, double *E) {
   if (!c || !E) return BBW_ERR_ARG;
   int rc = check_ready(c);
-  if (rc) return rc;
+  if (rc || (rc = check_tiled_order(c, "the energy reduction"))) return rc;
   if ((rc = dalloc(c, &c->partial, (size_t)BBW_ENERGY_BLOCKS + 1))) return rc;
   hipError_t e = launch_energy(c, kind, c->partial, c->partial + BBW_ENERGY_BLOCKS);
   if (e != hipSuccess) return hipfail(c, e, "energy launch");
@@ -943,7 +1006,7 @@ int bbw_energy_wadg(bbw_ctx *c, double *E) { return energy_common(c, BBW_ENERGY_
 int bbw_energy_noalloc(bbw_ctx *c, double *E) {
   if (!c || !E) return BBW_ERR_ARG;
   int rc = check_ready(c);
-  if (rc) return rc;
+  if (rc || (rc = check_tiled_order(c, "the energy reduction"))) return rc;
   double *scr = energy_scratch(c->dev);
   if (!scr) return fail(c, BBW_ERR_STATE, "energy scratch array not set up on this device");
   // the reduction writes one partial sum per block of a grid capped at BBW_ENERGY_BLOCKS (so any
@@ -1010,7 +1073,7 @@ int device_misfit(bbw_ctx *c, int nsteps, const float *d_dev, float *resid, doub
 
 int check_inverse(bbw_ctx *c, int nsteps, const float *d_obs) {
   int rc = check_ready(c);
-  if (rc) return rc;
+  if (rc || (rc = check_tiled_order(c, "the misfit / adjoint path"))) return rc;
   if (c->Nf != 4) return fail(c, BBW_ERR_UNSUPPORTED, "the adjoint is acoustic only");
   if (!d_obs || nsteps <= 0) return fail(c, BBW_ERR_ARG, "bad observations");
   if (c->nrec == 0 || c->rec_max_steps < nsteps) return fail(c, BBW_ERR_STATE, "receivers not set for nsteps");
@@ -1198,7 +1261,12 @@ int bbw_set_option(bbw_ctx *c, int option, int value) {
     case BBW_OPT_ADJ_CACHE_MB: c->ycache_mb = value; break;
     case BBW_OPT_ADJ_IMPL: c->adj_impl = value ? 1 : 0; break;
     case BBW_OPT_PLAIN_IMPL:
-      if (value != BBW_PLAIN_FLAT && value != BBW_PLAIN_TILED) return fail(c, BBW_ERR_ARG, "bad plain impl");
+      if (value != BBW_PLAIN_FLAT && value != BBW_PLAIN_TILED && value != BBW_PLAIN_SLICE)
+        return fail(c, BBW_ERR_ARG, "bad plain impl");
+      if (value == BBW_PLAIN_TILED && c->N > BBW_MAX_ORDER_TILED)
+        return fail(c, BBW_ERR_UNSUPPORTED, "the tiled plain kernels are compiled for N <= 6");
+      if (value == BBW_PLAIN_SLICE && c->Nf != 4)
+        return fail(c, BBW_ERR_UNSUPPORTED, "the slice-by-slice lift is acoustic");
       c->plain_impl = value;
       break;
     case BBW_OPT_SRC_STEP_TIME: c->src_step_time = value ? 1 : 0; break;

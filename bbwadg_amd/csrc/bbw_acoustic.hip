@@ -535,7 +535,7 @@ static hipError_t fused_n(bbw_ctx *c, const float *Qin, float *Qout, float fa, f
   return hipGetLastError();
 }
 
-bool order_supported(int N) { return N >= 1 && N <= 6; }
+bool order_supported(int N) { return N >= 1 && N <= 9; }
 
 hipError_t launch_volume(bbw_ctx *c) {
   if (c->plain_impl == BBW_PLAIN_TILED) return launch_volume_tiled(c);
@@ -546,12 +546,16 @@ hipError_t launch_volume(bbw_ctx *c) {
     case 4: return volume_n<4>(c);
     case 5: return volume_n<5>(c);
     case 6: return volume_n<6>(c);
+    case 7: return volume_n<7>(c);
+    case 8: return volume_n<8>(c);
+    case 9: return volume_n<9>(c);
   }
   return hipErrorInvalidValue;
 }
 
 hipError_t launch_surface(bbw_ctx *c) {
   if (c->plain_impl == BBW_PLAIN_TILED) return launch_surface_tiled(c);
+  if (c->plain_impl == BBW_PLAIN_SLICE) return launch_surface_slice(c);
   switch (c->N) {
     case 1: return surface_n<1>(c);
     case 2: return surface_n<2>(c);
@@ -559,6 +563,9 @@ hipError_t launch_surface(bbw_ctx *c) {
     case 4: return surface_n<4>(c);
     case 5: return surface_n<5>(c);
     case 6: return surface_n<6>(c);
+    case 7: return surface_n<7>(c);
+    case 8: return surface_n<8>(c);
+    case 9: return surface_n<9>(c);
   }
   return hipErrorInvalidValue;
 }
@@ -573,6 +580,9 @@ hipError_t launch_update_wadg(bbw_ctx *c, float fa, float fb, float fdt, const f
     case 4: return update_wadg_n<4>(c, fa, fb, fdt, ftd, ft);
     case 5: return update_wadg_n<5>(c, fa, fb, fdt, ftd, ft);
     case 6: return update_wadg_n<6>(c, fa, fb, fdt, ftd, ft);
+    case 7: return update_wadg_n<7>(c, fa, fb, fdt, ftd, ft);
+    case 8: return update_wadg_n<8>(c, fa, fb, fdt, ftd, ft);
+    case 9: return update_wadg_n<9>(c, fa, fb, fdt, ftd, ft);
   }
   return hipErrorInvalidValue;
 }

@@ -224,6 +224,7 @@ extern "C" {
 int bbw_batch_create(int N, int device, bbw_batch **out) {
   if (!out) return BBW_ERR_ARG;
   *out = nullptr;
+  if (N < 1 || N > BBW_MAX_ORDER_TILED) return BBW_ERR_UNSUPPORTED;
   bbw_config cfg;
   std::memset(&cfg, 0, sizeof cfg);
   cfg.N = N;

@@ -16,8 +16,10 @@ struct Dims {
   static constexpr int Np = (N + 1) * (N + 2) * (N + 3) / 6;
   static constexpr int Nfp = (N + 1) * (N + 2) / 2;
   static constexpr int NfpNf = 4 * Nfp;
-  // points of the degree 2N+1 tetrahedral rule used by the WADG update
-  static constexpr int Nq = N == 1 ? 6 : N == 2 ? 14 : N == 3 ? 31 : N == 4 ? 57 : N == 5 ? 95 : 146;
+  // points of the WADG rule: the degree 2N+1 tetrahedral rule for N <= 7, the (N+1)^3
+  // tensor-product set for N = 8, 9
+  static constexpr int Nq = N == 1 ? 6 : N == 2 ? 14 : N == 3 ? 31 : N == 4 ? 57 : N == 5 ? 95
+                          : N == 6 ? 146 : N == 7 ? 214 : (N + 1) * (N + 1) * (N + 1);
   static constexpr int EEL_nnz = Nfp + 3;
   static constexpr int L0_nnz = Nfp < 7 ? Nfp : 7;
 };
@@ -25,9 +27,12 @@ struct Dims {
 inline int np_of(int N) { return (N + 1) * (N + 2) * (N + 3) / 6; }
 inline int nfp_of(int N) { return (N + 1) * (N + 2) / 2; }
 inline int nq_of(int N) {
-  static const int t[7] = {0, 6, 14, 31, 57, 95, 146};
-  return (N >= 1 && N <= 6) ? t[N] : -1;
+  static const int t[10] = {0, 6, 14, 31, 57, 95, 146, 214, 729, 1000};
+  return (N >= 1 && N <= 9) ? t[N] : -1;
 }
+// Orders 7..9 run the acoustic plain stage on the flat kernels with the slice-by-slice lift;
+// the tiled, fused, adjoint, batched, elastic and energy kernels are instantiated for N <= 6.
+constexpr int BBW_MAX_ORDER_TILED = 6;
 
 // Energy reductions: kinds and the fixed number of partial sums (one per block of the grid).
 enum { BBW_ENERGY_QUADRATURE = 0, BBW_ENERGY_WADG = 1 };
@@ -56,7 +61,8 @@ struct bbw_ctx {
   int KblkV = 0, KblkS = 0, KblkU = 0, KblkF = 0;
   int fused_impl = 1;  // BBW_FUSED_VECTOR or BBW_FUSED_MFMA
   int adj_impl = 1;    // 0: per-element adjoint kernels, 1: tiled MFMA adjoint kernels
-  int plain_impl = 1;  // plain kernels: 0 flat per-element kernels (Kblk*), 1 16-element tiles
+  int plain_impl = 1;  // plain kernels: 0 flat per-element kernels (Kblk*), 1 16-element tiles,
+                       // 2 flat kernels with the slice-by-slice lift
   hipStream_t own_stream = nullptr, stream = nullptr;
   std::string err;
 
@@ -89,6 +95,13 @@ struct bbw_ctx {
   double *Pq64 = nullptr;  // (Vq64^T W Vq64)^-1 Vq64^T W, column-major Np x Nq (energy reductions)
   double rk4a[5] = {0}, rk4b[5] = {0}, rk4c[6] = {0};
   bool have_ops = false;
+  // slice-by-slice lift (bbw_upload_slice_lift): per node and face, the node's row in that
+  // face's concatenated slices plus (slice << 16) [Np] int4; one-step reductions [Np-Nfp][3]
+  // (ids into the previous slice, values); cEL [N+1]
+  int4 *slice_inv = nullptr;
+  int *red_ids = nullptr;
+  float *red_vals = nullptr, *cEL = nullptr;
+  bool have_slice = false;
 
   // mesh
   float *vgeo = nullptr, *fgeo = nullptr;
@@ -178,6 +191,7 @@ hipError_t launch_volume_elas_tiled(bbw_ctx *c);
 hipError_t launch_update_elas_mfma(bbw_ctx *c, float fa, float fb, float fdt, const float *ftime_dev,
                                    float ftime);
 hipError_t launch_surface_tiled(bbw_ctx *c);
+hipError_t launch_surface_slice(bbw_ctx *c);
 hipError_t launch_fused(bbw_ctx *c, const float *Qin, float *Qout, float fa, float fb, float fdt,
                         const float *ftime_dev, float ftime, int project_velocity);
 hipError_t launch_fused_mfma(bbw_ctx *c, const float *Qin, float *Qout, float fa, float fb,

@@ -1,4 +1,4 @@
-"""Regenerate ``nodedata.json``: interpolation nodes and cubature rules for N = 1..7.
+"""Regenerate ``nodedata.json``: interpolation nodes for N = 1..9 and cubature rules of degree 1..19.
 
 The solver needs the same point sets as the reference so that the WADG
 projection, the nodal initial-condition interpolation and the cubature-based
@@ -9,8 +9,9 @@ degree, triangle rules ``p_rfq_N#``/``p_wfq_N#``; loaded by ``Nodes3D``,
 ``tet_cubature`` and ``tri_cubature`` at ``cpp/src/Basis.cpp:799-949``).
 
 This script reads that header as text and writes the numbers only, as JSON,
-for node orders 1..7 and cubature degrees 1..15 (the solver uses orders 3..5,
-i.e. cubature degrees 2N+1 = 7, 9, 11).  It needs the reference checkout and
+for node orders 1..9 and cubature degrees 1..19 (the solver supports orders 1..9: the WADG
+update uses the degree 2N+1 rule up to N = 7, the face and mass-matrix rules go to degree
+2N+1 = 19 at N = 9).  It needs the reference checkout and
 is only run when the table must be rebuilt; the JSON it writes is what the
 package loads.  The tables are checked independently by ``tests/test_tables.py``
 (cubature exactness on monomials, node symmetry and face counts).
@@ -41,9 +42,9 @@ def main(path):
         text = f.read()
     arrays, ints = parse(text)
     out = {"source": "cpp/include/NodeData.h (jlchan/bbwadg)", "nodes": {}, "tet": {}, "tri": {}}
-    for N in range(1, 8):
+    for N in range(1, 10):
         out["nodes"][str(N)] = {k: arrays[("p_" + k, N)] for k in ("r", "s", "t")}
-    for deg in range(1, 16):
+    for deg in range(1, 20):
         nq = ints[("p_Nq", deg)]
         out["tet"][str(deg)] = {k: arrays[("p_" + k + "q", deg)] for k in ("r", "s", "t", "w")}
         assert all(len(v) == nq for v in out["tet"][str(deg)].values())

@@ -137,8 +137,13 @@ class RefOps:
         return NFACES * self.Nfp
 
 
-def build_ref_ops(N):
-    """Build every reference-element operator for order N (``StartUp3d`` + WADG)."""
+def build_ref_ops(N, tp_rule="reference"):
+    """Build every reference-element operator for order N (``StartUp3d`` + WADG).
+
+    ``tp_rule`` picks the WADG cubature for 2N+1 > 15 (N = 8, 9): "reference" is the
+    reference's un-collapsed tensor-product set (``wadg_rule``), "collapsed" the same 1D rules
+    mapped through the collapsed coordinates (``wadg_rule_collapsed``).  Orders N <= 7 use the
+    tetrahedral rule of degree 2N+1 either way."""
     if N < 1:
         raise ValueError("N must be >= 1")
     Np, Nfp = basis.np_tet(N), basis.np_tri(N)
@@ -211,18 +216,140 @@ def build_ref_ops(N):
     L0_ids, L0_vals, widths["L0"] = _fixed_width(cols, vals, L0_nnz)
 
     # --- WADG projection on the degree 2N+1 rule (WaveOKL3d.cpp:425-483)
-    if 2 * N + 1 > 15:
-        raise NotImplementedError("orders with 2N+1 > 15 use a tensor-product rule (not provided)")
-    rqw, sqw, tqw, wqw = basis.tet_cubature(min(21, 2 * N + 1))
+    if N > 9:
+        raise ValueError("orders above 9 are not supported")
+    if tp_rule == "reference":
+        rqw, sqw, tqw, wqw = wadg_rule(N)
+    elif tp_rule == "collapsed":
+        rqw, sqw, tqw, wqw = wadg_rule_collapsed(N)
+    else:
+        raise ValueError("tp_rule must be 'reference' or 'collapsed'")
     Vq = basis.bern_tet(N, rqw, sqw, tqw)
-    Mq = Vq.T @ (wqw[:, None] * Vq)
-    Pq = np.linalg.solve(Mq, Vq.T * wqw[None, :])
+    if 2 * N + 1 <= 15:
+        Mq = Vq.T @ (wqw[:, None] * Vq)
+        Pq = np.linalg.solve(Mq, Vq.T * wqw[None, :])
+    else:
+        # same operator through a QR of W^1/2 Vq: the tensor-product points lie partly outside
+        # the element, where Bernstein values reach 1e4 and Vq^T W Vq has condition ~1e13
+        sw = np.sqrt(wqw)
+        Qf, Rf = np.linalg.qr(Vq * sw[:, None])
+        Pq = np.linalg.solve(Rf, Qf.T) * sw[None, :]
 
     return RefOps(N=N, Np=Np, Nfp=Nfp, r=r, s=s, t=t, Fmask=Fmask, VB=VB, invVB=invVB,
                   DB=DB, D_ids=D_ids, D_vals=D_vals, LIFTB=LIFTB, EEL=EEL,
                   EEL_ids=EEL_ids, EEL_vals=EEL_vals, EEL_nnz=EEL_nnz, L0=L0,
                   L0_ids=L0_ids, L0_vals=L0_vals, L0_nnz=L0_nnz, cEL=cEL, rowperm=rowperm,
                   MB=MB, rq=rqw, sq=sqw, tq=tqw, wq=wqw, Vq=Vq, Pq=Pq, sparse_widths=widths)
+
+
+def wadg_rule(N):
+    """The reference's WADG cubature (``InitWADG_subelem``, cpp/src/WaveOKL3d.cpp:425-458): the
+    tetrahedral rule of degree 2N+1 when 2N+1 <= 15, else its (N+1)^3-point tensor-product set:
+    r = a_i, s = a_j, t = c_k with a, wa the (N+1)-point Gauss-Jacobi(0,0) rule and c, wc the
+    Gauss-Jacobi(2,0) rule, w = wa_i wa_j wc_k (k outer, i, j inner).  That set is used as the
+    reference uses it, without a collapsed-coordinate map, so its points are not all inside the
+    tetrahedron: Pq = (Vq^T W Vq)^-1 Vq^T W is then a weighted least-squares fit on those points
+    (Pq Vq = I still holds)."""
+    if 2 * N + 1 <= 15:
+        return basis.tet_cubature(min(21, 2 * N + 1))
+    from scipy.special import roots_jacobi
+    a, wa = roots_jacobi(N + 1, 0.0, 0.0)
+    c, wc = roots_jacobi(N + 1, 2.0, 0.0)
+    K, I, J = np.meshgrid(np.arange(N + 1), np.arange(N + 1), np.arange(N + 1), indexing="ij")
+    K, I, J = K.ravel(), I.ravel(), J.ravel()
+    return a[I], a[J], c[K], wa[I] * wa[J] * wc[K]
+
+
+def wadg_rule_collapsed(N):
+    """A degree 2N+1 rule on the reference tetrahedron with (N+1)^3 points for 2N+1 > 15: the
+    Gauss-Jacobi (0,0), (1,0) and (2,0) rules in the collapsed coordinates (a, b, c), mapped by
+    t = c, s = (1+b)(1-c)/2 - 1, r = (1+a)(1-b)(1-c)/4 - 1 (Jacobian (1-b)(1-c)^2/8); the
+    weights of the (1,0) and (2,0) rules absorb (1-b) and (1-c)^2, so w = wa wb wc / 8.  All
+    points lie inside the element, so the projection is a true L2 projection there.
+
+    The reference's set (``wadg_rule``) skips this map: its points fill the cube [-1,1]^3 and the
+    WADG operator Pq diag(c2) Vq has eigenvalues ~1e7 once c2 varies, so explicit time
+    stepping diverges for heterogeneous media at N = 8, 9; with constant c2 both give Pq Vq = I.
+    For 2N+1 <= 15 the tetrahedral rule is returned, as by ``wadg_rule``."""
+    if 2 * N + 1 <= 15:
+        return basis.tet_cubature(min(21, 2 * N + 1))
+    from scipy.special import roots_jacobi
+    a, wa = roots_jacobi(N + 1, 0.0, 0.0)
+    b, wb = roots_jacobi(N + 1, 1.0, 0.0)
+    c, wc = roots_jacobi(N + 1, 2.0, 0.0)
+    K, I, J = np.meshgrid(np.arange(N + 1), np.arange(N + 1), np.arange(N + 1), indexing="ij")
+    A, B, C = a[I.ravel()], b[J.ravel()], c[K.ravel()]
+    t = C
+    s = 0.5 * (1 + B) * (1 - C) - 1
+    r = 0.25 * (1 + A) * (1 - B) * (1 - C) - 1
+    w = wa[I.ravel()] * wb[J.ravel()] * wc[K.ravel()] / 8.0
+    return r, s, t, w
+
+
+def slice_lift_ops(ops):
+    """Tables of the slice-by-slice Bernstein lift (``rk_surface_bern_slice``,
+    cpp/okl/WaveKernels.okl:652-939; set up in cpp/src/StartUp3d.cpp:409-505).
+
+    The lift of face f's data phi_f is EEL (L0 phi_f): the face's block of EEL puts
+    cEL_s (E_{N-s}^N)^T (L0 phi_f) on the nodes of slice s (the nodes whose barycentric index for
+    the vertex opposite face f equals s), and (E_{N-s}^N)^T is s one-step degree reductions, so
+    slice s follows from slice s - 1 with a 3-non-zero operator.  Returns
+
+    * ``slice_ids`` (Np, 4) int32: row sk = (slice s, its k-th node), column f: the volume node.
+      Within a slice the nodes are in ascending volume order, which is the triangle BB order of
+      the face's nodes (``Fmask``) at degree N - s;
+    * ``red_ids`` / ``red_vals`` (Np - Nfp, 3): the one-step reductions (E_{N-s}^{N-s+1})^T for
+      s = 1..N stacked (rows of slice s), at most 3 non-zeros (zero-padded);
+    * ``cEL`` (N + 1,).
+    """
+    N, Np = ops.N, ops.Np
+    mi = basis.tet_multi_indices(N)
+    opposite = [3, 2, 0, 1]   # faces t = -1, s = -1, r+s+t = -1, r = -1: lam4, lam3, lam1, lam2 = 0
+    slice_ids = np.zeros((Np, 4), dtype=np.int32)
+    for f in range(4):
+        v = opposite[f]
+        order = sorted(range(Np), key=lambda n: (mi[n][v], n))
+        slice_ids[:, f] = order
+    # one-step elevation E (degree d-1 -> d on the triangle) from the BB identity
+    # B^{d-1}_a = sum_m (a_m + 1)/d B^d_{a + e_m}; its transpose has <= 3 non-zeros per row
+    rows_ids, rows_vals = [], []
+    for sl in range(1, N + 1):
+        d = N - sl + 1
+        hi = {tuple(x): n for n, x in enumerate(basis.tri_multi_indices(d))}
+        for a in basis.tri_multi_indices(d - 1):
+            ents = []
+            for m in range(3):
+                b = a.copy()
+                b[m] += 1
+                ents.append((hi[tuple(b)], (a[m] + 1) / d))
+            ents.sort()
+            rows_ids.append([e[0] for e in ents])
+            rows_vals.append([e[1] for e in ents])
+    red_ids = np.array(rows_ids, dtype=np.int32).reshape(-1, 3)
+    red_vals = np.array(rows_vals, dtype=np.float64).reshape(-1, 3)
+    return dict(slice_ids=slice_ids, red_ids=red_ids, red_vals=red_vals, cEL=ops.cEL.copy())
+
+
+def slice_lift_apply(ops, sl, phi):
+    """Host restatement of the slice-by-slice lift for one element: phi (4, Nfp) face data in
+    Fmask order -> (Np,) = LIFTB @ phi.ravel() computed slice by slice (for checking the tables)."""
+    N, Nfp = ops.N, ops.Nfp
+    out = np.zeros(ops.Np)
+    for f in range(4):
+        cur = ops.L0 @ phi[f]
+        sk, r0 = 0, 0
+        for s_ in range(N + 1):
+            nn = (N - s_ + 1) * (N - s_ + 2) // 2
+            if s_ > 0:
+                nxt = np.zeros(nn)
+                for r in range(nn):
+                    for c in range(3):
+                        nxt[r] += sl["red_vals"][r0 + r, c] * cur[sl["red_ids"][r0 + r, c]]
+                r0 += nn
+                cur = nxt
+            out[sl["slice_ids"][sk:sk + nn, f]] += sl["cEL"][s_] * cur
+            sk += nn
+    return out
 
 
 def packed_device_ops(ops):

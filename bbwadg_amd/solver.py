@@ -88,6 +88,12 @@ class WaveSolver:
                      wq=ptr(k["wq"]).value, Vq64=ptr(k["Vq64"]).value, rk4a=ptr(k["rk4a"]).value, rk4b=ptr(k["rk4b"]).value,
                      rk4c=ptr(k["rk4c"]).value)
         check(self.ctx, self.lib.bbw_upload_operators(self.ctx, ctypes.byref(s)))
+        if self.nfields == 4:
+            # slice-by-slice lift tables (the surface kernel for N >= 7, BBW_PLAIN_SLICE at any N)
+            sl = operators.slice_lift_ops(self.ops)
+            self._slice_keep = [i32(sl["slice_ids"]), i32(sl["red_ids"]), f32(sl["red_vals"]),
+                                f32(sl["cEL"])]
+            check(self.ctx, self.lib.bbw_upload_slice_lift(self.ctx, *[ptr(a) for a in self._slice_keep]))
 
     def cubature_points(self):
         return media.cubature_points(self.mesh, self.ops)
@@ -228,15 +234,24 @@ class WaveSolver:
         self.surface()
         self.update_wadg(fa, fb, fdt, ftime)
 
-    def step(self, nsteps, dt, mode=MODE_FUSED, project_velocity=False, graph=True, t0=None):
+    def _mode(self, mode):
+        # the fused stage is compiled for N <= 6; higher orders run the plain stage
+        if mode is None:
+            return MODE_FUSED if self.N <= 6 else MODE_PLAIN
+        return mode
+
+    def step(self, nsteps, dt, mode=None, project_velocity=False, graph=True, t0=None):
+        """``nsteps`` LSRK45 steps; ``mode`` None picks the fused stage for N <= 6, else plain."""
+        mode = self._mode(mode)
         t0 = self.time if t0 is None else t0
         flags = (1 if project_velocity else 0) | (2 if graph else 0)
         check(self.ctx, self.lib.bbw_step(self.ctx, int(nsteps), float(dt), float(t0), int(mode), flags))
         self.time = t0 + nsteps * dt
 
-    def prepare_graph(self, nsteps, dt, mode=MODE_FUSED, project_velocity=False):
+    def prepare_graph(self, nsteps, dt, mode=None, project_velocity=False):
         """Capture, instantiate and upload the step graph for these arguments and size the
         stage-time buffers for a call of ``nsteps`` steps, without stepping."""
+        mode = self._mode(mode)
         flags = (1 if project_velocity else 0) | 2 | 4
         check(self.ctx, self.lib.bbw_step(self.ctx, int(nsteps), float(dt), float(self.time), int(mode), flags))
 

@@ -49,7 +49,8 @@ extern "C" {
 typedef struct bbw_ctx bbw_ctx;
 
 typedef struct {
-  int N;        /* polynomial order (1..6 compiled) */
+  int N;        /* polynomial order: 1..6 every path; 7..9 acoustic plain stages (flat kernels,
+                   slice-by-slice lift), see bbw_upload_slice_lift */
   int Nfields;  /* 4 acoustic (p,u,v,w), 9 elastic (v1,v2,v3,sxx,syy,szz,syz,sxz,sxy) */
   int K;        /* number of elements */
   int Nq;       /* WADG cubature points per element */
@@ -86,6 +87,20 @@ int bbw_sync(bbw_ctx *ctx);
 
 /* ≈ the operator uploads of WaveInitOCCA3d (:1426-1526) and InitWADG_subelem (:906-915). */
 int bbw_upload_operators(bbw_ctx *ctx, const bbw_ops *ops);
+/* ≈ the slice-lift upload of WaveInitOCCA3d (slice_ids, EEL_val_vec / EEL_id_vec and cEL,
+ * cpp/src/WaveOKL3d.cpp:1515-1526; tables built in cpp/src/StartUp3d.cpp:409-505) for the
+ * surface kernel rk_surface_bern_slice (cpp/okl/WaveKernels.okl:652-939).  The lift of face f
+ * puts cEL[s] (E_{N-s}^N)^T (L0 flux_f) on slice s of that face (the nodes s layers from it),
+ * computed slice by slice with one-step degree reductions:
+ *   slice_ids  int32 [Np][4]: row sk of face f (slices 0..N concatenated, slice s holding
+ *              (N-s+1)(N-s+2)/2 nodes) -> volume node; slice 0 must be the face in Fmask order
+ *   red_ids    int32 [Np-Nfp][3], red_vals float [Np-Nfp][3]: row r of slice s >= 1 (rows of
+ *              slices 1..N concatenated) = sum_c red_vals * (slice s-1 value at red_ids), the
+ *              rows of (E_{N-s}^{N-s+1})^T; ids index the previous slice, unused entries 0
+ *   cEL        float [N+1]
+ * Required for N >= 7 (the default surface kernel there) and for BBW_PLAIN_SLICE. */
+int bbw_upload_slice_lift(bbw_ctx *ctx, const int32_t *slice_ids, const int32_t *red_ids,
+                          const float *red_vals, const float *cEL);
 /* ≈ the vgeo/fgeo/vmapP uploads of WaveInitOCCA3d (:1528-1616).  J: FP64 [K] (energy). */
 int bbw_upload_mesh(bbw_ctx *ctx, const float *vgeo, const float *fgeo, const int32_t *vmapP,
                     const double *J);
@@ -232,7 +247,8 @@ double bbw_batch_last_ms(const bbw_batch *b); /* device time of the last bbw_bat
 #define BBW_OPT_ADJ_IMPL 6 /* adjoint kernels: 0 per-element, 1 tiled MFMA (default) */
 #define BBW_OPT_PLAIN_IMPL 7 /* plain kernels (bbw_volume, bbw_surface, bbw_update_wadg): */
 #define BBW_PLAIN_FLAT 0     /* per-element loops over Kblk* elements per block */
-#define BBW_PLAIN_TILED 1    /* default: 16-element tiles, WADG projections on MFMA */
+#define BBW_PLAIN_TILED 1    /* default for N <= 6: 16-element tiles, WADG projections on MFMA */
+#define BBW_PLAIN_SLICE 2    /* flat kernels, surface lifted slice by slice (default for N >= 7) */
 #define BBW_FUSED_VECTOR 0
 #define BBW_FUSED_MFMA 1
 /* source time: 0 (default) evaluates the Ricker pulse at each stage's time t + c_i dt; 1 at the

@@ -154,7 +154,7 @@ void orc_update_wadg_##SFX(int K, int Np, int Nq, int nproj, const R *Vq, const 
   const int Nf = 4;                                                                      \
   _Pragma("omp parallel for schedule(static) if(K >= 2048)")                                           \
   for (int k = 0; k < K; ++k) {                                                          \
-    R sq[4][256], r[4][256];                                                             \
+    R sq[4][1024], r[4][256];                                                            \
     for (int q = 0; q < Nq; ++q) {                                                       \
       R acc[4] = {0, 0, 0, 0};                                                           \
       for (int j = 0; j < Np; ++j) {                                                     \
@@ -326,7 +326,7 @@ void orc_update_elas_##SFX(int K, int Np, int Nq, const R *Vq, const R *Pq,     
   const int Nf = 9;                                                                      \
   _Pragma("omp parallel for schedule(static) if(K >= 2048)")                                           \
   for (int k = 0; k < K; ++k) {                                                          \
-    R sQ[9][256];                                                                        \
+    R sQ[9][1024];                                                                       \
     for (int i = 0; i < Nq; ++i) {                                                       \
       R rq[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};                                             \
       for (int j = 0; j < Np; ++j) {                                                     \

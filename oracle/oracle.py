@@ -58,6 +58,9 @@ class OracleOps:
         dt = np.dtype(dtype)
         self.dtype = dt
         self.N, self.Np, self.Nfp, self.Nq = ops.N, ops.Np, ops.Nfp, ops.Nq
+        # per-element arrays of bbw_oracle.c: Np <= 256 (512 volume), 4 Nfp <= 256, Nq <= 1024
+        if ops.Np > 256 or 4 * ops.Nfp > 256 or ops.Nq > 1024:
+            raise ValueError(f"order {ops.N} exceeds the oracle's per-element arrays")
         self.Dids = np.ascontiguousarray(np.transpose(ops.D_ids, (2, 1, 0)), dtype=np.int32)
         self.Dvals = np.ascontiguousarray(ops.D_vals, dtype=dt)
         self.Fmask = np.ascontiguousarray(ops.Fmask.reshape(-1), dtype=np.int32)

@@ -30,8 +30,12 @@ def test_library_exports_all_symbols():
 def test_bad_config_rejected_without_gpu_work():
     lib = _lib.load()
     h = ctypes.c_void_p()
-    cfg = _lib.Config(N=9, Nfields=4, K=10, Nq=95)
+    cfg = _lib.Config(N=10, Nfields=4, K=10, Nq=95)
     assert lib.bbw_create(ctypes.byref(cfg), ctypes.byref(h)) == _lib.BBW_ERR_UNSUPPORTED
+    cfg = _lib.Config(N=7, Nfields=9, K=10, Nq=214)   # elastic kernels: N <= 6
+    assert lib.bbw_create(ctypes.byref(cfg), ctypes.byref(h)) == _lib.BBW_ERR_UNSUPPORTED
+    cfg = _lib.Config(N=9, Nfields=4, K=10, Nq=729)   # N = 9 has 1000 WADG points
+    assert lib.bbw_create(ctypes.byref(cfg), ctypes.byref(h)) == _lib.BBW_ERR_ARG
     cfg = _lib.Config(N=5, Nfields=5, K=10, Nq=95)
     assert lib.bbw_create(ctypes.byref(cfg), ctypes.byref(h)) == _lib.BBW_ERR_UNSUPPORTED
     cfg = _lib.Config(N=5, Nfields=4, K=10, Nq=94)

@@ -1,0 +1,121 @@
+"""GPU parity of the slice-by-slice lift (``rk_surface_bern_slice``,
+cpp/okl/WaveKernels.okl:652-939) at N = 1..9 and of the orders 7..9 plain stage (flat volume,
+slice surface, WADG update) against the oracle (oracle/bbw_oracle.c, the lift pinned to LIFTB and
+the face-quadrature lift in tests/test_operators.py).
+
+Tolerances: single kernels 2e-5 of the output's max-norm against the FP32 oracle and 1e-5
+against the FP64 oracle (FP32 arithmetic, other summation order); the WADG update at N = 8, 9 on
+the reference's tensor-product set 1e-4 against FP64, since its Vq entries reach 1e4 at the
+set's outside points and the projection cancels them (the FP32 oracle itself differs from FP64
+by 1.6e-5 there); whole runs 2e-5 relative L2 against the FP64 oracle (1e-4 on that set, where
+the FP32 oracle run is itself 3.5e-5 away from FP64).
+"""
+import numpy as np
+import pytest
+
+from bbwadg_amd import _lib, media, mesh as M, operators as O
+from bbwadg_amd.solver import WaveSolver, MODE_FUSED, MODE_PLAIN
+import oracle as orc
+from cases import Problem, rel, rel_l2
+
+pytestmark = pytest.mark.gpu
+
+
+def make(N, n, seed=0, c2="fourier", tp_rule="reference"):
+    P = Problem(N, n, seed=seed, c2=c2)
+    if tp_rule != "reference":
+        P.ops = O.build_ref_ops(N, tp_rule=tp_rule)
+        xq, yq, zq = media.cubature_points(P.mesh, P.ops)
+        P.c2q = media.c2_fourier(seed)(xq, yq, zq) if c2 == "fourier" else np.ones_like(xq)
+    s = WaveSolver(P.mesh, N, ops=P.ops, dmesh=P.dm)
+    s.InitWADG_subelem(P.c2q)
+    return P, s
+
+
+@pytest.mark.parametrize("N", [1, 2, 3, 4, 5, 6, 7, 8, 9])
+def test_surface_slice(N):
+    print(f"case surface slice N={N}", flush=True)
+    P, s = make(N, 3)
+    s.set_option(_lib.OPT_PLAIN_IMPL, _lib.PLAIN_SLICE)
+    Q, R0 = P.random_state(), P.random_state()
+    s.WaveSetData3d(Q)
+    s.set_rhs(R0)
+    s.surface()
+    got = s.get_rhs()
+    for dt, tol in ((np.float32, 2e-5), (np.float64, 1e-5)):
+        ref = R0.astype(dt).copy()
+        orc.surface(orc.OracleOps(P.ops, dt), P.K, P.dm.fgeo.astype(dt), P.dm.vmapP, Q.astype(dt), ref)
+        assert rel(got, ref) < tol
+    # the EEL form of the same lift (flat kernel) agrees
+    s.set_option(_lib.OPT_PLAIN_IMPL, _lib.PLAIN_FLAT)
+    s.set_rhs(R0)
+    s.surface()
+    assert rel(got, s.get_rhs()) < 2e-5
+    s.close()
+
+
+@pytest.mark.parametrize("N", [7, 8, 9])
+def test_flat_kernels_high_order(N):
+    print(f"case flat kernels N={N}", flush=True)
+    P, s = make(N, 3, seed=N)
+    Q, R, res = P.random_state(), P.random_state(), P.random_state()
+    s.WaveSetData3d(Q)
+    s.volume()
+    got = s.get_rhs()
+    for dt, tol in ((np.float32, 2e-5), (np.float64, 1e-5)):
+        ref = np.zeros(Q.shape, dtype=dt)
+        orc.volume(orc.OracleOps(P.ops, dt), P.K, P.dm.vgeo.astype(dt), Q.astype(dt), ref)
+        assert rel(got, ref) < tol
+    s.set_rhs(R)
+    s.set_res(res)
+    fa, fb, fdt = np.float32(P.ops.rk4a[2]), np.float32(P.ops.rk4b[2]), np.float32(1e-2)
+    s.update_wadg(fa, fb, fdt)
+    gQ, gres = s.WaveGetData3d(), s.get_res()
+    tol = 2e-5 if N <= 7 else 1e-4
+    rQ, rres = Q.astype(np.float64).copy(), res.astype(np.float64).copy()
+    orc.update_wadg(orc.OracleOps(P.ops, np.float64), P.K, P.c2q, fa, fb, fdt,
+                    R.astype(np.float64), rres, rQ)
+    assert rel(gres, rres) < tol
+    assert rel(gQ, rQ) < tol
+    s.close()
+
+
+@pytest.mark.parametrize("N,c2,tp_rule", [(7, "fourier", "reference"), (8, "one", "reference"),
+                                           (8, "fourier", "collapsed"), (9, "fourier", "collapsed")])
+def test_run_high_order(N, c2, tp_rule):
+    """Cavity initial state, 6 plain steps through the step graph against the FP64 oracle."""
+    print(f"case run N={N} c2={c2} rule={tp_rule}", flush=True)
+    P, s = make(N, 2, seed=N, c2=c2, tp_rule=tp_rule)
+    Q0 = P.cavity_state()
+    s.WaveSetData3d(Q0)
+    dt, nsteps = P.dt_cfl(), 6
+    s.step(nsteps, dt, graph=True)   # N >= 7: plain stage, slice lift
+    got = s.WaveGetData3d()
+    ref = Q0.astype(np.float64).copy()
+    orc.rk_run(orc.OracleOps(P.ops, np.float64), P.K, P.dm.vgeo, P.dm.fgeo, P.dm.vmapP, P.c2q,
+               ref, dt, nsteps)
+    assert np.abs(ref).max() < 2.0
+    # the reference's N = 8 set loses ~3.5e-5 to FP32 rounding in 6 steps even with c2 = 1 (the
+    # FP32 oracle does the same): 1e-4 there
+    assert rel_l2(got, ref) < (1e-4 if N >= 8 and tp_rule == "reference" else 2e-5)
+    s.close()
+
+
+def test_high_order_limits():
+    """Orders 7..9: the paths compiled for N <= 6 fail with BBW_ERR_UNSUPPORTED, not silently."""
+    print("case high-order limits", flush=True)
+    P, s = make(7, 2)
+    s.WaveSetData3d(P.random_state())
+    for call in (lambda: s.step(1, 1e-3, mode=MODE_FUSED, graph=False),
+                 lambda: s.energy(),
+                 lambda: s.set_option(_lib.OPT_PLAIN_IMPL, _lib.PLAIN_TILED)):
+        with pytest.raises(_lib.BBWError) as ei:
+            call()
+        assert ei.value.code == _lib.BBW_ERR_UNSUPPORTED
+    s.step(1, 1e-3, mode=MODE_PLAIN, graph=False)
+    s.close()
+    ops = O.build_ref_ops(7)
+    m = M.cube_mesh(1)
+    M.face_pair(m)
+    with pytest.raises(_lib.BBWError):
+        WaveSolver(m, 7, nfields=9, ops=ops)

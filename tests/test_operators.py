@@ -9,7 +9,7 @@ from bbwadg_amd import basis, mesh as M, operators as O
 import oracle as orc
 import ref_ops
 
-NS = [1, 2, 3, 4, 5, 6]
+NS = [1, 2, 3, 4, 5, 6, 7, 8, 9]
 
 
 @pytest.fixture(scope="module", params=NS)
@@ -51,8 +51,9 @@ def test_lift_factorisation(ops):
     # EEL . blkdiag(L0) = LIFTB and both equal the independent face-quadrature lift
     Lref = ref_ops.lift_face_quadrature(ops.N, ops.Fmask)
     scale = np.abs(Lref).max()
-    assert np.abs(ops.LIFTB - Lref).max() < 1e-11 * scale
-    assert np.abs(ops.EEL @ B - Lref).max() < 1e-11 * scale
+    tol = 1e-11 if ops.N <= 7 else 1e-10  # the BB mass matrix's condition grows ~4^N
+    assert np.abs(ops.LIFTB - Lref).max() < tol * scale
+    assert np.abs(ops.EEL @ B - Lref).max() < tol * scale
     # the fixed-width sparse forms hold every non-zero (nothing truncated)
     assert ops.sparse_widths["EEL"] <= ops.EEL_nnz
     assert ops.sparse_widths["L0"] <= ops.L0_nnz
@@ -64,11 +65,47 @@ def test_lift_factorisation(ops):
 
 
 def test_projection(ops):
-    assert np.abs(ops.Pq @ ops.Vq - np.eye(ops.Np)).max() < 1e-11
+    # orders 8, 9 project on the tensor-product set, whose outside points make Vq ~1e4
+    tol = 1e-11 if ops.N <= 7 else 1e-7
+    assert np.abs(ops.Pq @ ops.Vq - np.eye(ops.Np)).max() < tol
     # Pq is the L2 projection: Pq f reproduces degree-N polynomials sampled at cubature
     rng = np.random.default_rng(0)
     c = rng.standard_normal(ops.Np)
-    assert np.abs(ops.Pq @ (ops.Vq @ c) - c).max() < 1e-11
+    assert np.abs(ops.Pq @ (ops.Vq @ c) - c).max() < tol
+
+
+def test_slice_lift(ops):
+    """The slice-by-slice lift tables (StartUp3d.cpp:409-505, WaveKernels.okl:652-939): each
+    face's slices partition the nodes, slice s holds the nodes s steps from the face, the
+    one-step reductions are the transposed degree elevations, and lifting slice by slice
+    equals LIFTB (itself pinned to the face-quadrature lift above)."""
+    N, Np, Nfp = ops.N, ops.Np, ops.Nfp
+    sl = O.slice_lift_ops(ops)
+    mi = basis.tet_multi_indices(N)
+    opposite = [3, 2, 0, 1]
+    for f in range(4):
+        assert sorted(sl["slice_ids"][:, f]) == list(range(Np))
+        # slice 0 is the face itself, in Fmask order
+        assert list(sl["slice_ids"][:Nfp, f]) == list(ops.Fmask[f])
+        sk = 0
+        for s_ in range(N + 1):
+            nn = (N - s_ + 1) * (N - s_ + 2) // 2
+            assert np.all(mi[sl["slice_ids"][sk:sk + nn, f], opposite[f]] == s_)
+            sk += nn
+    # a row of E^T (degree d -> d - 1, multi-index a) holds (a_m + 1)/d, m = 0..2: sum 1 + 2/d
+    r0 = 0
+    for s_ in range(1, N + 1):
+        d = N - s_ + 1
+        nn = (d + 1) * d // 2
+        rows = sl["red_vals"][r0:r0 + nn]
+        assert np.allclose(rows.sum(axis=1), 1.0 + 2.0 / d)
+        r0 += nn
+    assert r0 == Np - Nfp
+    rng = np.random.default_rng(N)
+    phi = rng.standard_normal((4, Nfp))
+    ref = ops.LIFTB @ phi.ravel()
+    got = O.slice_lift_apply(ops, sl, phi)
+    assert np.abs(got - ref).max() < 1e-10 * np.abs(ref).max()
 
 
 def test_lsrk_coefficients():

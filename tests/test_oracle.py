@@ -63,6 +63,49 @@ def test_rk_run_cavity_converges_at_order_N_plus_1(N, meshes):
     assert r2 < {3: 2e-4, 4: 2e-5, 5: 2e-5}[N], r2
 
 
+def test_rk_run_cavity_high_orders_converge_in_N():
+    """N = 7, 8, 9 on one mesh (the N = 8, 9 WADG rule in collapsed coordinates): the error
+    against the exact standing wave drops by more than 4x per order (spectral convergence)."""
+    errs = []
+    for N in (7, 8, 9):
+        ops = O.build_ref_ops(N, tp_rule="collapsed")
+        m = M.cube_mesh(2)
+        M.face_pair(m)
+        dm = M.device_arrays(m, ops, 4)
+        Q = np.zeros((m.K, 4, ops.Np))
+        Q[:, 0] = _interp(ops, m, lambda x, y, z: cavity(x, y, z, 0.0))
+        T = 0.05
+        nsteps = int(math.ceil(T / (0.5 * 0.5 / N ** 2)))
+        orc.rk_run(orc.OracleOps(ops, np.float64), m.K, dm.vgeo, dm.fgeo, dm.vmapP,
+                   np.ones((m.K, ops.Nq)), Q, T / nsteps, nsteps, nproj=1)
+        errs.append(orc.l2_error(ops, *media.cubature_points(m, ops), dm.J, Q, cavity, T)[1])
+    assert errs[1] < errs[0] / 4 and errs[2] < errs[1] / 4, errs
+    assert errs[2] < 1e-6, errs
+
+
+def test_reference_tensor_rule_diverges_with_variable_c2():
+    """The reference's N = 8 WADG set (un-collapsed tensor product) with a Fourier c^2: the
+    FP64 oracle grows by orders of magnitude within two steps, while the collapsed rule keeps
+    the WADG energy non-increasing (a property of the scheme, not of this run)."""
+    out = {}
+    for rule in ("reference", "collapsed"):
+        ops = O.build_ref_ops(8, tp_rule=rule)
+        m = M.cube_mesh(2)
+        M.face_pair(m)
+        dm = M.device_arrays(m, ops, 4)
+        c2q = media.c2_fourier(3)(*media.cubature_points(m, ops))
+        Q = np.zeros((m.K, 4, ops.Np))
+        Q[:, 0] = _interp(ops, m, lambda x, y, z: cavity(x, y, z, 0.0))
+        E0 = orc.energy_wadg(ops, dm.J, c2q, Q) if rule == "collapsed" else None
+        orc.rk_run(orc.OracleOps(ops, np.float64), m.K, dm.vgeo, dm.fgeo, dm.vmapP, c2q, Q,
+                   0.5 * 0.5 / 64, 2)
+        out[rule] = np.abs(Q).max()
+        if rule == "collapsed":
+            assert orc.energy_wadg(ops, dm.J, c2q, Q) <= E0 * (1 + 1e-12)
+    assert out["reference"] > 1e3
+    assert out["collapsed"] < 1.1
+
+
 def _source_run(nsteps, T, src_time, N=2, n=2):
     ops, m, dm = _setup(N, n)
     c2q = media.c2_fourier(1)(*media.cubature_points(m, ops))
