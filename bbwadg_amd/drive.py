@@ -17,7 +17,9 @@ from .solver import WaveSolver, MODE_FUSED, MODE_PLAIN
 
 
 def setup(n, N, seed=1, **kw):
-    ops = O.build_ref_ops(N)
+    # N = 8, 9: the collapsed form of the tensor-product WADG set (the reference's own set
+    # diverges with a variable c^2; the kernels and their cost are the same)
+    ops = O.build_ref_ops(N, tp_rule="collapsed")
     m = M.cube_mesh(n)
     M.face_pair(m)
     dm = M.device_arrays(m, ops, 4)
@@ -51,6 +53,8 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--mode", choices=["fused", "plain"], default="fused")
     ap.add_argument("--impl", choices=["mfma", "vector"], default="mfma")
+    ap.add_argument("--plain-impl", choices=["tiled", "flat", "slice"], default=None,
+                    help="plain kernels (default: tiled for N <= 6, flat with the slice lift above)")
     ap.add_argument("--project-velocity", action="store_true")
     ap.add_argument("--elastic", action="store_true", help="elastic (9 fields), fused or plain stage")
     ap.add_argument("--test-rk", action="store_true",
@@ -90,8 +94,12 @@ def main():
         mode = MODE_FUSED if args.mode == "fused" else MODE_PLAIN
     else:
         s = setup(args.mesh, args.N)
-        s.set_option(_lib.OPT_FUSED_IMPL, _lib.FUSED_MFMA if args.impl == "mfma" else _lib.FUSED_VECTOR)
-        mode = MODE_FUSED if args.mode == "fused" else MODE_PLAIN
+        if args.N <= 6:
+            s.set_option(_lib.OPT_FUSED_IMPL, _lib.FUSED_MFMA if args.impl == "mfma" else _lib.FUSED_VECTOR)
+        mode = MODE_FUSED if args.mode == "fused" and args.N <= 6 else MODE_PLAIN
+    if args.plain_impl:
+        s.set_option(_lib.OPT_PLAIN_IMPL, {"tiled": _lib.PLAIN_TILED, "flat": _lib.PLAIN_FLAT,
+                                           "slice": _lib.PLAIN_SLICE}[args.plain_impl])
     # acoustic: CFL 0.5 h / N^2 at c ~ 1; elastic: the reference's estimate (faster P waves)
     dt = s.dt_ref if args.elastic else 0.5 / args.mesh / args.N ** 2
     s.step(2, dt, mode=mode, project_velocity=args.project_velocity)
@@ -102,7 +110,7 @@ def main():
     el = time.perf_counter() - t0
     dofs = s.K * s.ops.Np * s.nfields * 5 * args.steps
     print(f"K={s.K} N={args.N} fields={s.nfields} mode={'plain' if mode == MODE_PLAIN else 'fused'} "
-          f"impl={args.impl} steps={args.steps} "
+          f"impl={args.impl} plain_impl={args.plain_impl} steps={args.steps} "
           f"ms/stage={el / (5 * args.steps) * 1e3:.4f} DOF-stage/s={dofs / el:.4e}", flush=True)
 
 
