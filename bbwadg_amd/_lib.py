@@ -19,6 +19,7 @@ BBW_ERR_UNSUPPORTED = -4
 BBW_ERR_NOMEM = -5
 MODE_PLAIN = 0
 MODE_FUSED = 1
+MODE_CURVED = 2
 
 # every symbol include/bbwadg.h declares
 EXPORTS = [
@@ -32,7 +33,8 @@ EXPORTS = [
     "bbw_adjoint_gradient", "bbw_batch_create", "bbw_batch_destroy", "bbw_batch_last_error",
     "bbw_batch_upload_operators", "bbw_batch_add_mesh", "bbw_batch_add_shot", "bbw_batch_run",
     "bbw_batch_get_state", "bbw_batch_reset", "bbw_batch_last_ms", "bbw_batch_set_step_cap",
-    "bbw_upload_bbwadg", "bbw_update_bbwadg", "bbw_upload_slice_lift",
+    "bbw_upload_bbwadg", "bbw_update_bbwadg", "bbw_upload_slice_lift", "bbw_upload_curved",
+    "bbw_curved_rhs",
 ]
 OPT_FUSED_IMPL, OPT_KBLK_V, OPT_KBLK_S, OPT_KBLK_U, OPT_KBLK_F, OPT_ADJ_CACHE_MB, OPT_ADJ_IMPL = 0, 1, 2, 3, 4, 5, 6
 OPT_PLAIN_IMPL = 7
@@ -64,6 +66,12 @@ class Ops(ctypes.Structure):
                 ("Vq", ctypes.c_void_p), ("Pq", ctypes.c_void_p), ("wq", ctypes.c_void_p),
                 ("Vq64", ctypes.c_void_p),
                 ("rk4a", ctypes.c_void_p), ("rk4b", ctypes.c_void_p), ("rk4c", ctypes.c_void_p)]
+
+
+class CurvedOps(ctypes.Structure):
+    _fields_ = [("Nfq", ctypes.c_int)] + [(n, ctypes.c_void_p) for n in (
+        "Vq", "Vrq", "Vsq", "Vtq", "Pq", "Prq", "Psq", "Ptq", "VfqFace", "Pfq", "Fmask", "wq",
+        "rk4a", "rk4b", "rk4c")]
 
 
 _lib = None
@@ -115,6 +123,8 @@ def load():
         "bbw_upload_adjoint_maps": ([P, P], I),
         "bbw_upload_bbwadg": ([P, P, P, P, P, P, P, P], I),
         "bbw_upload_slice_lift": ([P, P, P, P, P], I),
+        "bbw_upload_curved": ([P, ctypes.POINTER(CurvedOps), P, P, P, P, I, P, I, P], I),
+        "bbw_curved_rhs": ([P], I),
         "bbw_update_bbwadg": ([P], I),
         "bbw_misfit": ([P, I, D, D, P, ctypes.POINTER(D)], I),
         "bbw_misfit_device": ([P, I, D, D, P, ctypes.POINTER(D)], I),

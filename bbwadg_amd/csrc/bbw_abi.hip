@@ -94,6 +94,13 @@ int check_ready(bbw_ctx *c) {
   return BBW_OK;
 }
 
+// paths that need the Bernstein operators (not set up in a curvilinear context)
+int check_not_curved(bbw_ctx *c, const char *what) {
+  if (c->curved)
+    return fail(c, BBW_ERR_UNSUPPORTED, std::string(what) + " is not available in a curvilinear context");
+  return BBW_OK;
+}
+
 // paths whose kernels are instantiated for N <= BBW_MAX_ORDER_TILED only
 int check_tiled_order(bbw_ctx *c, const char *what) {
   if (c->N > BBW_MAX_ORDER_TILED)
@@ -113,6 +120,12 @@ int do_stage(bbw_ctx *c, int i, double dt, const float *ftd, float ft, int mode,
              bool more_steps) {
   const float fa = (float)c->rk4a[i], fb = (float)c->rk4b[i], fdt = (float)dt;
   hipError_t e;
+  if (mode == BBW_MODE_CURVED) {
+    if ((e = launch_curved_rhs(c)) != hipSuccess) return hipfail(c, e, "curved rhs launch");
+    if ((e = launch_update_wadg(c, fa, fb, fdt, ftd, ft)) != hipSuccess)
+      return hipfail(c, e, "update launch");
+    return BBW_OK;
+  }
   if (c->Nf == 4) {
     if (mode == BBW_MODE_FUSED) {
       const int store_res = !(i == 4 && more_steps && c->rk4a[0] == 0.0);
@@ -215,6 +228,9 @@ void bbw_destroy(bbw_ctx *c) {
   dfree(c->EEL_vals); dfree(c->L0_vals); dfree(c->Vq); dfree(c->Pq); dfree(c->Vq64); dfree(c->Pq64); dfree(c->VqPad); dfree(c->PqPad); dfree(c->EEL_ids4); dfree(c->LiftPad); dfree(c->VqBlk); dfree(c->PqBlk); dfree(c->LiftBlk); dfree(c->PqTBlk); dfree(c->VqTBlk); dfree(c->LiftTBlk); dfree(c->PqTPad); dfree(c->VqTPad);
   dfree(c->bb_CN); dfree(c->bb_PBlk); dfree(c->c2bb);
   dfree(c->slice_inv); dfree(c->red_ids); dfree(c->red_vals); dfree(c->cEL);
+  dfree(c->cv_Vq); dfree(c->cv_Vr); dfree(c->cv_Vs); dfree(c->cv_Vt); dfree(c->cv_Pq); dfree(c->cv_Pr);
+  dfree(c->cv_Ps); dfree(c->cv_Pt); dfree(c->cv_Vf); dfree(c->cv_Pf); dfree(c->vgeoq); dfree(c->cv_Jq);
+  dfree(c->fgeoq); dfree(c->Jplanar); dfree(c->Klist_curved); dfree(c->Klist_planar);
   dfree(c->DTids); dfree(c->DTvals); dfree(c->Sids); dfree(c->Eids); dfree(c->Evals); dfree(c->Swts); dfree(c->EELT_ids); dfree(c->EELT_vals); dfree(c->L0T_ids);
   dfree(c->L0T_vals); dfree(c->FmaskInv); dfree(c->mapP); dfree(c->lQ); dfree(c->lres);
   dfree(c->Zadj); dfree(c->Hadj); dfree(c->grad); dfree(c->Ybuf); dfree(c->ckpt);
@@ -579,6 +595,86 @@ int bbw_upload_slice_lift(bbw_ctx *c, const int32_t *slice_ids, const int32_t *r
   return BBW_OK;
 }
 
+int bbw_upload_curved(bbw_ctx *c, const bbw_curved_ops *o, const float *vgeoq, const float *Jq,
+                      const float *fgeoq, const float *Jplanar, int Kcurved,
+                      const int32_t *KlistCurved, int Kplanar, const int32_t *KlistPlanar) {
+  if (!c || !o || !vgeoq || !Jq || !fgeoq || !Jplanar || Kcurved < 0 || Kplanar < 0 ||
+      (Kcurved && !KlistCurved) || (Kplanar && !KlistPlanar))
+    return BBW_ERR_ARG;
+  if (!o->Vq || !o->Vrq || !o->Vsq || !o->Vtq || !o->Pq || !o->Prq || !o->Psq || !o->Ptq ||
+      !o->VfqFace || !o->Pfq || !o->Fmask || !o->wq || !o->rk4a || !o->rk4b || !o->rk4c)
+    return fail(c, BBW_ERR_ARG, "null curvilinear operator array");
+  if (c->Nf != 4) return fail(c, BBW_ERR_UNSUPPORTED, "the curvilinear path is acoustic");
+  if (int rc = check_tiled_order(c, "the curvilinear path")) return rc;
+  if (c->have_ops && !c->curved)
+    return fail(c, BBW_ERR_STATE, "Bernstein operators already uploaded: a curvilinear context is a new context");
+  const int N = c->N, Np = c->Np, Nfp = c->Nfp, Nq = c->Nq, K = c->K;
+  if (o->Nfq != curved_nfq(N)) return fail(c, BBW_ERR_ARG, "Nfq must be the degree 2N+1 triangle rule's size");
+  const int Nfq = o->Nfq;
+  if ((double)K * 4 * Np * 4 >= 2147483648.0 || (double)K * 9 * Nq * 4 >= 4294967296.0 ||
+      (double)K * 16 * Nfq * 4 >= 4294967296.0)
+    return fail(c, BBW_ERR_ARG, "mesh too large for 32-bit offsets of the curvilinear kernels");
+  if (Kcurved + Kplanar != K) return fail(c, BBW_ERR_ARG, "the element lists must partition the mesh");
+  std::vector<char> seen(K, 0);
+  for (int i = 0; i < Kcurved + Kplanar; ++i) {
+    const int e = i < Kcurved ? KlistCurved[i] : KlistPlanar[i - Kcurved];
+    if (e < 0 || e >= K || seen[e]) return fail(c, BBW_ERR_ARG, "the element lists must partition the mesh");
+    seen[e] = 1;
+  }
+  for (int i = 0; i < 4 * Nfp; ++i)
+    if (o->Fmask[i] < 0 || o->Fmask[i] >= Np) return fail(c, BBW_ERR_ARG, "Fmask out of range");
+  const int NQP = 16 * ((Nq + 15) / 16), NFQP = 16 * ((Nfq + 15) / 16);
+  std::vector<float> b[10];
+  const float *V4[4] = {o->Vq, o->Vrq, o->Vsq, o->Vtq};
+  const float *P4[4] = {o->Pq, o->Prq, o->Psq, o->Ptq};
+  for (int m = 0; m < 4; ++m) {
+    const float *V = V4[m], *P = P4[m];
+    lane_blocks(Nq, Np, [&](int q, int j) { return V[q + (size_t)j * Nq]; }, b[m]);
+    lane_blocks(Np, NQP, [&](int n, int q) { return q < Nq ? P[n + (size_t)q * Np] : 0.f; }, b[4 + m]);
+  }
+  lane_blocks(Nfq, Nfp, [&](int q, int j) { return o->VfqFace[q + (size_t)j * Nfq]; }, b[8]);
+  lane_blocks(Np, 4 * NFQP, [&](int n, int k) {
+    const int f = k / NFQP, j = k - f * NFQP;
+    return j < Nfq ? o->Pfq[n + (size_t)(f * Nfq + j) * Np] : 0.f;
+  }, b[9]);
+  float **dst[10] = {&c->cv_Vq, &c->cv_Vr, &c->cv_Vs, &c->cv_Vt, &c->cv_Pq, &c->cv_Pr,
+                     &c->cv_Ps, &c->cv_Pt, &c->cv_Vf, &c->cv_Pf};
+  int rc;
+  for (int m = 0; m < 10; ++m) {
+    dfree(*dst[m]);
+    if ((rc = upload(c, dst[m], b[m].data(), b[m].size()))) return rc;
+  }
+  // the WADG update's operators: the same Vq and Pq (flat copies and lane-ordered blocks)
+  std::vector<float> vb, pb;
+  lane_blocks(Nq, Np, [&](int q, int j) { return o->Vq[q + (size_t)j * Nq]; }, vb);
+  lane_blocks(Np, Nq, [&](int n, int q) { return o->Pq[n + (size_t)q * Np]; }, pb);
+  dfree(c->Vq); dfree(c->Pq); dfree(c->VqBlk); dfree(c->PqBlk); dfree(c->Fmask);
+  if ((rc = upload(c, &c->Vq, o->Vq, (size_t)Nq * Np))) return rc;
+  if ((rc = upload(c, &c->Pq, o->Pq, (size_t)Np * Nq))) return rc;
+  if ((rc = upload(c, &c->VqBlk, vb.data(), vb.size()))) return rc;
+  if ((rc = upload(c, &c->PqBlk, pb.data(), pb.size()))) return rc;
+  if ((rc = upload(c, &c->Fmask, o->Fmask, (size_t)4 * Nfp))) return rc;
+  dfree(c->vgeoq); dfree(c->cv_Jq); dfree(c->fgeoq); dfree(c->Jplanar);
+  dfree(c->Klist_curved); dfree(c->Klist_planar);
+  if ((rc = upload(c, &c->vgeoq, vgeoq, (size_t)K * 9 * Nq))) return rc;
+  if ((rc = upload(c, &c->cv_Jq, Jq, (size_t)K * Nq))) return rc;
+  if ((rc = upload(c, &c->fgeoq, fgeoq, (size_t)K * 16 * Nfq))) return rc;
+  if ((rc = upload(c, &c->Jplanar, Jplanar, (size_t)K))) return rc;
+  if (Kcurved && (rc = upload(c, &c->Klist_curved, KlistCurved, (size_t)Kcurved))) return rc;
+  if (Kplanar && (rc = upload(c, &c->Klist_planar, KlistPlanar, (size_t)Kplanar))) return rc;
+  std::memcpy(c->rk4a, o->rk4a, 5 * sizeof(double));
+  std::memcpy(c->rk4b, o->rk4b, 5 * sizeof(double));
+  std::memcpy(c->rk4c, o->rk4c, 6 * sizeof(double));
+  c->Nfq = Nfq;
+  c->Kcurved = Kcurved;
+  c->Kplanar = Kplanar;
+  c->plain_impl = BBW_PLAIN_TILED;   // the WADG update on MFMA (with 1/J from bbw_upload_media)
+  c->curved = true;
+  c->have_ops = true;
+  reset_graph(c);
+  return BBW_OK;
+}
+
 int bbw_upload_mesh(bbw_ctx *c, const float *vgeo, const float *fgeo, const int32_t *vmapP,
                     const double *J) {
   if (!c || !vgeo || !fgeo || !vmapP || !J) return BBW_ERR_ARG;
@@ -728,9 +824,19 @@ int bbw_get_res(bbw_ctx *c, float *r) { return c ? copy_state(c, c->res, nullptr
 int bbw_set_rhs(bbw_ctx *c, const float *r) { return c ? copy_state(c, c->rhs, r, nullptr) : BBW_ERR_ARG; }
 int bbw_get_rhs(bbw_ctx *c, float *r) { return c ? copy_state(c, c->rhs, nullptr, r) : BBW_ERR_ARG; }
 
+int bbw_curved_rhs(bbw_ctx *c) {
+  if (!c) return BBW_ERR_ARG;
+  int rc = check_ready(c);
+  if (rc) return rc;
+  if (!c->curved) return fail(c, BBW_ERR_STATE, "curvilinear operators not uploaded");
+  hipError_t e = launch_curved_rhs(c);
+  return e == hipSuccess ? BBW_OK : hipfail(c, e, "curved rhs launch");
+}
+
 int bbw_volume(bbw_ctx *c) {
   if (!c) return BBW_ERR_ARG;
   if (!c->have_ops || !c->have_mesh) return fail(c, BBW_ERR_STATE, "operators/mesh not uploaded");
+  if (int rc = check_not_curved(c, "bbw_volume")) return rc;
   hipError_t e = c->Nf == 4 ? launch_volume(c) : launch_volume_elas(c);
   return e == hipSuccess ? BBW_OK : hipfail(c, e, "volume launch");
 }
@@ -738,6 +844,7 @@ int bbw_volume(bbw_ctx *c) {
 int bbw_surface(bbw_ctx *c) {
   if (!c) return BBW_ERR_ARG;
   if (!c->have_ops || !c->have_mesh) return fail(c, BBW_ERR_STATE, "operators/mesh not uploaded");
+  if (int rc = check_not_curved(c, "bbw_surface")) return rc;
   hipError_t e = c->Nf == 4 ? launch_surface(c) : launch_surface_elas(c);
   return e == hipSuccess ? BBW_OK : hipfail(c, e, "surface launch");
 }
@@ -758,6 +865,7 @@ int bbw_upload_bbwadg(bbw_ctx *c, const int32_t *subtet_ids, const double *CNsca
     return BBW_ERR_ARG;
   if (c->Nf != 4) return fail(c, BBW_ERR_UNSUPPORTED, "the exact BB-WADG update is acoustic");
   if (int rc = check_tiled_order(c, "the exact BB-WADG update")) return rc;
+  if (int rc = check_not_curved(c, "the exact BB-WADG update")) return rc;
   const int N = c->N, Np = c->Np, N2 = 2 * N, N2p = np_of(N2);
   auto npd = [](int d) { return (d + 1) * (d + 2) * (d + 3) / 6; };
   // the kernel forms the degree-2N index of a + b arithmetically, Np(2N) - Np(2N - l) +
@@ -842,7 +950,8 @@ int bbw_update(bbw_ctx *c, float fa, float fb, float fdt) {
 int bbw_stage_fused(bbw_ctx *c, float fa, float fb, float fdt, float ftime, int projU) {
   if (!c) return BBW_ERR_ARG;
   int rc = check_ready(c);
-  if (rc || (rc = check_tiled_order(c, "the fused stage"))) return rc;
+  if (rc || (rc = check_tiled_order(c, "the fused stage")) || (rc = check_not_curved(c, "the fused stage")))
+    return rc;
   if (c->Nf == 9) {
     hipError_t e = launch_fused_elas(c, c->Q[c->cur], c->Q[1 - c->cur], fa, fb, fdt, nullptr, ftime);
     if (e != hipSuccess) return hipfail(c, e, "fused elastic stage launch");
@@ -858,10 +967,14 @@ int bbw_stage_fused(bbw_ctx *c, float fa, float fb, float fdt, float ftime, int 
 }
 
 int bbw_step(bbw_ctx *c, int nsteps, double dt, double t0, int mode, int flags) {
-  if (!c || nsteps < 0 || !(dt > 0) || (mode != BBW_MODE_PLAIN && mode != BBW_MODE_FUSED))
+  if (!c || nsteps < 0 || !(dt > 0) ||
+      (mode != BBW_MODE_PLAIN && mode != BBW_MODE_FUSED && mode != BBW_MODE_CURVED))
     return BBW_ERR_ARG;
   int rc = check_ready(c);
   if (rc) return rc;
+  if ((mode == BBW_MODE_CURVED) != c->curved)
+    return fail(c, BBW_ERR_ARG, c->curved ? "a curvilinear context steps with BBW_MODE_CURVED"
+                                          : "BBW_MODE_CURVED needs bbw_upload_curved");
   if (mode == BBW_MODE_FUSED && (rc = check_tiled_order(c, "the fused stage"))) return rc;
   if (c->nrec && c->rec_steps + nsteps > c->rec_max_steps)
     return fail(c, BBW_ERR_ARG, "receiver trace buffer too short");
@@ -990,7 +1103,9 @@ int bbw_step(bbw_ctx *c, int nsteps, double dt, double t0, int mode, int flags) 
 static int energy_common(bbw_ctx *c, int kind, double *E) {
   if (!c || !E) return BBW_ERR_ARG;
   int rc = check_ready(c);
-  if (rc || (rc = check_tiled_order(c, "the energy reduction"))) return rc;
+  if (rc || (rc = check_tiled_order(c, "the energy reduction")) ||
+      (rc = check_not_curved(c, "the energy reduction")))
+    return rc;
   if ((rc = dalloc(c, &c->partial, (size_t)BBW_ENERGY_BLOCKS + 1))) return rc;
   hipError_t e = launch_energy(c, kind, c->partial, c->partial + BBW_ENERGY_BLOCKS);
   if (e != hipSuccess) return hipfail(c, e, "energy launch");
@@ -1006,7 +1121,9 @@ int bbw_energy_wadg(bbw_ctx *c, double *E) { return energy_common(c, BBW_ENERGY_
 int bbw_energy_noalloc(bbw_ctx *c, double *E) {
   if (!c || !E) return BBW_ERR_ARG;
   int rc = check_ready(c);
-  if (rc || (rc = check_tiled_order(c, "the energy reduction"))) return rc;
+  if (rc || (rc = check_tiled_order(c, "the energy reduction")) ||
+      (rc = check_not_curved(c, "the energy reduction")))
+    return rc;
   double *scr = energy_scratch(c->dev);
   if (!scr) return fail(c, BBW_ERR_STATE, "energy scratch array not set up on this device");
   // the reduction writes one partial sum per block of a grid capped at BBW_ENERGY_BLOCKS (so any
@@ -1073,7 +1190,9 @@ int device_misfit(bbw_ctx *c, int nsteps, const float *d_dev, float *resid, doub
 
 int check_inverse(bbw_ctx *c, int nsteps, const float *d_obs) {
   int rc = check_ready(c);
-  if (rc || (rc = check_tiled_order(c, "the misfit / adjoint path"))) return rc;
+  if (rc || (rc = check_tiled_order(c, "the misfit / adjoint path")) ||
+      (rc = check_not_curved(c, "the misfit / adjoint path")))
+    return rc;
   if (c->Nf != 4) return fail(c, BBW_ERR_UNSUPPORTED, "the adjoint is acoustic only");
   if (!d_obs || nsteps <= 0) return fail(c, BBW_ERR_ARG, "bad observations");
   if (c->nrec == 0 || c->rec_max_steps < nsteps) return fail(c, BBW_ERR_STATE, "receivers not set for nsteps");
@@ -1261,6 +1380,7 @@ int bbw_set_option(bbw_ctx *c, int option, int value) {
     case BBW_OPT_ADJ_CACHE_MB: c->ycache_mb = value; break;
     case BBW_OPT_ADJ_IMPL: c->adj_impl = value ? 1 : 0; break;
     case BBW_OPT_PLAIN_IMPL:
+      if (c->curved) return fail(c, BBW_ERR_UNSUPPORTED, "a curvilinear context keeps its kernels");
       if (value != BBW_PLAIN_FLAT && value != BBW_PLAIN_TILED && value != BBW_PLAIN_SLICE)
         return fail(c, BBW_ERR_ARG, "bad plain impl");
       if (value == BBW_PLAIN_TILED && c->N > BBW_MAX_ORDER_TILED)

@@ -102,6 +102,15 @@ struct bbw_ctx {
   int *red_ids = nullptr;
   float *red_vals = nullptr, *cEL = nullptr;
   bool have_slice = false;
+  // curvilinear path (bbw_upload_curved): nodal lane-ordered operator blocks, per-element
+  // cubature geometry and the element lists
+  bool curved = false;
+  int Nfq = 0, Kcurved = 0, Kplanar = 0;
+  float *cv_Vq = nullptr, *cv_Vr = nullptr, *cv_Vs = nullptr, *cv_Vt = nullptr;
+  float *cv_Pq = nullptr, *cv_Pr = nullptr, *cv_Ps = nullptr, *cv_Pt = nullptr;
+  float *cv_Vf = nullptr, *cv_Pf = nullptr;
+  float *vgeoq = nullptr, *cv_Jq = nullptr, *fgeoq = nullptr, *Jplanar = nullptr;
+  int *Klist_curved = nullptr, *Klist_planar = nullptr;
 
   // mesh
   float *vgeo = nullptr, *fgeo = nullptr;
@@ -192,6 +201,8 @@ hipError_t launch_update_elas_mfma(bbw_ctx *c, float fa, float fb, float fdt, co
                                    float ftime);
 hipError_t launch_surface_tiled(bbw_ctx *c);
 hipError_t launch_surface_slice(bbw_ctx *c);
+hipError_t launch_curved_rhs(bbw_ctx *c);
+int curved_nfq(int N);   // face cubature points of the curvilinear path (-1: order not compiled)
 hipError_t launch_fused(bbw_ctx *c, const float *Qin, float *Qout, float fa, float fb, float fdt,
                         const float *ftime_dev, float ftime, int project_velocity);
 hipError_t launch_fused_mfma(bbw_ctx *c, const float *Qin, float *Qout, float fa, float fb,

@@ -301,3 +301,96 @@ class WaveSolver:
             self.close()
         except Exception:
             pass
+
+
+class CurvedWaveSolver(WaveSolver):
+    """The curvilinear WADG run of the reference's ``main`` (cpp/main.cpp:40-130): ball mesh,
+    Gordon-Hall sphere map (``sphere``; else every element stays planar), nodal state,
+    ``RK_step_WADG`` stages (BBW_MODE_CURVED).
+
+        s = CurvedWaveSolver(mesh, N)            # StartUp3d + WaveInitOCCA3d + GordonHallSphere
+                                                 # + InitQuadratureArrays + InitWADG_curved
+        s.WaveSetData3d(s.WaveSetU0(curved.bessel))
+        s.step(nsteps, dt)                       # Wave_RK(mesh, FinalTime, dt, 2)
+        s.compute_error(T, curved.bessel)
+
+    ``WaveSetU0`` interpolates at the curved nodes and returns nodal values (the reference builds
+    this path with the nodal basis); c^2 defaults to 1, as in ``InitWADG_curved``."""
+
+    def __init__(self, mesh, N, sphere=True, device=0, ops=None):
+        from . import curved
+        self.lib = _lib.load()
+        self.mesh = mesh
+        if mesh.EToE is None:
+            meshmod.face_pair(mesh)
+        self.ops = ops if ops is not None else operators.build_ref_ops(N)
+        self.nops = curved.build_nodal_ops(self.ops)
+        self.N, self.nfields, self.K = N, 4, mesh.K
+        self.cm = curved.setup_curved(mesh, self.ops, self.nops, sphere=sphere)
+        self.dmesh = meshmod.device_arrays(mesh, self.ops, 4)
+        cfg = _lib.Config(N=N, Nfields=4, K=self.K, Nq=self.nops.Nq, device=device)
+        h = ctypes.c_void_p()
+        rc = self.lib.bbw_create(ctypes.byref(cfg), ctypes.byref(h))
+        if rc != 0:
+            raise _lib.BBWError(rc, "bbw_create failed")
+        self.ctx = h
+        dm, cm, no = self.dmesh, self.cm, self.nops
+        self._keep = [f32(dm.vgeo), f32(dm.fgeo), i32(dm.vmapP), f64(dm.J)]
+        check(self.ctx, self.lib.bbw_upload_mesh(self.ctx, *[ptr(a) for a in self._keep]))
+
+        def cm32(a):   # column-major FP32
+            return np.ascontiguousarray(np.asarray(a).T, dtype=np.float32)
+
+        k = dict(Vq=cm32(no.Vq), Vrq=cm32(no.Vrq), Vsq=cm32(no.Vsq), Vtq=cm32(no.Vtq),
+                 Pq=cm32(no.Pq), Prq=cm32(no.Prq), Psq=cm32(no.Psq), Ptq=cm32(no.Ptq),
+                 VfqFace=cm32(no.VfqFace), Pfq=cm32(no.Pfq), Fmask=i32(no.Fmask.reshape(-1)),
+                 wq=f64(no.wq), rk4a=f64(self.ops.rk4a), rk4b=f64(self.ops.rk4b), rk4c=f64(self.ops.rk4c))
+        geo = dict(vgeoq=f32(cm.vgeoq), Jq=f32(cm.Jq), fgeoq=f32(cm.fgeoq), Jplanar=f32(cm.Jplanar),
+                   Kc=i32(cm.Klist_curved), Kp=i32(cm.Klist_planar))
+        self._curved_keep = (k, geo)
+        s = _lib.CurvedOps(Nfq=no.Nfq, **{n: ptr(v).value for n, v in k.items()})
+        check(self.ctx, self.lib.bbw_upload_curved(
+            self.ctx, ctypes.byref(s), ptr(geo["vgeoq"]), ptr(geo["Jq"]), ptr(geo["fgeoq"]),
+            ptr(geo["Jplanar"]), len(geo["Kc"]), ptr(geo["Kc"]) if len(geo["Kc"]) else None,
+            len(geo["Kp"]), ptr(geo["Kp"]) if len(geo["Kp"]) else None))
+        # reference time step (cpp/src/WaveOKL3d.cpp:1677) from the affine face scales
+        self.dt_ref = 0.25 / ((N + 1) ** 2 * dm.FscaleMax)
+        self.time = 0.0
+        self.set_c2(np.ones((self.K, no.Nq)))
+
+    def set_c2(self, c2q):
+        """c^2 at the volume cubature points (K, Nq) and 1/J there for the WADG update
+        (``InitWADG_curved``'s c2q and Jq_reduced, cpp/src/WaveOKL3d.cpp:385-408)."""
+        self.c2q, self._Jq = f32(c2q), f32(self.cm.Jq)
+        check(self.ctx, self.lib.bbw_upload_media(self.ctx, ptr(self.c2q), ptr(self._Jq)))
+
+    def cubature_points(self):
+        return self.cm.xq, self.cm.yq, self.cm.zq
+
+    def WaveSetU0(self, uex, field=0, time=0.0):
+        """uex at the curved nodes, nodal values (``WaveSetU0`` with the nodal basis)."""
+        Q = np.zeros((self.K, 4, self.ops.Np))
+        Q[:, field] = uex(self.cm.x, self.cm.y, self.cm.z, time)
+        return f32(Q)
+
+    def rhs(self):
+        """rhsQ = the split-form right-hand side times J (``RK_step_WADG``'s kernels)."""
+        check(self.ctx, self.lib.bbw_curved_rhs(self.ctx))
+
+    def _mode(self, mode):
+        if mode not in (None, _lib.MODE_CURVED):
+            raise ValueError("a curvilinear solver steps with MODE_CURVED")
+        return _lib.MODE_CURVED
+
+    def Wave_RK(self, FinalTime, dt, mode=None, **kw):
+        super().Wave_RK(FinalTime, dt, mode=_lib.MODE_CURVED, **kw)
+
+    def compute_error(self, time, uex, Q=None, field=0):
+        """(L2err, relL2err) at the volume cubature with J there (``compute_error``)."""
+        from . import curved
+        Q = self.WaveGetData3d() if Q is None else Q
+        return curved.l2_error(self.nops, self.cm, np.asarray(Q, dtype=np.float64)[:, field:field + 1],
+                               uex, time)
+
+    def to_nodal(self, Q):
+        return np.asarray(Q)

@@ -45,6 +45,7 @@ extern "C" {
 
 #define BBW_MODE_PLAIN 0  /* volume, surface, update launches per stage (RK_step_WADG_subelem) */
 #define BBW_MODE_FUSED 1  /* one fused stage kernel per stage, ping-pong state */
+#define BBW_MODE_CURVED 2 /* curvilinear context (bbw_upload_curved): RK_step_WADG's stage */
 
 typedef struct bbw_ctx bbw_ctx;
 
@@ -148,6 +149,39 @@ int bbw_update(bbw_ctx *ctx, float fa, float fb, float fdt);
  * the other state buffer, which becomes current.  project_velocity: 1 projects the
  * velocity fields through Pq*Vq as rk_update_WADG does, 0 uses Pq*Vq = I. */
 int bbw_stage_fused(bbw_ctx *ctx, float fa, float fb, float fdt, float ftime, int project_velocity);
+
+/* Curvilinear WADG (the reference's main run: InitQuadratureArrays, InitWADG_curved,
+ * GordonHallSphere, RK_step_WADG; cpp/src/StartUp3d.cpp:829-960, cpp/src/WaveOKL3d.cpp:164-417,
+ * :929-1297, :2111-2185).  The context holds NODAL coefficients (the reference builds this path
+ * with the nodal basis).  Operators, single precision, the reference's column-major layouts:
+ *   Vq, Vrq, Vsq, Vtq  Nq x Np   nodal interpolation / r, s, t derivatives at the volume rule
+ *                                (the degree 2N+1 rule of the WADG update, Nq = bbw_config.Nq)
+ *   Pq, Prq, Psq, Ptq  Np x Nq   M^-1 V*^T W
+ *   VfqFace            Nfq x Nfp face nodes (Fmask order) -> the face rule
+ *   Pfq                Np x 4Nfq M^-1 Vfq^T Wf (faces in order)
+ * Vq and Pq also serve as the WADG update's Vq_reduced / Pq_reduced (the same rule; its 1/J is
+ * bbw_upload_media's Jq).  Per element: vgeoq [K][9][Nq] (rx sx tx ry sy ty rz sz tz at the volume
+ * points), Jq [K][Nq], fgeoq [K][4][4][Nfq] (per face nx ny nz sJ at the face points), Jplanar [K],
+ * and the element lists KlistCurved / KlistPlanar, which must partition 0..K-1.  Planar elements
+ * use bbw_upload_mesh's vgeo / fgeo.  Orders 1..6, acoustic. */
+typedef struct {
+  int Nfq;
+  const float *Vq, *Vrq, *Vsq, *Vtq;
+  const float *Pq, *Prq, *Psq, *Ptq;
+  const float *VfqFace;
+  const float *Pfq;
+  const int32_t *Fmask;
+  const double *wq;
+  const double *rk4a, *rk4b, *rk4c;
+} bbw_curved_ops;
+int bbw_upload_curved(bbw_ctx *ctx, const bbw_curved_ops *ops, const float *vgeoq, const float *Jq,
+                      const float *fgeoq, const float *Jplanar, int Kcurved,
+                      const int32_t *KlistCurved, int Kplanar, const int32_t *KlistPlanar);
+/* ≈ rk_volume_WADG_skew_combine + rk_surface_WADG_skew_combine on KlistCurved and
+ * rk_volume_planar + rk_surface_planar on KlistPlanar (cpp/okl/WaveKernelsWADG.okl:514-682,
+ * :3024-3195, :2203-2441): rhsQ = the split-form right-hand side of the current state times J.
+ * bbw_update_wadg then applies rk_update_WADG with 1/J. */
+int bbw_curved_rhs(bbw_ctx *ctx);
 
 /* Exact Bernstein-Bezier WADG pressure update, ≈ rk_update_BBWADG
  * (cpp/okl/WaveKernelsBBWADG.okl:14-321, driven by test_RK, cpp/src/WaveOKL3d.cpp:2288-2296):
