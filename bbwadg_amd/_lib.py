@@ -39,6 +39,7 @@ EXPORTS = [
 OPT_FUSED_IMPL, OPT_KBLK_V, OPT_KBLK_S, OPT_KBLK_U, OPT_KBLK_F, OPT_ADJ_CACHE_MB, OPT_ADJ_IMPL = 0, 1, 2, 3, 4, 5, 6
 OPT_PLAIN_IMPL = 7
 OPT_SRC_STEP_TIME = 8
+OPT_PLANAR_SPLIT = 9
 FUSED_VECTOR, FUSED_MFMA = 0, 1
 PLAIN_FLAT, PLAIN_TILED, PLAIN_SLICE = 0, 1, 2
 
@@ -70,7 +71,8 @@ class Ops(ctypes.Structure):
 
 class CurvedOps(ctypes.Structure):
     _fields_ = [("Nfq", ctypes.c_int)] + [(n, ctypes.c_void_p) for n in (
-        "Vq", "Vrq", "Vsq", "Vtq", "Pq", "Prq", "Psq", "Ptq", "VfqFace", "Pfq", "Fmask", "wq",
+        "Vq", "Vrq", "Vsq", "Vtq", "Pq", "Prq", "Psq", "Ptq", "VfqFace", "Pfq", "Dr", "Ds", "Dt",
+        "LIFT", "Fmask", "wq",
         "rk4a", "rk4b", "rk4c")]
 
 

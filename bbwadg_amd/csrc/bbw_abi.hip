@@ -231,6 +231,7 @@ void bbw_destroy(bbw_ctx *c) {
   dfree(c->cv_Vq); dfree(c->cv_Vr); dfree(c->cv_Vs); dfree(c->cv_Vt); dfree(c->cv_Pq); dfree(c->cv_Pr);
   dfree(c->cv_Ps); dfree(c->cv_Pt); dfree(c->cv_Vf); dfree(c->cv_Pf); dfree(c->vgeoq); dfree(c->cv_Jq);
   dfree(c->fgeoq); dfree(c->Jplanar); dfree(c->Klist_curved); dfree(c->Klist_planar);
+  dfree(c->cv_Dr); dfree(c->cv_Ds); dfree(c->cv_Dt); dfree(c->cv_Lift);
   dfree(c->DTids); dfree(c->DTvals); dfree(c->Sids); dfree(c->Eids); dfree(c->Evals); dfree(c->Swts); dfree(c->EELT_ids); dfree(c->EELT_vals); dfree(c->L0T_ids);
   dfree(c->L0T_vals); dfree(c->FmaskInv); dfree(c->mapP); dfree(c->lQ); dfree(c->lres);
   dfree(c->Zadj); dfree(c->Hadj); dfree(c->grad); dfree(c->Ybuf); dfree(c->ckpt);
@@ -602,7 +603,8 @@ int bbw_upload_curved(bbw_ctx *c, const bbw_curved_ops *o, const float *vgeoq, c
       (Kcurved && !KlistCurved) || (Kplanar && !KlistPlanar))
     return BBW_ERR_ARG;
   if (!o->Vq || !o->Vrq || !o->Vsq || !o->Vtq || !o->Pq || !o->Prq || !o->Psq || !o->Ptq ||
-      !o->VfqFace || !o->Pfq || !o->Fmask || !o->wq || !o->rk4a || !o->rk4b || !o->rk4c)
+      !o->VfqFace || !o->Pfq || !o->Dr || !o->Ds || !o->Dt || !o->LIFT || !o->Fmask || !o->wq ||
+      !o->rk4a || !o->rk4b || !o->rk4c)
     return fail(c, BBW_ERR_ARG, "null curvilinear operator array");
   if (c->Nf != 4) return fail(c, BBW_ERR_UNSUPPORTED, "the curvilinear path is acoustic");
   if (int rc = check_tiled_order(c, "the curvilinear path")) return rc;
@@ -624,7 +626,7 @@ int bbw_upload_curved(bbw_ctx *c, const bbw_curved_ops *o, const float *vgeoq, c
   for (int i = 0; i < 4 * Nfp; ++i)
     if (o->Fmask[i] < 0 || o->Fmask[i] >= Np) return fail(c, BBW_ERR_ARG, "Fmask out of range");
   const int NQP = 16 * ((Nq + 15) / 16), NFQP = 16 * ((Nfq + 15) / 16);
-  std::vector<float> b[10];
+  std::vector<float> b[14];
   const float *V4[4] = {o->Vq, o->Vrq, o->Vsq, o->Vtq};
   const float *P4[4] = {o->Pq, o->Prq, o->Psq, o->Ptq};
   for (int m = 0; m < 4; ++m) {
@@ -637,10 +639,17 @@ int bbw_upload_curved(bbw_ctx *c, const bbw_curved_ops *o, const float *vgeoq, c
     const int f = k / NFQP, j = k - f * NFQP;
     return j < Nfq ? o->Pfq[n + (size_t)(f * Nfq + j) * Np] : 0.f;
   }, b[9]);
-  float **dst[10] = {&c->cv_Vq, &c->cv_Vr, &c->cv_Vs, &c->cv_Vt, &c->cv_Pq, &c->cv_Pr,
-                     &c->cv_Ps, &c->cv_Pt, &c->cv_Vf, &c->cv_Pf};
+  const float *D3[3] = {o->Dr, o->Ds, o->Dt};
+  for (int m = 0; m < 3; ++m) {
+    const float *D = D3[m];
+    lane_blocks(Np, Np, [&](int n, int j) { return D[n + (size_t)j * Np]; }, b[10 + m]);
+  }
+  lane_blocks(Np, 4 * Nfp, [&](int n, int j) { return o->LIFT[n + (size_t)j * Np]; }, b[13]);
+  float **dst[14] = {&c->cv_Vq, &c->cv_Vr, &c->cv_Vs, &c->cv_Vt, &c->cv_Pq, &c->cv_Pr,
+                     &c->cv_Ps, &c->cv_Pt, &c->cv_Vf, &c->cv_Pf, &c->cv_Dr, &c->cv_Ds,
+                     &c->cv_Dt, &c->cv_Lift};
   int rc;
-  for (int m = 0; m < 10; ++m) {
+  for (int m = 0; m < 14; ++m) {
     dfree(*dst[m]);
     if ((rc = upload(c, dst[m], b[m].data(), b[m].size()))) return rc;
   }
@@ -1390,6 +1399,7 @@ int bbw_set_option(bbw_ctx *c, int option, int value) {
       c->plain_impl = value;
       break;
     case BBW_OPT_SRC_STEP_TIME: c->src_step_time = value ? 1 : 0; break;
+    case BBW_OPT_PLANAR_SPLIT: c->planar_split = value ? 1 : 0; break;
     default: return fail(c, BBW_ERR_ARG, "unknown option");
   }
   reset_graph(c);

@@ -3,10 +3,10 @@
 // :3024-3195) for a list of elements in one launch, written to rhs once.
 //
 // CURVED = true takes the geometric factors at every cubature point (vgeoq, Jq, fgeoq); CURVED =
-// false is the same form with the element's constant affine factors (vgeo, J, fgeo), used for the
-// planar elements in place of the reference's strong-form rk_volume_planar / rk_surface_planar
-// (:2203-2441): with the degree 2N+1 rules both forms are exact integrals of the same terms, so
-// they agree to rounding.
+// false is the same form with the element's constant affine factors (vgeo, J, fgeo), an option for
+// the planar elements (BBW_OPT_PLANAR_SPLIT); by default they take k_planar_rhs below, the
+// reference's strong form (rk_volume_planar / rk_surface_planar, :2203-2441).  With the degree
+// 2N+1 rules both forms are exact integrals of the same terms, so they agree to rounding.
 //
 // One workgroup (4 waves) owns a tile of TB = 16 listed elements (the N width of the FP32
 // 16x16x4 MFMA); A operands are lane-ordered blocks (as bbw_ctx::VqBlk), B operands LDS tiles:
@@ -368,6 +368,227 @@ __global__ __launch_bounds__(256) void k_curved_rhs(int Kl, const int *__restric
   }
 }
 
+// ------------------------------------------------------------------ planar elements
+// The reference's strong form for the planar elements of a curved mesh (rk_volume_planar +
+// rk_surface_planar, cpp/okl/WaveKernelsWADG.okl:2203-2441), nodal, scaled by J, on MFMA tiles of
+// 16 listed elements:
+//   1  state rows -> LDS; the upwind fluxes at the face nodes (Fscale J, the face normal) -> LDS
+//      F [4][FR][16] (p-flux and the U-flux times nx, ny, nz); the element's constants -> LDS
+//   2  u, v, w rows replaced by U.grad r, U.grad s, U.grad t (in place)
+//   3  per node tile: div = Dr Ur + Ds Us + Dt Ut, p_r,s,t = (Dr, Ds, Dt) p (M = nodes, K = nodes);
+//      rhs = J (-div, -grad p), then rhs += LIFT F (K = face nodes), all in the accumulators
+//   4  accumulators -> LDS rows -> rhs
+template <int N>
+struct PlanarLayout {
+  static constexpr int Np = Dims<N>::Np, Nfp = Dims<N>::Nfp, NfpNf = Dims<N>::NfpNf;
+  static constexpr int TB = 16, NW = 4 * Np, NPT = (Np + 15) / 16, MTW = (NPT + 3) / 4;
+  static constexpr int KTD = (Np + 3) / 4, KBD = (KTD + 3) / 4;       // K = nodes
+  static constexpr int KTL = (NfpNf + 3) / 4, KBL = (KTL + 3) / 4;    // K = face nodes
+  static constexpr int FR = 4 * KTL;                                  // flux rows (zero tail)
+  static constexpr int RS0 = NW > 3 * Np + 4 * KTD ? NW : 3 * Np + 4 * KTD;
+  static constexpr int RS = RS0 + ((34 - RS0 % 32) % 32);
+  static constexpr int OF = TB * RS, OG = OF + 4 * FR * 16, OEL = OG + TB * 10;
+  static constexpr size_t floats = (size_t)OEL + TB;
+};
+
+template <int N>
+__global__ __launch_bounds__(256) void k_planar_rhs(int Kl, const int *__restrict__ Klist, int K,
+                                                    const float *__restrict__ DrB,
+                                                    const float *__restrict__ DsB,
+                                                    const float *__restrict__ DtB,
+                                                    const float *__restrict__ LiftB,
+                                                    const int *__restrict__ Fmask,
+                                                    const float *__restrict__ vgeo,
+                                                    const float *__restrict__ fgeo,
+                                                    const float *__restrict__ Jpl,
+                                                    const int *__restrict__ vmapP,
+                                                    const float *__restrict__ Q,
+                                                    float *__restrict__ rhs) {
+  using L = PlanarLayout<N>;
+  constexpr int Np = L::Np, Nfp = L::Nfp, NfpNf = L::NfpNf, TB = L::TB, NW = L::NW, RS = L::RS;
+  constexpr int FR = L::FR;
+  extern __shared__ float lds[];
+  float *sQ = lds;                    // [TB][RS]
+  float *F = lds + L::OF;             // [4][FR][16]
+  float *sG = lds + L::OG;            // [TB][10]: rx ry rz sx sy sz tx ty tz J
+  int *sEl = (int *)(lds + L::OEL);   // [TB]
+  const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int col = lane & 15, kr = lane >> 4;
+  const int kk0 = blockIdx.x * TB, nk = min(TB, Kl - kk0);
+  const brsrc rQ = mkrsrc(Q, (unsigned)K * NW * 4u);
+  const unsigned DB = (unsigned)L::NPT * L::KBD * 1024u;
+  const brsrc rDr = mkrsrc(DrB, DB), rDs = mkrsrc(DsB, DB), rDt = mkrsrc(DtB, DB);
+  const brsrc rL = mkrsrc(LiftB, (unsigned)L::NPT * L::KBL * 1024u);
+  if (tid < TB) sEl[tid] = Klist[kk0 + (tid < nk ? tid : 0)];
+  if (tid < TB * 10) {
+    const int e = tid / 10, c = tid - e * 10, ke = Klist[kk0 + (e < nk ? e : 0)];
+    sG[tid] = c < 9 ? vgeo[(size_t)ke * 9 + c] : Jpl[ke];
+  }
+
+  // ---- 1: state rows -> LDS; fluxes at the face nodes -> F
+  {
+    constexpr int RR = (NW + 255) / 256;
+    float h[RR][TB];
+#pragma unroll
+    for (int k = 0; k < RR; ++k) {
+      const int rr = tid + 256 * k;
+#pragma unroll
+      for (int e = 0; e < TB; ++e) {
+        const int ke = Klist[kk0 + (e < nk ? e : 0)];
+        h[k][e] = bld(rQ, rr < NW ? (unsigned)(ke * NW + rr) * 4u : 0x80000000u);
+      }
+    }
+    for (int p = tid; p < TB * FR; p += 256) {
+      const int e = p / FR, m = p - e * FR;
+      float fp = 0.f, fu = 0.f, fv = 0.f, fw = 0.f;
+      if (m < NfpNf) {
+        const int f = m / Nfp, ke = Klist[kk0 + (e < nk ? e : 0)];
+        const int idM = Fmask[m] + ke * NW, idP = vmapP[(size_t)ke * NfpNf + m];
+        const float *fg = fgeo + (size_t)ke * 16 + 4 * f;
+        const float Fs = fg[0] * Jpl[ke], nx = fg[1], ny = fg[2], nz = fg[3];
+        const float pM = bld(rQ, idM * 4u), uM = bld(rQ, (idM + Np) * 4u);
+        const float vM = bld(rQ, (idM + 2 * Np) * 4u), wM = bld(rQ, (idM + 3 * Np) * 4u);
+        const float pP = bld(rQ, idP * 4u), uP = bld(rQ, (idP + Np) * 4u);
+        const float vP = bld(rQ, (idP + 2 * Np) * 4u), wP = bld(rQ, (idP + 3 * Np) * 4u);
+        float pj = pP - pM, Uj = (uP - uM) * nx + (vP - vM) * ny + (wP - wM) * nz;
+        if (idM == idP) {
+          pj = -2.f * pM;
+          Uj = 0.f;
+        }
+        fp = .5f * (pj - Uj) * Fs;
+        const float Uf = .5f * (Uj - pj) * Fs;
+        fu = Uf * nx; fv = Uf * ny; fw = Uf * nz;
+      }
+      float *Fp = F + m * 16 + e;
+      Fp[0] = fp;
+      Fp[FR * 16] = fu;
+      Fp[2 * FR * 16] = fv;
+      Fp[3 * FR * 16] = fw;
+    }
+#pragma unroll
+    for (int k = 0; k < RR; ++k) {
+      const int rr = tid + 256 * k;
+      if (rr < NW) {
+#pragma unroll
+        for (int e = 0; e < TB; ++e) sQ[e * RS + rr] = h[k][e];
+      }
+    }
+    constexpr int KPAD = L::RS0 - NW;
+    static_assert(TB * KPAD <= 256, "one padding word per thread");
+    if (KPAD > 0 && tid < TB * KPAD) sQ[(tid / KPAD) * RS + NW + tid % KPAD] = 0.f;
+  }
+  __syncthreads();
+
+  // ---- 2: u, v, w -> U.grad r, U.grad s, U.grad t in place
+  for (int p = tid; p < TB * Np; p += 256) {
+    const int e = p / Np, n = p - e * Np;
+    float *row = sQ + e * RS;
+    const float *G = sG + 10 * e;
+    const float u = row[Np + n], v = row[2 * Np + n], w = row[3 * Np + n];
+    row[Np + n] = u * G[0] + v * G[1] + w * G[2];
+    row[2 * Np + n] = u * G[3] + v * G[4] + w * G[5];
+    row[3 * Np + n] = u * G[6] + v * G[7] + w * G[8];
+  }
+  __syncthreads();
+
+  // ---- 3: volume and lift per node tile
+  const float *G = sG + 10 * col;
+  const float rx = G[0], ry = G[1], rz = G[2], sx = G[3], sy = G[4], sz = G[5];
+  const float tx = G[6], ty = G[7], tz = G[8], Jk = G[9];
+  f4 o[L::MTW][4];
+#pragma unroll
+  for (int i = 0; i < L::MTW; ++i) {
+    const int mt = wave + 4 * i;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) o[i][c] = f4{0.f, 0.f, 0.f, 0.f};
+    if (mt < L::NPT) {
+      f4 ad = f4{0.f, 0.f, 0.f, 0.f}, ar = ad, as = ad, at = ad;
+      const float *Bp = sQ + col * RS + kr;
+#pragma unroll
+      for (int kb = 0; kb < L::KBD; ++kb) {
+        const unsigned ao = (unsigned)(mt * L::KBD + kb) * 1024u;
+        const f4 aR = bld4_s(rDr, lane * 16u, ao), aS = bld4_s(rDs, lane * 16u, ao);
+        const f4 aT = bld4_s(rDt, lane * 16u, ao);
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+          const int kt = 4 * kb + s;
+          if (kt < L::KTD) {
+            const float bp = Bp[kt * 4], b1 = Bp[Np + kt * 4], b2 = Bp[2 * Np + kt * 4];
+            const float b3 = Bp[3 * Np + kt * 4];
+            ad = __builtin_amdgcn_mfma_f32_16x16x4f32(aR[s], b1, ad, 0, 0, 0);
+            ad = __builtin_amdgcn_mfma_f32_16x16x4f32(aS[s], b2, ad, 0, 0, 0);
+            ad = __builtin_amdgcn_mfma_f32_16x16x4f32(aT[s], b3, ad, 0, 0, 0);
+            ar = __builtin_amdgcn_mfma_f32_16x16x4f32(aR[s], bp, ar, 0, 0, 0);
+            as = __builtin_amdgcn_mfma_f32_16x16x4f32(aS[s], bp, as, 0, 0, 0);
+            at = __builtin_amdgcn_mfma_f32_16x16x4f32(aT[s], bp, at, 0, 0, 0);
+          }
+        }
+      }
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        o[i][0][r] = -Jk * ad[r];
+        o[i][1][r] = -Jk * (rx * ar[r] + sx * as[r] + tx * at[r]);
+        o[i][2][r] = -Jk * (ry * ar[r] + sy * as[r] + ty * at[r]);
+        o[i][3][r] = -Jk * (rz * ar[r] + sz * as[r] + tz * at[r]);
+      }
+      const float *Fb = F + kr * 16 + col;
+#pragma unroll
+      for (int kb = 0; kb < L::KBL; ++kb) {
+        const f4 a = bld4_s(rL, lane * 16u, (unsigned)(mt * L::KBL + kb) * 1024u);
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+          const int kt = 4 * kb + s;
+          if (kt < L::KTL) {
+#pragma unroll
+            for (int c = 0; c < 4; ++c)
+              o[i][c] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[s], Fb[(c * FR + kt * 4) * 16], o[i][c], 0, 0, 0);
+          }
+        }
+      }
+    }
+  }
+  __syncthreads();
+
+  // ---- 4: accumulators -> LDS rows -> rhs
+#pragma unroll
+  for (int i = 0; i < L::MTW; ++i) {
+    const int mt = wave + 4 * i;
+    if (mt < L::NPT) {
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int n = mt * 16 + 4 * kr + r;
+        if (n < Np) {
+#pragma unroll
+          for (int c = 0; c < 4; ++c) sQ[col * RS + c * Np + n] = o[i][c][r];
+        }
+      }
+    }
+  }
+  __syncthreads();
+  const brsrc rR = mkrsrc(rhs, (unsigned)K * NW * 4u);
+  for (int rr = tid; rr < NW; rr += 256) {
+#pragma unroll
+    for (int e = 0; e < TB; ++e)
+      if (e < nk) bst(rR, sQ[e * RS + rr], (unsigned)(sEl[e] * NW + rr) * 4u);
+  }
+}
+
+template <int N>
+static hipError_t planar_rhs_n(bbw_ctx *c, int Kl, const int *Klist) {
+  using L = PlanarLayout<N>;
+  if (Kl <= 0) return hipSuccess;
+  const size_t sh = sizeof(float) * L::floats;
+  if (sh > 65536) {
+    hipError_t e = hipFuncSetAttribute((const void *)k_planar_rhs<N>,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh);
+    if (e != hipSuccess) return e;
+  }
+  hipLaunchKernelGGL(k_planar_rhs<N>, dim3((Kl + L::TB - 1) / L::TB), dim3(256), sh, c->stream, Kl,
+                     Klist, c->K, c->cv_Dr, c->cv_Ds, c->cv_Dt, c->cv_Lift, c->Fmask, c->vgeo,
+                     c->fgeo, c->Jplanar, c->vmapP, c->Q[c->cur], c->rhs);
+  return hipGetLastError();
+}
+
 template <int N, bool CURVED>
 static hipError_t curved_rhs_n(bbw_ctx *c, int Kl, const int *Klist) {
   using L = CurvedLayout<N>;
@@ -391,7 +612,9 @@ template <int N>
 static hipError_t curved_n(bbw_ctx *c) {
   hipError_t e = curved_rhs_n<N, true>(c, c->Kcurved, c->Klist_curved);
   if (e != hipSuccess) return e;
-  return curved_rhs_n<N, false>(c, c->Kplanar, c->Klist_planar);
+  // planar elements: the strong form (default) or the split form with constant factors
+  return c->planar_split ? curved_rhs_n<N, false>(c, c->Kplanar, c->Klist_planar)
+                         : planar_rhs_n<N>(c, c->Kplanar, c->Klist_planar);
 }
 
 hipError_t launch_curved_rhs(bbw_ctx *c) {

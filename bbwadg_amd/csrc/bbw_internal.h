@@ -109,6 +109,8 @@ struct bbw_ctx {
   float *cv_Vq = nullptr, *cv_Vr = nullptr, *cv_Vs = nullptr, *cv_Vt = nullptr;
   float *cv_Pq = nullptr, *cv_Pr = nullptr, *cv_Ps = nullptr, *cv_Pt = nullptr;
   float *cv_Vf = nullptr, *cv_Pf = nullptr;
+  float *cv_Dr = nullptr, *cv_Ds = nullptr, *cv_Dt = nullptr, *cv_Lift = nullptr;  // planar elements
+  int planar_split = 0;   // 1: planar elements take the split form with constant factors
   float *vgeoq = nullptr, *cv_Jq = nullptr, *fgeoq = nullptr, *Jplanar = nullptr;
   int *Klist_curved = nullptr, *Klist_planar = nullptr;
 

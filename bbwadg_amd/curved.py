@@ -187,37 +187,39 @@ def gordon_hall_sphere(mesh, ops, radius=1.0, tol=SPHERE_TOL):
     x, y, z = (np.ascontiguousarray(a.T) for a in M.affine_points(mesh, ops.r, ops.s, ops.t))
     K = mesh.K
 
-    def radius_err(ids, e, X, Y, Z):
-        return np.abs(np.sqrt(X[e, ids] ** 2 + Y[e, ids] ** 2 + Z[e, ids] ** 2) - radius).max()
+    def on_sphere(ids, X, Y, Z):   # (K,) all nodes ids of each element within tol of the sphere
+        r = np.sqrt(X[:, ids] ** 2 + Y[:, ids] ** 2 + Z[:, ids] ** 2)
+        return np.abs(r - radius).max(axis=1) < tol
 
-    edge_curved = np.zeros(K, bool)
-    face_curved = np.zeros(K, bool)
+    def snap(sel, ids, X, Y, Z, Xc, Yc, Zc):
+        e = np.nonzero(sel)[0][:, None]
+        rb = np.sqrt(X[e, ids] ** 2 + Y[e, ids] ** 2 + Z[e, ids] ** 2) / radius
+        Xc[e, ids], Yc[e, ids], Zc[e, ids] = X[e, ids] / rb, Y[e, ids] / rb, Z[e, ids] / rb
+
     xc, yc, zc = x.copy(), y.copy(), z.copy()
-    for e in range(K):
-        for k, (a, b) in enumerate(_EDGES):
-            if radius_err([verts[a], verts[b]], e, x, y, z) < tol:
-                edge_curved[e] = True
-                ids = edge_nodes[k]
-                rb = np.sqrt(x[e, ids] ** 2 + y[e, ids] ** 2 + z[e, ids] ** 2) / radius
-                xc[e, ids], yc[e, ids], zc[e, ids] = x[e, ids] / rb, y[e, ids] / rb, z[e, ids] / rb
-    for e in np.nonzero(edge_curved)[0]:
-        for X, Xc in ((x, xc), (y, yc), (z, zc)):
-            X[e] = X[e] + edge_blend @ (Xc[e, eint_all] - X[e, eint_all])
+    edge_curved = np.zeros(K, bool)
+    for k, (a, b) in enumerate(_EDGES):
+        sel = on_sphere([verts[a], verts[b]], x, y, z)
+        edge_curved |= sel
+        if sel.any():
+            snap(sel, edge_nodes[k], x, y, z, xc, yc, zc)
+    ec = np.nonzero(edge_curved)[0]
+    for X, Xc in ((x, xc), (y, yc), (z, zc)):
+        X[ec] = X[ec] + (Xc[ec][:, eint_all] - X[ec][:, eint_all]) @ edge_blend.T
+    face_curved = np.zeros(K, bool)
     if N > 2:
         xc, yc, zc = x.copy(), y.copy(), z.copy()
-        for e in range(K):
-            for f in range(4):
-                fv = [verts[v] for v in _face_vertices(f)]
-                if radius_err(fv, e, x, y, z) < tol:
-                    face_curved[e] = True
-                    ids = ops.Fmask[f]
-                    rb = np.sqrt(x[e, ids] ** 2 + y[e, ids] ** 2 + z[e, ids] ** 2) / radius
-                    xc[e, ids], yc[e, ids], zc[e, ids] = x[e, ids] / rb, y[e, ids] / rb, z[e, ids] / rb
-        for e in np.nonzero(face_curved)[0]:
-            keep = np.concatenate([np.full(len(face_int[f]), mesh.EToF[e, f] == f) for f in range(4)])
-            for X, Xc in ((x, xc), (y, yc), (z, zc)):
-                d = (Xc[e, fint_all] - X[e, fint_all]) * keep
-                X[e] = X[e] + face_blend @ d
+        for f in range(4):
+            sel = on_sphere([verts[v] for v in _face_vertices(f)], x, y, z)
+            face_curved |= sel
+            if sel.any():
+                snap(sel, ops.Fmask[f], x, y, z, xc, yc, zc)
+        fc = np.nonzero(face_curved)[0]
+        keep = np.concatenate([np.repeat((mesh.EToF[fc, f] == f)[:, None], len(face_int[f]), axis=1)
+                               for f in range(4)], axis=1)
+        for X, Xc in ((x, xc), (y, yc), (z, zc)):
+            d = (Xc[fc][:, fint_all] - X[fc][:, fint_all]) * keep
+            X[fc] = X[fc] + d @ face_blend.T
     curved = edge_curved | face_curved
     return CurvedGeometry(x=x, y=y, z=z, curved=curved,
                           Klist_curved=np.nonzero(curved)[0].astype(np.int32),

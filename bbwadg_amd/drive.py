@@ -57,6 +57,9 @@ def main():
                     help="plain kernels (default: tiled for N <= 6, flat with the slice lift above)")
     ap.add_argument("--project-velocity", action="store_true")
     ap.add_argument("--elastic", action="store_true", help="elastic (9 fields), fused or plain stage")
+    ap.add_argument("--curved", default=None, metavar="MSH",
+                    help="curvilinear run of a ball mesh (GMSH file; 'sphere49748' is the packaged "
+                         "mesh): Gordon-Hall map, Bessel start, RK_step_WADG stages")
     ap.add_argument("--test-rk", action="store_true",
                     help="instead: time the WADG update variants per launch (test_RK): the plain "
                          "MFMA update (all four fields + LSRK) and the exact BB-WADG pressure update")
@@ -72,6 +75,29 @@ def main():
         b.copy_(a)
         torch.cuda.synchronize()
         print(f"copied {n * 4} bytes", flush=True)
+        return
+    if args.curved:
+        import os
+        from . import curved
+        from .solver import CurvedWaveSolver
+        path = args.curved
+        if not os.path.exists(path):
+            path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data", "meshes",
+                                args.curved + ".msh")
+        m = M.read_gmsh(path)
+        M.face_pair(m)
+        s = CurvedWaveSolver(m, args.N)
+        s.WaveSetData3d(s.WaveSetU0(curved.bessel))
+        dt = s.dt_ref
+        s.step(2, dt)
+        s.sync()
+        t0 = time.perf_counter()
+        s.step(args.steps, dt)
+        s.sync()
+        el = time.perf_counter() - t0
+        print(f"K={s.K} (curved {len(s.cm.Klist_curved)}) N={args.N} curved steps={args.steps} "
+              f"ms/stage={el / (5 * args.steps) * 1e3:.4f} "
+              f"DOF-stage/s={s.K * s.ops.Np * 4 * 5 * args.steps / el:.4e}", flush=True)
         return
     if args.test_rk:
         s = setup(args.mesh, args.N)

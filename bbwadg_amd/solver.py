@@ -343,7 +343,8 @@ class CurvedWaveSolver(WaveSolver):
 
         k = dict(Vq=cm32(no.Vq), Vrq=cm32(no.Vrq), Vsq=cm32(no.Vsq), Vtq=cm32(no.Vtq),
                  Pq=cm32(no.Pq), Prq=cm32(no.Prq), Psq=cm32(no.Psq), Ptq=cm32(no.Ptq),
-                 VfqFace=cm32(no.VfqFace), Pfq=cm32(no.Pfq), Fmask=i32(no.Fmask.reshape(-1)),
+                 VfqFace=cm32(no.VfqFace), Pfq=cm32(no.Pfq), Dr=cm32(no.Dr), Ds=cm32(no.Ds),
+                 Dt=cm32(no.Dt), LIFT=cm32(no.LIFT), Fmask=i32(no.Fmask.reshape(-1)),
                  wq=f64(no.wq), rk4a=f64(self.ops.rk4a), rk4b=f64(self.ops.rk4b), rk4c=f64(self.ops.rk4c))
         geo = dict(vgeoq=f32(cm.vgeoq), Jq=f32(cm.Jq), fgeoq=f32(cm.fgeoq), Jplanar=f32(cm.Jplanar),
                    Kc=i32(cm.Klist_curved), Kp=i32(cm.Klist_planar))

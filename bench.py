@@ -8,8 +8,8 @@ FP32.  A step is one LSRK45 step (5 stages) over the whole mesh, inputs resident
 Reported: the fused path (one kernel per stage, HIP-graph replay) as ``value``; the plain
 three-launch path of the reference (volume, surface, WADG update) beside it; the
 roofline of the fused stage kernel; the adjoint-gradient / forward ratio; the batched
-multi-shot run; the elastic configuration's stage rate; the CPU oracle timed on a bounded
-sample.
+multi-shot run; the elastic configuration's stage rate; the curvilinear run of the reference's
+sphere mesh (49 748 tets, N = 5); the CPU oracle timed on a bounded sample.
 
     python bench.py [--steps K] [--warmup W] [--gpus 1] [--mesh 48] [--N 5]
 """
@@ -189,6 +189,63 @@ def elastic_throughput(nsteps, mesh_n=32, N=4):
             "frac_of_stage_roofline": stage_bytes(N, 9, 3) * K / (ms_stage / 1e3) / 1e9 / HBM_PEAK_GBS}
 
 
+def curved_bytes(N, curved):
+    """Compulsory HBM bytes per element-stage of the curvilinear stage (right-hand side kernel
+    plus WADG update): state read (4 Np) and rhs written and read again (2 x 4 Np), res and Q
+    read and written (4 x 4 Np), neighbour traces (4 x 4 Nfp), vmapP (4 Nfp), c2 and 1/J at the
+    Nq points (2 Nq), and the element's geometry: at the cubature points for a curved element
+    (vgeoq 9 Nq, Jq Nq, fgeoq 16 Nfq), constants for a planar one (9 + 1 + 16); 4 bytes a word."""
+    Np = (N + 1) * (N + 2) * (N + 3) // 6
+    Nfp = (N + 1) * (N + 2) // 2
+    Nq = {1: 6, 2: 14, 3: 31, 4: 57, 5: 95, 6: 146}[N]
+    Nfq = {1: 6, 2: 7, 3: 15, 4: 19, 5: 28, 6: 40}[N]
+    geo = 10 * Nq + 16 * Nfq if curved else 26
+    return 4 * (4 * Np + 2 * 4 * Np + 4 * 4 * Np + 4 * 4 * Nfp + 4 * Nfp + 2 * Nq + geo)
+
+
+def curved_throughput(nsteps, N=5):
+    """The reference's curvilinear run (cpp/main.cpp with runCurvedBlkTimings' mesh): the
+    49 748-tet ball mesh sphere49748.msh, Gordon-Hall sphere map, Bessel standing mode, c^2 = 1,
+    the reference's time step; nsteps RK_step_WADG steps from a HIP graph, HIP events on the
+    solver's stream."""
+    import torch
+    from bbwadg_amd import curved, mesh as M
+    from bbwadg_amd.solver import CurvedWaveSolver
+    m = M.read_gmsh(os.path.join(ROOT, "bbwadg_amd", "data", "meshes", "sphere49748.msh"))
+    M.face_pair(m)
+    s = CurvedWaveSolver(m, N)
+    stream = torch.cuda.Stream()
+    s.set_stream(stream.cuda_stream)
+    dt = s.dt_ref
+    s.WaveSetData3d(s.WaveSetU0(curved.bessel))
+    s.step(2, dt)
+    s.prepare_graph(nsteps, dt)
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    s.step(nsteps, dt)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1)
+    T = s.time
+    err = s.compute_error(T, curved.bessel)
+    if not np.isfinite(err[0]):
+        raise RuntimeError("non-finite curved state after the timed run")
+    K, Kc = s.K, len(s.cm.Klist_curved)
+    dofs = K * s.ops.Np * 4 * 5 * nsteps
+    s.close()
+    ms_stage = ms / (5 * nsteps)
+    B = (Kc * curved_bytes(N, True) + (K - Kc) * curved_bytes(N, False))
+    return {"workload": f"curvilinear acoustic, sphere49748.msh ({K} tets, {Kc} curved), N={N}, "
+                        "Gordon-Hall sphere map, Bessel mode, c^2 = 1",
+            "value": dofs / (ms / 1e3), "unit": "DOF-stage/s", "steps": nsteps, "dt": dt,
+            "ms_per_stage": ms_stage, "bytes_per_stage": B,
+            "achieved_GBs": B / (ms_stage / 1e3) / 1e9,
+            "frac_of_hbm_peak": B / (ms_stage / 1e3) / 1e9 / HBM_PEAK_GBS,
+            "l2_error_vs_exact": err[0], "rel_l2_error_vs_exact": err[1], "time": T}
+
+
 def batched_throughput(step_scale=1.0, tune_steps=20):
     """Batched configuration (64 shots on 16^3..40^3 x 6 meshes, N = 5, durations 300-1200
     steps): one persistent launch per stage with the autotuned shots-per-tile and
@@ -258,6 +315,8 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=1 << 30)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--adjoint-steps", type=int, default=800)
+    ap.add_argument("--curved-steps", type=int, default=50,
+                    help="steps of the curvilinear leg (0 skips it)")
     ap.add_argument("--elastic-steps", type=int, default=50,
                     help="steps of the elastic configuration (0 skips it)")
     ap.add_argument("--batched-scale", type=float, default=1.0,
@@ -333,6 +392,7 @@ def main():
     s.close()
     adj = adjoint_ratio(args.adjoint_steps) if args.adjoint_steps > 0 else None
     elas = elastic_throughput(args.elastic_steps) if args.elastic_steps > 0 else None
+    curv = curved_throughput(args.curved_steps) if args.curved_steps > 0 else None
     bat = batched_throughput(args.batched_scale) if args.batched_scale > 0 else None
 
     cpu = None
@@ -371,6 +431,7 @@ def main():
         "adjoint": adj,
         "batched": bat,
         "elastic": elas,
+        "curved": curv,
         "energy_after": E,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,

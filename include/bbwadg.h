@@ -159,6 +159,8 @@ int bbw_stage_fused(bbw_ctx *ctx, float fa, float fb, float fdt, float ftime, in
  *   Pq, Prq, Psq, Ptq  Np x Nq   M^-1 V*^T W
  *   VfqFace            Nfq x Nfp face nodes (Fmask order) -> the face rule
  *   Pfq                Np x 4Nfq M^-1 Vfq^T Wf (faces in order)
+ *   Dr, Ds, Dt         Np x Np   nodal derivatives at the nodes (planar elements)
+ *   LIFT               Np x 4Nfp nodal lift (planar elements)
  * Vq and Pq also serve as the WADG update's Vq_reduced / Pq_reduced (the same rule; its 1/J is
  * bbw_upload_media's Jq).  Per element: vgeoq [K][9][Nq] (rx sx tx ry sy ty rz sz tz at the volume
  * points), Jq [K][Nq], fgeoq [K][4][4][Nfq] (per face nx ny nz sJ at the face points), Jplanar [K],
@@ -170,6 +172,8 @@ typedef struct {
   const float *Pq, *Prq, *Psq, *Ptq;
   const float *VfqFace;
   const float *Pfq;
+  const float *Dr, *Ds, *Dt;
+  const float *LIFT;
   const int32_t *Fmask;
   const double *wq;
   const double *rk4a, *rk4b, *rk4c;
@@ -289,6 +293,9 @@ double bbw_batch_last_ms(const bbw_batch *b); /* device time of the last bbw_bat
  * step's start time for all five stages, as the reference's elastic driver does
  * (elas_cpp/src/WaveOKL3d.cpp:1675-1680; first order in time for the source term) */
 #define BBW_OPT_SRC_STEP_TIME 8
+#define BBW_OPT_PLANAR_SPLIT 9 /* curvilinear context: 1 runs the planar elements through the split
+                                 form with constant factors; 0 (default) the strong nodal form of
+                                 rk_volume_planar / rk_surface_planar */
 int bbw_set_option(bbw_ctx *ctx, int option, int value);
 
 /* Device addresses of the context's buffers (for copies into caller-owned device memory):

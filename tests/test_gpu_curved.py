@@ -4,8 +4,9 @@ oracle/curved.py (pinned in tests/test_curved_oracle.py), on the reference's sph
 the Gordon-Hall map (cpp/main.cpp).
 
 Tolerances: the right-hand side 5e-5 of its max-norm against the FP64 oracle (FP32 products of
-nodal operators, whose entries reach ~10, summed in another order; the planar elements take
-the split form where the oracle takes the reference's strong form, equal in exact arithmetic),
+nodal operators, whose entries reach ~10, summed in another order; with BBW_OPT_PLANAR_SPLIT the
+planar elements take the split form where the oracle takes the reference's strong form, equal
+in exact arithmetic),
 the update 2e-5, whole runs 1e-5 relative L2 against the FP64 oracle.
 """
 import math
@@ -33,10 +34,14 @@ def make(N, name="sphere385.msh", sphere=True):
     return CurvedWaveSolver(m, N, sphere=sphere)
 
 
+@pytest.mark.parametrize("split", [0, 1])
 @pytest.mark.parametrize("N", [1, 2, 3, 4, 5, 6])
-def test_curved_rhs_matches_oracle(N):
-    print(f"case curved rhs N={N}", flush=True)
+def test_curved_rhs_matches_oracle(N, split):
+    """Planar elements through the strong nodal form (default) and through the split form with
+    constant factors (BBW_OPT_PLANAR_SPLIT)."""
+    print(f"case curved rhs N={N} planar_split={split}", flush=True)
     s = make(N)
+    s.set_option(_lib.OPT_PLANAR_SPLIT, split)
     assert len(s.cm.Klist_curved) == 321 and len(s.cm.Klist_planar) == 64
     Q = np.random.default_rng(N).uniform(-1, 1, (s.K, 4, s.ops.Np)).astype(np.float32)
     s.WaveSetData3d(Q)
