@@ -243,6 +243,10 @@ void bbw_destroy(bbw_ctx *c) {
   dfree(c->partial); dfree(c->ftime_win); dfree(c->ft_all); dfree(c->dobs);
   if (c->ft_host) (void)hipHostFree(c->ft_host);
   if (c->ft_evt) (void)hipEventDestroy(c->ft_evt);
+  if (c->side_stream) (void)hipStreamSynchronize(c->side_stream);
+  if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
+  if (c->ev_join) (void)hipEventDestroy(c->ev_join);
+  if (c->side_stream) (void)hipStreamDestroy(c->side_stream);
   if (c->own_stream) (void)hipStreamDestroy(c->own_stream);
   delete c;
 }
@@ -674,6 +678,11 @@ int bbw_upload_curved(bbw_ctx *c, const bbw_curved_ops *o, const float *vgeoq, c
   std::memcpy(c->rk4a, o->rk4a, 5 * sizeof(double));
   std::memcpy(c->rk4b, o->rk4b, 5 * sizeof(double));
   std::memcpy(c->rk4c, o->rk4c, 6 * sizeof(double));
+  if (!c->side_stream) {
+    HIPCHK(c, hipStreamCreateWithFlags(&c->side_stream, hipStreamNonBlocking));
+    HIPCHK(c, hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
+    HIPCHK(c, hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming));
+  }
   c->Nfq = Nfq;
   c->Kcurved = Kcurved;
   c->Kplanar = Kplanar;

@@ -574,7 +574,7 @@ __global__ __launch_bounds__(256) void k_planar_rhs(int Kl, const int *__restric
 }
 
 template <int N>
-static hipError_t planar_rhs_n(bbw_ctx *c, int Kl, const int *Klist) {
+static hipError_t planar_rhs_n(bbw_ctx *c, int Kl, const int *Klist, hipStream_t st) {
   using L = PlanarLayout<N>;
   if (Kl <= 0) return hipSuccess;
   const size_t sh = sizeof(float) * L::floats;
@@ -583,14 +583,14 @@ static hipError_t planar_rhs_n(bbw_ctx *c, int Kl, const int *Klist) {
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh);
     if (e != hipSuccess) return e;
   }
-  hipLaunchKernelGGL(k_planar_rhs<N>, dim3((Kl + L::TB - 1) / L::TB), dim3(256), sh, c->stream, Kl,
+  hipLaunchKernelGGL(k_planar_rhs<N>, dim3((Kl + L::TB - 1) / L::TB), dim3(256), sh, st, Kl,
                      Klist, c->K, c->cv_Dr, c->cv_Ds, c->cv_Dt, c->cv_Lift, c->Fmask, c->vgeo,
                      c->fgeo, c->Jplanar, c->vmapP, c->Q[c->cur], c->rhs);
   return hipGetLastError();
 }
 
 template <int N, bool CURVED>
-static hipError_t curved_rhs_n(bbw_ctx *c, int Kl, const int *Klist) {
+static hipError_t curved_rhs_n(bbw_ctx *c, int Kl, const int *Klist, hipStream_t st) {
   using L = CurvedLayout<N>;
   if (Kl <= 0) return hipSuccess;
   const size_t sh = sizeof(float) * L::floats;
@@ -603,18 +603,34 @@ static hipError_t curved_rhs_n(bbw_ctx *c, int Kl, const int *Klist) {
   const CurvedOps o{c->cv_Vq, c->cv_Vr, c->cv_Vs, c->cv_Vt, c->cv_Pq, c->cv_Pr, c->cv_Ps,
                     c->cv_Pt, c->cv_Vf, c->cv_Pf, c->Fmask};
   hipLaunchKernelGGL((k_curved_rhs<N, CURVED>), dim3((Kl + L::TB - 1) / L::TB), dim3(256), sh,
-                     c->stream, Kl, Klist, c->K, o, c->vgeoq, c->cv_Jq, c->fgeoq, c->vgeo,
+                     st, Kl, Klist, c->K, o, c->vgeoq, c->cv_Jq, c->fgeoq, c->vgeo,
                      c->fgeo, c->Jplanar, c->vmapP, c->Q[c->cur], c->rhs);
   return hipGetLastError();
 }
 
+// The two element lists write disjoint rhs rows, so the curved list runs on a second stream
+// beside the planar one (fork and join through events; inside a graph capture the second stream
+// joins the capture): the curved list alone does not fill the GPU.
 template <int N>
 static hipError_t curved_n(bbw_ctx *c) {
-  hipError_t e = curved_rhs_n<N, true>(c, c->Kcurved, c->Klist_curved);
-  if (e != hipSuccess) return e;
+  const bool fork = c->Kcurved > 0 && c->Kplanar > 0 && c->side_stream;
+  hipError_t e;
+  hipStream_t sc = c->stream;
+  if (fork) {
+    if ((e = hipEventRecord(c->ev_fork, c->stream)) != hipSuccess) return e;
+    if ((e = hipStreamWaitEvent(c->side_stream, c->ev_fork, 0)) != hipSuccess) return e;
+    sc = c->side_stream;
+  }
+  if ((e = curved_rhs_n<N, true>(c, c->Kcurved, c->Klist_curved, sc)) != hipSuccess) return e;
   // planar elements: the strong form (default) or the split form with constant factors
-  return c->planar_split ? curved_rhs_n<N, false>(c, c->Kplanar, c->Klist_planar)
-                         : planar_rhs_n<N>(c, c->Kplanar, c->Klist_planar);
+  e = c->planar_split ? curved_rhs_n<N, false>(c, c->Kplanar, c->Klist_planar, c->stream)
+                      : planar_rhs_n<N>(c, c->Kplanar, c->Klist_planar, c->stream);
+  if (e != hipSuccess) return e;
+  if (fork) {
+    if ((e = hipEventRecord(c->ev_join, c->side_stream)) != hipSuccess) return e;
+    if ((e = hipStreamWaitEvent(c->stream, c->ev_join, 0)) != hipSuccess) return e;
+  }
+  return hipSuccess;
 }
 
 hipError_t launch_curved_rhs(bbw_ctx *c) {

@@ -111,6 +111,8 @@ struct bbw_ctx {
   float *cv_Vf = nullptr, *cv_Pf = nullptr;
   float *cv_Dr = nullptr, *cv_Ds = nullptr, *cv_Dt = nullptr, *cv_Lift = nullptr;  // planar elements
   int planar_split = 0;   // 1: planar elements take the split form with constant factors
+  hipStream_t side_stream = nullptr;            // the curved list's stream (default priority)
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   float *vgeoq = nullptr, *cv_Jq = nullptr, *fgeoq = nullptr, *Jplanar = nullptr;
   int *Klist_curved = nullptr, *Klist_planar = nullptr;
 
