@@ -61,7 +61,7 @@ def test_face_pairing(name):
 
 def test_reference_triangle_face_key_would_pair_distinct_triangles():
     """The reference keys a triangular face by [-1, v0, v1] of its sorted vertices (MeshHybrid.cpp:
-    343-352): two triangles with the same two smallest vertices collide.  Two prisms stacked on
+    348-351): two triangles with the same two smallest vertices collide.  Two prisms stacked on
     a common quad face, whose bottom triangles share an edge: full-vertex keys keep them apart."""
     m = H.HybridMesh(VX=np.array([0., 1, 0, 0, 0, 1, 0, 0]), VY=np.array([0., 0, 1, -1, 0, 0, 1, -1]),
                      VZ=np.array([0., 0, 0, 0, 1, 1, 1, 1]),
