@@ -572,7 +572,9 @@ hipError_t launch_surface(bbw_ctx *c) {
 
 hipError_t launch_update_wadg(bbw_ctx *c, float fa, float fb, float fdt, const float *ftd,
                               float ft) {
-  if (c->plain_impl == BBW_PLAIN_TILED) return launch_update_wadg_mfma(c, fa, fb, fdt, ftd, ft);
+  // orders 7..9 (no tiled kernels): the chunked MFMA update unless the flat kernels are asked for
+  if (c->plain_impl == BBW_PLAIN_TILED || (c->N > BBW_MAX_ORDER_TILED && c->plain_impl == BBW_PLAIN_SLICE))
+    return launch_update_wadg_mfma(c, fa, fb, fdt, ftd, ft);
   switch (c->N) {
     case 1: return update_wadg_n<1>(c, fa, fb, fdt, ftd, ft);
     case 2: return update_wadg_n<2>(c, fa, fb, fdt, ftd, ft);

@@ -333,6 +333,169 @@ static hipError_t update_wadg_mfma_n(bbw_ctx *c, float fa, float fb, float fdt, 
   return hipGetLastError();
 }
 
+// The same update for orders 7..9, whose cubature (214, 729, 1000 points) does not fit LDS as a
+// whole: the points are taken in chunks of QC; per chunk S_f = (Vq_chunk rhs_f) x c2/J or 1/J
+// goes through LDS and rhs_f accumulates Pq_chunk S_f in registers over the chunks.
+template <int N>
+struct UpdChunkLayout {
+  static constexpr int Np = Dims<N>::Np, Nq = Dims<N>::Nq, TB = 16, NpNf = 4 * Np;
+  static constexpr int NQT = (Nq + 15) / 16, NPT = (Np + 15) / 16, MTW = (NPT + 3) / 4;
+  static constexpr int KTV = (Np + 3) / 4, KBV = (KTV + 3) / 4;
+  static constexpr int QCT = 4, QC = 16 * QCT;           // q tiles and points per chunk
+  static constexpr int NCH = (NQT + QCT - 1) / QCT;
+  static constexpr int RS0 = 4 * Np > 3 * Np + 4 * KTV ? 4 * Np : 3 * Np + 4 * KTV;
+  static constexpr int RS = RS0 + ((34 - RS0 % 32) % 32);
+  // rhs rows [TB][RS] | S [4 fields][QC][16] | source slots [TB] | one scratch word
+  static constexpr size_t floats = (size_t)TB * RS + 4 * QC * 16 + TB + 1;
+};
+
+template <int N>
+__global__ __launch_bounds__(256) void k_update_wadg_chunk(int K, const float *__restrict__ VqBlk,
+                                                           const float *__restrict__ PqBlk,
+                                                           const float *__restrict__ c2q,
+                                                           const float *__restrict__ Jq, float fa,
+                                                           float fb, float fdt, SrcArgs src,
+                                                           const float *ftime_dev, float ftime,
+                                                           const float *__restrict__ rhs,
+                                                           float *__restrict__ res,
+                                                           float *__restrict__ Q) {
+  using L = UpdChunkLayout<N>;
+  constexpr int Np = L::Np, Nq = L::Nq, TB = L::TB, NpNf = L::NpNf, RS = L::RS, QC = L::QC;
+  constexpr unsigned ROW = NpNf * 4u;
+  extern __shared__ float lds[];
+  float *sR = lds;                         // [TB][RS]
+  float *S = sR + TB * RS;                 // [4][QC][16]
+  int *sSlot = (int *)(S + 4 * QC * 16);   // [TB]
+  float *sScr = (float *)(sSlot + TB);
+  const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int col = lane & 15, kr = lane >> 4;
+  const int k0 = blockIdx.x * TB, nk = min(TB, K - k0);
+  const size_t base = (size_t)k0 * NpNf;
+  const unsigned tbytes = (unsigned)nk * ROW, qbytes = (unsigned)nk * Nq * 4u;
+  const brsrc rRh = mkrsrc(rhs + base, tbytes), rRs = mkrsrc(res + base, tbytes);
+  const brsrc rQ = mkrsrc(Q + base, tbytes), rC = mkrsrc(c2q + (size_t)k0 * Nq, qbytes);
+  const brsrc rJ = mkrsrc(Jq ? Jq + (size_t)k0 * Nq : nullptr, Jq ? qbytes : 0u);
+  const brsrc rAV = mkrsrc(VqBlk, (unsigned)L::NQT * L::KBV * 1024u);
+  const brsrc rAP = mkrsrc(PqBlk, (unsigned)L::NPT * L::NQT * 1024u);
+  const bool has_src = src.nsrc > 0 && k0 + nk - 1 >= src.first && k0 <= src.last;
+
+  // ---- 1: rhs tile -> LDS (zero padding past the last field for the K tail)
+  {
+    TileRows<NpNf, TB> rows;
+    rows.load(rRh, ROW, tid);
+    constexpr int KPAD = L::RS0 - NpNf;
+    static_assert(TB * KPAD <= 256, "one padding word per thread");
+    if (KPAD > 0 && tid < TB * KPAD) sR[(tid / KPAD) * RS + NpNf + tid % KPAD] = 0.f;
+    if (has_src && tid < TB) sSlot[tid] = tid < nk ? src_slot2(src, k0 + tid) : -1;
+    rows.template store<RS>(sR, sScr, tid);
+  }
+  __syncthreads();
+
+  f4 out[L::MTW][4];
+#pragma unroll
+  for (int i = 0; i < L::MTW; ++i)
+#pragma unroll
+    for (int b = 0; b < 4; ++b) out[i][b] = f4{0.f, 0.f, 0.f, 0.f};
+  const unsigned co = (unsigned)(col * Nq + 4 * kr) * 4u;
+  for (int ch = 0; ch < L::NCH; ++ch) {
+    // ---- 2: S_f for the chunk's q tiles, one per wave
+    {
+      const int qt = ch * L::QCT + wave;       // QCT = 4 tiles per chunk, one per wave
+      f4 acc[4];
+#pragma unroll
+      for (int b = 0; b < 4; ++b) acc[b] = f4{0.f, 0.f, 0.f, 0.f};
+      if (qt < L::NQT) {
+        const float *Bp = sR + col * RS + kr;
+        for (int kb = 0; kb < L::KBV; ++kb) {
+          const f4 a = bld4_s(rAV, lane * 16u, (unsigned)(qt * L::KBV + kb) * 1024u);
+#pragma unroll
+          for (int sb = 0; sb < 4; ++sb) {
+            const int kt = 4 * kb + sb;
+            if (kt < L::KTV) {
+#pragma unroll
+              for (int b = 0; b < 4; ++b)
+                acc[b] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[sb], Bp[b * Np + kt * 4], acc[b], 0, 0, 0);
+            }
+          }
+        }
+      }
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int q = qt * 16 + 4 * kr + r;
+        const bool in = qt < L::NQT && q < Nq;
+        const unsigned qo = co + (unsigned)(qt * 16 + r) * 4u;
+        const float c2 = in ? bld(rC, qo) : 0.f;
+        const float jr = in ? (Jq ? 1.f / bld(rJ, qo) : 1.f) : 0.f;
+        const int ql = wave * 16 + 4 * kr + r;
+        S[ql * 16 + col] = acc[0][r] * c2 * jr;
+#pragma unroll
+        for (int b = 1; b < 4; ++b) S[(b * QC + ql) * 16 + col] = acc[b][r] * jr;
+      }
+    }
+    __syncthreads();
+    // ---- 3: rhs_f += Pq_chunk S_f (M = nodes, K = the chunk's points)
+#pragma unroll
+    for (int i = 0; i < L::MTW; ++i) {
+      const int mt = wave + 4 * i;
+      if (mt < L::NPT) {
+        const float *Bp = S + kr * 16 + col;
+#pragma unroll
+        for (int t = 0; t < L::QCT; ++t) {
+          const int qt = ch * L::QCT + t;
+          if (qt < L::NQT) {
+            const f4 a = bld4_s(rAP, lane * 16u, (unsigned)(mt * L::NQT + qt) * 1024u);
+#pragma unroll
+            for (int sb = 0; sb < 4; ++sb) {
+              const int k = t * 16 + sb * 4;
+#pragma unroll
+              for (int b = 0; b < 4; ++b)
+                out[i][b] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[sb], Bp[(b * QC + k) * 16], out[i][b], 0, 0, 0);
+            }
+          }
+        }
+      }
+    }
+    __syncthreads();
+  }
+  // ---- 4: rhs rows in LDS, then the LSRK stage
+#pragma unroll
+  for (int i = 0; i < L::MTW; ++i) {
+    const int mt = wave + 4 * i;
+    if (mt < L::NPT) {
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int n = mt * 16 + 4 * kr + r;
+        if (n < Np) {
+#pragma unroll
+          for (int b = 0; b < 4; ++b) sR[col * RS + b * Np + n] = out[i][b][r];
+        }
+      }
+    }
+  }
+  __syncthreads();
+  const float ft = ftime_dev ? *ftime_dev : ftime;
+  lsrk_rows<NpNf, RS, TB, Np>(rRs, rQ, ROW, sR, sSlot, has_src, src, ft, fa, fb, fdt, tid,
+                              [&](int f) { return w4(src, f); });
+}
+
+template <int N>
+static hipError_t update_wadg_chunk_n(bbw_ctx *c, float fa, float fb, float fdt, const float *ftd,
+                                      float ft, const SrcArgs &s) {
+  using L = UpdChunkLayout<N>;
+  const size_t sh = sizeof(float) * L::floats;
+  if ((unsigned long long)c->K * 16 * L::Np >= (1ull << 32)) return hipErrorInvalidValue;
+  if (sh > 160 * 1024) return hipErrorInvalidValue;
+  if (sh > 65536) {
+    hipError_t e = hipFuncSetAttribute((const void *)k_update_wadg_chunk<N>,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh);
+    if (e != hipSuccess) return e;
+  }
+  hipLaunchKernelGGL(k_update_wadg_chunk<N>, dim3((c->K + L::TB - 1) / L::TB), dim3(256), sh,
+                     c->stream, c->K, c->VqBlk, c->PqBlk, c->c2q, c->Jq, fa, fb, fdt, s, ftd, ft,
+                     c->rhs, c->res, c->Q[c->cur]);
+  return hipGetLastError();
+}
+
 hipError_t launch_update_wadg_mfma(bbw_ctx *c, float fa, float fb, float fdt, const float *ftd,
                                    float ft) {
   SrcArgs s;
@@ -349,6 +512,9 @@ hipError_t launch_update_wadg_mfma(bbw_ctx *c, float fa, float fb, float fdt, co
     case 4: return update_wadg_mfma_n<4>(c, fa, fb, fdt, ftd, ft, s);
     case 5: return update_wadg_mfma_n<5>(c, fa, fb, fdt, ftd, ft, s);
     case 6: return update_wadg_mfma_n<6>(c, fa, fb, fdt, ftd, ft, s);
+    case 7: return update_wadg_chunk_n<7>(c, fa, fb, fdt, ftd, ft, s);
+    case 8: return update_wadg_chunk_n<8>(c, fa, fb, fdt, ftd, ft, s);
+    case 9: return update_wadg_chunk_n<9>(c, fa, fb, fdt, ftd, ft, s);
   }
   return hipErrorInvalidValue;
 }

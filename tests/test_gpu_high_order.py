@@ -54,10 +54,14 @@ def test_surface_slice(N):
     s.close()
 
 
+@pytest.mark.parametrize("impl", [_lib.PLAIN_SLICE, _lib.PLAIN_FLAT])
 @pytest.mark.parametrize("N", [7, 8, 9])
-def test_flat_kernels_high_order(N):
-    print(f"case flat kernels N={N}", flush=True)
+def test_flat_kernels_high_order(N, impl):
+    """Volume and WADG update at orders 7..9: the default (chunked MFMA update) and the flat
+    per-element update."""
+    print(f"case high-order kernels N={N} impl={impl}", flush=True)
     P, s = make(N, 3, seed=N)
+    s.set_option(_lib.OPT_PLAIN_IMPL, impl)
     Q, R, res = P.random_state(), P.random_state(), P.random_state()
     s.WaveSetData3d(Q)
     s.volume()
