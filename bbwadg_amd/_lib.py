@@ -40,6 +40,7 @@ OPT_FUSED_IMPL, OPT_KBLK_V, OPT_KBLK_S, OPT_KBLK_U, OPT_KBLK_F, OPT_ADJ_CACHE_MB
 OPT_PLAIN_IMPL = 7
 OPT_SRC_STEP_TIME = 8
 OPT_PLANAR_SPLIT = 9
+OPT_REC_FIELD = 10
 FUSED_VECTOR, FUSED_MFMA = 0, 1
 PLAIN_FLAT, PLAIN_TILED, PLAIN_SLICE = 0, 1, 2
 

@@ -1,15 +1,19 @@
-"""Receiver-misfit gradients w.r.t. c^2 through the hand-written discrete adjoint.
+"""Receiver-misfit gradients w.r.t. the media through the hand-written discrete adjoint.
 
-No counterpart in the reference (its solver has no adjoint); the forward scheme is the
-reference's WADG LSRK45 path (``cpp/src/WaveOKL3d.cpp:2014-2109``).
+No counterpart in the reference (its solvers have no adjoint); the forward schemes are the
+reference's WADG LSRK45 paths (acoustic ``cpp/src/WaveOKL3d.cpp:2014-2109``, elastic
+``elas_cpp/src/WaveOKL3d.cpp:1660-1700``).
 
     inv = AcousticInversion(solver, shots, rec_points, nsteps, dt, ckpt_every=50)
     J, g = inv.misfit_and_gradient(c2q, d_obs)          # numpy, summed over shots
     J = c2_misfit(c2q_tensor, inv, d_obs)               # torch.autograd.Function, on the GPU
     J.backward(); c2q_tensor.grad                       # dJ/dc2q at the cubature points
 
-Each shot is a source (element ids, BB coefficients, Ricker f0, amplitude).  All shots
-share the receivers.  Every call goes to libbbwadg_hip.so (bbw_misfit,
+    inv = ElasticInversion(solver, shots, rec_points, nsteps, dt, rec_field=0)
+    J, g = inv.misfit_and_gradient((rhoinvq, lamq, muq), d_obs)   # g: (3, K, Nq)
+
+Each shot is a source (element ids, BB coefficients, Ricker f0, amplitude, field weights).
+All shots share the receivers.  Every call goes to libbbwadg_hip.so (bbw_misfit,
 bbw_adjoint_gradient); the gradient is that of the FP32 discrete scheme.
 """
 import ctypes
@@ -25,31 +29,37 @@ class Shot:
         self.elems, self.coef, self.f0, self.amp, self.weights = elems, coef, f0, amp, weights
 
 
-class AcousticInversion:
-    def __init__(self, solver, shots, rec_points, nsteps, dt, ckpt_every=50, t0=0.0):
-        if solver.nfields != 4:
-            raise ValueError("the adjoint is acoustic only")
+class _Inversion:
+    NFIELDS = 4
+    NPARAM = 1
+
+    def __init__(self, solver, shots, rec_points, nsteps, dt, ckpt_every=50, t0=0.0, rec_field=0):
+        if solver.nfields != self.NFIELDS:
+            raise ValueError(f"{type(self).__name__} needs a solver with {self.NFIELDS} fields")
         self.s = solver
         self.shots = list(shots)
         self.nsteps, self.dt, self.ckpt_every, self.t0 = int(nsteps), float(dt), int(ckpt_every), t0
         relems, B = media.receivers(solver.mesh, solver.ops, rec_points)
         self.rec_elems, self.rec_B = relems, B
+        self.rec_field = int(rec_field)
+        solver.set_option(_lib.OPT_REC_FIELD, self.rec_field)
         solver.set_receivers(relems, B, self.nsteps)
         mapP = i32(meshmod.build_mapP(solver.mesh, solver.ops))
         check(solver.ctx, solver.lib.bbw_upload_adjoint_maps(solver.ctx, ptr(mapP)))
-        self.Q0 = np.zeros((solver.K, 4, solver.ops.Np), dtype=np.float32)
+        self.Q0 = np.zeros((solver.K, solver.nfields, solver.ops.Np), dtype=np.float32)
         self._stream = None
+
+    def _grad_shape(self):
+        K, Nq = self.s.K, self.s.ops.Nq
+        return (K, Nq) if self.NPARAM == 1 else (self.NPARAM, K, Nq)
 
     def _set_shot(self, shot):
         self.s.set_source(shot.elems, shot.coef, shot.f0, shot.amp, shot.weights)
         self.s.WaveSetData3d(self.Q0)
 
-    def set_c2(self, c2q):
-        self.s.InitWADG_subelem(np.asarray(c2q, dtype=np.float32))
-
-    def observe(self, c2q):
-        """Synthetic observations: receiver traces (nshots, nsteps, nrec) for model c2q."""
-        self.set_c2(c2q)
+    def observe(self, model):
+        """Synthetic observations: receiver traces (nshots, nsteps, nrec) for the model."""
+        self.set_model(model)
         out = []
         for sh in self.shots:
             self._set_shot(sh)
@@ -58,8 +68,8 @@ class AcousticInversion:
             out.append(self.s.get_traces(self.nsteps))
         return np.stack(out)
 
-    def misfit(self, c2q, d_obs):
-        self.set_c2(c2q)
+    def misfit(self, model, d_obs):
+        self.set_model(model)
         J = 0.0
         for sh, d in zip(self.shots, d_obs):
             self._set_shot(sh)
@@ -70,11 +80,11 @@ class AcousticInversion:
             J += v.value
         return J
 
-    def misfit_and_gradient(self, c2q, d_obs):
-        self.set_c2(c2q)
+    def misfit_and_gradient(self, model, d_obs):
+        self.set_model(model)
         J = 0.0
-        g = np.zeros((self.s.K, self.s.ops.Nq), dtype=np.float64)
-        gs = np.zeros((self.s.K, self.s.ops.Nq), dtype=np.float32)
+        g = np.zeros(self._grad_shape(), dtype=np.float64)
+        gs = np.zeros(self._grad_shape(), dtype=np.float32)
         for sh, d in zip(self.shots, d_obs):
             self._set_shot(sh)
             v = ctypes.c_double()
@@ -86,6 +96,33 @@ class AcousticInversion:
             g += gs
         return J, g
 
+
+class ElasticInversion(_Inversion):
+    """Misfit of one receiver field (``rec_field``: 0..2 velocities, 3..8 stresses) and its
+    gradient in (1/rho, lambda, mu) at the cubature points, shape (3, K, Nq); the model is the
+    triple (rhoinvq, lamq, muq), the penalties those of the constructor."""
+    NFIELDS = 9
+    NPARAM = 3
+
+    def __init__(self, solver, shots, rec_points, nsteps, dt, ckpt_every=50, t0=0.0, rec_field=0,
+                 tau_v=1.0, tau_s=1.0):
+        super().__init__(solver, shots, rec_points, nsteps, dt, ckpt_every, t0, rec_field)
+        self.tau_v, self.tau_s = tau_v, tau_s
+
+    def set_model(self, model):
+        rhoinvq, lamq, muq = model
+        self.s.InitElastic(rhoinvq, lamq, muq, tau_v=self.tau_v, tau_s=self.tau_s)
+
+
+class AcousticInversion(_Inversion):
+    """Misfit of the pressure (or the field ``rec_field``) at the receivers and its gradient in c2
+    at the cubature points, shape (K, Nq)."""
+
+    def set_model(self, c2q):
+        self.s.InitWADG_subelem(np.asarray(c2q, dtype=np.float32))
+
+    def set_c2(self, c2q):
+        self.set_model(c2q)
 
     # -- device-resident path (torch tensors on the GPU) -------------------------------------
     def torch_stream(self):

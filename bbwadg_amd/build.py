@@ -11,7 +11,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libbbwadg_hip.so")
 SOURCES = ["bbw_abi.hip", "bbw_acoustic.hip", "bbw_fused.hip", "bbw_elastic.hip", "bbw_adjoint.hip", "bbw_batch.hip", "bbw_energy.hip",
-           "bbw_fused_elas.hip", "bbw_slice.hip", "bbw_curved.hip"]
+           "bbw_fused_elas.hip", "bbw_slice.hip", "bbw_curved.hip", "bbw_adjoint_elas.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [

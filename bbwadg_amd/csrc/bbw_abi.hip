@@ -1211,7 +1211,6 @@ int check_inverse(bbw_ctx *c, int nsteps, const float *d_obs) {
   if (rc || (rc = check_tiled_order(c, "the misfit / adjoint path")) ||
       (rc = check_not_curved(c, "the misfit / adjoint path")))
     return rc;
-  if (c->Nf != 4) return fail(c, BBW_ERR_UNSUPPORTED, "the adjoint is acoustic only");
   if (!d_obs || nsteps <= 0) return fail(c, BBW_ERR_ARG, "bad observations");
   if (c->nrec == 0 || c->rec_max_steps < nsteps) return fail(c, BBW_ERR_STATE, "receivers not set for nsteps");
   return BBW_OK;
@@ -1253,8 +1252,13 @@ int gradient_impl(bbw_ctx *c, int nsteps, double dt, double t0, const float *d_o
   int rc = check_inverse(c, nsteps, d_obs);
   if (rc) return rc;
   if (!c->mapP) return fail(c, BBW_ERR_STATE, "adjoint maps not uploaded");
-  const size_t n = nstate(c), nq = (size_t)c->K * c->Nq, nh = (size_t)c->K * 4 * c->Nfp;
-  const size_t ystep = 5 * (size_t)c->K * c->Np;  // floats of (A Q)_p per step (5 stages)
+  // acoustic: gradient in c2 [K][Nq], one face value per face node, (A Q)_p per stage;
+  // elastic: gradient in (1/rho, lambda, mu) [3][K][Nq], six face values, A Q of all 9 fields
+  const bool elas = c->Nf == 9;
+  const size_t n = nstate(c), nq = (size_t)c->K * c->Nq * (elas ? 3 : 1);
+  const size_t nh = (size_t)c->K * 4 * c->Nfp * (elas ? 6 : 1);
+  const size_t ystage = (size_t)c->K * c->Np * (elas ? 9 : 1);
+  const size_t ystep = 5 * ystage;  // floats of the saved stage values per step (5 stages)
   if ((rc = dalloc(c, &c->lQ, n)) || (rc = dalloc(c, &c->lres, n)) || (rc = dalloc(c, &c->Zadj, n)) ||
       (rc = dalloc(c, &c->Hadj, nh)) || (rc = dalloc(c, &c->grad, nq)))
     return rc;
@@ -1302,10 +1306,14 @@ int gradient_impl(bbw_ctx *c, int nsteps, double dt, double t0, const float *d_o
     for (int i = 0; i < 5; ++i) {
       // the residual left by the last stage is dead when a_0 = 0 (the state is restored and the
       // residual cleared after the gradient)
-      hipError_t e = launch_fused_mfma(c, c->Q[c->cur], c->Q[1 - c->cur], (float)c->rk4a[i],
-                                       (float)c->rk4b[i], (float)dt, nullptr, stage_value(step, i),
-                                       0, Y + (size_t)i * c->K * c->Np, !(i == 4 && a0_zero));
-      if (e != hipSuccess) return hipfail(c, e, "forward stage with Vq (A Q)_p capture");
+      hipError_t e =
+          elas ? launch_fused_elas(c, c->Q[c->cur], c->Q[1 - c->cur], (float)c->rk4a[i], (float)c->rk4b[i],
+                                   (float)dt, nullptr, stage_value(step, i), !(i == 4 && a0_zero),
+                                   Y + (size_t)i * ystage)
+               : launch_fused_mfma(c, c->Q[c->cur], c->Q[1 - c->cur], (float)c->rk4a[i],
+                                   (float)c->rk4b[i], (float)dt, nullptr, stage_value(step, i), 0,
+                                   Y + (size_t)i * ystage, !(i == 4 && a0_zero));
+      if (e != hipSuccess) return hipfail(c, e, "forward stage with A Q capture");
       c->cur = 1 - c->cur;
     }
     return BBW_OK;
@@ -1314,12 +1322,13 @@ int gradient_impl(bbw_ctx *c, int nsteps, double dt, double t0, const float *d_o
     hipError_t e = launch_adj_inject(c, c->resid + (size_t)step * c->nrec);
     if (e != hipSuccess) return hipfail(c, e, "receiver injection");
     for (int i = 4; i >= 0; --i) {
-      const float *Yi = Y + (size_t)i * c->K * c->Np;
+      const float *Yi = Y + (size_t)i * ystage;
       // with a_0 = 0 the reverse of stage 0 sets lres to 0 (dead store), so the reverse of stage 4,
       // which always follows it or the zeroed start, reads zeros
       const int lres_io = a0_zero ? (i == 4 ? 0 : 1) | (i == 0 ? 0 : 2) : 3;
-      e = c->adj_impl ? launch_adj_stage_tiled(c, Yi, (float)c->rk4a[i], (float)c->rk4b[i], (float)dt, lres_io)
-                      : launch_adj_stage(c, Yi, (float)c->rk4a[i], (float)c->rk4b[i], (float)dt);
+      e = elas ? launch_adj_elas_stage(c, Yi, (float)c->rk4a[i], (float)c->rk4b[i], (float)dt, lres_io)
+          : c->adj_impl ? launch_adj_stage_tiled(c, Yi, (float)c->rk4a[i], (float)c->rk4b[i], (float)dt, lres_io)
+                        : launch_adj_stage(c, Yi, (float)c->rk4a[i], (float)c->rk4b[i], (float)dt);
       if (e != hipSuccess) return hipfail(c, e, "adjoint stage");
     }
     return BBW_OK;
@@ -1408,6 +1417,11 @@ int bbw_set_option(bbw_ctx *c, int option, int value) {
       c->plain_impl = value;
       break;
     case BBW_OPT_SRC_STEP_TIME: c->src_step_time = value ? 1 : 0; break;
+    case BBW_OPT_REC_FIELD:
+      if (value < 0 || value >= c->Nf) return fail(c, BBW_ERR_ARG, "receiver field out of range");
+      c->rec_field = value;
+      reset_graph(c);
+      break;
     case BBW_OPT_PLANAR_SPLIT: c->planar_split = value ? 1 : 0; break;
     default: return fail(c, BBW_ERR_ARG, "unknown option");
   }

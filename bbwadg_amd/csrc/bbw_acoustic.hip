@@ -613,7 +613,7 @@ hipError_t launch_fused(bbw_ctx *c, const float *Qin, float *Qout, float fa, flo
 hipError_t launch_record(bbw_ctx *c, const float *Q, float *row) {
   const int grid = (c->nrec + 127) / 128;
   hipLaunchKernelGGL(k_record, dim3(grid), dim3(128), 0, c->stream, c->nrec, c->Np,
-                     c->Nf * c->Np, c->rec_elems, c->rec_basis, Q, row);
+                     c->Nf * c->Np, c->rec_elems, c->rec_basis, Q + (size_t)c->rec_field * c->Np, row);
   return hipGetLastError();
 }
 

@@ -249,7 +249,8 @@ hipError_t launch_adj_stage(bbw_ctx *c, const float *Y, float fa, float fb, floa
 hipError_t launch_adj_inject(bbw_ctx *c, const float *resid_row) {
   const int n = c->nrec * c->Np;
   hipLaunchKernelGGL(k_adj_inject, dim3((n + 255) / 256), dim3(256), 0, c->stream, c->nrec, c->Np,
-                     c->Nf * c->Np, c->rec_elems, c->rec_basis, resid_row, c->lQ);
+                     c->Nf * c->Np, c->rec_elems, c->rec_basis, resid_row,
+                     c->lQ + (size_t)c->rec_field * c->Np);
   return hipGetLastError();
 }
 

@@ -75,10 +75,13 @@ struct ElasMedia {
   float tau_v, tau_s;
 };
 
-template <int N>
+// CAP: also write the right-hand side A Q_in (before the WADG projection, without the source)
+// to yout [K][9][Np], the stage value the discrete adjoint's media gradient needs.
+template <int N, bool CAP>
 __global__ __launch_bounds__(256, N <= 4 ? 3 : N == 5 ? 2 : 1) void k_fused_elas(FusedMesh m, FusedOps o, ElasMedia md, const float *Qin,
                                                         float *Qout, float *res, SrcArgs src, float fa, float fb,
-                                                        float fdt, float ft, const float *ftime_dev, int store_res) {
+                                                        float fdt, float ft, const float *ftime_dev, int store_res,
+                                                        float *yout) {
   using L = ElasFusedLayout<N>;
   constexpr int Np = L::Np, Nfp = L::Nfp, NfpNf = L::NfpNf, Nq = L::Nq, TB = L::TB, RowF = L::RowF;
   constexpr int RG = L::RG, KB = L::KB, BS = L::BS, NQP = L::NQP, NPT = L::NPT, Np1 = L::Np1;
@@ -437,6 +440,13 @@ __global__ __launch_bounds__(256, N <= 4 ? 3 : N == 5 ? 2 : 1) void k_fused_elas
       }
     }
     __syncthreads();
+    if constexpr (CAP) {
+      const brsrc rY = mkrsrc(yout + (size_t)k0 * RowF, tbytes);
+      for (int i = tid; i < TB * 3 * Np; i += 256) {
+        const int e = i / (3 * Np), r = i - e * (3 * Np);
+        bst(rY, sR[e * RG + r], (unsigned)(e * RowF + 3 * g * Np + r) * 4u);
+      }
+    }
 
     // ---- 6: S_b = map_g(Vq rhs_{3g+b}) (M = cubature points, N = elements, K = nodes)
     {
@@ -548,26 +558,33 @@ __global__ __launch_bounds__(256, N <= 4 ? 3 : N == 5 ? 2 : 1) void k_fused_elas
   }
 }
 
-template <int N>
+template <int N, bool CAP>
 static hipError_t fused_elas_n(bbw_ctx *c, const float *Qin, float *Qout, float fa, float fb, float fdt,
-                               const float *ftd, float ft, const SrcArgs &s, int store_res) {
+                               const float *ftd, float ft, const SrcArgs &s, int store_res, float *yout) {
   using L = ElasFusedLayout<N>;
   const size_t sh = sizeof(float) * L::floats;
   if ((unsigned long long)c->K * 36 * L::Np >= (1ull << 32)) return hipErrorInvalidValue;
   if (sh > 160 * 1024) return hipErrorInvalidValue;
   if (sh > 65536) {
-    hipError_t e = hipFuncSetAttribute((const void *)k_fused_elas<N>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh);
+    hipError_t e = hipFuncSetAttribute((const void *)k_fused_elas<N, CAP>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh);
     if (e != hipSuccess) return e;
   }
   FusedMesh m{c->K, c->vgeo, c->fgeo, c->vmapP};
   ElasMedia md{c->rhoinv, c->lambda, c->mu, c->tau_v, c->tau_s};
-  hipLaunchKernelGGL(k_fused_elas<N>, dim3((c->K + L::TB - 1) / L::TB), dim3(256), sh, c->stream, m, fused_ops(c), md,
-                     Qin, Qout, c->res, s, fa, fb, fdt, ft, ftd, store_res);
+  hipLaunchKernelGGL((k_fused_elas<N, CAP>), dim3((c->K + L::TB - 1) / L::TB), dim3(256), sh, c->stream, m, fused_ops(c), md,
+                     Qin, Qout, c->res, s, fa, fb, fdt, ft, ftd, store_res, yout);
   return hipGetLastError();
 }
 
+template <int N>
+static hipError_t fused_elas_any(bbw_ctx *c, const float *Qin, float *Qout, float fa, float fb, float fdt,
+                                 const float *ftd, float ft, const SrcArgs &s, int store_res, float *yout) {
+  return yout ? fused_elas_n<N, true>(c, Qin, Qout, fa, fb, fdt, ftd, ft, s, store_res, yout)
+              : fused_elas_n<N, false>(c, Qin, Qout, fa, fb, fdt, ftd, ft, s, store_res, nullptr);
+}
+
 hipError_t launch_fused_elas(bbw_ctx *c, const float *Qin, float *Qout, float fa, float fb, float fdt,
-                             const float *ftd, float ft, int store_res) {
+                             const float *ftd, float ft, int store_res, float *yout) {
   SrcArgs s;
   s.nsrc = c->nsrc;
   s.first = c->src_first;
@@ -576,12 +593,12 @@ hipError_t launch_fused_elas(bbw_ctx *c, const float *Qin, float *Qout, float fa
   s.coef = c->src_coef;
   for (int f = 0; f < 9; ++f) s.w[f] = c->src_w[f];
   switch (c->N) {
-    case 1: return fused_elas_n<1>(c, Qin, Qout, fa, fb, fdt, ftd, ft, s, store_res);
-    case 2: return fused_elas_n<2>(c, Qin, Qout, fa, fb, fdt, ftd, ft, s, store_res);
-    case 3: return fused_elas_n<3>(c, Qin, Qout, fa, fb, fdt, ftd, ft, s, store_res);
-    case 4: return fused_elas_n<4>(c, Qin, Qout, fa, fb, fdt, ftd, ft, s, store_res);
-    case 5: return fused_elas_n<5>(c, Qin, Qout, fa, fb, fdt, ftd, ft, s, store_res);
-    case 6: return fused_elas_n<6>(c, Qin, Qout, fa, fb, fdt, ftd, ft, s, store_res);
+    case 1: return fused_elas_any<1>(c, Qin, Qout, fa, fb, fdt, ftd, ft, s, store_res, yout);
+    case 2: return fused_elas_any<2>(c, Qin, Qout, fa, fb, fdt, ftd, ft, s, store_res, yout);
+    case 3: return fused_elas_any<3>(c, Qin, Qout, fa, fb, fdt, ftd, ft, s, store_res, yout);
+    case 4: return fused_elas_any<4>(c, Qin, Qout, fa, fb, fdt, ftd, ft, s, store_res, yout);
+    case 5: return fused_elas_any<5>(c, Qin, Qout, fa, fb, fdt, ftd, ft, s, store_res, yout);
+    case 6: return fused_elas_any<6>(c, Qin, Qout, fa, fb, fdt, ftd, ft, s, store_res, yout);
   }
   return hipErrorInvalidValue;
 }

@@ -143,6 +143,7 @@ struct bbw_ctx {
 
   // receivers
   int nrec = 0, rec_max_steps = 0, rec_steps = 0;
+  int rec_field = 0;   // the field the receivers sample
   int *rec_elems = nullptr;
   float *rec_basis = nullptr, *rec_trace = nullptr;
 
@@ -212,9 +213,12 @@ hipError_t launch_fused(bbw_ctx *c, const float *Qin, float *Qout, float fa, flo
 hipError_t launch_fused_mfma(bbw_ctx *c, const float *Qin, float *Qout, float fa, float fb,
                              float fdt, const float *ftime_dev, float ftime, int project_velocity,
                              float *yout = nullptr, int store_res = 1);
-// fused elastic stage: Qin -> Qout, res updated (stored unless store_res = 0)
+// fused elastic stage: Qin -> Qout, res updated (stored unless store_res = 0); with yout, also
+// A Q_in [K][9][Np] (the right-hand side before the WADG projection) for the adjoint
 hipError_t launch_fused_elas(bbw_ctx *c, const float *Qin, float *Qout, float fa, float fb, float fdt,
-                             const float *ftime_dev, float ftime, int store_res = 1);
+                             const float *ftime_dev, float ftime, int store_res = 1, float *yout = nullptr);
+// reverse elastic stage (two kernels): see bbw_adjoint_elas.hip
+hipError_t launch_adj_elas_stage(bbw_ctx *c, const float *Y, float fa, float fb, float fdt, int lres_io = 3);
 hipError_t launch_adj_stage(bbw_ctx *c, const float *Y, float fa, float fb, float fdt);
 // lres_io bit 0: read the adjoint residual (else it is zero); bit 1: store it (else it is dead)
 hipError_t launch_adj_stage_tiled(bbw_ctx *c, const float *Y, float fa, float fb, float fdt, int lres_io = 3);

@@ -232,11 +232,14 @@ int bbw_energy_wadg(bbw_ctx *ctx, double *E);
  * GPU's array: calls that overlap on one GPU (on different streams or threads) race. */
 int bbw_energy_noalloc(bbw_ctx *ctx, double *E);
 
-/* Discrete adjoint (no reference counterpart; acoustic, fused forward with Pq Vq = I on the
- * velocity).  Misfit J = 1/2 sum_n sum_r (p_r(Q_{n+1}) - d_obs[n][r])^2 over nsteps steps from the
- * current state, receivers as set by bbw_set_receivers; gradient dJ/dc2q [K][Nq] by the
- * reverse LSRK45 stages with the state checkpointed every ckpt_every steps and each segment
- * recomputed.  Both calls leave the context's state as it was. */
+/* Discrete adjoint (no reference counterpart) of the fused forward stage, both physics.
+ * Misfit J = 1/2 sum_n sum_r (u_r(Q_{n+1}) - d_obs[n][r])^2 over nsteps steps from the current
+ * state, u_r the receiver sample (bbw_set_receivers) of the field BBW_OPT_REC_FIELD selects
+ * (default 0: p, or v1 for the elastic fields).  Gradient by the reverse LSRK45 stages with the
+ * state checkpointed every ckpt_every steps and each segment recomputed: acoustic (velocity
+ * not projected, Pq Vq = I) dJ/dc2q [K][Nq]; elastic [3][K][Nq], dJ/d(1/rho), dJ/dlambda, dJ/dmu
+ * at the cubature points (the media of bbw_upload_elastic_media).  Both calls leave the
+ * context's state as it was. */
 int bbw_upload_adjoint_maps(bbw_ctx *ctx, const int32_t *mapP); /* [K][4Nfp] face-node map */
 int bbw_misfit(bbw_ctx *ctx, int nsteps, double dt, double t0, const float *d_obs, double *J);
 int bbw_adjoint_gradient(bbw_ctx *ctx, int nsteps, double dt, double t0, const float *d_obs,
@@ -282,7 +285,7 @@ double bbw_batch_last_ms(const bbw_batch *b); /* device time of the last bbw_bat
  * pass, so those steps are not recomputed (0 = automatic: up to 60% of free memory, < 0 = none;
  * earlier steps are checkpointed and recomputed) */
 #define BBW_OPT_ADJ_CACHE_MB 5
-#define BBW_OPT_ADJ_IMPL 6 /* adjoint kernels: 0 per-element, 1 tiled MFMA (default) */
+#define BBW_OPT_ADJ_IMPL 6 /* acoustic adjoint kernels: 0 per-element, 1 tiled MFMA (default) */
 #define BBW_OPT_PLAIN_IMPL 7 /* plain kernels (bbw_volume, bbw_surface, bbw_update_wadg): */
 #define BBW_PLAIN_FLAT 0     /* per-element loops over Kblk* elements per block */
 #define BBW_PLAIN_TILED 1    /* default for N <= 6: 16-element tiles, WADG projections on MFMA */
@@ -296,6 +299,7 @@ double bbw_batch_last_ms(const bbw_batch *b); /* device time of the last bbw_bat
 #define BBW_OPT_PLANAR_SPLIT 9 /* curvilinear context: 1 runs the planar elements through the split
                                  form with constant factors; 0 (default) the strong nodal form of
                                  rk_volume_planar / rk_surface_planar */
+#define BBW_OPT_REC_FIELD 10  /* the field the receivers sample (0 .. Nfields-1; default 0) */
 int bbw_set_option(bbw_ctx *ctx, int option, int value);
 
 /* Device addresses of the context's buffers (for copies into caller-owned device memory):
