@@ -29,6 +29,7 @@ EXPORTS = [
     "bbw_set_res", "bbw_get_res", "bbw_set_rhs", "bbw_get_rhs", "bbw_volume", "bbw_surface",
     "bbw_update_wadg", "bbw_update", "bbw_stage_fused", "bbw_step", "bbw_energy", "bbw_energy_wadg",
     "bbw_energy_noalloc", "bbw_upload_media_device", "bbw_misfit_device", "bbw_adjoint_gradient_device",
+    "bbw_upload_elastic_media_device",
     "bbw_device_ptr", "bbw_set_option", "bbw_upload_adjoint_maps", "bbw_misfit",
     "bbw_adjoint_gradient", "bbw_batch_create", "bbw_batch_destroy", "bbw_batch_last_error",
     "bbw_batch_upload_operators", "bbw_batch_add_mesh", "bbw_batch_add_shot", "bbw_batch_run",
@@ -133,6 +134,7 @@ def load():
         "bbw_misfit_device": ([P, I, D, D, P, ctypes.POINTER(D)], I),
         "bbw_adjoint_gradient_device": ([P, I, D, D, P, I, P, ctypes.POINTER(D)], I),
         "bbw_upload_media_device": ([P, P], I),
+        "bbw_upload_elastic_media_device": ([P, P, P, P, F, F], I),
         "bbw_adjoint_gradient": ([P, I, D, D, P, I, P, ctypes.POINTER(D)], I),
         "bbw_batch_create": ([I, I, ctypes.POINTER(P)], I),
         "bbw_batch_destroy": ([P], None),

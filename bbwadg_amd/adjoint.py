@@ -2,7 +2,7 @@
 
 No counterpart in the reference (its solvers have no adjoint); the forward schemes are the
 reference's WADG LSRK45 paths (acoustic ``cpp/src/WaveOKL3d.cpp:2014-2109``, elastic
-``elas_cpp/src/WaveOKL3d.cpp:1660-1700``).
+``elas_cpp/src/WaveOKL3d.cpp:1605-1693``).
 
     inv = AcousticInversion(solver, shots, rec_points, nsteps, dt, ckpt_every=50)
     J, g = inv.misfit_and_gradient(c2q, d_obs)          # numpy, summed over shots
@@ -97,33 +97,6 @@ class _Inversion:
         return J, g
 
 
-class ElasticInversion(_Inversion):
-    """Misfit of one receiver field (``rec_field``: 0..2 velocities, 3..8 stresses) and its
-    gradient in (1/rho, lambda, mu) at the cubature points, shape (3, K, Nq); the model is the
-    triple (rhoinvq, lamq, muq), the penalties those of the constructor."""
-    NFIELDS = 9
-    NPARAM = 3
-
-    def __init__(self, solver, shots, rec_points, nsteps, dt, ckpt_every=50, t0=0.0, rec_field=0,
-                 tau_v=1.0, tau_s=1.0):
-        super().__init__(solver, shots, rec_points, nsteps, dt, ckpt_every, t0, rec_field)
-        self.tau_v, self.tau_s = tau_v, tau_s
-
-    def set_model(self, model):
-        rhoinvq, lamq, muq = model
-        self.s.InitElastic(rhoinvq, lamq, muq, tau_v=self.tau_v, tau_s=self.tau_s)
-
-
-class AcousticInversion(_Inversion):
-    """Misfit of the pressure (or the field ``rec_field``) at the receivers and its gradient in c2
-    at the cubature points, shape (K, Nq)."""
-
-    def set_model(self, c2q):
-        self.s.InitWADG_subelem(np.asarray(c2q, dtype=np.float32))
-
-    def set_c2(self, c2q):
-        self.set_model(c2q)
-
     # -- device-resident path (torch tensors on the GPU) -------------------------------------
     def torch_stream(self):
         """The torch stream (default priority) the solver's work goes to on the tensor path."""
@@ -133,20 +106,23 @@ class AcousticInversion(_Inversion):
             self.s.set_stream(self._stream.cuda_stream)
         return self._stream
 
-    def misfit_and_gradient_device(self, c2q, d_obs, need_grad=True):
-        """c2q: (K, Nq) float32 CUDA tensor; d_obs: (nshots, nsteps, nrec) float32 CUDA tensor.
-        Returns (J, grad) with grad a (K, Nq) float32 CUDA tensor (None when need_grad is false:
-        then only the forward misfit runs).  Everything is enqueued on ``torch_stream()``, which
-        the caller orders against its own stream; J is reduced in FP64 on the device."""
+    def misfit_and_gradient_device(self, model, d_obs, need_grad=True):
+        """model: float32 CUDA tensor of the gradient's shape ((K, Nq) c2, or (3, K, Nq)
+        1/rho, lambda, mu); d_obs: (nshots, nsteps, nrec) float32 CUDA tensor.  Returns (J, grad)
+        with grad a CUDA tensor shaped like the model (None when need_grad is false: then only
+        the forward misfit runs).  Everything is enqueued on ``torch_stream()``, which the caller
+        orders against its own stream; J is reduced in FP64 on the device."""
         import torch
         lib, ctx = self.s.lib, self.s.ctx
         st = self.torch_stream()
         with torch.cuda.stream(st):
-            c2q = c2q.detach().to(torch.float32).contiguous()
+            model = model.detach().to(torch.float32).contiguous()
+            if tuple(model.shape) != self._grad_shape():
+                raise ValueError(f"model shape {tuple(model.shape)}, expected {self._grad_shape()}")
             d_obs = d_obs.detach().to(torch.float32).contiguous()
-            check(ctx, lib.bbw_upload_media_device(ctx, ctypes.c_void_p(c2q.data_ptr())))
-            g = torch.zeros_like(c2q) if need_grad else None
-            gs = torch.empty_like(c2q) if need_grad else None
+            self._upload_model_device(model)
+            g = torch.zeros_like(model) if need_grad else None
+            gs = torch.empty_like(model) if need_grad else None
             J = 0.0
             for sh, d in zip(self.shots, d_obs):
                 self.s.set_source(sh.elems, sh.coef, sh.f0, sh.amp, sh.weights)
@@ -164,46 +140,94 @@ class AcousticInversion(_Inversion):
         return J, g
 
 
+class ElasticInversion(_Inversion):
+    """Misfit of one receiver field (``rec_field``: 0..2 velocities, 3..8 stresses) and its
+    gradient in (1/rho, lambda, mu) at the cubature points, shape (3, K, Nq); the model is the
+    triple (rhoinvq, lamq, muq) (or one (3, K, Nq) array), the penalties those of the
+    constructor."""
+    NFIELDS = 9
+    NPARAM = 3
+
+    def __init__(self, solver, shots, rec_points, nsteps, dt, ckpt_every=50, t0=0.0, rec_field=0,
+                 tau_v=1.0, tau_s=1.0):
+        super().__init__(solver, shots, rec_points, nsteps, dt, ckpt_every, t0, rec_field)
+        self.tau_v, self.tau_s = tau_v, tau_s
+
+    def set_model(self, model):
+        rhoinvq, lamq, muq = model
+        self.s.InitElastic(rhoinvq, lamq, muq, tau_v=self.tau_v, tau_s=self.tau_s)
+
+    def _upload_model_device(self, m):
+        lib, ctx = self.s.lib, self.s.ctx
+        check(ctx, lib.bbw_upload_elastic_media_device(ctx, *(ctypes.c_void_p(m[i].data_ptr()) for i in range(3)),
+                                                       float(self.tau_v), float(self.tau_s)))
+
+
+class AcousticInversion(_Inversion):
+    """Misfit of the pressure (or the field ``rec_field``) at the receivers and its gradient in c2
+    at the cubature points, shape (K, Nq)."""
+
+    def set_model(self, c2q):
+        self.s.InitWADG_subelem(np.asarray(c2q, dtype=np.float32))
+
+    def set_c2(self, c2q):
+        self.set_model(c2q)
+
+    def _upload_model_device(self, c2q):
+        check(self.s.ctx, self.s.lib.bbw_upload_media_device(self.s.ctx, ctypes.c_void_p(c2q.data_ptr())))
+
+
 def _torch_function():
     import torch
 
-    class C2Misfit(torch.autograd.Function):
-        """J(c2q) = sum over shots of 1/2 ||receiver traces - d_obs||^2 for a CUDA tensor c2q;
-        backward returns the adjoint gradient, computed with the forward (checkpoint and
-        recompute inside) only when c2q needs a gradient.  c2q, the observations and the
-        gradient stay on the GPU; the work is ordered after, and its results before, the
-        caller's current stream."""
+    class ModelMisfit(torch.autograd.Function):
+        """J(m) = sum over shots of 1/2 ||receiver traces - d_obs||^2 for a CUDA tensor m (c2 at
+        the cubature points, or 1/rho, lambda, mu); backward returns the adjoint gradient,
+        computed with the forward (checkpoint and recompute inside) only when m needs a
+        gradient.  m, the observations and the gradient stay on the GPU; the work is ordered
+        after, and its results before, the caller's current stream."""
 
         @staticmethod
-        def forward(ctx, c2q, inv, d_obs):
-            if not c2q.is_cuda:
-                raise ValueError("c2_misfit takes c2q on the GPU")
+        def forward(ctx, m, inv, d_obs):
+            if not m.is_cuda:
+                raise ValueError("the misfit takes the model on the GPU")
             need = bool(ctx.needs_input_grad[0])
-            cur = torch.cuda.current_stream(c2q.device)
-            d = torch.as_tensor(d_obs, device=c2q.device)   # on the caller's stream
+            cur = torch.cuda.current_stream(m.device)
+            d = torch.as_tensor(d_obs, device=m.device)   # on the caller's stream
             st = inv.torch_stream()
             st.wait_stream(cur)
-            J, g = inv.misfit_and_gradient_device(c2q, d, need)
+            J, g = inv.misfit_and_gradient_device(m, d, need)
             cur.wait_stream(st)
             if need:
                 g.record_stream(cur)
-                ctx.save_for_backward(g.to(c2q.dtype))
-            return torch.tensor(J, dtype=torch.float64, device=c2q.device)
+                ctx.save_for_backward(g.to(m.dtype))
+            return torch.tensor(J, dtype=torch.float64, device=m.device)
 
         @staticmethod
         def backward(ctx, gJ):
             (g,) = ctx.saved_tensors
             return (gJ.to(g.dtype) * g, None, None)
 
-    return C2Misfit
+    return ModelMisfit
 
 
-_C2MISFIT = None
+_MISFIT = None
+
+
+def model_misfit(m, inv, d_obs):
+    """torch entry point: ``model_misfit(m_tensor, inv, d_obs)`` -> scalar tensor with grad, for
+    an AcousticInversion (m: (K, Nq) c2) or an ElasticInversion (m: (3, K, Nq))."""
+    global _MISFIT
+    if _MISFIT is None:
+        _MISFIT = _torch_function()
+    return _MISFIT.apply(m, inv, d_obs)
 
 
 def c2_misfit(c2q, inv, d_obs):
     """torch entry point: ``c2_misfit(c2q_tensor, inv, d_obs)`` -> scalar tensor with grad."""
-    global _C2MISFIT
-    if _C2MISFIT is None:
-        _C2MISFIT = _torch_function()
-    return _C2MISFIT.apply(c2q, inv, d_obs)
+    return model_misfit(c2q, inv, d_obs)
+
+
+def elastic_misfit(m, inv, d_obs):
+    """torch entry point for an ElasticInversion: m (3, K, Nq) = (1/rho, lambda, mu) on the GPU."""
+    return model_misfit(m, inv, d_obs)

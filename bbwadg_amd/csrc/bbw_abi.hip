@@ -759,6 +759,25 @@ int bbw_upload_elastic_media(bbw_ctx *c, const float *rhoinvq, const float *lamb
   return BBW_OK;
 }
 
+int bbw_upload_elastic_media_device(bbw_ctx *c, const float *rhoinvq_dev, const float *lambdaq_dev,
+                                    const float *muq_dev, float tau_v, float tau_s) {
+  if (!c || !rhoinvq_dev || !lambdaq_dev || !muq_dev) return BBW_ERR_ARG;
+  if (c->Nf != 9) return fail(c, BBW_ERR_ARG, "elastic media on an acoustic context");
+  const size_t n = (size_t)c->K * c->Nq;
+  int rc;
+  if ((rc = dalloc(c, &c->rhoinv, n)) || (rc = dalloc(c, &c->lambda, n)) || (rc = dalloc(c, &c->mu, n)))
+    return rc;
+  HIPCHK(c, hipMemcpyAsync(c->rhoinv, rhoinvq_dev, n * sizeof(float), hipMemcpyDeviceToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(c->lambda, lambdaq_dev, n * sizeof(float), hipMemcpyDeviceToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(c->mu, muq_dev, n * sizeof(float), hipMemcpyDeviceToDevice, c->stream));
+  // the penalties are kernel arguments: captured graphs hold the old ones
+  if (tau_v != c->tau_v || tau_s != c->tau_s) reset_graph(c);
+  c->tau_v = tau_v;
+  c->tau_s = tau_s;
+  c->have_media = true;
+  return BBW_OK;
+}
+
 int bbw_set_source(bbw_ctx *c, int nsrc, const int32_t *elems, const float *coef,
                    const float *weights, double f0, double amp) {
   if (!c || nsrc < 0) return BBW_ERR_ARG;

@@ -8,8 +8,9 @@ FP32.  A step is one LSRK45 step (5 stages) over the whole mesh, inputs resident
 Reported: the fused path (one kernel per stage, HIP-graph replay) as ``value``; the plain
 three-launch path of the reference (volume, surface, WADG update) beside it; the
 roofline of the fused stage kernel; the adjoint-gradient / forward ratio; the batched
-multi-shot run; the elastic configuration's stage rate; the curvilinear run of the reference's
-sphere mesh (49 748 tets, N = 5); the CPU oracle timed on a bounded sample.
+multi-shot run; the elastic configuration's stage rate and its adjoint-gradient / forward
+ratio; the curvilinear run of the reference's sphere mesh (49 748 tets, N = 5); the CPU oracle
+timed on a bounded sample.
 
     python bench.py [--steps K] [--warmup W] [--gpus 1] [--mesh 48] [--N 5]
 """
@@ -139,6 +140,45 @@ def adjoint_ratio(nsteps, mesh_n=32, N=5, ckpt=50):
     s.close()
     return {"ratio": tg / tf, "forward_s": tf, "gradient_s": tg, "steps": nsteps, "K": m.K,
             "ckpt_every": ckpt, "misfit": Jf, "grad_norm": float(np.linalg.norm(g)),
+            "misfit_agree": abs(Jf - Jg) / max(Jf, 1e-300)}
+
+
+def elastic_adjoint_ratio(nsteps, mesh_n=32, N=4, ckpt=50):
+    """The elastic counterpart of adjoint_ratio on the elastic configuration's shape (32^3 x 6
+    tets, N = 4, 9 fields): gradient in (1/rho, lambda, mu) of the misfit of v_z at 256 surface
+    receivers, start model the means of the seeded Lame/density fields, one explosive
+    moment-tensor Ricker shot (f0 = 40), the reference's elastic time step."""
+    from bbwadg_amd import media, mesh as M, operators as O
+    from bbwadg_amd.adjoint import ElasticInversion, Shot
+    from bbwadg_amd.solver import WaveSolver
+    ops = O.build_ref_ops(N)
+    m = M.cube_mesh(mesh_n)
+    M.face_pair(m)
+    dm = M.device_arrays(m, ops, 9)
+    s = WaveSolver(m, N, nfields=9, ops=ops, dmesh=dm)
+    xq, yq, zq = media.cubature_points(m, ops)
+    lam, mu, rho = media.lame_fourier(1)
+    m_true = (1.0 / rho(xq, yq, zq), lam(xq, yq, zq), mu(xq, yq, zq))
+    s.InitElastic(*m_true)
+    # the shot 0.05 below the receiver plane, so that the 200-step window records its arrivals
+    el, co = media.gaussian_source(m, ops, (0.05, -0.08, 0.45), a=100.0)
+    w = np.array([0, 0, 0, 1, 1, 1, 0, 0, 0], dtype=np.float64)
+    inv = ElasticInversion(s, [Shot(el, co, f0=40.0, amp=1.0, weights=w)],
+                           media.surface_receivers(4, 256), nsteps, s.dt_ref, ckpt_every=ckpt,
+                           rec_field=2)
+    d = inv.observe(m_true)
+    m0 = tuple(np.full_like(a, float(np.mean(a))) for a in m_true)
+    inv.misfit(m0, d)  # warm-up (graph capture, allocations)
+    inv.misfit_and_gradient(m0, d)
+    t0 = time.perf_counter()
+    Jf = inv.misfit(m0, d)
+    tf = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    Jg, g = inv.misfit_and_gradient(m0, d)
+    tg = time.perf_counter() - t0
+    s.close()
+    return {"ratio": tg / tf, "forward_s": tf, "gradient_s": tg, "steps": nsteps, "K": m.K, "N": N,
+            "ckpt_every": ckpt, "misfit": Jf, "grad_norm": [float(np.linalg.norm(x)) for x in g],
             "misfit_agree": abs(Jf - Jg) / max(Jf, 1e-300)}
 
 
@@ -319,6 +359,8 @@ def main():
                     help="steps of the curvilinear leg (0 skips it)")
     ap.add_argument("--elastic-steps", type=int, default=50,
                     help="steps of the elastic configuration (0 skips it)")
+    ap.add_argument("--elastic-adjoint-steps", type=int, default=200,
+                    help="steps of the elastic adjoint leg (0 skips it)")
     ap.add_argument("--batched-scale", type=float, default=1.0,
                     help="duration scale of the batched configuration (0 skips it)")
     args = ap.parse_args()
@@ -392,6 +434,7 @@ def main():
     s.close()
     adj = adjoint_ratio(args.adjoint_steps) if args.adjoint_steps > 0 else None
     elas = elastic_throughput(args.elastic_steps) if args.elastic_steps > 0 else None
+    eadj = elastic_adjoint_ratio(args.elastic_adjoint_steps) if args.elastic_adjoint_steps > 0 else None
     curv = curved_throughput(args.curved_steps) if args.curved_steps > 0 else None
     bat = batched_throughput(args.batched_scale) if args.batched_scale > 0 else None
 
@@ -431,6 +474,8 @@ def main():
         "adjoint": adj,
         "batched": bat,
         "elastic": elas,
+        "elastic_adjoint_over_forward": eadj["ratio"] if eadj else None,
+        "elastic_adjoint": eadj,
         "curved": curv,
         "energy_after": E,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",

@@ -114,6 +114,9 @@ int bbw_upload_media_device(bbw_ctx *ctx, const float *c2q_dev);
  * multiplies), lambda, mu, each [K][Nq]; penalties tau_v, tau_s (:516-517). */
 int bbw_upload_elastic_media(bbw_ctx *ctx, const float *rhoinvq, const float *lambdaq,
                              const float *muq, float tau_v, float tau_s);
+/* The same from device memory (no reference counterpart), copied on the context's stream. */
+int bbw_upload_elastic_media_device(bbw_ctx *ctx, const float *rhoinvq_dev, const float *lambdaq_dev,
+                                    const float *muq_dev, float tau_v, float tau_s);
 /* Point-like source: nsrc elements (ascending ids), BB coefficients coef[nsrc][Np] added to
  * the right-hand side of each field f as amp * w[f] * ricker(t, f0) * coef (elas_cpp fsrc/ftime,
  * :487-538, :1677-1680).  weights: Nfields floats.  nsrc = 0 removes the source. */

@@ -1,7 +1,7 @@
 """FP64 host adjoint of the elastic BB-WADG LSRK45 scheme -- TEST INFRASTRUCTURE ONLY.
 
 Discrete adjoint of the elastic forward scheme restated in ``oracle.py`` (``rk_run_elas``: the
-stage of ``elas_cpp/src/WaveOKL3d.cpp:1660-1700`` with the kernels of
+stage of ``elas_cpp/src/WaveOKL3d.cpp:1605-1693`` with the kernels of
 ``elas_cpp/okl/ElasKernelsWADG.okl:25-433``), fields v1 v2 v3 sxx syy szz syz sxz sxy:
 
     stage i:  r = W(m) A Q + f_i s;  res' = a_i res + dt r;  Q' = Q + b_i res'
