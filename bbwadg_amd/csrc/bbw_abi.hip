@@ -1272,13 +1272,14 @@ int gradient_impl(bbw_ctx *c, int nsteps, double dt, double t0, const float *d_o
   if (rc) return rc;
   if (!c->mapP) return fail(c, BBW_ERR_STATE, "adjoint maps not uploaded");
   // acoustic: gradient in c2 [K][Nq], one face value per face node, (A Q)_p per stage;
-  // elastic: gradient in (1/rho, lambda, mu) [3][K][Nq], six face values, A Q of all 9 fields
+  // elastic: gradient in (1/rho, lambda, mu) [3][K][Nq], six face values per face node in two
+  // buffers (the stage before's and this stage's), A Q of all 9 fields
   const bool elas = c->Nf == 9;
   const size_t n = nstate(c), nq = (size_t)c->K * c->Nq * (elas ? 3 : 1);
-  const size_t nh = (size_t)c->K * 4 * c->Nfp * (elas ? 6 : 1);
+  const size_t nh1 = (size_t)c->K * 4 * c->Nfp * (elas ? 6 : 1), nh = elas ? 2 * nh1 : nh1;
   const size_t ystage = (size_t)c->K * c->Np * (elas ? 9 : 1);
   const size_t ystep = 5 * ystage;  // floats of the saved stage values per step (5 stages)
-  if ((rc = dalloc(c, &c->lQ, n)) || (rc = dalloc(c, &c->lres, n)) || (rc = dalloc(c, &c->Zadj, n)) ||
+  if ((rc = dalloc(c, &c->lQ, n)) || (rc = dalloc(c, &c->lres, n)) || (!elas && (rc = dalloc(c, &c->Zadj, n))) ||
       (rc = dalloc(c, &c->Hadj, nh)) || (rc = dalloc(c, &c->grad, nq)))
     return rc;
   if (c->resid_len < (size_t)nsteps * c->nrec) { dfree(c->resid); c->resid_len = 0; }
@@ -1337,6 +1338,7 @@ int gradient_impl(bbw_ctx *c, int nsteps, double dt, double t0, const float *d_o
     }
     return BBW_OK;
   };
+  int hsel = 0;   // elastic: the H buffer the next reverse stage reads
   auto adj_step = [&](int step, const float *Y) -> int {
     hipError_t e = launch_adj_inject(c, c->resid + (size_t)step * c->nrec);
     if (e != hipSuccess) return hipfail(c, e, "receiver injection");
@@ -1345,9 +1347,14 @@ int gradient_impl(bbw_ctx *c, int nsteps, double dt, double t0, const float *d_o
       // with a_0 = 0 the reverse of stage 0 sets lres to 0 (dead store), so the reverse of stage 4,
       // which always follows it or the zeroed start, reads zeros
       const int lres_io = a0_zero ? (i == 4 ? 0 : 1) | (i == 0 ? 0 : 2) : 3;
-      e = elas ? launch_adj_elas_stage(c, Yi, (float)c->rk4a[i], (float)c->rk4b[i], (float)dt, lres_io)
-          : c->adj_impl ? launch_adj_stage_tiled(c, Yi, (float)c->rk4a[i], (float)c->rk4b[i], (float)dt, lres_io)
+      if (elas) {
+        e = launch_adj_elas_stage(c, Yi, (float)c->rk4a[i], (float)c->rk4b[i], (float)dt, lres_io,
+                                  c->Hadj + (size_t)hsel * nh1, c->Hadj + (size_t)(1 - hsel) * nh1);
+        hsel = 1 - hsel;
+      } else {
+        e = c->adj_impl ? launch_adj_stage_tiled(c, Yi, (float)c->rk4a[i], (float)c->rk4b[i], (float)dt, lres_io)
                         : launch_adj_stage(c, Yi, (float)c->rk4a[i], (float)c->rk4b[i], (float)dt);
+      }
       if (e != hipSuccess) return hipfail(c, e, "adjoint stage");
     }
     return BBW_OK;
@@ -1375,6 +1382,8 @@ int gradient_impl(bbw_ctx *c, int nsteps, double dt, double t0, const float *d_o
   HIPCHK(c, hipMemsetAsync(c->lQ, 0, n * sizeof(float), c->stream));
   HIPCHK(c, hipMemsetAsync(c->lres, 0, n * sizeof(float), c->stream));
   HIPCHK(c, hipMemsetAsync(c->grad, 0, nq * sizeof(float), c->stream));
+  // elastic: no neighbour terms are pending before the first reverse stage
+  if (elas) HIPCHK(c, hipMemsetAsync(c->Hadj, 0, nh1 * sizeof(float), c->stream));
   for (int s = nsteps - 1; s >= s_cache0; --s)
     if ((rc = adj_step(s, c->Ycache + (size_t)(s - s_cache0) * ystep))) return rc;
   for (int seg = nseg - 1; seg >= 0; --seg) {

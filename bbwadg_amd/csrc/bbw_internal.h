@@ -217,8 +217,10 @@ hipError_t launch_fused_mfma(bbw_ctx *c, const float *Qin, float *Qout, float fa
 // A Q_in [K][9][Np] (the right-hand side before the WADG projection) for the adjoint
 hipError_t launch_fused_elas(bbw_ctx *c, const float *Qin, float *Qout, float fa, float fb, float fdt,
                              const float *ftime_dev, float ftime, int store_res = 1, float *yout = nullptr);
-// reverse elastic stage (two kernels): see bbw_adjoint_elas.hip
-hipError_t launch_adj_elas_stage(bbw_ctx *c, const float *Y, float fa, float fb, float fdt, int lres_io = 3);
+// reverse elastic stage (one launch): the neighbour flux terms of the previous reverse stage come in
+// through Hin, this stage's go out through Hout (see bbw_adjoint_elas.hip)
+hipError_t launch_adj_elas_stage(bbw_ctx *c, const float *Y, float fa, float fb, float fdt, int lres_io,
+                                 const float *Hin, float *Hout);
 hipError_t launch_adj_stage(bbw_ctx *c, const float *Y, float fa, float fb, float fdt);
 // lres_io bit 0: read the adjoint residual (else it is zero); bit 1: store it (else it is dead)
 hipError_t launch_adj_stage_tiled(bbw_ctx *c, const float *Y, float fa, float fb, float fdt, int lres_io = 3);
