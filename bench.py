@@ -74,6 +74,7 @@ def cpu_baseline(ops, dm, c2q, K, sample_elems, threads):
     os.environ["OMP_NUM_THREADS"] = str(threads)
     import ctypes
     import oracle as orc
+    orc.select_build("liborc_fast.so")   # -O3, AVX2/FMA build of the same C sources
     orc.lib()
     ctypes.CDLL("libgomp.so.1").omp_set_num_threads(int(threads))
     oo = orc.OracleOps(ops, np.float32)
@@ -444,7 +445,8 @@ def main():
         rate, el, Ks = cpu_baseline(ops, dm, c2q, K, args.cpu_sample, threads)
         rate1, el1, Ks1 = cpu_baseline(ops, dm, c2q, K, max(1, min(args.cpu_sample, K) // 16), 1)
         cpu = {"value": rate, "unit": "DOF-stage/s", "cores": threads, "kind": "port",
-               "sample": f"oracle FP32 C restatement (OpenMP), 1 LSRK45 step (5 stages) over the "
+               "sample": f"oracle FP32 C restatement (gcc -O3 -march=x86-64-v3, OpenMP), 1 LSRK45 "
+                         f"step (5 stages) over the "
                          f"first {Ks} of {K} elements of the same mesh/order, {el:.1f} s",
                "value_1_thread": rate1,
                "sample_1_thread": f"the same on 1 thread over the first {Ks1} elements, {el1:.1f} s",

@@ -31,14 +31,24 @@ import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB = None
+_LIBNAME = "liborc.so"
+
+
+def select_build(name):
+    """Load ``name`` (``liborc.so``, the checker, or ``liborc_fast.so``, the same sources
+    optimised for the CPU baseline timing) on first use; call before anything else."""
+    global _LIBNAME
+    if _LIB is not None and name != _LIBNAME:
+        raise RuntimeError("the oracle library is already loaded")
+    _LIBNAME = name
 
 
 def lib():
     global _LIB
     if _LIB is None:
-        path = os.path.join(_HERE, "liborc.so")
+        path = os.path.join(_HERE, _LIBNAME)
         if not os.path.exists(path):
-            raise RuntimeError("oracle/liborc.so is not built (run make -C oracle)")
+            raise RuntimeError(f"oracle/{_LIBNAME} is not built (run make -C oracle)")
         _LIB = ctypes.CDLL(path)
     return _LIB
 
