@@ -40,7 +40,8 @@ struct AdjElasLayout {
   static constexpr int Np = Dims<N>::Np, Nfp = Dims<N>::Nfp, NfpNf = Dims<N>::NfpNf;
   static constexpr int Nq = Dims<N>::Nq, TB = 16, RowF = 9 * Np;
   static constexpr int NQT = (Nq + 15) / 16, NQP = 16 * NQT, NPT = (Np + 15) / 16;
-  static constexpr int KTV = (Np + 3) / 4, KTP = NQP / 4, QTW = (NQT + 3) / 4, MTW = (NPT + 3) / 4;
+  // KTP: K-steps over the cubature points (S is zero past Nq, so the all-zero steps are skipped)
+  static constexpr int KTV = (Np + 3) / 4, KTP = (Nq + 3) / 4, QTW = (NQT + 3) / 4, MTW = (NPT + 3) / 4;
   static constexpr int KBV = (KTV + 3) / 4, KBP = (KTP + 3) / 4;
   static constexpr int FMT = (Nfp + 15) / 16, NPR = 4 * FMT;   // (face, node tile) pairs
   static constexpr int Np1 = N * (N + 1) * (N + 2) / 6;

@@ -825,7 +825,8 @@ template <int N>
 struct ElasUpdLayout {
   static constexpr int Np = Dims<N>::Np, Nq = Dims<N>::Nq, TB = 16, NpNf = 9 * Np;
   static constexpr int NQT = (Nq + 15) / 16, NQP = 16 * NQT, NPT = (Np + 15) / 16;
-  static constexpr int KTV = (Np + 3) / 4, KTP = NQP / 4, QTW = (NQT + 3) / 4;
+  // KTP: K-steps over the cubature points (S is zero past Nq, so the all-zero steps are skipped)
+  static constexpr int KTV = (Np + 3) / 4, KTP = (Nq + 3) / 4, QTW = (NQT + 3) / 4;
   static constexpr int KBV = (KTV + 3) / 4, KBP = (KTP + 3) / 4;
   static constexpr int RS0 = 9 * Np > 8 * Np + 4 * KTV ? 9 * Np : 8 * Np + 4 * KTV;
   static constexpr int RS = RS0 + ((34 - RS0 % 32) % 32);

@@ -39,7 +39,8 @@ struct ElasFusedLayout {
   static constexpr int Np = Dims<N>::Np, Nfp = Dims<N>::Nfp, NfpNf = Dims<N>::NfpNf, Nq = Dims<N>::Nq;
   static constexpr int TB = 16, RowF = 9 * Np;
   static constexpr int NQT = (Nq + 15) / 16, NQP = NQT * 16, NPT = (Np + 15) / 16;
-  static constexpr int KTV = (Np + 3) / 4, KTP = NQP / 4, QTW = (NQT + 3) / 4;
+  // KTP: K-steps over the cubature points (S is zero past Nq, so the all-zero steps are skipped)
+  static constexpr int KTV = (Np + 3) / 4, KTP = (Nq + 3) / 4, QTW = (NQT + 3) / 4;
   static constexpr int KBV = (KTV + 3) / 4, KBP = (KTP + 3) / 4;
   static constexpr int KF = 4 * ((Nfp + 3) / 4), KTL = KF / 4, KBL = (KTL + 3) / 4;
   static constexpr int MTW = (NPT + 3) / 4;   // node tiles per wave in the lift and Pq products
