@@ -42,6 +42,9 @@ OPT_PLAIN_IMPL = 7
 OPT_SRC_STEP_TIME = 8
 OPT_PLANAR_SPLIT = 9
 OPT_REC_FIELD = 10
+OPT_ELAS_FUSED = 11
+ELAS_LIFT_NODES = 0
+ELAS_LIFT_CUBATURE = 1
 FUSED_VECTOR, FUSED_MFMA = 0, 1
 PLAIN_FLAT, PLAIN_TILED, PLAIN_SLICE = 0, 1, 2
 

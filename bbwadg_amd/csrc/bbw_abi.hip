@@ -175,6 +175,9 @@ int bbw_create(const bbw_config *cfg, bbw_ctx **out) {
   c->Nfp = nfp_of(cfg->N);
   c->Nq = cfg->Nq;
   c->Nf = cfg->Nfields;
+  // the folded elastic stage is the faster form up to N = 4; at N = 5 its operator blocks no
+  // longer stay in registers and the nodal form is faster
+  c->elas_fused = c->N <= 4 ? BBW_ELAS_LIFT_CUBATURE : BBW_ELAS_LIFT_NODES;
   c->K = cfg->K;
   c->dev = cfg->device;
   c->KblkV = cfg->KblkV;
@@ -225,7 +228,7 @@ void bbw_destroy(bbw_ctx *c) {
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   reset_graph(c);
   dfree(c->Dids); dfree(c->Dvals); dfree(c->Fmask); dfree(c->EEL_ids); dfree(c->L0_ids);
-  dfree(c->EEL_vals); dfree(c->L0_vals); dfree(c->Vq); dfree(c->Pq); dfree(c->Vq64); dfree(c->Pq64); dfree(c->VqPad); dfree(c->PqPad); dfree(c->EEL_ids4); dfree(c->LiftPad); dfree(c->VqBlk); dfree(c->PqBlk); dfree(c->LiftBlk); dfree(c->PqTBlk); dfree(c->VqTBlk); dfree(c->LiftTBlk); dfree(c->PqTPad); dfree(c->VqTPad);
+  dfree(c->EEL_vals); dfree(c->L0_vals); dfree(c->Vq); dfree(c->Pq); dfree(c->Vq64); dfree(c->Pq64); dfree(c->VqPad); dfree(c->PqPad); dfree(c->EEL_ids4); dfree(c->LiftPad); dfree(c->VqBlk); dfree(c->PqBlk); dfree(c->LiftBlk); dfree(c->VqCBlk); dfree(c->PqTBlk); dfree(c->VqTBlk); dfree(c->LiftTBlk); dfree(c->PqTPad); dfree(c->VqTPad);
   dfree(c->bb_CN); dfree(c->bb_PBlk); dfree(c->c2bb);
   dfree(c->slice_inv); dfree(c->red_ids); dfree(c->red_vals); dfree(c->cEL);
   dfree(c->cv_Vq); dfree(c->cv_Vr); dfree(c->cv_Vs); dfree(c->cv_Vt); dfree(c->cv_Pq); dfree(c->cv_Pr);
@@ -296,6 +299,8 @@ int bbw_upload_operators(bbw_ctx *c, const bbw_ops *o) {
   {
     const int KJ = 4 * ((Np + 3) / 4), NQP = 16 * ((Nq + 15) / 16), NPP = 16 * ((Np + 15) / 16);
     std::vector<float> vp((size_t)KJ * NQP, 0.f), pp((size_t)NQP * NPP, 0.f);
+    std::vector<int> elev_ids;    // the degree elevation E of the factored derivative: node a
+    std::vector<float> elev_w;    // takes sum_j elev_w[4a+j] F[elev_ids[4a+j]] (degree N-1 nodes)
     for (int j = 0; j < Np; ++j)
       for (int q = 0; q < Nq; ++q) vp[(size_t)j * NQP + q] = o->Vq[q + (size_t)j * Nq];
     for (int q = 0; q < Nq; ++q)
@@ -405,6 +410,8 @@ int bbw_upload_operators(bbw_ctx *c, const bbw_ops *o) {
       if ((rc = upload(c, &c->Sids, (const int4 *)sid.data(), (size_t)Np1))) return rc;
       if ((rc = upload(c, &c->Eids, (const int4 *)eid.data(), (size_t)Np))) return rc;
       if ((rc = upload(c, &c->Evals, (const float4 *)ev.data(), (size_t)Np))) return rc;
+      elev_ids = eid;
+      elev_w = ev;
     }
     const int NfpNf = 4 * Nfp;
     auto transpose_sparse = [&](const int32_t *ids, const float *vals, int rows, int cols, int nnz,
@@ -482,6 +489,30 @@ int bbw_upload_operators(bbw_ctx *c, const bbw_ops *o) {
       for (int f = 0; f < 4; ++f) {
         blocks(Np, Nfp, [&](int n, int t) { return (float)lift[(size_t)n * NfpNf + f * Nfp + t]; }, lf);
         lb.insert(lb.end(), lf.begin(), lf.end());
+      }
+      // The folded elastic stage's A operand (M = cubature points): Vq (E F + sum_f LIFT_f flux_f)
+      // as one product over K = [the degree-(N-1) nodes | face 0 | ... | face 3], each part padded
+      // to whole K-steps of 4 (KT1 steps for the Np1 nodes; KF / 4 per face, zero past Nfp).
+      // Formed in double from the FP64 Vq when given.
+      {
+        const int N = c->N, Np1 = N * (N + 1) * (N + 2) / 6, KT1 = (Np1 + 3) / 4, KTL = KF / 4;
+        auto vq = [&](int q, int a) { return o->Vq64 ? o->Vq64[q + (size_t)a * Nq] : (double)o->Vq[q + (size_t)a * Nq]; };
+        std::vector<double> vqe((size_t)Nq * Np1, 0.0), vql((size_t)Nq * NfpNf, 0.0);
+        for (int q = 0; q < Nq; ++q)
+          for (int a = 0; a < Np; ++a) {
+            const double v = vq(q, a);
+            for (int j = 0; j < 4; ++j) vqe[(size_t)q * Np1 + elev_ids[4 * a + j]] += v * elev_w[4 * a + j];
+            for (int t = 0; t < NfpNf; ++t) vql[(size_t)q * NfpNf + t] += v * lift[(size_t)a * NfpNf + t];
+          }
+        std::vector<float> cb;
+        blocks(Nq, 4 * (KT1 + 4 * KTL), [&](int q, int col) -> float {
+          const int kt = col / 4;
+          if (kt < KT1) return col < Np1 ? (float)vqe[(size_t)q * Np1 + col] : 0.f;
+          const int f = (kt - KT1) / KTL, t = 4 * ((kt - KT1) % KTL) + col % 4;
+          return t < Nfp ? (float)vql[(size_t)q * NfpNf + f * Nfp + t] : 0.f;
+        }, cb);
+        dfree(c->VqCBlk);
+        if ((rc = upload(c, &c->VqCBlk, cb.data(), cb.size()))) return rc;
       }
       dfree(c->VqBlk);
       dfree(c->PqBlk);
@@ -1049,7 +1080,8 @@ int bbw_step(bbw_ctx *c, int nsteps, double dt, double t0, int mode, int flags) 
     float *rec_win = c->ftime_win + G * 5;
     const int parity = mode == BBW_MODE_FUSED ? c->cur : 0;
     Graph &g = c->graph;
-    if (!g.exec || g.steps != G || g.mode != mode || g.flags != flags || g.parity != parity || g.dt != dt) {
+    if (!g.exec || g.steps != G || g.mode != mode || g.flags != flags || g.parity != parity || g.dt != dt ||
+        g.form != elas_form(c)) {
       reset_graph(c);
       HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
       const int cur0 = c->cur;
@@ -1074,7 +1106,7 @@ int bbw_step(bbw_ctx *c, int nsteps, double dt, double t0, int mode, int flags) 
       e = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
       (void)hipGraphDestroy(graph);
       if (e != hipSuccess) return hipfail(c, e, "hipGraphInstantiate");
-      g.steps = G; g.mode = mode; g.flags = flags; g.parity = parity; g.dt = dt;
+      g.steps = G; g.mode = mode; g.flags = flags; g.parity = parity; g.dt = dt; g.form = elas_form(c);
     }
     if (has_src) {
       // the call's stage values go to the device once, through a persistent pinned staging
@@ -1253,8 +1285,16 @@ int restore_state(bbw_ctx *c) {
   return BBW_OK;
 }
 
+// Marks the adjoint driver's forward passes for the duration of a misfit or gradient call.
+struct AdjPass {
+  bbw_ctx *c;
+  explicit AdjPass(bbw_ctx *ctx) : c(ctx) { c->adj_pass = 1; }
+  ~AdjPass() { c->adj_pass = 0; }
+};
+
 int misfit_impl(bbw_ctx *c, int nsteps, double dt, double t0, const float *d_obs, bool obs_dev, double *J) {
   if (!c || !J) return BBW_ERR_ARG;
+  AdjPass pass(c);
   int rc = check_inverse(c, nsteps, d_obs);
   if (rc) return rc;
   const float *d_dev = nullptr;
@@ -1268,6 +1308,7 @@ int misfit_impl(bbw_ctx *c, int nsteps, double dt, double t0, const float *d_obs
 int gradient_impl(bbw_ctx *c, int nsteps, double dt, double t0, const float *d_obs, bool obs_dev,
                   int ckpt_every, float *grad_out, bool grad_dev, double *J) {
   if (!c || !grad_out || !J || ckpt_every <= 0) return BBW_ERR_ARG;
+  AdjPass pass(c);
   int rc = check_inverse(c, nsteps, d_obs);
   if (rc) return rc;
   if (!c->mapP) return fail(c, BBW_ERR_STATE, "adjoint maps not uploaded");
@@ -1451,6 +1492,10 @@ int bbw_set_option(bbw_ctx *c, int option, int value) {
       reset_graph(c);
       break;
     case BBW_OPT_PLANAR_SPLIT: c->planar_split = value ? 1 : 0; break;
+    case BBW_OPT_ELAS_FUSED:
+      if (value != BBW_ELAS_LIFT_NODES && value != BBW_ELAS_LIFT_CUBATURE) return fail(c, BBW_ERR_ARG, "bad elastic fused impl");
+      c->elas_fused = value;
+      break;
     default: return fail(c, BBW_ERR_ARG, "unknown option");
   }
   reset_graph(c);

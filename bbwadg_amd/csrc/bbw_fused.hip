@@ -1571,6 +1571,7 @@ FusedOps fused_ops(const bbw_ctx *c) {
   o.VqBlk = c->VqBlk;
   o.PqBlk = c->PqBlk;
   o.LiftBlk = c->LiftBlk;
+  o.VqCBlk = c->VqCBlk;
   o.Sids = c->Sids;
   o.Eids = c->Eids;
   o.Evals = c->Evals;

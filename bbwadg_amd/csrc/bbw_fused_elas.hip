@@ -11,11 +11,26 @@
 //      table rows
 //   2  penalty flux of the 9 fields at the thread's face nodes (Q+ = 0 on the boundary), in
 //      registers: the 3 velocity fluxes and the 3-vector w that determines the 6 stress fluxes
-//   then for each group g of three fields (velocities; normal stresses; shear stresses), whose
-//   cubature-point maps are independent (1/rho; the isotropic stiffness; mu):
-//   3  volume step 1: F_g at the degree-(N-1) nodes b from the shifted coefficients u[b + e_m]
-//      of the fields the group's outputs differentiate (div sigma for v; the strain rates of v
-//      for the stresses); the group's flux -> LDS as MFMA B operands
+//   then, for the groups of three fields whose cubature-point maps are independent (velocities:
+//   1/rho; normal stresses: the isotropic stiffness; shear stresses: mu), one of two forms:
+//
+//   k_fused_elas_cq (BBW_ELAS_LIFT_CUBATURE, the default for N <= 4): the volume elevation and the
+//   lift are folded into the cubature operator, Y = Vq (E F + sum_f LIFT_f flux_f) = [Vq E |
+//   Vq LIFT_0 | .. | Vq LIFT_3] [F; flux_0; ..; flux_3], one MFMA product per field (K = the
+//   degree-(N-1) nodes and the face nodes), so the right-hand side never exists at the nodes before
+//   the projection.  One pass for the velocities, one for the six stresses:
+//   A  volume step 1: F at the degree-(N-1) nodes b from the shifted coefficients u[b + e_m] of the
+//      fields the outputs differentiate; the face values (velocity fluxes z, or the 3-vector w of
+//      the stress fluxes) -> LDS as B operand rows
+//   B  Y on MFMA (M = cubature points).  The stresses' flux part is formed per face f and component
+//      c as P = [Vq LIFT_f] w_c, which enters three stresses with the face's normal (sxx += nx P_x,
+//      sxz += nz P_x, sxy += ny P_x, ...), so w is lifted once for both stress groups
+//   then per group: S = map_g(Y) -> LDS; C rhs = Pq S on MFMA (M = nodes); D LSRK for the group's
+//   rows: res = fa res + fdt (rhs + source); Q_out = Q_in + fb res
+//
+//   k_fused_elas (BBW_ELAS_LIFT_NODES, and the adjoint driver's forward passes, which keep A Q at
+//   the nodes): per group, the right-hand side is formed at the nodes, then projected:
+//   3  volume step 1 (as A); the group's flux -> LDS as MFMA B operands
 //   4  volume step 2: rhs_g[a] = sum_j a_j F_g[a - e_j] (elevation) -> LDS
 //   5  lift on MFMA: rhs_g += LIFT_f flux_f per face (M = nodes, N = elements, K = face nodes);
 //      the six stress fluxes are the symmetric part of n (x) w with one 3-vector w per face node
@@ -23,7 +38,7 @@
 //      both stress groups) and apply the normals afterwards
 //   6  S = map_g(Vq rhs_g) on MFMA (M = cubature points, K = nodes)
 //   7  rhs_g = Pq S on MFMA (M = nodes, K = cubature points)
-//   8  LSRK for the group's rows: res = fa res + fdt (rhs + source); Q_out = Q_in + fb res
+//   8  LSRK as D
 // The derivative in factored form: D_m u[a] = sum_j a_j u[a - e_j + e_m], and every output is
 // linear in the D_m u, so the volume terms are an elevation of values formed at the degree-(N-1)
 // nodes.  The lift is LIFT = EEL blkdiag(L0) applied densely (as rk_surface_elas lifts densely).
@@ -34,22 +49,16 @@
 
 namespace bbw {
 
+// Thread mappings and sizes shared by both forms of the stage.
 template <int N>
-struct ElasFusedLayout {
+struct ElasTileLayout {
   static constexpr int Np = Dims<N>::Np, Nfp = Dims<N>::Nfp, NfpNf = Dims<N>::NfpNf, Nq = Dims<N>::Nq;
   static constexpr int TB = 16, RowF = 9 * Np;
   static constexpr int NQT = (Nq + 15) / 16, NQP = NQT * 16, NPT = (Np + 15) / 16;
   // KTP: K-steps over the cubature points (S is zero past Nq, so the all-zero steps are skipped)
-  static constexpr int KTV = (Np + 3) / 4, KTP = (Nq + 3) / 4, QTW = (NQT + 3) / 4;
-  static constexpr int KBV = (KTV + 3) / 4, KBP = (KTP + 3) / 4;
+  static constexpr int KTP = (Nq + 3) / 4, QTW = (NQT + 3) / 4, KBP = (KTP + 3) / 4;
   static constexpr int KF = 4 * ((Nfp + 3) / 4), KTL = KF / 4, KBL = (KTL + 3) / 4;
-  static constexpr int MTW = (NPT + 3) / 4;   // node tiles per wave in the lift and Pq products
-  // a group's rhs rows: 3 fields of Np plus the K padding the Vq product reads past the third,
-  // row stride rounded up to 2 mod 32 (MFMA operand reads: 16 rows x 2 consecutive words)
-  static constexpr int RG0 = 3 * Np > 2 * Np + 4 * KTV ? 3 * Np : 2 * Np + 4 * KTV;
-  static constexpr int RG = RG0 + ((34 - RG0 % 32) % 32);
-  static constexpr int KB = NfpNf + 4, BS = TB + 1;  // flux rows (zero tail) and their stride
-  // thread mappings
+  static constexpr int MTW = (NPT + 3) / 4;   // node tiles per wave in the products with M = nodes
   static constexpr int NG3 = 256 / NfpNf, EPT3 = (TB + NG3 - 1) / NG3;   // face nodes
   static constexpr int GS = ((Np + 63) / 64) * 64, NG = 256 / GS, EPTV = (TB + NG - 1) / NG;  // nodes
   static constexpr int Np1 = N * (N + 1) * (N + 2) / 6;
@@ -59,22 +68,280 @@ struct ElasFusedLayout {
   static constexpr int LR = 3 * Np, HE = 2 * LR <= 256 ? 2 : 1, EPL = TB / HE;
   static_assert(LR <= 256, "one LSRK row per thread");
   static constexpr bool PFR = HE == 2;   // residual rows prefetched into registers
-  // LDS: Q_in tile padded to whole 4 KB rounds | geometry vgeo [TB][9], fgeo [TB][16] |
-  // F float4 [TB][Np1] | rhs_g [TB][RG] | W: flux [3][KB][BS] or S [3][NQP][16] | slots [TB]
+  // the Q_in tile padded to whole 4 KB rounds of 256 threads x 16 bytes
   static constexpr unsigned QB = TB * RowF * 4u, QIT = (QB + 4095) / 4096;
   static constexpr int QTP = (int)QIT * 1024;
-  static constexpr int OQ = 0, OG = QTP, OF = OG + TB * 25, OR = OF + 4 * TB * Np1;
-  static constexpr int WSZ = 3 * KB * BS > 3 * NQP * 16 ? 3 * KB * BS : 3 * NQP * 16;
-  static constexpr int OW = OR + TB * RG, OSL = OW + WSZ;
-  static constexpr size_t floats = (size_t)OSL + TB;
-  static_assert(3 * (KB - NfpNf) * BS <= 256, "one flux tail word per thread");
-  static_assert(TB * (RG0 - 3 * Np) <= 256, "one rhs padding word per thread");
+};
+
+template <int N>
+struct ElasFusedLayout : ElasTileLayout<N> {
+  using B = ElasTileLayout<N>;
+  static constexpr int KTV = (B::Np + 3) / 4, KBV = (KTV + 3) / 4;
+  // a group's rhs rows: 3 fields of Np plus the K padding the Vq product reads past the third,
+  // row stride rounded up to 2 mod 32 (MFMA operand reads: 16 rows x 2 consecutive words)
+  static constexpr int RG0 = 3 * B::Np > 2 * B::Np + 4 * KTV ? 3 * B::Np : 2 * B::Np + 4 * KTV;
+  static constexpr int RG = RG0 + ((34 - RG0 % 32) % 32);
+  static constexpr int KB = B::NfpNf + 4, BS = B::TB + 1;  // flux rows (zero tail) and their stride
+  // LDS: Q_in tile | geometry vgeo [TB][9], fgeo [TB][16] | F float4 [TB][Np1] | rhs_g [TB][RG] |
+  // W: flux [3][KB][BS] or S [3][NQP][16] | slots [TB]
+  static constexpr int OQ = 0, OG = B::QTP, OF = OG + B::TB * 25, OR = OF + 4 * B::TB * B::Np1;
+  static constexpr int WSZ = 3 * KB * BS > 3 * B::NQP * 16 ? 3 * KB * BS : 3 * B::NQP * 16;
+  static constexpr int OW = OR + B::TB * RG, OSL = OW + WSZ;
+  static constexpr size_t floats = (size_t)OSL + B::TB;
+  static_assert(3 * (KB - B::NfpNf) * BS <= 256, "one flux tail word per thread");
+  static_assert(B::TB * (RG0 - 3 * B::Np) <= 256, "one rhs padding word per thread");
+};
+
+template <int N>
+struct ElasCqLayout : ElasTileLayout<N> {
+  using B = ElasTileLayout<N>;
+  static constexpr int KT1 = (B::Np1 + 3) / 4;                         // K-steps over F
+  static constexpr int KTC = KT1 + 4 * B::KTL, KBC = (KTC + 3) / 4;    // the folded product
+  // B operand rows (stride BS = 17, which spreads column writes over banks).  F of a field:
+  // rows [0, Np1), zero up to R1 = 4 KT1.  Face values of a field: face node m (face-major, Nfp
+  // per face) at row m, and a zero tail of 4 rows (the last face's last K-step reads past its Nfp
+  // rows, against zero columns of the operator).
+  //   velocity pass: per field b, [F (R1 rows) | face values (RT rows)] at b RBV
+  //   stress pass:   F of the 6 stresses at b R1, then the 3 components of w at 6 R1 + c RT
+  static constexpr int R1 = 4 * KT1, RT = B::NfpNf + 4, RBV = R1 + RT, BS = B::TB + 1;
+  static constexpr int WV = 3 * RBV * BS, WT = (6 * R1 + 3 * RT) * BS, WS = 3 * B::NQP * 16;
+  static constexpr int W = WT > WV ? (WT > WS ? WT : WS) : (WV > WS ? WV : WS);
+  static constexpr int NZF = R1 - B::Np1;                               // zero F rows per field
+  static constexpr int NZV = 3 * (NZF + 4) * BS, NZT = (6 * NZF + 3 * 4) * BS;   // zero words
+  // rhs rows [TB][RG] after the Pq product (odd stride: its column stores spread over banks)
+  static constexpr int RG = (3 * B::Np) | 1;
+  // LDS: Q_in tile | geometry | W: B operands, then S [3][NQP][16] | rhs | slots
+  static constexpr int OQ = 0, OG = B::QTP, OW = OG + B::TB * 25, OR = OW + W, OSL = OR + B::TB * RG;
+  static constexpr size_t floats = (size_t)OSL + B::TB;
 };
 
 struct ElasMedia {
   const float *rhoinv, *lambda, *mu;
   float tau_v, tau_s;
 };
+
+// A tile's register inputs (phase 1) and, after elas_flux, its face fluxes.
+template <int N>
+struct ElasTileIn {
+  using L = ElasTileLayout<N>;
+  int fid3;                  // the thread's face node (flux phase): its node id in the element
+  int vm[L::EPT3];           // vmapP of the thread's face nodes
+  float4 qreg[L::QIT];       // the Q_in tile on its way to LDS
+  float gv0, gf0;            // one geometry word each (vgeo, fgeo)
+  int4 Sr, Er;               // shift / elevation table rows
+  float4 evr;
+  float rho[L::QTW][4], lam[L::QTW][4], mu[L::QTW][4];   // media in the MFMA output layout
+  float rreg[L::PFR ? 3 : 1][L::PFR ? L::EPL : 1];       // residual rows (with PFR)
+  float fl[L::EPT3][9];      // neighbour traces, then the fluxes (elas_flux)
+};
+
+// Phase 1 loads: vmapP first (the trace gather waits for it), the Q_in tile and geometry, the
+// table rows, media, residual rows, then the traces.  Requires K * 9 * Np * 4 < 2^32.
+template <int N>
+__device__ __forceinline__ void elas_tile_load(ElasTileIn<N> &in, const FusedMesh &m, const FusedOps &o,
+                                               const ElasMedia &md, const float *Qin, const float *res,
+                                               float fa, int k0, int nk, const int tid) {
+  using L = ElasTileLayout<N>;
+  constexpr int Np = L::Np, NfpNf = L::NfpNf, Nq = L::Nq, RowF = L::RowF;
+  const int lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int col = lane & 15, kr = lane >> 4;
+  const unsigned tbytes = (unsigned)nk * RowF * 4u;
+  const int g3 = tid / NfpNf, m3 = tid - g3 * NfpNf;
+  in.fid3 = o.Fmask[m3 < NfpNf ? m3 : 0];
+  {
+    const brsrc rv = mkrsrc(m.vmapP + (size_t)k0 * NfpNf, (unsigned)nk * NfpNf * 4u);
+#pragma unroll
+    for (int i = 0; i < L::EPT3; ++i) in.vm[i] = bldi(rv, (unsigned)((g3 + L::NG3 * i) * NfpNf + m3) * 4u);
+  }
+  {
+    const brsrc rQ = mkrsrc(Qin + (size_t)k0 * RowF, tbytes);
+#pragma unroll
+    for (unsigned it = 0; it < L::QIT; ++it) in.qreg[it] = bld4(rQ, tid * 16u + it * 4096u);
+  }
+  in.gv0 = bld(mkrsrc(m.vgeo + (size_t)k0 * 9, (unsigned)nk * 36u), tid * 4u);
+  in.gf0 = bld(mkrsrc(m.fgeo + (size_t)k0 * 16, (unsigned)nk * 64u), tid * 4u);
+  const int nv = tid % L::GS, n1 = tid % L::GS1;
+  in.Sr = o.Sids[n1 < L::Np1 ? n1 : 0];
+  in.Er = o.Eids[nv < Np ? nv : 0];
+  in.evr = o.Evals[nv < Np ? nv : 0];
+  // rows q >= Nq read a neighbouring value; the padded Vq rows make them 0
+  {
+    const unsigned qbytes = (unsigned)nk * Nq * 4u;
+    const brsrc rRho = mkrsrc(md.rhoinv + (size_t)k0 * Nq, qbytes);
+    const brsrc rLam = mkrsrc(md.lambda + (size_t)k0 * Nq, qbytes);
+    const brsrc rMu = mkrsrc(md.mu + (size_t)k0 * Nq, qbytes);
+    const unsigned co = (unsigned)(col * Nq + 4 * kr) * 4u;
+#pragma unroll
+    for (int t = 0; t < L::QTW; ++t) {
+      const int qt = wave + 4 * t < L::NQT ? wave + 4 * t : L::NQT - 1;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        in.rho[t][r] = bld(rRho, co + (qt * 16 + r) * 4u);
+        in.lam[t][r] = bld(rLam, co + (qt * 16 + r) * 4u);
+        in.mu[t][r] = bld(rMu, co + (qt * 16 + r) * 4u);
+      }
+    }
+  }
+  // LSRK rows: row lr of the group (field 3g + lr / Np, node lr % Np), elements he * EPL ..
+  if constexpr (L::PFR) {
+    const int lr = tid % L::LR, he = tid / L::LR;
+    const bool onl = he < L::HE;
+    const brsrc rRes = mkrsrc(res + (size_t)k0 * RowF, tbytes);
+#pragma unroll
+    for (int g = 0; g < 3; ++g)
+#pragma unroll
+      for (int i = 0; i < L::EPL; ++i) {
+        const unsigned off = onl ? (unsigned)((he * L::EPL + i) * RowF + 3 * g * Np + lr) * 4u : 0x80000000u;
+        in.rreg[g][i] = fa != 0.f ? bld(rRes, off) : 0.f;
+      }
+  }
+  // neighbour traces of the 9 fields (then the flux, in the same registers)
+  {
+    const brsrc rT = mkrsrc(Qin, (unsigned)m.K * RowF * 4u);
+#pragma unroll
+    for (int i = 0; i < L::EPT3; ++i)
+#pragma unroll
+      for (int c = 0; c < 9; ++c) in.fl[i][c] = bld(rT, (unsigned)(in.vm[i] + c * Np) * 4u);
+  }
+}
+
+template <int N>
+__device__ __forceinline__ void elas_tile_store(const ElasTileIn<N> &in, float *sQ, float *sG, float *sFg,
+                                                const int tid) {
+  using L = ElasTileLayout<N>;
+  float4 *sQ4 = (float4 *)sQ;
+#pragma unroll
+  for (unsigned it = 0; it < L::QIT; ++it) sQ4[tid + it * 256u] = in.qreg[it];
+  if (tid < L::TB * 9) sG[tid] = in.gv0;
+  sFg[tid] = in.gf0;
+}
+
+// Phase 2: penalty flux at the face nodes (central flux + tau_v / tau_s penalties, Q+ = 0 on
+// the boundary), the same formula as rk_surface_elas: fl[i][0..2] the velocity fluxes,
+// fl[i][3..5] the 3-vector w of the stress fluxes.
+template <int N>
+__device__ __forceinline__ void elas_flux(ElasTileIn<N> &in, const float *sQ, const float *sFg,
+                                          const ElasMedia &md, int k0, const int tid) {
+  using L = ElasTileLayout<N>;
+  constexpr int Np = L::Np, Nfp = L::Nfp, NfpNf = L::NfpNf, TB = L::TB, RowF = L::RowF;
+  const int g3 = tid / NfpNf, m3 = tid - g3 * NfpNf, f3 = m3 / Nfp;
+  if (g3 < L::NG3) {
+#pragma unroll
+    for (int i = 0; i < L::EPT3; ++i) {
+      const int e = g3 + L::NG3 * i;
+      if (L::NG3 * L::EPT3 == TB || e < TB) {
+        const float *q = sQ + e * RowF + in.fid3;
+        const float *F = sFg + 16 * e + 4 * f3;
+        const float Fs = .5f * F[0], nx = F[1], ny = F[2], nz = F[3];
+        const bool isB = in.vm[i] == in.fid3 + (k0 + e) * RowF;
+        float dQ[9];
+#pragma unroll
+        for (int c = 0; c < 9; ++c) dQ[c] = (isB ? 0.f : in.fl[i][c]) - q[c * Np];
+        float fc[9];
+        fc[0] = dQ[3] * nx + dQ[8] * ny + dQ[7] * nz;
+        fc[1] = dQ[8] * nx + dQ[4] * ny + dQ[6] * nz;
+        fc[2] = dQ[7] * nx + dQ[6] * ny + dQ[5] * nz;
+        fc[3] = dQ[0] * nx;
+        fc[4] = dQ[1] * ny;
+        fc[5] = dQ[2] * nz;
+        fc[6] = dQ[2] * ny + dQ[1] * nz;
+        fc[7] = dQ[2] * nx + dQ[0] * nz;
+        fc[8] = dQ[1] * nx + dQ[0] * ny;
+        // velocity fluxes z = Fs (sigma jump . n + tau_v A_n^T A_n v jump)
+        in.fl[i][0] = Fs * (fc[0] + md.tau_v * (fc[3] * nx + fc[8] * ny + fc[7] * nz));
+        in.fl[i][1] = Fs * (fc[1] + md.tau_v * (fc[8] * nx + fc[4] * ny + fc[6] * nz));
+        in.fl[i][2] = Fs * (fc[2] + md.tau_v * (fc[7] * nx + fc[6] * ny + fc[5] * nz));
+        // stress fluxes: the symmetric part of n (x) w with w = Fs (v jump + tau_s sigma jump . n)
+        // (sxx: nx w_x, syz: ny w_z + nz w_y, ...); only w is kept, the normals are applied later
+        in.fl[i][3] = Fs * (dQ[0] + md.tau_s * fc[0]);
+        in.fl[i][4] = Fs * (dQ[1] + md.tau_s * fc[1]);
+        in.fl[i][5] = Fs * (dQ[2] + md.tau_s * fc[2]);
+      }
+    }
+  }
+}
+
+// Volume step 1 of group g (a constant after unrolling) at the thread's degree-(N-1) node for
+// element e: the values F whose elevation is the group's volume term, from the shifted
+// coefficients u[b + e_m] of the Q_in tile and the element's geometric factors.  g = 0: F[0..2]
+// = div sigma; g = 1, 2: F[0..2] the normal and F[3..5] the engineering shear strain rates of v
+// (both stress groups differentiate the same three fields).
+template <int N>
+__device__ __forceinline__ void elas_vol1(int g, const float *q, const float *G, const int4 Sr, float (&F)[6]) {
+  constexpr int Np = ElasTileLayout<N>::Np;
+  // gradients from (d/dr, d/ds, d/dt) = ((u[b+e2] - u[b+e1]) / 2, (u[b+e3] - u[b+e1]) / 2,
+  // (u[b+e4] - u[b+e1]) / 2)
+  auto grad = [&](int c, float &dx, float &dy, float &dz) {
+    const float *qc = q + c * Np;
+    const float s0 = qc[Sr.x], s1 = qc[Sr.y], s2 = qc[Sr.z], s3 = qc[Sr.w];
+    const float dr = .5f * (s1 - s0), ds = .5f * (s2 - s0), dt = .5f * (s3 - s0);
+    dx = G[0] * dr + G[3] * ds + G[6] * dt;
+    dy = G[1] * dr + G[4] * ds + G[7] * dt;
+    dz = G[2] * dr + G[5] * ds + G[8] * dt;
+  };
+  if (g == 0) {
+    // div sigma: (sxx_x + sxy_y + sxz_z, sxy_x + syy_y + syz_z, sxz_x + syz_y + szz_z)
+    float x3, y3, z3, x4, y4, z4, x5, y5, z5, x6, y6, z6, x7, y7, z7, x8, y8, z8;
+    grad(3, x3, y3, z3);
+    grad(4, x4, y4, z4);
+    grad(5, x5, y5, z5);
+    grad(6, x6, y6, z6);
+    grad(7, x7, y7, z7);
+    grad(8, x8, y8, z8);
+    F[0] = x3 + y8 + z7;
+    F[1] = x8 + y4 + z6;
+    F[2] = x7 + y6 + z5;
+  } else {
+    float x0, y0, z0, x1, y1, z1, x2, y2, z2;
+    grad(0, x0, y0, z0);
+    grad(1, x1, y1, z1);
+    grad(2, x2, y2, z2);
+    F[0] = x0;        // normal strain rates
+    F[1] = y1;
+    F[2] = z2;
+    F[3] = y2 + z1;   // engineering shear strain rates (yz, xz, xy)
+    F[4] = x2 + z0;
+    F[5] = x1 + y0;
+  }
+}
+
+// Phase D / 8: the LSRK stage for group g's rows from the projected rhs rows sR [TB][RG]:
+// res = fa res + fdt (rhs + source); Q_out = Q_in + fb res.
+template <int N, int RG>
+__device__ __forceinline__ void elas_lsrk(int g, const ElasTileIn<N> &in, const float *sR, const float *sQ,
+                                          const int *sSlot, const SrcArgs &src, bool has_src, float ft,
+                                          float fa, float fb, float fdt, int store_res, brsrc rRes, brsrc rOut,
+                                          const int tid) {
+  using L = ElasTileLayout<N>;
+  constexpr int Np = L::Np, RowF = L::RowF;
+  const int lr = tid % L::LR, he = tid / L::LR;
+  if (he < L::HE) {
+    const int fr = 3 * g + lr / Np, nr = lr - (lr / Np) * Np;
+    float wsel = 0.f;
+#pragma unroll
+    for (int f = 0; f < 9; ++f) wsel = fr == f ? src.w[f] : wsel;
+    float rv[L::EPL];
+#pragma unroll
+    for (int i = 0; i < L::EPL; ++i) {
+      const unsigned off = (unsigned)((he * L::EPL + i) * RowF + 3 * g * Np + lr) * 4u;
+      if constexpr (L::PFR) rv[i] = in.rreg[g][i];
+      else rv[i] = fa != 0.f ? bld(rRes, off) : 0.f;
+    }
+#pragma unroll
+    for (int i = 0; i < L::EPL; ++i) {
+      const int e = he * L::EPL + i;
+      float v = sR[e * RG + lr];
+      if (has_src) {
+        const int slot = sSlot[e];
+        if (slot >= 0) v += wsel * ft * src.coef[(size_t)slot * Np + nr];
+      }
+      const float x = fa * rv[i] + fdt * v;
+      const unsigned off = (unsigned)(e * RowF + 3 * g * Np + lr) * 4u;
+      bst(rOut, sQ[e * RowF + 3 * g * Np + lr] + fb * x, off);
+      if (store_res) bst(rRes, x, off);
+    }
+  }
+}
 
 // CAP: also write the right-hand side A Q_in (before the WADG projection, without the source)
 // to yout [K][9][Np], the stage value the discrete adjoint's media gradient needs.
@@ -87,7 +354,6 @@ __global__ __launch_bounds__(256, N <= 4 ? 3 : N == 5 ? 2 : 1) void k_fused_elas
   constexpr int Np = L::Np, Nfp = L::Nfp, NfpNf = L::NfpNf, Nq = L::Nq, TB = L::TB, RowF = L::RowF;
   constexpr int RG = L::RG, KB = L::KB, BS = L::BS, NQP = L::NQP, NPT = L::NPT, Np1 = L::Np1;
   constexpr int KTV = L::KTV, KTP = L::KTP, KBV = L::KBV, KBP = L::KBP, KTL = L::KTL, KBL = L::KBL;
-  constexpr unsigned ROW = RowF * 4u;
   extern __shared__ float lds[];
   float *sQ = lds + L::OQ;                 // [TB][RowF]
   float *sG = lds + L::OG;                 // vgeo [TB][9]
@@ -100,131 +366,34 @@ __global__ __launch_bounds__(256, N <= 4 ? 3 : N == 5 ? 2 : 1) void k_fused_elas
   const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int col = lane & 15, kr = lane >> 4;
   const int k0 = blockIdx.x * TB, nk = min(TB, m.K - k0);
-  const unsigned tbytes = (unsigned)nk * ROW;
+  const unsigned tbytes = (unsigned)nk * RowF * 4u;
   if (ftime_dev) ft = *ftime_dev;
   const bool has_src = src.nsrc > 0 && k0 + nk - 1 >= src.first && k0 <= src.last;
 
-  // ---- 1: loads.  vmapP first (the trace gather waits for it), the Q_in tile and geometry,
-  // the table rows, media, residual rows, then the traces; the LDS stores after all of them.
+  // ---- 1: loads, then the LDS stores
+  ElasTileIn<N> in;
+  elas_tile_load<N>(in, m, o, md, Qin, res, fa, k0, nk, tid);
   const int g3 = tid / NfpNf, m3 = tid - g3 * NfpNf;
   const bool on3 = g3 < L::NG3;
-  const int f3 = m3 / Nfp;
-  const int fid3 = o.Fmask[m3 < NfpNf ? m3 : 0];
-  int vm[L::EPT3];
-  {
-    const brsrc rv = mkrsrc(m.vmapP + (size_t)k0 * NfpNf, (unsigned)nk * NfpNf * 4u);
-#pragma unroll
-    for (int i = 0; i < L::EPT3; ++i) vm[i] = bldi(rv, (unsigned)((g3 + L::NG3 * i) * NfpNf + m3) * 4u);
-  }
-  float4 qreg[L::QIT];
-  {
-    const brsrc rQ = mkrsrc(Qin + (size_t)k0 * RowF, tbytes);
-#pragma unroll
-    for (unsigned it = 0; it < L::QIT; ++it) qreg[it] = bld4(rQ, tid * 16u + it * 4096u);
-  }
-  const float gv0 = bld(mkrsrc(m.vgeo + (size_t)k0 * 9, (unsigned)nk * 36u), tid * 4u);
-  const float gf0 = bld(mkrsrc(m.fgeo + (size_t)k0 * 16, (unsigned)nk * 64u), tid * 4u);
   const int gv = tid / L::GS, nv = tid - gv * L::GS;
   const bool onv = gv < L::NG && nv < Np;
   const int g1 = tid / L::GS1, n1 = tid - g1 * L::GS1;
   const bool on1 = n1 < Np1;
-  const int4 Sr = o.Sids[on1 ? n1 : 0];
-  const int4 Er = o.Eids[nv < Np ? nv : 0];
-  const float4 evr = o.Evals[nv < Np ? nv : 0];
-  // media in the MFMA output layout of the Vq product (rows q >= Nq read a neighbouring value;
-  // the padded Vq rows make them 0)
-  float rho[L::QTW][4], lam[L::QTW][4], mu[L::QTW][4];
-  {
-    const unsigned qbytes = (unsigned)nk * Nq * 4u;
-    const brsrc rRho = mkrsrc(md.rhoinv + (size_t)k0 * Nq, qbytes);
-    const brsrc rLam = mkrsrc(md.lambda + (size_t)k0 * Nq, qbytes);
-    const brsrc rMu = mkrsrc(md.mu + (size_t)k0 * Nq, qbytes);
-    const unsigned co = (unsigned)(col * Nq + 4 * kr) * 4u;
-#pragma unroll
-    for (int t = 0; t < L::QTW; ++t) {
-      const int qt = wave + 4 * t < L::NQT ? wave + 4 * t : L::NQT - 1;
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        rho[t][r] = bld(rRho, co + (qt * 16 + r) * 4u);
-        lam[t][r] = bld(rLam, co + (qt * 16 + r) * 4u);
-        mu[t][r] = bld(rMu, co + (qt * 16 + r) * 4u);
-      }
-    }
-  }
-  // LSRK rows: row lr of the group (field 3g + lr / Np, node lr % Np), elements he * EPL ..
-  const int lr = tid % L::LR, he = tid / L::LR;
-  const bool onl = he < L::HE;
+  const int4 Sr = in.Sr, Er = in.Er;
+  const float4 evr = in.evr;
+  const auto &rho = in.rho, &lam = in.lam, &mu = in.mu;
+  const auto &fl = in.fl;
   const brsrc rRes = mkrsrc(res + (size_t)k0 * RowF, tbytes);
   const brsrc rOut = mkrsrc(Qout + (size_t)k0 * RowF, tbytes);
-  float rreg[L::PFR ? 3 : 1][L::PFR ? L::EPL : 1];
-  if constexpr (L::PFR) {
-#pragma unroll
-    for (int g = 0; g < 3; ++g)
-#pragma unroll
-      for (int i = 0; i < L::EPL; ++i) {
-        const unsigned off = onl ? (unsigned)((he * L::EPL + i) * RowF + 3 * g * Np + lr) * 4u : 0x80000000u;
-        rreg[g][i] = fa != 0.f ? bld(rRes, off) : 0.f;
-      }
-  }
-  // neighbour traces of the 9 fields (then the flux, in the same registers)
-  float fl[L::EPT3][9];
-  {
-    const brsrc rT = mkrsrc(Qin, (unsigned)m.K * ROW);
-#pragma unroll
-    for (int i = 0; i < L::EPT3; ++i)
-#pragma unroll
-      for (int c = 0; c < 9; ++c) fl[i][c] = bld(rT, (unsigned)(vm[i] + c * Np) * 4u);
-  }
-  {
-    float4 *sQ4 = (float4 *)sQ;
-#pragma unroll
-    for (unsigned it = 0; it < L::QIT; ++it) sQ4[tid + it * 256u] = qreg[it];
-  }
-  if (tid < TB * 9) sG[tid] = gv0;
-  sFg[tid] = gf0;
+  elas_tile_store<N>(in, sQ, sG, sFg, tid);
   // zero K padding of the rhs rows (read by the Vq product, never written otherwise)
   constexpr int KPAD = L::RG0 - 3 * Np;
   if (KPAD > 0 && tid < TB * KPAD) sR[(tid / KPAD) * RG + 3 * Np + tid % KPAD] = 0.f;
   if (has_src && tid < TB) sSlot[tid] = tid < nk ? src_slot2(src, k0 + tid) : -1;
   __syncthreads();
 
-  // ---- 2: penalty flux at the face nodes (central flux + tau_v / tau_s penalties, Q+ = 0 on
-  // the boundary), the same formula as rk_surface_elas
-  if (on3) {
-#pragma unroll
-    for (int i = 0; i < L::EPT3; ++i) {
-      const int e = g3 + L::NG3 * i;
-      if (L::NG3 * L::EPT3 == TB || e < TB) {
-        const float *q = sQ + e * RowF + fid3;
-        const float *F = sFg + 16 * e + 4 * f3;
-        const float Fs = .5f * F[0], nx = F[1], ny = F[2], nz = F[3];
-        const bool isB = vm[i] == fid3 + (k0 + e) * RowF;
-        float dQ[9];
-#pragma unroll
-        for (int c = 0; c < 9; ++c) dQ[c] = (isB ? 0.f : fl[i][c]) - q[c * Np];
-        float fc[9];
-        fc[0] = dQ[3] * nx + dQ[8] * ny + dQ[7] * nz;
-        fc[1] = dQ[8] * nx + dQ[4] * ny + dQ[6] * nz;
-        fc[2] = dQ[7] * nx + dQ[6] * ny + dQ[5] * nz;
-        fc[3] = dQ[0] * nx;
-        fc[4] = dQ[1] * ny;
-        fc[5] = dQ[2] * nz;
-        fc[6] = dQ[2] * ny + dQ[1] * nz;
-        fc[7] = dQ[2] * nx + dQ[0] * nz;
-        fc[8] = dQ[1] * nx + dQ[0] * ny;
-        // velocity fluxes z = Fs (sigma jump . n + tau_v A_n^T A_n v jump)
-        fl[i][0] = Fs * (fc[0] + md.tau_v * (fc[3] * nx + fc[8] * ny + fc[7] * nz));
-        fl[i][1] = Fs * (fc[1] + md.tau_v * (fc[8] * nx + fc[4] * ny + fc[6] * nz));
-        fl[i][2] = Fs * (fc[2] + md.tau_v * (fc[7] * nx + fc[6] * ny + fc[5] * nz));
-        // stress fluxes: the symmetric part of n (x) w with w = Fs (v jump + tau_s sigma jump . n)
-        // (sxx: nx w_x, syz: ny w_z + nz w_y, ...); only w is kept, and the normals, constant
-        // on a face, are applied after the lift
-        fl[i][3] = Fs * (dQ[0] + md.tau_s * fc[0]);
-        fl[i][4] = Fs * (dQ[1] + md.tau_s * fc[1]);
-        fl[i][5] = Fs * (dQ[2] + md.tau_s * fc[2]);
-      }
-    }
-  }
+  // ---- 2: penalty flux at the face nodes
+  elas_flux<N>(in, sQ, sFg, md, k0, tid);
 
   // MFMA A operands: lift blocks of the wave's first node tile (every group uses them), the Vq
   // blocks of the wave's cubature tiles and the Pq blocks of its first node tile.  The loads
@@ -264,46 +433,9 @@ __global__ __launch_bounds__(256, N <= 4 ? 3 : N == 5 ? 2 : 1) void k_fused_elas
 #pragma unroll
       for (int i = 0; i < L::EPT1; ++i) {
         const int e = g1 + L::NG1 * i;
-        const float *q = sQ + e * RowF;
-        const float *G = sG + 9 * e;
-        // gradients of the fields the group differentiates: (x, y, z) derivatives from
-        // (d/dr, d/ds, d/dt) = ((u[b+e2] - u[b+e1]) / 2, (u[b+e3] - u[b+e1]) / 2, (u[b+e4] - u[b+e1]) / 2)
-        auto grad = [&](int c, float &dx, float &dy, float &dz) {
-          const float *qc = q + c * Np;
-          const float s0 = qc[Sr.x], s1 = qc[Sr.y], s2 = qc[Sr.z], s3 = qc[Sr.w];
-          const float dr = .5f * (s1 - s0), ds = .5f * (s2 - s0), dt = .5f * (s3 - s0);
-          dx = G[0] * dr + G[3] * ds + G[6] * dt;
-          dy = G[1] * dr + G[4] * ds + G[7] * dt;
-          dz = G[2] * dr + G[5] * ds + G[8] * dt;
-        };
-        float F0, F1, F2;
-        if (g == 0) {
-          // div sigma: (sxx_x + sxy_y + sxz_z, sxy_x + syy_y + syz_z, sxz_x + syz_y + szz_z)
-          float x3, y3, z3, x4, y4, z4, x5, y5, z5, x6, y6, z6, x7, y7, z7, x8, y8, z8;
-          grad(3, x3, y3, z3);
-          grad(4, x4, y4, z4);
-          grad(5, x5, y5, z5);
-          grad(6, x6, y6, z6);
-          grad(7, x7, y7, z7);
-          grad(8, x8, y8, z8);
-          F0 = x3 + y8 + z7;
-          F1 = x8 + y4 + z6;
-          F2 = x7 + y6 + z5;
-        } else {
-          float x0, y0, z0, x1, y1, z1, x2, y2, z2;
-          grad(0, x0, y0, z0);
-          grad(1, x1, y1, z1);
-          grad(2, x2, y2, z2);
-          if (g == 1) {   // normal strain rates
-            F0 = x0;
-            F1 = y1;
-            F2 = z2;
-          } else {        // engineering shear strain rates (yz, xz, xy)
-            F0 = y2 + z1;
-            F1 = x2 + z0;
-            F2 = x1 + y0;
-          }
-        }
+        float Fv[6];
+        elas_vol1<N>(g, sQ + e * RowF, sG + 9 * e, Sr, Fv);
+        const float F0 = g == 2 ? Fv[3] : Fv[0], F1 = g == 2 ? Fv[4] : Fv[1], F2 = g == 2 ? Fv[5] : Fv[2];
         sF[e * Np1 + n1] = make_float4(F0, F1, F2, 0.f);
       }
     }
@@ -528,35 +660,313 @@ __global__ __launch_bounds__(256, N <= 4 ? 3 : N == 5 ? 2 : 1) void k_fused_elas
     __syncthreads();
 
     // ---- 8: LSRK for the group's rows: res = fa res + fdt (rhs + source); Q_out = Q_in + fb res
-    if (onl) {
-      const int fr = 3 * g + lr / Np, nr = lr - (lr / Np) * Np;
-      float wsel = 0.f;
-#pragma unroll
-      for (int f = 0; f < 9; ++f) wsel = fr == f ? src.w[f] : wsel;
-      float rv[L::EPL];
-#pragma unroll
-      for (int i = 0; i < L::EPL; ++i) {
-        const unsigned off = (unsigned)((he * L::EPL + i) * RowF + 3 * g * Np + lr) * 4u;
-        if constexpr (L::PFR) rv[i] = rreg[g][i];
-        else rv[i] = fa != 0.f ? bld(rRes, off) : 0.f;
-      }
-#pragma unroll
-      for (int i = 0; i < L::EPL; ++i) {
-        const int e = he * L::EPL + i;
-        float v = sR[e * RG + lr];
-        if (has_src) {
-          const int slot = sSlot[e];
-          if (slot >= 0) v += wsel * ft * src.coef[(size_t)slot * Np + nr];
-        }
-        const float x = fa * rv[i] + fdt * v;
-        const unsigned off = (unsigned)(e * RowF + 3 * g * Np + lr) * 4u;
-        bst(rOut, sQ[e * RowF + 3 * g * Np + lr] + fb * x, off);
-        if (store_res) bst(rRes, x, off);
-      }
-    }
+    elas_lsrk<N, RG>(g, in, sR, sQ, sSlot, src, has_src, ft, fa, fb, fdt, store_res, rRes, rOut, tid);
     // the next group's phase 3 writes F and the flux region, which nothing reads now; its
     // phase 4 writes rhs_g after the barrier that ends phase 3
   }
+}
+
+
+// The folded form (the default forward stage).  The velocities take one pass (A, B, C, D of
+// the header).  The stresses take one A and B for both groups: the E F parts of the six stresses,
+// and per face f and component c the product P = [Vq LIFT_f] w_c, which enters three stresses
+// with the face's normal (sxx += nx P_x, sxz += nz P_x, sxy += ny P_x, ...), so w is lifted once
+// for both groups; then C and D for the normal and for the shear stresses.
+template <int N>
+__global__ __launch_bounds__(256, N <= 4 ? 3 : N == 5 ? 2 : 1) void k_fused_elas_cq(FusedMesh m, FusedOps o, ElasMedia md,
+                                                                                  const float *Qin, float *Qout, float *res,
+                                                                                  SrcArgs src, float fa, float fb, float fdt,
+                                                                                  float ft, const float *ftime_dev, int store_res) {
+  using L = ElasCqLayout<N>;
+  constexpr int Np = L::Np, Nfp = L::Nfp, NfpNf = L::NfpNf, Nq = L::Nq, TB = L::TB, RowF = L::RowF;
+  constexpr int NQT = L::NQT, NQP = L::NQP, NPT = L::NPT, Np1 = L::Np1, QTW = L::QTW, MTW = L::MTW;
+  constexpr int KT1 = L::KT1, KTL = L::KTL, KTC = L::KTC, KBC = L::KBC, KTP = L::KTP, KBP = L::KBP;
+  constexpr int R1 = L::R1, RT = L::RT, RBV = L::RBV, BS = L::BS, RG = L::RG;
+  extern __shared__ float lds[];
+  float *sQ = lds + L::OQ;                 // [TB][RowF]
+  float *sG = lds + L::OG;                 // vgeo [TB][9]
+  float *sFg = sG + TB * 9;                // fgeo [TB][16]
+  float *sW = lds + L::OW;                 // B operands (A, B), S [3][NQP][16] (B, C)
+  float *sR = lds + L::OR;                 // [TB][RG] (C, D)
+  int *sSlot = (int *)(lds + L::OSL);
+  const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int col = lane & 15, kr = lane >> 4;
+  const int k0 = blockIdx.x * TB, nk = min(TB, m.K - k0);
+  const unsigned tbytes = (unsigned)nk * RowF * 4u;
+  if (ftime_dev) ft = *ftime_dev;
+  const bool has_src = src.nsrc > 0 && k0 + nk - 1 >= src.first && k0 <= src.last;
+
+  // ---- 1: loads, then the LDS stores
+  ElasTileIn<N> in;
+  elas_tile_load<N>(in, m, o, md, Qin, res, fa, k0, nk, tid);
+  const brsrc rRes = mkrsrc(res + (size_t)k0 * RowF, tbytes);
+  const brsrc rOut = mkrsrc(Qout + (size_t)k0 * RowF, tbytes);
+  elas_tile_store<N>(in, sQ, sG, sFg, tid);
+  if (has_src && tid < TB) sSlot[tid] = tid < nk ? src_slot2(src, k0 + tid) : -1;
+  __syncthreads();
+
+  // ---- 2: penalty flux at the face nodes
+  elas_flux<N>(in, sQ, sFg, md, k0, tid);
+
+  // MFMA A operands.  The folded operator's blocks of the wave's cubature tile stay in registers
+  // for both passes when the wave has one tile (N <= 4), and the Pq blocks of its node tile when
+  // it has one (N <= 5); otherwise they are loaded per tile.  The loads are unconditional
+  // (clamped to the last tile): loads under a wave test leave the compiler unsure how many are in
+  // flight where the paths join.
+  const brsrc rAC = mkrsrc(o.VqCBlk, (unsigned)NQT * KBC * 1024u);
+  const brsrc rAP = mkrsrc(o.PqBlk, (unsigned)NPT * KBP * 1024u);
+  constexpr bool KEEPC = QTW == 1, KEEPP = MTW == 1;
+  f4 aC[KBC], aP[KBP];
+  auto load_c = [&](int qt) {
+    qt = qt < NQT ? qt : NQT - 1;
+#pragma unroll
+    for (int kb = 0; kb < KBC; ++kb) aC[kb] = bld_blk<KTC>(rAC, lane * 16u, (unsigned)(qt * KBC + kb) * 1024u, kb);
+  };
+  auto load_p = [&](int mt) {
+    mt = mt < NPT ? mt : NPT - 1;
+#pragma unroll
+    for (int kb = 0; kb < KBP; ++kb) aP[kb] = bld_blk<KTP>(rAP, lane * 16u, (unsigned)(mt * KBP + kb) * 1024u, kb);
+  };
+  if constexpr (KEEPC) load_c(wave);
+  if constexpr (KEEPP) load_p(wave);
+
+  const int g3 = tid / NfpNf, m3 = tid - g3 * NfpNf, f3 = m3 / Nfp;
+  const bool on3 = g3 < L::NG3;
+  const int g1 = tid / L::GS1, n1 = tid - g1 * L::GS1;
+  const bool on1 = n1 < Np1;
+  const float *Bp = sW + kr * BS + col;     // the lane's B operand reads (row k, element col)
+
+  // one MFMA K-step of the folded operator: acc += A[kt] x (B rows rb + 4 (kt - k0) .. + 3)
+  auto kstep = [&](f4 &acc, int kt, int rb) {
+    acc = __builtin_amdgcn_mfma_f32_16x16x4f32(aC[kt / 4][kt % 4], Bp[rb * BS], acc, 0, 0, 0);
+  };
+  // S_b = map(acc) -> LDS for the wave's cubature tiles (rows q >= Nq are zero)
+  auto store_s = [&](const f4 (&y)[QTW][3], auto &&map) {
+#pragma unroll
+    for (int t = 0; t < QTW; ++t) {
+      const int qt = wave + 4 * t;
+      if (qt >= NQT) break;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int q = qt * 16 + 4 * kr + r;
+        float s3[3];
+        map(t, r, y[t][0][r], y[t][1][r], y[t][2][r], s3);
+        const bool qin = q < Nq;
+#pragma unroll
+        for (int b = 0; b < 3; ++b) sW[(b * NQP + q) * 16 + col] = qin ? s3[b] : 0.f;
+      }
+    }
+  };
+  // C: rhs_b = Pq S_b (M = nodes, K = cubature points) -> sR
+  auto pq = [&]() {
+#pragma unroll
+    for (int j = 0; j < MTW; ++j) {
+      const int mt = wave + 4 * j;
+      if (mt >= NPT) break;
+      if constexpr (!KEEPP) load_p(mt);
+      f4 pc[3];
+#pragma unroll
+      for (int b = 0; b < 3; ++b) pc[b] = f4{0.f, 0.f, 0.f, 0.f};
+      const float *Sp = sW + kr * 16 + col;
+#pragma unroll
+      for (int kb = 0; kb < KBP; ++kb)
+#pragma unroll
+        for (int sb = 0; sb < 4; ++sb) {
+          const int kt = 4 * kb + sb;
+          if (kt < KTP) {
+#pragma unroll
+            for (int b = 0; b < 3; ++b)
+              pc[b] = __builtin_amdgcn_mfma_f32_16x16x4f32(aP[kb][sb], Sp[(b * NQP + kt * 4) * 16], pc[b], 0, 0, 0);
+          }
+        }
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int n = mt * 16 + 4 * kr + r;
+        if (n < Np) {
+#pragma unroll
+          for (int b = 0; b < 3; ++b) sR[col * RG + b * Np + n] = pc[b][r];
+        }
+      }
+    }
+  };
+  auto lsrk = [&](int g) {
+    elas_lsrk<N, RG>(g, in, sR, sQ, sSlot, src, has_src, ft, fa, fb, fdt, store_res, rRes, rOut, tid);
+  };
+
+  // ================= velocities =================
+  // ---- A: F (div sigma) -> rows b RBV + [0, Np1); velocity fluxes z -> rows b RBV + R1 + m
+  if (on1) {
+#pragma unroll
+    for (int i = 0; i < L::EPT1; ++i) {
+      const int e = g1 + L::NG1 * i;
+      float Fv[6];
+      elas_vol1<N>(0, sQ + e * RowF, sG + 9 * e, in.Sr, Fv);
+#pragma unroll
+      for (int b = 0; b < 3; ++b) sW[(b * RBV + n1) * BS + e] = Fv[b];
+    }
+  }
+  if (on3) {
+#pragma unroll
+    for (int i = 0; i < L::EPT3; ++i) {
+      const int e = g3 + L::NG3 * i;
+      if (L::NG3 * L::EPT3 == TB || e < TB) {
+#pragma unroll
+        for (int b = 0; b < 3; ++b) sW[(b * RBV + R1 + m3) * BS + e] = in.fl[i][b];
+      }
+    }
+  }
+  {
+    constexpr int NZ = L::NZV, PER = NZ / 3;
+    for (int z = tid; z < NZ; z += 256) {
+      const int b = z / PER, r = (z - b * PER) / BS, cc = z % BS;
+      const int row = r < L::NZF ? Np1 + r : R1 + NfpNf + (r - L::NZF);
+      sW[(b * RBV + row) * BS + cc] = 0.f;
+    }
+  }
+  __syncthreads();
+  // ---- B: Y_b = [Vq E | Vq LIFT_0 .. 3] [F_b; z_b]
+  {
+    f4 acc[QTW][3];
+#pragma unroll
+    for (int t = 0; t < QTW; ++t) {
+      const int qt = wave + 4 * t;
+#pragma unroll
+      for (int b = 0; b < 3; ++b) acc[t][b] = f4{0.f, 0.f, 0.f, 0.f};
+      if (qt < NQT) {
+        if (!KEEPC) load_c(qt);
+#pragma unroll
+        for (int kt = 0; kt < KTC; ++kt) {
+          const int row = kt < KT1 ? 4 * kt : R1 + ((kt - KT1) / KTL) * Nfp + 4 * ((kt - KT1) % KTL);
+#pragma unroll
+          for (int b = 0; b < 3; ++b) kstep(acc[t][b], kt, b * RBV + row);
+        }
+      }
+    }
+    __syncthreads();
+    store_s(acc, [&](int t, int r, float y0, float y1, float y2, float (&s3)[3]) {
+      s3[0] = in.rho[t][r] * y0;
+      s3[1] = in.rho[t][r] * y1;
+      s3[2] = in.rho[t][r] * y2;
+    });
+  }
+  __syncthreads();
+  // ---- C, D
+  pq();
+  __syncthreads();
+  lsrk(0);
+
+  // ================= stresses =================
+  // ---- A: F of the normal (b = 0..2) and shear (b = 3..5) strain rates -> rows b R1 + [0, Np1);
+  // the components of w -> rows 6 R1 + c RT + m.  (S of the velocities, which these words held,
+  // was read before the last barrier; D reads sR and the Q_in tile only.)
+  if (on1) {
+#pragma unroll
+    for (int i = 0; i < L::EPT1; ++i) {
+      const int e = g1 + L::NG1 * i;
+      float Fv[6];
+      elas_vol1<N>(1, sQ + e * RowF, sG + 9 * e, in.Sr, Fv);
+#pragma unroll
+      for (int b = 0; b < 6; ++b) sW[(b * R1 + n1) * BS + e] = Fv[b];
+    }
+  }
+  if (on3) {
+#pragma unroll
+    for (int i = 0; i < L::EPT3; ++i) {
+      const int e = g3 + L::NG3 * i;
+      if (L::NG3 * L::EPT3 == TB || e < TB) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) sW[(6 * R1 + c * RT + m3) * BS + e] = in.fl[i][3 + c];
+      }
+    }
+  }
+  {
+    constexpr int NZ = L::NZT, NF = 6 * L::NZF * BS;
+    for (int z = tid; z < NZ; z += 256) {
+      const int cc = z % BS;
+      int row = 0;
+      if constexpr (L::NZF > 0) {
+        if (z < NF) {
+          const int b = z / (L::NZF * BS), r = (z - b * (L::NZF * BS)) / BS;
+          row = b * R1 + Np1 + r;
+        }
+      }
+      if (z >= NF) {
+        const int zz = z - NF, c = zz / (4 * BS), r = (zz - c * (4 * BS)) / BS;
+        row = 6 * R1 + c * RT + NfpNf + r;
+      }
+      sW[row * BS + cc] = 0.f;
+    }
+  }
+  __syncthreads();
+  // ---- B: Y_n, Y_s (normal, shear) = [Vq E] F + sum_f normal combinations of [Vq LIFT_f] w_c
+  f4 yn[QTW][3], ys[QTW][3];
+  {
+    float nx[4], ny[4], nz[4];
+#pragma unroll
+    for (int f = 0; f < 4; ++f) {
+      nx[f] = sFg[16 * col + 4 * f + 1];
+      ny[f] = sFg[16 * col + 4 * f + 2];
+      nz[f] = sFg[16 * col + 4 * f + 3];
+    }
+#pragma unroll
+    for (int t = 0; t < QTW; ++t) {
+      const int qt = wave + 4 * t;
+#pragma unroll
+      for (int b = 0; b < 3; ++b) yn[t][b] = ys[t][b] = f4{0.f, 0.f, 0.f, 0.f};
+      if (qt < NQT) {
+        if (!KEEPC) load_c(qt);
+#pragma unroll
+        for (int kt = 0; kt < KT1; ++kt) {
+#pragma unroll
+          for (int b = 0; b < 3; ++b) {
+            kstep(yn[t][b], kt, b * R1 + 4 * kt);
+            kstep(ys[t][b], kt, (3 + b) * R1 + 4 * kt);
+          }
+        }
+#pragma unroll
+        for (int f = 0; f < 4; ++f)
+#pragma unroll
+          for (int c = 0; c < 3; ++c) {
+            f4 p = f4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int kk = 0; kk < KTL; ++kk) kstep(p, KT1 + f * KTL + kk, 6 * R1 + c * RT + f * Nfp + 4 * kk);
+            // c = x: sxx nx, sxz nz, sxy ny; c = y: syy ny, syz nz, sxy nx; c = z: szz nz, syz ny, sxz nx
+            const float nn = c == 0 ? nx[f] : c == 1 ? ny[f] : nz[f];
+            const float na = c == 0 ? nz[f] : c == 1 ? nz[f] : ny[f];
+            const float nb = c == 0 ? ny[f] : c == 1 ? nx[f] : nx[f];
+            const int sa = c == 0 ? 1 : 0, sb2 = c == 2 ? 1 : 2;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+              yn[t][c][r] += nn * p[r];
+              ys[t][sa][r] += na * p[r];
+              ys[t][sb2][r] += nb * p[r];
+            }
+          }
+      }
+    }
+  }
+  __syncthreads();
+  // ---- normal stresses: S = the isotropic stiffness of Y_n; C; then S of the shear stresses
+  // (mu Y_s) and D of the normal ones between the same barriers
+  store_s(yn, [&](int t, int r, float y0, float y1, float y2, float (&s3)[3]) {
+    const float A = 2.f * in.mu[t][r] + in.lam[t][r], l = in.lam[t][r];
+    s3[0] = A * y0 + l * y1 + l * y2;
+    s3[1] = A * y1 + l * y0 + l * y2;
+    s3[2] = A * y2 + l * y0 + l * y1;
+  });
+  __syncthreads();
+  pq();
+  __syncthreads();
+  store_s(ys, [&](int t, int r, float y0, float y1, float y2, float (&s3)[3]) {
+    s3[0] = in.mu[t][r] * y0;
+    s3[1] = in.mu[t][r] * y1;
+    s3[2] = in.mu[t][r] * y2;
+  });
+  lsrk(1);
+  __syncthreads();
+  pq();
+  __syncthreads();
+  lsrk(2);
 }
 
 template <int N, bool CAP>
@@ -578,10 +988,29 @@ static hipError_t fused_elas_n(bbw_ctx *c, const float *Qin, float *Qout, float 
 }
 
 template <int N>
+static hipError_t fused_elas_cq_n(bbw_ctx *c, const float *Qin, float *Qout, float fa, float fb, float fdt,
+                                  const float *ftd, float ft, const SrcArgs &s, int store_res) {
+  using L = ElasCqLayout<N>;
+  const size_t sh = sizeof(float) * L::floats;
+  if ((unsigned long long)c->K * 36 * L::Np >= (1ull << 32)) return hipErrorInvalidValue;
+  if (sh > 160 * 1024) return hipErrorInvalidValue;
+  if (sh > 65536) {
+    hipError_t e = hipFuncSetAttribute((const void *)k_fused_elas_cq<N>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh);
+    if (e != hipSuccess) return e;
+  }
+  FusedMesh m{c->K, c->vgeo, c->fgeo, c->vmapP};
+  ElasMedia md{c->rhoinv, c->lambda, c->mu, c->tau_v, c->tau_s};
+  hipLaunchKernelGGL((k_fused_elas_cq<N>), dim3((c->K + L::TB - 1) / L::TB), dim3(256), sh, c->stream, m, fused_ops(c), md,
+                     Qin, Qout, c->res, s, fa, fb, fdt, ft, ftd, store_res);
+  return hipGetLastError();
+}
+
+template <int N>
 static hipError_t fused_elas_any(bbw_ctx *c, const float *Qin, float *Qout, float fa, float fb, float fdt,
                                  const float *ftd, float ft, const SrcArgs &s, int store_res, float *yout) {
-  return yout ? fused_elas_n<N, true>(c, Qin, Qout, fa, fb, fdt, ftd, ft, s, store_res, yout)
-              : fused_elas_n<N, false>(c, Qin, Qout, fa, fb, fdt, ftd, ft, s, store_res, nullptr);
+  if (yout) return fused_elas_n<N, true>(c, Qin, Qout, fa, fb, fdt, ftd, ft, s, store_res, yout);
+  if (elas_form(c) == BBW_ELAS_LIFT_CUBATURE) return fused_elas_cq_n<N>(c, Qin, Qout, fa, fb, fdt, ftd, ft, s, store_res);
+  return fused_elas_n<N, false>(c, Qin, Qout, fa, fb, fdt, ftd, ft, s, store_res, nullptr);
 }
 
 hipError_t launch_fused_elas(bbw_ctx *c, const float *Qin, float *Qout, float fa, float fb, float fdt,

@@ -76,6 +76,7 @@ struct FusedOps {
   const float *PqPad;
   const float *LiftPad;
   const float *VqBlk, *PqBlk, *LiftBlk;   // lane-ordered A operands (bbw_ctx::VqBlk)
+  const float *VqCBlk;                    // the folded elastic stage's operator (bbw_ctx::VqCBlk)
   const int4 *Sids;     // [Np1]: degree-N nodes b + e_1 .. b + e_4 of degree-(N-1) node b
   const int4 *Eids;     // [Np]: degree-(N-1) nodes a - e_1 .. a - e_4 of degree-N node a
   const float4 *Evals;  // [Np]: (a_1, a_2, a_3, a_4), the elevation weights

@@ -50,6 +50,7 @@ struct SrcArgs {
 struct Graph {
   hipGraphExec_t exec = nullptr;
   int steps = 0, mode = -1, flags = -1, parity = -1;
+  int form = -1;   // the elastic stage's form at capture (elas_form)
   double dt = 0;
 };
 
@@ -60,6 +61,11 @@ struct bbw_ctx {
   int ncu = 0;  // compute units of the device
   int KblkV = 0, KblkS = 0, KblkU = 0, KblkF = 0;
   int fused_impl = 1;  // BBW_FUSED_VECTOR or BBW_FUSED_MFMA
+  int elas_fused = 1;  // BBW_ELAS_LIFT_NODES or BBW_ELAS_LIFT_CUBATURE (default by N: bbw_create)
+  // set inside the adjoint driver (misfit and gradient): its forward passes run the elastic stage
+  // in the nodal form, whose A Q capture the reverse pass reads, so J and the states of every
+  // forward pass agree bitwise
+  int adj_pass = 0;
   int adj_impl = 1;    // 0: per-element adjoint kernels, 1: tiled MFMA adjoint kernels
   int plain_impl = 1;  // plain kernels: 0 flat per-element kernels (Kblk*), 1 16-element tiles,
                        // 2 flat kernels with the slice-by-slice lift
@@ -87,6 +93,9 @@ struct bbw_ctx {
   // points, K = nodes); PqBlk: A = Pq (M = nodes, K = cubature points); LiftBlk: per face f,
   // A = LIFT_f (M = nodes, K = face nodes), face blocks of NPT * KB tiles
   float *VqBlk = nullptr, *PqBlk = nullptr, *LiftBlk = nullptr;
+  // the folded elastic stage's A = Vq [E | LIFT_0 | .. | LIFT_3] (M = cubature points, K = the
+  // degree-(N-1) nodes of the factored derivative, then the face nodes of each face)
+  float *VqCBlk = nullptr;
   // the same for the adjoint stage: PqTBlk: A = Pq^T (M = cubature points, K = nodes);
   // VqTBlk: A = Vq^T (M = nodes, K = cubature points); LiftTBlk: per face f, A = LIFT_f^T
   // (M = face nodes, K = nodes), face blocks of ceil(Nfp/16) * KB tiles
@@ -246,5 +255,9 @@ hipError_t launch_update_elas(bbw_ctx *c, float fa, float fb, float fdt, const f
                               float ftime);
 
 bool order_supported(int N);
+
+// the form of the fused elastic stage a launch runs (BBW_ELAS_LIFT_*): the nodal form inside
+// the adjoint driver, else the context's option
+inline int elas_form(const bbw_ctx *c) { return c->adj_pass ? BBW_ELAS_LIFT_NODES : c->elas_fused; }
 
 }  // namespace bbw

@@ -57,6 +57,8 @@ def main():
                     help="plain kernels (default: tiled for N <= 6, flat with the slice lift above)")
     ap.add_argument("--project-velocity", action="store_true")
     ap.add_argument("--elastic", action="store_true", help="elastic (9 fields), fused or plain stage")
+    ap.add_argument("--elas-lift", choices=["cubature", "nodes"], default="cubature",
+                    help="fused elastic stage: lift folded into the cubature product (default) or at the nodes")
     ap.add_argument("--curved", default=None, metavar="MSH",
                     help="curvilinear run of a ball mesh (GMSH file; 'sphere49748' is the packaged "
                          "mesh): Gordon-Hall map, Bessel start, RK_step_WADG stages")
@@ -117,6 +119,8 @@ def main():
         return
     if args.elastic:
         s = setup_elastic(args.mesh, args.N)
+        s.set_option(_lib.OPT_ELAS_FUSED, _lib.ELAS_LIFT_CUBATURE if args.elas_lift == "cubature"
+                     else _lib.ELAS_LIFT_NODES)
         mode = MODE_FUSED if args.mode == "fused" else MODE_PLAIN
     else:
         s = setup(args.mesh, args.N)

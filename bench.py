@@ -190,6 +190,7 @@ def elastic_throughput(nsteps, mesh_n=32, N=4):
     HIP events on the solver's stream; the roofline fraction prices the fused rate at the
     compulsory bytes of an elastic element-stage."""
     import torch
+    from bbwadg_amd import _lib
     from bbwadg_amd.drive import setup_elastic
     from bbwadg_amd.solver import MODE_FUSED, MODE_PLAIN
     s = setup_elastic(mesh_n, N)
@@ -198,7 +199,10 @@ def elastic_throughput(nsteps, mesh_n=32, N=4):
     dt = s.dt_ref   # the reference's elastic time step estimate
     Q0 = s.WaveGetData3d()
     out = {}
-    for name, mode in (("fused", MODE_FUSED), ("plain", MODE_PLAIN)):
+    form0 = _lib.ELAS_LIFT_CUBATURE if N <= 4 else _lib.ELAS_LIFT_NODES   # the library's default
+    for name, mode, form in (("fused", MODE_FUSED, form0), ("nodal", MODE_FUSED, _lib.ELAS_LIFT_NODES),
+                             ("plain", MODE_PLAIN, form0)):
+        s.set_option(_lib.OPT_ELAS_FUSED, form)
         s.WaveSetData3d(Q0)
         s.set_res(np.zeros_like(Q0))
         s.step(2, dt, mode=mode)
@@ -222,6 +226,9 @@ def elastic_throughput(nsteps, mesh_n=32, N=4):
                         "Lame/density fields, fused stage + HIP graph",
             "value": dofs / (out["fused"] / 1e3), "unit": "DOF-stage/s", "steps": nsteps, "dt": dt,
             "ms_per_stage": ms_stage,
+            "fused_form": "lift folded into the cubature product" if form0 == _lib.ELAS_LIFT_CUBATURE
+                          else "lift at the nodes",
+            "nodal_form_ms_per_stage": out["nodal"] / (5 * nsteps),
             "plain_value": dofs / (out["plain"] / 1e3), "plain_ms_per_stage": out["plain"] / (5 * nsteps),
             "fused_over_plain": out["plain"] / out["fused"],
             "bytes_per_element_stage": Bst, "bytes_per_element_stage_full": stage_bytes(N, 9, 3),

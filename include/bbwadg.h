@@ -303,6 +303,13 @@ double bbw_batch_last_ms(const bbw_batch *b); /* device time of the last bbw_bat
                                  form with constant factors; 0 (default) the strong nodal form of
                                  rk_volume_planar / rk_surface_planar */
 #define BBW_OPT_REC_FIELD 10  /* the field the receivers sample (0 .. Nfields-1; default 0) */
+/* fused elastic stage: BBW_ELAS_LIFT_CUBATURE (default for N <= 4) forms Vq (E F + sum_f
+ * LIFT_f flux_f) in one product at the cubature points (the volume elevation and the lift folded
+ * into the Vq operator); BBW_ELAS_LIFT_NODES (default for N >= 5) lifts and elevates at the
+ * nodes, then applies Vq.  The adjoint driver's forward passes always run the nodal form. */
+#define BBW_OPT_ELAS_FUSED 11
+#define BBW_ELAS_LIFT_NODES 0
+#define BBW_ELAS_LIFT_CUBATURE 1
 int bbw_set_option(bbw_ctx *ctx, int option, int value);
 
 /* Device addresses of the context's buffers (for copies into caller-owned device memory):

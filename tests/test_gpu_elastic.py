@@ -76,12 +76,18 @@ def test_elastic_run_with_moment_source():
     assert rel_l2(s.WaveGetData3d(), ref) < 1e-5
 
 
+LIFTS = [_lib.ELAS_LIFT_CUBATURE, _lib.ELAS_LIFT_NODES]
+
+
+@pytest.mark.parametrize("lift", LIFTS)
 @pytest.mark.parametrize("N", [1, 2, 3, 4, 5, 6])
-def test_fused_elastic_stage(N):
-    """One fused elastic stage (volume + penalty surface + Lame WADG update + LSRK in one launch)
-    against the FP64 oracle's three kernels on the same FP32 inputs."""
-    print(f"case fused elastic stage N={N}", flush=True)
+def test_fused_elastic_stage(N, lift):
+    """One fused elastic stage (volume + penalty surface + Lame WADG update + LSRK in one launch),
+    in both forms (lift folded into the cubature product, or at the nodes), against the FP64
+    oracle's three kernels on the same FP32 inputs."""
+    print(f"case fused elastic stage N={N} lift={lift}", flush=True)
     ops, m, dm, mats, s = make(N, 3, tau=(0.8, 1.2))
+    s.set_option(_lib.OPT_ELAS_FUSED, lift)
     rng = np.random.default_rng(10 + N)
     Q = rng.uniform(-1, 1, (m.K, 9, ops.Np)).astype(np.float32)
     res = rng.uniform(-1, 1, Q.shape).astype(np.float32)
@@ -101,12 +107,14 @@ def test_fused_elastic_stage(N):
     s.close()
 
 
+@pytest.mark.parametrize("lift", LIFTS)
 @pytest.mark.parametrize("N", [3, 4])
-def test_fused_elastic_run_with_moment_source(N):
+def test_fused_elastic_run_with_moment_source(N, lift):
     """Fused elastic steps with the explosive moment-tensor source: graph replay bitwise equal to
     eager stepping, within 1e-5 of the FP64 oracle and of the three-launch path."""
-    print(f"case fused elastic run N={N}", flush=True)
+    print(f"case fused elastic run N={N} lift={lift}", flush=True)
     ops, m, dm, mats, s = make(N, 3)
+    s.set_option(_lib.OPT_ELAS_FUSED, lift)
     elems, coef = media.gaussian_source(m, ops, (0.02, -0.03, 0.05), a=15.0)
     w = np.array([0, 0, 0, 1, 1, 1, 0, 0, 0], dtype=np.float64)
     f0, amp, nsteps = 10.0, 5.0, 23
