@@ -71,6 +71,9 @@ struct ElasTileLayout {
   // the Q_in tile padded to whole 4 KB rounds of 256 threads x 16 bytes
   static constexpr unsigned QB = TB * RowF * 4u, QIT = (QB + 4095) / 4096;
   static constexpr int QTP = (int)QIT * 1024;
+  // fgeo rows in LDS [TB][FGS]: the odd stride spreads the per-element normal reads of a wave
+  // (16 elements, one per lane column) over the banks
+  static constexpr int FGS = 17, GW = TB * 9 + TB * FGS;
 };
 
 template <int N>
@@ -84,7 +87,7 @@ struct ElasFusedLayout : ElasTileLayout<N> {
   static constexpr int KB = B::NfpNf + 4, BS = B::TB + 1;  // flux rows (zero tail) and their stride
   // LDS: Q_in tile | geometry vgeo [TB][9], fgeo [TB][16] | F float4 [TB][Np1] | rhs_g [TB][RG] |
   // W: flux [3][KB][BS] or S [3][NQP][16] | slots [TB]
-  static constexpr int OQ = 0, OG = B::QTP, OF = OG + B::TB * 25, OR = OF + 4 * B::TB * B::Np1;
+  static constexpr int OQ = 0, OG = B::QTP, OF = OG + B::GW, OR = OF + 4 * B::TB * B::Np1;
   static constexpr int WSZ = 3 * KB * BS > 3 * B::NQP * 16 ? 3 * KB * BS : 3 * B::NQP * 16;
   static constexpr int OW = OR + B::TB * RG, OSL = OW + WSZ;
   static constexpr size_t floats = (size_t)OSL + B::TB;
@@ -104,14 +107,18 @@ struct ElasCqLayout : ElasTileLayout<N> {
   //   velocity pass: per field b, [F (R1 rows) | face values (RT rows)] at b RBV
   //   stress pass:   F of the 6 stresses at b R1, then the 3 components of w at 6 R1 + c RT
   static constexpr int R1 = 4 * KT1, RT = B::NfpNf + 4, RBV = R1 + RT, BS = B::TB + 1;
-  static constexpr int WV = 3 * RBV * BS, WT = (6 * R1 + 3 * RT) * BS, WS = 3 * B::NQP * 16;
+  // S transposed, [3][TB][SQS] (element rows of the cubature values): the product's four
+  // consecutive points per lane go out in one 128-bit store, and SQS = 4 mod 32 spreads the Pq
+  // operand reads (16 elements x 4 consecutive points) over the banks
+  static constexpr int SQS = B::NQP + 4;
+  static constexpr int WV = 3 * RBV * BS, WT = (6 * R1 + 3 * RT) * BS, WS = 3 * B::TB * SQS;
   static constexpr int W = WT > WV ? (WT > WS ? WT : WS) : (WV > WS ? WV : WS);
   static constexpr int NZF = R1 - B::Np1;                               // zero F rows per field
   static constexpr int NZV = 3 * (NZF + 4) * BS, NZT = (6 * NZF + 3 * 4) * BS;   // zero words
   // rhs rows [TB][RG] after the Pq product (odd stride: its column stores spread over banks)
   static constexpr int RG = (3 * B::Np) | 1;
-  // LDS: Q_in tile | geometry | W: B operands, then S [3][NQP][16] | rhs | slots
-  static constexpr int OQ = 0, OG = B::QTP, OW = OG + B::TB * 25, OR = OW + W, OSL = OR + B::TB * RG;
+  // LDS: Q_in tile | geometry | W: B operands, then S [3][TB][SQS] | rhs | slots
+  static constexpr int OQ = 0, OG = B::QTP, OW = OG + B::GW, OR = OW + W, OSL = OR + B::TB * RG;
   static constexpr size_t floats = (size_t)OSL + B::TB;
 };
 
@@ -213,7 +220,7 @@ __device__ __forceinline__ void elas_tile_store(const ElasTileIn<N> &in, float *
 #pragma unroll
   for (unsigned it = 0; it < L::QIT; ++it) sQ4[tid + it * 256u] = in.qreg[it];
   if (tid < L::TB * 9) sG[tid] = in.gv0;
-  sFg[tid] = in.gf0;
+  sFg[(tid >> 4) * L::FGS + (tid & 15)] = in.gf0;
 }
 
 // Phase 2: penalty flux at the face nodes (central flux + tau_v / tau_s penalties, Q+ = 0 on
@@ -231,7 +238,7 @@ __device__ __forceinline__ void elas_flux(ElasTileIn<N> &in, const float *sQ, co
       const int e = g3 + L::NG3 * i;
       if (L::NG3 * L::EPT3 == TB || e < TB) {
         const float *q = sQ + e * RowF + in.fid3;
-        const float *F = sFg + 16 * e + 4 * f3;
+        const float *F = sFg + L::FGS * e + 4 * f3;
         const float Fs = .5f * F[0], nx = F[1], ny = F[2], nz = F[3];
         const bool isB = in.vm[i] == in.fid3 + (k0 + e) * RowF;
         float dQ[9];
@@ -357,7 +364,7 @@ __global__ __launch_bounds__(256, N <= 4 ? 3 : N == 5 ? 2 : 1) void k_fused_elas
   extern __shared__ float lds[];
   float *sQ = lds + L::OQ;                 // [TB][RowF]
   float *sG = lds + L::OG;                 // vgeo [TB][9]
-  float *sFg = sG + TB * 9;                // fgeo [TB][16]
+  float *sFg = sG + TB * 9;                // fgeo [TB][FGS]
   float4 *sF = (float4 *)(lds + L::OF);    // [TB][Np1]
   float *sR = lds + L::OR;                 // [TB][RG]
   float *sB = lds + L::OW;                 // [3][KB][BS]  (phases 3-5)
@@ -532,9 +539,9 @@ __global__ __launch_bounds__(256, N <= 4 ? 3 : N == 5 ? 2 : 1) void k_fused_elas
           float nx[4], ny[4], nz[4];
 #pragma unroll
           for (int f = 0; f < 4; ++f) {
-            nx[f] = sFg[16 * col + 4 * f + 1];
-            ny[f] = sFg[16 * col + 4 * f + 2];
-            nz[f] = sFg[16 * col + 4 * f + 3];
+            nx[f] = sFg[L::FGS * col + 4 * f + 1];
+            ny[f] = sFg[L::FGS * col + 4 * f + 2];
+            nz[f] = sFg[L::FGS * col + 4 * f + 3];
           }
 #pragma unroll
           for (int r = 0; r < 4; ++r) {
@@ -685,8 +692,8 @@ __global__ __launch_bounds__(256, N <= 4 ? 3 : N == 5 ? 2 : 1) void k_fused_elas
   extern __shared__ float lds[];
   float *sQ = lds + L::OQ;                 // [TB][RowF]
   float *sG = lds + L::OG;                 // vgeo [TB][9]
-  float *sFg = sG + TB * 9;                // fgeo [TB][16]
-  float *sW = lds + L::OW;                 // B operands (A, B), S [3][NQP][16] (B, C)
+  float *sFg = sG + TB * 9;                // fgeo [TB][FGS]
+  float *sW = lds + L::OW;                 // B operands (A, B), S [3][TB][SQS] (B, C)
   float *sR = lds + L::OR;                 // [TB][RG] (C, D)
   int *sSlot = (int *)(lds + L::OSL);
   const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
@@ -740,12 +747,14 @@ __global__ __launch_bounds__(256, N <= 4 ? 3 : N == 5 ? 2 : 1) void k_fused_elas
   auto kstep = [&](f4 &acc, int kt, int rb) {
     acc = __builtin_amdgcn_mfma_f32_16x16x4f32(aC[kt / 4][kt % 4], Bp[rb * BS], acc, 0, 0, 0);
   };
-  // S_b = map(acc) -> LDS for the wave's cubature tiles (rows q >= Nq are zero)
+  // S_b = map(acc) -> LDS for the wave's cubature tiles (points q >= Nq are zero)
+  constexpr int SQS = L::SQS;
   auto store_s = [&](const f4 (&y)[QTW][3], auto &&map) {
 #pragma unroll
     for (int t = 0; t < QTW; ++t) {
       const int qt = wave + 4 * t;
       if (qt >= NQT) break;
+      f4 sv[3];
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int q = qt * 16 + 4 * kr + r;
@@ -753,8 +762,10 @@ __global__ __launch_bounds__(256, N <= 4 ? 3 : N == 5 ? 2 : 1) void k_fused_elas
         map(t, r, y[t][0][r], y[t][1][r], y[t][2][r], s3);
         const bool qin = q < Nq;
 #pragma unroll
-        for (int b = 0; b < 3; ++b) sW[(b * NQP + q) * 16 + col] = qin ? s3[b] : 0.f;
+        for (int b = 0; b < 3; ++b) sv[b][r] = qin ? s3[b] : 0.f;
       }
+#pragma unroll
+      for (int b = 0; b < 3; ++b) *(f4 *)(sW + (b * TB + col) * SQS + qt * 16 + 4 * kr) = sv[b];
     }
   };
   // C: rhs_b = Pq S_b (M = nodes, K = cubature points) -> sR
@@ -767,7 +778,7 @@ __global__ __launch_bounds__(256, N <= 4 ? 3 : N == 5 ? 2 : 1) void k_fused_elas
       f4 pc[3];
 #pragma unroll
       for (int b = 0; b < 3; ++b) pc[b] = f4{0.f, 0.f, 0.f, 0.f};
-      const float *Sp = sW + kr * 16 + col;
+      const float *Sp = sW + col * SQS + kr;
 #pragma unroll
       for (int kb = 0; kb < KBP; ++kb)
 #pragma unroll
@@ -776,7 +787,7 @@ __global__ __launch_bounds__(256, N <= 4 ? 3 : N == 5 ? 2 : 1) void k_fused_elas
           if (kt < KTP) {
 #pragma unroll
             for (int b = 0; b < 3; ++b)
-              pc[b] = __builtin_amdgcn_mfma_f32_16x16x4f32(aP[kb][sb], Sp[(b * NQP + kt * 4) * 16], pc[b], 0, 0, 0);
+              pc[b] = __builtin_amdgcn_mfma_f32_16x16x4f32(aP[kb][sb], Sp[b * TB * SQS + kt * 4], pc[b], 0, 0, 0);
           }
         }
 #pragma unroll
@@ -904,9 +915,9 @@ __global__ __launch_bounds__(256, N <= 4 ? 3 : N == 5 ? 2 : 1) void k_fused_elas
     float nx[4], ny[4], nz[4];
 #pragma unroll
     for (int f = 0; f < 4; ++f) {
-      nx[f] = sFg[16 * col + 4 * f + 1];
-      ny[f] = sFg[16 * col + 4 * f + 2];
-      nz[f] = sFg[16 * col + 4 * f + 3];
+      nx[f] = sFg[L::FGS * col + 4 * f + 1];
+      ny[f] = sFg[L::FGS * col + 4 * f + 2];
+      nz[f] = sFg[L::FGS * col + 4 * f + 3];
     }
 #pragma unroll
     for (int t = 0; t < QTW; ++t) {
