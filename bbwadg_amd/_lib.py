@@ -45,6 +45,7 @@ OPT_REC_FIELD = 10
 OPT_ELAS_FUSED = 11
 ELAS_LIFT_NODES = 0
 ELAS_LIFT_CUBATURE = 1
+OPT_XCD_CHUNK = 12
 FUSED_VECTOR, FUSED_MFMA = 0, 1
 PLAIN_FLAT, PLAIN_TILED, PLAIN_SLICE = 0, 1, 2
 

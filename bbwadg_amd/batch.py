@@ -135,15 +135,16 @@ def _c2_evaluator(m, ops):
 
 
 def c4_workload(seed=0, nshots=64, mesh_sizes=(16, 24, 32, 40), durations=(300, 1200), N=5,
-                step_scale=1.0, ops=None, log=None):
+                step_scale=1.0, ops=None, log=None, order="x"):
     """The batched configuration; returns (batch, description).  ``step_scale`` < 1 shortens
-    every duration by that factor (the description states it).  ``log(msg)`` reports progress."""
+    every duration by that factor (the description states it).  ``log(msg)`` reports progress.
+    ``order``: the cube meshes' element numbering (``mesh.cube_mesh``)."""
     rng = np.random.default_rng(seed)
     ops = ops if ops is not None else operators.build_ref_ops(N)
     b = ShotBatch(N, ops=ops)
     meshes = []
     for n in mesh_sizes:
-        m = meshmod.cube_mesh(n)
+        m = meshmod.cube_mesh(n, order=order)
         meshmod.face_pair(m)
         mid = b.add_mesh(m)
         cub, c2eval = _c2_evaluator(m, ops)

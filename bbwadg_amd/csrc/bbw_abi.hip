@@ -242,7 +242,7 @@ void bbw_destroy(bbw_ctx *c) {
   dfree(c->wq64); dfree(c->vgeo); dfree(c->fgeo); dfree(c->vmapP); dfree(c->J);
   dfree(c->c2q); dfree(c->Jq); dfree(c->rhoinv); dfree(c->lambda); dfree(c->mu);
   dfree(c->Q[0]); dfree(c->Q[1]); dfree(c->res); dfree(c->rhs); dfree(c->src_elems);
-  dfree(c->src_coef); dfree(c->rec_elems); dfree(c->rec_basis); dfree(c->rec_trace);
+  dfree(c->src_coef); dfree(c->src_tiles); dfree(c->rec_elems); dfree(c->rec_basis); dfree(c->rec_trace);
   dfree(c->partial); dfree(c->ftime_win); dfree(c->ft_all); dfree(c->dobs);
   if (c->ft_host) (void)hipHostFree(c->ft_host);
   if (c->ft_evt) (void)hipEventDestroy(c->ft_evt);
@@ -814,9 +814,12 @@ int bbw_set_source(bbw_ctx *c, int nsrc, const int32_t *elems, const float *coef
   if (!c || nsrc < 0) return BBW_ERR_ARG;
   dfree(c->src_elems);
   dfree(c->src_coef);
+  dfree(c->src_tiles);
   c->nsrc = 0;
   c->src_first = 0;
   c->src_last = -1;
+  c->src_tfirst = 0;
+  c->src_tlast = -1;
   reset_graph(c);
   if (nsrc == 0) return BBW_OK;
   if (!elems || !coef || !weights || !(f0 > 0)) return fail(c, BBW_ERR_ARG, "bad source");
@@ -827,9 +830,13 @@ int bbw_set_source(bbw_ctx *c, int nsrc, const int32_t *elems, const float *coef
   int rc;
   if ((rc = upload(c, &c->src_elems, (const int *)elems, (size_t)nsrc))) return rc;
   if ((rc = upload(c, &c->src_coef, coef, (size_t)nsrc * c->Np))) return rc;
+  const std::vector<int2> tiles = src_tile_table(elems, nsrc);
+  if ((rc = upload(c, &c->src_tiles, tiles.data(), tiles.size()))) return rc;
   c->nsrc = nsrc;
   c->src_first = elems[0];
   c->src_last = elems[nsrc - 1];
+  c->src_tfirst = elems[0] >> 4;
+  c->src_tlast = elems[nsrc - 1] >> 4;
   for (int f = 0; f < 9; ++f) c->src_w[f] = f < c->Nf ? (float)(weights[f] * amp) : 0.f;
   c->src_f0 = f0;
   c->src_amp = amp;
@@ -1495,6 +1502,10 @@ int bbw_set_option(bbw_ctx *c, int option, int value) {
     case BBW_OPT_ELAS_FUSED:
       if (value != BBW_ELAS_LIFT_NODES && value != BBW_ELAS_LIFT_CUBATURE) return fail(c, BBW_ERR_ARG, "bad elastic fused impl");
       c->elas_fused = value;
+      break;
+    case BBW_OPT_XCD_CHUNK:
+      if (value < 0 || value > 1024 || (value & (value - 1))) return fail(c, BBW_ERR_ARG, "XCD run length must be 0 or a power of 2 up to 1024");
+      c->xcd_chunk = value;
       break;
     default: return fail(c, BBW_ERR_ARG, "unknown option");
   }

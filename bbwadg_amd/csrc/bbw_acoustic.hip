@@ -467,13 +467,7 @@ __global__ void k_record(int nrec, int Np, int NfNp, const int *__restrict__ ele
 static inline int pick_kb(int req, int dflt) { return req > 0 ? req : dflt; }
 
 static SrcArgs make_src(const bbw_ctx *c) {
-  SrcArgs s;
-  s.nsrc = c->nsrc;
-  s.first = c->src_first;
-  s.last = c->src_last;
-  s.elems = c->src_elems;
-  s.coef = c->src_coef;
-  for (int f = 0; f < 9; ++f) s.w[f] = c->src_w[f];
+  SrcArgs s = ctx_src(c);
   return s;
 }
 

@@ -313,7 +313,7 @@ __global__ __launch_bounds__(256, N <= 5 ? 4 : 2) void k_adj1(int K, FusedOps o,
                                               float *__restrict__ H, float *__restrict__ grad,
                                               const float *__restrict__ PqTBlk,
                                               const float *__restrict__ VqTBlk,
-                                              const float *__restrict__ LiftTBlk) {
+                                              const float *__restrict__ LiftTBlk, int xcd_chunk) {
   using L = AdjLayout<N>;
   constexpr int Np = L::Np, Nfp = L::Nfp, NfpNf = L::NfpNf, Nq = L::Nq, TB = L::TB, RS = L::RS;
   constexpr int NpNf = 4 * Np, KTV = L::KTV, KTP = L::KTP, KBV = L::KBV, KBP = L::KBP;
@@ -330,7 +330,7 @@ __global__ __launch_bounds__(256, N <= 5 ? 4 : 2) void k_adj1(int K, FusedOps o,
   float *sHt = sC;                        // [TB][NfpNf] H       (phases 4-5)
   const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int col = lane & 15, kr = lane >> 4;
-  const int k0 = blockIdx.x * TB, nk = min(TB, K - k0);
+  const int k0 = tile_index(xcd_chunk) * TB, nk = min(TB, K - k0);
   const size_t base = (size_t)k0 * NpNf;
   const unsigned tbytes = (unsigned)nk * ROW, qbytes = (unsigned)nk * Nq * 4u;
   const brsrc rQl = mkrsrc(lQ + base, tbytes), rRl = mkrsrc(lres + base, tbytes);
@@ -553,7 +553,7 @@ __global__ __launch_bounds__(256) void k_adj2(int K, const float *__restrict__ v
                                               const int *__restrict__ mapP,
                                               const float *__restrict__ Z,
                                               const float *__restrict__ H, float fdt,
-                                              float *__restrict__ lQ) {
+                                              float *__restrict__ lQ, int xcd_chunk) {
   using L = AdjLayout<N>;
   constexpr int Np = L::Np, Nfp = L::Nfp, NfpNf = L::NfpNf, TB = L::TB, NpNf = 4 * Np;
   constexpr unsigned ROW = NpNf * 4u;
@@ -563,7 +563,7 @@ __global__ __launch_bounds__(256) void k_adj2(int K, const float *__restrict__ v
   float *sG = sH + L::HIT * 256;           // [TB][9], 256 words
   float *sF = sG + 256;                    // [TB][16]
   const int tid = threadIdx.x;
-  const int k0 = blockIdx.x * TB, nk = min(TB, K - k0);
+  const int k0 = tile_index(xcd_chunk) * TB, nk = min(TB, K - k0);
   const size_t base = (size_t)k0 * NpNf;
   const brsrc rZ = mkrsrc(Z + base, (unsigned)nk * ROW), rL = mkrsrc(lQ + base, (unsigned)nk * ROW);
   const brsrc rHt = mkrsrc(H + (size_t)k0 * NfpNf, (unsigned)nk * NfpNf * 4u);
@@ -720,12 +720,12 @@ static hipError_t adj_tiled_n(bbw_ctx *c, const float *Y, float fa, float fb, fl
   if ((unsigned long long)c->K * 16 * L::Np >= (1ull << 32)) return hipErrorInvalidValue;
   hipLaunchKernelGGL(k_adj1<N>, dim3(grid), dim3(256), s1, c->stream, c->K, fused_ops(c), c->c2q, Y,
                      c->fgeo, fa, fb, fdt, lres_io, c->lQ, c->lres, c->Zadj, c->Hadj, c->grad, c->PqTBlk,
-                     c->VqTBlk, c->LiftTBlk);
+                     c->VqTBlk, c->LiftTBlk, c->xcd_chunk);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(k_adj2<N>, dim3(grid), dim3(256), s2, c->stream, c->K, c->vgeo, c->fgeo,
                      c->Sids, c->Swts, c->Eids, c->Evals, c->FmaskInv, c->mapP, c->Zadj, c->Hadj, fdt,
-                     c->lQ);
+                     c->lQ, c->xcd_chunk);
   return hipGetLastError();
 }
 

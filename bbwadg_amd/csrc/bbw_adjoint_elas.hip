@@ -82,7 +82,7 @@ __global__ __launch_bounds__(256) void k_adje(int K, FusedOps o, AdjElasMedia md
                                               const float *__restrict__ Hin, float *__restrict__ Hout,
                                               float *__restrict__ grad, const float *__restrict__ PqTBlk,
                                               const float *__restrict__ VqTBlk,
-                                              const float *__restrict__ LiftTBlk) {
+                                              const float *__restrict__ LiftTBlk, int xcd_chunk) {
   using L = AdjElasLayout<N>;
   constexpr int Np = L::Np, Nfp = L::Nfp, NfpNf = L::NfpNf, Nq = L::Nq, TB = L::TB, RS = L::RS;
   constexpr int RowF = L::RowF, NQP = L::NQP, KTV = L::KTV, KTP = L::KTP, KBV = L::KBV, KBP = L::KBP;
@@ -100,7 +100,7 @@ __global__ __launch_bounds__(256) void k_adje(int K, FusedOps o, AdjElasMedia md
   float *sScr = lds + L::OSC;            // written by lanes past a row, never read
   const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int col = lane & 15, kr = lane >> 4;
-  const int k0 = blockIdx.x * TB, nk = min(TB, K - k0);
+  const int k0 = tile_index(xcd_chunk) * TB, nk = min(TB, K - k0);
   const size_t base = (size_t)k0 * RowF, KNq = (size_t)K * Nq;
   const unsigned tbytes = (unsigned)nk * ROW, qbytes = (unsigned)nk * Nq * 4u;
   const brsrc rL = mkrsrc(lQ + base, tbytes), rR = mkrsrc(lres + base, tbytes), rY = mkrsrc(Y + base, tbytes);
@@ -539,7 +539,8 @@ static hipError_t adj_elas_n(bbw_ctx *c, const float *Y, float fa, float fb, flo
   AdjElasMedia md{c->rhoinv, c->lambda, c->mu, c->tau_v, c->tau_s};
   AdjElasMesh mm{c->vgeo, c->fgeo, c->mapP, c->FmaskInv, c->Sids, c->Eids, c->Swts, c->Evals};
   hipLaunchKernelGGL(k_adje<N>, dim3(grid), dim3(256), sh, c->stream, c->K, fused_ops(c), md, mm, Y, fa, fb,
-                     fdt, lres_io, c->lQ, c->lres, Hin, Hout, c->grad, c->PqTBlk, c->VqTBlk, c->LiftTBlk);
+                     fdt, lres_io, c->lQ, c->lres, Hin, Hout, c->grad, c->PqTBlk, c->VqTBlk, c->LiftTBlk,
+                     c->xcd_chunk);
   return hipGetLastError();
 }
 

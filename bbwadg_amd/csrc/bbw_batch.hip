@@ -102,6 +102,9 @@ __global__ __launch_bounds__(256, PERSIST ? 3 : 4) void k_fused_batch(
       src.last = su(sd->src.last);
       src.elems = sp(sd->src.elems);
       src.coef = sp(sd->src.coef);
+      src.tiles = sp(sd->src.tiles);
+      src.tfirst = su(sd->src.tfirst);
+      src.tlast = su(sd->src.tlast);
 #pragma unroll
       for (int f = 0; f < 9; ++f) src.w[f] = sf(sd->src.w[f]);
       sh.ft = src.nsrc ? sf(sp(sd->ftab)[stp * 5 + stage]) : 0.f;
@@ -143,6 +146,8 @@ struct bbw_batch {
     float *res = nullptr, *c2q = nullptr, *ftab = nullptr, *coef = nullptr, *Q0 = nullptr;
     int *elems = nullptr;
     int nsrc = 0, first = 0, last = -1;
+    int2 *tiles = nullptr;   // src_tile_table of elems
+    int tfirst = 0, tlast = -1;
     float w[9] = {0};
   };
   std::vector<Shot> shots;
@@ -274,6 +279,7 @@ int bbw_batch_add_mesh(bbw_batch *b, int K, const float *vgeo, const float *fgeo
       return bfail(b, BBW_ERR_ARG, "vmapP entry out of range");
   FusedMesh m;
   m.K = K;
+  m.xcd_chunk = 0;
   float *dv = nullptr, *df = nullptr;
   int *dm = nullptr;
   int rc;
@@ -318,9 +324,13 @@ int bbw_batch_add_shot(bbw_batch *b, int mesh_id, const float *c2q, const float 
     if ((rc = bup(b, &s.elems, (const int *)src_elems, (size_t)nsrc)) ||
         (rc = bup(b, &s.coef, src_coef, (size_t)nsrc * Np)))
       return rc;
+    const std::vector<int2> tiles = src_tile_table(src_elems, nsrc);
+    if ((rc = bup(b, &s.tiles, tiles.data(), tiles.size()))) return rc;
     s.nsrc = nsrc;
     s.first = src_elems[0];
     s.last = src_elems[nsrc - 1];
+    s.tfirst = src_elems[0] >> 4;
+    s.tlast = src_elems[nsrc - 1] >> 4;
     for (int f = 0; f < 4; ++f) s.w[f] = (float)(weights[f] * amp);
     std::vector<float> ft((size_t)std::max(nsteps, 1) * 5);
     for (int st = 0; st < nsteps; ++st)
@@ -373,6 +383,9 @@ int bbw_batch_run(bbw_batch *b, int shots_per_tile, int blocks_per_cu, int use_g
     d.src.last = s.last;
     d.src.elems = s.elems;
     d.src.coef = s.coef;
+    d.src.tiles = s.tiles;
+    d.src.tfirst = s.tfirst;
+    d.src.tlast = s.tlast;
     for (int f = 0; f < 9; ++f) d.src.w[f] = s.w[f];
     d.dt = (float)s.dt;
     d.mesh = s.mesh;

@@ -235,13 +235,7 @@ __global__ __launch_bounds__(256) void k_update_elas(
 }
 
 static SrcArgs make_src_e(const bbw_ctx *c) {
-  SrcArgs s;
-  s.nsrc = c->nsrc;
-  s.first = c->src_first;
-  s.last = c->src_last;
-  s.elems = c->src_elems;
-  s.coef = c->src_coef;
-  for (int f = 0; f < 9; ++f) s.w[f] = c->src_w[f];
+  SrcArgs s = ctx_src(c);
   return s;
 }
 

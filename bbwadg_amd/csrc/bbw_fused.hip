@@ -40,7 +40,7 @@ __global__ __launch_bounds__(256, BBW_FUSED_WG_PER_CU) void k_fused_mfma(FusedMe
   using L = FusedLayout<N, PROJ_U>;
   extern __shared__ float lds[];
   const int tid = threadIdx.x;
-  const int k0 = blockIdx.x * L::TB, nk = min(L::TB, m.K - k0);
+  const int k0 = tile_index(m.xcd_chunk) * L::TB, nk = min(L::TB, m.K - k0);
   if (ftime_dev) sh.ft = *ftime_dev;
   const int fid3 = face_node<N, PROJ_U>(o, tid);
   // vmapP first: the trace gather that depends on it is issued behind the tile loads
@@ -173,7 +173,9 @@ __global__ __launch_bounds__(256, N <= 4 ? 4 : N == 5 ? 3 : 2) void k_update_wad
   const brsrc rJ = mkrsrc(Jq ? Jq + (size_t)k0 * Nq : nullptr, Jq ? qbytes : 0u);
   const brsrc rAV = mkrsrc(o.VqBlk, (unsigned)L::NQT * KBV * 1024u);
   const brsrc rAP = mkrsrc(o.PqBlk, (unsigned)L::NPT * KBP * 1024u);
-  const bool has_src = src.nsrc > 0 && k0 + nk - 1 >= src.first && k0 <= src.last;
+  static_assert(TB == 16, "the per-tile source table is built for 16-element tiles");
+  const int2 stile = src_tile(src, k0, nk);
+  const bool has_src = stile.y != 0;
 
   // ---- 1: rhs tile -> LDS; res and Q rows, c2 and 1/J (MFMA output layout), Vq blocks.
   // With fa = 0 (the first LSRK45 stage) res is not read: fa res = 0 for any finite res.
@@ -200,7 +202,7 @@ __global__ __launch_bounds__(256, N <= 4 ? 4 : N == 5 ? 3 : 2) void k_update_wad
   constexpr int KPAD = L::RS0 - NpNf;
   static_assert(TB * KPAD <= 256, "one padding word per thread");
   if (KPAD > 0 && tid < TB * KPAD) sR[(tid / KPAD) * RS + NpNf + tid % KPAD] = 0.f;
-  if (has_src && tid < TB) sSlot[tid] = tid < nk ? src_slot2(src, k0 + tid) : -1;
+  if (has_src && tid < TB) sSlot[tid] = src_slot_tile(src, stile, k0, tid);
   float c2r[L::QTW][4], jr[L::QTW][4];
   f4 aV[L::QTW][KBV];
   const unsigned co = (unsigned)(col * Nq + 4 * kr) * 4u;
@@ -377,7 +379,9 @@ __global__ __launch_bounds__(256) void k_update_wadg_chunk(int K, const float *_
   const brsrc rJ = mkrsrc(Jq ? Jq + (size_t)k0 * Nq : nullptr, Jq ? qbytes : 0u);
   const brsrc rAV = mkrsrc(VqBlk, (unsigned)L::NQT * L::KBV * 1024u);
   const brsrc rAP = mkrsrc(PqBlk, (unsigned)L::NPT * L::NQT * 1024u);
-  const bool has_src = src.nsrc > 0 && k0 + nk - 1 >= src.first && k0 <= src.last;
+  static_assert(TB == 16, "the per-tile source table is built for 16-element tiles");
+  const int2 stile = src_tile(src, k0, nk);
+  const bool has_src = stile.y != 0;
 
   // ---- 1: rhs tile -> LDS (zero padding past the last field for the K tail)
   {
@@ -386,7 +390,7 @@ __global__ __launch_bounds__(256) void k_update_wadg_chunk(int K, const float *_
     constexpr int KPAD = L::RS0 - NpNf;
     static_assert(TB * KPAD <= 256, "one padding word per thread");
     if (KPAD > 0 && tid < TB * KPAD) sR[(tid / KPAD) * RS + NpNf + tid % KPAD] = 0.f;
-    if (has_src && tid < TB) sSlot[tid] = tid < nk ? src_slot2(src, k0 + tid) : -1;
+    if (has_src && tid < TB) sSlot[tid] = src_slot_tile(src, stile, k0, tid);
     rows.template store<RS>(sR, sScr, tid);
   }
   __syncthreads();
@@ -498,13 +502,7 @@ static hipError_t update_wadg_chunk_n(bbw_ctx *c, float fa, float fb, float fdt,
 
 hipError_t launch_update_wadg_mfma(bbw_ctx *c, float fa, float fb, float fdt, const float *ftd,
                                    float ft) {
-  SrcArgs s;
-  s.nsrc = c->nsrc;
-  s.first = c->src_first;
-  s.last = c->src_last;
-  s.elems = c->src_elems;
-  s.coef = c->src_coef;
-  for (int f = 0; f < 9; ++f) s.w[f] = c->src_w[f];
+  SrcArgs s = ctx_src(c);
   switch (c->N) {
     case 1: return update_wadg_mfma_n<1>(c, fa, fb, fdt, ftd, ft, s);
     case 2: return update_wadg_mfma_n<2>(c, fa, fb, fdt, ftd, ft, s);
@@ -859,7 +857,9 @@ __global__ __launch_bounds__(256, N <= 4 ? 4 : N == 5 ? 3 : 2) void k_update_ela
   const brsrc rMu = mkrsrc(muq + (size_t)k0 * Nq, qbytes);
   const brsrc rAV = mkrsrc(o.VqBlk, (unsigned)L::NQT * KBV * 1024u);
   const brsrc rAP = mkrsrc(o.PqBlk, (unsigned)L::NPT * KBP * 1024u);
-  const bool has_src = src.nsrc > 0 && k0 + nk - 1 >= src.first && k0 <= src.last;
+  static_assert(TB == 16, "the per-tile source table is built for 16-element tiles");
+  const int2 stile = src_tile(src, k0, nk);
+  const bool has_src = stile.y != 0;
 
   // ---- 1: rhs rows, materials at the cubature points (MFMA output layout), Vq and Pq blocks;
   // then the rows -> LDS
@@ -890,7 +890,7 @@ __global__ __launch_bounds__(256, N <= 4 ? 4 : N == 5 ? 3 : 2) void k_update_ela
   constexpr int KPAD = L::RS0 - NpNf;
   static_assert(TB * KPAD <= 256, "one padding word per thread");
   if (KPAD > 0 && tid < TB * KPAD) sR[(tid / KPAD) * RS + NpNf + tid % KPAD] = 0.f;
-  if (has_src && tid < TB) sSlot[tid] = tid < nk ? src_slot2(src, k0 + tid) : -1;
+  if (has_src && tid < TB) sSlot[tid] = src_slot_tile(src, stile, k0, tid);
   __syncthreads();
 
 #pragma unroll
@@ -1008,13 +1008,7 @@ static hipError_t update_elas_mfma_n(bbw_ctx *c, float fa, float fb, float fdt, 
 
 hipError_t launch_update_elas_mfma(bbw_ctx *c, float fa, float fb, float fdt, const float *ftd,
                                    float ft) {
-  SrcArgs s;
-  s.nsrc = c->nsrc;
-  s.first = c->src_first;
-  s.last = c->src_last;
-  s.elems = c->src_elems;
-  s.coef = c->src_coef;
-  for (int f = 0; f < 9; ++f) s.w[f] = c->src_w[f];
+  SrcArgs s = ctx_src(c);
   switch (c->N) {
     case 1: return update_elas_mfma_n<1>(c, fa, fb, fdt, ftd, ft, s);
     case 2: return update_elas_mfma_n<2>(c, fa, fb, fdt, ftd, ft, s);
@@ -1591,7 +1585,7 @@ static hipError_t fused_mfma_n(bbw_ctx *c, const float *Qin, float *Qout, float 
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh);
     if (e != hipSuccess) return e;
   }
-  FusedMesh m{c->K, c->vgeo, c->fgeo, c->vmapP};
+  FusedMesh m{c->K, c->vgeo, c->fgeo, c->vmapP, c->xcd_chunk};
   FusedShot shot{c->c2q, Qin, Qout, c->res, yout, ft, fdt, store_res};
   hipLaunchKernelGGL((k_fused_mfma<N, P>), dim3(grid), dim3(256), sh, c->stream, m, fused_ops(c),
                      shot, s, fa, fb, ftd);
@@ -1609,13 +1603,7 @@ static hipError_t fused_mfma_sel(bbw_ctx *c, const float *Qin, float *Qout, floa
 hipError_t launch_fused_mfma(bbw_ctx *c, const float *Qin, float *Qout, float fa, float fb,
                              float fdt, const float *ftd, float ft, int projU, float *yout,
                              int store_res) {
-  SrcArgs s;
-  s.nsrc = c->nsrc;
-  s.first = c->src_first;
-  s.last = c->src_last;
-  s.elems = c->src_elems;
-  s.coef = c->src_coef;
-  for (int f = 0; f < 9; ++f) s.w[f] = c->src_w[f];
+  SrcArgs s = ctx_src(c);
   switch (c->N) {
     case 1: return fused_mfma_sel<1>(c, Qin, Qout, fa, fb, fdt, ftd, ft, projU, s, yout, store_res);
     case 2: return fused_mfma_sel<2>(c, Qin, Qout, fa, fb, fdt, ftd, ft, projU, s, yout, store_res);

@@ -372,10 +372,12 @@ __global__ __launch_bounds__(256, N <= 4 ? 3 : N == 5 ? 2 : 1) void k_fused_elas
   int *sSlot = (int *)(lds + L::OSL);
   const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int col = lane & 15, kr = lane >> 4;
-  const int k0 = blockIdx.x * TB, nk = min(TB, m.K - k0);
+  const int k0 = tile_index(m.xcd_chunk) * TB, nk = min(TB, m.K - k0);
   const unsigned tbytes = (unsigned)nk * RowF * 4u;
   if (ftime_dev) ft = *ftime_dev;
-  const bool has_src = src.nsrc > 0 && k0 + nk - 1 >= src.first && k0 <= src.last;
+  static_assert(TB == 16, "the per-tile source table is built for 16-element tiles");
+  const int2 stile = src_tile(src, k0, nk);
+  const bool has_src = stile.y != 0;
 
   // ---- 1: loads, then the LDS stores
   ElasTileIn<N> in;
@@ -396,7 +398,7 @@ __global__ __launch_bounds__(256, N <= 4 ? 3 : N == 5 ? 2 : 1) void k_fused_elas
   // zero K padding of the rhs rows (read by the Vq product, never written otherwise)
   constexpr int KPAD = L::RG0 - 3 * Np;
   if (KPAD > 0 && tid < TB * KPAD) sR[(tid / KPAD) * RG + 3 * Np + tid % KPAD] = 0.f;
-  if (has_src && tid < TB) sSlot[tid] = tid < nk ? src_slot2(src, k0 + tid) : -1;
+  if (has_src && tid < TB) sSlot[tid] = src_slot_tile(src, stile, k0, tid);
   __syncthreads();
 
   // ---- 2: penalty flux at the face nodes
@@ -698,10 +700,12 @@ __global__ __launch_bounds__(256, N <= 4 ? 3 : N == 5 ? 2 : 1) void k_fused_elas
   int *sSlot = (int *)(lds + L::OSL);
   const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int col = lane & 15, kr = lane >> 4;
-  const int k0 = blockIdx.x * TB, nk = min(TB, m.K - k0);
+  const int k0 = tile_index(m.xcd_chunk) * TB, nk = min(TB, m.K - k0);
   const unsigned tbytes = (unsigned)nk * RowF * 4u;
   if (ftime_dev) ft = *ftime_dev;
-  const bool has_src = src.nsrc > 0 && k0 + nk - 1 >= src.first && k0 <= src.last;
+  static_assert(TB == 16, "the per-tile source table is built for 16-element tiles");
+  const int2 stile = src_tile(src, k0, nk);
+  const bool has_src = stile.y != 0;
 
   // ---- 1: loads, then the LDS stores
   ElasTileIn<N> in;
@@ -709,7 +713,7 @@ __global__ __launch_bounds__(256, N <= 4 ? 3 : N == 5 ? 2 : 1) void k_fused_elas
   const brsrc rRes = mkrsrc(res + (size_t)k0 * RowF, tbytes);
   const brsrc rOut = mkrsrc(Qout + (size_t)k0 * RowF, tbytes);
   elas_tile_store<N>(in, sQ, sG, sFg, tid);
-  if (has_src && tid < TB) sSlot[tid] = tid < nk ? src_slot2(src, k0 + tid) : -1;
+  if (has_src && tid < TB) sSlot[tid] = src_slot_tile(src, stile, k0, tid);
   __syncthreads();
 
   // ---- 2: penalty flux at the face nodes
@@ -991,7 +995,7 @@ static hipError_t fused_elas_n(bbw_ctx *c, const float *Qin, float *Qout, float 
     hipError_t e = hipFuncSetAttribute((const void *)k_fused_elas<N, CAP>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh);
     if (e != hipSuccess) return e;
   }
-  FusedMesh m{c->K, c->vgeo, c->fgeo, c->vmapP};
+  FusedMesh m{c->K, c->vgeo, c->fgeo, c->vmapP, c->xcd_chunk};
   ElasMedia md{c->rhoinv, c->lambda, c->mu, c->tau_v, c->tau_s};
   hipLaunchKernelGGL((k_fused_elas<N, CAP>), dim3((c->K + L::TB - 1) / L::TB), dim3(256), sh, c->stream, m, fused_ops(c), md,
                      Qin, Qout, c->res, s, fa, fb, fdt, ft, ftd, store_res, yout);
@@ -1009,7 +1013,7 @@ static hipError_t fused_elas_cq_n(bbw_ctx *c, const float *Qin, float *Qout, flo
     hipError_t e = hipFuncSetAttribute((const void *)k_fused_elas_cq<N>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh);
     if (e != hipSuccess) return e;
   }
-  FusedMesh m{c->K, c->vgeo, c->fgeo, c->vmapP};
+  FusedMesh m{c->K, c->vgeo, c->fgeo, c->vmapP, c->xcd_chunk};
   ElasMedia md{c->rhoinv, c->lambda, c->mu, c->tau_v, c->tau_s};
   hipLaunchKernelGGL((k_fused_elas_cq<N>), dim3((c->K + L::TB - 1) / L::TB), dim3(256), sh, c->stream, m, fused_ops(c), md,
                      Qin, Qout, c->res, s, fa, fb, fdt, ft, ftd, store_res);
@@ -1026,13 +1030,7 @@ static hipError_t fused_elas_any(bbw_ctx *c, const float *Qin, float *Qout, floa
 
 hipError_t launch_fused_elas(bbw_ctx *c, const float *Qin, float *Qout, float fa, float fb, float fdt,
                              const float *ftd, float ft, int store_res, float *yout) {
-  SrcArgs s;
-  s.nsrc = c->nsrc;
-  s.first = c->src_first;
-  s.last = c->src_last;
-  s.elems = c->src_elems;
-  s.coef = c->src_coef;
-  for (int f = 0; f < 9; ++f) s.w[f] = c->src_w[f];
+  SrcArgs s = ctx_src(c);
   switch (c->N) {
     case 1: return fused_elas_any<1>(c, Qin, Qout, fa, fb, fdt, ftd, ft, s, store_res, yout);
     case 2: return fused_elas_any<2>(c, Qin, Qout, fa, fb, fdt, ftd, ft, s, store_res, yout);

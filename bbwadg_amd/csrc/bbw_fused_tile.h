@@ -20,6 +20,26 @@ __device__ __forceinline__ int src_slot2(const SrcArgs &s, int k) {
   return -1;
 }
 
+// Source elements of the 16-element tile at k0 (nk elements): .y a mask of the elements that
+// may carry a source (0: none, uniform over the workgroup), .x the slot of the first.  With the
+// per-tile table this is one scalar load and the mask is exact; without it, the elements are
+// sorted, so a tile outside [first, last] has none and the others search the list.
+__device__ __forceinline__ int2 src_tile(const SrcArgs &s, int k0, int nk) {
+  if (s.nsrc == 0) return make_int2(0, 0);
+  if (s.tiles) {
+    const int t = k0 >> 4;
+    return (t < s.tfirst || t > s.tlast) ? make_int2(0, 0) : s.tiles[t - s.tfirst];
+  }
+  return make_int2(0, (k0 + nk - 1 >= s.first && k0 <= s.last) ? (1 << nk) - 1 : 0);
+}
+
+// slot of the tile's element e (-1: no source) from src_tile's value st
+__device__ __forceinline__ int src_slot_tile(const SrcArgs &s, int2 st, int k0, int e) {
+  if (e >= 16 || !((st.y >> e) & 1)) return -1;
+  if (s.tiles) return st.x + __builtin_popcount(st.y & ((1 << e) - 1));
+  return src_slot2(s, k0 + e);
+}
+
 // source weight of acoustic field f (0..3) without indexing the array by a per-lane value
 __device__ __forceinline__ float w4(const SrcArgs &s, int f) {
   return f == 0 ? s.w[0] : f == 1 ? s.w[1] : f == 2 ? s.w[2] : s.w[3];
@@ -86,7 +106,22 @@ struct FusedMesh {
   int K;
   const float *vgeo, *fgeo;
   const int *vmapP;
+  int xcd_chunk;  // tiles per XCD run (power of 2), 0: the dispatch order (tile = workgroup)
 };
+
+// Tile of this workgroup.  Workgroups go to the 8 XCDs round-robin (b mod 8), so in the
+// dispatch order consecutive tiles, and the neighbours whose traces a tile gathers, sit in
+// different L2s.  With xcd_chunk = C, XCD x takes the runs of C consecutive tiles
+// x, x + 8, x + 16, ... (run r = tiles rC .. rC + C - 1): workgroup b = 8i + x takes tile
+// (8 (i / C) + x) C + i mod C.  The workgroups past the last whole 8C span keep b.
+__device__ __forceinline__ int tile_index(const int C) {
+  const int b = blockIdx.x;
+  if (C == 0) return b;
+  const int lc = __builtin_ctz(C), full = (int)gridDim.x & ~(8 * C - 1);
+  if (b >= full) return b;
+  const int i = b >> 3;
+  return ((((i >> lc) << 3) + (b & 7)) << lc) + (i & (C - 1));
+}
 
 struct FusedShot {
   const float *c2q;
@@ -324,12 +359,14 @@ __device__ __forceinline__ void tile_shot(const FusedMesh &m, const FusedOps &o,
   // node mapping (phases 2, 3, 4)
   const int gv = tid / L::GS, nv = tid - gv * L::GS;
   const bool onv = gv < L::NG && nv < Np;
-  // source: elements are sorted, so a tile outside [first, last] has none (uniform test)
-  const bool has_src = src.nsrc > 0 && k0 + nk - 1 >= src.first && k0 <= src.last;
+  // source elements of the tile (uniform test)
+  static_assert(TB == 16, "the per-tile source table is built for 16-element tiles");
+  const int2 stile = src_tile(src, k0, nk);
+  const bool has_src = stile.y != 0;
 
   // ---- 1: source slots of the tile's elements (read in phase 8, after several barriers); the
   // register inputs (res, c2, traces) were issued by tile_prefetch before the Q_in tile's barrier
-  if (has_src && tid < TB) sSlot[tid] = tid < nk ? src_slot2(src, k0 + tid) : -1;
+  if (has_src && tid < TB) sSlot[tid] = src_slot_tile(src, stile, k0, tid);
   at_start();
 
   // ---- 2: upwind flux at face nodes into registers.  The p and U fluxes are (phi, -phi) with

@@ -45,7 +45,26 @@ struct SrcArgs {
   const int *elems;       // ascending element ids
   const float *coef;      // [nsrc][Np]
   float w[9];             // per-field weights x amplitude x time function (set per stage)
+  // per 16-element tile t = tfirst .. tlast: (slot of the tile's first source element, mask of
+  // its source elements); null: the 16-element tile kernels search elems instead
+  const int2 *tiles = nullptr;
+  int tfirst = 0, tlast = -1;
 };
+
+// The per-tile source table of a sorted element list (SrcArgs::tiles), for tiles
+// elems[0] / 16 .. elems[n - 1] / 16.
+inline std::vector<int2> src_tile_table(const int32_t *elems, int n) {
+  std::vector<int2> t;
+  if (n <= 0) return t;
+  const int t0 = elems[0] >> 4;
+  t.assign((size_t)((elems[n - 1] >> 4) - t0 + 1), make_int2(0, 0));
+  for (int i = n - 1; i >= 0; --i) {
+    int2 &v = t[(size_t)((elems[i] >> 4) - t0)];
+    v.x = i;   // descending, so the last write is the tile's first source element
+    v.y |= 1 << (elems[i] & 15);
+  }
+  return t;
+}
 
 struct Graph {
   hipGraphExec_t exec = nullptr;
@@ -62,6 +81,7 @@ struct bbw_ctx {
   int KblkV = 0, KblkS = 0, KblkU = 0, KblkF = 0;
   int fused_impl = 1;  // BBW_FUSED_VECTOR or BBW_FUSED_MFMA
   int elas_fused = 1;  // BBW_ELAS_LIFT_NODES or BBW_ELAS_LIFT_CUBATURE (default by N: bbw_create)
+  int xcd_chunk = 16;  // fused and adjoint stages: tiles per XCD run (BBW_OPT_XCD_CHUNK)
   // set inside the adjoint driver (misfit and gradient): its forward passes run the elastic stage
   // in the nodal form, whose A Q capture the reverse pass reads, so J and the states of every
   // forward pass agree bitwise
@@ -146,6 +166,8 @@ struct bbw_ctx {
   int nsrc = 0, src_first = 0, src_last = -1;
   int *src_elems = nullptr;
   float *src_coef = nullptr;
+  int2 *src_tiles = nullptr;   // src_tile_table of src_elems
+  int src_tfirst = 0, src_tlast = -1;
   float src_w[9] = {0};
   double src_f0 = 0, src_amp = 0;
   int src_step_time = 0;   // 1: every stage evaluates the source at its step's start time
@@ -258,6 +280,22 @@ bool order_supported(int N);
 
 // the form of the fused elastic stage a launch runs (BBW_ELAS_LIFT_*): the nodal form inside
 // the adjoint driver, else the context's option
+// The context's source as the update kernels take it (weights as set, time function applied
+// by the kernels).
+inline SrcArgs ctx_src(const bbw_ctx *c) {
+  SrcArgs s;
+  s.nsrc = c->nsrc;
+  s.first = c->src_first;
+  s.last = c->src_last;
+  s.elems = c->src_elems;
+  s.coef = c->src_coef;
+  for (int f = 0; f < 9; ++f) s.w[f] = c->src_w[f];
+  s.tiles = c->src_tiles;
+  s.tfirst = c->src_tfirst;
+  s.tlast = c->src_tlast;
+  return s;
+}
+
 inline int elas_form(const bbw_ctx *c) { return c->adj_pass ? BBW_ELAS_LIFT_NODES : c->elas_fused; }
 
 }  // namespace bbw

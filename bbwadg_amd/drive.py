@@ -3,7 +3,7 @@ reference's ``runBlkTimings`` / ``profileKernels`` runs, ``cpp/runBlkTimings:4-2
 
     python -m bbwadg_amd.drive --mesh 32 --N 5 --steps 10 --mode fused --impl mfma
 
-Builds a seeded cube mesh with Fourier c^2 (or, with ``--elastic``, seeded Lame fields and
+Builds a seeded cube mesh (cubes along the Z-order curve unless ``--order x``) with Fourier c^2 (or, with ``--elastic``, seeded Lame fields and
 9 fields), runs ``--steps`` LSRK45 steps in the chosen mode and prints one
 line with the time per stage.
 """
@@ -16,11 +16,11 @@ from . import _lib, media, mesh as M, operators as O
 from .solver import WaveSolver, MODE_FUSED, MODE_PLAIN
 
 
-def setup(n, N, seed=1, **kw):
+def setup(n, N, seed=1, order="morton", **kw):
     # N = 8, 9: the collapsed form of the tensor-product WADG set (the reference's own set
     # diverges with a variable c^2; the kernels and their cost are the same)
     ops = O.build_ref_ops(N, tp_rule="collapsed")
-    m = M.cube_mesh(n)
+    m = M.cube_mesh(n, order=order)
     M.face_pair(m)
     dm = M.device_arrays(m, ops, 4)
     xq, yq, zq = media.cubature_points(m, ops)
@@ -32,9 +32,9 @@ def setup(n, N, seed=1, **kw):
     return s
 
 
-def setup_elastic(n, N, seed=1):
+def setup_elastic(n, N, seed=1, order="morton"):
     ops = O.build_ref_ops(N)
-    m = M.cube_mesh(n)
+    m = M.cube_mesh(n, order=order)
     M.face_pair(m)
     dm = M.device_arrays(m, ops, 9)
     xq, yq, zq = media.cubature_points(m, ops)
@@ -56,6 +56,10 @@ def main():
     ap.add_argument("--plain-impl", choices=["tiled", "flat", "slice"], default=None,
                     help="plain kernels (default: tiled for N <= 6, flat with the slice lift above)")
     ap.add_argument("--project-velocity", action="store_true")
+    ap.add_argument("--order", choices=["morton", "x"], default="morton",
+                    help="cube numbering: Z-order curve (default) or x fastest")
+    ap.add_argument("--xcd-chunk", type=int, default=None,
+                    help="tiles per XCD run of the fused stages (0: dispatch order; default: the library's)")
     ap.add_argument("--elastic", action="store_true", help="elastic (9 fields), fused or plain stage")
     ap.add_argument("--elas-lift", choices=["cubature", "nodes"], default="cubature",
                     help="fused elastic stage: lift folded into the cubature product (default) or at the nodes")
@@ -118,15 +122,17 @@ def main():
             print(f"K={s.K} N={args.N} {name}: {el * 1e6:.1f} us per launch", flush=True)
         return
     if args.elastic:
-        s = setup_elastic(args.mesh, args.N)
+        s = setup_elastic(args.mesh, args.N, order=args.order)
         s.set_option(_lib.OPT_ELAS_FUSED, _lib.ELAS_LIFT_CUBATURE if args.elas_lift == "cubature"
                      else _lib.ELAS_LIFT_NODES)
         mode = MODE_FUSED if args.mode == "fused" else MODE_PLAIN
     else:
-        s = setup(args.mesh, args.N)
+        s = setup(args.mesh, args.N, order=args.order)
         if args.N <= 6:
             s.set_option(_lib.OPT_FUSED_IMPL, _lib.FUSED_MFMA if args.impl == "mfma" else _lib.FUSED_VECTOR)
         mode = MODE_FUSED if args.mode == "fused" and args.N <= 6 else MODE_PLAIN
+    if args.xcd_chunk is not None:
+        s.set_option(_lib.OPT_XCD_CHUNK, args.xcd_chunk)
     if args.plain_impl:
         s.set_option(_lib.OPT_PLAIN_IMPL, {"tiled": _lib.PLAIN_TILED, "flat": _lib.PLAIN_FLAT,
                                            "slice": _lib.PLAIN_SLICE}[args.plain_impl])

@@ -6,7 +6,10 @@ Restates the mesh layer of the reference:
 * ``write_gmsh_node_data`` -- per-element nodal values as a GMSH ``$ElementNodeData`` view,
   ``writeVisToGMSH`` (``cpp/src/WaveOKL3d.cpp:2560-2642``);
 * ``cube_mesh``       -- structured n^3 cubes x 6 tetrahedra on [lo, hi]^3 (new: the
-  reference reads cube meshes from ``cpp/meshes/cube*.msh``);
+  reference reads cube meshes from ``cpp/meshes/cube*.msh``), cubes numbered x-fastest or
+  along a Z-order curve;
+* ``sfc_order``/``renumber`` -- Z-order renumbering of any mesh's elements (new: a
+  cache-locality step for the fused stages' neighbour-trace reads);
 * ``face_pair``       -- EToE/EToF by sorted vertex triples, ``FacePair3d``
   (``cpp/src/Mesh3d.cpp:335-410``) with the all-three-vertices comparison of
   ``elas_cpp/src/Mesh3d.cpp:392`` (the acoustic copy compares only two);
@@ -73,14 +76,29 @@ def _orient(VX, VY, VZ, EToV):
     return EToV
 
 
-def cube_mesh(n, lo=-0.5, hi=0.5):
+def morton_codes(ix, iy, iz, bits=21):
+    """Z-order (Morton) codes of non-negative integer coordinates below 2^bits: the bits of
+    x, y, z interleaved, x lowest."""
+    ix, iy, iz = (np.asarray(a, dtype=np.int64) for a in (ix, iy, iz))
+    code = np.zeros(ix.shape, np.int64)
+    for b in range(bits):
+        code |= (((ix >> b) & 1) << (3 * b)) | (((iy >> b) & 1) << (3 * b + 1)) | \
+            (((iz >> b) & 1) << (3 * b + 2))
+    return code
+
+
+def cube_mesh(n, lo=-0.5, hi=0.5, order="x"):
     """n^3 cubes, each split into 6 tets along its main diagonal (conforming Kuhn split).
 
-    Elements are numbered cube by cube (x fastest), 6 per cube, so neighbouring
-    elements are close in memory.
+    Elements are numbered cube by cube, 6 per cube.  ``order`` = "x": cubes x fastest, then
+    y, then z; "morton": cubes along the Z-order curve of their indices, so that most of an
+    element's face neighbours are a few elements away in memory in every direction (their
+    traces are then read while still cached).  The two numberings are the same mesh.
     """
     if n < 1:
         raise ValueError("n must be >= 1")
+    if order not in ("x", "morton"):
+        raise ValueError("order must be 'x' or 'morton'")
     g = np.linspace(lo, hi, n + 1)
     Z, Y, X = np.meshgrid(g, g, g, indexing="ij")
     VX, VY, VZ = X.ravel(), Y.ravel(), Z.ravel()
@@ -91,6 +109,9 @@ def cube_mesh(n, lo=-0.5, hi=0.5):
     ii, jj, kk = np.meshgrid(np.arange(n), np.arange(n), np.arange(n), indexing="ij")
     # x fastest ordering of cubes
     ii, jj, kk = (a.transpose(2, 1, 0).ravel() for a in (ii, jj, kk))
+    if order == "morton":
+        zo = np.argsort(morton_codes(ii, jj, kk), kind="stable")
+        ii, jj, kk = ii[zo], jj[zo], kk[zo]
     perms = [(0, 1, 2), (0, 2, 1), (1, 0, 2), (1, 2, 0), (2, 0, 1), (2, 1, 0)]
     tets = []
     for p in perms:
@@ -103,6 +124,27 @@ def cube_mesh(n, lo=-0.5, hi=0.5):
     EToV = np.stack(tets, axis=1).reshape(-1, 4).astype(np.int64)
     EToV = _orient(VX, VY, VZ, EToV)
     return Mesh(VX=VX, VY=VY, VZ=VZ, EToV=EToV)
+
+
+def sfc_order(mesh, bits=21):
+    """Element permutation along the Z-order curve of the element centroids (quantised to
+    2^bits cells per axis of the bounding box): ``renumber(mesh, sfc_order(mesh))`` numbers
+    face neighbours close together in memory, so that the fused stages read most neighbour
+    traces from cache.  A mesh-preparation step; the solvers accept any numbering."""
+    c = np.stack([mesh.GX.mean(1), mesh.GY.mean(1), mesh.GZ.mean(1)])
+    lo, hi = c.min(1, keepdims=True), c.max(1, keepdims=True)
+    q = np.floor((c - lo) / np.maximum(hi - lo, 1e-300) * ((1 << bits) - 1)).astype(np.int64)
+    return np.argsort(morton_codes(q[0], q[1], q[2], bits), kind="stable")
+
+
+def renumber(mesh, perm):
+    """The same mesh with its elements renumbered: new element e is old element perm[e].
+    Per-element arrays map as a_new = a_old[perm]; connectivity is rebuilt by ``face_pair``."""
+    perm = np.asarray(perm, dtype=np.int64)
+    if perm.shape != (mesh.K,) or not np.array_equal(np.sort(perm), np.arange(mesh.K)):
+        raise MeshError("perm must be a permutation of the elements")
+    return Mesh(VX=mesh.VX, VY=mesh.VY, VZ=mesh.VZ, EToV=mesh.EToV[perm].copy(),
+                gmsh_ids=None if mesh.gmsh_ids is None else mesh.gmsh_ids[perm].copy())
 
 
 def read_gmsh(path):

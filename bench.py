@@ -54,10 +54,10 @@ def run_bytes(N, nsteps, graph_steps=10, Nf=4, Nm=1):
     return full - row * (nsteps + skipped_writes) / (5.0 * nsteps)
 
 
-def build_problem(n, N, seed=1):
+def build_problem(n, N, seed=1, order="morton"):
     from bbwadg_amd import media, mesh as M, operators as O
     ops = O.build_ref_ops(N)
-    m = M.cube_mesh(n)
+    m = M.cube_mesh(n, order=order)
     M.face_pair(m)
     dm = M.device_arrays(m, ops, 4)
     xq, yq, zq = media.cubature_points(m, ops)
@@ -118,7 +118,7 @@ def adjoint_ratio(nsteps, mesh_n=32, N=5, ckpt=50):
     from bbwadg_amd.adjoint import AcousticInversion, Shot
     from bbwadg_amd.solver import WaveSolver
     ops = O.build_ref_ops(N)
-    m = M.cube_mesh(mesh_n)
+    m = M.cube_mesh(mesh_n, order="morton")
     M.face_pair(m)
     dm = M.device_arrays(m, ops, 4)
     s = WaveSolver(m, N, ops=ops, dmesh=dm)
@@ -153,7 +153,7 @@ def elastic_adjoint_ratio(nsteps, mesh_n=32, N=4, ckpt=50):
     from bbwadg_amd.adjoint import ElasticInversion, Shot
     from bbwadg_amd.solver import WaveSolver
     ops = O.build_ref_ops(N)
-    m = M.cube_mesh(mesh_n)
+    m = M.cube_mesh(mesh_n, order="morton")
     M.face_pair(m)
     dm = M.device_arrays(m, ops, 9)
     s = WaveSolver(m, N, nfields=9, ops=ops, dmesh=dm)
@@ -304,7 +304,7 @@ def batched_throughput(step_scale=1.0, tune_steps=20):
     from bbwadg_amd.batch import c4_workload
     ops = O.build_ref_ops(5)
     log = lambda msg: print(msg, file=sys.stderr, flush=True)
-    b, desc = c4_workload(seed=7, step_scale=step_scale, ops=ops, log=log)
+    b, desc = c4_workload(seed=7, step_scale=step_scale, ops=ops, log=log, order="morton")
     best, trials = autotune.tune_batch(b, step_cap=tune_steps)
     log(f"batch autotune: {best} from {trials}")
     est = min(t for _, t in trials) / 1e3 * sum(b.meshes[m][0].K * n for m, n in b.shots) / \
@@ -379,6 +379,7 @@ def main():
         sys.exit(2)
 
     import torch
+    from bbwadg_amd import _lib
     from bbwadg_amd.solver import WaveSolver, MODE_FUSED, MODE_PLAIN
 
     N, n = args.N, args.mesh
@@ -400,12 +401,13 @@ def main():
     setup_s = time.perf_counter() - t_setup
     print(f"setup {setup_s:.1f} s, K={K}, N={N}, dt={dt:.3e}", file=sys.stderr, flush=True)
 
-    def timed(nsteps, mode):
+    def timed(nsteps, mode, sol=None):
+        sol = sol or s
         torch.cuda.synchronize()
         e0 = torch.cuda.Event(enable_timing=True)
         e1 = torch.cuda.Event(enable_timing=True)
         e0.record(stream)
-        s.step(nsteps, dt, mode=mode, graph=True, t0=0.0)
+        sol.step(nsteps, dt, mode=mode, graph=True, t0=0.0)
         e1.record(stream)
         torch.cuda.synchronize()
         return e0.elapsed_time(e1)
@@ -440,6 +442,23 @@ def main():
             traffic = json.load(f).get("bytes_per_launch")
 
     s.close()
+    # the same workload on the x-fastest cube numbering in the dispatch tile order (the element
+    # numbering and tile order of the earlier measurements), for comparison
+    ops_x, m_x, dm_x, c2q_x, (se_x, sc_x), _ = build_problem(n, N, order="x")
+    sx = WaveSolver(m_x, N, ops=ops_x, dmesh=dm_x)
+    sx.InitWADG_subelem(c2q_x)
+    sx.set_source(se_x, sc_x, f0=10.0, amp=1.0)
+    sx.set_stream(stream.cuda_stream)
+    sx.set_option(_lib.OPT_XCD_CHUNK, 0)
+    sx.WaveSetData3d(np.random.default_rng(11).uniform(-1, 1, (K, 4, ops.Np)).astype(np.float32))
+    xsteps = min(args.steps, 200)
+    sx.step(warm, dt, mode=MODE_FUSED, graph=True, t0=0.0)
+    sx.prepare_graph(xsteps, dt, mode=MODE_FUSED)
+    ms_x = timed(xsteps, MODE_FUSED, sx)
+    sx.close()
+    del m_x, dm_x, c2q_x
+    fused_x = {"steps": xsteps, "ms_per_step": ms_x / xsteps, "value": dof_stage * xsteps / (ms_x / 1e3),
+               "element_order": "x-fastest cubes", "tile_order": "dispatch"}
     adj = adjoint_ratio(args.adjoint_steps) if args.adjoint_steps > 0 else None
     elas = elastic_throughput(args.elastic_steps) if args.elastic_steps > 0 else None
     eadj = elastic_adjoint_ratio(args.elastic_adjoint_steps) if args.elastic_adjoint_steps > 0 else None
@@ -474,7 +493,9 @@ def main():
         "data": "synthetic (seeded Fourier c^2, seeded Ricker source position)",
         "config": {"workload": f"acoustic heterogeneous single shot, {n}^3x6={K} tets, N={N}, "
                                f"{args.steps} steps, fused stage kernel + HIP graph",
-                   "K": K, "N": N, "Np": ops.Np, "Nfields": 4, "dt": dt},
+                   "K": K, "N": N, "Np": ops.Np, "Nfields": 4, "dt": dt,
+                   "element_order": "cubes along the Z-order curve", "tile_order": "runs of 16 tiles per XCD"},
+        "fused_x_order": fused_x,
         "plain": {"value": plain_value, "steps": args.plain_steps,
                   "ms_per_step": ms_plain / args.plain_steps},
         "fused_over_plain": value / plain_value,

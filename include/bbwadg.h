@@ -310,6 +310,12 @@ double bbw_batch_last_ms(const bbw_batch *b); /* device time of the last bbw_bat
 #define BBW_OPT_ELAS_FUSED 11
 #define BBW_ELAS_LIFT_NODES 0
 #define BBW_ELAS_LIFT_CUBATURE 1
+/* fused and adjoint stages (acoustic and elastic): 16-element tiles per XCD run.  Workgroups
+ * go to the device's 8 XCDs round-robin; with a run length C (a power of 2, at most 1024;
+ * default 16) each XCD takes runs of C consecutive tiles, so that neighbouring elements' face
+ * traces are read through one L2.  0 keeps the dispatch order (tile = workgroup).  Results do
+ * not depend on it. */
+#define BBW_OPT_XCD_CHUNK 12
 int bbw_set_option(bbw_ctx *ctx, int option, int value);
 
 /* Device addresses of the context's buffers (for copies into caller-owned device memory):

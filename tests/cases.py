@@ -10,10 +10,10 @@ def cavity(x, y, z, t):
 
 
 class Problem:
-    def __init__(self, N, n, nfields=4, seed=0, c2="fourier"):
+    def __init__(self, N, n, nfields=4, seed=0, c2="fourier", order="x"):
         self.N, self.n = N, n
         self.ops = O.build_ref_ops(N)
-        self.mesh = M.cube_mesh(n)
+        self.mesh = M.cube_mesh(n, order=order)
         M.face_pair(self.mesh)
         self.dm = M.device_arrays(self.mesh, self.ops, nfields)
         self.K = self.mesh.K
