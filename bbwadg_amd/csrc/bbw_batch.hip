@@ -56,13 +56,14 @@ __device__ __forceinline__ T *sp(T *p) {
 #endif
 
 // Work items blockIdx.x, + gridDim.x, ... (PERSIST; otherwise one item per workgroup, which
-// keeps the register count of the single-shot kernel), each with its shots in order on the
-// same tile (the shots after the first read the tile's geometry and vmapP entries from cache).
+// keeps the register count of the single-shot kernel, in runs of xcd_chunk items per XCD),
+// each with its shots in order on the same tile (the shots after the first read the tile's
+// geometry and vmapP entries from cache).
 template <int N, bool PERSIST>
 __global__ __launch_bounds__(256, PERSIST ? 3 : 4) void k_fused_batch(
     int nitems, const BatchItem *__restrict__ items, const FusedMesh *__restrict__ meshes,
     const BatchShotDev *__restrict__ shots, const int *__restrict__ list, FusedOps o, int stage,
-    int par, float fa, float fb, int store_res, const int *__restrict__ step) {
+    int par, float fa, float fb, int store_res, const int *__restrict__ step, int xcd_chunk) {
   using L = FusedLayout<N, false>;
   extern __shared__ float lds[];
   const int stp = *step;
@@ -124,7 +125,9 @@ __global__ __launch_bounds__(256, PERSIST ? 3 : 4) void k_fused_batch(
 #pragma unroll 1
     for (int w = blockIdx.x; w < nitems; w += gridDim.x) item(w);
   } else {
-    item(blockIdx.x);
+    // consecutive items are consecutive tiles of one shot group: runs of them per XCD, as in
+    // the single-shot stage
+    item(tile_index(xcd_chunk));
   }
 }
 
@@ -198,12 +201,14 @@ hipError_t launch_batch_n(bbw_batch *b, int nitems, const BatchItem *items, cons
     if (sh > 65536)
       hipFuncSetAttribute((const void *)k_fused_batch<N, true>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh);
     hipLaunchKernelGGL((k_fused_batch<N, true>), dim3(grid), dim3(256), sh, b->ops->stream, nitems, items,
-                       meshes, shots, list, fused_ops(b->ops), stage, par, fa, fb, store_res, step);
+                       meshes, shots, list, fused_ops(b->ops), stage, par, fa, fb, store_res, step,
+                       b->ops->xcd_chunk);
   } else {
     if (sh > 65536)
       hipFuncSetAttribute((const void *)k_fused_batch<N, false>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh);
     hipLaunchKernelGGL((k_fused_batch<N, false>), dim3(nitems), dim3(256), sh, b->ops->stream, nitems, items,
-                       meshes, shots, list, fused_ops(b->ops), stage, par, fa, fb, store_res, step);
+                       meshes, shots, list, fused_ops(b->ops), stage, par, fa, fb, store_res, step,
+                       b->ops->xcd_chunk);
   }
   return hipGetLastError();
 }
