@@ -38,6 +38,17 @@ def _elastic(P, Q0, chunk, form, nsteps=3):
     return s.WaveGetData3d()
 
 
+def test_run_length_must_be_a_power_of_two_up_to_1024():
+    print("case run-length option", flush=True)
+    P = Problem(2, 2)
+    s = WaveSolver(P.mesh, 2, ops=P.ops, dmesh=P.dm)
+    for bad in (3, 12, 2048, -16):
+        with pytest.raises(_lib.BBWError):
+            s.set_option(_lib.OPT_XCD_CHUNK, bad)
+    for good in (0, 1, 2, 1024):
+        s.set_option(_lib.OPT_XCD_CHUNK, good)
+
+
 @pytest.mark.parametrize("N", [3, 5])
 def test_acoustic_fused_stage_same_bits_under_every_tile_order(N):
     print(f"case acoustic tile order N={N}", flush=True)
