@@ -278,8 +278,6 @@ hipError_t launch_update_elas(bbw_ctx *c, float fa, float fb, float fdt, const f
 
 bool order_supported(int N);
 
-// the form of the fused elastic stage a launch runs (BBW_ELAS_LIFT_*): the nodal form inside
-// the adjoint driver, else the context's option
 // The context's source as the update kernels take it (weights as set, time function applied
 // by the kernels).
 inline SrcArgs ctx_src(const bbw_ctx *c) {
@@ -296,6 +294,8 @@ inline SrcArgs ctx_src(const bbw_ctx *c) {
   return s;
 }
 
+// the form of the fused elastic stage a launch runs (BBW_ELAS_LIFT_*): the nodal form inside
+// the adjoint driver, else the context's option
 inline int elas_form(const bbw_ctx *c) { return c->adj_pass ? BBW_ELAS_LIFT_NODES : c->elas_fused; }
 
 }  // namespace bbw
